@@ -1,0 +1,91 @@
+// engine.h — device-side types of the native nonbonded engine, shared by the
+// kernels and the host fixed-point replay.
+#pragma once
+
+#include "dyn_math.h"
+
+// Site classes.  Clusters hold sites of one class only, so a cluster pair
+// knows from its two classes which interactions exist: solute–solute LJ +
+// Coulomb, solute–O LJ, solute–charged Coulomb, O–O LJ, charged–charged
+// Coulomb; O–charged pairs have neither and are never listed.
+#define DYN_CLS_SOLUTE 0
+#define DYN_CLS_WATER_O 1
+#define DYN_CLS_WATER_Q 2
+
+#define DYN_CL_SIZE 8  // sites per cluster; a wave holds an 8×8 cluster pair
+
+// Interaction kinds of a cluster pair (from the two classes).
+#define DYN_KIND_SS 0  // LJ (type table) + Coulomb
+#define DYN_KIND_SO 1  // LJ (type table)
+#define DYN_KIND_SQ 2  // Coulomb
+#define DYN_KIND_OO 3  // LJ (O–O constants)
+#define DYN_KIND_QQ 4  // Coulomb
+#define DYN_KIND_NONE 5
+
+DYN_HD int dyn_kind(int ci, int cj) {
+  if (ci > cj) {
+    const int t = ci;
+    ci = cj;
+    cj = t;
+  }
+  if (ci == DYN_CLS_SOLUTE) return cj == DYN_CLS_SOLUTE ? DYN_KIND_SS : (cj == DYN_CLS_WATER_O ? DYN_KIND_SO : DYN_KIND_SQ);
+  if (ci == DYN_CLS_WATER_O) return cj == DYN_CLS_WATER_O ? DYN_KIND_OO : DYN_KIND_NONE;
+  return DYN_KIND_QQ;
+}
+
+struct DynPairParams {
+  float ext[3], inv_ext[3];
+  float rc2;        // Coulomb (and list) cutoff squared
+  float lj_rc2;     // LJ cutoff squared
+  float beta, beta2, two_b_sqrtpi;
+  float c6_oo, c12_oo;
+  int n_types;      // LJ type table is (n_types × n_types) float2 (c6, c12)
+};
+
+// One lane of an 8×8 tile: the pair (i, j) of the given kind.  Returns the
+// scalar s (F_i = s·d) and energies; s = e = 0 for an inactive lane.
+// `active` already includes the list mask; the cutoff test happens here.
+template <int KIND>
+DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, bool active,
+                          const DynPairParams& P, const float2* lj_table, float d[3],
+                          float* s_out, float* e_lj, float* e_c, float* r2_out, int* hit) {
+  float d0 = pi[0] - pj[0], d1 = pi[1] - pj[1], d2 = pi[2] - pj[2];
+  d0 = dyn_min_image_rint(d0, P.ext[0], P.inv_ext[0]);
+  d1 = dyn_min_image_rint(d1, P.ext[1], P.inv_ext[1]);
+  d2 = dyn_min_image_rint(d2, P.ext[2], P.inv_ext[2]);
+  d[0] = d0;
+  d[1] = d1;
+  d[2] = d2;
+  const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+  const bool in = active && r2 < P.rc2 && r2 > 1e-16f;
+  const float r2s = in ? r2 : 1.0f;
+  const float rinv = dyn_rsqrtf(r2s);
+  float s = 0.f, elj = 0.f, ec = 0.f;
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) {
+    float c6, c12;
+    if (KIND == DYN_KIND_OO) {
+      c6 = P.c6_oo;
+      c12 = P.c12_oo;
+    } else {
+      const float2 c = lj_table[ti * P.n_types + tj];
+      c6 = c.x;
+      c12 = c.y;
+    }
+    float sl, el;
+    dyn_lj(rinv * rinv, c6, c12, &sl, &el);
+    const bool lj_in = in && r2 <= P.lj_rc2;
+    s = lj_in ? sl : 0.f;
+    elj = lj_in ? el : 0.f;
+  }
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SQ || KIND == DYN_KIND_QQ) {
+    float sc, ee;
+    dyn_coul_erfc(r2s, rinv, pi[3] * pj[3], P.beta, P.beta2, P.two_b_sqrtpi, &sc, &ee);
+    s = s + (in ? sc : 0.f);
+    ec = in ? ee : 0.f;
+  }
+  *s_out = s;
+  *e_lj = elj;
+  *e_c = ec;
+  *r2_out = in ? r2 : 0.f;
+  *hit = in ? 1 : 0;
+}

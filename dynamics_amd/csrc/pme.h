@@ -1,0 +1,67 @@
+// pme.h — SPME reciprocal-space core shared by the dyn_pme_* entry points
+// and the native engine.  Smooth PME (Essmann et al., J. Chem. Phys. 103,
+// 8577 (1995)) with cardinal B-splines of order 4 or 5:
+//   Q(k)   = Σ_i q_i Π_a M_n(u_ia − k_a),       u_ia = K_a x_ia / L_a
+//   E_rec  = ½ Σ_{m≠0} G(m) |F[Q](m)|²,
+//   G(m)   = exp(−π² |m|²/β²) / (π V |m|²) · Π_a |b_a(m_a)|²
+//   φ      = F⁻¹[G · F[Q]]  (unnormalised transforms), F_i = −Σ_k ∂Q(k)/∂r_i · φ(k)
+//   W_rec  = Σ_m ½ G(m)|F[Q](m)|² (1 − 2π²|m|²/β²)   (trace of the k-space virial)
+// Charge spreading accumulates in int64 fixed point (2^-32 units), so the
+// grid is independent of the order of the atomic adds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace dyn {
+
+struct FftAxis {
+  int N;
+  int nstages;
+  int radix[16];
+};
+
+FftAxis make_fft_axis(int N);  // throws if N has a prime factor > 5
+bool is_235_smooth(uint32_t n);
+
+// Position/charge input for spreading and gathering: float4 (x, y, z, q).
+// `site` maps each charge to the fixed-point force row it adds into (engine
+// use); null when forces are written to a float array instead.
+struct PmeCore {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Profiler* prof = nullptr;
+  uint32_t K[3] = {0, 0, 0};
+  float L[3] = {0, 0, 0};
+  float beta = 0.f;
+  int order = 4;
+  size_t ngrid = 0;
+  FftAxis ax[3];
+  DevBuf<int64_t> grid_fx;      // K0·K1·K2 fixed-point charge grid
+  DevBuf<float2> cgrid;         // complex working grid
+  DevBuf<float> G;              // influence function
+  DevBuf<float2> tw[3];         // e^{-2πi k/N} tables per axis
+  DevBuf<double> partials;      // energy/virial partials of the convolution pass
+  PinnedBuf<double> h_part;
+  unsigned conv_blocks = 0;
+  DevBuf<double> res_dev;       // [energy, virial] reduced on the device
+
+  void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
+            float beta, int order);
+  void set_box(const float l[3]);  // recomputes G
+  // Enqueue spread → FFT → convolve → inverse FFT → gather for n charges.
+  // xyzq: device float4 array (positions wrapped or not; wrapped internally).
+  // If site != null, forces are atomically added (fixed point, 2^32) into
+  // fx_sites[3·site[i] + c] (component-interleaved); else written as floats
+  // into f_out[3i + c].  Energy and virial (without self energy) end up in
+  // res_dev[0], res_dev[1] after the final reduction kernel.
+  void run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
+           int fx_component_major, uint32_t fx_nsites, float* f_out);
+};
+
+}  // namespace dyn

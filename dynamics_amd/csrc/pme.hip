@@ -1,0 +1,753 @@
+// pme.hip — SPME reciprocal space on gfx950: fixed-point B-spline charge
+// spreading, a mixed-radix (2/3/4/5) Stockham FFT staged through LDS, the
+// influence-function convolution fused into the x-axis pass with FP64 energy
+// and virial reductions, and the force gather.  Replaces the `ewald` crate's
+// PmeRecip (used at src/non_bonded.rs:681-795, :850-875).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.h"
+#include "dyn_math.h"
+#include "pme.h"
+
+#pragma clang fp contract(off)
+
+namespace dyn {
+
+bool is_235_smooth(uint32_t n) {
+  if (n == 0) return false;
+  for (uint32_t p : {2u, 3u, 5u})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+FftAxis make_fft_axis(int N) {
+  FftAxis a;
+  a.N = N;
+  a.nstages = 0;
+  int n = N;
+  // radix-4 first, then 2, 3, 5
+  while (n % 4 == 0) {
+    a.radix[a.nstages++] = 4;
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    a.radix[a.nstages++] = 2;
+    n /= 2;
+  }
+  while (n % 3 == 0) {
+    a.radix[a.nstages++] = 3;
+    n /= 3;
+  }
+  while (n % 5 == 0) {
+    a.radix[a.nstages++] = 5;
+    n /= 5;
+  }
+  if (n != 1) throw Error(DYN_ERR_ARG, fmt("grid size %d has a prime factor above 5", N));
+  return a;
+}
+
+}  // namespace dyn
+
+using namespace dyn;
+
+namespace {
+
+constexpr int kFftBlock = 256;
+constexpr int kFftCap = 2048;  // complex elements per workgroup (two LDS buffers: 32 KB)
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by SIGN·i
+template <int SIGN>
+__device__ __forceinline__ float2 cmul_si(float2 a) {
+  return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+// v · w where w = tw (SIGN = −1) or conj(tw) (SIGN = +1)
+template <int SIGN>
+__device__ __forceinline__ float2 ctw(float2 v, float2 t) {
+  const float ti = SIGN < 0 ? t.y : -t.y;
+  return make_float2(v.x * t.x - v.y * ti, v.x * ti + v.y * t.x);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft2(float2* v) {
+  const float2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2* v) {
+  const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  const float2 s13 = cadd(v[1], v[3]), d13 = cmul_si<SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(s02, s13);
+  v[2] = csub(s02, s13);
+  v[1] = cadd(d02, d13);
+  v[3] = csub(d02, d13);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft3(float2* v) {
+  const float c = -0.5f, s = 0.866025403784438647f;
+  const float2 t = cadd(v[1], v[2]);
+  const float2 d = cscale(cmul_si<SIGN>(csub(v[1], v[2])), s);
+  const float2 a = cadd(v[0], cscale(t, c));
+  v[0] = cadd(v[0], t);
+  v[1] = cadd(a, d);
+  v[2] = csub(a, d);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft5(float2* v) {
+  const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;
+  const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;
+  const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+  const float2 a1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+  const float2 a2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+  const float2 b1 = cmul_si<SIGN>(cadd(cscale(d1, s1), cscale(d2, s2)));
+  const float2 b2 = cmul_si<SIGN>(csub(cscale(d1, s2), cscale(d2, s1)));
+  v[0] = cadd(v[0], cadd(t1, t2));
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+// Stockham autosort FFT of nl lines of length N held in LDS as a[l·N + n].
+// Returns the buffer holding the (natural-order) result.
+template <int SIGN>
+__device__ float2* lds_fft(float2* a, float2* b, int nl, const FftAxis& ax,
+                           const float2* __restrict__ tw) {
+  const int N = ax.N;
+  int Ns = 1;
+  for (int s = 0; s < ax.nstages; ++s) {
+    const int R = ax.radix[s];
+    const int NR = N / R;
+    const int tw_step = N / (Ns * R);
+    const int total = nl * NR;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+      const int l = i / NR;
+      const int j = i - l * NR;
+      const int k = j % Ns;
+      const float2* src = a + l * N;
+      float2* dst = b + l * N;
+      float2 v[5];
+      for (int r = 0; r < R; ++r) {
+        float2 x = src[j + r * NR];
+        if (r) x = ctw<SIGN>(x, tw[(r * k * tw_step) % N]);
+        v[r] = x;
+      }
+      if (R == 4) dft4<SIGN>(v);
+      else if (R == 2) dft2<SIGN>(v);
+      else if (R == 3) dft3<SIGN>(v);
+      else dft5<SIGN>(v);
+      const int d = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; ++r) dst[d + r * Ns] = v[r];
+    }
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+struct PassGeom {
+  int K[3];
+  int lines;      // total lines in this pass
+  int L;          // lines per workgroup
+};
+
+// Address of element n of line `line` for the axis pass AXIS (0 = x, 1 = y, 2 = z).
+template <int AXIS>
+__device__ __forceinline__ size_t pass_addr(const PassGeom& g, int line, int n) {
+  if (AXIS == 2) return (size_t)line * g.K[2] + n;
+  if (AXIS == 1) {
+    const int x = line / g.K[2], z = line - x * g.K[2];
+    return ((size_t)x * g.K[1] + n) * g.K[2] + z;
+  }
+  const int y = line / g.K[2], z = line - y * g.K[2];
+  return ((size_t)n * g.K[1] + y) * g.K[2] + z;
+}
+
+// LDS element e of a group ↔ (line l, point n).  z lines are contiguous in
+// memory so n runs fastest; for y and x lines, consecutive lines are
+// consecutive in z, so l runs fastest and loads stay coalesced.
+template <int AXIS>
+__device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
+  if (AXIS == 2) {
+    l = e / N;
+    n = e - l * N;
+  } else {
+    n = e / L;
+    l = e - n * L;
+  }
+}
+
+// One FFT pass along AXIS over complex data in place.  FROM_FX: the input is
+// the int64 fixed-point charge grid (z pass only).
+template <int AXIS, int SIGN, bool FROM_FX>
+__global__ void __launch_bounds__(kFftBlock)
+k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
+           FftAxis ax, const float2* __restrict__ tw) {
+  extern __shared__ float2 smem[];
+  const int N = ax.N;
+  const int line0 = blockIdx.x * g.L;
+  const int nl = min(g.L, g.lines - line0);
+  float2* A = smem;
+  float2* B = smem + g.L * N;
+  const int tot = nl * N;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<AXIS>(e, nl, N, l, n);
+    const size_t addr = pass_addr<AXIS>(g, line0 + l, n);
+    float2 v;
+    if (FROM_FX) v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
+    else v = data[addr];
+    A[l * N + n] = v;
+  }
+  __syncthreads();
+  float2* R = lds_fft<SIGN>(A, B, nl, ax, tw);
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<AXIS>(e, nl, N, l, n);
+    data[pass_addr<AXIS>(g, line0 + l, n)] = R[l * N + n];
+  }
+}
+
+struct ConvArgs {
+  float invL[3];
+  double pi2_over_b2;  // π²/β²
+};
+
+// x pass fused with the convolution: forward FFT along x, multiply by G(m)
+// (accumulating ½G|F[Q]|² and its virial weight in f64), inverse FFT along x.
+__global__ void __launch_bounds__(kFftBlock)
+k_fft_conv_x(float2* __restrict__ data, PassGeom g, FftAxis ax, const float2* __restrict__ tw,
+             const float* __restrict__ G, ConvArgs C, double* __restrict__ partials) {
+  extern __shared__ float2 smem[];
+  __shared__ double red[2][kFftBlock];
+  const int N = ax.N;
+  const int line0 = blockIdx.x * g.L;
+  const int nl = min(g.L, g.lines - line0);
+  float2* A = smem;
+  float2* B = smem + g.L * N;
+  const int tot = nl * N;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<0>(e, nl, N, l, n);
+    A[l * N + n] = data[pass_addr<0>(g, line0 + l, n)];
+  }
+  __syncthreads();
+  float2* R = lds_fft<-1>(A, B, nl, ax, tw);
+  float2* O = (R == A) ? B : A;
+  double e_acc = 0.0, w_acc = 0.0;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<0>(e, nl, N, l, n);
+    const int line = line0 + l;
+    const int my = line / g.K[2], mz = line - my * g.K[2];
+    const size_t gi = pass_addr<0>(g, line, n);
+    const float gv = G[gi];
+    float2 v = R[l * N + n];
+    const double en = 0.5 * (double)gv * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+    const int mxp = n <= g.K[0] / 2 ? n : n - g.K[0];
+    const int myp = my <= g.K[1] / 2 ? my : my - g.K[1];
+    const int mzp = mz <= g.K[2] / 2 ? mz : mz - g.K[2];
+    const double fx = mxp * (double)C.invL[0], fy = myp * (double)C.invL[1],
+                 fz = mzp * (double)C.invL[2];
+    const double msq = fx * fx + fy * fy + fz * fz;
+    e_acc += en;
+    w_acc += en * (1.0 - 2.0 * C.pi2_over_b2 * msq);
+    v.x *= gv;
+    v.y *= gv;
+    R[l * N + n] = v;
+  }
+  red[0][threadIdx.x] = e_acc;
+  red[1][threadIdx.x] = w_acc;
+  __syncthreads();
+  for (int s = kFftBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = red[0][0];
+    partials[2 * blockIdx.x + 1] = red[1][0];
+  }
+  float2* R2 = lds_fft<+1>(R, O, nl, ax, tw);
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<0>(e, nl, N, l, n);
+    data[pass_addr<0>(g, line0 + l, n)] = R2[l * N + n];
+  }
+}
+
+__global__ void k_pme_finalize(const double* __restrict__ partials, int nb,
+                               double* __restrict__ res) {
+  __shared__ double red[2][256];
+  double e = 0, w = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    e += partials[2 * b];
+    w += partials[2 * b + 1];
+  }
+  red[0][threadIdx.x] = e;
+  red[1][threadIdx.x] = w;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = red[0][0];
+    res[1] = red[1][0];
+  }
+}
+
+struct SplineArgs {
+  int K[3];
+  float sc[3];  // K/L
+};
+
+// Fractional coordinate → base index (mod K) and in-cell weight.
+__device__ __forceinline__ void frac_coord(float x, float sc, int K, int& k0, float& w) {
+  const float u = x * sc;
+  const float fl = floorf(u);
+  w = u - fl;
+  int k = (int)fl % K;
+  if (k < 0) k += K;
+  k0 = k;
+}
+
+template <int ORDER>
+__global__ void k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
+                             int64_t* __restrict__ grid) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = xyzq[i];
+  if (p.w == 0.0f) return;
+  int k0[3];
+  float th[3][ORDER], dth[ORDER];
+  const float xs[3] = {p.x, p.y, p.z};
+  for (int a = 0; a < 3; ++a) {
+    float w;
+    frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
+    dyn_bspline<ORDER>(w, th[a], dth);
+  }
+  for (int jx = 0; jx < ORDER; ++jx) {
+    const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
+    const float qx = p.w * th[0][jx];
+    for (int jy = 0; jy < ORDER; ++jy) {
+      const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
+      const float qxy = qx * th[1][jy];
+      int64_t* row = grid + ((size_t)gx * S.K[1] + gy) * S.K[2];
+      for (int jz = 0; jz < ORDER; ++jz) {
+        const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
+        const float v = qxy * th[2][jz];
+        atomicAdd(reinterpret_cast<unsigned long long*>(row + gz),
+                  (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
+      }
+    }
+  }
+}
+
+template <int ORDER>
+__global__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
+                             const float2* __restrict__ phi, const uint32_t* __restrict__ site,
+                             int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
+                             float* __restrict__ fout) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = xyzq[i];
+  float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+  if (p.w != 0.0f) {
+    int k0[3];
+    float th[3][ORDER], dth[3][ORDER];
+    const float xs[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      float w;
+      frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
+      dyn_bspline<ORDER>(w, th[a], dth[a]);
+    }
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int jx = 0; jx < ORDER; ++jx) {
+      const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
+      for (int jy = 0; jy < ORDER; ++jy) {
+        const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
+        const float2* row = phi + ((size_t)gx * S.K[1] + gy) * S.K[2];
+        const float txy = th[0][jx] * th[1][jy];
+        const float dxy = dth[0][jx] * th[1][jy];
+        const float tdy = th[0][jx] * dth[1][jy];
+        for (int jz = 0; jz < ORDER; ++jz) {
+          const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
+          const float ph = row[gz].x;
+          sx = dyn_fmaf(ph, dxy * th[2][jz], sx);
+          sy = dyn_fmaf(ph, tdy * th[2][jz], sy);
+          sz = dyn_fmaf(ph, txy * dth[2][jz], sz);
+        }
+      }
+    }
+    f0 = -p.w * S.sc[0] * sx;
+    f1 = -p.w * S.sc[1] * sy;
+    f2 = -p.w * S.sc[2] * sz;
+  }
+  if (site) {
+    const uint32_t s = site[i];
+    if (comp_major) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + s), (unsigned long long)dyn_fx(f0));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + nsites + s), (unsigned long long)dyn_fx(f1));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 2 * (size_t)nsites + s), (unsigned long long)dyn_fx(f2));
+    } else {
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s), (unsigned long long)dyn_fx(f0));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 1), (unsigned long long)dyn_fx(f1));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 2), (unsigned long long)dyn_fx(f2));
+    }
+  } else {
+    fout[3 * i] = f0;
+    fout[3 * i + 1] = f1;
+    fout[3 * i + 2] = f2;
+  }
+}
+
+PassGeom pass_geom(const uint32_t K[3], int axis) {
+  PassGeom g;
+  for (int a = 0; a < 3; ++a) g.K[a] = (int)K[a];
+  const int N = (int)K[axis];
+  g.lines = (int)(K[0] * K[1] * K[2] / K[axis]);
+  // aim for ≥ 512 workgroups, at most kFftCap/N lines per group
+  int L = std::max(1, std::min(kFftCap / N, g.lines / 512));
+  int p2 = 1;
+  while (p2 * 2 <= L) p2 *= 2;
+  g.L = p2;
+  return g;
+}
+
+}  // namespace
+
+namespace dyn {
+
+void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
+                   float b, int ord) {
+  device = dev;
+  stream = s;
+  prof = p;
+  for (int a = 0; a < 3; ++a) {
+    DYN_REQUIRE(k[a] >= (uint32_t)ord && k[a] <= 1024, "grid size %u out of range [%d, 1024]",
+                k[a], ord);
+    DYN_REQUIRE(is_235_smooth(k[a]), "grid size %u must factor into 2, 3 and 5", k[a]);
+    K[a] = k[a];
+    ax[a] = make_fft_axis((int)k[a]);
+  }
+  DYN_REQUIRE(ord == 4 || ord == 5, "B-spline order must be 4 or 5 (got %d)", ord);
+  DYN_REQUIRE(b > 0.f && std::isfinite(b), "beta must be > 0");
+  order = ord;
+  beta = b;
+  ngrid = (size_t)K[0] * K[1] * K[2];
+  grid_fx.reserve(ngrid);
+  cgrid.reserve(ngrid);
+  G.reserve(ngrid);
+  for (int a = 0; a < 3; ++a) {
+    std::vector<float2> t(K[a]);
+    for (uint32_t m = 0; m < K[a]; ++m) {
+      const double ang = -2.0 * M_PI * (double)m / (double)K[a];
+      t[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    tw[a].reserve(K[a]);
+    DYN_HIP(hipMemcpyAsync(tw[a].p, t.data(), sizeof(float2) * K[a], hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+  }
+  const PassGeom gx = pass_geom(K, 0);
+  conv_blocks = (unsigned)((gx.lines + gx.L - 1) / gx.L);
+  partials.reserve(2 * (size_t)conv_blocks);
+  res_dev.reserve(2);
+  set_box(l);
+}
+
+void PmeCore::set_box(const float l[3]) {
+  for (int a = 0; a < 3; ++a) {
+    DYN_REQUIRE(l[a] > 0.f && std::isfinite(l[a]), "box length must be > 0");
+    L[a] = l[a];
+  }
+  // |b(m)|² per axis (Essmann eq. 4.4), from the order-n spline at integers.
+  std::vector<double> bsq[3];
+  std::vector<double> Mn(order + 1, 0.0);
+  {
+    float th[5], dth[5];
+    if (order == 4) dyn_bspline<4>(0.0f, th, dth);
+    else dyn_bspline<5>(0.0f, th, dth);
+    // th[j] = M_n(n-1-j) at w = 0 → M_n(k) = th[n-1-k]
+    for (int k = 1; k <= order - 1; ++k) Mn[k] = th[order - 1 - k];
+    // exact rational values (tests pin these against the closed form)
+    if (order == 4) {
+      Mn[1] = 1.0 / 6;
+      Mn[2] = 4.0 / 6;
+      Mn[3] = 1.0 / 6;
+    } else {
+      Mn[1] = 1.0 / 24;
+      Mn[2] = 11.0 / 24;
+      Mn[3] = 11.0 / 24;
+      Mn[4] = 1.0 / 24;
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const int Ka = (int)K[a];
+    std::vector<double> mod(Ka);
+    for (int m = 0; m < Ka; ++m) {
+      double re = 0, im = 0;
+      for (int k = 0; k <= order - 2; ++k) {
+        const double ang = 2.0 * M_PI * (double)m * k / Ka;
+        re += Mn[k + 1] * std::cos(ang);
+        im += Mn[k + 1] * std::sin(ang);
+      }
+      mod[m] = re * re + im * im;
+    }
+    for (int m = 0; m < Ka; ++m)
+      if (mod[m] < 1e-7) mod[m] = 0.5 * (mod[(m - 1 + Ka) % Ka] + mod[(m + 1) % Ka]);
+    bsq[a].resize(Ka);
+    for (int m = 0; m < Ka; ++m) bsq[a][m] = 1.0 / mod[m];
+  }
+  const double V = (double)L[0] * L[1] * L[2];
+  const double b2 = (double)beta * beta;
+  std::vector<float> g(ngrid);
+  for (uint32_t mx = 0; mx < K[0]; ++mx) {
+    const int mxp = mx <= K[0] / 2 ? (int)mx : (int)mx - (int)K[0];
+    const double fx = mxp / (double)L[0];
+    for (uint32_t my = 0; my < K[1]; ++my) {
+      const int myp = my <= K[1] / 2 ? (int)my : (int)my - (int)K[1];
+      const double fy = myp / (double)L[1];
+      for (uint32_t mz = 0; mz < K[2]; ++mz) {
+        const int mzp = mz <= K[2] / 2 ? (int)mz : (int)mz - (int)K[2];
+        const double fz = mzp / (double)L[2];
+        const double msq = fx * fx + fy * fy + fz * fz;
+        double val = 0.0;
+        if (msq > 0)
+          val = std::exp(-M_PI * M_PI * msq / b2) / (M_PI * V * msq) * bsq[0][mx] * bsq[1][my] *
+                bsq[2][mz];
+        g[((size_t)mx * K[1] + my) * K[2] + mz] = (float)val;
+      }
+    }
+  }
+  DYN_HIP(hipMemcpyAsync(G.p, g.data(), sizeof(float) * ngrid, hipMemcpyHostToDevice, stream));
+  DYN_HIP(hipStreamSynchronize(stream));
+}
+
+void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
+                  int comp_major, uint32_t nsites, float* f_out) {
+  SplineArgs S;
+  for (int a = 0; a < 3; ++a) {
+    S.K[a] = (int)K[a];
+    S.sc[a] = (float)((double)K[a] / (double)L[a]);
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_zero");
+    DYN_HIP(hipMemsetAsync(grid_fx.p, 0, sizeof(int64_t) * ngrid, stream));
+  }
+  if (n) {
+    Profiler::Scope ps(*prof, "pme_spread");
+    if (order == 4)
+      hipLaunchKernelGGL(k_pme_spread<4>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
+                         S, grid_fx.p);
+    else
+      hipLaunchKernelGGL(k_pme_spread<5>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
+                         S, grid_fx.p);
+    DYN_CHECK_LAUNCH("k_pme_spread");
+  }
+  const PassGeom gz = pass_geom(K, 2), gy = pass_geom(K, 1), gx = pass_geom(K, 0);
+  auto shm = [&](const PassGeom& g, int N) { return (size_t)2 * g.L * N * sizeof(float2); };
+  const unsigned bz = (gz.lines + gz.L - 1) / gz.L, by = (gy.lines + gy.L - 1) / gy.L,
+                 bx = (gx.lines + gx.L - 1) / gx.L;
+  {
+    Profiler::Scope ps(*prof, "pme_fft_z_fwd");
+    hipLaunchKernelGGL((k_fft_pass<2, -1, true>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
+                       stream, cgrid.p, grid_fx.p, gz, ax[2], tw[2].p);
+    DYN_CHECK_LAUNCH("k_fft_pass z fwd");
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_fft_y_fwd");
+    hipLaunchKernelGGL((k_fft_pass<1, -1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
+                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
+    DYN_CHECK_LAUNCH("k_fft_pass y fwd");
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_fft_x_conv");
+    ConvArgs C;
+    for (int a = 0; a < 3; ++a) C.invL[a] = 1.0f / L[a];
+    C.pi2_over_b2 = M_PI * M_PI / ((double)beta * beta);
+    hipLaunchKernelGGL(k_fft_conv_x, dim3(bx), dim3(kFftBlock), shm(gx, K[0]), stream, cgrid.p,
+                       gx, ax[0], tw[0].p, G.p, C, partials.p);
+    DYN_CHECK_LAUNCH("k_fft_conv_x");
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_fft_y_inv");
+    hipLaunchKernelGGL((k_fft_pass<1, 1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
+                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
+    DYN_CHECK_LAUNCH("k_fft_pass y inv");
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_fft_z_inv");
+    hipLaunchKernelGGL((k_fft_pass<2, 1, false>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
+                       stream, cgrid.p, (const int64_t*)nullptr, gz, ax[2], tw[2].p);
+    DYN_CHECK_LAUNCH("k_fft_pass z inv");
+  }
+  {
+    Profiler::Scope ps(*prof, "pme_finalize");
+    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(256), 0, stream, partials.p,
+                       (int)conv_blocks, res_dev.p);
+    DYN_CHECK_LAUNCH("k_pme_finalize");
+  }
+  if (n) {
+    Profiler::Scope ps(*prof, "pme_gather");
+    if (order == 4)
+      hipLaunchKernelGGL(k_pme_gather<4>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
+                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out);
+    else
+      hipLaunchKernelGGL(k_pme_gather<5>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
+                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out);
+    DYN_CHECK_LAUNCH("k_pme_gather");
+  }
+}
+
+}  // namespace dyn
+
+// ---------------------------------------------------------------------------
+// Standalone PmeRecip replacement
+// ---------------------------------------------------------------------------
+struct dyn_pme {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Profiler prof;
+  PmeCore core;
+  DevBuf<float4> xyzq;
+  DevBuf<float> fout;
+  PinnedBuf<float4> h_xyzq;
+  bool computed = false;
+  ~dyn_pme() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" int dyn_pme_create(dyn_pme** out, int device, uint32_t nx, uint32_t ny, uint32_t nz,
+                              float lx, float ly, float lz, float alpha, uint32_t order) {
+  try {
+    DYN_REQUIRE(out != nullptr, "out is null");
+    *out = nullptr;
+    int ndev = 0;
+    DYN_HIP(hipGetDeviceCount(&ndev));
+    DYN_REQUIRE(device >= 0 && device < ndev, "device %d not present (%d devices)", device, ndev);
+    std::unique_ptr<dyn_pme> p(new dyn_pme());
+    p->device = device;
+    DeviceGuard g(device);
+    DYN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+    p->prof.stream = p->stream;
+    const uint32_t K[3] = {nx, ny, nz};
+    const float L[3] = {lx, ly, lz};
+    p->core.init(device, p->stream, &p->prof, K, L, alpha, (int)order);
+    *out = p.release();
+    return DYN_OK;
+  } catch (const Error& e) {
+    set_global_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_global_error(e.what());
+    return DYN_ERR_HIP;
+  }
+}
+
+extern "C" void dyn_pme_destroy(dyn_pme* pme) {
+  if (!pme) return;
+  try {
+    DeviceGuard g(pme->device);
+    delete pme;
+  } catch (...) {
+  }
+}
+
+extern "C" const char* dyn_pme_last_error(const dyn_pme* pme) {
+  return pme ? pme->err.c_str() : dyn_global_last_error();
+}
+
+extern "C" int dyn_pme_forces_and_virial(dyn_pme* pme, uint32_t n, const float* pos,
+                                         const float* q, float* f_out, double* energy,
+                                         double* virial) {
+  if (!pme) {
+    set_global_error("null handle");
+    return DYN_ERR_ARG;
+  }
+  DYN_API_BODY(pme, {
+    DeviceGuard g(pme->device);
+    DYN_REQUIRE(n == 0 || (pos && q), "null positions or charges");
+    const size_t nn = std::max<uint32_t>(n, 1);
+    pme->xyzq.reserve(nn);
+    pme->fout.reserve(3 * nn);
+    pme->h_xyzq.reserve(nn);
+    double q2 = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+      DYN_REQUIRE(std::isfinite(pos[3 * i]) && std::isfinite(pos[3 * i + 1]) &&
+                      std::isfinite(pos[3 * i + 2]) && std::isfinite(q[i]),
+                  "non-finite input at charge %u", i);
+      pme->h_xyzq.p[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], q[i]);
+      q2 += (double)q[i] * (double)q[i];
+    }
+    if (n)
+      DYN_HIP(hipMemcpyAsync(pme->xyzq.p, pme->h_xyzq.p, sizeof(float4) * n,
+                             hipMemcpyHostToDevice, pme->stream));
+    pme->core.run(n, pme->xyzq.p, nullptr, nullptr, 0, 0, pme->fout.p);
+    double res[2] = {0, 0};
+    DYN_HIP(hipMemcpyAsync(res, pme->core.res_dev.p, sizeof(res), hipMemcpyDeviceToHost,
+                           pme->stream));
+    if (n && f_out)
+      DYN_HIP(hipMemcpyAsync(f_out, pme->fout.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost,
+                             pme->stream));
+    DYN_HIP(hipStreamSynchronize(pme->stream));
+    const double e_self = (double)pme->core.beta / std::sqrt(M_PI) * q2;
+    if (energy) *energy = res[0] - e_self;
+    if (virial) *virial = res[1];
+    pme->computed = true;
+  });
+}
+
+extern "C" int dyn_pme_get_fixed_grid(dyn_pme* pme, int64_t* out, uint64_t n_values) {
+  if (!pme) {
+    set_global_error("null handle");
+    return DYN_ERR_ARG;
+  }
+  DYN_API_BODY(pme, {
+    DeviceGuard g(pme->device);
+    if (!pme->computed) throw Error(DYN_ERR_STATE, "no compute yet");
+    DYN_REQUIRE(out && n_values >= pme->core.ngrid, "need %zu values", pme->core.ngrid);
+    DYN_HIP(hipMemcpyAsync(out, pme->core.grid_fx.p, sizeof(int64_t) * pme->core.ngrid,
+                           hipMemcpyDeviceToHost, pme->stream));
+    DYN_HIP(hipStreamSynchronize(pme->stream));
+  });
+}
+
+extern "C" int dyn_pme_grid_n(float lx, float ly, float lz, float spacing, uint32_t out[3]) {
+  try {
+    DYN_REQUIRE(out, "out is null");
+    DYN_REQUIRE(spacing > 0.f && std::isfinite(spacing), "spacing must be > 0");
+    const float l[3] = {lx, ly, lz};
+    for (int a = 0; a < 3; ++a) {
+      DYN_REQUIRE(l[a] > 0.f && std::isfinite(l[a]), "box length must be > 0");
+      uint32_t n = (uint32_t)std::ceil((double)l[a] / (double)spacing);
+      if (n < 5) n = 5;
+      while (!is_235_smooth(n)) ++n;
+      out[a] = n;
+    }
+    return DYN_OK;
+  } catch (const Error& e) {
+    set_global_error(e.what());
+    return e.code;
+  }
+}

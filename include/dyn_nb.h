@@ -1,0 +1,275 @@
+/*
+ * dyn_nb.h — C ABI of the MI355X (gfx950) nonbonded force engine for the
+ * `dynamics` crate.  Plain pointers and sizes only; every entry point returns
+ * an int status (DYN_OK = 0) and never aborts; the message for the last
+ * failure on a handle is available from the matching *_last_error call.
+ *
+ * Units follow the reference: Å, kcal/mol, kcal/(mol·Å); charges are stored
+ * pre-multiplied by CHARGE_UNIT_SCALER = 18.2223 (src/non_bonded.rs:63).
+ *
+ * Three groups of entry points:
+ *   1. dyn_nb_*      the reference's device boundary (src/gpu_interface.rs),
+ *                    same inputs, outputs and semantics, over the reference's
+ *                    flattened pair list; plus the neighbor-list build
+ *                    (src/neighbors.rs:180-281) and the 1-4 reciprocal
+ *                    correction (src/non_bonded.rs:763-790).
+ *   2. dyn_pme_*     the SPME reciprocal-space object the reference takes
+ *                    from the `ewald` crate (PmeRecip, get_grid_n).
+ *   3. dyn_engine_*  the native path: positions + topology in, forces,
+ *                    energies and virial out, with the engine owning its
+ *                    cluster-pair list, exclusion masks and SPME grid.
+ *
+ * Host pointers unless a name ends in _device.  A handle is not thread-safe;
+ * each owns one HIP stream on its device and all calls are blocking unless
+ * documented otherwise.
+ */
+#ifndef DYN_NB_H
+#define DYN_NB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define DYN_OK 0
+#define DYN_ERR_ARG 1       /* invalid argument (null, size, range)          */
+#define DYN_ERR_HIP 2       /* HIP runtime error (message has the HIP text)  */
+#define DYN_ERR_STATE 3     /* call order (e.g. compute before set_pairs)    */
+#define DYN_ERR_CAPACITY 4  /* caller buffer too small; required size given  */
+#define DYN_ERR_NOMEM 5     /* device allocation failed                      */
+
+/* Version of the library (major*10000 + minor*100 + patch). */
+int dyn_version(void);
+/* Number of HIP devices visible (0 when none; never fails). */
+int dyn_device_count(void);
+/* Message of the last failure that had no handle to hold it (create calls). */
+const char* dyn_global_last_error(void);
+
+/* ======================================================================== */
+/* 1. Reference device boundary                                             */
+/* ======================================================================== */
+
+typedef struct dyn_nb_ctx dyn_nb_ctx;
+
+typedef struct dyn_nb_opts {
+  /* LJ cutoff applied by dyn_nb_compute.  The reference's CUDA kernel has
+   * none (src/cuda/cuda.cu:277-279): pass INFINITY (the default when opts is
+   * NULL).  Its CPU path cuts LJ at lj_cutoff (src/non_bonded.rs:1001): pass
+   * that value to reproduce it. */
+  float lj_cutoff;
+  /* 1 = also report the energy of pairs whose target is a water site (the
+   * reference drops them, src/cuda/cuda.cu:316); 0 = reference behaviour. */
+  uint32_t energy_all_pairs;
+} dyn_nb_opts;
+
+/* ForcesPositsGpu::new (src/gpu_interface.rs:123-129): device buffers for
+ * n_dyn solute atoms and n_water 4-site waters, Ewald cutoff and alpha. */
+int dyn_nb_create(dyn_nb_ctx** out, int device, uint32_t n_dyn, uint32_t n_water,
+                  float cutoff_ewald, float alpha_ewald, const dyn_nb_opts* opts);
+void dyn_nb_destroy(dyn_nb_ctx* ctx);
+const char* dyn_nb_last_error(const dyn_nb_ctx* ctx);
+
+/* PerNeighborGpu::new (src/gpu_interface.rs:242-411): upload the flattened
+ * pair list built by setup_pairs (src/non_bonded.rs:541-677).  Arrays have
+ * n entries; index meaning as in the reference: atom_type 0 = solute, 1 =
+ * water; water_site 1 = O, 2 = M, 3 = H0, 4 = H1; the u8 arrays are bools.
+ * alch may be NULL (no alchemical pairs). */
+int dyn_nb_set_pairs(dyn_nb_ctx* ctx, uint32_t n, const uint32_t* tgt, const uint32_t* src,
+                     const float* sigma, const float* eps, const float* q_tgt,
+                     const float* q_src, const uint8_t* atom_type_tgt,
+                     const uint8_t* water_site_tgt, const uint8_t* atom_type_src,
+                     const uint8_t* water_site_src, const uint8_t* scale14,
+                     const uint8_t* calc_lj, const uint8_t* calc_coul,
+                     const uint8_t* symmetric, const uint8_t* alch);
+
+/* force_nonbonded_gpu (src/gpu_interface.rs:463-632): positions in the
+ * reference's per-class flattened layout (3 floats per site), cell extent,
+ * override flags and λ.  Outputs: f_dyn[3·n_dyn], f_water[12·n_water] in
+ * ForcesOnWaterMol field order (f_o, f_h0, f_h1, f_m; src/solvent/mod.rs:266-273),
+ * virial Σ r·F, energy (pairs whose target is a solute atom), dH/dλ.  Forces
+ * are accumulated in 64-bit fixed point (2^-32 kcal/(mol·Å)), so results are
+ * deterministic.  Any output pointer may be NULL. */
+int dyn_nb_compute(dyn_nb_ctx* ctx, const float* pos_dyn, const float* pos_w_o,
+                   const float* pos_w_m, const float* pos_w_h0, const float* pos_w_h1,
+                   const float cell_extent[3], uint8_t coulomb_disabled, uint8_t lj_disabled,
+                   double lambda_alch, double* f_dyn, double* f_water, double* virial,
+                   double* energy, double* dh_dl);
+
+/* Raw 64-bit fixed-point force sums of the last dyn_nb_compute, one int64
+ * per component, sites ordered solute, water O, M, H0, H1 (class-major),
+ * i.e. n_dyn + 4·n_water sites × 3.  For bit-exact replay checks. */
+int dyn_nb_get_fixed_forces(dyn_nb_ctx* ctx, int64_t* out, uint64_t n_values);
+
+/* The 1-4 Coulomb correction of handle_spme_recip (src/non_bonded.rs:763-790):
+ * for each pair (i, j) of solute atoms adds (1/1.2 − 1)·q_i q_j/r² along r̂
+ * to f_dyn[i] and its negative to f_dyn[j] (force only, no energy), and
+ * r·ΔF to *virial.  f_dyn and virial are accumulated into, not overwritten. */
+int dyn_nb_correct_14(dyn_nb_ctx* ctx, uint32_t n14, const uint32_t* i14, const uint32_t* j14,
+                      const float* pos_dyn, const float* q_dyn, const float cell_extent[3],
+                      double* f_dyn, double* virial);
+
+/* build_neighbors (src/neighbors.rs:180-281): for each outer point, the inner
+ * indices with |min_image(p_outer − p_inner)|² < skin_sq_w_cutoff, min image
+ * and |·|² evaluated exactly as the reference's f32 CPU code does, so the
+ * lists are identical to its brute-force build.  symmetric = 1 requires
+ * inner == outer and returns full symmetric lists (i ≠ j) with static–static
+ * pairs skipped when is_static is non-NULL.  Output is CSR: row_ptr[n_outer+1]
+ * and col_idx, each row ascending (the reference's order).  If col_capacity
+ * is too small the call returns DYN_ERR_CAPACITY and *n_total holds the size
+ * needed. */
+int dyn_nb_build_neighbors(dyn_nb_ctx* ctx, const float* posits_outer, uint32_t n_outer,
+                           const float* posits_inner, uint32_t n_inner,
+                           const uint8_t* is_static, const float cell_extent[3],
+                           uint8_t symmetric, float skin_sq_w_cutoff, uint32_t* row_ptr,
+                           uint32_t* col_idx, uint64_t col_capacity, uint64_t* n_total);
+
+/* ======================================================================== */
+/* 2. SPME reciprocal space (replaces ewald::PmeRecip)                      */
+/* ======================================================================== */
+
+typedef struct dyn_pme dyn_pme;
+
+/* PmeRecip::new((nx,ny,nz), (lx,ly,lz), alpha) (src/non_bonded.rs:853-873);
+ * order is the B-spline order (4 as the reference uses, or 5). */
+int dyn_pme_create(dyn_pme** out, int device, uint32_t nx, uint32_t ny, uint32_t nz,
+                   float lx, float ly, float lz, float alpha, uint32_t order);
+void dyn_pme_destroy(dyn_pme* pme);
+const char* dyn_pme_last_error(const dyn_pme* pme);
+
+/* PmeRecip::forces_and_virial(&pos, &q) (src/non_bonded.rs:692-694): n
+ * charges at positions wrapped into the box (3 floats each).  f_out[3n]
+ * receives the reciprocal-space forces; *energy the reciprocal energy minus
+ * the Ewald self energy β/√π Σq² (the convention tests/spme.rs:147-168
+ * requires); *virial the analytic k-space virial (trace, same sign as Σ r·F).
+ * Charge spreading accumulates in 64-bit fixed point; energy and virial are
+ * reduced in FP64.  Outputs may be NULL. */
+int dyn_pme_forces_and_virial(dyn_pme* pme, uint32_t n, const float* pos, const float* q,
+                              float* f_out, double* energy, double* virial);
+
+/* Fixed-point charge grid of the last call (nx·ny·nz int64, x-major, z
+ * fastest), for bit-exact replay checks. */
+int dyn_pme_get_fixed_grid(dyn_pme* pme, int64_t* out, uint64_t n_values);
+
+/* get_grid_n(l, spacing): grid sizes ≥ l/spacing, rounded up to sizes whose
+ * prime factors are 2, 3 and 5 (the hand-written FFT's radices). */
+int dyn_pme_grid_n(float lx, float ly, float lz, float spacing, uint32_t out[3]);
+
+/* ======================================================================== */
+/* 3. Native engine                                                         */
+/* ======================================================================== */
+
+typedef struct dyn_engine dyn_engine;
+
+#define DYN_MODE_REFERENCE 0 /* the reference's conventions (see below)      */
+#define DYN_MODE_STANDARD 1  /* textbook Ewald with exclusion corrections     */
+
+typedef struct dyn_engine_params {
+  float cutoff;        /* real-space cutoff for Coulomb (and LJ), Å          */
+  float lj_cutoff;     /* LJ cutoff, ≤ cutoff                                */
+  float beta;          /* Ewald splitting parameter, 1/Å                     */
+  uint32_t grid[3];    /* PME grid; 0 disables reciprocal space              */
+  uint32_t order;      /* B-spline order, 4 or 5                             */
+  uint32_t mode;       /* DYN_MODE_REFERENCE or DYN_MODE_STANDARD            */
+  float list_buffer;   /* extra list radius (Å) for list reuse across calls  */
+} dyn_engine_params;
+
+/* Topology: n_dyn solute atoms (charges, LJ type per atom, n_types (σ, ε)
+ * pairs combined by Lorentz–Berthelot), n_water OPC waters (parameters of
+ * src/solvent/mod.rs:58-93), excluded 1-2/1-3 pairs and scaled 1-4 pairs as
+ * (i, j) solute index pairs (2 uint32 each; setup_nonbonded_exclusion_scale_flags,
+ * src/prep.rs:114-143).  Water intramolecular pairs are always excluded.
+ *
+ * Reference mode reproduces the reference's evaluation at a fresh neighbor
+ * list: excluded pairs get no reciprocal correction, 1-4 pairs get the
+ * force-only vacuum correction of src/non_bonded.rs:763-790, and
+ * energy_reference counts short-range pairs that involve a solute atom
+ * (src/non_bonded.rs:337-342) plus the reciprocal energy minus self energy.
+ * Standard mode subtracts qᵢqⱼ·erf(βr)/r for excluded pairs (and the
+ * (1−1/1.2) share for 1-4 pairs) from the reciprocal sum. */
+int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_params* params,
+                      uint32_t n_dyn, uint32_t n_water, const float* q_dyn,
+                      const int32_t* type_dyn, uint32_t n_types, const float* lj_sigma_eps,
+                      uint32_t n_excl, const uint32_t* excl_pairs, uint32_t n_14,
+                      const uint32_t* pairs_14);
+void dyn_engine_destroy(dyn_engine* eng);
+const char* dyn_engine_last_error(const dyn_engine* eng);
+
+/* Box edge lengths (orthorhombic, origin at 0). */
+int dyn_engine_set_box(dyn_engine* eng, const float box[3]);
+
+/* Positions in the reference's layout (host pointers, 3 floats per site). */
+int dyn_engine_set_positions(dyn_engine* eng, const float* pos_dyn, const float* pos_w_o,
+                             const float* pos_w_m, const float* pos_w_h0,
+                             const float* pos_w_h1);
+
+/* Positions already on the device: one float array of n_sites × 4 floats
+ * (x, y, z, unused) with sites ordered solute, then per water O, M, H0, H1.
+ * The pointer must stay valid until the next compute returns. */
+int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device);
+
+#define DYN_COMPUTE_SHORT 1u     /* real-space pair forces                 */
+#define DYN_COMPUTE_RECIP 2u     /* SPME reciprocal space                  */
+#define DYN_COMPUTE_CORR 4u      /* exclusion and 1-4 corrections          */
+#define DYN_COMPUTE_ALL 7u
+#define DYN_COMPUTE_REBUILD 8u   /* force a cluster-pair list rebuild      */
+#define DYN_COMPUTE_ASYNC 16u    /* enqueue only; dyn_engine_sync waits    */
+
+/* One nonbonded evaluation.  Rebuilds the cluster-pair list when asked, on
+ * the first call, after set_box, or when a site moved more than
+ * list_buffer/2 since the last build. */
+int dyn_engine_compute(dyn_engine* eng, uint32_t flags);
+int dyn_engine_sync(dyn_engine* eng);
+
+/* Forces of the last compute in the reference's layout: f_dyn[3·n_dyn],
+ * f_water[12·n_water] in ForcesOnWaterMol order (f_o, f_h0, f_h1, f_m). */
+int dyn_engine_get_forces(dyn_engine* eng, double* f_dyn, double* f_water);
+/* Raw fixed-point sums, sites in the order of set_positions_device. */
+int dyn_engine_get_fixed_forces(dyn_engine* eng, int64_t* out, uint64_t n_values);
+/* Device pointer to the int64 fixed-point force sums (n_sites × 3,
+ * component-major: all x, then all y, then all z), valid until destroy. */
+const int64_t* dyn_engine_forces_device(const dyn_engine* eng);
+
+typedef struct dyn_energies {
+  double lj;            /* LJ, all pairs (1-4 scaled)                         */
+  double coul_sr;       /* erfc Coulomb, all pairs (1-4 scaled)               */
+  double recip;         /* reciprocal-space energy                            */
+  double self;          /* −β/√π Σq² (and net-charge term)                    */
+  double excl_corr;     /* exclusion/1-4 reciprocal corrections (standard)    */
+  double total;         /* lj + coul_sr + recip + self + excl_corr            */
+  double reference;     /* the reference's reported energy (see create)       */
+  double virial_sr;     /* Σ r·F over real-space pairs                        */
+  double virial_recip;  /* k-space virial + corrections                       */
+  double dh_dl;         /* reserved                                           */
+  uint64_t pair_count;  /* pairs inside the cutoff that interacted            */
+  uint64_t list_entries;/* cluster-pair entries in the current list           */
+  uint64_t list_builds; /* list builds so far                                 */
+} dyn_energies;
+
+int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out);
+
+/* Diagnostics (no counterpart in the reference): copies of the current
+ * cluster layout and list for host replay checks.  Sizes come first: call
+ * with null arrays to read n_slots, n_clusters and n_entries.  slot_site
+ * [n_slots] (site index or −1 for padding), xyzq[4·n_slots] (wrapped x, y, z,
+ * charge), slot_type[n_slots], cl_class[n_clusters], row_ptr[n_clusters+1],
+ * cj[n_entries], cmask[n_entries], fx_slot[3·n_slots] (fixed-point pair
+ * forces per slot, component-major). */
+int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clusters,
+                            uint64_t* n_entries, int32_t* slot_site, float* xyzq,
+                            int32_t* slot_type, uint8_t* cl_class, uint32_t* row_ptr,
+                            uint32_t* cj, uint64_t* cmask, int64_t* fx_slot);
+
+/* Per-kernel device time from HIP events recorded on the engine's stream
+ * around every launch while profiling is on.  names/ms/counts receive up to
+ * max entries; *n the number of kernels seen. */
+int dyn_engine_profile(dyn_engine* eng, int enable);
+int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char** names, double* ms,
+                            uint64_t* counts, uint32_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DYN_NB_H */

@@ -1,0 +1,261 @@
+// replay.cpp — host replays of the GPU kernels' fixed-point sums
+// (test infrastructure only).
+//
+// Compiles the product's arithmetic headers (dyn_math.h, pair_ref.h,
+// engine.h) with g++ and contraction off, evaluates every pair exactly as the
+// kernels do, and sums the int64 fixed-point contributions; since integer
+// addition is order-independent, the GPU's atomically accumulated sums must
+// match these bit for bit.  For the engine, the 8×8 wave tile and its xor-tree
+// reductions are emulated lane by lane.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "dyn_math.h"
+#include "engine.h"
+#include "pair_ref.h"
+
+extern "C" {
+
+// ---- arithmetic probes ----------------------------------------------------
+float orc_rsqrtf(float x) { return dyn_rsqrtf(x); }
+float orc_exp_neg(float y) { return dyn_exp_neg(y); }
+float orc_erfcx(float x) { return dyn_erfcx(x); }
+float orc_erfcf(float x) { return dyn_erfcf(x); }
+float orc_inv_cbrt(float x) { return dyn_inv_cbrt(x); }
+int64_t orc_fx(float f) { return dyn_fx(f); }
+void orc_coul_erfc(float r2, float qq, float beta, float* s, float* e) {
+  const float rinv = dyn_rsqrtf(r2);
+  dyn_coul_erfc(r2, rinv, qq, beta, beta * beta, 2.0f * beta * DYN_INV_SQRT_PI, s, e);
+}
+void orc_coul_erf(float r2, float qq, float beta, float* s, float* e) {
+  const float rinv = dyn_rsqrtf(r2);
+  dyn_coul_erf(r2, rinv, qq, beta, beta * beta, 2.0f * beta * DYN_INV_SQRT_PI, s, e);
+}
+void orc_lj(float r2, float c6, float c12, float* s, float* e) {
+  const float rinv = dyn_rsqrtf(r2);
+  dyn_lj(rinv * rinv, c6, c12, s, e);
+}
+void orc_soft_core(float r2, float sigma, float eps, float lambda, float* s, float* e,
+                   float* dhdl) {
+  dyn_lj_soft_core(r2, sigma, eps, lambda, s, e, dhdl);
+}
+void orc_bspline4(float w, float* th, float* dth) { dyn_bspline<4>(w, th, dth); }
+void orc_bspline5(float w, float* th, float* dth) { dyn_bspline<5>(w, th, dth); }
+float orc_min_image_ref(float dv, float ext) { return dyn_min_image_ref(dv, ext); }
+float orc_norm2_ref(float x, float y, float z) { return dyn_norm2_ref(x, y, z); }
+
+// ---- reference pair-list kernel ----------------------------------------------
+// pos: all sites, class-major (solute, O, M, H0, H1), 3 floats each.
+void orc_ref_pairs(const float* pos, uint32_t n_dyn, uint32_t n_water, uint32_t n,
+                   const uint32_t* tgt, const uint32_t* src, const float* sigma,
+                   const float* eps, const float* q_tgt, const float* q_src,
+                   const uint8_t* at_t, const uint8_t* ws_t, const uint8_t* at_s,
+                   const uint8_t* ws_s, const uint8_t* scale14, const uint8_t* calc_lj,
+                   const uint8_t* calc_coul, const uint8_t* symmetric, const uint8_t* alch,
+                   const float* ext, float cutoff, float alpha, float lj_cutoff,
+                   int coulomb_disabled, int lj_disabled, double lambda, int energy_all,
+                   int64_t* fx, double* energy, double* virial, double* dhdl) {
+  DynRefParams P;
+  for (int k = 0; k < 3; ++k) {
+    P.ext[k] = ext[k];
+    P.inv_ext[k] = 1.0f / ext[k];
+  }
+  P.cutoff = cutoff;
+  P.beta = alpha;
+  P.beta2 = alpha * alpha;
+  P.two_b_sqrtpi = 2.0f * alpha * DYN_INV_SQRT_PI;
+  P.lj_cutoff = lj_cutoff;
+  P.coulomb_disabled = coulomb_disabled;
+  P.lj_disabled = lj_disabled;
+  P.sch = dyn_schedule((float)lambda);
+  const uint32_t ns = n_dyn + 4 * n_water;
+  memset(fx, 0, sizeof(int64_t) * 3 * ns);
+  double e = 0, w = 0, d = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    DynPairRec r{};
+    r.tgt = at_t[k] == 0 ? tgt[k] : n_dyn + (ws_t[k] - 1u) * n_water + tgt[k];
+    r.src = at_s[k] == 0 ? src[k] : n_dyn + (ws_s[k] - 1u) * n_water + src[k];
+    r.sigma = sigma[k];
+    r.eps = eps[k];
+    r.q_tgt = q_tgt[k];
+    r.q_src = q_src[k];
+    uint32_t f = 0;
+    if (scale14[k]) f |= DYN_PF_SCALE14;
+    if (calc_lj[k]) f |= DYN_PF_LJ;
+    if (calc_coul[k]) f |= DYN_PF_COUL;
+    if (symmetric[k]) f |= DYN_PF_SYMMETRIC;
+    if (alch && alch[k]) f |= DYN_PF_ALCH;
+    if (at_t[k] == 0) f |= DYN_PF_TGT_SOLUTE;
+    r.flags = f;
+    float ff[3], ee, dh;
+    double vir;
+    if (!dyn_ref_pair(pos, r, P, ff, &ee, &dh, &vir)) continue;
+    w += vir;
+    if (energy_all || (f & DYN_PF_TGT_SOLUTE)) e += ee;
+    if (f & DYN_PF_ALCH) d += dh;
+    for (int c = 0; c < 3; ++c) {
+      const int64_t v = dyn_fx(ff[c]);
+      fx[3 * r.tgt + c] += v;
+      if (f & DYN_PF_SYMMETRIC) fx[3 * r.src + c] -= v;
+    }
+  }
+  *energy = e;
+  *virial = w;
+  *dhdl = d;
+}
+
+}  // extern "C"
+
+// ---- native engine pair tiles -----------------------------------------------
+template <int KIND>
+static void tile(const float* xyzq, const int32_t* slot_type, uint32_t I, uint32_t J,
+                 uint64_t mask, const DynPairParams& P, const float2* lj, float fi[64][3],
+                 int64_t* fx, uint32_t n_slots) {
+  float g[64][3];
+  for (int l = 0; l < 64; ++l) {
+    const int ia = l & 7, jb = l >> 3;
+    const float* a = xyzq + 4 * (size_t)(I * DYN_CL_SIZE + ia);
+    const float* b = xyzq + 4 * (size_t)(J * DYN_CL_SIZE + jb);
+    const float pi[4] = {a[0], a[1], a[2], a[3]};
+    const float pj[4] = {b[0], b[1], b[2], b[3]};
+    const int ti = slot_type[I * DYN_CL_SIZE + ia];
+    const int tj = (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO) ? slot_type[J * DYN_CL_SIZE + jb] : 0;
+    const bool active = (mask >> l) & 1ull;
+    float d[3], s, elj, ec, r2;
+    int hit;
+    dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj, d, &s, &elj, &ec, &r2, &hit);
+    const float f0 = s * d[0], f1 = s * d[1], f2 = s * d[2];
+    fi[l][0] += f0;
+    fi[l][1] += f1;
+    fi[l][2] += f2;
+    g[l][0] = -f0;
+    g[l][1] = -f1;
+    g[l][2] = -f2;
+  }
+  for (int m : {1, 2, 4}) {
+    float h[64][3];
+    for (int l = 0; l < 64; ++l)
+      for (int c = 0; c < 3; ++c) h[l][c] = g[l][c] + g[l ^ m][c];
+    memcpy(g, h, sizeof g);
+  }
+  for (int jb = 0; jb < 8; ++jb) {
+    if (!((mask >> (8 * jb)) & 0xffull)) continue;
+    const uint32_t slot = J * DYN_CL_SIZE + jb;
+    for (int c = 0; c < 3; ++c) fx[(size_t)c * n_slots + slot] += dyn_fx(g[8 * jb][c]);
+  }
+}
+
+extern "C" {
+
+void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const uint8_t* cl_class,
+                      const uint32_t* row_ptr, const uint32_t* cj, const uint64_t* cmask,
+                      uint32_t n_rows, uint32_t n_slots, const float* lj_table, int n_types,
+                      const float* box, float cutoff, float lj_cutoff, float beta, float c6_oo,
+                      float c12_oo, int64_t* fx) {
+  DynPairParams P;
+  for (int k = 0; k < 3; ++k) {
+    P.ext[k] = box[k];
+    P.inv_ext[k] = 1.0f / box[k];
+  }
+  P.rc2 = cutoff * cutoff;
+  P.lj_rc2 = lj_cutoff * lj_cutoff;
+  P.beta = beta;
+  P.beta2 = beta * beta;
+  P.two_b_sqrtpi = 2.0f * beta * DYN_INV_SQRT_PI;
+  P.c6_oo = c6_oo;
+  P.c12_oo = c12_oo;
+  P.n_types = n_types;
+  const float2* lj = reinterpret_cast<const float2*>(lj_table);
+  memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
+  static float fi[64][3];
+  for (uint32_t I = 0; I < n_rows; ++I) {
+    memset(fi, 0, sizeof fi);
+    const int ci = cl_class[I];
+    for (uint32_t e = row_ptr[I]; e < row_ptr[I + 1]; ++e) {
+      const uint32_t J = cj[e];
+      const uint64_t mask = cmask[e];
+      switch (dyn_kind(ci, cl_class[J])) {
+        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        default: break;
+      }
+    }
+    if (row_ptr[I + 1] == row_ptr[I]) continue;
+    float v[64][3];
+    memcpy(v, fi, sizeof v);
+    for (int m : {8, 16, 32}) {
+      float h[64][3];
+      for (int l = 0; l < 64; ++l)
+        for (int c = 0; c < 3; ++c) h[l][c] = v[l][c] + v[l ^ m][c];
+      memcpy(v, h, sizeof v);
+    }
+    for (int ia = 0; ia < 8; ++ia) {
+      const uint32_t slot = I * DYN_CL_SIZE + ia;
+      for (int c = 0; c < 3; ++c) fx[(size_t)c * n_slots + slot] += dyn_fx(v[ia][c]);
+    }
+  }
+}
+
+// LJ table as the engine builds it: types + water O + "none" (zero).
+void orc_engine_lj_table(const float* sig_eps, int n_types_in, float* table_out) {
+  const int nt = n_types_in + 2;
+  std::vector<float> sig(nt, 0.f), eps(nt, 0.f);
+  for (int t = 0; t < n_types_in; ++t) {
+    sig[t] = sig_eps[2 * t];
+    eps[t] = sig_eps[2 * t + 1];
+  }
+  sig[n_types_in] = 1.777167268f * (2.0f / 1.122462048309373f);
+  eps[n_types_in] = 0.2128008130f;
+  for (int a = 0; a < nt; ++a)
+    for (int b = 0; b < nt; ++b) {
+      const float s_ab = 0.5f * (sig[a] + sig[b]);
+      const float e_ab = std::sqrt(eps[a] * eps[b]);
+      const double s6 = std::pow((double)s_ab, 6.0);
+      table_out[2 * (a * nt + b)] = (float)(4.0 * e_ab * s6);
+      table_out[2 * (a * nt + b) + 1] = (float)(4.0 * e_ab * s6 * s6);
+    }
+}
+
+// ---- SPME charge spreading -----------------------------------------------------
+void orc_pme_spread(const float* xyzq, uint32_t n, const int* K, const float* box, int order,
+                    int64_t* grid) {
+  float sc[3];
+  for (int a = 0; a < 3; ++a) sc[a] = (float)((double)K[a] / (double)box[a]);
+  memset(grid, 0, sizeof(int64_t) * (size_t)K[0] * K[1] * K[2]);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* p = xyzq + 4 * (size_t)i;
+    if (p[3] == 0.0f) continue;
+    int k0[3];
+    float th[3][5], dth[5];
+    for (int a = 0; a < 3; ++a) {
+      const float u = p[a] * sc[a];
+      const float fl = floorf(u);
+      const float w = u - fl;
+      int k = (int)fl % K[a];
+      if (k < 0) k += K[a];
+      k0[a] = k;
+      if (order == 4) dyn_bspline<4>(w, th[a], dth);
+      else dyn_bspline<5>(w, th[a], dth);
+    }
+    for (int jx = 0; jx < order; ++jx) {
+      const int gx = (k0[0] + K[0] - order + 1 + jx) % K[0];
+      const float qx = p[3] * th[0][jx];
+      for (int jy = 0; jy < order; ++jy) {
+        const int gy = (k0[1] + K[1] - order + 1 + jy) % K[1];
+        const float qxy = qx * th[1][jy];
+        for (int jz = 0; jz < order; ++jz) {
+          const int gz = (k0[2] + K[2] - order + 1 + jz) % K[2];
+          const float v = qxy * th[2][jz];
+          grid[((size_t)gx * K[1] + gy) * K[2] + gz] += dyn_fx_d((double)v, DYN_GRID_FX_SCALE);
+        }
+      }
+    }
+  }
+}
+
+}  // extern "C"

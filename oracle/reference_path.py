@@ -1,0 +1,222 @@
+"""Exact f32 restatement of the reference's neighbor-list build and pair-list
+flattening (test infrastructure only).
+
+  min_image_ref        src/barostat.rs:162-170  dv − round(dv/ext)·ext, round
+                       half away from zero, true f32 division, separate
+                       multiply and subtract (Rust does not contract)
+  wrap_ref             src/barostat.rs:141-157  (p − lo).rem_euclid(ext) + lo
+  build_neighbors      src/neighbors.rs:180-281 brute force, strict '<',
+                       symmetric lists ascending, static–static pairs skipped
+  build_all_neighbors  src/neighbors.rs:102-156 (water proxied by wrapped O)
+  setup_pairs          src/non_bonded.rs:541-677 (+ the σ/ε/charge resolution
+                       of PerNeighborGpu::new, src/gpu_interface.rs:242-411)
+
+Pinning: build_neighbors *is* the reference's brute-force algorithm; it is
+additionally checked against an f64 distance test away from the threshold and
+against symmetry/ordering invariants.  setup_pairs is checked against counts
+fixed by its rules (10 site pairs per water pair, 4 per solute–water pair,
+exclusions dropped, 1-4 flagged) in tests/test_oracle_reference_path.py.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def round_half_away_f32(x: np.ndarray) -> np.ndarray:
+    """Rust f32::round on f32 arrays, exactly (x − trunc(x) is exact)."""
+    x = np.asarray(x, F32)
+    t = np.trunc(x)
+    frac = (x - t).astype(F32)
+    adj = np.where(np.abs(frac) >= F32(0.5), np.sign(x), F32(0)).astype(F32)
+    return (t + adj).astype(F32)
+
+
+def min_image_ref(dv: np.ndarray, ext: np.ndarray) -> np.ndarray:
+    dv = np.asarray(dv, F32)
+    ext = np.asarray(ext, F32)
+    k = round_half_away_f32((dv / ext).astype(F32))
+    m = (k * ext).astype(F32)
+    return (dv - m).astype(F32)
+
+
+def norm2_ref(d: np.ndarray) -> np.ndarray:
+    d = np.asarray(d, F32)
+    a = (d[..., 0] * d[..., 0]).astype(F32)
+    b = (d[..., 1] * d[..., 1]).astype(F32)
+    c = (d[..., 2] * d[..., 2]).astype(F32)
+    return ((a + b).astype(F32) + c).astype(F32)
+
+
+def wrap_ref(p: np.ndarray, lo: np.ndarray, ext: np.ndarray) -> np.ndarray:
+    p = np.asarray(p, F32)
+    lo = np.asarray(lo, F32)
+    ext = np.asarray(ext, F32)
+    r = np.fmod((p - lo).astype(F32), ext).astype(F32)
+    r = np.where(r < 0, (r + np.abs(ext)).astype(F32), r).astype(F32)
+    return (r + lo).astype(F32)
+
+
+def build_neighbors(outer: np.ndarray, inner: np.ndarray, is_static, ext, symmetric: bool,
+                    skin_sq_w_cutoff: float, chunk: int = 512):
+    """Returns CSR (row_ptr uint32, col uint32), rows ascending."""
+    outer = np.asarray(outer, F32).reshape(-1, 3)
+    inner = np.asarray(inner, F32).reshape(-1, 3)
+    ext = np.asarray(ext, F32)
+    thr = F32(skin_sq_w_cutoff)
+    if is_static is not None and not symmetric:
+        raise ValueError("Invalid neighbor build config; can't pass static indices if non-symmetric.")
+    st = None if is_static is None else np.asarray(is_static, bool)
+    n_o, n_i = outer.shape[0], inner.shape[0]
+    rows = []
+    for b in range(0, n_o, chunk):
+        e = min(n_o, b + chunk)
+        d = min_image_ref(outer[b:e, None, :] - inner[None, :, :], ext)
+        ok = norm2_ref(d) < thr
+        if symmetric:
+            idx = np.arange(b, e)
+            ok[np.arange(e - b), idx] = False  # no self pairs
+            if st is not None:
+                ok &= ~(st[b:e, None] & st[None, :])
+        for r in range(e - b):
+            rows.append(np.nonzero(ok[r])[0].astype(np.uint32))
+    row_ptr = np.zeros(n_o + 1, np.uint32)
+    if rows:
+        row_ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.concatenate(rows).astype(np.uint32) if rows else np.zeros(0, np.uint32)
+    return row_ptr, col
+
+
+def csr_rows(row_ptr, col):
+    return [col[row_ptr[i]:row_ptr[i + 1]] for i in range(len(row_ptr) - 1)]
+
+
+def build_all_neighbors(system, skin: float = 4.0, cutoff: float = 10.0):
+    """std_std, std_water, water_water CSR lists at radius cutoff + skin;
+    water proxied by its O wrapped into the box (lo = 0)."""
+    ext = np.asarray(system.box, F32)
+    thr = F32((F32(cutoff) + F32(skin)) * (F32(cutoff) + F32(skin)))
+    lo = np.zeros(3, F32)
+    wpos = wrap_ref(system.pos_o, lo, ext)
+    apos = np.asarray(system.pos_dyn, F32)
+    static = np.zeros(system.n_dyn, bool)
+    ss = build_neighbors(apos, apos, static, ext, True, thr)
+    sw = build_neighbors(apos, wpos, None, ext, False, thr)
+    ww = build_neighbors(wpos, wpos, None, ext, True, thr)
+    return {"std_std": ss, "std_water": sw, "water_water": ww, "thr": thr, "water_proxy": wpos}
+
+
+def _lj_combine(s0, e0, s1, e1):
+    s = (F32(0.5) * (F32(s0) + F32(s1))).astype(F32) if isinstance(s0, np.ndarray) else F32(F32(0.5) * (F32(s0) + F32(s1)))
+    e = np.sqrt((np.asarray(e0, F32) * np.asarray(e1, F32)).astype(F32)).astype(F32)
+    return s, e
+
+
+def setup_pairs(system, nb, alch_atoms=None):
+    """Flattened pair list as PerNeighborGpu::new uploads it.
+
+    Order: solute–solute (j > i, excluded pairs dropped, 1-4 flagged), then
+    solute–water (4 site pairs per (atom, water): O LJ-only, M/H0/H1
+    Coulomb-only), then water–water (j > i, 10 site pairs per molecule pair:
+    O–O LJ plus 9 charged-site pairs).  All pairs symmetric."""
+    from dynamics_amd.systems import O_SIGMA, O_EPS, Q_H, Q_EP
+    nd, nw = system.n_dyn, system.n_water
+    excl = set(map(tuple, np.asarray(system.excl_12_13).tolist()))
+    s14 = set(map(tuple, np.asarray(system.pairs_14).tolist()))
+    alch = np.zeros(nd, bool) if alch_atoms is None else np.asarray(alch_atoms, bool)
+    sig = np.asarray(system.sigma_dyn, F32)
+    eps = np.asarray(system.eps_dyn, F32)
+    q = np.asarray(system.q_dyn, F32)
+    cols = {k: [] for k in ["tgt", "src", "sigma", "eps", "q_tgt", "q_src", "atom_type_tgt",
+                            "water_site_tgt", "atom_type_src", "water_site_src", "scale14",
+                            "calc_lj", "calc_coul", "symmetric", "alch"]}
+
+    def emit(**kw):
+        for k, v in kw.items():
+            cols[k].append(v)
+
+    # solute–solute
+    rp, col = nb["std_std"]
+    ti, tj = [], []
+    for i in range(nd):
+        for j in col[rp[i]:rp[i + 1]]:
+            j = int(j)
+            if j > i and (i, j) not in excl:
+                ti.append(i)
+                tj.append(j)
+    ti = np.array(ti, np.int64)
+    tj = np.array(tj, np.int64)
+    if ti.size:
+        s_ab = (F32(0.5) * (sig[ti] + sig[tj]).astype(F32)).astype(F32)
+        e_ab = np.sqrt((eps[ti] * eps[tj]).astype(F32)).astype(F32)
+        sc = np.array([(int(a), int(b)) in s14 for a, b in zip(ti, tj)], np.uint8)
+        n = ti.size
+        emit(tgt=ti, src=tj, sigma=s_ab, eps=e_ab, q_tgt=q[ti], q_src=q[tj],
+             atom_type_tgt=np.zeros(n, np.uint8), water_site_tgt=np.zeros(n, np.uint8),
+             atom_type_src=np.zeros(n, np.uint8), water_site_src=np.zeros(n, np.uint8),
+             scale14=sc, calc_lj=np.ones(n, np.uint8), calc_coul=np.ones(n, np.uint8),
+             symmetric=np.ones(n, np.uint8), alch=(alch[ti] ^ alch[tj]).astype(np.uint8))
+    # solute–water
+    rp, col = nb["std_water"]
+    ii, ww = [], []
+    for i in range(nd):
+        for w in col[rp[i]:rp[i + 1]]:
+            ii.append(i)
+            ww.append(int(w))
+    ii = np.array(ii, np.int64)
+    ww = np.array(ww, np.int64)
+    if ii.size:
+        s_w = (F32(0.5) * (sig[ii] + O_SIGMA).astype(F32)).astype(F32)
+        e_w = np.sqrt((eps[ii] * O_EPS).astype(F32)).astype(F32)
+        qsite = np.array([0.0, Q_EP, Q_H, Q_H], F32)
+        for site in (1, 2, 3, 4):
+            pass
+        # interleave the 4 sites per (atom, water) as the reference does
+        n = ii.size
+        rep = lambda a: np.repeat(a, 4)
+        site = np.tile(np.array([1, 2, 3, 4], np.uint8), n)
+        emit(tgt=rep(ii), src=rep(ww), sigma=rep(s_w), eps=rep(e_w), q_tgt=rep(q[ii]),
+             q_src=qsite[site - 1], atom_type_tgt=np.zeros(4 * n, np.uint8),
+             water_site_tgt=np.zeros(4 * n, np.uint8), atom_type_src=np.ones(4 * n, np.uint8),
+             water_site_src=site, scale14=np.zeros(4 * n, np.uint8),
+             calc_lj=(site == 1).astype(np.uint8), calc_coul=(site != 1).astype(np.uint8),
+             symmetric=np.ones(4 * n, np.uint8), alch=rep(alch[ii]).astype(np.uint8))
+    # water–water
+    rp, col = nb["water_water"]
+    a0, a1 = [], []
+    for i in range(nw):
+        for j in col[rp[i]:rp[i + 1]]:
+            j = int(j)
+            if j > i:
+                a0.append(i)
+                a1.append(j)
+    a0 = np.array(a0, np.int64)
+    a1 = np.array(a1, np.int64)
+    if a0.size:
+        combos = [(s0, s1) for s0 in (1, 2, 3, 4) for s1 in (1, 2, 3, 4)
+                  if (s0 == 1 and s1 == 1) or (s0 != 1 and s1 != 1)]
+        s0 = np.array([c[0] for c in combos], np.uint8)
+        s1 = np.array([c[1] for c in combos], np.uint8)
+        n = a0.size
+        k = len(combos)
+        qsite = np.array([0.0, Q_EP, Q_H, Q_H], F32)
+        site0 = np.tile(s0, n)
+        site1 = np.tile(s1, n)
+        lj = ((site0 == 1) & (site1 == 1))
+        emit(tgt=np.repeat(a0, k), src=np.repeat(a1, k),
+             sigma=np.full(n * k, O_SIGMA, F32), eps=np.full(n * k, O_EPS, F32),
+             q_tgt=qsite[site0 - 1], q_src=qsite[site1 - 1],
+             atom_type_tgt=np.ones(n * k, np.uint8), water_site_tgt=site0,
+             atom_type_src=np.ones(n * k, np.uint8), water_site_src=site1,
+             scale14=np.zeros(n * k, np.uint8), calc_lj=lj.astype(np.uint8),
+             calc_coul=(~lj).astype(np.uint8), symmetric=np.ones(n * k, np.uint8),
+             alch=np.zeros(n * k, np.uint8))
+    out = {}
+    dts = {"tgt": np.uint32, "src": np.uint32, "sigma": F32, "eps": F32, "q_tgt": F32,
+           "q_src": F32}
+    for k, v in cols.items():
+        dt = dts.get(k, np.uint8)
+        out[k] = np.concatenate(v).astype(dt) if v else np.zeros(0, dt)
+    return out

@@ -1,0 +1,143 @@
+"""Shared test setup: the `gpu` marker, in-tree builds when missing, and
+ctypes access to the oracle's host replay library."""
+
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+LIB = os.path.join(ROOT, "dynamics_amd", "libdyn_nb.so")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+CPUREF_LIB = os.path.join(ROOT, "oracle", "_build", "libcpuref.so")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    # A fresh checkout has no built libraries (they are git-ignored); build them
+    # here so the CPU suite can check the ABI and the host replays.
+    if not os.path.exists(LIB):
+        from dynamics_amd import _build
+        _build.build(verbose=False)
+    if not (os.path.exists(ORACLE_LIB) and os.path.exists(CPUREF_LIB)):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "cpu")], check=True)
+
+
+def has_gpu() -> bool:
+    from dynamics_amd import api
+    return api.device_count() > 0
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    if not has_gpu():
+        pytest.fail("no GPU visible: gpu-marked tests need an MI355X")
+    return 0
+
+
+_orc = None
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        L = C.CDLL(ORACLE_LIB)
+        f = C.c_float
+        for name in ["orc_rsqrtf", "orc_exp_neg", "orc_erfcx", "orc_erfcf", "orc_inv_cbrt"]:
+            getattr(L, name).restype = f
+            getattr(L, name).argtypes = [f]
+        L.orc_fx.restype = C.c_int64
+        L.orc_fx.argtypes = [f]
+        for name in ["orc_coul_erfc", "orc_coul_erf"]:
+            getattr(L, name).argtypes = [f, f, f, C.POINTER(f), C.POINTER(f)]
+        L.orc_lj.argtypes = [f, f, f, C.POINTER(f), C.POINTER(f)]
+        L.orc_soft_core.argtypes = [f, f, f, f, C.POINTER(f), C.POINTER(f), C.POINTER(f)]
+        L.orc_bspline4.argtypes = [f, C.c_void_p, C.c_void_p]
+        L.orc_bspline5.argtypes = [f, C.c_void_p, C.c_void_p]
+        L.orc_min_image_ref.restype = f
+        L.orc_min_image_ref.argtypes = [f, f]
+        L.orc_norm2_ref.restype = f
+        L.orc_norm2_ref.argtypes = [f, f, f]
+        _orc = L
+    return _orc
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def replay_ref_pairs(system, pairs, cutoff, alpha, lj_cutoff=np.inf, lambda_alch=0.0,
+                     coulomb_disabled=False, lj_disabled=False, energy_all=False):
+    """Host replay of the reference pair-list kernel: (fx (n_sites,3) int64,
+    energy, virial, dhdl), sites class-major."""
+    L = oracle_lib()
+    nd, nw = system.n_dyn, system.n_water
+    pos = np.ascontiguousarray(np.concatenate(
+        [system.pos_dyn, system.pos_o, system.pos_m, system.pos_h0, system.pos_h1]).astype(np.float32))
+    fx = np.zeros((nd + 4 * nw, 3), np.int64)
+    e, w, d = C.c_double(), C.c_double(), C.c_double()
+    arr = {k: np.ascontiguousarray(v) for k, v in pairs.items() if v is not None}
+    n = len(arr["tgt"])
+    L.orc_ref_pairs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32] + [C.c_void_p] * 15 + [
+        C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_double, C.c_int,
+        C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    ext = np.ascontiguousarray(system.box, np.float32)
+    L.orc_ref_pairs(ptr(pos), nd, nw, n, *[ptr(arr.get(k)) for k in [
+        "tgt", "src", "sigma", "eps", "q_tgt", "q_src", "atom_type_tgt", "water_site_tgt",
+        "atom_type_src", "water_site_src", "scale14", "calc_lj", "calc_coul", "symmetric",
+        "alch"]], ptr(ext), cutoff, alpha, lj_cutoff, int(coulomb_disabled), int(lj_disabled),
+        float(lambda_alch), int(energy_all), ptr(fx), C.byref(e), C.byref(w), C.byref(d))
+    return fx, e.value, w.value, d.value
+
+
+def engine_lj_table(system):
+    L = oracle_lib()
+    nt = system.lj_types.shape[0]
+    tab = np.zeros(((nt + 2) * (nt + 2), 2), np.float32)
+    L.orc_engine_lj_table.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    se = np.ascontiguousarray(system.lj_types, np.float32)
+    L.orc_engine_lj_table(ptr(se) if nt else None, nt, ptr(tab))
+    return tab
+
+
+def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta):
+    L = oracle_lib()
+    tab = engine_lj_table(system)
+    nt = system.lj_types.shape[0] + 2
+    o = system.lj_types.shape[0]
+    c6_oo, c12_oo = tab[o * nt + o]
+    ns = len(dbg["slot_site"])
+    fx = np.zeros((3, ns), np.int64)
+    L.orc_engine_tiles.argtypes = [C.c_void_p] * 6 + [C.c_uint32, C.c_uint32, C.c_void_p, C.c_int,
+                                                      C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                                      C.c_float, C.c_float, C.c_void_p]
+    box = np.ascontiguousarray(system.box, np.float32)
+    cm = np.ascontiguousarray(dbg["cmask"].astype(np.uint64))
+    L.orc_engine_tiles(ptr(dbg["xyzq"]), ptr(dbg["slot_type"]), ptr(dbg["cl_class"]),
+                       ptr(dbg["row_ptr"]), ptr(dbg["cj"]) if len(dbg["cj"]) else None,
+                       ptr(cm) if len(cm) else None, len(dbg["cl_class"]), ns, ptr(tab), nt,
+                       ptr(box), cutoff, lj_cutoff, beta, float(c6_oo), float(c12_oo), ptr(fx))
+    return fx
+
+
+def replay_pme_spread(xyzq, K, box, order):
+    L = oracle_lib()
+    grid = np.zeros(tuple(K), np.int64)
+    L.orc_pme_spread.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+    xyzq = np.ascontiguousarray(xyzq, np.float32)
+    Ka = np.ascontiguousarray(K, np.int32)
+    b = np.ascontiguousarray(box, np.float32)
+    L.orc_pme_spread(ptr(xyzq), xyzq.shape[0], ptr(Ka), ptr(b), order, ptr(grid))
+    return grid
+
+
+def rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
