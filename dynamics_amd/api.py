@@ -69,10 +69,12 @@ class Energies(C.Structure):
                 ("reference", C.c_double), ("virial_sr", C.c_double),
                 ("virial_recip", C.c_double), ("dh_dl", C.c_double),
                 ("pair_count", C.c_uint64), ("list_entries", C.c_uint64),
-                ("list_builds", C.c_uint64)]
+                ("list_builds", C.c_uint64), ("pair_count_kind", C.c_uint64 * 3)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["pair_count_kind"] = tuple(int(v) for v in self.pair_count_kind)
+        return d
 
 
 def _sig(name, res, args):

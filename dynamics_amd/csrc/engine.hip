@@ -348,7 +348,8 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
   if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) acc[solute ? 1 : 0] += (double)elj;
   if (KIND != DYN_KIND_SO && KIND != DYN_KIND_OO) acc[solute ? 3 : 2] += (double)ec;
   acc[4] += (double)(s * r2);
-  acc[5] += (double)hit;
+  // interacting pairs by work kind: LJ + Coulomb, LJ only, Coulomb only
+  acc[KIND == DYN_KIND_SS ? 5 : ((KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) ? 6 : 7)] += (double)hit;
   // Force on the j site: −f summed over the 8 i lanes of this j group.
   float g0 = -f0, g1 = -f1, g2 = -f2;
   g0 = xor_sum(g0, 1);
@@ -410,13 +411,13 @@ k_eng_pairs(PairArgs A) {
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_slots + slot), (unsigned long long)dyn_fx(fi[1]));
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_slots + slot), (unsigned long long)dyn_fx(fi[2]));
   }
-  for (int k = 0; k < 6; ++k) {
+  for (int k = 0; k < kNumEnergy; ++k) {
     double v = acc[k];
     for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
     acc[k] = v;
   }
   if (lane == 0)
-    for (int k = 0; k < kNumEnergy; ++k) A.partials[(size_t)I * kNumEnergy + k] = k < 6 ? acc[k] : 0.0;
+    for (int k = 0; k < kNumEnergy; ++k) A.partials[(size_t)I * kNumEnergy + k] = acc[k];
 }
 
 // Correction pairs: kind 1 = 1-4 pair (scaled real-space LJ + Coulomb, plus
@@ -432,6 +433,7 @@ struct CorrArgs {
   const uint8_t* kind;
   uint32_t n;
   int mode;
+  int recip_on;            // reciprocal corrections only accompany the SPME sum
   int64_t* fx;             // site forces, component-major
   uint32_t n_sites;
   double* partials;        // 6 per block
@@ -453,6 +455,7 @@ k_eng_corr(CorrArgs A) {
     const float rinv = dyn_rsqrtf(r2);
     const float qq = a.w * b.w;
     float s_sr = 0.f, s_rc = 0.f;
+    if (A.kind[k] == 0 && !A.recip_on) continue;
     if (A.kind[k] == 1) {
       if (r2 <= A.P.lj_rc2) {
         const float2 c = A.lj_table[A.ltype[i] * A.P.n_types + A.ltype[j]];
@@ -467,7 +470,9 @@ k_eng_corr(CorrArgs A) {
         s_sr += 0.833333333f * sc;
         acc[1] += (double)(0.833333333f * ec);
       }
-      if (A.mode == DYN_MODE_REFERENCE) {
+      if (!A.recip_on) {
+        // no reciprocal sum, nothing to correct
+      } else if (A.mode == DYN_MODE_REFERENCE) {
         // vacuum correction (1/1.2 − 1)·qq/r² along r̂, force only
         s_rc = (DYN_SCALE_COUL_14 - 1.0f) * qq * (rinv * rinv * rinv);
       } else {
@@ -900,6 +905,7 @@ void dyn_engine::compute(uint32_t flags) {
     C.kind = corr_kind.p;
     C.n = n_corr;
     C.mode = (int)prm.mode;
+    C.recip_on = (pme_on && (flags & DYN_COMPUTE_RECIP)) ? 1 : 0;
     C.fx = fx_site.p;
     C.n_sites = n_sites;
     C.partials = corr_part.p;
@@ -1273,7 +1279,7 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
                            hipMemcpyDeviceToHost, eng->stream));
     DYN_HIP(hipStreamSynchronize(eng->stream));
     const double* v = eng->h_eng.p;
-    // v[0..5]: e_lj_ww, e_lj_s, e_c_ww, e_c_s, virial_pairs, pair count
+    // v[0..7]: e_lj_ww, e_lj_s, e_c_ww, e_c_s, virial_pairs, pair counts by kind
     // v[8..12]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr; v[14..15]: pme energy, virial
     dyn_energies E{};
     E.lj = v[0] + v[1] + v[8];
@@ -1291,7 +1297,10 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     E.virial_sr = v[4] + v[11];
     E.virial_recip = (eng->pme_on ? v[15] : 0.0) + v[12];
     E.dh_dl = 0.0;
-    E.pair_count = (uint64_t)std::llround(v[5]);
+    E.pair_count_kind[0] = (uint64_t)std::llround(v[5]);
+    E.pair_count_kind[1] = (uint64_t)std::llround(v[6]);
+    E.pair_count_kind[2] = (uint64_t)std::llround(v[7]);
+    E.pair_count = E.pair_count_kind[0] + E.pair_count_kind[1] + E.pair_count_kind[2];
     E.list_entries = eng->n_entries;
     E.list_builds = eng->n_builds;
     *out = E;

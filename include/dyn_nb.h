@@ -245,6 +245,7 @@ typedef struct dyn_energies {
   uint64_t pair_count;  /* pairs inside the cutoff that interacted            */
   uint64_t list_entries;/* cluster-pair entries in the current list           */
   uint64_t list_builds; /* list builds so far                                 */
+  uint64_t pair_count_kind[3]; /* of pair_count: LJ + Coulomb, LJ only, Coulomb only */
 } dyn_energies;
 
 int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out);

@@ -436,7 +436,8 @@ def reference_energy_and_forces(system, rc, lj_rc, beta, K, order, mode="referen
         if K is not None:
             V = float(np.prod(box))
             e_self += -np.pi * float(q.sum()) ** 2 / (2 * V * beta ** 2)
-    out.update(forces=F, e_recip=e_rec, e_self=e_self, e_corr=e_corr, virial_recip=w_rec + w_corr)
+    out.update(forces=F, e_recip=e_rec, e_self=e_self, e_corr=e_corr, virial_recip=w_rec + w_corr,
+               virial_kspace=w_rec, virial_corr=w_corr)
     out["e_total"] = out["e_lj"] + out["e_coul"] + e_rec + e_self + e_corr
     out["e_reference"] = sr["e_lj_solute"] + sr["e_coul_solute"] + e_rec + e_self
     return out
