@@ -123,8 +123,8 @@ _sig("dyn_engine_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
 _sig("dyn_engine_forces_device", C.c_void_p, [_p])
 _sig("dyn_engine_get_energies", C.c_int, [_p, C.POINTER(Energies)])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
-                                          C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p,
-                                          _p])
+                                          C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
+                                          _p, C.POINTER(C.c_uint64), _p])
 _sig("dyn_engine_profile", C.c_int, [_p, C.c_int])
 _sig("dyn_engine_profile_read", C.c_int, [_p, C.c_uint32, C.POINTER(C.c_char_p),
                                           C.POINTER(C.c_double), C.POINTER(C.c_uint64),
@@ -381,20 +381,23 @@ class Engine:
         return e.as_dict()
 
     def debug_export(self):
-        """Cluster layout, list and per-slot fixed-point pair forces."""
-        ns, nc, ne = C.c_uint32(), C.c_uint32(), C.c_uint64()
-        _check(lib.dyn_engine_debug_export(self.h, C.byref(ns), C.byref(nc), C.byref(ne), *[None] * 8),
-               self._err)
-        ns, nc, ne = ns.value, nc.value, ne.value
+        """Cluster layout, list, work items and per-slot fixed-point pair forces."""
+        ns, nc, ne, ni = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint64()
+        _check(lib.dyn_engine_debug_export(self.h, C.byref(ns), C.byref(nc), C.byref(ne),
+                                           *[None] * 9, C.byref(ni), None), self._err)
+        ns, nc, ne, ni = ns.value, nc.value, ne.value, ni.value
         d = {"slot_site": np.zeros(ns, np.int32), "xyzq": np.zeros((ns, 4), np.float32),
-             "slot_type": np.zeros(ns, np.int32), "cl_class": np.zeros(nc, np.uint8),
-             "row_ptr": np.zeros(nc + 1, np.uint32), "cj": np.zeros(max(ne, 1), np.uint32),
-             "cmask": np.zeros(max(ne, 1), np.uint64), "fx_slot": np.zeros((3, ns), np.int64)}
-        _check(lib.dyn_engine_debug_export(self.h, None, None, None, *[_ptr(d[k]) for k in [
-            "slot_site", "xyzq", "slot_type", "cl_class", "row_ptr", "cj", "cmask", "fx_slot"]]),
-            self._err)
+             "slot_type": np.zeros(ns, np.int32), "slot_gid": np.zeros(ns, np.int32),
+             "cl_class": np.zeros(nc, np.uint8), "row_ptr": np.zeros(nc + 1, np.uint32),
+             "cj": np.zeros(max(ne, 1), np.uint32), "cmask": np.zeros(max(ne, 1), np.uint64),
+             "items": np.zeros((max(ni, 1), 2), np.uint32), "fx_slot": np.zeros((3, ns), np.int64)}
+        _check(lib.dyn_engine_debug_export(
+            self.h, None, None, None, *[_ptr(d[k]) for k in [
+                "slot_site", "xyzq", "slot_type", "slot_gid", "cl_class", "row_ptr", "cj",
+                "cmask", "items"]], None, _ptr(d["fx_slot"])), self._err)
         d["cj"] = d["cj"][:ne]
         d["cmask"] = d["cmask"][:ne]
+        d["items"] = d["items"][:ni]
         return d
 
     def profile(self, enable=True):

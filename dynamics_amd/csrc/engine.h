@@ -13,6 +13,18 @@
 #define DYN_CLS_WATER_Q 2
 
 #define DYN_CL_SIZE 8  // sites per cluster; a wave holds an 8×8 cluster pair
+#define DYN_CHUNK 32   // list entries per work item of the pair kernel
+
+// Exclusion group of a cluster slot: the water molecule (≥ 0) for water
+// sites, −2 − site for solute atoms (unique), −1 for padding.  Two slots
+// interact only if their groups differ and neither is padding; solute–solute
+// cluster pairs additionally carry an explicit 64-bit mask for the 1-2, 1-3
+// and 1-4 exclusions.
+#define DYN_GID_PAD (-1)
+
+DYN_HD bool dyn_slot_pair_valid(int gi, int gj) {
+  return gi != gj && gi != DYN_GID_PAD && gj != DYN_GID_PAD;
+}
 
 // Interaction kinds of a cluster pair (from the two classes).
 #define DYN_KIND_SS 0  // LJ (type table) + Coulomb

@@ -1,21 +1,28 @@
 // engine.hip — native nonbonded engine on gfx950.
 //
-// Pipeline of one evaluation (all on the handle's stream):
-//   prepare    wrap positions into the box, per-class column binning (rebuild)
-//   colsort    per column, bitonic sort by z, cut into clusters of 8 (rebuild)
-//   bbox/list  cluster bounding boxes, cluster-pair list with 64-bit
-//              exclusion masks, half list (each pair once)          (rebuild)
-//   slots      gather current positions/charges/types into cluster slots
-//   pairs      8×8 wave tiles over the list: LJ + erfc Coulomb, int64
-//              fixed-point force sums, f64 energy/virial per wave
-//   corr       1-4 pairs (scaled real-space) and reciprocal corrections
+// One evaluation, all on the handle's stream and without host round trips:
+//   bin        wrap positions into the box and counting-sort each site class
+//              into x–y columns in LDS (one workgroup per class)      (rebuild)
+//   colsort    per column, bitonic sort by z, cut into clusters of 8, write
+//              cluster slots and bounding boxes                       (rebuild)
+//   list       full cluster-pair list: every cluster lists every cluster
+//              whose box is within the cutoff (+ buffer), both ways; rows
+//              are cut into work items of ≤ DYN_CHUNK entries           (rebuild)
+//   slots      gather current positions into cluster slots       (reuse path)
+//   pairs      8×8 wave tiles over the work items: LJ + erfc Coulomb; each
+//              lane keeps its i-site force in registers, one fixed-point add
+//              per work item; no per-pair atomics and no j-side reductions
+//   corr       1-4 pairs (scaled real space) and reciprocal corrections
 //   pme        spread → FFT → convolve → inverse FFT → gather
-//   finalize   slot forces + site forces → site-order fixed-point forces,
+//   finalize   slot + site forces → site-order fixed-point forces, and a
 //              fixed-order energy reduction
+// Capacities are checked on the device; an overflow sets a status bit that
+// the host reads at the next synchronization and reports as an error, after
+// growing the buffers for the next call.
 //
 // Reference semantics: src/non_bonded.rs:258-412 (pair sum), :541-677 (pair
 // set), :681-795 (SPME and the 1-4 correction), src/prep.rs:114-143
-// (exclusions), src/cuda/cuda.cu:185-362 (kernel physics).
+// (exclusions), src/cuda/cuda.cu:185-362 (pair physics).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,21 +44,28 @@ using namespace dyn;
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kPairBlock = 256;       // 4 waves, one i-cluster row each
-constexpr int kListCap = 2048;        // j-clusters per i-cluster row
-constexpr int kColCap = 4096;         // sites per column
-constexpr int kNumEnergy = 8;         // per-row partials
+constexpr int kPairBlock = 256;       // 4 waves, one work item each at a time
+constexpr int kListCap = 2048;        // entries per cluster row (LDS staging)
+constexpr int kColCap = 4096;         // sites per column (LDS sort)
+constexpr int kMaxColsPerClass = 4096;
+constexpr int kNumPart = 8;           // f64 partials per work item
+
+// status bits
+constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u;
 
 struct ClassGrid {
   int ncx, ncy;
   float wx, wy;
-  int col_off;  // first global column of this class
+  int col_off;    // first global column of this class
+  int n;          // sites of this class
+  int site_base;  // first entry of this class in the column-sorted item array
 };
 
 struct EngGeom {
   float L[3], invL[3];
   ClassGrid cg[3];
   int n_cols;
+  uint32_t n_dyn, n_water;
 };
 
 __device__ __forceinline__ float wrap1(float x, float L, float invL) {
@@ -61,26 +75,77 @@ __device__ __forceinline__ float wrap1(float x, float L, float invL) {
   return w;
 }
 
-// Wrap all sites; when `bin`, also count sites per class column.
-__global__ void k_eng_prepare(const float4* __restrict__ pos, uint32_t n,
-                              const float* __restrict__ q, const uint8_t* __restrict__ cls,
-                              EngGeom g, int bin, float4* __restrict__ wpos,
-                              uint32_t* __restrict__ col_of, uint32_t* __restrict__ col_count) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const float4 p = pos[s];
-  const float x = wrap1(p.x, g.L[0], g.invL[0]);
-  const float y = wrap1(p.y, g.L[1], g.invL[1]);
-  const float z = wrap1(p.z, g.L[2], g.invL[2]);
-  wpos[s] = make_float4(x, y, z, q[s]);
-  if (bin) {
-    const ClassGrid cg = g.cg[cls[s]];
-    int cx = (int)(x / cg.wx), cy = (int)(y / cg.wy);
-    cx = min(max(cx, 0), cg.ncx - 1);
-    cy = min(max(cy, 0), cg.ncy - 1);
-    const uint32_t c = (uint32_t)(cg.col_off + cx * cg.ncy + cy);
-    col_of[s] = c;
-    atomicAdd(&col_count[c], 1u);
+// Site of the t-th member of a class (sites: solute, then water O, M, H0, H1
+// molecule-major).
+__device__ __forceinline__ uint32_t class_site(int cls, uint32_t t, uint32_t nd) {
+  if (cls == DYN_CLS_SOLUTE) return t;
+  if (cls == DYN_CLS_WATER_O) return nd + 4u * t;
+  const uint32_t m = t / 3u;
+  return nd + 4u * m + 1u + (t - 3u * m);
+}
+
+__device__ __forceinline__ uint32_t site_column(const ClassGrid& cg, float x, float y) {
+  int cx = (int)(x / cg.wx), cy = (int)(y / cg.wy);
+  cx = min(max(cx, 0), cg.ncx - 1);
+  cy = min(max(cy, 0), cg.ncy - 1);
+  return (uint32_t)(cx * cg.ncy + cy);
+}
+
+// One workgroup per class: wrap the class's sites into the box, count them
+// per column in LDS, scan, and scatter site indices into the column ranges.
+constexpr int kBinBlock = 1024;
+__global__ void __launch_bounds__(kBinBlock)
+k_eng_bin(const float4* __restrict__ pos, const float* __restrict__ q, EngGeom g,
+          float4* __restrict__ wpos, uint32_t* __restrict__ col_count,
+          uint32_t* __restrict__ col_start, uint32_t* __restrict__ items) {
+  __shared__ uint32_t hist[kMaxColsPerClass];
+  __shared__ uint32_t warp_tot[kBinBlock / kWave];
+  const int cls = blockIdx.x;
+  const ClassGrid cg = g.cg[cls];
+  if (cg.n == 0) return;
+  const int ncol = cg.ncx * cg.ncy;
+  for (int k = threadIdx.x; k < ncol; k += kBinBlock) hist[k] = 0;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < (uint32_t)cg.n; t += kBinBlock) {
+    const uint32_t s = class_site(cls, t, g.n_dyn);
+    const float4 p = pos[s];
+    const float x = wrap1(p.x, g.L[0], g.invL[0]);
+    const float y = wrap1(p.y, g.L[1], g.invL[1]);
+    const float z = wrap1(p.z, g.L[2], g.invL[2]);
+    wpos[s] = make_float4(x, y, z, q[s]);
+    atomicAdd(&hist[site_column(cg, x, y)], 1u);
+  }
+  __syncthreads();
+  // exclusive scan of hist (ncol ≤ 4096), a contiguous range per thread
+  const int per = (ncol + kBinBlock - 1) / kBinBlock;
+  const int b = threadIdx.x * per, e = min(ncol, b + per);
+  uint32_t loc = 0;
+  for (int k = b; k < e; ++k) loc += hist[k];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t incl = loc;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += v;
+  }
+  if (lane == kWave - 1) warp_tot[w] = incl;
+  __syncthreads();
+  uint32_t wbase = 0;
+  for (int k = 0; k < w; ++k) wbase += warp_tot[k];
+  uint32_t run = wbase + incl - loc;
+  __syncthreads();
+  for (int k = b; k < e; ++k) {
+    const uint32_t c = hist[k];
+    col_count[cg.col_off + k] = c;
+    col_start[cg.col_off + k] = (uint32_t)cg.site_base + run;
+    hist[k] = (uint32_t)cg.site_base + run;  // becomes the scatter cursor
+    run += c;
+  }
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < (uint32_t)cg.n; t += kBinBlock) {
+    const uint32_t s = class_site(cls, t, g.n_dyn);
+    const float4 p = wpos[s];
+    const uint32_t slot = atomicAdd(&hist[site_column(cg, p.x, p.y)], 1u);
+    items[slot] = s;
   }
 }
 
@@ -90,36 +155,94 @@ __global__ void k_eng_col_clusters(const uint32_t* __restrict__ cnt, uint32_t n,
   if (c < n) ncl[c] = (cnt[c] + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
 }
 
-__global__ void k_eng_scatter(uint32_t n, const uint32_t* __restrict__ col_of,
-                              const uint32_t* __restrict__ col_start,
-                              uint32_t* __restrict__ cursor, uint32_t* __restrict__ items) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t c = col_of[s];
-  items[col_start[c] + atomicAdd(&cursor[c], 1u)] = s;
+// Single-workgroup exclusive scan of n uint32 values (n from *n_dev when
+// non-null) into out[0..n], out[n] = total (also to *total_out); *maxv = max.
+constexpr int kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock)
+k_eng_scan(const uint32_t* __restrict__ in, uint32_t n_host, const uint32_t* __restrict__ n_dev,
+           uint32_t* __restrict__ out, uint32_t* __restrict__ total_out,
+           uint32_t* __restrict__ maxv) {
+  __shared__ uint32_t sh[kScanBlock];
+  __shared__ uint32_t shm[kScanBlock];
+  const uint32_t n = n_dev ? *n_dev : n_host;
+  const int t = threadIdx.x;
+  const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
+  const uint32_t b = t * per;
+  const uint32_t e = min(n, b + per);
+  uint32_t s = 0, m = 0;
+  for (uint32_t i = b; i < e; ++i) {
+    s += in[i];
+    m = max(m, in[i]);
+  }
+  sh[t] = s;
+  shm[t] = m;
+  __syncthreads();
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    const uint32_t v = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  for (int off = kScanBlock / 2; off > 0; off >>= 1) {
+    if (t < off) shm[t] = max(shm[t], shm[t + off]);
+    __syncthreads();
+  }
+  uint32_t run = sh[t] - s;
+  for (uint32_t i = b; i < e; ++i) {
+    out[i] = run;
+    run += in[i];
+  }
+  if (t == kScanBlock - 1) {
+    out[n] = sh[t];
+    if (total_out) *total_out = sh[t];
+  }
+  if (t == 0 && maxv) *maxv = shm[0];
 }
 
-// One workgroup per column: sort the column's sites by (z, site id) and cut
-// them into clusters of 8 (the last one padded with −1).
+struct SlotArgs {
+  const float4* wpos;
+  const int32_t* ltype;   // per site
+  const int32_t* wmol;    // per site: water molecule or −1
+  int none_type;
+  float4* xyzq;
+  int32_t* slot_type;
+  int32_t* slot_gid;
+  int32_t* slot_site;
+  uint32_t* slot_of;
+};
+
+__device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
+  const int32_t m = wmol[s];
+  return m >= 0 ? m : -2 - s;
+}
+
+// One workgroup per column: sort the column's sites by (z, site) and cut
+// them into clusters of 8 (the last padded), writing the slot arrays and each
+// cluster's bounding box.
 constexpr int kSortBlock = 256;
 __global__ void __launch_bounds__(kSortBlock)
-k_eng_colsort(const uint32_t* __restrict__ col_start, const uint32_t* __restrict__ items,
-              const uint32_t* __restrict__ col_cl_start, const float4* __restrict__ wpos,
-              const int* __restrict__ col_class, int32_t* __restrict__ slot_site,
-              uint8_t* __restrict__ cl_class, uint32_t* __restrict__ cl_col) {
+k_eng_colsort(const uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
+              const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
+              const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
+              float4* __restrict__ bb_c, float4* __restrict__ bb_h,
+              unsigned int* __restrict__ status) {
   __shared__ unsigned long long key[kColCap];
   const int c = blockIdx.x;
-  const uint32_t b = col_start[c], e = col_start[c + 1];
-  const uint32_t n = e - b;
+  const uint32_t n = col_count[c];
+  const uint32_t b = col_start[c];
   const uint32_t cl0 = col_cl_start[c], ncl = col_cl_start[c + 1] - cl0;
   if (ncl == 0) return;
+  if (n > (uint32_t)kColCap) {
+    if (threadIdx.x == 0) atomicOr(status, kStColumn);
+    return;
+  }
   uint32_t np2 = 1;
   while (np2 < n) np2 <<= 1;
   for (uint32_t k = threadIdx.x; k < np2; k += kSortBlock) {
     unsigned long long v = ~0ull;
     if (k < n) {
       const uint32_t s = items[b + k];
-      v = ((unsigned long long)__float_as_uint(wpos[s].z) << 32) | s;
+      v = ((unsigned long long)__float_as_uint(S.wpos[s].z) << 32) | s;
     }
     key[k] = v;
   }
@@ -139,116 +262,111 @@ k_eng_colsort(const uint32_t* __restrict__ col_start, const uint32_t* __restrict
       }
       __syncthreads();
     }
-  for (uint32_t k = threadIdx.x; k < ncl * DYN_CL_SIZE; k += kSortBlock)
-    slot_site[(size_t)cl0 * DYN_CL_SIZE + k] = k < n ? (int32_t)(key[k] & 0xffffffffu) : -1;
+  for (uint32_t k = threadIdx.x; k < ncl * DYN_CL_SIZE; k += kSortBlock) {
+    const size_t slot = (size_t)cl0 * DYN_CL_SIZE + k;
+    if (k < n) {
+      const int32_t s = (int32_t)(key[k] & 0xffffffffu);
+      S.slot_site[slot] = s;
+      S.xyzq[slot] = S.wpos[s];
+      S.slot_type[slot] = S.ltype[s];
+      S.slot_gid[slot] = site_gid(S.wmol, s);
+      S.slot_of[s] = (uint32_t)slot;
+    } else {
+      const int32_t s0 = (int32_t)(key[k & ~(uint32_t)(DYN_CL_SIZE - 1)] & 0xffffffffu);
+      float4 p = S.wpos[s0];
+      p.w = 0.0f;
+      S.slot_site[slot] = -1;
+      S.xyzq[slot] = p;
+      S.slot_type[slot] = S.none_type;
+      S.slot_gid[slot] = DYN_GID_PAD;
+    }
+  }
   for (uint32_t k = threadIdx.x; k < ncl; k += kSortBlock) {
-    cl_class[cl0 + k] = (uint8_t)col_class[c];
-    cl_col[cl0 + k] = (uint32_t)c;
+    const uint32_t cl = cl0 + k;
+    cl_class[cl] = (uint8_t)col_class[c];
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (uint32_t m = 0; m < DYN_CL_SIZE; ++m) {
+      const uint32_t r = k * DYN_CL_SIZE + m;
+      if (r >= n) break;
+      const float4 p = S.wpos[(int32_t)(key[r] & 0xffffffffu)];
+      lo[0] = fminf(lo[0], p.x);
+      lo[1] = fminf(lo[1], p.y);
+      lo[2] = fminf(lo[2], p.z);
+      hi[0] = fmaxf(hi[0], p.x);
+      hi[1] = fmaxf(hi[1], p.y);
+      hi[2] = fmaxf(hi[2], p.z);
+    }
+    bb_c[cl] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
+    bb_h[cl] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
   }
 }
 
-// Slot data from the current positions.  Padding slots copy the cluster's
-// first site position with q = 0 and the no-LJ type.
-__global__ void k_eng_slots(const int32_t* __restrict__ slot_site, uint32_t n_slots,
-                            const float4* __restrict__ wpos, const int32_t* __restrict__ ltype,
-                            int none_type, float4* __restrict__ xyzq,
-                            int32_t* __restrict__ slot_type, uint32_t* __restrict__ slot_of,
-                            int write_map) {
+// Wrap all sites (list-reuse path).
+__global__ void k_eng_wrap(const float4* __restrict__ pos, const float* __restrict__ q,
+                           uint32_t n, EngGeom g, float4* __restrict__ wpos) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float4 p = pos[s];
+  wpos[s] = make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
+                        wrap1(p.z, g.L[2], g.invL[2]), q[s]);
+}
+
+// Slot positions from the current wrapped positions (list-reuse path).
+__global__ void k_eng_slots(const int32_t* __restrict__ slot_site, const uint32_t* __restrict__ n_cl,
+                            const float4* __restrict__ wpos, float4* __restrict__ xyzq) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_slots) return;
+  if (k >= *n_cl * DYN_CL_SIZE) return;
   const int32_t s = slot_site[k];
   if (s >= 0) {
     xyzq[k] = wpos[s];
-    slot_type[k] = ltype[s];
-    if (write_map) slot_of[s] = k;
   } else {
-    const int32_t s0 = slot_site[k & ~(uint32_t)(DYN_CL_SIZE - 1)];
-    float4 p = wpos[s0];
+    float4 p = wpos[slot_site[k & ~(uint32_t)(DYN_CL_SIZE - 1)]];
     p.w = 0.0f;
     xyzq[k] = p;
-    slot_type[k] = none_type;
   }
-}
-
-__global__ void k_eng_bbox(const float4* __restrict__ xyzq, const int32_t* __restrict__ slot_site,
-                           uint32_t n_cl, float4* __restrict__ bb_c, float4* __restrict__ bb_h) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cl) return;
-  float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-  for (int k = 0; k < DYN_CL_SIZE; ++k) {
-    if (slot_site[c * DYN_CL_SIZE + k] < 0) continue;
-    const float4 p = xyzq[c * DYN_CL_SIZE + k];
-    lo[0] = fminf(lo[0], p.x);
-    lo[1] = fminf(lo[1], p.y);
-    lo[2] = fminf(lo[2], p.z);
-    hi[0] = fmaxf(hi[0], p.x);
-    hi[1] = fmaxf(hi[1], p.y);
-    hi[2] = fmaxf(hi[2], p.z);
-  }
-  bb_c[c] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
-  bb_h[c] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
 }
 
 struct ListArgs {
   EngGeom g;
-  float rlist2;
+  float rlist2, rlist;
+  const uint32_t* n_cl;
   const float4* bb_c;
   const float4* bb_h;
   const uint32_t* col_cl_start;
   const uint8_t* cl_class;
   const int32_t* slot_site;
-  const int32_t* wmol;       // per site: water molecule or −1
   const uint32_t* ex_ptr;    // per solute site: exclusion CSR (1-2, 1-3 and 1-4)
   const uint32_t* ex_idx;
   uint32_t n_dyn;
+  uint64_t cap_entries, cap_chunks;
+  unsigned int* status;
 };
 
-__device__ __forceinline__ bool excluded(const ListArgs& A, int32_t sa, int32_t sb) {
-  if (sa < 0 || sb < 0) return true;
-  if ((uint32_t)sa < A.n_dyn && (uint32_t)sb < A.n_dyn) {
-    uint32_t lo = A.ex_ptr[sa], hi = A.ex_ptr[sa + 1];
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      const uint32_t v = A.ex_idx[mid];
-      if (v == (uint32_t)sb) return true;
-      if (v < (uint32_t)sb) lo = mid + 1;
-      else hi = mid;
-    }
-    return false;
+__device__ __forceinline__ bool solute_excluded(const ListArgs& A, int32_t sa, int32_t sb) {
+  uint32_t lo = A.ex_ptr[sa], hi = A.ex_ptr[sa + 1];
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint32_t v = A.ex_idx[mid];
+    if (v == (uint32_t)sb) return true;
+    if (v < (uint32_t)sb) lo = mid + 1;
+    else hi = mid;
   }
-  if ((uint32_t)sa >= A.n_dyn && (uint32_t)sb >= A.n_dyn) return A.wmol[sa] == A.wmol[sb];
   return false;
 }
 
-__device__ __forceinline__ bool class_pair_allowed(int ci, int cj) {
-  // Half list over class-major cluster numbering: a row lists classes ≥ its
-  // own; O–charged pairs carry no interaction.
-  if (cj < ci) return false;
-  return dyn_kind(ci, cj) != DYN_KIND_NONE;
-}
-
-// One wave (one workgroup of 64) per i-cluster.  PASS 0 counts entries;
-// PASS 1 collects them in LDS in enumeration order, builds each entry's mask
-// and writes the row.
-template <int PASS>
-__global__ void __launch_bounds__(kWave)
-k_eng_list(ListArgs A, uint32_t n_cl, uint32_t* __restrict__ row_cnt,
-           const uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ cj,
-           unsigned long long* __restrict__ cmask) {
-  __shared__ uint32_t ent[PASS == 1 ? kListCap : 1];
-  const uint32_t I = blockIdx.x;
-  if (I >= n_cl) return;
-  const int lane = threadIdx.x;
-  const int ci = A.cl_class[I];
+// Enumerates, in a fixed order, the clusters J whose bounding box is within
+// rlist of cluster I's box (periodic), for the classes that interact with
+// I's class; fn(J, ok) receives each 64-lane batch of candidates.
+template <typename F>
+__device__ __forceinline__ void for_candidates(const ListArgs& A, uint32_t I, int ci, F fn) {
+  const int lane = threadIdx.x & (kWave - 1);
   const float4 c = A.bb_c[I], h = A.bb_h[I];
-  const float r = sqrtf(A.rlist2);
-  uint32_t cnt = 0;
   for (int cls = 0; cls < 3; ++cls) {
-    if (!class_pair_allowed(ci, cls)) continue;
+    if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
     const ClassGrid cg = A.g.cg[cls];
-    if (cg.ncx == 0) continue;
-    int x0 = (int)floorf((c.x - h.x - r) / cg.wx), x1 = (int)floorf((c.x + h.x + r) / cg.wx);
-    int y0 = (int)floorf((c.y - h.y - r) / cg.wy), y1 = (int)floorf((c.y + h.y + r) / cg.wy);
+    if (cg.n == 0) continue;
+    int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
+    int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
     if (x1 - x0 + 1 >= cg.ncx) {
       x0 = 0;
       x1 = cg.ncx - 1;
@@ -266,7 +384,6 @@ k_eng_list(ListArgs A, uint32_t n_cl, uint32_t* __restrict__ row_cnt,
         for (uint32_t base = js; base < je; base += kWave) {
           const uint32_t J = base + lane;
           bool ok = J < je;
-          if (ok && cls == ci) ok = J >= I;
           if (ok) {
             const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
             float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
@@ -278,36 +395,79 @@ k_eng_list(ListArgs A, uint32_t n_cl, uint32_t* __restrict__ row_cnt,
             dz = fmaxf(0.f, dz - h.z - hJ.z);
             ok = dx * dx + dy * dy + dz * dz < A.rlist2;
           }
-          const unsigned long long bal = __ballot(ok);
-          if (PASS == 1 && ok) {
-            const uint32_t pre = __popcll(bal & ((1ull << lane) - 1ull));
-            const uint32_t slot = cnt + pre;
-            if (slot < (uint32_t)kListCap) ent[slot] = J;
-          }
-          cnt += (uint32_t)__popcll(bal);
+          fn(J, ok);
         }
       }
     }
   }
-  if (PASS == 0) {
-    if (lane == 0) row_cnt[I] = cnt;
-    return;
-  }
-  __syncthreads();
-  const uint32_t base = row_ptr[I];
-  const int ia = lane & 7, jb = lane >> 3;
-  const int32_t sa = A.slot_site[I * DYN_CL_SIZE + ia];
-  const uint32_t m = min(cnt, (uint32_t)kListCap);
-  for (uint32_t k = 0; k < m; ++k) {
-    const uint32_t J = ent[k];
-    const int32_t sb = A.slot_site[J * DYN_CL_SIZE + jb];
-    bool ok = !(I == J && ia >= jb);
-    if (ok) ok = !excluded(A, sa, sb);
-    const unsigned long long mk = __ballot(ok);
-    if (lane == 0) {
-      cj[base + k] = J;
-      cmask[base + k] = mk;
+}
+
+// Pass 0: entries per row and work items per row.
+__global__ void __launch_bounds__(kWave)
+k_eng_list_count(ListArgs A, uint32_t* __restrict__ row_cnt, uint32_t* __restrict__ row_chunks) {
+  const uint32_t n_cl = *A.n_cl;
+  for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
+    uint32_t cnt = 0;
+    for_candidates(A, I, A.cl_class[I], [&](uint32_t, bool ok) {
+      cnt += (uint32_t)__popcll(__ballot(ok));
+    });
+    if (threadIdx.x == 0) {
+      if (cnt > (uint32_t)kListCap) atomicOr(A.status, kStRow);
+      const uint32_t m = min(cnt, (uint32_t)kListCap);
+      row_cnt[I] = m;
+      row_chunks[I] = (m + DYN_CHUNK - 1) / DYN_CHUNK;
     }
+  }
+}
+
+// Pass 1: write the row's entries (enumeration order), the solute–solute
+// exclusion masks, and the row's work items (I, first entry).
+__global__ void __launch_bounds__(kWave)
+k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ row_ptr,
+                const uint32_t* __restrict__ chunk_ptr, uint32_t* __restrict__ cj,
+                unsigned long long* __restrict__ cmask, uint2* __restrict__ chunks) {
+  __shared__ uint32_t ent[kListCap];
+  const uint32_t n_cl = *A.n_cl;
+  const int lane = threadIdx.x;
+  const int ia = lane & 7, jb = lane >> 3;
+  for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
+    const int ci = A.cl_class[I];
+    uint32_t cnt = 0;
+    for_candidates(A, I, ci, [&](uint32_t J, bool ok) {
+      const unsigned long long bal = __ballot(ok);
+      if (ok) {
+        const uint32_t slot = cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (slot < (uint32_t)kListCap) ent[slot] = J;
+      }
+      cnt += (uint32_t)__popcll(bal);
+    });
+    __syncthreads();
+    const uint32_t m = min(cnt, (uint32_t)kListCap);
+    const uint64_t base = row_ptr[I];
+    if (base + m <= A.cap_entries) {
+      for (uint32_t k = lane; k < m; k += kWave) cj[base + k] = ent[k];
+      if (ci == DYN_CLS_SOLUTE) {
+        const int32_t sa = A.slot_site[I * DYN_CL_SIZE + ia];
+        for (uint32_t k = 0; k < m; ++k) {
+          const uint32_t J = ent[k];
+          if (A.cl_class[J] != DYN_CLS_SOLUTE) break;  // solute clusters come first
+          const int32_t sb = A.slot_site[J * DYN_CL_SIZE + jb];
+          const bool ok = !(sa >= 0 && sb >= 0 && solute_excluded(A, sa, sb));
+          const unsigned long long mk = __ballot(ok);
+          if (lane == 0) cmask[base + k] = mk;
+        }
+      }
+    } else if (lane == 0) {
+      atomicOr(A.status, kStEntries);
+    }
+    const uint32_t c0 = chunk_ptr[I], nch = (m + DYN_CHUNK - 1) / DYN_CHUNK;
+    if ((uint64_t)c0 + nch <= A.cap_chunks) {
+      for (uint32_t k = lane; k < nch; k += kWave)
+        chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * DYN_CHUNK);
+    } else if (lane == 0) {
+      atomicOr(A.status, kStChunks);
+    }
+    __syncthreads();
   }
 }
 
@@ -315,109 +475,135 @@ struct PairArgs {
   DynPairParams P;
   const float4* xyzq;
   const int32_t* slot_type;
+  const int32_t* slot_gid;
   const uint8_t* cl_class;
   const uint32_t* row_ptr;
   const uint32_t* cj;
   const unsigned long long* cmask;
+  const uint2* chunks;
+  const uint32_t* n_chunks;
   const float2* lj_table;
   int64_t* fx;          // component-major, n_slots per component
   uint32_t n_slots;
-  double* partials;     // kNumEnergy per row
-  uint32_t n_rows;
+  double* partials;     // kNumPart per work item
+  uint64_t cap_chunks;
+  const unsigned int* status;  // a list-build overflow leaves the list unusable
 };
 
 __device__ __forceinline__ float xor_sum(float v, int m) { return v + __shfl_xor(v, m, kWave); }
 __device__ __forceinline__ double xor_sum_d(double v, int m) { return v + __shfl_xor(v, m, kWave); }
 
+struct TileAcc {
+  float fi[3];
+  float e_lj_s, e_lj_w, e_c_s, e_c_w, vir;
+  uint32_t n_ss, n_lj, n_c;  // wave-uniform counts (popcounts)
+};
+
 template <int KIND>
 __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4], int ti,
-                                           uint32_t J, unsigned long long mask, int lane, int ia,
-                                           int jb, float fi[3], double acc[kNumEnergy]) {
-  const float4 pjv = A.xyzq[J * DYN_CL_SIZE + jb];
+                                           const float4& pjv, int tj, bool active, TileAcc& T) {
   const float pj[4] = {pjv.x, pjv.y, pjv.z, pjv.w};
-  const int tj = (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO) ? A.slot_type[J * DYN_CL_SIZE + jb] : 0;
-  const bool active = (mask >> lane) & 1ull;
   float d[3], s, elj, ec, r2;
   int hit;
   dyn_tile_pair<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, d, &s, &elj, &ec, &r2, &hit);
-  const float f0 = s * d[0], f1 = s * d[1], f2 = s * d[2];
-  fi[0] += f0;
-  fi[1] += f1;
-  fi[2] += f2;
+  T.fi[0] += s * d[0];
+  T.fi[1] += s * d[1];
+  T.fi[2] += s * d[2];
   const bool solute = KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_SQ;
-  if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) acc[solute ? 1 : 0] += (double)elj;
-  if (KIND != DYN_KIND_SO && KIND != DYN_KIND_OO) acc[solute ? 3 : 2] += (double)ec;
-  acc[4] += (double)(s * r2);
-  // interacting pairs by work kind: LJ + Coulomb, LJ only, Coulomb only
-  acc[KIND == DYN_KIND_SS ? 5 : ((KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) ? 6 : 7)] += (double)hit;
-  // Force on the j site: −f summed over the 8 i lanes of this j group.
-  float g0 = -f0, g1 = -f1, g2 = -f2;
-  g0 = xor_sum(g0, 1);
-  g1 = xor_sum(g1, 1);
-  g2 = xor_sum(g2, 1);
-  g0 = xor_sum(g0, 2);
-  g1 = xor_sum(g1, 2);
-  g2 = xor_sum(g2, 2);
-  g0 = xor_sum(g0, 4);
-  g1 = xor_sum(g1, 4);
-  g2 = xor_sum(g2, 4);
-  if (ia == 0 && (mask >> (8 * jb)) & 0xffull) {
-    const uint32_t slot = J * DYN_CL_SIZE + jb;
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + slot), (unsigned long long)dyn_fx(g0));
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_slots + slot), (unsigned long long)dyn_fx(g1));
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_slots + slot), (unsigned long long)dyn_fx(g2));
+  if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) {
+    if (solute) T.e_lj_s += elj;
+    else T.e_lj_w += elj;
   }
+  if (KIND != DYN_KIND_SO && KIND != DYN_KIND_OO) {
+    if (solute) T.e_c_s += ec;
+    else T.e_c_w += ec;
+  }
+  T.vir += s * r2;
+  const uint32_t h = (uint32_t)__popcll(__ballot(hit != 0));
+  if (KIND == DYN_KIND_SS) T.n_ss += h;
+  else if (KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) T.n_lj += h;
+  else T.n_c += h;
 }
 
-// One wave per i-cluster row; lane = ia + 8·jb holds pair (slot 8I+ia, slot 8J+jb).
+// One wave per work item (≤ DYN_CHUNK entries of one cluster row); lane =
+// ia + 8·jb holds the pair (slot 8I+ia, slot 8J+jb).  The next entry's j data
+// is loaded before the current entry is evaluated.
 __global__ void __launch_bounds__(kPairBlock)
 k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
-  const uint32_t I = blockIdx.x * (kPairBlock / kWave) + (threadIdx.x / kWave);
-  if (I >= A.n_rows) return;
   const int ia = lane & 7, jb = lane >> 3;
-  const float4 piv = A.xyzq[I * DYN_CL_SIZE + ia];
-  const float pi[4] = {piv.x, piv.y, piv.z, piv.w};
-  const int ti = A.slot_type[I * DYN_CL_SIZE + ia];
-  const int ci = A.cl_class[I];
-  float fi[3] = {0.f, 0.f, 0.f};
-  double acc[kNumEnergy];
-  for (int k = 0; k < kNumEnergy; ++k) acc[k] = 0.0;
-  const uint32_t rs = A.row_ptr[I], re = A.row_ptr[I + 1];
-  for (uint32_t e = rs; e < re; ++e) {
-    const uint32_t J = __builtin_amdgcn_readfirstlane(A.cj[e]);
-    const unsigned long long mask = A.cmask[e];
-    const int kind = dyn_kind(ci, A.cl_class[J]);
-    switch (kind) {
-      case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ>(A, pi, ti, J, mask, lane, ia, jb, fi, acc); break;
-      case DYN_KIND_OO: tile_entry<DYN_KIND_OO>(A, pi, ti, J, mask, lane, ia, jb, fi, acc); break;
-      case DYN_KIND_SS: tile_entry<DYN_KIND_SS>(A, pi, ti, J, mask, lane, ia, jb, fi, acc); break;
-      case DYN_KIND_SO: tile_entry<DYN_KIND_SO>(A, pi, ti, J, mask, lane, ia, jb, fi, acc); break;
-      case DYN_KIND_SQ: tile_entry<DYN_KIND_SQ>(A, pi, ti, J, mask, lane, ia, jb, fi, acc); break;
-      default: break;
+  if (*A.status) return;
+  const uint32_t n_items = (uint32_t)min((uint64_t)*A.n_chunks, A.cap_chunks);
+  const uint32_t wave0 = blockIdx.x * (kPairBlock / kWave) + (threadIdx.x / kWave);
+  const uint32_t nwaves = gridDim.x * (kPairBlock / kWave);
+  for (uint32_t item = wave0; item < n_items; item += nwaves) {
+    const uint2 ch = A.chunks[item];
+    const uint32_t I = __builtin_amdgcn_readfirstlane(ch.x);
+    const uint32_t e0 = __builtin_amdgcn_readfirstlane(ch.y);
+    const uint32_t e1 = min(e0 + (uint32_t)DYN_CHUNK, A.row_ptr[I + 1]);
+    const float4 piv = A.xyzq[I * DYN_CL_SIZE + ia];
+    const float pi[4] = {piv.x, piv.y, piv.z, piv.w};
+    const int ti = A.slot_type[I * DYN_CL_SIZE + ia];
+    const int gi = A.slot_gid[I * DYN_CL_SIZE + ia];
+    const int ci = A.cl_class[I];
+    TileAcc T;
+    T.fi[0] = T.fi[1] = T.fi[2] = 0.f;
+    T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = 0.f;
+    T.n_ss = T.n_lj = T.n_c = 0;
+    uint32_t J = __builtin_amdgcn_readfirstlane(A.cj[e0]);
+    float4 pj = A.xyzq[J * DYN_CL_SIZE + jb];
+    int tj = A.slot_type[J * DYN_CL_SIZE + jb];
+    int gj = A.slot_gid[J * DYN_CL_SIZE + jb];
+    for (uint32_t e = e0; e < e1; ++e) {
+      const uint32_t Jc = J;
+      const float4 pjc = pj;
+      const int tjc = tj, gjc = gj;
+      if (e + 1 < e1) {
+        J = __builtin_amdgcn_readfirstlane(A.cj[e + 1]);
+        pj = A.xyzq[J * DYN_CL_SIZE + jb];
+        tj = A.slot_type[J * DYN_CL_SIZE + jb];
+        gj = A.slot_gid[J * DYN_CL_SIZE + jb];
+      }
+      const int kind = dyn_kind(ci, A.cl_class[Jc]);
+      bool active = dyn_slot_pair_valid(gi, gjc);
+      if (kind == DYN_KIND_SS) active = active && ((A.cmask[e] >> lane) & 1ull);
+      switch (kind) {
+        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_OO: tile_entry<DYN_KIND_OO>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_SS: tile_entry<DYN_KIND_SS>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_SO: tile_entry<DYN_KIND_SO>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_SQ: tile_entry<DYN_KIND_SQ>(A, pi, ti, pjc, tjc, active, T); break;
+        default: break;
+      }
+    }
+    // i forces: sum over the 8 j groups (lanes ia, ia+8, …, ia+56)
+    for (int c = 0; c < 3; ++c) {
+      float v = T.fi[c];
+      v = xor_sum(v, 8);
+      v = xor_sum(v, 16);
+      v = xor_sum(v, 32);
+      T.fi[c] = v;
+    }
+    if (jb == 0) {
+      const uint32_t slot = I * DYN_CL_SIZE + ia;
+      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + slot), (unsigned long long)dyn_fx(T.fi[0]));
+      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_slots + slot), (unsigned long long)dyn_fx(T.fi[1]));
+      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_slots + slot), (unsigned long long)dyn_fx(T.fi[2]));
+    }
+    double acc[5] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir};
+    for (int k = 0; k < 5; ++k) {
+      double v = acc[k];
+      for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
+      acc[k] = v;
+    }
+    if (lane == 0) {
+      double* p = A.partials + (size_t)item * kNumPart;
+      for (int k = 0; k < 5; ++k) p[k] = acc[k];
+      p[5] = (double)T.n_ss;
+      p[6] = (double)T.n_lj;
+      p[7] = (double)T.n_c;
     }
   }
-  // i forces: sum over the 8 j groups (lanes ia, ia+8, …, ia+56).
-  for (int c = 0; c < 3; ++c) {
-    float v = fi[c];
-    v = xor_sum(v, 8);
-    v = xor_sum(v, 16);
-    v = xor_sum(v, 32);
-    fi[c] = v;
-  }
-  if (jb == 0 && re > rs) {
-    const uint32_t slot = I * DYN_CL_SIZE + ia;
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + slot), (unsigned long long)dyn_fx(fi[0]));
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_slots + slot), (unsigned long long)dyn_fx(fi[1]));
-    atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_slots + slot), (unsigned long long)dyn_fx(fi[2]));
-  }
-  for (int k = 0; k < kNumEnergy; ++k) {
-    double v = acc[k];
-    for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
-    acc[k] = v;
-  }
-  if (lane == 0)
-    for (int k = 0; k < kNumEnergy; ++k) A.partials[(size_t)I * kNumEnergy + k] = acc[k];
 }
 
 // Correction pairs: kind 1 = 1-4 pair (scaled real-space LJ + Coulomb, plus
@@ -445,6 +631,7 @@ k_eng_corr(CorrArgs A) {
   __shared__ double red[kCorrBlock];
   double acc[6] = {0, 0, 0, 0, 0, 0};  // e_lj14, e_c14, e_corr, vir_sr, vir_recip, unused
   for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += gridDim.x * kCorrBlock) {
+    if (A.kind[k] == 0 && !A.recip_on) continue;
     const uint32_t i = A.ci[k], j = A.cj[k];
     const float4 a = A.wpos[i], b = A.wpos[j];
     float d0 = dyn_min_image_rint(a.x - b.x, A.P.ext[0], A.P.inv_ext[0]);
@@ -455,7 +642,6 @@ k_eng_corr(CorrArgs A) {
     const float rinv = dyn_rsqrtf(r2);
     const float qq = a.w * b.w;
     float s_sr = 0.f, s_rc = 0.f;
-    if (A.kind[k] == 0 && !A.recip_on) continue;
     if (A.kind[k] == 1) {
       if (r2 <= A.P.lj_rc2) {
         const float2 c = A.lj_table[A.ltype[i] * A.P.n_types + A.ltype[j]];
@@ -525,33 +711,30 @@ __global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
                                uint32_t n_slots, int64_t* __restrict__ out) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_sites) return;
-  const uint32_t k = slot_of[s];
+  const uint32_t k = min(slot_of[s], n_slots - 1);
   for (int c = 0; c < 3; ++c)
     out[(size_t)c * n_sites + s] = fx_site[(size_t)c * n_sites + s] + fx_slot[(size_t)c * n_slots + k];
 }
 
-// Fixed-order sums of the per-row and per-block partials.
-__global__ void k_eng_energy_reduce(const double* __restrict__ pp, uint32_t n_rows,
-                                    const double* __restrict__ cp, uint32_t n_cblk,
-                                    double* __restrict__ out) {
-  __shared__ double red[256];
-  for (int v = 0; v < 14; ++v) {
-    double a = 0.0;
-    if (v < kNumEnergy) {
-      for (uint32_t r = threadIdx.x; r < n_rows; r += 256) a += pp[(size_t)r * kNumEnergy + v];
-    } else {
-      const int w = v - kNumEnergy;
-      for (uint32_t r = threadIdx.x; r < n_cblk; r += 256) a += cp[(size_t)r * 6 + w];
-    }
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[v] = red[0];
-    __syncthreads();
+// Fixed-order sums: one wave per value over the per-item partials (the full
+// list visits every pair twice, hence the ½) and the correction partials.
+// out[0..7]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c;
+// out[8..13]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, 0.
+__global__ void __launch_bounds__(1024)
+k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
+                    uint64_t cap_items, const double* __restrict__ cp, uint32_t n_cblk,
+                    double* __restrict__ out) {
+  const int lane = threadIdx.x & (kWave - 1), v = threadIdx.x / kWave;
+  if (v >= 14) return;
+  double a = 0.0;
+  if (v < kNumPart) {
+    const uint32_t n = (uint32_t)min((uint64_t)*n_items_dev, cap_items);
+    for (uint32_t r = lane; r < n; r += kWave) a += pp[(size_t)r * kNumPart + v];
+  } else {
+    for (uint32_t r = lane; r < n_cblk; r += kWave) a += cp[(size_t)r * 6 + (v - kNumPart)];
   }
+  for (int m = 1; m < kWave; m <<= 1) a += __shfl_xor(a, m, kWave);
+  if (lane == 0) out[v] = v < kNumPart ? 0.5 * a : a;
 }
 
 __global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
@@ -564,44 +747,6 @@ __global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __
   dy -= g.L[1] * rintf(dy * g.invL[1]);
   dz -= g.L[2] * rintf(dz * g.invL[2]);
   atomicMax(out_bits, __float_as_uint(dx * dx + dy * dy + dz * dz));
-}
-
-// scan helpers (single workgroup), as in compat.hip
-constexpr int kScanBlock = 1024;
-__global__ void __launch_bounds__(kScanBlock)
-k_eng_scan(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
-           uint32_t* __restrict__ maxv) {
-  __shared__ uint32_t sh[kScanBlock];
-  __shared__ uint32_t shm[kScanBlock];
-  const int t = threadIdx.x;
-  const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
-  const uint32_t b = t * per;
-  const uint32_t e = min(n, b + per);
-  uint32_t s = 0, m = 0;
-  for (uint32_t i = b; i < e; ++i) {
-    s += in[i];
-    m = max(m, in[i]);
-  }
-  sh[t] = s;
-  shm[t] = m;
-  __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t v = t >= off ? sh[t - off] : 0u;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
-  }
-  for (int off = kScanBlock / 2; off > 0; off >>= 1) {
-    if (t < off) shm[t] = max(shm[t], shm[t + off]);
-    __syncthreads();
-  }
-  uint32_t run = sh[t] - s;
-  for (uint32_t i = b; i < e; ++i) {
-    out[i] = run;
-    run += in[i];
-  }
-  if (t == kScanBlock - 1) out[n] = sh[t];
-  if (t == 0 && maxv) *maxv = shm[0];
 }
 
 }  // namespace
@@ -624,7 +769,6 @@ struct dyn_engine {
   // topology
   DevBuf<float> q;
   DevBuf<int32_t> ltype;
-  DevBuf<uint8_t> cls;
   DevBuf<int32_t> wmol;
   DevBuf<float2> lj_table;
   DevBuf<uint32_t> ex_ptr, ex_idx;
@@ -639,19 +783,23 @@ struct dyn_engine {
   const float4* pos_dev = nullptr;
   PinnedBuf<float4> h_pos;
   DevBuf<float4> wpos, wpos_ref;
-  // clustering
+  // clustering (capacities fixed by the box geometry)
   EngGeom geom{};
-  std::vector<int> h_col_class;
   DevBuf<int> col_class;
-  DevBuf<uint32_t> col_of, col_count, col_start, col_ncl, col_cl_start, cursor, items, scan_max;
-  uint32_t n_cols = 0, n_cl = 0, n_slots = 0;
-  DevBuf<int32_t> slot_site, slot_type;
+  DevBuf<uint32_t> col_count, col_start, col_ncl, col_cl_start, items;
+  uint32_t n_cols = 0, cap_cl = 0, cap_slots = 0;
+  DevBuf<int32_t> slot_site, slot_type, slot_gid;
   DevBuf<uint8_t> cl_class;
-  DevBuf<uint32_t> cl_col, slot_of;
+  DevBuf<uint32_t> slot_of;
   DevBuf<float4> xyzq, bb_c, bb_h;
-  DevBuf<uint32_t> row_cnt, row_ptr, cj;
+  DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj;
   DevBuf<unsigned long long> cmask;
-  uint64_t n_entries = 0, n_builds = 0;
+  DevBuf<uint2> chunks;
+  uint64_t cap_entries = 0, cap_chunks = 0;
+  // device counters: [0] n_cl, [1] n_entries, [2] n_items, [3] max row
+  DevBuf<uint32_t> counters;
+  DevBuf<unsigned int> status;
+  uint64_t n_builds = 0;
   // accumulators
   DevBuf<int64_t> fx_slot, fx_site, fx_out;
   DevBuf<double> pair_part, corr_part, eng_out;
@@ -664,6 +812,7 @@ struct dyn_engine {
   // host copies
   PinnedBuf<int64_t> h_fx;
   PinnedBuf<double> h_eng;
+  PinnedBuf<uint32_t> h_misc;  // [0] status, [1..4] counters, [5] displacement bits
   bool fx_fresh = false;
 
   ~dyn_engine() {
@@ -671,152 +820,145 @@ struct dyn_engine {
   }
 
   void set_geometry();
+  void reserve_list(uint64_t entries, uint64_t items);
   void build_list();
-  void compute(uint32_t flags);
+  void enqueue(uint32_t flags);
+  void check_status();
 };
 
 void dyn_engine::set_geometry() {
-  const float V = box[0] * box[1] * box[2];
-  uint32_t ncls[3] = {n_dyn, n_water, 3 * n_water};
-  int off = 0;
-  h_col_class.clear();
+  const double V = (double)box[0] * box[1] * box[2];
+  const uint32_t ncls[3] = {n_dyn, n_water, 3 * n_water};
+  int off = 0, site_base = 0;
+  std::vector<int> h_col_class;
   for (int c = 0; c < 3; ++c) {
     ClassGrid& g = geom.cg[c];
+    g.n = (int)ncls[c];
+    g.site_base = site_base;
+    site_base += g.n;
+    g.col_off = off;
     if (ncls[c] == 0) {
       g.ncx = g.ncy = 0;
       g.wx = g.wy = 1.f;
-      g.col_off = off;
       continue;
     }
-    const double s = std::cbrt(8.0 * (double)V / (double)ncls[c]);
-    g.ncx = std::max(1, std::min(256, (int)std::floor(box[0] / s)));
-    g.ncy = std::max(1, std::min(256, (int)std::floor(box[1] / s)));
+    const double s = std::cbrt(8.0 * V / (double)ncls[c]);
+    g.ncx = std::max(1, std::min(64, (int)std::floor(box[0] / s)));
+    g.ncy = std::max(1, std::min(64, (int)std::floor(box[1] / s)));
     g.wx = box[0] / g.ncx;
     g.wy = box[1] / g.ncy;
-    g.col_off = off;
     off += g.ncx * g.ncy;
     for (int k = 0; k < g.ncx * g.ncy; ++k) h_col_class.push_back(c);
   }
   geom.n_cols = off;
+  geom.n_dyn = n_dyn;
+  geom.n_water = n_water;
   n_cols = (uint32_t)off;
   for (int a = 0; a < 3; ++a) {
     geom.L[a] = box[a];
     geom.invL[a] = 1.0f / box[a];
   }
+  // clusters: Σ ceil(count/8) ≤ n_sites/8 + n_cols
+  cap_cl = n_sites / DYN_CL_SIZE + n_cols + 1;
+  cap_slots = cap_cl * DYN_CL_SIZE;
   col_class.reserve(std::max<uint32_t>(n_cols, 1));
-  col_of.reserve(n_sites);
   col_count.reserve(n_cols + 1);
   col_start.reserve(n_cols + 1);
   col_ncl.reserve(n_cols + 1);
   col_cl_start.reserve(n_cols + 1);
-  cursor.reserve(n_cols + 1);
   items.reserve(n_sites);
-  scan_max.reserve(2);
+  slot_site.reserve(cap_slots);
+  slot_type.reserve(cap_slots);
+  slot_gid.reserve(cap_slots);
+  cl_class.reserve(cap_cl);
+  xyzq.reserve(cap_slots);
+  bb_c.reserve(cap_cl);
+  bb_h.reserve(cap_cl);
+  row_cnt.reserve(cap_cl + 1);
+  row_chunks.reserve(cap_cl + 1);
+  row_ptr.reserve(cap_cl + 1);
+  chunk_ptr.reserve(cap_cl + 1);
+  fx_slot.reserve(3 * (size_t)cap_slots);
+  // first guess of the list size from the site density; grown on overflow
+  const double rl = prm.cutoff + prm.list_buffer + 4.0;
+  const double per_row =
+      std::min<double>(kListCap, 4.19 * rl * rl * rl * n_sites / V / DYN_CL_SIZE * 1.3 + 64);
+  reserve_list((uint64_t)(cap_cl * per_row), (uint64_t)(cap_cl * (per_row / DYN_CHUNK + 1)));
   DYN_HIP(hipMemcpyAsync(col_class.p, h_col_class.data(), sizeof(int) * n_cols,
                          hipMemcpyHostToDevice, stream));
   DYN_HIP(hipStreamSynchronize(stream));
   list_valid = false;
 }
 
+void dyn_engine::reserve_list(uint64_t entries, uint64_t n_items) {
+  if (entries > cap_entries) {
+    cj.reserve(entries);
+    cmask.reserve(entries);
+    cap_entries = entries;
+  }
+  if (n_items > cap_chunks) {
+    chunks.reserve(n_items);
+    pair_part.reserve((size_t)kNumPart * n_items);
+    cap_chunks = n_items;
+  }
+}
+
 void dyn_engine::build_list() {
-  // bin
-  DYN_HIP(hipMemsetAsync(col_count.p, 0, 4 * (size_t)n_cols, stream));
-  DYN_HIP(hipMemsetAsync(cursor.p, 0, 4 * (size_t)n_cols, stream));
   {
-    Profiler::Scope ps(prof, "nbl_prepare_bin");
-    hipLaunchKernelGGL(k_eng_prepare, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       pos_dev, n_sites, q.p, cls.p, geom, 1, wpos.p, col_of.p, col_count.p);
-    DYN_CHECK_LAUNCH("k_eng_prepare");
+    Profiler::Scope ps(prof, "nbl_bin");
+    hipLaunchKernelGGL(k_eng_bin, dim3(3), dim3(kBinBlock), 0, stream, pos_dev, q.p, geom, wpos.p,
+                       col_count.p, col_start.p, items.p);
+    DYN_CHECK_LAUNCH("k_eng_bin");
   }
   {
-    Profiler::Scope ps(prof, "nbl_scan");
-    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, col_count.p, n_cols,
-                       col_start.p, scan_max.p);
+    Profiler::Scope ps(prof, "nbl_col_scan");
     hipLaunchKernelGGL(k_eng_col_clusters, dim3(grid_for(n_cols, 256)), dim3(256), 0, stream,
                        col_count.p, n_cols, col_ncl.p);
     hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, col_ncl.p, n_cols,
-                       col_cl_start.p, (uint32_t*)nullptr);
+                       (const uint32_t*)nullptr, col_cl_start.p, counters.p + 0, (uint32_t*)nullptr);
     DYN_CHECK_LAUNCH("k_eng_scan");
-  }
-  uint32_t hv[2] = {0, 0};
-  DYN_HIP(hipMemcpyAsync(&hv[0], scan_max.p, 4, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipMemcpyAsync(&hv[1], col_cl_start.p + n_cols, 4, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipStreamSynchronize(stream));
-  if (hv[0] > (uint32_t)kColCap)
-    throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds %u sites; at most %d are supported",
-                                      hv[0], kColCap));
-  n_cl = hv[1];
-  n_slots = n_cl * DYN_CL_SIZE;
-  slot_site.reserve(n_slots);
-  slot_type.reserve(n_slots);
-  cl_class.reserve(n_cl);
-  cl_col.reserve(n_cl);
-  xyzq.reserve(n_slots);
-  bb_c.reserve(n_cl);
-  bb_h.reserve(n_cl);
-  row_cnt.reserve(n_cl + 1);
-  row_ptr.reserve(n_cl + 1);
-  fx_slot.reserve(3 * (size_t)n_slots);
-  pair_part.reserve((size_t)kNumEnergy * n_cl);
-  {
-    Profiler::Scope ps(prof, "nbl_scatter");
-    hipLaunchKernelGGL(k_eng_scatter, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       n_sites, col_of.p, col_start.p, cursor.p, items.p);
-    DYN_CHECK_LAUNCH("k_eng_scatter");
   }
   {
     Profiler::Scope ps(prof, "nbl_colsort");
-    hipLaunchKernelGGL(k_eng_colsort, dim3(n_cols), dim3(kSortBlock), 0, stream, col_start.p,
-                       items.p, col_cl_start.p, wpos.p, col_class.p, slot_site.p, cl_class.p,
-                       cl_col.p);
+    SlotArgs S{wpos.p, ltype.p, wmol.p, type_none, xyzq.p, slot_type.p, slot_gid.p, slot_site.p, slot_of.p};
+    hipLaunchKernelGGL(k_eng_colsort, dim3(n_cols), dim3(kSortBlock), 0, stream, col_count.p,
+                       col_start.p, items.p, col_cl_start.p, col_class.p, S, cl_class.p, bb_c.p,
+                       bb_h.p, status.p);
     DYN_CHECK_LAUNCH("k_eng_colsort");
-  }
-  {
-    Profiler::Scope ps(prof, "nbl_slots_bbox");
-    hipLaunchKernelGGL(k_eng_slots, dim3(grid_for(n_slots, 256)), dim3(256), 0, stream,
-                       slot_site.p, n_slots, wpos.p, ltype.p, type_none, xyzq.p, slot_type.p,
-                       slot_of.p, 1);
-    hipLaunchKernelGGL(k_eng_bbox, dim3(grid_for(n_cl, 256)), dim3(256), 0, stream, xyzq.p,
-                       slot_site.p, n_cl, bb_c.p, bb_h.p);
-    DYN_CHECK_LAUNCH("k_eng_bbox");
   }
   ListArgs A;
   A.g = geom;
-  const float rl = prm.cutoff + std::max(0.f, prm.list_buffer);
-  A.rlist2 = rl * rl;
+  A.rlist = prm.cutoff + std::max(0.f, prm.list_buffer);
+  A.rlist2 = A.rlist * A.rlist;
+  A.n_cl = counters.p + 0;
   A.bb_c = bb_c.p;
   A.bb_h = bb_h.p;
   A.col_cl_start = col_cl_start.p;
   A.cl_class = cl_class.p;
   A.slot_site = slot_site.p;
-  A.wmol = wmol.p;
   A.ex_ptr = ex_ptr.p;
   A.ex_idx = ex_idx.p;
   A.n_dyn = n_dyn;
+  A.cap_entries = cap_entries;
+  A.cap_chunks = cap_chunks;
+  A.status = status.p;
+  const unsigned list_grid = std::min<uint32_t>(cap_cl, 8192);
   {
     Profiler::Scope ps(prof, "nbl_list_count");
-    hipLaunchKernelGGL(k_eng_list<0>, dim3(n_cl), dim3(kWave), 0, stream, A, n_cl, row_cnt.p,
-                       (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                       (unsigned long long*)nullptr);
-    DYN_CHECK_LAUNCH("k_eng_list<0>");
-    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p, n_cl,
-                       row_ptr.p, scan_max.p);
+    hipLaunchKernelGGL(k_eng_list_count, dim3(list_grid), dim3(kWave), 0, stream, A, row_cnt.p,
+                       row_chunks.p);
+    DYN_CHECK_LAUNCH("k_eng_list_count");
+    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p, 0u,
+                       counters.p + 0, row_ptr.p, counters.p + 1, counters.p + 3);
+    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, row_chunks.p, 0u,
+                       counters.p + 0, chunk_ptr.p, counters.p + 2, (uint32_t*)nullptr);
     DYN_CHECK_LAUNCH("k_eng_scan");
   }
-  DYN_HIP(hipMemcpyAsync(&hv[0], scan_max.p, 4, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipMemcpyAsync(&hv[1], row_ptr.p + n_cl, 4, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipStreamSynchronize(stream));
-  if (hv[0] > (uint32_t)kListCap)
-    throw Error(DYN_ERR_CAPACITY, fmt("a cluster has %u cluster-pair entries; at most %d are "
-                                      "supported (reduce list_buffer)", hv[0], kListCap));
-  n_entries = hv[1];
-  cj.reserve(std::max<uint64_t>(n_entries, 1));
-  cmask.reserve(std::max<uint64_t>(n_entries, 1));
   {
     Profiler::Scope ps(prof, "nbl_list_fill");
-    hipLaunchKernelGGL(k_eng_list<1>, dim3(n_cl), dim3(kWave), 0, stream, A, n_cl,
-                       (uint32_t*)nullptr, row_ptr.p, cj.p, cmask.p);
-    DYN_CHECK_LAUNCH("k_eng_list<1>");
+    hipLaunchKernelGGL(k_eng_list_fill, dim3(list_grid), dim3(kWave), 0, stream, A, row_ptr.p,
+                       chunk_ptr.p, cj.p, cmask.p, chunks.p);
+    DYN_CHECK_LAUNCH("k_eng_list_fill");
   }
   DYN_HIP(hipMemcpyAsync(wpos_ref.p, wpos.p, sizeof(float4) * n_sites, hipMemcpyDeviceToDevice,
                          stream));
@@ -824,37 +966,30 @@ void dyn_engine::build_list() {
   ++n_builds;
 }
 
-void dyn_engine::compute(uint32_t flags) {
+void dyn_engine::enqueue(uint32_t flags) {
   DYN_REQUIRE(box_set, "box not set");
   if (!pos_set) throw Error(DYN_ERR_STATE, "positions not set");
-  bool rebuild = (flags & DYN_COMPUTE_REBUILD) || !list_valid;
+  bool rebuild = (flags & DYN_COMPUTE_REBUILD) || !list_valid || prm.list_buffer <= 0.f;
   if (!rebuild) {
-    if (prm.list_buffer <= 0.f) {
-      rebuild = true;
-    } else {
-      hipLaunchKernelGGL(k_eng_prepare, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                         pos_dev, n_sites, q.p, cls.p, geom, 0, wpos.p, (uint32_t*)nullptr,
-                         (uint32_t*)nullptr);
-      DYN_CHECK_LAUNCH("k_eng_prepare");
-      DYN_HIP(hipMemsetAsync(disp_bits.p, 0, 4, stream));
-      hipLaunchKernelGGL(k_eng_max_disp, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                         wpos.p, wpos_ref.p, n_sites, geom, disp_bits.p);
-      unsigned int bits = 0;
-      DYN_HIP(hipMemcpyAsync(&bits, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
-      DYN_HIP(hipStreamSynchronize(stream));
-      float d2;
-      memcpy(&d2, &bits, 4);
-      const float half = 0.5f * prm.list_buffer;
-      if (!(d2 < half * half)) rebuild = true;
-    }
+    hipLaunchKernelGGL(k_eng_wrap, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, pos_dev,
+                       q.p, n_sites, geom, wpos.p);
+    DYN_CHECK_LAUNCH("k_eng_wrap");
+    DYN_HIP(hipMemsetAsync(disp_bits.p, 0, 4, stream));
+    hipLaunchKernelGGL(k_eng_max_disp, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
+                       wpos.p, wpos_ref.p, n_sites, geom, disp_bits.p);
+    DYN_HIP(hipMemcpyAsync(h_misc.p + 5, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+    float d2;
+    memcpy(&d2, h_misc.p + 5, 4);
+    const float half = 0.5f * prm.list_buffer;
+    if (!(d2 < half * half)) rebuild = true;
   }
   if (rebuild) {
     build_list();
   } else {
     Profiler::Scope ps(prof, "slots");
-    hipLaunchKernelGGL(k_eng_slots, dim3(grid_for(n_slots, 256)), dim3(256), 0, stream,
-                       slot_site.p, n_slots, wpos.p, ltype.p, type_none, xyzq.p, slot_type.p,
-                       slot_of.p, 0);
+    hipLaunchKernelGGL(k_eng_slots, dim3(grid_for(cap_slots, 256)), dim3(256), 0, stream,
+                       slot_site.p, counters.p + 0, wpos.p, xyzq.p);
     DYN_CHECK_LAUNCH("k_eng_slots");
   }
   DynPairParams P;
@@ -870,29 +1005,37 @@ void dyn_engine::compute(uint32_t flags) {
   P.c6_oo = c6_oo;
   P.c12_oo = c12_oo;
   P.n_types = n_types;
-  DYN_HIP(hipMemsetAsync(fx_slot.p, 0, sizeof(int64_t) * 3 * n_slots, stream));
-  DYN_HIP(hipMemsetAsync(fx_site.p, 0, sizeof(int64_t) * 3 * n_sites, stream));
-  DYN_HIP(hipMemsetAsync(pair_part.p, 0, sizeof(double) * kNumEnergy * n_cl, stream));
-  DYN_HIP(hipMemsetAsync(corr_part.p, 0, sizeof(double) * 6 * corr_blocks, stream));
-  if (flags & DYN_COMPUTE_SHORT) {
+  {
+    Profiler::Scope ps(prof, "zero");
+    DYN_HIP(hipMemsetAsync(fx_slot.p, 0, sizeof(int64_t) * 3 * cap_slots, stream));
+    DYN_HIP(hipMemsetAsync(fx_site.p, 0, sizeof(int64_t) * 3 * n_sites, stream));
+    DYN_HIP(hipMemsetAsync(corr_part.p, 0, sizeof(double) * 6 * corr_blocks, stream));
+  }
+  const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
+  if (short_on) {
     Profiler::Scope ps(prof, "pairs");
     PairArgs A;
     A.P = P;
     A.xyzq = xyzq.p;
     A.slot_type = slot_type.p;
+    A.slot_gid = slot_gid.p;
     A.cl_class = cl_class.p;
     A.row_ptr = row_ptr.p;
     A.cj = cj.p;
     A.cmask = cmask.p;
+    A.chunks = chunks.p;
+    A.n_chunks = counters.p + 2;
     A.lj_table = lj_table.p;
     A.fx = fx_slot.p;
-    A.n_slots = n_slots;
+    A.n_slots = cap_slots;
     A.partials = pair_part.p;
-    A.n_rows = n_cl;
-    const unsigned blocks = (n_cl + (kPairBlock / kWave) - 1) / (kPairBlock / kWave);
+    A.cap_chunks = cap_chunks;
+    A.status = status.p;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((cap_chunks + 3) / 4, 4096);
     hipLaunchKernelGGL(k_eng_pairs, dim3(blocks), dim3(kPairBlock), 0, stream, A);
     DYN_CHECK_LAUNCH("k_eng_pairs");
   }
+  const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
   if ((flags & DYN_COMPUTE_CORR) && n_corr) {
     Profiler::Scope ps(prof, "corr");
     CorrArgs C;
@@ -905,15 +1048,14 @@ void dyn_engine::compute(uint32_t flags) {
     C.kind = corr_kind.p;
     C.n = n_corr;
     C.mode = (int)prm.mode;
-    C.recip_on = (pme_on && (flags & DYN_COMPUTE_RECIP)) ? 1 : 0;
+    C.recip_on = recip_on ? 1 : 0;
     C.fx = fx_site.p;
     C.n_sites = n_sites;
     C.partials = corr_part.p;
     hipLaunchKernelGGL(k_eng_corr, dim3(corr_blocks), dim3(kCorrBlock), 0, stream, C);
     DYN_CHECK_LAUNCH("k_eng_corr");
   }
-  const bool recip_res_needed = pme_on && (flags & DYN_COMPUTE_RECIP);
-  if (recip_res_needed) {
+  if (recip_on) {
     {
       Profiler::Scope ps(prof, "pme_input");
       hipLaunchKernelGGL(k_eng_pme_input, dim3(grid_for(n_q, 256)), dim3(256), 0, stream, wpos.p,
@@ -925,18 +1067,42 @@ void dyn_engine::compute(uint32_t flags) {
   {
     Profiler::Scope ps(prof, "finalize");
     hipLaunchKernelGGL(k_eng_finalize, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       fx_site.p, fx_slot.p, slot_of.p, n_sites, n_slots, fx_out.p);
-    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(256), 0, stream, pair_part.p, n_cl,
-                       corr_part.p, corr_blocks, eng_out.p);
+                       fx_site.p, fx_slot.p, slot_of.p, n_sites, cap_slots, fx_out.p);
+    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(1024), 0, stream, pair_part.p,
+                       counters.p + 2, short_on ? cap_chunks : (uint64_t)0, corr_part.p,
+                       corr_blocks, eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
-  if (recip_res_needed)
+  if (recip_on)
     DYN_HIP(hipMemcpyAsync(eng_out.p + 14, pme.res_dev.p, 2 * sizeof(double),
                            hipMemcpyDeviceToDevice, stream));
   else
     DYN_HIP(hipMemsetAsync(eng_out.p + 14, 0, 2 * sizeof(double), stream));
   computed = true;
   fx_fresh = false;
+}
+
+// Reads the status bits and counters (synchronizes).  On an overflow the
+// buffers grow to fit and the caller gets DYN_ERR_CAPACITY; the list is
+// rebuilt on the next call.
+void dyn_engine::check_status() {
+  DYN_HIP(hipMemcpyAsync(h_misc.p, status.p, 4, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 16, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipStreamSynchronize(stream));
+  const uint32_t st = h_misc.p[0];
+  if (!st) return;
+  DYN_HIP(hipMemsetAsync(status.p, 0, 4, stream));
+  list_valid = false;
+  if (st & kStColumn)
+    throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
+  if (st & kStRow)
+    throw Error(DYN_ERR_CAPACITY, fmt("a cluster has more than %d cluster-pair entries (reduce "
+                                      "list_buffer)", kListCap));
+  const uint64_t ne = h_misc.p[2], nch = h_misc.p[3];
+  reserve_list(ne + ne / 4 + 1024, nch + nch / 4 + 256);
+  throw Error(DYN_ERR_CAPACITY, fmt("cluster-pair list outgrew its buffers (%llu entries, %llu "
+                                    "work items); they have been enlarged, call again",
+                                    (unsigned long long)ne, (unsigned long long)nch));
 }
 
 // ---------------------------------------------------------------------------
@@ -960,7 +1126,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_REQUIRE(p.mode == DYN_MODE_REFERENCE || p.mode == DYN_MODE_STANDARD, "unknown mode");
     DYN_REQUIRE(p.list_buffer >= 0.f && p.list_buffer < 10.f, "list_buffer must be in [0, 10)");
     DYN_REQUIRE(n_dyn + 4ull * n_water > 0, "empty system");
-    DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 31), "system too large");
+    DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 30), "system too large");
     DYN_REQUIRE(n_dyn == 0 || (q_dyn && type_dyn), "null solute arrays");
     DYN_REQUIRE(n_dyn == 0 || (n_types > 0 && lj_sigma_eps), "solute needs LJ types");
     DYN_REQUIRE(n_types <= 64, "at most 64 LJ types");
@@ -1009,14 +1175,12 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     const float qh = 0.6791f * 18.2223f, qm = -2.0f * qh;
     std::vector<float> hq(ns);
     std::vector<int32_t> ht(ns), hw(ns);
-    std::vector<uint8_t> hc(ns);
     for (uint32_t i = 0; i < n_dyn; ++i) {
       DYN_REQUIRE(type_dyn[i] >= 0 && (uint32_t)type_dyn[i] < n_types, "atom %u: bad type", i);
       DYN_REQUIRE(std::isfinite(q_dyn[i]), "atom %u: non-finite charge", i);
       hq[i] = q_dyn[i];
       ht[i] = type_dyn[i];
       hw[i] = -1;
-      hc[i] = DYN_CLS_SOLUTE;
     }
     const float wq[4] = {0.f, qm, qh, qh};
     for (uint32_t m = 0; m < n_water; ++m)
@@ -1025,9 +1189,8 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
         hq[s] = wq[k];
         ht[s] = k == 0 ? E->type_o : E->type_none;
         hw[s] = (int32_t)m;
-        hc[s] = k == 0 ? DYN_CLS_WATER_O : DYN_CLS_WATER_Q;
       }
-    // exclusions (1-2, 1-3 and 1-4 all leave the main kernel), CSR both ways
+    // exclusions (1-2, 1-3 and 1-4 all leave the pair kernel), CSR both ways
     std::vector<std::vector<uint32_t>> ex(n_dyn);
     std::vector<uint32_t> ci, cjv;
     std::vector<uint8_t> ck;
@@ -1090,7 +1253,6 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     up(E->q, hq);
     up(E->ltype, ht);
     up(E->wmol, hw);
-    up(E->cls, hc);
     up(E->lj_table, tab);
     up(E->ex_ptr, hptr);
     up(E->ex_idx, hidx);
@@ -1111,9 +1273,15 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->h_fx.reserve(3 * (size_t)ns);
     E->eng_out.reserve(16);
     E->h_eng.reserve(16);
+    E->h_misc.reserve(8);
+    E->counters.reserve(4);
+    E->status.reserve(1);
     E->disp_bits.reserve(1);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
+    DYN_HIP(hipMemsetAsync(E->counters.p, 0, 16, E->stream));
+    DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
+    DYN_HIP(hipMemsetAsync(E->status.p, 0, 4, E->stream));
     DYN_HIP(hipStreamSynchronize(E->stream));
     *out = E.release();
     return DYN_OK;
@@ -1209,8 +1377,19 @@ extern "C" int dyn_engine_compute(dyn_engine* eng, uint32_t flags) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
     DeviceGuard g(eng->device);
-    eng->compute(flags);
-    if (!(flags & DYN_COMPUTE_ASYNC)) DYN_HIP(hipStreamSynchronize(eng->stream));
+    if (flags & DYN_COMPUTE_ASYNC) {
+      eng->enqueue(flags);
+    } else {
+      // a list that outgrew its buffers is rebuilt once with enlarged ones
+      eng->enqueue(flags);
+      try {
+        eng->check_status();
+      } catch (const Error& e) {
+        if (e.code != DYN_ERR_CAPACITY || eng->list_valid) throw;
+        eng->enqueue(flags | DYN_COMPUTE_REBUILD);
+        eng->check_status();
+      }
+    }
   });
 }
 
@@ -1218,13 +1397,14 @@ extern "C" int dyn_engine_sync(dyn_engine* eng) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
     DeviceGuard g(eng->device);
-    DYN_HIP(hipStreamSynchronize(eng->stream));
+    eng->check_status();
   });
 }
 
 static void fetch_fx(dyn_engine* eng) {
   if (!eng->computed) throw Error(DYN_ERR_STATE, "no compute yet");
   if (eng->fx_fresh) return;
+  eng->check_status();
   DYN_HIP(hipMemcpyAsync(eng->h_fx.p, eng->fx_out.p, sizeof(int64_t) * 3 * eng->n_sites,
                          hipMemcpyDeviceToHost, eng->stream));
   DYN_HIP(hipStreamSynchronize(eng->stream));
@@ -1275,11 +1455,12 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     DeviceGuard g(eng->device);
     DYN_REQUIRE(out, "out is null");
     if (!eng->computed) throw Error(DYN_ERR_STATE, "no compute yet");
+    eng->check_status();
     DYN_HIP(hipMemcpyAsync(eng->h_eng.p, eng->eng_out.p, 16 * sizeof(double),
                            hipMemcpyDeviceToHost, eng->stream));
     DYN_HIP(hipStreamSynchronize(eng->stream));
     const double* v = eng->h_eng.p;
-    // v[0..7]: e_lj_ww, e_lj_s, e_c_ww, e_c_s, virial_pairs, pair counts by kind
+    // v[0..7]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial_pairs, pair counts by kind
     // v[8..12]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr; v[14..15]: pme energy, virial
     dyn_energies E{};
     E.lj = v[0] + v[1] + v[8];
@@ -1301,9 +1482,43 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     E.pair_count_kind[1] = (uint64_t)std::llround(v[6]);
     E.pair_count_kind[2] = (uint64_t)std::llround(v[7]);
     E.pair_count = E.pair_count_kind[0] + E.pair_count_kind[1] + E.pair_count_kind[2];
-    E.list_entries = eng->n_entries;
+    E.list_entries = eng->h_misc.p[2];
     E.list_builds = eng->n_builds;
     *out = E;
+  });
+}
+
+extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clusters,
+                                       uint64_t* n_entries, int32_t* slot_site, float* xyzq,
+                                       int32_t* slot_type, int32_t* slot_gid, uint8_t* cl_class,
+                                       uint32_t* row_ptr, uint32_t* cj, uint64_t* cmask,
+                                       uint32_t* items, uint64_t* n_items, int64_t* fx_slot) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DeviceGuard g(eng->device);
+    if (!eng->list_valid) throw Error(DYN_ERR_STATE, "no list built yet");
+    eng->check_status();
+    const size_t nc = eng->h_misc.p[1], ns = nc * DYN_CL_SIZE, ne = eng->h_misc.p[2],
+                 nit = eng->h_misc.p[3];
+    if (n_slots) *n_slots = (uint32_t)ns;
+    if (n_clusters) *n_clusters = (uint32_t)nc;
+    if (n_entries) *n_entries = ne;
+    if (n_items) *n_items = nit;
+    hipStream_t s = eng->stream;
+    if (slot_site) DYN_HIP(hipMemcpyAsync(slot_site, eng->slot_site.p, 4 * ns, hipMemcpyDeviceToHost, s));
+    if (xyzq) DYN_HIP(hipMemcpyAsync(xyzq, eng->xyzq.p, 16 * ns, hipMemcpyDeviceToHost, s));
+    if (slot_type) DYN_HIP(hipMemcpyAsync(slot_type, eng->slot_type.p, 4 * ns, hipMemcpyDeviceToHost, s));
+    if (slot_gid) DYN_HIP(hipMemcpyAsync(slot_gid, eng->slot_gid.p, 4 * ns, hipMemcpyDeviceToHost, s));
+    if (cl_class) DYN_HIP(hipMemcpyAsync(cl_class, eng->cl_class.p, nc, hipMemcpyDeviceToHost, s));
+    if (row_ptr) DYN_HIP(hipMemcpyAsync(row_ptr, eng->row_ptr.p, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
+    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * ne, hipMemcpyDeviceToHost, s));
+    if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, eng->cmask.p, 8 * ne, hipMemcpyDeviceToHost, s));
+    if (items && nit) DYN_HIP(hipMemcpyAsync(items, eng->chunks.p, 8 * nit, hipMemcpyDeviceToHost, s));
+    if (fx_slot)
+      for (int c = 0; c < 3; ++c)
+        DYN_HIP(hipMemcpyAsync(fx_slot + c * ns, eng->fx_slot.p + (size_t)c * eng->cap_slots, 8 * ns,
+                               hipMemcpyDeviceToHost, s));
+    DYN_HIP(hipStreamSynchronize(s));
   });
 }
 
@@ -1331,30 +1546,5 @@ extern "C" int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char
       if (ms) ms[i] = it->second.first;
       if (counts) counts[i] = it->second.second;
     }
-  });
-}
-
-extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clusters,
-                                       uint64_t* n_entries, int32_t* slot_site, float* xyzq,
-                                       int32_t* slot_type, uint8_t* cl_class, uint32_t* row_ptr,
-                                       uint32_t* cj, uint64_t* cmask, int64_t* fx_slot) {
-  ENG_CHECK(eng);
-  DYN_API_BODY(eng, {
-    DeviceGuard g(eng->device);
-    if (!eng->list_valid) throw Error(DYN_ERR_STATE, "no list built yet");
-    if (n_slots) *n_slots = eng->n_slots;
-    if (n_clusters) *n_clusters = eng->n_cl;
-    if (n_entries) *n_entries = eng->n_entries;
-    hipStream_t s = eng->stream;
-    const size_t ns = eng->n_slots, nc = eng->n_cl, ne = eng->n_entries;
-    if (slot_site) DYN_HIP(hipMemcpyAsync(slot_site, eng->slot_site.p, 4 * ns, hipMemcpyDeviceToHost, s));
-    if (xyzq) DYN_HIP(hipMemcpyAsync(xyzq, eng->xyzq.p, 16 * ns, hipMemcpyDeviceToHost, s));
-    if (slot_type) DYN_HIP(hipMemcpyAsync(slot_type, eng->slot_type.p, 4 * ns, hipMemcpyDeviceToHost, s));
-    if (cl_class) DYN_HIP(hipMemcpyAsync(cl_class, eng->cl_class.p, nc, hipMemcpyDeviceToHost, s));
-    if (row_ptr) DYN_HIP(hipMemcpyAsync(row_ptr, eng->row_ptr.p, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
-    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * ne, hipMemcpyDeviceToHost, s));
-    if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, eng->cmask.p, 8 * ne, hipMemcpyDeviceToHost, s));
-    if (fx_slot) DYN_HIP(hipMemcpyAsync(fx_slot, eng->fx_slot.p, 24 * ns, hipMemcpyDeviceToHost, s));
-    DYN_HIP(hipStreamSynchronize(s));
   });
 }

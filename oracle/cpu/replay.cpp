@@ -109,11 +109,14 @@ void orc_ref_pairs(const float* pos, uint32_t n_dyn, uint32_t n_water, uint32_t 
 }  // extern "C"
 
 // ---- native engine pair tiles -----------------------------------------------
+// Full cluster-pair list, rows cut into work items of ≤ DYN_CHUNK entries;
+// each lane (ia, jb) accumulates the force on slot 8I+ia over the item's
+// entries in list order, then the 8 j groups are summed by the xor-8/16/32
+// tree and converted to fixed point once per item.
 template <int KIND>
-static void tile(const float* xyzq, const int32_t* slot_type, uint32_t I, uint32_t J,
-                 uint64_t mask, const DynPairParams& P, const float2* lj, float fi[64][3],
-                 int64_t* fx, uint32_t n_slots) {
-  float g[64][3];
+static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
+                 uint32_t I, uint32_t J, uint64_t mask, const DynPairParams& P, const float2* lj,
+                 float fi[64][3]) {
   for (int l = 0; l < 64; ++l) {
     const int ia = l & 7, jb = l >> 3;
     const float* a = xyzq + 4 * (size_t)(I * DYN_CL_SIZE + ia);
@@ -121,39 +124,26 @@ static void tile(const float* xyzq, const int32_t* slot_type, uint32_t I, uint32
     const float pi[4] = {a[0], a[1], a[2], a[3]};
     const float pj[4] = {b[0], b[1], b[2], b[3]};
     const int ti = slot_type[I * DYN_CL_SIZE + ia];
-    const int tj = (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO) ? slot_type[J * DYN_CL_SIZE + jb] : 0;
-    const bool active = (mask >> l) & 1ull;
+    const int tj = slot_type[J * DYN_CL_SIZE + jb];
+    bool active = dyn_slot_pair_valid(slot_gid[I * DYN_CL_SIZE + ia], slot_gid[J * DYN_CL_SIZE + jb]);
+    if (KIND == DYN_KIND_SS) active = active && ((mask >> l) & 1ull);
     float d[3], s, elj, ec, r2;
     int hit;
     dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj, d, &s, &elj, &ec, &r2, &hit);
-    const float f0 = s * d[0], f1 = s * d[1], f2 = s * d[2];
-    fi[l][0] += f0;
-    fi[l][1] += f1;
-    fi[l][2] += f2;
-    g[l][0] = -f0;
-    g[l][1] = -f1;
-    g[l][2] = -f2;
-  }
-  for (int m : {1, 2, 4}) {
-    float h[64][3];
-    for (int l = 0; l < 64; ++l)
-      for (int c = 0; c < 3; ++c) h[l][c] = g[l][c] + g[l ^ m][c];
-    memcpy(g, h, sizeof g);
-  }
-  for (int jb = 0; jb < 8; ++jb) {
-    if (!((mask >> (8 * jb)) & 0xffull)) continue;
-    const uint32_t slot = J * DYN_CL_SIZE + jb;
-    for (int c = 0; c < 3; ++c) fx[(size_t)c * n_slots + slot] += dyn_fx(g[8 * jb][c]);
+    fi[l][0] += s * d[0];
+    fi[l][1] += s * d[1];
+    fi[l][2] += s * d[2];
   }
 }
 
 extern "C" {
 
-void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const uint8_t* cl_class,
-                      const uint32_t* row_ptr, const uint32_t* cj, const uint64_t* cmask,
-                      uint32_t n_rows, uint32_t n_slots, const float* lj_table, int n_types,
-                      const float* box, float cutoff, float lj_cutoff, float beta, float c6_oo,
-                      float c12_oo, int64_t* fx) {
+void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
+                      const uint8_t* cl_class, const uint32_t* row_ptr, const uint32_t* cj,
+                      const uint64_t* cmask, const uint32_t* items, uint64_t n_items,
+                      uint32_t n_slots, const float* lj_table, int n_types, const float* box,
+                      float cutoff, float lj_cutoff, float beta, float c6_oo, float c12_oo,
+                      int64_t* fx) {
   DynPairParams P;
   for (int k = 0; k < 3; ++k) {
     P.ext[k] = box[k];
@@ -170,22 +160,23 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const uint8_t
   const float2* lj = reinterpret_cast<const float2*>(lj_table);
   memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
   static float fi[64][3];
-  for (uint32_t I = 0; I < n_rows; ++I) {
+  for (uint64_t it = 0; it < n_items; ++it) {
+    const uint32_t I = items[2 * it], e0 = items[2 * it + 1];
+    const uint32_t e1 = std::min<uint32_t>(e0 + DYN_CHUNK, row_ptr[I + 1]);
     memset(fi, 0, sizeof fi);
     const int ci = cl_class[I];
-    for (uint32_t e = row_ptr[I]; e < row_ptr[I + 1]; ++e) {
+    for (uint32_t e = e0; e < e1; ++e) {
       const uint32_t J = cj[e];
       const uint64_t mask = cmask[e];
       switch (dyn_kind(ci, cl_class[J])) {
-        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
-        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
-        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
-        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
-        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, I, J, mask, P, lj, fi, fx, n_slots); break;
+        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
+        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
+        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
+        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
+        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
         default: break;
       }
     }
-    if (row_ptr[I + 1] == row_ptr[I]) continue;
     float v[64][3];
     memcpy(v, fi, sizeof v);
     for (int m : {8, 16, 32}) {

@@ -114,15 +114,18 @@ def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta):
     c6_oo, c12_oo = tab[o * nt + o]
     ns = len(dbg["slot_site"])
     fx = np.zeros((3, ns), np.int64)
-    L.orc_engine_tiles.argtypes = [C.c_void_p] * 6 + [C.c_uint32, C.c_uint32, C.c_void_p, C.c_int,
+    L.orc_engine_tiles.argtypes = [C.c_void_p] * 8 + [C.c_uint64, C.c_uint32, C.c_void_p, C.c_int,
                                                       C.c_void_p, C.c_float, C.c_float, C.c_float,
                                                       C.c_float, C.c_float, C.c_void_p]
     box = np.ascontiguousarray(system.box, np.float32)
     cm = np.ascontiguousarray(dbg["cmask"].astype(np.uint64))
-    L.orc_engine_tiles(ptr(dbg["xyzq"]), ptr(dbg["slot_type"]), ptr(dbg["cl_class"]),
-                       ptr(dbg["row_ptr"]), ptr(dbg["cj"]) if len(dbg["cj"]) else None,
-                       ptr(cm) if len(cm) else None, len(dbg["cl_class"]), ns, ptr(tab), nt,
-                       ptr(box), cutoff, lj_cutoff, beta, float(c6_oo), float(c12_oo), ptr(fx))
+    items = np.ascontiguousarray(dbg["items"], np.uint32)
+    L.orc_engine_tiles(ptr(dbg["xyzq"]), ptr(dbg["slot_type"]), ptr(dbg["slot_gid"]),
+                       ptr(dbg["cl_class"]), ptr(dbg["row_ptr"]),
+                       ptr(dbg["cj"]) if len(dbg["cj"]) else None,
+                       ptr(cm) if len(cm) else None, ptr(items) if len(items) else None,
+                       len(items), ns, ptr(tab), nt, ptr(box), cutoff, lj_cutoff, beta,
+                       float(c6_oo), float(c12_oo), ptr(fx))
     return fx
 
 
