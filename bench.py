@@ -112,6 +112,8 @@ def main() -> int:
         },
         "pair_interactions_per_s": n_pairs * steps_per_s,
         "pair_interactions_per_step": n_pairs,
+        "cluster_pair_entries": E["list_entries"],
+        "tile_lane_efficiency": 2.0 * E["pair_count"] / max(64.0 * E["list_entries"], 1.0),
         "roofline": {
             "bound": "valu",
             "kernel": "k_eng_pairs",

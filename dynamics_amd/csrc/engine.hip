@@ -1,21 +1,22 @@
 // engine.hip — native nonbonded engine on gfx950.
 //
 // One evaluation, all on the handle's stream and without host round trips:
-//   bin        wrap positions into the box and counting-sort each site class
-//              into x–y columns in LDS (one workgroup per class)      (rebuild)
-//   colsort    per column, bitonic sort by z, cut into clusters of 8, write
-//              cluster slots and bounding boxes                       (rebuild)
+//   bin        wrap positions into the box, find each site's x–y column of
+//              its class, count and scatter the sites into column ranges (rebuild)
+//   colsort    per column, bitonic sort by (z, site), cut into clusters of
+//              8, write cluster slots and bounding boxes             (rebuild)
 //   list       full cluster-pair list: every cluster lists every cluster
 //              whose box is within the cutoff (+ buffer), both ways; rows
-//              are cut into work items of ≤ DYN_CHUNK entries           (rebuild)
+//              are cut into work items of ≤ DYN_CHUNK entries; solute–solute
+//              entries get exclusion masks                            (rebuild)
 //   slots      gather current positions into cluster slots       (reuse path)
 //   pairs      8×8 wave tiles over the work items: LJ + erfc Coulomb; each
-//              lane keeps its i-site force in registers, one fixed-point add
-//              per work item; no per-pair atomics and no j-side reductions
+//              lane keeps its i-site force in registers and a work item
+//              stores its 8 i-site forces once: no atomics, no j-side sums
 //   corr       1-4 pairs (scaled real space) and reciprocal corrections
 //   pme        spread → FFT → convolve → inverse FFT → gather
-//   finalize   slot + site forces → site-order fixed-point forces, and a
-//              fixed-order energy reduction
+//   finalize   site forces + the work-item rows of each site's slot →
+//              site-order fixed-point forces; fixed-order energy sums
 // Capacities are checked on the device; an overflow sets a status bit that
 // the host reads at the next synchronization and reports as an error, after
 // growing the buffers for the next call.
@@ -48,7 +49,9 @@ constexpr int kPairBlock = 256;       // 4 waves, one work item each at a time
 constexpr int kListCap = 2048;        // entries per cluster row (LDS staging)
 constexpr int kColCap = 4096;         // sites per column (LDS sort)
 constexpr int kMaxColsPerClass = 4096;
-constexpr int kNumPart = 8;           // f64 partials per work item
+constexpr int kNumPart = 8;           // f64 partials per wave of the pair kernel
+constexpr int kPairGrid = 1792;       // pair kernel blocks: 7 waves per SIMD on 256 CUs (fixed: the sums' order depends on it)
+constexpr int kPairWaves = kPairGrid * (kPairBlock / kWave);
 
 // status bits
 constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u;
@@ -91,112 +94,105 @@ __device__ __forceinline__ uint32_t site_column(const ClassGrid& cg, float x, fl
   return (uint32_t)(cx * cg.ncy + cy);
 }
 
-// One workgroup per class: wrap the class's sites into the box, count them
-// per column in LDS, scan, and scatter site indices into the column ranges.
-constexpr int kBinBlock = 1024;
-__global__ void __launch_bounds__(kBinBlock)
-k_eng_bin(const float4* __restrict__ pos, const float* __restrict__ q, EngGeom g,
-          float4* __restrict__ wpos, uint32_t* __restrict__ col_count,
-          uint32_t* __restrict__ col_start, uint32_t* __restrict__ items) {
-  __shared__ uint32_t hist[kMaxColsPerClass];
-  __shared__ uint32_t warp_tot[kBinBlock / kWave];
-  const int cls = blockIdx.x;
-  const ClassGrid cg = g.cg[cls];
-  if (cg.n == 0) return;
-  const int ncol = cg.ncx * cg.ncy;
-  for (int k = threadIdx.x; k < ncol; k += kBinBlock) hist[k] = 0;
-  __syncthreads();
-  for (uint32_t t = threadIdx.x; t < (uint32_t)cg.n; t += kBinBlock) {
-    const uint32_t s = class_site(cls, t, g.n_dyn);
-    const float4 p = pos[s];
-    const float x = wrap1(p.x, g.L[0], g.invL[0]);
-    const float y = wrap1(p.y, g.L[1], g.invL[1]);
-    const float z = wrap1(p.z, g.L[2], g.invL[2]);
-    wpos[s] = make_float4(x, y, z, q[s]);
-    atomicAdd(&hist[site_column(cg, x, y)], 1u);
-  }
-  __syncthreads();
-  // exclusive scan of hist (ncol ≤ 4096), a contiguous range per thread
-  const int per = (ncol + kBinBlock - 1) / kBinBlock;
-  const int b = threadIdx.x * per, e = min(ncol, b + per);
-  uint32_t loc = 0;
-  for (int k = b; k < e; ++k) loc += hist[k];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  uint32_t incl = loc;
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t v = __shfl_up(incl, off, kWave);
-    if (lane >= off) incl += v;
-  }
-  if (lane == kWave - 1) warp_tot[w] = incl;
-  __syncthreads();
-  uint32_t wbase = 0;
-  for (int k = 0; k < w; ++k) wbase += warp_tot[k];
-  uint32_t run = wbase + incl - loc;
-  __syncthreads();
-  for (int k = b; k < e; ++k) {
-    const uint32_t c = hist[k];
-    col_count[cg.col_off + k] = c;
-    col_start[cg.col_off + k] = (uint32_t)cg.site_base + run;
-    hist[k] = (uint32_t)cg.site_base + run;  // becomes the scatter cursor
-    run += c;
-  }
-  __syncthreads();
-  for (uint32_t t = threadIdx.x; t < (uint32_t)cg.n; t += kBinBlock) {
-    const uint32_t s = class_site(cls, t, g.n_dyn);
-    const float4 p = wpos[s];
-    const uint32_t slot = atomicAdd(&hist[site_column(cg, p.x, p.y)], 1u);
-    items[slot] = s;
-  }
+// Wrap every site into the box, find its x–y column and take a rank in it.
+// The rank order depends on atomic timing; the column sort that follows
+// orders each column by (z, site), so the clusters do not.
+__global__ void k_eng_bin_count(const float4* __restrict__ pos, const float* __restrict__ q,
+                                EngGeom g, float4* __restrict__ wpos, float4* __restrict__ wpos_ref,
+                                uint32_t* __restrict__ col_count, uint2* __restrict__ site_col) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= g.n_dyn + 4u * g.n_water) return;
+  const float4 p = pos[s];
+  const float x = wrap1(p.x, g.L[0], g.invL[0]);
+  const float y = wrap1(p.y, g.L[1], g.invL[1]);
+  const float z = wrap1(p.z, g.L[2], g.invL[2]);
+  const float4 w = make_float4(x, y, z, q[s]);
+  wpos[s] = w;
+  if (wpos_ref) wpos_ref[s] = w;
+  const bool solute = s < g.n_dyn;
+  const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
+  const ClassGrid cg = solute ? g.cg[0] : (oxy ? g.cg[1] : g.cg[2]);
+  const uint32_t col = (uint32_t)cg.col_off + site_column(cg, x, y);
+  const uint32_t rank = atomicAdd(&col_count[col], 1u);
+  site_col[s] = make_uint2(col, rank);
 }
 
-__global__ void k_eng_col_clusters(const uint32_t* __restrict__ cnt, uint32_t n,
-                                   uint32_t* __restrict__ ncl) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < n) ncl[c] = (cnt[c] + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+__global__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
+                                  const uint32_t* __restrict__ col_start, uint32_t n,
+                                  uint32_t* __restrict__ items) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint2 cr = site_col[s];
+  const uint32_t k = col_start[cr.x] + cr.y;
+  if (k < n) items[k] = s;
 }
 
-// Single-workgroup exclusive scan of n uint32 values (n from *n_dev when
-// non-null) into out[0..n], out[n] = total (also to *total_out); *maxv = max.
+// Single-workgroup exclusive scans of two uint32 sequences of length n (n
+// from *n_dev when non-null).  MODE 0 (columns): a = site count, b =
+// ceil(a/8) clusters.  MODE 1 (rows): a = entries, b = work items.  Writes
+// out_x[0..n] with out_x[n] = total, the totals to tot_x and max(a) to *max_a.
 constexpr int kScanBlock = 1024;
+template <int MODE>
 __global__ void __launch_bounds__(kScanBlock)
-k_eng_scan(const uint32_t* __restrict__ in, uint32_t n_host, const uint32_t* __restrict__ n_dev,
-           uint32_t* __restrict__ out, uint32_t* __restrict__ total_out,
-           uint32_t* __restrict__ maxv) {
-  __shared__ uint32_t sh[kScanBlock];
-  __shared__ uint32_t shm[kScanBlock];
+k_eng_scan2(const uint32_t* __restrict__ in_a, const uint32_t* __restrict__ in_b, uint32_t n_host,
+            const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ out_a,
+            uint32_t* __restrict__ out_b, uint32_t* __restrict__ tot_a,
+            uint32_t* __restrict__ tot_b, uint32_t* __restrict__ max_a) {
+  __shared__ uint32_t wa[kScanBlock / kWave], wb[kScanBlock / kWave], wm[kScanBlock / kWave];
   const uint32_t n = n_dev ? *n_dev : n_host;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
-  const uint32_t b = t * per;
-  const uint32_t e = min(n, b + per);
-  uint32_t s = 0, m = 0;
-  for (uint32_t i = b; i < e; ++i) {
-    s += in[i];
-    m = max(m, in[i]);
+  const uint32_t b0 = t * per, e0 = min(n, b0 + per);
+  auto val_b = [&](uint32_t i, uint32_t a) {
+    return MODE == 0 ? (a + DYN_CL_SIZE - 1) / DYN_CL_SIZE : in_b[i];
+  };
+  uint32_t sa = 0, sb = 0, m = 0;
+  for (uint32_t i = b0; i < e0; ++i) {
+    const uint32_t a = in_a[i];
+    sa += a;
+    sb += val_b(i, a);
+    m = max(m, a);
   }
-  sh[t] = s;
-  shm[t] = m;
+  uint32_t ia = sa, ib = sb, im = m;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t va = __shfl_up(ia, off, kWave), vb = __shfl_up(ib, off, kWave);
+    if (lane >= off) {
+      ia += va;
+      ib += vb;
+    }
+    im = max(im, __shfl_xor(im, off, kWave));
+  }
+  if (lane == kWave - 1) {
+    wa[w] = ia;
+    wb[w] = ib;
+  }
+  if (lane == 0) wm[w] = im;
   __syncthreads();
-  for (int off = 1; off < kScanBlock; off <<= 1) {
-    const uint32_t v = t >= off ? sh[t - off] : 0u;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
+  uint32_t ba = 0, bb = 0, ta = 0, tb = 0, tm = 0;
+  for (int k = 0; k < kScanBlock / kWave; ++k) {
+    if (k < w) {
+      ba += wa[k];
+      bb += wb[k];
+    }
+    ta += wa[k];
+    tb += wb[k];
+    tm = max(tm, wm[k]);
   }
-  for (int off = kScanBlock / 2; off > 0; off >>= 1) {
-    if (t < off) shm[t] = max(shm[t], shm[t + off]);
-    __syncthreads();
+  uint32_t ra = ba + ia - sa, rb = bb + ib - sb;
+  for (uint32_t i = b0; i < e0; ++i) {
+    const uint32_t a = in_a[i];
+    out_a[i] = ra;
+    out_b[i] = rb;
+    ra += a;
+    rb += val_b(i, a);
   }
-  uint32_t run = sh[t] - s;
-  for (uint32_t i = b; i < e; ++i) {
-    out[i] = run;
-    run += in[i];
+  if (t == 0) {
+    out_a[n] = ta;
+    out_b[n] = tb;
+    if (tot_a) *tot_a = ta;
+    if (tot_b) *tot_b = tb;
+    if (max_a) *max_a = tm;
   }
-  if (t == kScanBlock - 1) {
-    out[n] = sh[t];
-    if (total_out) *total_out = sh[t];
-  }
-  if (t == 0 && maxv) *maxv = shm[0];
 }
 
 struct SlotArgs {
@@ -218,21 +214,24 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
 
 // One workgroup per column: sort the column's sites by (z, site) and cut
 // them into clusters of 8 (the last padded), writing the slot arrays and each
-// cluster's bounding box.
+// cluster's bounding box.  The column's count is cleared for the next
+// binning pass (the binning kernel counts with atomics into zeroed counts).
 constexpr int kSortBlock = 256;
 __global__ void __launch_bounds__(kSortBlock)
-k_eng_colsort(const uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
+k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
               const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
               const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
-              float4* __restrict__ bb_c, float4* __restrict__ bb_h,
+              float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
               unsigned int* __restrict__ status) {
   __shared__ unsigned long long key[kColCap];
   const int c = blockIdx.x;
   const uint32_t n = col_count[c];
+  __syncthreads();
+  if (threadIdx.x == 0 && n) col_count[c] = 0;
   const uint32_t b = col_start[c];
   const uint32_t cl0 = col_cl_start[c], ncl = col_cl_start[c + 1] - cl0;
   if (ncl == 0) return;
-  if (n > (uint32_t)kColCap) {
+  if (n > (uint32_t)kColCap || b + n > n_sites) {
     if (threadIdx.x == 0) atomicOr(status, kStColumn);
     return;
   }
@@ -335,6 +334,7 @@ struct ListArgs {
   const uint32_t* col_cl_start;
   const uint8_t* cl_class;
   const int32_t* slot_site;
+  const uint32_t* slot_of;   // per site: its slot in this build
   const uint32_t* ex_ptr;    // per solute site: exclusion CSR (1-2, 1-3 and 1-4)
   const uint32_t* ex_idx;
   uint32_t n_dyn;
@@ -342,76 +342,118 @@ struct ListArgs {
   unsigned int* status;
 };
 
-__device__ __forceinline__ bool solute_excluded(const ListArgs& A, int32_t sa, int32_t sb) {
-  uint32_t lo = A.ex_ptr[sa], hi = A.ex_ptr[sa + 1];
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    const uint32_t v = A.ex_idx[mid];
-    if (v == (uint32_t)sb) return true;
-    if (v < (uint32_t)sb) lo = mid + 1;
-    else hi = mid;
-  }
-  return false;
-}
+// LDS of the candidate walk, per batch of up to 64 columns: each column's
+// first cluster and its [beg, end) range in the flattened candidate sequence.
+struct CandLds {
+  uint32_t js[kWave], beg[kWave], end[kWave];
+};
 
-// Enumerates, in a fixed order, the clusters J whose bounding box is within
-// rlist of cluster I's box (periodic), for the classes that interact with
-// I's class; fn(J, ok) receives each 64-lane batch of candidates.
+// Enumerates, in a fixed order (class, column x, column y, cluster), the
+// clusters J whose bounding box is within rlist of cluster I's box
+// (periodic), for the classes that interact with I's class.  The columns of
+// a batch are flattened so that the wave tests 64 candidates per step;
+// fn(J, ok) receives each batch (all lanes call it).
 template <typename F>
-__device__ __forceinline__ void for_candidates(const ListArgs& A, uint32_t I, int ci, F fn) {
+__device__ __forceinline__ void for_candidates(const ListArgs& A, uint32_t I, int ci, CandLds& L,
+                                               F fn) {
   const int lane = threadIdx.x & (kWave - 1);
   const float4 c = A.bb_c[I], h = A.bb_h[I];
+  // a column is skipped when its x–y rectangle is beyond rlist of I's box;
+  // the margin keeps the test conservative against rounding of the binning
+  const float col_r2 = A.rlist2 * 1.0001f + 1e-3f;
   for (int cls = 0; cls < 3; ++cls) {
     if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
     const ClassGrid cg = A.g.cg[cls];
     if (cg.n == 0) continue;
     int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
     int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
+    bool full_x = false, full_y = false;
     if (x1 - x0 + 1 >= cg.ncx) {
       x0 = 0;
       x1 = cg.ncx - 1;
+      full_x = true;
     }
     if (y1 - y0 + 1 >= cg.ncy) {
       y0 = 0;
       y1 = cg.ncy - 1;
+      full_y = true;
     }
-    for (int xx = x0; xx <= x1; ++xx) {
-      const int cx = ((xx % cg.ncx) + cg.ncx) % cg.ncx;
-      for (int yy = y0; yy <= y1; ++yy) {
-        const int cy = ((yy % cg.ncy) + cg.ncy) % cg.ncy;
-        const int col = cg.col_off + cx * cg.ncy + cy;
-        const uint32_t js = A.col_cl_start[col], je = A.col_cl_start[col + 1];
-        for (uint32_t base = js; base < je; base += kWave) {
-          const uint32_t J = base + lane;
-          bool ok = J < je;
-          if (ok) {
-            const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
-            float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
-            dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
-            dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
-            dz = fabsf(dz - A.g.L[2] * rintf(dz * A.g.invL[2]));
-            dx = fmaxf(0.f, dx - h.x - hJ.x);
-            dy = fmaxf(0.f, dy - h.y - hJ.y);
-            dz = fmaxf(0.f, dz - h.z - hJ.z);
-            ok = dx * dx + dy * dy + dz * dz < A.rlist2;
-          }
-          fn(J, ok);
+    const int ny = y1 - y0 + 1, n_colc = (x1 - x0 + 1) * ny;
+    for (int cb = 0; cb < n_colc; cb += kWave) {
+      const int k = cb + lane;
+      uint32_t js = 0, cnt = 0;
+      if (k < n_colc) {
+        const int xx = x0 + k / ny, yy = y0 + k % ny;
+        float dx = 0.f, dy = 0.f;
+        if (!full_x) dx = fmaxf(0.f, fmaxf(xx * cg.wx - (c.x + h.x), (c.x - h.x) - (xx + 1) * cg.wx));
+        if (!full_y) dy = fmaxf(0.f, fmaxf(yy * cg.wy - (c.y + h.y), (c.y - h.y) - (yy + 1) * cg.wy));
+        if (dx * dx + dy * dy < col_r2) {
+          const int cx = ((xx % cg.ncx) + cg.ncx) % cg.ncx;
+          const int cy = ((yy % cg.ncy) + cg.ncy) % cg.ncy;
+          const int col = cg.col_off + cx * cg.ncy + cy;
+          js = A.col_cl_start[col];
+          cnt = A.col_cl_start[col + 1] - js;
         }
       }
+      uint32_t incl = cnt;
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+      }
+      const uint32_t tot = (uint32_t)__shfl(incl, kWave - 1, kWave);
+      L.js[lane] = js;
+      L.beg[lane] = incl - cnt;
+      L.end[lane] = incl;
+      __syncthreads();
+      for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        bool ok = t < tot;
+        uint32_t J = 0;
+        if (ok) {
+          int lo = 0, hi = kWave - 1;  // first column whose range ends after t
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L.end[mid] > t) hi = mid;
+            else lo = mid + 1;
+          }
+          J = L.js[lo] + (t - L.beg[lo]);
+          const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
+          float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
+          dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
+          dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
+          dz = fabsf(dz - A.g.L[2] * rintf(dz * A.g.invL[2]));
+          dx = fmaxf(0.f, dx - h.x - hJ.x);
+          dy = fmaxf(0.f, dy - h.y - hJ.y);
+          dz = fmaxf(0.f, dz - h.z - hJ.z);
+          ok = dx * dx + dy * dy + dz * dz < A.rlist2;
+        }
+        fn(J, ok);
+      }
+      __syncthreads();
     }
   }
 }
 
-// Pass 0: entries per row and work items per row.
+// Pass 0: each row's entries (enumeration order) into the row's
+// fixed-capacity slice of ent_tmp, and the row's entry and work-item counts.
 __global__ void __launch_bounds__(kWave)
-k_eng_list_count(ListArgs A, uint32_t* __restrict__ row_cnt, uint32_t* __restrict__ row_chunks) {
+k_eng_list_count(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
+                 uint32_t* __restrict__ row_chunks) {
+  __shared__ CandLds L;
   const uint32_t n_cl = *A.n_cl;
+  const int lane = threadIdx.x;
   for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
+    uint32_t* ent = ent_tmp + (size_t)I * kListCap;
     uint32_t cnt = 0;
-    for_candidates(A, I, A.cl_class[I], [&](uint32_t, bool ok) {
-      cnt += (uint32_t)__popcll(__ballot(ok));
+    for_candidates(A, I, A.cl_class[I], L, [&](uint32_t J, bool ok) {
+      const unsigned long long bal = __ballot(ok);
+      if (ok) {
+        const uint32_t k = cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (k < (uint32_t)kListCap) ent[k] = J;
+      }
+      cnt += (uint32_t)__popcll(bal);
     });
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       if (cnt > (uint32_t)kListCap) atomicOr(A.status, kStRow);
       const uint32_t m = min(cnt, (uint32_t)kListCap);
       row_cnt[I] = m;
@@ -420,43 +462,20 @@ k_eng_list_count(ListArgs A, uint32_t* __restrict__ row_cnt, uint32_t* __restric
   }
 }
 
-// Pass 1: write the row's entries (enumeration order), the solute–solute
-// exclusion masks, and the row's work items (I, first entry).
+// Pass 1: compact the rows into cj and write the work items (I, first entry).
 __global__ void __launch_bounds__(kWave)
-k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ row_ptr,
+k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
+                const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ chunk_ptr, uint32_t* __restrict__ cj,
-                unsigned long long* __restrict__ cmask, uint2* __restrict__ chunks) {
-  __shared__ uint32_t ent[kListCap];
+                uint2* __restrict__ chunks) {
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x;
-  const int ia = lane & 7, jb = lane >> 3;
   for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
-    const int ci = A.cl_class[I];
-    uint32_t cnt = 0;
-    for_candidates(A, I, ci, [&](uint32_t J, bool ok) {
-      const unsigned long long bal = __ballot(ok);
-      if (ok) {
-        const uint32_t slot = cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        if (slot < (uint32_t)kListCap) ent[slot] = J;
-      }
-      cnt += (uint32_t)__popcll(bal);
-    });
-    __syncthreads();
-    const uint32_t m = min(cnt, (uint32_t)kListCap);
+    const uint32_t m = row_cnt[I];
     const uint64_t base = row_ptr[I];
     if (base + m <= A.cap_entries) {
+      const uint32_t* ent = ent_tmp + (size_t)I * kListCap;
       for (uint32_t k = lane; k < m; k += kWave) cj[base + k] = ent[k];
-      if (ci == DYN_CLS_SOLUTE) {
-        const int32_t sa = A.slot_site[I * DYN_CL_SIZE + ia];
-        for (uint32_t k = 0; k < m; ++k) {
-          const uint32_t J = ent[k];
-          if (A.cl_class[J] != DYN_CLS_SOLUTE) break;  // solute clusters come first
-          const int32_t sb = A.slot_site[J * DYN_CL_SIZE + jb];
-          const bool ok = !(sa >= 0 && sb >= 0 && solute_excluded(A, sa, sb));
-          const unsigned long long mk = __ballot(ok);
-          if (lane == 0) cmask[base + k] = mk;
-        }
-      }
     } else if (lane == 0) {
       atomicOr(A.status, kStEntries);
     }
@@ -467,6 +486,77 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ row_ptr,
     } else if (lane == 0) {
       atomicOr(A.status, kStChunks);
     }
+  }
+}
+
+// Pass 2 (solute rows only; solute clusters are numbered first and come
+// first in every row): the 64-bit masks of the solute–solute entries.  Bit
+// ia + 8·jb is cleared when the sites in slots (I, ia) and (J, jb) are a
+// 1-2, 1-3 or 1-4 pair; the lanes walk the i sites' exclusion lists.
+constexpr int kMaskCap = 1024;  // solute entries of one solute row
+__global__ void __launch_bounds__(kWave)
+k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
+                const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
+                const uint32_t* __restrict__ col_cl_start, unsigned long long* __restrict__ cmask) {
+  __shared__ uint32_t s_ent[kMaskCap];
+  __shared__ unsigned long long s_mask[kMaskCap];
+  __shared__ uint32_t s_beg[DYN_CL_SIZE + 1], s_ex0[DYN_CL_SIZE];
+  const uint32_t n_sol = col_cl_start[A.g.cg[1].col_off];
+  const int lane = threadIdx.x;
+  for (uint32_t I = blockIdx.x; I < n_sol; I += gridDim.x) {
+    const uint32_t m = row_cnt[I];
+    const uint64_t base = row_ptr[I];
+    if (base + m > A.cap_entries) continue;  // the fill pass has flagged it
+    const uint32_t* ent = ent_tmp + (size_t)I * kListCap;
+    uint32_t nss = 0;
+    for (uint32_t k0 = 0; k0 < m; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      const unsigned long long bal = __ballot(k < m && ent[k] < n_sol);
+      nss += (uint32_t)__popcll(bal);
+      if (bal != ~0ull) break;
+    }
+    if (nss > (uint32_t)kMaskCap) {
+      if (lane == 0) atomicOr(A.status, kStRow);
+      continue;
+    }
+    for (uint32_t k = lane; k < nss; k += kWave) {
+      s_ent[k] = ent[k];
+      s_mask[k] = ~0ull;
+    }
+    uint32_t len = 0, ex0 = 0;
+    if (lane < DYN_CL_SIZE) {
+      const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
+      if (sa >= 0) {
+        ex0 = A.ex_ptr[sa];
+        len = A.ex_ptr[sa + 1] - ex0;
+      }
+    }
+    uint32_t incl = len;
+    for (int off = 1; off < DYN_CL_SIZE; off <<= 1) {
+      const uint32_t v = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += v;
+    }
+    if (lane < DYN_CL_SIZE) {
+      s_beg[lane] = incl - len;
+      s_ex0[lane] = ex0;
+    }
+    if (lane == DYN_CL_SIZE - 1) s_beg[DYN_CL_SIZE] = incl;
+    __syncthreads();
+    const uint32_t tot = s_beg[DYN_CL_SIZE];
+    for (uint32_t t = lane; t < tot; t += kWave) {
+      int ia = 0;
+      while (ia + 1 < DYN_CL_SIZE && s_beg[ia + 1] <= t) ++ia;
+      const uint32_t sb = A.ex_idx[s_ex0[ia] + (t - s_beg[ia])];
+      const uint32_t slot = A.slot_of[sb];
+      const uint32_t Jb = slot / DYN_CL_SIZE, jb = slot % DYN_CL_SIZE;
+      for (uint32_t k = 0; k < nss; ++k)
+        if (s_ent[k] == Jb) {
+          atomicAnd(&s_mask[k], ~(1ull << (ia + DYN_CL_SIZE * jb)));
+          break;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = lane; k < nss; k += kWave) cmask[base + k] = s_mask[k];
     __syncthreads();
   }
 }
@@ -483,9 +573,8 @@ struct PairArgs {
   const uint2* chunks;
   const uint32_t* n_chunks;
   const float2* lj_table;
-  int64_t* fx;          // component-major, n_slots per component
-  uint32_t n_slots;
-  double* partials;     // kNumPart per work item
+  int64_t* fitem;       // per work item: 3 components × 8 i slots, fixed point
+  double* partials;     // kNumPart values × kPairWaves (value-major)
   uint64_t cap_chunks;
   const unsigned int* status;  // a list-build overflow leaves the list unusable
 };
@@ -527,7 +616,11 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
 
 // One wave per work item (≤ DYN_CHUNK entries of one cluster row); lane =
 // ia + 8·jb holds the pair (slot 8I+ia, slot 8J+jb).  The next entry's j data
-// is loaded before the current entry is evaluated.
+// is loaded before the current entry is evaluated.  A work item's i-slot
+// forces are stored (not added) in its own row of fitem; energies and
+// counts accumulate per lane in f64 over the wave's items and are written
+// once per wave.  The grid is fixed (kPairWaves waves), so the item → wave
+// assignment and the order of every sum are the same on every call.
 __global__ void __launch_bounds__(kPairBlock)
 k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -536,6 +629,8 @@ k_eng_pairs(PairArgs A) {
   const uint32_t n_items = (uint32_t)min((uint64_t)*A.n_chunks, A.cap_chunks);
   const uint32_t wave0 = blockIdx.x * (kPairBlock / kWave) + (threadIdx.x / kWave);
   const uint32_t nwaves = gridDim.x * (kPairBlock / kWave);
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  uint32_t n_ss = 0, n_lj = 0, n_c = 0;
   for (uint32_t item = wave0; item < n_items; item += nwaves) {
     const uint2 ch = A.chunks[item];
     const uint32_t I = __builtin_amdgcn_readfirstlane(ch.x);
@@ -585,24 +680,30 @@ k_eng_pairs(PairArgs A) {
       T.fi[c] = v;
     }
     if (jb == 0) {
-      const uint32_t slot = I * DYN_CL_SIZE + ia;
-      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + slot), (unsigned long long)dyn_fx(T.fi[0]));
-      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_slots + slot), (unsigned long long)dyn_fx(T.fi[1]));
-      atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_slots + slot), (unsigned long long)dyn_fx(T.fi[2]));
+      int64_t* o = A.fitem + (size_t)item * (3 * DYN_CL_SIZE);
+      o[ia] = dyn_fx(T.fi[0]);
+      o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
+      o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
     }
-    double acc[5] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir};
-    for (int k = 0; k < 5; ++k) {
-      double v = acc[k];
-      for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
-      acc[k] = v;
-    }
-    if (lane == 0) {
-      double* p = A.partials + (size_t)item * kNumPart;
-      for (int k = 0; k < 5; ++k) p[k] = acc[k];
-      p[5] = (double)T.n_ss;
-      p[6] = (double)T.n_lj;
-      p[7] = (double)T.n_c;
-    }
+    acc[0] += (double)T.e_lj_w;
+    acc[1] += (double)T.e_lj_s;
+    acc[2] += (double)T.e_c_w;
+    acc[3] += (double)T.e_c_s;
+    acc[4] += (double)T.vir;
+    n_ss += T.n_ss;
+    n_lj += T.n_lj;
+    n_c += T.n_c;
+  }
+  for (int k = 0; k < 5; ++k) {
+    double v = acc[k];
+    for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
+    acc[k] = v;
+  }
+  if (lane == 0) {
+    for (int k = 0; k < 5; ++k) A.partials[(size_t)k * nwaves + wave0] = acc[k];
+    A.partials[(size_t)5 * nwaves + wave0] = (double)n_ss;
+    A.partials[(size_t)6 * nwaves + wave0] = (double)n_lj;
+    A.partials[(size_t)7 * nwaves + wave0] = (double)n_c;
   }
 }
 
@@ -697,44 +798,93 @@ k_eng_corr(CorrArgs A) {
   if (threadIdx.x == 0) A.partials[6 * blockIdx.x + 5] = 0.0;
 }
 
-__global__ void k_eng_pme_input(const float4* __restrict__ wpos, const uint32_t* __restrict__ qsite,
-                                uint32_t nq, float4* __restrict__ out) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nq) out[k] = wpos[qsite[k]];
+// Per-site set-up of an evaluation: clear the site force accumulators and
+// gather the charged sites' (x, y, z, q) for the SPME spread.
+__global__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
+                                const uint32_t* __restrict__ qsite, uint32_t nq,
+                                float4* __restrict__ pme_xyzq, int64_t* __restrict__ fx_site) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_sites) {
+    fx_site[s] = 0;
+    fx_site[(size_t)n_sites + s] = 0;
+    fx_site[2 * (size_t)n_sites + s] = 0;
+  }
+  if (pme_xyzq && s < nq) pme_xyzq[s] = wpos[qsite[s]];
 }
 
-// Site-order forces: site accumulators plus the slot accumulators of the
-// pair kernel.
+// Sum of the work-item force rows of slot k's cluster row (fixed point).
+__device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fitem,
+                                                 const uint32_t* __restrict__ chunk_ptr,
+                                                 uint64_t cap_chunks, uint32_t k, int64_t f[3]) {
+  const uint32_t I = k / DYN_CL_SIZE, ia = k % DYN_CL_SIZE;
+  const uint64_t c0 = min((uint64_t)chunk_ptr[I], cap_chunks);
+  const uint64_t c1 = min((uint64_t)chunk_ptr[I + 1], cap_chunks);
+  for (uint64_t c = c0; c < c1; ++c) {
+    const int64_t* o = fitem + c * (3 * DYN_CL_SIZE);
+    f[0] += o[ia];
+    f[1] += o[DYN_CL_SIZE + ia];
+    f[2] += o[2 * DYN_CL_SIZE + ia];
+  }
+}
+
+// Site-order forces: the site accumulators (corrections, SPME) plus the
+// pair kernel's work-item rows of the site's slot (when the pair kernel ran).
 __global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
-                               const int64_t* __restrict__ fx_slot,
+                               const int64_t* __restrict__ fitem,
+                               const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
                                const uint32_t* __restrict__ slot_of, uint32_t n_sites,
                                uint32_t n_slots, int64_t* __restrict__ out) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_sites) return;
-  const uint32_t k = min(slot_of[s], n_slots - 1);
-  for (int c = 0; c < 3; ++c)
-    out[(size_t)c * n_sites + s] = fx_site[(size_t)c * n_sites + s] + fx_slot[(size_t)c * n_slots + k];
+  int64_t f[3] = {fx_site[s], fx_site[(size_t)n_sites + s], fx_site[2 * (size_t)n_sites + s]};
+  if (fitem) slot_item_forces(fitem, chunk_ptr, cap_chunks, min(slot_of[s], n_slots - 1), f);
+  for (int c = 0; c < 3; ++c) out[(size_t)c * n_sites + s] = f[c];
 }
 
-// Fixed-order sums: one wave per value over the per-item partials (the full
-// list visits every pair twice, hence the ½) and the correction partials.
-// out[0..7]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c;
-// out[8..13]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, 0.
-__global__ void __launch_bounds__(1024)
-k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
-                    uint64_t cap_items, const double* __restrict__ cp, uint32_t n_cblk,
-                    double* __restrict__ out) {
-  const int lane = threadIdx.x & (kWave - 1), v = threadIdx.x / kWave;
-  if (v >= 14) return;
-  double a = 0.0;
-  if (v < kNumPart) {
-    const uint32_t n = (uint32_t)min((uint64_t)*n_items_dev, cap_items);
-    for (uint32_t r = lane; r < n; r += kWave) a += pp[(size_t)r * kNumPart + v];
-  } else {
-    for (uint32_t r = lane; r < n_cblk; r += kWave) a += cp[(size_t)r * 6 + (v - kNumPart)];
+// Slot-order pair forces (component-major), for inspection.
+__global__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
+                                  const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
+                                  const uint32_t* __restrict__ n_cl, uint32_t n_slots,
+                                  int64_t* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_slots || k >= *n_cl * DYN_CL_SIZE) return;
+  int64_t f[3] = {0, 0, 0};
+  slot_item_forces(fitem, chunk_ptr, cap_chunks, k, f);
+  for (int c = 0; c < 3; ++c) out[(size_t)c * n_slots + k] = f[c];
+}
+
+// Fixed-order sums of the partials.  out[0..7] from the pair kernel's
+// per-wave partials (× ½: the full list visits every pair twice): e_lj_w,
+// e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c; out[8..13] from the
+// correction partials: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, 0;
+// out[14..15]: SPME energy and virial (0 without SPME).
+constexpr int kRedBlock = 1024;
+__global__ void __launch_bounds__(kRedBlock)
+k_eng_energy_reduce(const double* __restrict__ pp, uint32_t n_waves,
+                    const double* __restrict__ cp, uint32_t n_cblk,
+                    const double* __restrict__ pme_res, double* __restrict__ out) {
+  constexpr int kVals = kNumPart + 6;
+  __shared__ double red[kVals][kRedBlock / kWave];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  double a[kVals];
+  for (int v = 0; v < kVals; ++v) a[v] = 0.0;
+  for (uint32_t r = t; r < n_waves; r += kRedBlock)
+    for (int v = 0; v < kNumPart; ++v) a[v] += pp[(size_t)v * n_waves + r];
+  for (uint32_t r = t; r < n_cblk; r += kRedBlock)
+    for (int v = 0; v < 6; ++v) a[kNumPart + v] += cp[(size_t)r * 6 + v];
+  for (int v = 0; v < kVals; ++v) {
+    double x = a[v];
+    for (int m = 1; m < kWave; m <<= 1) x += __shfl_xor(x, m, kWave);
+    if (lane == 0) red[v][w] = x;
   }
-  for (int m = 1; m < kWave; m <<= 1) a += __shfl_xor(a, m, kWave);
-  if (lane == 0) out[v] = v < kNumPart ? 0.5 * a : a;
+  __syncthreads();
+  if (t < kVals) {
+    double x = 0.0;
+    for (int k = 0; k < kRedBlock / kWave; ++k) x += red[t][k];
+    out[t] = t < kNumPart ? 0.5 * x : x;
+  }
+  if (t == kVals) out[14] = pme_res ? pme_res[0] : 0.0;
+  if (t == kVals + 1) out[15] = pme_res ? pme_res[1] : 0.0;
 }
 
 __global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
@@ -786,13 +936,14 @@ struct dyn_engine {
   // clustering (capacities fixed by the box geometry)
   EngGeom geom{};
   DevBuf<int> col_class;
-  DevBuf<uint32_t> col_count, col_start, col_ncl, col_cl_start, items;
+  DevBuf<uint32_t> col_count, col_start, col_cl_start, items;
+  DevBuf<uint2> site_col;
   uint32_t n_cols = 0, cap_cl = 0, cap_slots = 0;
   DevBuf<int32_t> slot_site, slot_type, slot_gid;
   DevBuf<uint8_t> cl_class;
   DevBuf<uint32_t> slot_of;
   DevBuf<float4> xyzq, bb_c, bb_h;
-  DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj;
+  DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj, ent_tmp;
   DevBuf<unsigned long long> cmask;
   DevBuf<uint2> chunks;
   uint64_t cap_entries = 0, cap_chunks = 0;
@@ -801,7 +952,7 @@ struct dyn_engine {
   DevBuf<unsigned int> status;
   uint64_t n_builds = 0;
   // accumulators
-  DevBuf<int64_t> fx_slot, fx_site, fx_out;
+  DevBuf<int64_t> fitem, fx_site, fx_out, fx_slot;  // fx_slot: inspection only
   DevBuf<double> pair_part, corr_part, eng_out;
   unsigned corr_blocks = 0;
   DevBuf<unsigned int> disp_bits;
@@ -864,7 +1015,6 @@ void dyn_engine::set_geometry() {
   col_class.reserve(std::max<uint32_t>(n_cols, 1));
   col_count.reserve(n_cols + 1);
   col_start.reserve(n_cols + 1);
-  col_ncl.reserve(n_cols + 1);
   col_cl_start.reserve(n_cols + 1);
   items.reserve(n_sites);
   slot_site.reserve(cap_slots);
@@ -878,7 +1028,9 @@ void dyn_engine::set_geometry() {
   row_chunks.reserve(cap_cl + 1);
   row_ptr.reserve(cap_cl + 1);
   chunk_ptr.reserve(cap_cl + 1);
-  fx_slot.reserve(3 * (size_t)cap_slots);
+  site_col.reserve(n_sites);
+  ent_tmp.reserve((size_t)cap_cl * kListCap);
+  pair_part.reserve((size_t)kNumPart * kPairWaves);
   // first guess of the list size from the site density; grown on overflow
   const double rl = prm.cutoff + prm.list_buffer + 4.0;
   const double per_row =
@@ -886,6 +1038,9 @@ void dyn_engine::set_geometry() {
   reserve_list((uint64_t)(cap_cl * per_row), (uint64_t)(cap_cl * (per_row / DYN_CHUNK + 1)));
   DYN_HIP(hipMemcpyAsync(col_class.p, h_col_class.data(), sizeof(int) * n_cols,
                          hipMemcpyHostToDevice, stream));
+  // the binning kernel counts into zeroed column counts; the column sort
+  // clears them again after use
+  DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
   DYN_HIP(hipStreamSynchronize(stream));
   list_valid = false;
 }
@@ -898,7 +1053,7 @@ void dyn_engine::reserve_list(uint64_t entries, uint64_t n_items) {
   }
   if (n_items > cap_chunks) {
     chunks.reserve(n_items);
-    pair_part.reserve((size_t)kNumPart * n_items);
+    fitem.reserve((size_t)3 * DYN_CL_SIZE * n_items);
     cap_chunks = n_items;
   }
 }
@@ -906,24 +1061,25 @@ void dyn_engine::reserve_list(uint64_t entries, uint64_t n_items) {
 void dyn_engine::build_list() {
   {
     Profiler::Scope ps(prof, "nbl_bin");
-    hipLaunchKernelGGL(k_eng_bin, dim3(3), dim3(kBinBlock), 0, stream, pos_dev, q.p, geom, wpos.p,
-                       col_count.p, col_start.p, items.p);
-    DYN_CHECK_LAUNCH("k_eng_bin");
-  }
-  {
-    Profiler::Scope ps(prof, "nbl_col_scan");
-    hipLaunchKernelGGL(k_eng_col_clusters, dim3(grid_for(n_cols, 256)), dim3(256), 0, stream,
-                       col_count.p, n_cols, col_ncl.p);
-    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, col_ncl.p, n_cols,
-                       (const uint32_t*)nullptr, col_cl_start.p, counters.p + 0, (uint32_t*)nullptr);
-    DYN_CHECK_LAUNCH("k_eng_scan");
+    const bool track = prm.list_buffer > 0.f;
+    hipLaunchKernelGGL(k_eng_bin_count, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
+                       pos_dev, q.p, geom, wpos.p, track ? wpos_ref.p : (float4*)nullptr,
+                       col_count.p, site_col.p);
+    DYN_CHECK_LAUNCH("k_eng_bin_count");
+    hipLaunchKernelGGL(k_eng_scan2<0>, dim3(1), dim3(kScanBlock), 0, stream, col_count.p,
+                       (const uint32_t*)nullptr, n_cols, (const uint32_t*)nullptr, col_start.p,
+                       col_cl_start.p, (uint32_t*)nullptr, counters.p + 0, (uint32_t*)nullptr);
+    DYN_CHECK_LAUNCH("k_eng_scan2<0>");
+    hipLaunchKernelGGL(k_eng_bin_scatter, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
+                       site_col.p, col_start.p, n_sites, items.p);
+    DYN_CHECK_LAUNCH("k_eng_bin_scatter");
   }
   {
     Profiler::Scope ps(prof, "nbl_colsort");
     SlotArgs S{wpos.p, ltype.p, wmol.p, type_none, xyzq.p, slot_type.p, slot_gid.p, slot_site.p, slot_of.p};
     hipLaunchKernelGGL(k_eng_colsort, dim3(n_cols), dim3(kSortBlock), 0, stream, col_count.p,
                        col_start.p, items.p, col_cl_start.p, col_class.p, S, cl_class.p, bb_c.p,
-                       bb_h.p, status.p);
+                       bb_h.p, n_sites, status.p);
     DYN_CHECK_LAUNCH("k_eng_colsort");
   }
   ListArgs A;
@@ -936,32 +1092,36 @@ void dyn_engine::build_list() {
   A.col_cl_start = col_cl_start.p;
   A.cl_class = cl_class.p;
   A.slot_site = slot_site.p;
+  A.slot_of = slot_of.p;
   A.ex_ptr = ex_ptr.p;
   A.ex_idx = ex_idx.p;
   A.n_dyn = n_dyn;
   A.cap_entries = cap_entries;
   A.cap_chunks = cap_chunks;
   A.status = status.p;
-  const unsigned list_grid = std::min<uint32_t>(cap_cl, 8192);
+  const unsigned list_grid = std::min<uint32_t>(cap_cl, 16384);
   {
     Profiler::Scope ps(prof, "nbl_list_count");
-    hipLaunchKernelGGL(k_eng_list_count, dim3(list_grid), dim3(kWave), 0, stream, A, row_cnt.p,
-                       row_chunks.p);
+    hipLaunchKernelGGL(k_eng_list_count, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
+                       row_cnt.p, row_chunks.p);
     DYN_CHECK_LAUNCH("k_eng_list_count");
-    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p, 0u,
-                       counters.p + 0, row_ptr.p, counters.p + 1, counters.p + 3);
-    hipLaunchKernelGGL(k_eng_scan, dim3(1), dim3(kScanBlock), 0, stream, row_chunks.p, 0u,
-                       counters.p + 0, chunk_ptr.p, counters.p + 2, (uint32_t*)nullptr);
-    DYN_CHECK_LAUNCH("k_eng_scan");
+    hipLaunchKernelGGL(k_eng_scan2<1>, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p,
+                       row_chunks.p, 0u, counters.p + 0, row_ptr.p, chunk_ptr.p, counters.p + 1,
+                       counters.p + 2, counters.p + 3);
+    DYN_CHECK_LAUNCH("k_eng_scan2<1>");
   }
   {
     Profiler::Scope ps(prof, "nbl_list_fill");
-    hipLaunchKernelGGL(k_eng_list_fill, dim3(list_grid), dim3(kWave), 0, stream, A, row_ptr.p,
-                       chunk_ptr.p, cj.p, cmask.p, chunks.p);
+    hipLaunchKernelGGL(k_eng_list_fill, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
+                       row_cnt.p, row_ptr.p, chunk_ptr.p, cj.p, chunks.p);
     DYN_CHECK_LAUNCH("k_eng_list_fill");
+    if (n_dyn) {
+      const uint32_t sol_rows = n_dyn / DYN_CL_SIZE + (uint32_t)(geom.cg[0].ncx * geom.cg[0].ncy) + 1;
+      hipLaunchKernelGGL(k_eng_list_mask, dim3(std::min<uint32_t>(sol_rows, 4096)), dim3(kWave), 0,
+                         stream, A, ent_tmp.p, row_cnt.p, row_ptr.p, col_cl_start.p, cmask.p);
+      DYN_CHECK_LAUNCH("k_eng_list_mask");
+    }
   }
-  DYN_HIP(hipMemcpyAsync(wpos_ref.p, wpos.p, sizeof(float4) * n_sites, hipMemcpyDeviceToDevice,
-                         stream));
   list_valid = true;
   ++n_builds;
 }
@@ -1005,13 +1165,15 @@ void dyn_engine::enqueue(uint32_t flags) {
   P.c6_oo = c6_oo;
   P.c12_oo = c12_oo;
   P.n_types = n_types;
-  {
-    Profiler::Scope ps(prof, "zero");
-    DYN_HIP(hipMemsetAsync(fx_slot.p, 0, sizeof(int64_t) * 3 * cap_slots, stream));
-    DYN_HIP(hipMemsetAsync(fx_site.p, 0, sizeof(int64_t) * 3 * n_sites, stream));
-    DYN_HIP(hipMemsetAsync(corr_part.p, 0, sizeof(double) * 6 * corr_blocks, stream));
-  }
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
+  const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
+  const bool corr_on = (flags & DYN_COMPUTE_CORR) && n_corr;
+  {
+    Profiler::Scope ps(prof, "site_prep");
+    hipLaunchKernelGGL(k_eng_site_prep, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, wpos.p,
+                       n_sites, qsite.p, n_q, recip_on ? pme_xyzq.p : (float4*)nullptr, fx_site.p);
+    DYN_CHECK_LAUNCH("k_eng_site_prep");
+  }
   if (short_on) {
     Profiler::Scope ps(prof, "pairs");
     PairArgs A;
@@ -1026,17 +1188,14 @@ void dyn_engine::enqueue(uint32_t flags) {
     A.chunks = chunks.p;
     A.n_chunks = counters.p + 2;
     A.lj_table = lj_table.p;
-    A.fx = fx_slot.p;
-    A.n_slots = cap_slots;
+    A.fitem = fitem.p;
     A.partials = pair_part.p;
     A.cap_chunks = cap_chunks;
     A.status = status.p;
-    const unsigned blocks = (unsigned)std::min<uint64_t>((cap_chunks + 3) / 4, 4096);
-    hipLaunchKernelGGL(k_eng_pairs, dim3(blocks), dim3(kPairBlock), 0, stream, A);
+    hipLaunchKernelGGL(k_eng_pairs, dim3(kPairGrid), dim3(kPairBlock), 0, stream, A);
     DYN_CHECK_LAUNCH("k_eng_pairs");
   }
-  const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
-  if ((flags & DYN_COMPUTE_CORR) && n_corr) {
+  if (corr_on) {
     Profiler::Scope ps(prof, "corr");
     CorrArgs C;
     C.P = P;
@@ -1055,29 +1214,19 @@ void dyn_engine::enqueue(uint32_t flags) {
     hipLaunchKernelGGL(k_eng_corr, dim3(corr_blocks), dim3(kCorrBlock), 0, stream, C);
     DYN_CHECK_LAUNCH("k_eng_corr");
   }
-  if (recip_on) {
-    {
-      Profiler::Scope ps(prof, "pme_input");
-      hipLaunchKernelGGL(k_eng_pme_input, dim3(grid_for(n_q, 256)), dim3(256), 0, stream, wpos.p,
-                         qsite.p, n_q, pme_xyzq.p);
-      DYN_CHECK_LAUNCH("k_eng_pme_input");
-    }
-    pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr);
-  }
+  if (recip_on) pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr);
   {
     Profiler::Scope ps(prof, "finalize");
     hipLaunchKernelGGL(k_eng_finalize, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       fx_site.p, fx_slot.p, slot_of.p, n_sites, cap_slots, fx_out.p);
-    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(1024), 0, stream, pair_part.p,
-                       counters.p + 2, short_on ? cap_chunks : (uint64_t)0, corr_part.p,
-                       corr_blocks, eng_out.p);
+                       fx_site.p, short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
+                       cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
+    DYN_CHECK_LAUNCH("k_eng_finalize");
+    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream, pair_part.p,
+                       short_on ? (uint32_t)kPairWaves : 0u, corr_part.p,
+                       corr_on ? corr_blocks : 0u, recip_on ? pme.res_dev.p : (const double*)nullptr,
+                       eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
-  if (recip_on)
-    DYN_HIP(hipMemcpyAsync(eng_out.p + 14, pme.res_dev.p, 2 * sizeof(double),
-                           hipMemcpyDeviceToDevice, stream));
-  else
-    DYN_HIP(hipMemsetAsync(eng_out.p + 14, 0, 2 * sizeof(double), stream));
   computed = true;
   fx_fresh = false;
 }
@@ -1092,6 +1241,7 @@ void dyn_engine::check_status() {
   const uint32_t st = h_misc.p[0];
   if (!st) return;
   DYN_HIP(hipMemsetAsync(status.p, 0, 4, stream));
+  DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
   list_valid = false;
   if (st & kStColumn)
     throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
@@ -1329,8 +1479,11 @@ extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
     eng->set_geometry();
     if (eng->pme_on) {
       if (eng->pme.ngrid == 0)
+      {
         eng->pme.init(eng->device, eng->stream, &eng->prof, eng->prm.grid, eng->box,
                       eng->prm.beta, (int)eng->prm.order);
+        eng->pme.clear_in_fft = true;
+      }
       else
         eng->pme.set_box(eng->box);
     }
@@ -1514,10 +1667,16 @@ extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint3
     if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * ne, hipMemcpyDeviceToHost, s));
     if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, eng->cmask.p, 8 * ne, hipMemcpyDeviceToHost, s));
     if (items && nit) DYN_HIP(hipMemcpyAsync(items, eng->chunks.p, 8 * nit, hipMemcpyDeviceToHost, s));
-    if (fx_slot)
+    if (fx_slot) {
+      eng->fx_slot.reserve(3 * (size_t)eng->cap_slots);
+      hipLaunchKernelGGL(k_eng_slot_forces, dim3(grid_for(eng->cap_slots, 256)), dim3(256), 0, s,
+                         eng->fitem.p, eng->chunk_ptr.p, eng->cap_chunks, eng->counters.p + 0,
+                         eng->cap_slots, eng->fx_slot.p);
+      DYN_CHECK_LAUNCH("k_eng_slot_forces");
       for (int c = 0; c < 3; ++c)
         DYN_HIP(hipMemcpyAsync(fx_slot + c * ns, eng->fx_slot.p + (size_t)c * eng->cap_slots, 8 * ns,
                                hipMemcpyDeviceToHost, s));
+    }
     DYN_HIP(hipStreamSynchronize(s));
   });
 }

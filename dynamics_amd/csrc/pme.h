@@ -50,6 +50,11 @@ struct PmeCore {
   PinnedBuf<double> h_part;
   unsigned conv_blocks = 0;
   DevBuf<double> res_dev;       // [energy, virial] reduced on the device
+  // When set, the first FFT pass clears the fixed-point grid as it reads it
+  // (the grid then stays zero between runs and no separate clear is
+  // needed); when not set, the grid is cleared before each spread and keeps
+  // the spread charges after the run.
+  bool clear_in_fft = false;
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
             float beta, int order);

@@ -194,7 +194,7 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 // the int64 fixed-point charge grid (z pass only).
 template <int AXIS, int SIGN, bool FROM_FX>
 __global__ void __launch_bounds__(kFftBlock)
-k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
+k_fft_pass(float2* __restrict__ data, int64_t* __restrict__ grid_fx, int clear_fx, PassGeom g,
            FftAxis ax, const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
   const int N = ax.N;
@@ -208,8 +208,12 @@ k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassG
     split_e<AXIS>(e, nl, N, l, n);
     const size_t addr = pass_addr<AXIS>(g, line0 + l, n);
     float2 v;
-    if (FROM_FX) v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
-    else v = data[addr];
+    if (FROM_FX) {
+      v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
+      if (clear_fx) grid_fx[addr] = 0;
+    } else {
+      v = data[addr];
+    }
     A[l * N + n] = v;
   }
   __syncthreads();
@@ -455,6 +459,7 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
   beta = b;
   ngrid = (size_t)K[0] * K[1] * K[2];
   grid_fx.reserve(ngrid);
+  DYN_HIP(hipMemsetAsync(grid_fx.p, 0, sizeof(int64_t) * ngrid, stream));
   cgrid.reserve(ngrid);
   G.reserve(ngrid);
   for (int a = 0; a < 3; ++a) {
@@ -549,7 +554,7 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
     S.K[a] = (int)K[a];
     S.sc[a] = (float)((double)K[a] / (double)L[a]);
   }
-  {
+  if (!clear_in_fft) {
     Profiler::Scope ps(*prof, "pme_zero");
     DYN_HIP(hipMemsetAsync(grid_fx.p, 0, sizeof(int64_t) * ngrid, stream));
   }
@@ -570,13 +575,13 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   {
     Profiler::Scope ps(*prof, "pme_fft_z_fwd");
     hipLaunchKernelGGL((k_fft_pass<2, -1, true>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, grid_fx.p, gz, ax[2], tw[2].p);
+                       stream, cgrid.p, grid_fx.p, clear_in_fft ? 1 : 0, gz, ax[2], tw[2].p);
     DYN_CHECK_LAUNCH("k_fft_pass z fwd");
   }
   {
     Profiler::Scope ps(*prof, "pme_fft_y_fwd");
     hipLaunchKernelGGL((k_fft_pass<1, -1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
+                       stream, cgrid.p, (int64_t*)nullptr, 0, gy, ax[1], tw[1].p);
     DYN_CHECK_LAUNCH("k_fft_pass y fwd");
   }
   {
@@ -591,13 +596,13 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   {
     Profiler::Scope ps(*prof, "pme_fft_y_inv");
     hipLaunchKernelGGL((k_fft_pass<1, 1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
+                       stream, cgrid.p, (int64_t*)nullptr, 0, gy, ax[1], tw[1].p);
     DYN_CHECK_LAUNCH("k_fft_pass y inv");
   }
   {
     Profiler::Scope ps(*prof, "pme_fft_z_inv");
     hipLaunchKernelGGL((k_fft_pass<2, 1, false>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gz, ax[2], tw[2].p);
+                       stream, cgrid.p, (int64_t*)nullptr, 0, gz, ax[2], tw[2].p);
     DYN_CHECK_LAUNCH("k_fft_pass z inv");
   }
   {

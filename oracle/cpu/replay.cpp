@@ -7,6 +7,7 @@
 // addition is order-independent, the GPU's atomically accumulated sums must
 // match these bit for bit.  For the engine, the 8×8 wave tile and its xor-tree
 // reductions are emulated lane by lane.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -247,6 +248,199 @@ void orc_pme_spread(const float* xyzq, uint32_t n, const int* K, const float* bo
       }
     }
   }
+}
+
+}  // extern "C"
+
+// ---- engine cluster-pair list ---------------------------------------------
+// Host restatement of the engine's list build (dynamics_amd/csrc/engine.hip:
+// binning, column sort, clusters, bounding boxes, candidate walk, exclusion
+// masks) with the same f32 operations, so the GPU list must equal it entry
+// for entry.  Geometry follows the engine's rule: per class, columns of
+// width ≈ the edge of a cube holding 8 sites of the class, at most 64 per
+// axis.
+namespace {
+
+struct ListGeom {
+  int ncx[3], ncy[3], col_off[3], n[3];
+  float wx[3], wy[3];
+  int n_cols;
+};
+
+float wrap_f(float x, float L, float invL) {
+  float w = x - L * std::floor(x * invL);
+  if (!(w < L)) w = 0.0f;
+  if (w < 0.0f) w = 0.0f;
+  return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+// pos4: n_sites × (x, y, z, ·) in the engine's site order.  ex_ptr/ex_idx:
+// per solute site, the sorted partners of its 1-2, 1-3 and 1-4 pairs.
+// Outputs (caller-sized): slot_site[8·cap_cl], cl_class[cap_cl],
+// row_ptr[cap_cl + 1], cj[cap_entries], cmask[cap_entries] (written for
+// solute–solute entries, all ones elsewhere).  Returns 0, or −1 when a
+// capacity is exceeded.
+int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const float* box,
+                    float cutoff, float list_buffer, const uint32_t* ex_ptr,
+                    const uint32_t* ex_idx, uint32_t cap_cl, uint64_t cap_entries,
+                    uint32_t* n_cl_out, int32_t* slot_site, uint8_t* cl_class, uint32_t* row_ptr,
+                    uint32_t* cj, uint64_t* cmask, uint64_t* n_entries_out) {
+  const uint32_t ns = n_dyn + 4 * n_water;
+  const float L[3] = {box[0], box[1], box[2]};
+  const float invL[3] = {1.0f / box[0], 1.0f / box[1], 1.0f / box[2]};
+  ListGeom g{};
+  const double V = (double)box[0] * box[1] * box[2];
+  const uint32_t ncls[3] = {n_dyn, n_water, 3 * n_water};
+  int off = 0;
+  for (int c = 0; c < 3; ++c) {
+    g.n[c] = (int)ncls[c];
+    g.col_off[c] = off;
+    if (ncls[c] == 0) {
+      g.ncx[c] = g.ncy[c] = 0;
+      g.wx[c] = g.wy[c] = 1.f;
+      continue;
+    }
+    const double s = std::cbrt(8.0 * V / (double)ncls[c]);
+    g.ncx[c] = std::max(1, std::min(64, (int)std::floor(box[0] / s)));
+    g.ncy[c] = std::max(1, std::min(64, (int)std::floor(box[1] / s)));
+    g.wx[c] = box[0] / g.ncx[c];
+    g.wy[c] = box[1] / g.ncy[c];
+    off += g.ncx[c] * g.ncy[c];
+  }
+  g.n_cols = off;
+  // wrap and bin
+  std::vector<float> w(4 * (size_t)ns);
+  std::vector<std::vector<uint32_t>> col(g.n_cols);
+  std::vector<int> col_cls(g.n_cols);
+  for (int c = 0; c < 3; ++c)
+    for (int k = 0; k < g.ncx[c] * g.ncy[c]; ++k) col_cls[g.col_off[c] + k] = c;
+  for (uint32_t s = 0; s < ns; ++s) {
+    for (int a = 0; a < 3; ++a) w[4 * s + a] = wrap_f(pos4[4 * s + a], L[a], invL[a]);
+    const int c = s < n_dyn ? 0 : (((s - n_dyn) & 3u) == 0 ? 1 : 2);
+    int cx = (int)(w[4 * s] / g.wx[c]), cy = (int)(w[4 * s + 1] / g.wy[c]);
+    cx = std::min(std::max(cx, 0), g.ncx[c] - 1);
+    cy = std::min(std::max(cy, 0), g.ncy[c] - 1);
+    col[g.col_off[c] + cx * g.ncy[c] + cy].push_back(s);
+  }
+  // sort columns by (z, site), cut into clusters
+  std::vector<uint32_t> col_cl(g.n_cols + 1, 0);
+  std::vector<float> bbc, bbh;
+  std::vector<int32_t> site;
+  std::vector<uint8_t> cls;
+  for (int k = 0; k < g.n_cols; ++k) {
+    auto& v = col[k];
+    std::vector<uint64_t> key(v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+      uint32_t zb;
+      memcpy(&zb, &w[4 * v[i] + 2], 4);
+      key[i] = ((uint64_t)zb << 32) | v[i];
+    }
+    std::sort(key.begin(), key.end());
+    const size_t ncl = (v.size() + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+    col_cl[k + 1] = col_cl[k] + (uint32_t)ncl;
+    for (size_t q = 0; q < ncl; ++q) {
+      float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+      for (int m = 0; m < DYN_CL_SIZE; ++m) {
+        const size_t r = q * DYN_CL_SIZE + m;
+        if (r < v.size()) {
+          const uint32_t s = (uint32_t)(key[r] & 0xffffffffu);
+          site.push_back((int32_t)s);
+          for (int a = 0; a < 3; ++a) {
+            lo[a] = std::fmin(lo[a], w[4 * s + a]);
+            hi[a] = std::fmax(hi[a], w[4 * s + a]);
+          }
+        } else {
+          site.push_back(-1);
+        }
+      }
+      for (int a = 0; a < 3; ++a) {
+        bbc.push_back(0.5f * (lo[a] + hi[a]));
+        bbh.push_back(0.5f * (hi[a] - lo[a]));
+      }
+      cls.push_back((uint8_t)col_cls[k]);
+    }
+  }
+  const uint32_t n_cl = col_cl[g.n_cols];
+  if (n_cl > cap_cl) return -1;
+  *n_cl_out = n_cl;
+  memcpy(slot_site, site.data(), sizeof(int32_t) * site.size());
+  memcpy(cl_class, cls.data(), cls.size());
+  std::vector<uint32_t> slot_of(ns, 0);
+  for (size_t k = 0; k < site.size(); ++k)
+    if (site[k] >= 0) slot_of[site[k]] = (uint32_t)k;
+  // candidate walk
+  const float rlist = cutoff + std::max(0.f, list_buffer);
+  const float rlist2 = rlist * rlist;
+  const float col_r2 = rlist2 * 1.0001f + 1e-3f;
+  uint64_t ne = 0;
+  row_ptr[0] = 0;
+  for (uint32_t I = 0; I < n_cl; ++I) {
+    const float c[3] = {bbc[3 * I], bbc[3 * I + 1], bbc[3 * I + 2]};
+    const float h[3] = {bbh[3 * I], bbh[3 * I + 1], bbh[3 * I + 2]};
+    const int ci = cls[I];
+    for (int k = 0; k < 3; ++k) {
+      if (dyn_kind(ci, k) == DYN_KIND_NONE || g.n[k] == 0) continue;
+      int x0 = (int)std::floor((c[0] - h[0] - rlist) / g.wx[k]);
+      int x1 = (int)std::floor((c[0] + h[0] + rlist) / g.wx[k]);
+      int y0 = (int)std::floor((c[1] - h[1] - rlist) / g.wy[k]);
+      int y1 = (int)std::floor((c[1] + h[1] + rlist) / g.wy[k]);
+      bool full_x = false, full_y = false;
+      if (x1 - x0 + 1 >= g.ncx[k]) {
+        x0 = 0;
+        x1 = g.ncx[k] - 1;
+        full_x = true;
+      }
+      if (y1 - y0 + 1 >= g.ncy[k]) {
+        y0 = 0;
+        y1 = g.ncy[k] - 1;
+        full_y = true;
+      }
+      for (int xx = x0; xx <= x1; ++xx)
+        for (int yy = y0; yy <= y1; ++yy) {
+          float dx = 0.f, dy = 0.f;
+          if (!full_x)
+            dx = std::fmax(0.f, std::fmax((float)xx * g.wx[k] - (c[0] + h[0]),
+                                          (c[0] - h[0]) - (float)(xx + 1) * g.wx[k]));
+          if (!full_y)
+            dy = std::fmax(0.f, std::fmax((float)yy * g.wy[k] - (c[1] + h[1]),
+                                          (c[1] - h[1]) - (float)(yy + 1) * g.wy[k]));
+          if (!(dx * dx + dy * dy < col_r2)) continue;
+          const int cx = ((xx % g.ncx[k]) + g.ncx[k]) % g.ncx[k];
+          const int cy = ((yy % g.ncy[k]) + g.ncy[k]) % g.ncy[k];
+          const int cc = g.col_off[k] + cx * g.ncy[k] + cy;
+          for (uint32_t J = col_cl[cc]; J < col_cl[cc + 1]; ++J) {
+            float d[3];
+            for (int a = 0; a < 3; ++a) {
+              float v = c[a] - bbc[3 * J + a];
+              v = std::fabs(v - L[a] * std::rint(v * invL[a]));
+              d[a] = std::fmax(0.f, v - h[a] - bbh[3 * J + a]);
+            }
+            if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rlist2)) continue;
+            if (ne >= cap_entries) return -1;
+            uint64_t mk = ~0ull;
+            if (ci == DYN_CLS_SOLUTE && cls[J] == DYN_CLS_SOLUTE)
+              for (int ia = 0; ia < DYN_CL_SIZE; ++ia) {
+                const int32_t sa = site[I * DYN_CL_SIZE + ia];
+                if (sa < 0) continue;
+                for (uint32_t e = ex_ptr[sa]; e < ex_ptr[sa + 1]; ++e) {
+                  const uint32_t sl = slot_of[ex_idx[e]];
+                  if (sl / DYN_CL_SIZE == J) mk &= ~(1ull << (ia + DYN_CL_SIZE * (sl % DYN_CL_SIZE)));
+                }
+              }
+            cj[ne] = J;
+            cmask[ne] = mk;
+            ++ne;
+          }
+        }
+    }
+    row_ptr[I + 1] = (uint32_t)ne;
+  }
+  *n_entries_out = ne;
+  return 0;
 }
 
 }  // extern "C"

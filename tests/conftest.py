@@ -144,3 +144,53 @@ def rel_rms(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def exclusion_csr(system):
+    """Per solute site, the sorted partners of its excluded and 1-4 pairs
+    (both directions), as the engine's list build uses them."""
+    nd = system.n_dyn
+    sets = [set() for _ in range(nd)]
+    for arr in (system.excl_12_13, system.pairs_14):
+        for i, j in np.asarray(arr, np.int64).reshape(-1, 2):
+            sets[i].add(int(j))
+            sets[j].add(int(i))
+    ptr = np.zeros(nd + 1, np.uint32)
+    idx = []
+    for i in range(nd):
+        v = sorted(sets[i])
+        idx.extend(v)
+        ptr[i + 1] = ptr[i] + len(v)
+    return ptr, np.asarray(idx if idx else [0], np.uint32)
+
+
+def replay_engine_list(system, cutoff, list_buffer=0.0, cap_cl=None, cap_entries=None):
+    """Host restatement of the engine's cluster-pair list: returns
+    (n_clusters, slot_site, cl_class, row_ptr, cj, cmask)."""
+    L = oracle_lib()
+    pos, _, _, _, _, _ = system.site_arrays()
+    ns = pos.shape[0]
+    pos4 = np.zeros((ns, 4), np.float32)
+    pos4[:, :3] = pos
+    box = np.ascontiguousarray(system.box, np.float32)
+    ptr_, idx = exclusion_csr(system)
+    cap_cl = cap_cl or (ns // 8 + 3 * 64 * 64 + 1)
+    cap_entries = cap_entries or cap_cl * 512
+    slot_site = np.zeros(8 * cap_cl, np.int32)
+    cl_class = np.zeros(cap_cl, np.uint8)
+    row_ptr = np.zeros(cap_cl + 1, np.uint32)
+    cj = np.zeros(cap_entries, np.uint32)
+    cm = np.zeros(cap_entries, np.uint64)
+    ncl = C.c_uint32()
+    ne = C.c_uint64()
+    L.orc_engine_list.restype = C.c_int
+    L.orc_engine_list.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_float,
+                                  C.c_float, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_void_p]
+    rc = L.orc_engine_list(system.n_dyn, system.n_water, ptr(pos4), ptr(box), cutoff, list_buffer,
+                           ptr(ptr_), ptr(idx), cap_cl, cap_entries, C.byref(ncl), ptr(slot_site),
+                           ptr(cl_class), ptr(row_ptr), ptr(cj), ptr(cm), C.byref(ne))
+    assert rc == 0, "replay capacity exceeded"
+    n, m = ncl.value, ne.value
+    return n, slot_site[:8 * n], cl_class[:n], row_ptr[:n + 1], cj[:m], cm[:m]

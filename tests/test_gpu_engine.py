@@ -3,16 +3,22 @@
 the GPU-built list; forces within 1e-5 relative RMS and energies against the
 FP64 oracle of the same Hamiltonian, in reference and standard modes."""
 
+import os
+import re
+
 import numpy as np
 import pytest
 
-from conftest import rel_rms, replay_engine_tiles
+from conftest import rel_rms, replay_engine_list, replay_engine_tiles
 from dynamics_amd import api, systems
 from oracle import fp64, reference_path as rp
 
 pytestmark = pytest.mark.gpu
 
 RC = 10.0
+# entries per work item (DYN_CHUNK of the engine's header)
+CHUNK = int(re.search(r"#define DYN_CHUNK (\d+)", open(os.path.join(
+    os.path.dirname(__file__), "..", "dynamics_amd", "csrc", "engine.h")).read()).group(1))
 BETA = float(np.float32(0.3123))  # the engine takes β as f32
 
 
@@ -54,6 +60,34 @@ def test_short_range_vs_fp64_and_replay(gpu, cases, name):
     dbg = eng.debug_export()
     fx = replay_engine_tiles(dbg, s, RC, RC, BETA)
     assert np.array_equal(fx, dbg["fx_slot"]), "slot fixed-point sums differ from replay"
+
+
+@pytest.mark.parametrize("name,buffer", [("water512", 0.0), ("solvated", 0.0), ("solvated", 1.5),
+                                         ("chain2600_opc8600", 0.0)])
+def test_cluster_list_bit_exact(gpu, cases, name, buffer):
+    """Clusters, slots, cluster-pair entries and exclusion masks equal the
+    host restatement of the list build exactly."""
+    s = cases[name] if name in cases else systems.config1()
+    print(f"list parity: {name} sites={s.n_sites} buffer={buffer}", flush=True)
+    eng = _engine(s, list_buffer=buffer)
+    eng.compute(api.COMPUTE_SHORT)
+    dbg = eng.debug_export()
+    n, slot_site, cl_class, row_ptr, cj, cm = replay_engine_list(s, RC, buffer)
+    assert len(dbg["cl_class"]) == n
+    assert np.array_equal(dbg["slot_site"], slot_site)
+    assert np.array_equal(dbg["cl_class"], cl_class)
+    assert np.array_equal(dbg["row_ptr"], row_ptr)
+    assert np.array_equal(dbg["cj"], cj)
+    rows = np.repeat(np.arange(n), np.diff(row_ptr.astype(np.int64)))
+    ss = (cl_class[rows] == 0) & (cl_class[cj] == 0)
+    assert np.array_equal(dbg["cmask"][ss], cm[ss]), "solute–solute exclusion masks differ"
+    # work items: every row cut into chunks of at most CHUNK entries, in order
+    items = dbg["items"]
+    ch = CHUNK
+    nch = (np.diff(row_ptr.astype(np.int64)) + ch - 1) // ch
+    want_I = np.repeat(np.arange(n), nch)
+    want_e = np.concatenate([row_ptr[i] + ch * np.arange(k) for i, k in enumerate(nch)])
+    assert np.array_equal(items[:, 0], want_I) and np.array_equal(items[:, 1], want_e)
 
 
 def test_pair_count_matches_oracle(gpu, cases):
@@ -156,6 +190,45 @@ def test_config0_full(gpu):
     assert err <= 1e-5, f"force rel RMS {err:.2e}"
     E = eng.energies()
     assert abs(E["pair_count"] - ref["pair_count"]) <= 4
+
+
+def _full_size_check(s, rc, K, mode=api.MODE_REFERENCE):
+    eng = _engine(s, cutoff=rc, lj_cutoff=rc, grid=K, order=4, mode=mode)
+    eng.compute(api.COMPUTE_ALL | api.COMPUTE_REBUILD)
+    F = eng.site_forces()
+    E = eng.energies()
+    eng.close()
+    ref = fp64.reference_energy_and_forces(
+        s, rc, rc, BETA, K, 4, mode="reference" if mode == api.MODE_REFERENCE else "standard")
+    err = rel_rms(F, ref["forces"])
+    print(f"force rel RMS vs FP64 {err:.3e}", flush=True)
+    assert err <= 1e-5, f"force rel RMS {err:.2e}"
+    tot_scale = abs(ref["e_lj"]) + abs(ref["e_coul"]) + abs(ref["e_recip"]) + abs(ref["e_self"]) + abs(ref["e_corr"])
+    assert abs(E["total"] - ref["e_total"]) <= 1e-6 * tot_scale, (E["total"], ref["e_total"])
+    assert abs(E["reference"] - ref["e_reference"]) <= 1e-6 * tot_scale
+    assert abs(E["recip"] - ref["e_recip"]) <= 1e-6 * abs(ref["e_recip"])
+    # pairs within f32 rounding of the cutoff may fall either side
+    assert abs(E["pair_count"] - ref["pair_count"]) <= max(4, 1e-6 * ref["pair_count"])
+    # net force: the pair and correction sums are antisymmetric; SPME's net
+    # force is a discretisation residue, small against the RMS force
+    net = np.abs(F.sum(axis=0)).max() / np.sqrt((F ** 2).sum(axis=1).mean())
+    assert net < 1e-2 * np.sqrt(len(F)), net
+
+
+def test_bench_workload_vs_fp64(gpu):
+    """The 37,000-site solvated chain at the benchmark's settings (rc 10 Å,
+    SPME 64³ order 4, reference mode, list rebuilt)."""
+    s = systems.config1()
+    print(f"bench workload parity: sites={s.n_sites}", flush=True)
+    _full_size_check(s, 10.0, (64, 64, 64))
+
+
+def test_water_128A_vs_fp64(gpu):
+    """69,900 OPC waters in a 128 Å box, rc 9 Å, SPME 128³ order 4: the
+    largest single-system workload, at full size against FP64."""
+    s = systems.config2()
+    print(f"128 A water parity: sites={s.n_sites}", flush=True)
+    _full_size_check(s, 9.0, (128, 128, 128))
 
 
 def test_engine_errors(gpu, cases):
