@@ -381,7 +381,9 @@ class Engine:
         return e.as_dict()
 
     def debug_export(self):
-        """Cluster layout, list, work items and per-slot fixed-point pair forces."""
+        """Cluster layout, list (row_ptr and cmask per group of 8 j entries, cj
+        per entry: slot | class << 30), work items and per-slot fixed-point
+        pair forces."""
         ns, nc, ne, ni = C.c_uint32(), C.c_uint32(), C.c_uint64(), C.c_uint64()
         _check(lib.dyn_engine_debug_export(self.h, C.byref(ns), C.byref(nc), C.byref(ne),
                                            *[None] * 9, C.byref(ni), None), self._err)
@@ -389,13 +391,13 @@ class Engine:
         d = {"slot_site": np.zeros(ns, np.int32), "xyzq": np.zeros((ns, 4), np.float32),
              "slot_type": np.zeros(ns, np.int32), "slot_gid": np.zeros(ns, np.int32),
              "cl_class": np.zeros(nc, np.uint8), "row_ptr": np.zeros(nc + 1, np.uint32),
-             "cj": np.zeros(max(ne, 1), np.uint32), "cmask": np.zeros(max(ne, 1), np.uint64),
+             "cj": np.zeros(max(8 * ne, 1), np.uint32), "cmask": np.zeros(max(ne, 1), np.uint64),
              "items": np.zeros((max(ni, 1), 2), np.uint32), "fx_slot": np.zeros((3, ns), np.int64)}
         _check(lib.dyn_engine_debug_export(
             self.h, None, None, None, *[_ptr(d[k]) for k in [
                 "slot_site", "xyzq", "slot_type", "slot_gid", "cl_class", "row_ptr", "cj",
                 "cmask", "items"]], None, _ptr(d["fx_slot"])), self._err)
-        d["cj"] = d["cj"][:ne]
+        d["cj"] = d["cj"][:8 * ne]
         d["cmask"] = d["cmask"][:ne]
         d["items"] = d["items"][:ni]
         return d

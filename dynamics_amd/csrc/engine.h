@@ -12,8 +12,16 @@
 #define DYN_CLS_WATER_O 1
 #define DYN_CLS_WATER_Q 2
 
-#define DYN_CL_SIZE 8  // sites per cluster; a wave holds an 8×8 cluster pair
-#define DYN_CHUNK 32   // list entries per work item of the pair kernel
+#define DYN_CL_SIZE 8  // sites per cluster; a wave holds 8 i slots × 8 j slots
+#define DYN_CHUNK 16   // j groups (of 8 j slots) per work item of the pair kernel
+
+// A row of the pair list (one i cluster) holds j slots: every slot of a
+// nearby cluster that lies within the list radius of at least one i slot it
+// interacts with.  The slots are grouped by class, each class segment padded
+// to a multiple of 8 with the null slot, and a list entry stores
+// slot | class << DYN_JCLS_SHIFT.  A group is 8 consecutive entries.
+#define DYN_JCLS_SHIFT 30
+#define DYN_JSLOT_MASK ((1u << DYN_JCLS_SHIFT) - 1u)
 
 // Exclusion group of a cluster slot: the water molecule (≥ 0) for water
 // sites, −2 − site for solute atoms (unique), −1 for padding.  Two slots

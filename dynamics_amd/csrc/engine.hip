@@ -46,15 +46,12 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kPairBlock = 256;       // 4 waves, one work item each at a time
-constexpr int kListCap = 2048;        // entries per cluster row (LDS staging)
 constexpr int kColCap = 4096;         // sites per column (LDS sort)
 constexpr int kMaxColsPerClass = 4096;
-constexpr int kNumPart = 8;           // f64 partials per wave of the pair kernel
-constexpr int kPairGrid = 1792;       // pair kernel blocks: 7 waves per SIMD on 256 CUs (fixed: the sums' order depends on it)
-constexpr int kPairWaves = kPairGrid * (kPairBlock / kWave);
+constexpr int kNumPart = 8;           // f64 partials per work item of the pair kernel
 
 // status bits
-constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u;
+constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u;
 
 struct ClassGrid {
   int ncx, ncy;
@@ -334,135 +331,168 @@ struct ListArgs {
   const uint32_t* col_cl_start;
   const uint8_t* cl_class;
   const int32_t* slot_site;
+  const float4* xyzq;        // per slot
+  const int32_t* slot_gid;   // per slot
   const uint32_t* slot_of;   // per site: its slot in this build
   const uint32_t* ex_ptr;    // per solute site: exclusion CSR (1-2, 1-3 and 1-4)
   const uint32_t* ex_idx;
   uint32_t n_dyn;
-  uint64_t cap_entries, cap_chunks;
+  uint64_t cap_entries, cap_chunks;  // groups, work items
+  uint32_t row_cap;                  // entries per row slice of ent_tmp (multiple of 8)
+  uint32_t null_slot;                // padding j slot (exclusion group: padding)
+  uint32_t* longest;                 // longest row that overflowed its slice
   unsigned int* status;
 };
 
-// LDS of the candidate walk, per batch of up to 64 columns: each column's
-// first cluster and its [beg, end) range in the flattened candidate sequence.
-struct CandLds {
+// LDS of the row walk: a batch of up to 64 columns (first cluster, range in
+// the flattened candidate sequence), then the batch's candidate clusters
+// that passed the box test and their slots' positions and groups.
+struct WalkLds {
   uint32_t js[kWave], beg[kWave], end[kWave];
+  uint32_t okJ[kWave];
+  float4 jp[kWave * DYN_CL_SIZE];
+  int32_t jg[kWave * DYN_CL_SIZE];
 };
 
-// Enumerates, in a fixed order (class, column x, column y, cluster), the
-// clusters J whose bounding box is within rlist of cluster I's box
-// (periodic), for the classes that interact with I's class.  The columns of
-// a batch are flattened so that the wave tests 64 candidates per step;
-// fn(J, ok) receives each batch (all lanes call it).
-template <typename F>
-__device__ __forceinline__ void for_candidates(const ListArgs& A, uint32_t I, int ci, CandLds& L,
-                                               F fn) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const float4 c = A.bb_c[I], h = A.bb_h[I];
+// Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
+// order (class, column x, column y, cluster), keep those whose bounding box
+// is within rlist of I's box (periodic), test their 8 slots against I's 8
+// slots (the same f32 min image and r² as the pair kernel), and append every
+// j slot within rlist of at least one i slot it can interact with.  Each
+// class segment is padded to a multiple of 8 with the null slot.  The row
+// goes into its fixed-capacity slice of ent_tmp; counts are in groups of 8.
+__global__ void __launch_bounds__(kWave)
+k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
+                uint32_t* __restrict__ row_chunks) {
+  __shared__ WalkLds L;
+  const uint32_t n_cl = *A.n_cl;
+  const int lane = threadIdx.x;
+  const int ia = lane & (DYN_CL_SIZE - 1), jb = lane / DYN_CL_SIZE;
+  const unsigned long long lt = (1ull << lane) - 1ull;
   // a column is skipped when its x–y rectangle is beyond rlist of I's box;
   // the margin keeps the test conservative against rounding of the binning
   const float col_r2 = A.rlist2 * 1.0001f + 1e-3f;
-  for (int cls = 0; cls < 3; ++cls) {
-    if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
-    const ClassGrid cg = A.g.cg[cls];
-    if (cg.n == 0) continue;
-    int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
-    int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
-    bool full_x = false, full_y = false;
-    if (x1 - x0 + 1 >= cg.ncx) {
-      x0 = 0;
-      x1 = cg.ncx - 1;
-      full_x = true;
-    }
-    if (y1 - y0 + 1 >= cg.ncy) {
-      y0 = 0;
-      y1 = cg.ncy - 1;
-      full_y = true;
-    }
-    const int ny = y1 - y0 + 1, n_colc = (x1 - x0 + 1) * ny;
-    for (int cb = 0; cb < n_colc; cb += kWave) {
-      const int k = cb + lane;
-      uint32_t js = 0, cnt = 0;
-      if (k < n_colc) {
-        const int xx = x0 + k / ny, yy = y0 + k % ny;
-        float dx = 0.f, dy = 0.f;
-        if (!full_x) dx = fmaxf(0.f, fmaxf(xx * cg.wx - (c.x + h.x), (c.x - h.x) - (xx + 1) * cg.wx));
-        if (!full_y) dy = fmaxf(0.f, fmaxf(yy * cg.wy - (c.y + h.y), (c.y - h.y) - (yy + 1) * cg.wy));
-        if (dx * dx + dy * dy < col_r2) {
-          const int cx = ((xx % cg.ncx) + cg.ncx) % cg.ncx;
-          const int cy = ((yy % cg.ncy) + cg.ncy) % cg.ncy;
-          const int col = cg.col_off + cx * cg.ncy + cy;
-          js = A.col_cl_start[col];
-          cnt = A.col_cl_start[col + 1] - js;
-        }
-      }
-      uint32_t incl = cnt;
-      for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += v;
-      }
-      const uint32_t tot = (uint32_t)__shfl(incl, kWave - 1, kWave);
-      L.js[lane] = js;
-      L.beg[lane] = incl - cnt;
-      L.end[lane] = incl;
-      __syncthreads();
-      for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
-        const uint32_t t = t0 + lane;
-        bool ok = t < tot;
-        uint32_t J = 0;
-        if (ok) {
-          int lo = 0, hi = kWave - 1;  // first column whose range ends after t
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (L.end[mid] > t) hi = mid;
-            else lo = mid + 1;
-          }
-          J = L.js[lo] + (t - L.beg[lo]);
-          const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
-          float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
-          dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
-          dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
-          dz = fabsf(dz - A.g.L[2] * rintf(dz * A.g.invL[2]));
-          dx = fmaxf(0.f, dx - h.x - hJ.x);
-          dy = fmaxf(0.f, dy - h.y - hJ.y);
-          dz = fmaxf(0.f, dz - h.z - hJ.z);
-          ok = dx * dx + dy * dy + dz * dz < A.rlist2;
-        }
-        fn(J, ok);
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Pass 0: each row's entries (enumeration order) into the row's
-// fixed-capacity slice of ent_tmp, and the row's entry and work-item counts.
-__global__ void __launch_bounds__(kWave)
-k_eng_list_count(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
-                 uint32_t* __restrict__ row_chunks) {
-  __shared__ CandLds L;
-  const uint32_t n_cl = *A.n_cl;
-  const int lane = threadIdx.x;
   for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
-    uint32_t* ent = ent_tmp + (size_t)I * kListCap;
-    uint32_t cnt = 0;
-    for_candidates(A, I, A.cl_class[I], L, [&](uint32_t J, bool ok) {
-      const unsigned long long bal = __ballot(ok);
-      if (ok) {
-        const uint32_t k = cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        if (k < (uint32_t)kListCap) ent[k] = J;
+    uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
+    const int ci = A.cl_class[I];
+    const float4 c = A.bb_c[I], h = A.bb_h[I];
+    const float4 pi = A.xyzq[I * DYN_CL_SIZE + ia];
+    const int gi = A.slot_gid[I * DYN_CL_SIZE + ia];
+    uint32_t n_out = 0;
+    for (int cls = 0; cls < 3; ++cls) {
+      if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
+      const ClassGrid cg = A.g.cg[cls];
+      if (cg.n == 0) continue;
+      const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
+      int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
+      int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
+      bool full_x = false, full_y = false;
+      if (x1 - x0 + 1 >= cg.ncx) {
+        x0 = 0;
+        x1 = cg.ncx - 1;
+        full_x = true;
       }
-      cnt += (uint32_t)__popcll(bal);
-    });
+      if (y1 - y0 + 1 >= cg.ncy) {
+        y0 = 0;
+        y1 = cg.ncy - 1;
+        full_y = true;
+      }
+      const int ny = y1 - y0 + 1, n_colc = (x1 - x0 + 1) * ny;
+      for (int cb = 0; cb < n_colc; cb += kWave) {
+        const int k = cb + lane;
+        uint32_t js = 0, cnt = 0;
+        if (k < n_colc) {
+          const int xx = x0 + k / ny, yy = y0 + k % ny;
+          float dx = 0.f, dy = 0.f;
+          if (!full_x) dx = fmaxf(0.f, fmaxf(xx * cg.wx - (c.x + h.x), (c.x - h.x) - (xx + 1) * cg.wx));
+          if (!full_y) dy = fmaxf(0.f, fmaxf(yy * cg.wy - (c.y + h.y), (c.y - h.y) - (yy + 1) * cg.wy));
+          if (dx * dx + dy * dy < col_r2) {
+            const int cx = ((xx % cg.ncx) + cg.ncx) % cg.ncx;
+            const int cy = ((yy % cg.ncy) + cg.ncy) % cg.ncy;
+            const int col = cg.col_off + cx * cg.ncy + cy;
+            js = A.col_cl_start[col];
+            cnt = A.col_cl_start[col + 1] - js;
+          }
+        }
+        uint32_t incl = cnt;
+        for (int off = 1; off < kWave; off <<= 1) {
+          const uint32_t v = __shfl_up(incl, off, kWave);
+          if (lane >= off) incl += v;
+        }
+        const uint32_t tot = (uint32_t)__shfl(incl, kWave - 1, kWave);
+        L.js[lane] = js;
+        L.beg[lane] = incl - cnt;
+        L.end[lane] = incl;
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
+          const uint32_t t = t0 + lane;
+          bool ok = t < tot;
+          uint32_t J = 0;
+          if (ok) {
+            int lo = 0, hi = kWave - 1;  // first column whose range ends after t
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (L.end[mid] > t) hi = mid;
+              else lo = mid + 1;
+            }
+            J = L.js[lo] + (t - L.beg[lo]);
+            const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
+            float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
+            dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
+            dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
+            dz = fabsf(dz - A.g.L[2] * rintf(dz * A.g.invL[2]));
+            dx = fmaxf(0.f, dx - h.x - hJ.x);
+            dy = fmaxf(0.f, dy - h.y - hJ.y);
+            dz = fmaxf(0.f, dz - h.z - hJ.z);
+            ok = dx * dx + dy * dy + dz * dz < A.rlist2;
+          }
+          const unsigned long long bal = __ballot(ok);
+          const uint32_t n_ok = (uint32_t)__popcll(bal);
+          if (ok) L.okJ[__popcll(bal & lt)] = J;
+          __syncthreads();
+          for (uint32_t u = lane; u < n_ok * DYN_CL_SIZE; u += kWave) {
+            const uint32_t slot = L.okJ[u / DYN_CL_SIZE] * DYN_CL_SIZE + (u % DYN_CL_SIZE);
+            L.jp[u] = A.xyzq[slot];
+            L.jg[u] = A.slot_gid[slot];
+          }
+          __syncthreads();
+          for (uint32_t kk = 0; kk < n_ok; ++kk) {
+            const float4 pj = L.jp[kk * DYN_CL_SIZE + jb];
+            const int gj = L.jg[kk * DYN_CL_SIZE + jb];
+            const float d0 = dyn_min_image_rint(pi.x - pj.x, A.g.L[0], A.g.invL[0]);
+            const float d1 = dyn_min_image_rint(pi.y - pj.y, A.g.L[1], A.g.invL[1]);
+            const float d2 = dyn_min_image_rint(pi.z - pj.z, A.g.L[2], A.g.invL[2]);
+            const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+            const unsigned long long hb = __ballot(r2 < A.rlist2 && dyn_slot_pair_valid(gi, gj));
+            const bool keep = lane < DYN_CL_SIZE && ((hb >> (DYN_CL_SIZE * lane)) & 0xffull) != 0ull;
+            const unsigned long long kb = __ballot(keep);
+            if (keep) {
+              const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
+              if (pos < A.row_cap) ent[pos] = (L.okJ[kk] * DYN_CL_SIZE + lane) | tag;
+            }
+            n_out += (uint32_t)__popcll(kb);
+          }
+          __syncthreads();
+        }
+      }
+      const uint32_t pad = (DYN_CL_SIZE - n_out % DYN_CL_SIZE) % DYN_CL_SIZE;
+      if ((uint32_t)lane < pad && n_out + lane < A.row_cap) ent[n_out + lane] = A.null_slot | tag;
+      n_out += pad;
+    }
     if (lane == 0) {
-      if (cnt > (uint32_t)kListCap) atomicOr(A.status, kStRow);
-      const uint32_t m = min(cnt, (uint32_t)kListCap);
+      if (n_out > A.row_cap) {
+        atomicOr(A.status, kStRow);
+        atomicMax(A.longest, n_out);
+      }
+      const uint32_t m = min(n_out, A.row_cap) / DYN_CL_SIZE;
       row_cnt[I] = m;
       row_chunks[I] = (m + DYN_CHUNK - 1) / DYN_CHUNK;
     }
   }
 }
 
-// Pass 1: compact the rows into cj and write the work items (I, first entry).
+// Pass 1: compact the rows into cj (groups of 8 entries) and write the work
+// items (I, first group).
 __global__ void __launch_bounds__(kWave)
 k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
@@ -474,8 +504,9 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
     const uint32_t m = row_cnt[I];
     const uint64_t base = row_ptr[I];
     if (base + m <= A.cap_entries) {
-      const uint32_t* ent = ent_tmp + (size_t)I * kListCap;
-      for (uint32_t k = lane; k < m; k += kWave) cj[base + k] = ent[k];
+      const uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
+      uint32_t* out = cj + base * DYN_CL_SIZE;
+      for (uint32_t k = lane; k < m * DYN_CL_SIZE; k += kWave) out[k] = ent[k];
     } else if (lane == 0) {
       atomicOr(A.status, kStEntries);
     }
@@ -489,40 +520,39 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
   }
 }
 
-// Pass 2 (solute rows only; solute clusters are numbered first and come
-// first in every row): the 64-bit masks of the solute–solute entries.  Bit
-// ia + 8·jb is cleared when the sites in slots (I, ia) and (J, jb) are a
-// 1-2, 1-3 or 1-4 pair; the lanes walk the i sites' exclusion lists.
-constexpr int kMaskCap = 1024;  // solute entries of one solute row
+// Pass 2 (solute rows only; solute clusters are numbered first, and the
+// solute segment comes first in every row): the 64-bit masks of the
+// solute–solute groups.  Bit ia + 8·jb of a group is cleared when the sites
+// in i slot ia and the group's j entry jb are a 1-2, 1-3 or 1-4 pair; the
+// lanes walk the i sites' exclusion lists.
+constexpr int kMaskCap = 2048;  // solute entries of one solute row
 __global__ void __launch_bounds__(kWave)
 k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ col_cl_start, unsigned long long* __restrict__ cmask) {
   __shared__ uint32_t s_ent[kMaskCap];
-  __shared__ unsigned long long s_mask[kMaskCap];
+  __shared__ unsigned long long s_mask[kMaskCap / DYN_CL_SIZE];
   __shared__ uint32_t s_beg[DYN_CL_SIZE + 1], s_ex0[DYN_CL_SIZE];
   const uint32_t n_sol = col_cl_start[A.g.cg[1].col_off];
   const int lane = threadIdx.x;
   for (uint32_t I = blockIdx.x; I < n_sol; I += gridDim.x) {
-    const uint32_t m = row_cnt[I];
+    const uint32_t m = row_cnt[I] * DYN_CL_SIZE;
     const uint64_t base = row_ptr[I];
-    if (base + m > A.cap_entries) continue;  // the fill pass has flagged it
-    const uint32_t* ent = ent_tmp + (size_t)I * kListCap;
+    if (base + row_cnt[I] > A.cap_entries) continue;  // the fill pass has flagged it
+    const uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
     uint32_t nss = 0;
     for (uint32_t k0 = 0; k0 < m; k0 += kWave) {
       const uint32_t k = k0 + lane;
-      const unsigned long long bal = __ballot(k < m && ent[k] < n_sol);
+      const unsigned long long bal = __ballot(k < m && (ent[k] >> DYN_JCLS_SHIFT) == DYN_CLS_SOLUTE);
       nss += (uint32_t)__popcll(bal);
       if (bal != ~0ull) break;
     }
     if (nss > (uint32_t)kMaskCap) {
-      if (lane == 0) atomicOr(A.status, kStRow);
+      if (lane == 0) atomicOr(A.status, kStMask);
       continue;
     }
-    for (uint32_t k = lane; k < nss; k += kWave) {
-      s_ent[k] = ent[k];
-      s_mask[k] = ~0ull;
-    }
+    for (uint32_t k = lane; k < nss; k += kWave) s_ent[k] = ent[k] & DYN_JSLOT_MASK;
+    for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) s_mask[g] = ~0ull;
     uint32_t len = 0, ex0 = 0;
     if (lane < DYN_CL_SIZE) {
       const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
@@ -548,15 +578,14 @@ k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
       while (ia + 1 < DYN_CL_SIZE && s_beg[ia + 1] <= t) ++ia;
       const uint32_t sb = A.ex_idx[s_ex0[ia] + (t - s_beg[ia])];
       const uint32_t slot = A.slot_of[sb];
-      const uint32_t Jb = slot / DYN_CL_SIZE, jb = slot % DYN_CL_SIZE;
       for (uint32_t k = 0; k < nss; ++k)
-        if (s_ent[k] == Jb) {
-          atomicAnd(&s_mask[k], ~(1ull << (ia + DYN_CL_SIZE * jb)));
+        if (s_ent[k] == slot) {
+          atomicAnd(&s_mask[k / DYN_CL_SIZE], ~(1ull << (ia + DYN_CL_SIZE * (k % DYN_CL_SIZE))));
           break;
         }
     }
     __syncthreads();
-    for (uint32_t k = lane; k < nss; k += kWave) cmask[base + k] = s_mask[k];
+    for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) cmask[base + g] = s_mask[g];
     __syncthreads();
   }
 }
@@ -567,14 +596,14 @@ struct PairArgs {
   const int32_t* slot_type;
   const int32_t* slot_gid;
   const uint8_t* cl_class;
-  const uint32_t* row_ptr;
-  const uint32_t* cj;
+  const uint32_t* row_ptr;   // groups
+  const uint32_t* cj;        // 8 entries per group
   const unsigned long long* cmask;
   const uint2* chunks;
   const uint32_t* n_chunks;
   const float2* lj_table;
-  int64_t* fitem;       // per work item: 3 components × 8 i slots, fixed point
-  double* partials;     // kNumPart values × kPairWaves (value-major)
+  int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
+  double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
   const unsigned int* status;  // a list-build overflow leaves the list unusable
 };
@@ -614,13 +643,13 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
   else T.n_c += h;
 }
 
-// One wave per work item (≤ DYN_CHUNK entries of one cluster row); lane =
-// ia + 8·jb holds the pair (slot 8I+ia, slot 8J+jb).  The next entry's j data
-// is loaded before the current entry is evaluated.  A work item's i-slot
-// forces are stored (not added) in its own row of fitem; energies and
-// counts accumulate per lane in f64 over the wave's items and are written
-// once per wave.  The grid is fixed (kPairWaves waves), so the item → wave
-// assignment and the order of every sum are the same on every call.
+// One wave per work item (≤ DYN_CHUNK groups of one row); lane ia + 8·jb
+// holds the pair (i slot 8I+ia, j entry jb of the group).  The next group's
+// j data is loaded before the current group is evaluated.  A work item
+// stores its 8 i-slot forces in its own row of fitem (no atomics) and its
+// energies, virial and pair counts as one partial per value; the grid has a
+// wave per work item, so the hardware's wave dispatch balances the load and
+// every sum keeps a fixed order.
 __global__ void __launch_bounds__(kPairBlock)
 k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -629,13 +658,11 @@ k_eng_pairs(PairArgs A) {
   const uint32_t n_items = (uint32_t)min((uint64_t)*A.n_chunks, A.cap_chunks);
   const uint32_t wave0 = blockIdx.x * (kPairBlock / kWave) + (threadIdx.x / kWave);
   const uint32_t nwaves = gridDim.x * (kPairBlock / kWave);
-  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  uint32_t n_ss = 0, n_lj = 0, n_c = 0;
   for (uint32_t item = wave0; item < n_items; item += nwaves) {
     const uint2 ch = A.chunks[item];
     const uint32_t I = __builtin_amdgcn_readfirstlane(ch.x);
-    const uint32_t e0 = __builtin_amdgcn_readfirstlane(ch.y);
-    const uint32_t e1 = min(e0 + (uint32_t)DYN_CHUNK, A.row_ptr[I + 1]);
+    const uint32_t g0 = __builtin_amdgcn_readfirstlane(ch.y);
+    const uint32_t g1 = min(g0 + (uint32_t)DYN_CHUNK, A.row_ptr[I + 1]);
     const float4 piv = A.xyzq[I * DYN_CL_SIZE + ia];
     const float pi[4] = {piv.x, piv.y, piv.z, piv.w};
     const int ti = A.slot_type[I * DYN_CL_SIZE + ia];
@@ -645,23 +672,25 @@ k_eng_pairs(PairArgs A) {
     T.fi[0] = T.fi[1] = T.fi[2] = 0.f;
     T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = 0.f;
     T.n_ss = T.n_lj = T.n_c = 0;
-    uint32_t J = __builtin_amdgcn_readfirstlane(A.cj[e0]);
-    float4 pj = A.xyzq[J * DYN_CL_SIZE + jb];
-    int tj = A.slot_type[J * DYN_CL_SIZE + jb];
-    int gj = A.slot_gid[J * DYN_CL_SIZE + jb];
-    for (uint32_t e = e0; e < e1; ++e) {
-      const uint32_t Jc = J;
+    uint32_t v = A.cj[(size_t)g0 * DYN_CL_SIZE + jb];
+    uint32_t js = v & DYN_JSLOT_MASK;
+    float4 pj = A.xyzq[js];
+    int tj = A.slot_type[js];
+    int gj = A.slot_gid[js];
+    for (uint32_t g = g0; g < g1; ++g) {
+      const int cls = (int)(__builtin_amdgcn_readfirstlane(v) >> DYN_JCLS_SHIFT);
       const float4 pjc = pj;
       const int tjc = tj, gjc = gj;
-      if (e + 1 < e1) {
-        J = __builtin_amdgcn_readfirstlane(A.cj[e + 1]);
-        pj = A.xyzq[J * DYN_CL_SIZE + jb];
-        tj = A.slot_type[J * DYN_CL_SIZE + jb];
-        gj = A.slot_gid[J * DYN_CL_SIZE + jb];
+      if (g + 1 < g1) {
+        v = A.cj[(size_t)(g + 1) * DYN_CL_SIZE + jb];
+        js = v & DYN_JSLOT_MASK;
+        pj = A.xyzq[js];
+        tj = A.slot_type[js];
+        gj = A.slot_gid[js];
       }
-      const int kind = dyn_kind(ci, A.cl_class[Jc]);
+      const int kind = dyn_kind(ci, cls);
       bool active = dyn_slot_pair_valid(gi, gjc);
-      if (kind == DYN_KIND_SS) active = active && ((A.cmask[e] >> lane) & 1ull);
+      if (kind == DYN_KIND_SS) active = active && ((A.cmask[g] >> lane) & 1ull);
       switch (kind) {
         case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ>(A, pi, ti, pjc, tjc, active, T); break;
         case DYN_KIND_OO: tile_entry<DYN_KIND_OO>(A, pi, ti, pjc, tjc, active, T); break;
@@ -671,13 +700,13 @@ k_eng_pairs(PairArgs A) {
         default: break;
       }
     }
-    // i forces: sum over the 8 j groups (lanes ia, ia+8, …, ia+56)
+    // i forces: sum over the 8 j lanes of each i slot (lanes ia, ia+8, …, ia+56)
     for (int c = 0; c < 3; ++c) {
-      float v = T.fi[c];
-      v = xor_sum(v, 8);
-      v = xor_sum(v, 16);
-      v = xor_sum(v, 32);
-      T.fi[c] = v;
+      float f = T.fi[c];
+      f = xor_sum(f, 8);
+      f = xor_sum(f, 16);
+      f = xor_sum(f, 32);
+      T.fi[c] = f;
     }
     if (jb == 0) {
       int64_t* o = A.fitem + (size_t)item * (3 * DYN_CL_SIZE);
@@ -685,25 +714,18 @@ k_eng_pairs(PairArgs A) {
       o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
       o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
     }
-    acc[0] += (double)T.e_lj_w;
-    acc[1] += (double)T.e_lj_s;
-    acc[2] += (double)T.e_c_w;
-    acc[3] += (double)T.e_c_s;
-    acc[4] += (double)T.vir;
-    n_ss += T.n_ss;
-    n_lj += T.n_lj;
-    n_c += T.n_c;
-  }
-  for (int k = 0; k < 5; ++k) {
-    double v = acc[k];
-    for (int m = 1; m < kWave; m <<= 1) v = xor_sum_d(v, m);
-    acc[k] = v;
-  }
-  if (lane == 0) {
-    for (int k = 0; k < 5; ++k) A.partials[(size_t)k * nwaves + wave0] = acc[k];
-    A.partials[(size_t)5 * nwaves + wave0] = (double)n_ss;
-    A.partials[(size_t)6 * nwaves + wave0] = (double)n_lj;
-    A.partials[(size_t)7 * nwaves + wave0] = (double)n_c;
+    double acc[5] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir};
+    for (int k = 0; k < 5; ++k) {
+      double x = acc[k];
+      for (int m = 1; m < kWave; m <<= 1) x = xor_sum_d(x, m);
+      acc[k] = x;
+    }
+    if (lane == 0) {
+      for (int k = 0; k < 5; ++k) A.partials[(size_t)k * A.cap_chunks + item] = acc[k];
+      A.partials[(size_t)5 * A.cap_chunks + item] = (double)T.n_ss;
+      A.partials[(size_t)6 * A.cap_chunks + item] = (double)T.n_lj;
+      A.partials[(size_t)7 * A.cap_chunks + item] = (double)T.n_c;
+    }
   }
 }
 
@@ -854,22 +876,23 @@ __global__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
 }
 
 // Fixed-order sums of the partials.  out[0..7] from the pair kernel's
-// per-wave partials (× ½: the full list visits every pair twice): e_lj_w,
+// per-item partials (× ½: the full list visits every pair twice): e_lj_w,
 // e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c; out[8..13] from the
 // correction partials: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, 0;
 // out[14..15]: SPME energy and virial (0 without SPME).
 constexpr int kRedBlock = 1024;
 __global__ void __launch_bounds__(kRedBlock)
-k_eng_energy_reduce(const double* __restrict__ pp, uint32_t n_waves,
-                    const double* __restrict__ cp, uint32_t n_cblk,
+k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
+                    uint64_t stride, const double* __restrict__ cp, uint32_t n_cblk,
                     const double* __restrict__ pme_res, double* __restrict__ out) {
   constexpr int kVals = kNumPart + 6;
   __shared__ double red[kVals][kRedBlock / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   double a[kVals];
   for (int v = 0; v < kVals; ++v) a[v] = 0.0;
-  for (uint32_t r = t; r < n_waves; r += kRedBlock)
-    for (int v = 0; v < kNumPart; ++v) a[v] += pp[(size_t)v * n_waves + r];
+  const uint32_t n_items = pp ? (uint32_t)min((uint64_t)*n_items_dev, stride) : 0u;
+  for (uint32_t r = t; r < n_items; r += kRedBlock)
+    for (int v = 0; v < kNumPart; ++v) a[v] += pp[(size_t)v * stride + r];
   for (uint32_t r = t; r < n_cblk; r += kRedBlock)
     for (int v = 0; v < 6; ++v) a[kNumPart + v] += cp[(size_t)r * 6 + v];
   for (int v = 0; v < kVals; ++v) {
@@ -946,8 +969,10 @@ struct dyn_engine {
   DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj, ent_tmp;
   DevBuf<unsigned long long> cmask;
   DevBuf<uint2> chunks;
-  uint64_t cap_entries = 0, cap_chunks = 0;
-  // device counters: [0] n_cl, [1] n_entries, [2] n_items, [3] max row
+  uint64_t cap_entries = 0, cap_chunks = 0;  // groups of 8 j entries, work items
+  uint32_t row_cap = 0;                      // j entries per row slice of ent_tmp
+  // device counters: [0] n_cl, [1] n_groups, [2] n_items, [3] max row (groups),
+  // [4] longest row that overflowed its slice (entries)
   DevBuf<uint32_t> counters;
   DevBuf<unsigned int> status;
   uint64_t n_builds = 0;
@@ -963,7 +988,7 @@ struct dyn_engine {
   // host copies
   PinnedBuf<int64_t> h_fx;
   PinnedBuf<double> h_eng;
-  PinnedBuf<uint32_t> h_misc;  // [0] status, [1..4] counters, [5] displacement bits
+  PinnedBuf<uint32_t> h_misc;  // [0] status, [1..5] counters, [6] displacement bits
   bool fx_fresh = false;
 
   ~dyn_engine() {
@@ -971,7 +996,8 @@ struct dyn_engine {
   }
 
   void set_geometry();
-  void reserve_list(uint64_t entries, uint64_t items);
+  void reserve_rows(uint32_t entries);
+  void reserve_list(uint64_t groups, uint64_t items);
   void build_list();
   void enqueue(uint32_t flags);
   void check_status();
@@ -1017,11 +1043,13 @@ void dyn_engine::set_geometry() {
   col_start.reserve(n_cols + 1);
   col_cl_start.reserve(n_cols + 1);
   items.reserve(n_sites);
-  slot_site.reserve(cap_slots);
-  slot_type.reserve(cap_slots);
-  slot_gid.reserve(cap_slots);
+  // slot arrays hold one more cluster: the null slot (index cap_slots) pads
+  // the j groups of the pair list
+  slot_site.reserve(cap_slots + DYN_CL_SIZE);
+  slot_type.reserve(cap_slots + DYN_CL_SIZE);
+  slot_gid.reserve(cap_slots + DYN_CL_SIZE);
   cl_class.reserve(cap_cl);
-  xyzq.reserve(cap_slots);
+  xyzq.reserve(cap_slots + DYN_CL_SIZE);
   bb_c.reserve(cap_cl);
   bb_h.reserve(cap_cl);
   row_cnt.reserve(cap_cl + 1);
@@ -1029,13 +1057,24 @@ void dyn_engine::set_geometry() {
   row_ptr.reserve(cap_cl + 1);
   chunk_ptr.reserve(cap_cl + 1);
   site_col.reserve(n_sites);
-  ent_tmp.reserve((size_t)cap_cl * kListCap);
-  pair_part.reserve((size_t)kNumPart * kPairWaves);
-  // first guess of the list size from the site density; grown on overflow
-  const double rl = prm.cutoff + prm.list_buffer + 4.0;
-  const double per_row =
-      std::min<double>(kListCap, 4.19 * rl * rl * rl * n_sites / V / DYN_CL_SIZE * 1.3 + 64);
-  reserve_list((uint64_t)(cap_cl * per_row), (uint64_t)(cap_cl * (per_row / DYN_CHUNK + 1)));
+  {
+    const int32_t none = type_none, pad = DYN_GID_PAD, nosite = -1;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    DYN_HIP(hipMemcpyAsync(slot_site.p + cap_slots, &nosite, 4, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipMemcpyAsync(slot_type.p + cap_slots, &none, 4, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipMemcpyAsync(slot_gid.p + cap_slots, &pad, 4, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipMemcpyAsync(xyzq.p + cap_slots, &zero, 16, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+  }
+  // first guesses from the site density, grown on overflow: a row holds the
+  // sites within about rlist + 3 Å of its cluster (all classes, padding)
+  const double dens = n_sites / V;
+  const double rl = prm.cutoff + prm.list_buffer;
+  const double row_est = dens * 4.19 * std::pow(rl + 3.0, 3) * 1.2 + 3 * DYN_CL_SIZE;
+  reserve_rows((uint32_t)std::min<double>(row_est, 1 << 20));
+  const double avg_groups = dens * 4.19 * std::pow(rl + 2.0, 3) / DYN_CL_SIZE + 3;
+  const uint64_t groups = (uint64_t)(cap_cl * avg_groups);
+  reserve_list(groups, groups / DYN_CHUNK + cap_cl);
   DYN_HIP(hipMemcpyAsync(col_class.p, h_col_class.data(), sizeof(int) * n_cols,
                          hipMemcpyHostToDevice, stream));
   // the binning kernel counts into zeroed column counts; the column sort
@@ -1045,15 +1084,24 @@ void dyn_engine::set_geometry() {
   list_valid = false;
 }
 
-void dyn_engine::reserve_list(uint64_t entries, uint64_t n_items) {
-  if (entries > cap_entries) {
-    cj.reserve(entries);
-    cmask.reserve(entries);
-    cap_entries = entries;
+// Row slices of the list walk: `entries` j entries per cluster (a multiple
+// of 64).
+void dyn_engine::reserve_rows(uint32_t entries) {
+  row_cap = std::max(row_cap, (entries + 63u) & ~63u);
+  ent_tmp.reserve((size_t)cap_cl * row_cap);  // grows with either factor
+}
+
+// List capacity in groups of 8 j entries, and work items.
+void dyn_engine::reserve_list(uint64_t groups, uint64_t n_items) {
+  if (groups > cap_entries) {
+    cj.reserve(groups * DYN_CL_SIZE);
+    cmask.reserve(groups);
+    cap_entries = groups;
   }
   if (n_items > cap_chunks) {
     chunks.reserve(n_items);
     fitem.reserve((size_t)3 * DYN_CL_SIZE * n_items);
+    pair_part.reserve((size_t)kNumPart * n_items);
     cap_chunks = n_items;
   }
 }
@@ -1092,19 +1140,24 @@ void dyn_engine::build_list() {
   A.col_cl_start = col_cl_start.p;
   A.cl_class = cl_class.p;
   A.slot_site = slot_site.p;
+  A.xyzq = xyzq.p;
+  A.slot_gid = slot_gid.p;
   A.slot_of = slot_of.p;
   A.ex_ptr = ex_ptr.p;
   A.ex_idx = ex_idx.p;
   A.n_dyn = n_dyn;
   A.cap_entries = cap_entries;
   A.cap_chunks = cap_chunks;
+  A.row_cap = row_cap;
+  A.null_slot = cap_slots;
+  A.longest = counters.p + 4;
   A.status = status.p;
   const unsigned list_grid = std::min<uint32_t>(cap_cl, 16384);
   {
-    Profiler::Scope ps(prof, "nbl_list_count");
-    hipLaunchKernelGGL(k_eng_list_count, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
+    Profiler::Scope ps(prof, "nbl_list_walk");
+    hipLaunchKernelGGL(k_eng_list_walk, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
                        row_cnt.p, row_chunks.p);
-    DYN_CHECK_LAUNCH("k_eng_list_count");
+    DYN_CHECK_LAUNCH("k_eng_list_walk");
     hipLaunchKernelGGL(k_eng_scan2<1>, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p,
                        row_chunks.p, 0u, counters.p + 0, row_ptr.p, chunk_ptr.p, counters.p + 1,
                        counters.p + 2, counters.p + 3);
@@ -1137,10 +1190,10 @@ void dyn_engine::enqueue(uint32_t flags) {
     DYN_HIP(hipMemsetAsync(disp_bits.p, 0, 4, stream));
     hipLaunchKernelGGL(k_eng_max_disp, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
                        wpos.p, wpos_ref.p, n_sites, geom, disp_bits.p);
-    DYN_HIP(hipMemcpyAsync(h_misc.p + 5, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
+    DYN_HIP(hipMemcpyAsync(h_misc.p + 6, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
     DYN_HIP(hipStreamSynchronize(stream));
     float d2;
-    memcpy(&d2, h_misc.p + 5, 4);
+    memcpy(&d2, h_misc.p + 6, 4);
     const float half = 0.5f * prm.list_buffer;
     if (!(d2 < half * half)) rebuild = true;
   }
@@ -1192,7 +1245,9 @@ void dyn_engine::enqueue(uint32_t flags) {
     A.partials = pair_part.p;
     A.cap_chunks = cap_chunks;
     A.status = status.p;
-    hipLaunchKernelGGL(k_eng_pairs, dim3(kPairGrid), dim3(kPairBlock), 0, stream, A);
+    // a wave per work item (the count is known on the device only)
+    const unsigned blocks = (unsigned)std::max<uint64_t>(1, (cap_chunks + 3) / 4);
+    hipLaunchKernelGGL(k_eng_pairs, dim3(blocks), dim3(kPairBlock), 0, stream, A);
     DYN_CHECK_LAUNCH("k_eng_pairs");
   }
   if (corr_on) {
@@ -1221,8 +1276,9 @@ void dyn_engine::enqueue(uint32_t flags) {
                        fx_site.p, short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
                        cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
     DYN_CHECK_LAUNCH("k_eng_finalize");
-    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream, pair_part.p,
-                       short_on ? (uint32_t)kPairWaves : 0u, corr_part.p,
+    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
+                       short_on ? pair_part.p : (const double*)nullptr, counters.p + 2, cap_chunks,
+                       corr_part.p,
                        corr_on ? corr_blocks : 0u, recip_on ? pme.res_dev.p : (const double*)nullptr,
                        eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
@@ -1236,7 +1292,7 @@ void dyn_engine::enqueue(uint32_t flags) {
 // rebuilt on the next call.
 void dyn_engine::check_status() {
   DYN_HIP(hipMemcpyAsync(h_misc.p, status.p, 4, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 16, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 20, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipStreamSynchronize(stream));
   const uint32_t st = h_misc.p[0];
   if (!st) return;
@@ -1245,14 +1301,20 @@ void dyn_engine::check_status() {
   list_valid = false;
   if (st & kStColumn)
     throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
-  if (st & kStRow)
-    throw Error(DYN_ERR_CAPACITY, fmt("a cluster has more than %d cluster-pair entries (reduce "
-                                      "list_buffer)", kListCap));
+  if (st & kStMask)
+    throw Error(DYN_ERR_CAPACITY, fmt("a solute cluster has more than %d solute neighbours "
+                                      "(reduce list_buffer)", kMaskCap));
+  if (st & kStRow) {
+    const uint32_t longest = h_misc.p[5];
+    DYN_HIP(hipMemsetAsync(counters.p + 4, 0, 4, stream));
+    reserve_rows(std::max<uint32_t>(longest + longest / 4, row_cap + row_cap / 2));
+  }
   const uint64_t ne = h_misc.p[2], nch = h_misc.p[3];
   reserve_list(ne + ne / 4 + 1024, nch + nch / 4 + 256);
-  throw Error(DYN_ERR_CAPACITY, fmt("cluster-pair list outgrew its buffers (%llu entries, %llu "
-                                    "work items); they have been enlarged, call again",
-                                    (unsigned long long)ne, (unsigned long long)nch));
+  throw Error(DYN_ERR_CAPACITY, fmt("pair list outgrew its buffers (%llu groups, %llu work "
+                                    "items, row slices of %u entries); they have been enlarged, "
+                                    "call again", (unsigned long long)ne,
+                                    (unsigned long long)nch, row_cap));
 }
 
 // ---------------------------------------------------------------------------
@@ -1424,12 +1486,12 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->eng_out.reserve(16);
     E->h_eng.reserve(16);
     E->h_misc.reserve(8);
-    E->counters.reserve(4);
+    E->counters.reserve(8);
     E->status.reserve(1);
     E->disp_bits.reserve(1);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
-    DYN_HIP(hipMemsetAsync(E->counters.p, 0, 16, E->stream));
+    DYN_HIP(hipMemsetAsync(E->counters.p, 0, 32, E->stream));
     DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
     DYN_HIP(hipMemsetAsync(E->status.p, 0, 4, E->stream));
     DYN_HIP(hipStreamSynchronize(E->stream));
@@ -1664,7 +1726,7 @@ extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint3
     if (slot_gid) DYN_HIP(hipMemcpyAsync(slot_gid, eng->slot_gid.p, 4 * ns, hipMemcpyDeviceToHost, s));
     if (cl_class) DYN_HIP(hipMemcpyAsync(cl_class, eng->cl_class.p, nc, hipMemcpyDeviceToHost, s));
     if (row_ptr) DYN_HIP(hipMemcpyAsync(row_ptr, eng->row_ptr.p, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
-    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * ne, hipMemcpyDeviceToHost, s));
+    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * DYN_CL_SIZE * ne, hipMemcpyDeviceToHost, s));
     if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, eng->cmask.p, 8 * ne, hipMemcpyDeviceToHost, s));
     if (items && nit) DYN_HIP(hipMemcpyAsync(items, eng->chunks.p, 8 * nit, hipMemcpyDeviceToHost, s));
     if (fx_slot) {

@@ -252,13 +252,14 @@ int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out);
 
 /* Diagnostics (no counterpart in the reference): copies of the current
  * cluster layout and list for host replay checks.  Sizes come first: call
- * with null arrays to read n_slots, n_clusters, n_entries and n_items.
- * slot_site[n_slots] (site index or −1 for padding), xyzq[4·n_slots]
- * (wrapped x, y, z, charge), slot_type[n_slots], slot_gid[n_slots]
- * (exclusion group), cl_class[n_clusters], row_ptr[n_clusters+1],
- * cj[n_entries], cmask[n_entries] (solute–solute entries only),
- * items[2·n_items] (cluster, first entry), fx_slot[3·n_slots] (fixed-point
- * pair forces per slot, component-major). */
+ * with null arrays to read n_slots, n_clusters, n_entries (groups of 8 j
+ * entries) and n_items.  slot_site[n_slots] (site index or −1 for padding),
+ * xyzq[4·n_slots] (wrapped x, y, z, charge), slot_type[n_slots],
+ * slot_gid[n_slots] (exclusion group), cl_class[n_clusters],
+ * row_ptr[n_clusters+1] (in groups), cj[8·n_entries] (j slot | class << 30;
+ * padding entries name a null slot beyond the exported ones), cmask[n_entries]
+ * (solute–solute groups only), items[2·n_items] (cluster, first group),
+ * fx_slot[3·n_slots] (fixed-point pair forces per slot, component-major). */
 int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clusters,
                             uint64_t* n_entries, int32_t* slot_site, float* xyzq,
                             int32_t* slot_type, int32_t* slot_gid, uint8_t* cl_class,

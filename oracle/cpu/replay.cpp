@@ -110,23 +110,28 @@ void orc_ref_pairs(const float* pos, uint32_t n_dyn, uint32_t n_water, uint32_t 
 }  // extern "C"
 
 // ---- native engine pair tiles -----------------------------------------------
-// Full cluster-pair list, rows cut into work items of ≤ DYN_CHUNK entries;
-// each lane (ia, jb) accumulates the force on slot 8I+ia over the item's
-// entries in list order, then the 8 j groups are summed by the xor-8/16/32
-// tree and converted to fixed point once per item.
+// Rows of j entries in groups of 8, cut into work items of ≤ DYN_CHUNK
+// groups; each lane (ia, jb) accumulates the force on i slot 8I+ia over the
+// item's groups in list order, then the 8 j lanes of each i slot are summed
+// by the xor-8/16/32 tree and converted to fixed point once per item.  A j
+// entry at or beyond n_slots is the null slot (padding).
 template <int KIND>
 static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
-                 uint32_t I, uint32_t J, uint64_t mask, const DynPairParams& P, const float2* lj,
-                 float fi[64][3]) {
+                 uint32_t n_slots, int type_none, uint32_t I, const uint32_t* jent, uint64_t mask,
+                 const DynPairParams& P, const float2* lj, float fi[64][3]) {
   for (int l = 0; l < 64; ++l) {
     const int ia = l & 7, jb = l >> 3;
+    const uint32_t js = jent[jb] & DYN_JSLOT_MASK;
+    const bool null_j = js >= n_slots;
     const float* a = xyzq + 4 * (size_t)(I * DYN_CL_SIZE + ia);
-    const float* b = xyzq + 4 * (size_t)(J * DYN_CL_SIZE + jb);
     const float pi[4] = {a[0], a[1], a[2], a[3]};
-    const float pj[4] = {b[0], b[1], b[2], b[3]};
+    float pj[4] = {0.f, 0.f, 0.f, 0.f};
+    if (!null_j)
+      for (int k = 0; k < 4; ++k) pj[k] = xyzq[4 * (size_t)js + k];
     const int ti = slot_type[I * DYN_CL_SIZE + ia];
-    const int tj = slot_type[J * DYN_CL_SIZE + jb];
-    bool active = dyn_slot_pair_valid(slot_gid[I * DYN_CL_SIZE + ia], slot_gid[J * DYN_CL_SIZE + jb]);
+    const int tj = null_j ? type_none : slot_type[js];
+    const int gj = null_j ? DYN_GID_PAD : slot_gid[js];
+    bool active = dyn_slot_pair_valid(slot_gid[I * DYN_CL_SIZE + ia], gj);
     if (KIND == DYN_KIND_SS) active = active && ((mask >> l) & 1ull);
     float d[3], s, elj, ec, r2;
     int hit;
@@ -139,6 +144,8 @@ static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slo
 
 extern "C" {
 
+// row_ptr and cmask per group, cj per entry (8 per group), items (I, first
+// group).  n_types includes water O and "none" (the last type).
 void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
                       const uint8_t* cl_class, const uint32_t* row_ptr, const uint32_t* cj,
                       const uint64_t* cmask, const uint32_t* items, uint64_t n_items,
@@ -162,19 +169,20 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
   memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
   static float fi[64][3];
   for (uint64_t it = 0; it < n_items; ++it) {
-    const uint32_t I = items[2 * it], e0 = items[2 * it + 1];
-    const uint32_t e1 = std::min<uint32_t>(e0 + DYN_CHUNK, row_ptr[I + 1]);
+    const uint32_t I = items[2 * it], g0 = items[2 * it + 1];
+    const uint32_t g1 = std::min<uint32_t>(g0 + DYN_CHUNK, row_ptr[I + 1]);
     memset(fi, 0, sizeof fi);
     const int ci = cl_class[I];
-    for (uint32_t e = e0; e < e1; ++e) {
-      const uint32_t J = cj[e];
-      const uint64_t mask = cmask[e];
-      switch (dyn_kind(ci, cl_class[J])) {
-        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
-        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
-        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
-        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
-        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, I, J, mask, P, lj, fi); break;
+    for (uint32_t g = g0; g < g1; ++g) {
+      const uint32_t* je = cj + (size_t)g * DYN_CL_SIZE;
+      const uint64_t mask = cmask[g];
+      const int tn = n_types - 1;
+      switch (dyn_kind(ci, (int)(je[0] >> DYN_JCLS_SHIFT))) {
+        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
+        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
+        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
+        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
+        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
         default: break;
       }
     }
@@ -281,9 +289,10 @@ extern "C" {
 // pos4: n_sites × (x, y, z, ·) in the engine's site order.  ex_ptr/ex_idx:
 // per solute site, the sorted partners of its 1-2, 1-3 and 1-4 pairs.
 // Outputs (caller-sized): slot_site[8·cap_cl], cl_class[cap_cl],
-// row_ptr[cap_cl + 1], cj[cap_entries], cmask[cap_entries] (written for
-// solute–solute entries, all ones elsewhere).  Returns 0, or −1 when a
-// capacity is exceeded.
+// row_ptr[cap_cl + 1] and cmask[cap_entries] per group of 8 j entries
+// (cmask written for solute–solute groups, all ones elsewhere),
+// cj[8·cap_entries] (slot | class << 30; padding names slot 2^30 − 1).
+// Returns 0, or −1 when a capacity is exceeded.
 int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const float* box,
                     float cutoff, float list_buffer, const uint32_t* ex_ptr,
                     const uint32_t* ex_idx, uint32_t cap_cl, uint64_t cap_entries,
@@ -372,18 +381,36 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
   std::vector<uint32_t> slot_of(ns, 0);
   for (size_t k = 0; k < site.size(); ++k)
     if (site[k] >= 0) slot_of[site[k]] = (uint32_t)k;
-  // candidate walk
+  // exclusion groups and slot positions as the engine stores them
+  std::vector<int32_t> gid(site.size());
+  for (size_t k = 0; k < site.size(); ++k) {
+    const int32_t st = site[k];
+    gid[k] = st < 0 ? DYN_GID_PAD
+                    : ((uint32_t)st < n_dyn ? -2 - st : (int32_t)(((uint32_t)st - n_dyn) / 4));
+  }
+  auto slot_pos = [&](size_t k, float p[3]) {
+    const int32_t st = site[k] >= 0 ? site[k] : site[k & ~(size_t)(DYN_CL_SIZE - 1)];
+    for (int a = 0; a < 3; ++a) p[a] = w[4 * (size_t)st + a];
+  };
+  // row walk: candidate clusters in the engine's order, then the j slots
+  // within rlist of an i slot they interact with, class segments padded to 8
   const float rlist = cutoff + std::max(0.f, list_buffer);
   const float rlist2 = rlist * rlist;
   const float col_r2 = rlist2 * 1.0001f + 1e-3f;
-  uint64_t ne = 0;
+  const uint32_t null_entry = DYN_JSLOT_MASK;
+  uint64_t ne = 0;  // groups
+  std::vector<uint32_t> row;
   row_ptr[0] = 0;
   for (uint32_t I = 0; I < n_cl; ++I) {
     const float c[3] = {bbc[3 * I], bbc[3 * I + 1], bbc[3 * I + 2]};
     const float h[3] = {bbh[3 * I], bbh[3 * I + 1], bbh[3 * I + 2]};
     const int ci = cls[I];
+    float pi[8][3];
+    for (int ia = 0; ia < 8; ++ia) slot_pos(I * DYN_CL_SIZE + ia, pi[ia]);
+    row.clear();
     for (int k = 0; k < 3; ++k) {
       if (dyn_kind(ci, k) == DYN_KIND_NONE || g.n[k] == 0) continue;
+      const uint32_t tag = (uint32_t)k << DYN_JCLS_SHIFT;
       int x0 = (int)std::floor((c[0] - h[0] - rlist) / g.wx[k]);
       int x1 = (int)std::floor((c[0] + h[0] + rlist) / g.wx[k]);
       int y0 = (int)std::floor((c[1] - h[1] - rlist) / g.wy[k]);
@@ -420,23 +447,43 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
               d[a] = std::fmax(0.f, v - h[a] - bbh[3 * J + a]);
             }
             if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rlist2)) continue;
-            if (ne >= cap_entries) return -1;
-            uint64_t mk = ~0ull;
-            if (ci == DYN_CLS_SOLUTE && cls[J] == DYN_CLS_SOLUTE)
-              for (int ia = 0; ia < DYN_CL_SIZE; ++ia) {
-                const int32_t sa = site[I * DYN_CL_SIZE + ia];
-                if (sa < 0) continue;
-                for (uint32_t e = ex_ptr[sa]; e < ex_ptr[sa + 1]; ++e) {
-                  const uint32_t sl = slot_of[ex_idx[e]];
-                  if (sl / DYN_CL_SIZE == J) mk &= ~(1ull << (ia + DYN_CL_SIZE * (sl % DYN_CL_SIZE)));
-                }
+            for (int jb = 0; jb < 8; ++jb) {
+              const size_t js = (size_t)J * DYN_CL_SIZE + jb;
+              float pj[3];
+              slot_pos(js, pj);
+              bool keep = false;
+              for (int ia = 0; ia < 8 && !keep; ++ia) {
+                const float d0 = dyn_min_image_rint(pi[ia][0] - pj[0], L[0], invL[0]);
+                const float d1 = dyn_min_image_rint(pi[ia][1] - pj[1], L[1], invL[1]);
+                const float d2 = dyn_min_image_rint(pi[ia][2] - pj[2], L[2], invL[2]);
+                const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+                keep = r2 < rlist2 && dyn_slot_pair_valid(gid[I * DYN_CL_SIZE + ia], gid[js]);
               }
-            cj[ne] = J;
-            cmask[ne] = mk;
-            ++ne;
+              if (keep) row.push_back((uint32_t)js | tag);
+            }
           }
         }
+      while (row.size() % DYN_CL_SIZE) row.push_back(null_entry | tag);
     }
+    const uint64_t ng = row.size() / DYN_CL_SIZE;
+    if (ne + ng > cap_entries) return -1;
+    for (uint64_t gq = 0; gq < ng; ++gq) {
+      const uint32_t* je = row.data() + gq * DYN_CL_SIZE;
+      uint64_t mk = ~0ull;
+      if (ci == DYN_CLS_SOLUTE && (je[0] >> DYN_JCLS_SHIFT) == DYN_CLS_SOLUTE)
+        for (int ia = 0; ia < DYN_CL_SIZE; ++ia) {
+          const int32_t sa = site[I * DYN_CL_SIZE + ia];
+          if (sa < 0) continue;
+          for (uint32_t e = ex_ptr[sa]; e < ex_ptr[sa + 1]; ++e) {
+            const uint32_t sl = slot_of[ex_idx[e]];
+            for (int jb = 0; jb < DYN_CL_SIZE; ++jb)
+              if ((je[jb] & DYN_JSLOT_MASK) == sl) mk &= ~(1ull << (ia + DYN_CL_SIZE * jb));
+          }
+        }
+      cmask[ne + gq] = mk;
+      memcpy(cj + (ne + gq) * DYN_CL_SIZE, je, sizeof(uint32_t) * DYN_CL_SIZE);
+    }
+    ne += ng;
     row_ptr[I + 1] = (uint32_t)ne;
   }
   *n_entries_out = ne;

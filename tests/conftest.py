@@ -164,9 +164,13 @@ def exclusion_csr(system):
     return ptr, np.asarray(idx if idx else [0], np.uint32)
 
 
-def replay_engine_list(system, cutoff, list_buffer=0.0, cap_cl=None, cap_entries=None):
-    """Host restatement of the engine's cluster-pair list: returns
-    (n_clusters, slot_site, cl_class, row_ptr, cj, cmask)."""
+NULL_ENTRY = (1 << 30) - 1  # j-slot field of a padding entry in the host replay
+
+
+def replay_engine_list(system, cutoff, list_buffer=0.0, cap_cl=None, cap_groups=None):
+    """Host restatement of the engine's pair list: returns (n_clusters,
+    slot_site, cl_class, row_ptr (groups), cj (8 per group: slot | class << 30,
+    padding = NULL_ENTRY | class << 30), cmask (per group))."""
     L = oracle_lib()
     pos, _, _, _, _, _ = system.site_arrays()
     ns = pos.shape[0]
@@ -175,12 +179,12 @@ def replay_engine_list(system, cutoff, list_buffer=0.0, cap_cl=None, cap_entries
     box = np.ascontiguousarray(system.box, np.float32)
     ptr_, idx = exclusion_csr(system)
     cap_cl = cap_cl or (ns // 8 + 3 * 64 * 64 + 1)
-    cap_entries = cap_entries or cap_cl * 512
+    cap_groups = cap_groups or cap_cl * 256
     slot_site = np.zeros(8 * cap_cl, np.int32)
     cl_class = np.zeros(cap_cl, np.uint8)
     row_ptr = np.zeros(cap_cl + 1, np.uint32)
-    cj = np.zeros(cap_entries, np.uint32)
-    cm = np.zeros(cap_entries, np.uint64)
+    cj = np.zeros(8 * cap_groups, np.uint32)
+    cm = np.zeros(cap_groups, np.uint64)
     ncl = C.c_uint32()
     ne = C.c_uint64()
     L.orc_engine_list.restype = C.c_int
@@ -189,8 +193,16 @@ def replay_engine_list(system, cutoff, list_buffer=0.0, cap_cl=None, cap_entries
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]
     rc = L.orc_engine_list(system.n_dyn, system.n_water, ptr(pos4), ptr(box), cutoff, list_buffer,
-                           ptr(ptr_), ptr(idx), cap_cl, cap_entries, C.byref(ncl), ptr(slot_site),
+                           ptr(ptr_), ptr(idx), cap_cl, cap_groups, C.byref(ncl), ptr(slot_site),
                            ptr(cl_class), ptr(row_ptr), ptr(cj), ptr(cm), C.byref(ne))
     assert rc == 0, "replay capacity exceeded"
     n, m = ncl.value, ne.value
-    return n, slot_site[:8 * n], cl_class[:n], row_ptr[:n + 1], cj[:m], cm[:m]
+    return n, slot_site[:8 * n], cl_class[:n], row_ptr[:n + 1], cj[:8 * m], cm[:m]
+
+
+def normalize_null_entries(cj, n_slots):
+    """GPU lists name their own null slot index; map it to NULL_ENTRY."""
+    cj = np.asarray(cj, np.uint32).copy()
+    slot = cj & np.uint32(NULL_ENTRY)
+    cj[slot >= n_slots] = (cj[slot >= n_slots] & ~np.uint32(NULL_ENTRY)) | np.uint32(NULL_ENTRY)
+    return cj

@@ -9,14 +9,14 @@ import re
 import numpy as np
 import pytest
 
-from conftest import rel_rms, replay_engine_list, replay_engine_tiles
+from conftest import normalize_null_entries, rel_rms, replay_engine_list, replay_engine_tiles
 from dynamics_amd import api, systems
 from oracle import fp64, reference_path as rp
 
 pytestmark = pytest.mark.gpu
 
 RC = 10.0
-# entries per work item (DYN_CHUNK of the engine's header)
+# j groups per work item (DYN_CHUNK of the engine's header)
 CHUNK = int(re.search(r"#define DYN_CHUNK (\d+)", open(os.path.join(
     os.path.dirname(__file__), "..", "dynamics_amd", "csrc", "engine.h")).read()).group(1))
 BETA = float(np.float32(0.3123))  # the engine takes β as f32
@@ -64,9 +64,9 @@ def test_short_range_vs_fp64_and_replay(gpu, cases, name):
 
 @pytest.mark.parametrize("name,buffer", [("water512", 0.0), ("solvated", 0.0), ("solvated", 1.5),
                                          ("chain2600_opc8600", 0.0)])
-def test_cluster_list_bit_exact(gpu, cases, name, buffer):
-    """Clusters, slots, cluster-pair entries and exclusion masks equal the
-    host restatement of the list build exactly."""
+def test_pair_list_bit_exact(gpu, cases, name, buffer):
+    """Clusters, slots, j entries (with padding) and exclusion masks equal
+    the host restatement of the list build exactly."""
     s = cases[name] if name in cases else systems.config1()
     print(f"list parity: {name} sites={s.n_sites} buffer={buffer}", flush=True)
     eng = _engine(s, list_buffer=buffer)
@@ -77,17 +77,17 @@ def test_cluster_list_bit_exact(gpu, cases, name, buffer):
     assert np.array_equal(dbg["slot_site"], slot_site)
     assert np.array_equal(dbg["cl_class"], cl_class)
     assert np.array_equal(dbg["row_ptr"], row_ptr)
-    assert np.array_equal(dbg["cj"], cj)
-    rows = np.repeat(np.arange(n), np.diff(row_ptr.astype(np.int64)))
-    ss = (cl_class[rows] == 0) & (cl_class[cj] == 0)
+    assert np.array_equal(normalize_null_entries(dbg["cj"], 8 * n), cj)
+    grp_row = np.repeat(np.arange(n), np.diff(row_ptr.astype(np.int64)))
+    ss = (cl_class[grp_row] == 0) & ((cj[::8] >> 30) == 0)
     assert np.array_equal(dbg["cmask"][ss], cm[ss]), "solute–solute exclusion masks differ"
-    # work items: every row cut into chunks of at most CHUNK entries, in order
+    # work items: every row cut into chunks of at most CHUNK groups, in order
     items = dbg["items"]
     ch = CHUNK
     nch = (np.diff(row_ptr.astype(np.int64)) + ch - 1) // ch
     want_I = np.repeat(np.arange(n), nch)
-    want_e = np.concatenate([row_ptr[i] + ch * np.arange(k) for i, k in enumerate(nch)])
-    assert np.array_equal(items[:, 0], want_I) and np.array_equal(items[:, 1], want_e)
+    want_g = np.concatenate([row_ptr[i] + ch * np.arange(k) for i, k in enumerate(nch)])
+    assert np.array_equal(items[:, 0], want_I) and np.array_equal(items[:, 1], want_g)
 
 
 def test_pair_count_matches_oracle(gpu, cases):
