@@ -117,6 +117,7 @@ _sig("dyn_engine_set_box", C.c_int, [_p, _f32p])
 _sig("dyn_engine_set_positions", C.c_int, [_p, _p, _p, _p, _p, _p])
 _sig("dyn_engine_set_positions_device", C.c_int, [_p, C.c_void_p])
 _sig("dyn_engine_compute", C.c_int, [_p, C.c_uint32])
+_sig("dyn_engine_set_alchemical", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_double])
 _sig("dyn_engine_sync", C.c_int, [_p])
 _sig("dyn_engine_get_forces", C.c_int, [_p, _p, _p])
 _sig("dyn_engine_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
@@ -139,6 +140,7 @@ EXPORTED = [
     "dyn_pme_get_fixed_grid", "dyn_pme_grid_n",
     "dyn_engine_create", "dyn_engine_destroy", "dyn_engine_last_error", "dyn_engine_set_box",
     "dyn_engine_set_positions", "dyn_engine_set_positions_device", "dyn_engine_compute",
+    "dyn_engine_set_alchemical",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
     "dyn_engine_profile",
@@ -355,6 +357,10 @@ class Engine:
 
     def compute(self, flags=COMPUTE_ALL):
         _check(lib.dyn_engine_compute(self.h, flags), self._err)
+
+    def set_alchemical(self, first, count, lam):
+        """Decouple solute atoms [first, first + count) at λ = lam (count 0: off)."""
+        _check(lib.dyn_engine_set_alchemical(self.h, int(first), int(count), float(lam)), self._err)
 
     def sync(self):
         _check(lib.dyn_engine_sync(self.h), self._err)

@@ -53,13 +53,20 @@ DYN_HD int dyn_kind(int ci, int cj) {
   return DYN_KIND_QQ;
 }
 
+// Type word of a slot: the LJ type in the low bits, DYN_TYPE_ALCH when the
+// site belongs to the decoupled (alchemical) molecule.
+#define DYN_TYPE_ALCH 0x10000
+#define DYN_TYPE_MASK 0xffff
+
 struct DynPairParams {
   float ext[3], inv_ext[3];
   float rc2;        // Coulomb (and list) cutoff squared
   float lj_rc2;     // LJ cutoff squared
   float beta, beta2, two_b_sqrtpi;
   float c6_oo, c12_oo;
-  int n_types;      // LJ type table is (n_types × n_types) float2 (c6, c12)
+  int n_types;      // LJ type tables are (n_types × n_types) float2
+  int alch_on;      // a decoupled molecule exists
+  DynSchedule sch;  // staged decoupling schedule at the current λ
 };
 
 // One lane of an 8×8 tile: the pair (i, j) of the given kind.  Returns the
@@ -87,7 +94,7 @@ DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, 
       c6 = P.c6_oo;
       c12 = P.c12_oo;
     } else {
-      const float2 c = lj_table[ti * P.n_types + tj];
+      const float2 c = lj_table[(ti & DYN_TYPE_MASK) * P.n_types + (tj & DYN_TYPE_MASK)];
       c6 = c.x;
       c12 = c.y;
     }
@@ -108,4 +115,44 @@ DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, 
   *e_c = ec;
   *r2_out = in ? r2 : 0.f;
   *hit = in ? 1 : 0;
+}
+
+// Alchemical variant (src/cuda/cuda.cu:367-566): a lane whose two sites lie
+// on different sides of the decoupled molecule (exactly one carries
+// DYN_TYPE_ALCH) takes Beutler soft-core LJ at the schedule's LJ λ and
+// Coulomb scaled by the schedule's factor, and returns dH/dλ; every other
+// lane returns dyn_tile_pair's result bit for bit.  lj_se holds the
+// combined (σ, ε) of each type pair.
+template <int KIND>
+DYN_HD void dyn_tile_pair_alch(const float pi[4], int ti, const float pj[4], int tj, bool active,
+                               const DynPairParams& P, const float2* lj_table,
+                               const float2* lj_se, float d[3], float* s_out, float* e_lj,
+                               float* e_c, float* r2_out, int* hit, float* dh) {
+  dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj_table, d, s_out, e_lj, e_c, r2_out, hit);
+  *dh = 0.f;
+  if (((ti ^ tj) & DYN_TYPE_ALCH) == 0 || !*hit) return;
+  const float r2 = *r2_out;
+  const float rinv = dyn_rsqrtf(r2);
+  float s = 0.f, elj = 0.f, ec = 0.f, dhh = 0.f;
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO) {
+    if (r2 <= P.lj_rc2) {
+      const float2 se = lj_se[(ti & DYN_TYPE_MASK) * P.n_types + (tj & DYN_TYPE_MASK)];
+      float ssc, esc, dd;
+      dyn_lj_soft_core(r2, se.x, se.y, P.sch.lj_lambda, &ssc, &esc, &dd);
+      s = ssc;
+      elj = esc;
+      dhh = dd * P.sch.lj_dlambda_dlambda;
+    }
+  }
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SQ) {
+    float sc, ee;
+    dyn_coul_erfc(r2, rinv, pi[3] * pj[3], P.beta, P.beta2, P.two_b_sqrtpi, &sc, &ee);
+    s = s + sc * P.sch.coulomb_scale;
+    ec = ee * P.sch.coulomb_scale;
+    dhh = dhh + ee * P.sch.coulomb_dscale_dlambda;
+  }
+  *s_out = s;
+  *e_lj = elj;
+  *e_c = ec;
+  *dh = dhh;
 }

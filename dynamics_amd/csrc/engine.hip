@@ -48,7 +48,7 @@ constexpr int kWave = 64;
 constexpr int kPairBlock = 256;       // 4 waves, one work item each at a time
 constexpr int kColCap = 4096;         // sites per column (LDS sort)
 constexpr int kMaxColsPerClass = 4096;
-constexpr int kNumPart = 8;           // f64 partials per work item of the pair kernel
+constexpr int kNumPart = 9;           // f64 partials per work item of the pair kernel
 
 // status bits
 constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u;
@@ -601,7 +601,8 @@ struct PairArgs {
   const unsigned long long* cmask;
   const uint2* chunks;
   const uint32_t* n_chunks;
-  const float2* lj_table;
+  const float2* lj_table;    // (c6, c12) per type pair
+  const float2* lj_se;       // (σ, ε) per type pair, for soft-core
   int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
@@ -613,17 +614,26 @@ __device__ __forceinline__ double xor_sum_d(double v, int m) { return v + __shfl
 
 struct TileAcc {
   float fi[3];
-  float e_lj_s, e_lj_w, e_c_s, e_c_w, vir;
+  float e_lj_s, e_lj_w, e_c_s, e_c_w, vir, dh;
   uint32_t n_ss, n_lj, n_c;  // wave-uniform counts (popcounts)
 };
 
-template <int KIND>
+// ALCH: the group holds at least one lane across the decoupled molecule's
+// boundary (the alchemical variant returns the plain result on the others).
+template <int KIND, bool ALCH>
 __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4], int ti,
                                            const float4& pjv, int tj, bool active, TileAcc& T) {
   const float pj[4] = {pjv.x, pjv.y, pjv.z, pjv.w};
   float d[3], s, elj, ec, r2;
   int hit;
-  dyn_tile_pair<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, d, &s, &elj, &ec, &r2, &hit);
+  if (ALCH) {
+    float dh;
+    dyn_tile_pair_alch<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, A.lj_se, d, &s, &elj, &ec,
+                             &r2, &hit, &dh);
+    T.dh += dh;
+  } else {
+    dyn_tile_pair<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, d, &s, &elj, &ec, &r2, &hit);
+  }
   T.fi[0] += s * d[0];
   T.fi[1] += s * d[1];
   T.fi[2] += s * d[2];
@@ -670,7 +680,7 @@ k_eng_pairs(PairArgs A) {
     const int ci = A.cl_class[I];
     TileAcc T;
     T.fi[0] = T.fi[1] = T.fi[2] = 0.f;
-    T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = 0.f;
+    T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = T.dh = 0.f;
     T.n_ss = T.n_lj = T.n_c = 0;
     uint32_t v = A.cj[(size_t)g0 * DYN_CL_SIZE + jb];
     uint32_t js = v & DYN_JSLOT_MASK;
@@ -691,12 +701,23 @@ k_eng_pairs(PairArgs A) {
       const int kind = dyn_kind(ci, cls);
       bool active = dyn_slot_pair_valid(gi, gjc);
       if (kind == DYN_KIND_SS) active = active && ((A.cmask[g] >> lane) & 1ull);
+      const bool alch = A.P.alch_on && kind <= DYN_KIND_SQ &&
+                        __ballot(((ti ^ tjc) & DYN_TYPE_ALCH) != 0) != 0ull;
       switch (kind) {
-        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_OO: tile_entry<DYN_KIND_OO>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_SS: tile_entry<DYN_KIND_SS>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_SO: tile_entry<DYN_KIND_SO>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_SQ: tile_entry<DYN_KIND_SQ>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_SS:
+          if (alch) tile_entry<DYN_KIND_SS, true>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SS, false>(A, pi, ti, pjc, tjc, active, T);
+          break;
+        case DYN_KIND_SO:
+          if (alch) tile_entry<DYN_KIND_SO, true>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SO, false>(A, pi, ti, pjc, tjc, active, T);
+          break;
+        case DYN_KIND_SQ:
+          if (alch) tile_entry<DYN_KIND_SQ, true>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SQ, false>(A, pi, ti, pjc, tjc, active, T);
+          break;
         default: break;
       }
     }
@@ -714,8 +735,8 @@ k_eng_pairs(PairArgs A) {
       o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
       o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
     }
-    double acc[5] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir};
-    for (int k = 0; k < 5; ++k) {
+    double acc[6] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir, T.dh};
+    for (int k = 0; k < 6; ++k) {
       double x = acc[k];
       for (int m = 1; m < kWave; m <<= 1) x = xor_sum_d(x, m);
       acc[k] = x;
@@ -725,6 +746,7 @@ k_eng_pairs(PairArgs A) {
       A.partials[(size_t)5 * A.cap_chunks + item] = (double)T.n_ss;
       A.partials[(size_t)6 * A.cap_chunks + item] = (double)T.n_lj;
       A.partials[(size_t)7 * A.cap_chunks + item] = (double)T.n_c;
+      A.partials[(size_t)8 * A.cap_chunks + item] = acc[5];
     }
   }
 }
@@ -735,8 +757,9 @@ k_eng_pairs(PairArgs A) {
 struct CorrArgs {
   DynPairParams P;
   const float4* wpos;      // per site (x, y, z, q)
-  const int32_t* ltype;    // per site
+  const int32_t* ltype;    // per site type word (DYN_TYPE_ALCH marks the decoupled molecule)
   const float2* lj_table;
+  const float2* lj_se;     // (σ, ε) per type pair, for soft-core
   const uint32_t* ci;
   const uint32_t* cj;
   const uint8_t* kind;
@@ -745,14 +768,14 @@ struct CorrArgs {
   int recip_on;            // reciprocal corrections only accompany the SPME sum
   int64_t* fx;             // site forces, component-major
   uint32_t n_sites;
-  double* partials;        // 6 per block
+  double* partials;        // 6 per block: e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ
 };
 
 constexpr int kCorrBlock = 256;
 __global__ void __launch_bounds__(kCorrBlock)
 k_eng_corr(CorrArgs A) {
   __shared__ double red[kCorrBlock];
-  double acc[6] = {0, 0, 0, 0, 0, 0};  // e_lj14, e_c14, e_corr, vir_sr, vir_recip, unused
+  double acc[6] = {0, 0, 0, 0, 0, 0};  // e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ
   for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += gridDim.x * kCorrBlock) {
     if (A.kind[k] == 0 && !A.recip_on) continue;
     const uint32_t i = A.ci[k], j = A.cj[k];
@@ -766,18 +789,36 @@ k_eng_corr(CorrArgs A) {
     const float qq = a.w * b.w;
     float s_sr = 0.f, s_rc = 0.f;
     if (A.kind[k] == 1) {
+      // a 1-4 pair across the decoupled molecule's boundary is also an
+      // alchemical pair (src/non_bonded.rs:581-599)
+      const bool cross = A.P.alch_on && ((A.ltype[i] ^ A.ltype[j]) & DYN_TYPE_ALCH) != 0;
+      const int tab = (A.ltype[i] & DYN_TYPE_MASK) * A.P.n_types + (A.ltype[j] & DYN_TYPE_MASK);
       if (r2 <= A.P.lj_rc2) {
-        const float2 c = A.lj_table[A.ltype[i] * A.P.n_types + A.ltype[j]];
         float sl, el;
-        dyn_lj(rinv * rinv, c.x, c.y, &sl, &el);
+        if (cross) {
+          const float2 se = A.lj_se[tab];
+          float dd;
+          dyn_lj_soft_core(r2, se.x, se.y, A.P.sch.lj_lambda, &sl, &el, &dd);
+          acc[5] += (double)(DYN_SCALE_LJ_14 * (dd * A.P.sch.lj_dlambda_dlambda));
+        } else {
+          const float2 c = A.lj_table[tab];
+          dyn_lj(rinv * rinv, c.x, c.y, &sl, &el);
+        }
         s_sr += DYN_SCALE_LJ_14 * sl;
         acc[0] += (double)(DYN_SCALE_LJ_14 * el);
       }
       if (r2 < A.P.rc2) {
         float sc, ec;
         dyn_coul_erfc(r2, rinv, qq, A.P.beta, A.P.beta2, A.P.two_b_sqrtpi, &sc, &ec);
-        s_sr += 0.833333333f * sc;
-        acc[1] += (double)(0.833333333f * ec);
+        sc = 0.833333333f * sc;
+        ec = 0.833333333f * ec;
+        if (cross) {
+          acc[5] += (double)(ec * A.P.sch.coulomb_dscale_dlambda);
+          sc = sc * A.P.sch.coulomb_scale;
+          ec = ec * A.P.sch.coulomb_scale;
+        }
+        s_sr += sc;
+        acc[1] += (double)ec;
       }
       if (!A.recip_on) {
         // no reciprocal sum, nothing to correct
@@ -807,7 +848,7 @@ k_eng_corr(CorrArgs A) {
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_sites + j), (unsigned long long)(-q1));
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_sites + j), (unsigned long long)(-q2));
   }
-  for (int v = 0; v < 5; ++v) {
+  for (int v = 0; v < 6; ++v) {
     red[threadIdx.x] = acc[v];
     __syncthreads();
     for (int s = kCorrBlock / 2; s > 0; s >>= 1) {
@@ -817,21 +858,28 @@ k_eng_corr(CorrArgs A) {
     if (threadIdx.x == 0) A.partials[6 * blockIdx.x + v] = red[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) A.partials[6 * blockIdx.x + 5] = 0.0;
 }
 
 // Per-site set-up of an evaluation: clear the site force accumulators and
 // gather the charged sites' (x, y, z, q) for the SPME spread.
+// With a decoupled molecule (sites [alch0, alch1)), pme_b receives a second
+// charge set with that molecule's charges negated.
 __global__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
                                 const uint32_t* __restrict__ qsite, uint32_t nq,
-                                float4* __restrict__ pme_xyzq, int64_t* __restrict__ fx_site) {
+                                float4* __restrict__ pme_xyzq, float4* __restrict__ pme_b,
+                                uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n_sites) {
     fx_site[s] = 0;
     fx_site[(size_t)n_sites + s] = 0;
     fx_site[2 * (size_t)n_sites + s] = 0;
   }
-  if (pme_xyzq && s < nq) pme_xyzq[s] = wpos[qsite[s]];
+  if (pme_xyzq && s < nq) {
+    const uint32_t site = qsite[s];
+    const float4 p = wpos[site];
+    pme_xyzq[s] = p;
+    if (pme_b) pme_b[s] = make_float4(p.x, p.y, p.z, site >= alch0 && site < alch1 ? -p.w : p.w);
+  }
 }
 
 // Sum of the work-item force rows of slot k's cluster row (fixed point).
@@ -875,16 +923,19 @@ __global__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
   for (int c = 0; c < 3; ++c) out[(size_t)c * n_slots + k] = f[c];
 }
 
-// Fixed-order sums of the partials.  out[0..7] from the pair kernel's
+// Fixed-order sums of the partials.  out[0..8] from the pair kernel's
 // per-item partials (× ½: the full list visits every pair twice): e_lj_w,
-// e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c; out[8..13] from the
-// correction partials: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, 0;
-// out[14..15]: SPME energy and virial (0 without SPME).
+// e_lj_s, e_c_w, e_c_s, virial, n_ss, n_lj, n_c, dH/dλ; out[9..14] from the
+// correction partials: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, dH/dλ;
+// out[16..17] and out[18..19] hold the SPME energy and virial of the first
+// and second SPME evaluation (written by SPME itself); those not evaluated
+// are cleared here.
 constexpr int kRedBlock = 1024;
+constexpr int kEngOut = 20;
 __global__ void __launch_bounds__(kRedBlock)
 k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
                     uint64_t stride, const double* __restrict__ cp, uint32_t n_cblk,
-                    const double* __restrict__ pme_res, double* __restrict__ out) {
+                    int pme_runs, double* __restrict__ out) {
   constexpr int kVals = kNumPart + 6;
   __shared__ double red[kVals][kRedBlock / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
@@ -906,8 +957,8 @@ k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ 
     for (int k = 0; k < kRedBlock / kWave; ++k) x += red[t][k];
     out[t] = t < kNumPart ? 0.5 * x : x;
   }
-  if (t == kVals) out[14] = pme_res ? pme_res[0] : 0.0;
-  if (t == kVals + 1) out[15] = pme_res ? pme_res[1] : 0.0;
+  if (t == kVals) out[15] = 0.0;
+  if (t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
 }
 
 __global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
@@ -943,7 +994,14 @@ struct dyn_engine {
   DevBuf<float> q;
   DevBuf<int32_t> ltype;
   DevBuf<int32_t> wmol;
-  DevBuf<float2> lj_table;
+  DevBuf<float2> lj_table;   // (c6, c12) per type pair
+  DevBuf<float2> lj_se;      // (σ, ε) per type pair (soft-core)
+  std::vector<int32_t> h_type;  // per-site type words (host copy)
+  // decoupled molecule: solute sites [alch0, alch1), λ
+  uint32_t alch0 = 0, alch1 = 0;
+  double lambda = 0.0;
+  bool last_recip_b = false;  // the last evaluation ran SPME twice
+  DynSchedule last_sch{};
   DevBuf<uint32_t> ex_ptr, ex_idx;
   DevBuf<uint32_t> corr_i, corr_j;
   DevBuf<uint8_t> corr_kind;
@@ -984,7 +1042,7 @@ struct dyn_engine {
   // PME
   bool pme_on = false;
   PmeCore pme;
-  DevBuf<float4> pme_xyzq;
+  DevBuf<float4> pme_xyzq, pme_b;  // charge sets of the two SPME evaluations
   // host copies
   PinnedBuf<int64_t> h_fx;
   PinnedBuf<double> h_eng;
@@ -1218,13 +1276,23 @@ void dyn_engine::enqueue(uint32_t flags) {
   P.c6_oo = c6_oo;
   P.c12_oo = c12_oo;
   P.n_types = n_types;
+  const bool alch_on = alch1 > alch0;
+  P.alch_on = alch_on ? 1 : 0;
+  P.sch = dyn_schedule((float)lambda);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
   const bool corr_on = (flags & DYN_COMPUTE_CORR) && n_corr;
+  // with a decoupled molecule the SPME sum runs twice, with its charges as
+  // they are (A) and negated (B): ((1+s)A + (1−s)B)/2 keeps the
+  // environment and molecule self terms and scales their cross term by s
+  const bool recip_b = recip_on && alch_on;
+  last_recip_b = recip_b;
+  last_sch = P.sch;
   {
     Profiler::Scope ps(prof, "site_prep");
     hipLaunchKernelGGL(k_eng_site_prep, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, wpos.p,
-                       n_sites, qsite.p, n_q, recip_on ? pme_xyzq.p : (float4*)nullptr, fx_site.p);
+                       n_sites, qsite.p, n_q, recip_on ? pme_xyzq.p : (float4*)nullptr,
+                       recip_b ? pme_b.p : (float4*)nullptr, alch0, alch1, fx_site.p);
     DYN_CHECK_LAUNCH("k_eng_site_prep");
   }
   if (short_on) {
@@ -1241,6 +1309,7 @@ void dyn_engine::enqueue(uint32_t flags) {
     A.chunks = chunks.p;
     A.n_chunks = counters.p + 2;
     A.lj_table = lj_table.p;
+    A.lj_se = lj_se.p;
     A.fitem = fitem.p;
     A.partials = pair_part.p;
     A.cap_chunks = cap_chunks;
@@ -1257,6 +1326,7 @@ void dyn_engine::enqueue(uint32_t flags) {
     C.wpos = wpos.p;
     C.ltype = ltype.p;
     C.lj_table = lj_table.p;
+    C.lj_se = lj_se.p;
     C.ci = corr_i.p;
     C.cj = corr_j.p;
     C.kind = corr_kind.p;
@@ -1269,7 +1339,17 @@ void dyn_engine::enqueue(uint32_t flags) {
     hipLaunchKernelGGL(k_eng_corr, dim3(corr_blocks), dim3(kCorrBlock), 0, stream, C);
     DYN_CHECK_LAUNCH("k_eng_corr");
   }
-  if (recip_on) pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr);
+  if (recip_on) {
+    if (!recip_b) {
+      pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 1.0f, eng_out.p + 16);
+    } else {
+      const float s = P.sch.coulomb_scale;
+      pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 0.5f * (1.0f + s),
+              eng_out.p + 16);
+      pme.run(n_q, pme_b.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 0.5f * (1.0f - s),
+              eng_out.p + 18);
+    }
+  }
   {
     Profiler::Scope ps(prof, "finalize");
     hipLaunchKernelGGL(k_eng_finalize, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
@@ -1279,8 +1359,7 @@ void dyn_engine::enqueue(uint32_t flags) {
     hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
                        short_on ? pair_part.p : (const double*)nullptr, counters.p + 2, cap_chunks,
                        corr_part.p,
-                       corr_on ? corr_blocks : 0u, recip_on ? pme.res_dev.p : (const double*)nullptr,
-                       eng_out.p);
+                       corr_on ? corr_blocks : 0u, recip_on ? (recip_b ? 2 : 1) : 0, eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
   computed = true;
@@ -1371,7 +1450,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     const float o_sigma = 1.777167268f * (2.0f / 1.122462048309373f);
     sig[E->type_o] = o_sigma;
     eps[E->type_o] = 0.2128008130f;
-    std::vector<float2> tab((size_t)nt * nt);
+    std::vector<float2> tab((size_t)nt * nt), tse((size_t)nt * nt);
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) {
         // Lorentz–Berthelot in f32 (src/non_bonded.rs:1081-1086), then
@@ -1380,6 +1459,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
         const float e_ab = std::sqrt(eps[a] * eps[b]);
         const double s6 = std::pow((double)s_ab, 6.0);
         tab[(size_t)a * nt + b] = make_float2((float)(4.0 * e_ab * s6), (float)(4.0 * e_ab * s6 * s6));
+        tse[(size_t)a * nt + b] = make_float2(s_ab, e_ab);
       }
     E->c6_oo = tab[(size_t)E->type_o * nt + E->type_o].x;
     E->c12_oo = tab[(size_t)E->type_o * nt + E->type_o].y;
@@ -1466,6 +1546,8 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     up(E->ltype, ht);
     up(E->wmol, hw);
     up(E->lj_table, tab);
+    up(E->lj_se, tse);
+    E->h_type = ht;
     up(E->ex_ptr, hptr);
     up(E->ex_idx, hidx);
     up(E->corr_i, ci);
@@ -1483,13 +1565,14 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->fx_site.reserve(3 * (size_t)ns);
     E->fx_out.reserve(3 * (size_t)ns);
     E->h_fx.reserve(3 * (size_t)ns);
-    E->eng_out.reserve(16);
-    E->h_eng.reserve(16);
+    E->eng_out.reserve(kEngOut);
+    E->h_eng.reserve(kEngOut);
     E->h_misc.reserve(8);
     E->counters.reserve(8);
     E->status.reserve(1);
     E->disp_bits.reserve(1);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
+    E->pme_b.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
     DYN_HIP(hipMemsetAsync(E->counters.p, 0, 32, E->stream));
     DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
@@ -1588,6 +1671,33 @@ extern "C" int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyz
   });
 }
 
+extern "C" int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32_t count,
+                                         double lambda) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DeviceGuard g(eng->device);
+    DYN_REQUIRE(std::isfinite(lambda), "lambda must be finite");
+    DYN_REQUIRE((uint64_t)first + count <= eng->n_dyn, "alchemical atoms [%u, %u) beyond the %u "
+                "solute atoms", first, first + count, eng->n_dyn);
+    DYN_REQUIRE(count == 0 || eng->prm.mode == DYN_MODE_REFERENCE,
+                "alchemical decoupling needs DYN_MODE_REFERENCE");
+    const uint32_t a0 = count ? first : 0, a1 = count ? first + count : 0;
+    if (a0 != eng->alch0 || a1 != eng->alch1) {
+      for (uint32_t i = 0; i < eng->n_dyn; ++i) {
+        int32_t& t = eng->h_type[i];
+        t = (t & DYN_TYPE_MASK) | (i >= a0 && i < a1 ? DYN_TYPE_ALCH : 0);
+      }
+      DYN_HIP(hipMemcpyAsync(eng->ltype.p, eng->h_type.data(), sizeof(int32_t) * eng->n_dyn,
+                             hipMemcpyHostToDevice, eng->stream));
+      DYN_HIP(hipStreamSynchronize(eng->stream));
+      eng->alch0 = a0;
+      eng->alch1 = a1;
+      eng->list_valid = false;  // slot types are refreshed by a rebuild
+    }
+    eng->lambda = lambda;
+  });
+}
+
 extern "C" int dyn_engine_compute(dyn_engine* eng, uint32_t flags) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
@@ -1671,28 +1781,38 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     DYN_REQUIRE(out, "out is null");
     if (!eng->computed) throw Error(DYN_ERR_STATE, "no compute yet");
     eng->check_status();
-    DYN_HIP(hipMemcpyAsync(eng->h_eng.p, eng->eng_out.p, 16 * sizeof(double),
+    DYN_HIP(hipMemcpyAsync(eng->h_eng.p, eng->eng_out.p, kEngOut * sizeof(double),
                            hipMemcpyDeviceToHost, eng->stream));
     DYN_HIP(hipStreamSynchronize(eng->stream));
     const double* v = eng->h_eng.p;
-    // v[0..7]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial_pairs, pair counts by kind
-    // v[8..12]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr; v[14..15]: pme energy, virial
+    // v[0..8]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial_pairs, pair counts by
+    // kind, dH/dλ; v[9..14]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, dH/dλ;
+    // v[16..17], v[18..19]: energy and virial of the SPME evaluations
+    double e_rec = v[16], w_rec = v[17], dh_rec = 0.0;
+    if (eng->last_recip_b) {
+      // A = E(q), B = E(q with the molecule's charges negated): the cross
+      // term is (A − B)/2, the rest (A + B)/2 (src/non_bonded.rs:709-742)
+      const double sc = eng->last_sch.coulomb_scale;
+      e_rec = 0.5 * (1.0 + sc) * v[16] + 0.5 * (1.0 - sc) * v[18];
+      w_rec = 0.5 * (1.0 + sc) * v[17] + 0.5 * (1.0 - sc) * v[19];
+      dh_rec = (double)eng->last_sch.coulomb_dscale_dlambda * 0.5 * (v[16] - v[18]);
+    }
     dyn_energies E{};
-    E.lj = v[0] + v[1] + v[8];
-    E.coul_sr = v[2] + v[3] + v[9];
-    E.recip = eng->pme_on ? v[14] : 0.0;
+    E.lj = v[0] + v[1] + v[9];
+    E.coul_sr = v[2] + v[3] + v[10];
+    E.recip = eng->pme_on ? e_rec : 0.0;
     double self = eng->pme_on ? eng->e_self : 0.0;
     if (eng->pme_on && eng->prm.mode == DYN_MODE_STANDARD) {
       const double V = (double)eng->box[0] * eng->box[1] * eng->box[2];
       self += -M_PI * eng->q_total * eng->q_total / (2.0 * V * (double)eng->prm.beta * eng->prm.beta);
     }
     E.self = self;
-    E.excl_corr = v[10];
+    E.excl_corr = v[11];
     E.total = E.lj + E.coul_sr + E.recip + E.self + E.excl_corr;
-    E.reference = v[1] + v[3] + v[8] + v[9] + E.recip + E.self;
-    E.virial_sr = v[4] + v[11];
-    E.virial_recip = (eng->pme_on ? v[15] : 0.0) + v[12];
-    E.dh_dl = 0.0;
+    E.reference = v[1] + v[3] + v[9] + v[10] + E.recip + E.self;
+    E.virial_sr = v[4] + v[12];
+    E.virial_recip = (eng->pme_on ? w_rec : 0.0) + v[13];
+    E.dh_dl = v[8] + v[14] + dh_rec;
     E.pair_count_kind[0] = (uint64_t)std::llround(v[5]);
     E.pair_count_kind[1] = (uint64_t)std::llround(v[6]);
     E.pair_count_kind[2] = (uint64_t)std::llround(v[7]);

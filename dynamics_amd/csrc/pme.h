@@ -64,9 +64,12 @@ struct PmeCore {
   // If site != null, forces are atomically added (fixed point, 2^32) into
   // fx_sites[3·site[i] + c] (component-interleaved); else written as floats
   // into f_out[3i + c].  Energy and virial (without self energy) end up in
-  // res_dev[0], res_dev[1] after the final reduction kernel.
+  // res_dev[0], res_dev[1] after the final reduction kernel (or in res[0],
+  // res[1] when res is given).  Forces are multiplied by fscale (exactly 1
+  // leaves them unchanged).
   void run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
-           int fx_component_major, uint32_t fx_nsites, float* f_out);
+           int fx_component_major, uint32_t fx_nsites, float* f_out, float fscale = 1.0f,
+           double* res = nullptr);
 };
 
 }  // namespace dyn

@@ -370,7 +370,7 @@ template <int ORDER>
 __global__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
                              const float2* __restrict__ phi, const uint32_t* __restrict__ site,
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
-                             float* __restrict__ fout) {
+                             float* __restrict__ fout, float fscale) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = xyzq[i];
@@ -402,9 +402,9 @@ __global__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, Spline
         }
       }
     }
-    f0 = -p.w * S.sc[0] * sx;
-    f1 = -p.w * S.sc[1] * sy;
-    f2 = -p.w * S.sc[2] * sz;
+    f0 = -p.w * S.sc[0] * sx * fscale;
+    f1 = -p.w * S.sc[1] * sy * fscale;
+    f2 = -p.w * S.sc[2] * sz * fscale;
   }
   if (site) {
     const uint32_t s = site[i];
@@ -548,7 +548,7 @@ void PmeCore::set_box(const float l[3]) {
 }
 
 void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
-                  int comp_major, uint32_t nsites, float* f_out) {
+                  int comp_major, uint32_t nsites, float* f_out, float fscale, double* res) {
   SplineArgs S;
   for (int a = 0; a < 3; ++a) {
     S.K[a] = (int)K[a];
@@ -608,17 +608,17 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   {
     Profiler::Scope ps(*prof, "pme_finalize");
     hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(256), 0, stream, partials.p,
-                       (int)conv_blocks, res_dev.p);
+                       (int)conv_blocks, res ? res : res_dev.p);
     DYN_CHECK_LAUNCH("k_pme_finalize");
   }
   if (n) {
     Profiler::Scope ps(*prof, "pme_gather");
     if (order == 4)
       hipLaunchKernelGGL(k_pme_gather<4>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out);
+                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out, fscale);
     else
       hipLaunchKernelGGL(k_pme_gather<5>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out);
+                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out, fscale);
     DYN_CHECK_LAUNCH("k_pme_gather");
   }
 }

@@ -165,7 +165,8 @@ class System:
         (molecule-major).  Returns (pos (n,3) f32, q (n,) f32, sigma, eps,
         water_mol (n,) int32 with -1 for solute, site code (n,) uint8)."""
         nd, nw = self.n_dyn, self.n_water
-        pos = np.empty((nd + 4 * nw, 3), F32)
+        # f32 unless a caller stored f64 positions (finite-difference checks)
+        pos = np.empty((nd + 4 * nw, 3), np.result_type(F32, self.pos_dyn.dtype, self.pos_o.dtype))
         pos[:nd] = self.pos_dyn
         w = np.stack([self.pos_o, self.pos_m, self.pos_h0, self.pos_h1], axis=1)
         pos[nd:] = w.reshape(-1, 3)

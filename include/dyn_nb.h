@@ -216,6 +216,18 @@ int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device);
 #define DYN_COMPUTE_REBUILD 8u   /* force a cluster-pair list rebuild      */
 #define DYN_COMPUTE_ASYNC 16u    /* enqueue only; dyn_engine_sync waits    */
 
+/* Alchemical decoupling of one molecule, as the reference's
+ * nonbonded_force_alchemical_kernel (src/cuda/cuda.cu:367-566) and its
+ * SPME cross-term scaling (src/non_bonded.rs:709-742) with the staged
+ * schedule (src/alchemical.rs:82-100).  Solute atoms [first, first + count)
+ * form the decoupled molecule (the reference decouples one whole molecule,
+ * src/alchemical.rs:103-108; pairs with exactly one atom in it are the
+ * alchemical ones, src/non_bonded.rs:581-584, :611-612); count = 0 turns
+ * decoupling off.  lambda is clamped to [0, 1].  A changed atom range
+ * rebuilds the list at the next compute.  DYN_MODE_REFERENCE only.
+ * dyn_energies.dh_dl returns dH/dλ. */
+int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32_t count, double lambda);
+
 /* One nonbonded evaluation.  Rebuilds the cluster-pair list when asked, on
  * the first call, after set_box, or when a site moved more than
  * list_buffer/2 since the last build. */

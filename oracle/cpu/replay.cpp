@@ -118,7 +118,7 @@ void orc_ref_pairs(const float* pos, uint32_t n_dyn, uint32_t n_water, uint32_t 
 template <int KIND>
 static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
                  uint32_t n_slots, int type_none, uint32_t I, const uint32_t* jent, uint64_t mask,
-                 const DynPairParams& P, const float2* lj, float fi[64][3]) {
+                 const DynPairParams& P, const float2* lj, const float2* lj_se, float fi[64][3]) {
   for (int l = 0; l < 64; ++l) {
     const int ia = l & 7, jb = l >> 3;
     const uint32_t js = jent[jb] & DYN_JSLOT_MASK;
@@ -133,9 +133,14 @@ static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slo
     const int gj = null_j ? DYN_GID_PAD : slot_gid[js];
     bool active = dyn_slot_pair_valid(slot_gid[I * DYN_CL_SIZE + ia], gj);
     if (KIND == DYN_KIND_SS) active = active && ((mask >> l) & 1ull);
-    float d[3], s, elj, ec, r2;
+    float d[3], s, elj, ec, r2, dh;
     int hit;
-    dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj, d, &s, &elj, &ec, &r2, &hit);
+    // the alchemical variant equals the plain one on every lane that does
+    // not cross the decoupled molecule's boundary
+    if (P.alch_on && KIND <= DYN_KIND_SQ)
+      dyn_tile_pair_alch<KIND>(pi, ti, pj, tj, active, P, lj, lj_se, d, &s, &elj, &ec, &r2, &hit, &dh);
+    else
+      dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj, d, &s, &elj, &ec, &r2, &hit);
     fi[l][0] += s * d[0];
     fi[l][1] += s * d[1];
     fi[l][2] += s * d[2];
@@ -149,9 +154,9 @@ extern "C" {
 void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
                       const uint8_t* cl_class, const uint32_t* row_ptr, const uint32_t* cj,
                       const uint64_t* cmask, const uint32_t* items, uint64_t n_items,
-                      uint32_t n_slots, const float* lj_table, int n_types, const float* box,
-                      float cutoff, float lj_cutoff, float beta, float c6_oo, float c12_oo,
-                      int64_t* fx) {
+                      uint32_t n_slots, const float* lj_table, const float* lj_se_table,
+                      int n_types, const float* box, float cutoff, float lj_cutoff, float beta,
+                      float c6_oo, float c12_oo, int alch_on, float lambda, int64_t* fx) {
   DynPairParams P;
   for (int k = 0; k < 3; ++k) {
     P.ext[k] = box[k];
@@ -165,7 +170,10 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
   P.c6_oo = c6_oo;
   P.c12_oo = c12_oo;
   P.n_types = n_types;
+  P.alch_on = alch_on;
+  P.sch = dyn_schedule(lambda);
   const float2* lj = reinterpret_cast<const float2*>(lj_table);
+  const float2* lse = reinterpret_cast<const float2*>(lj_se_table);
   memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
   static float fi[64][3];
   for (uint64_t it = 0; it < n_items; ++it) {
@@ -178,11 +186,11 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
       const uint64_t mask = cmask[g];
       const int tn = n_types - 1;
       switch (dyn_kind(ci, (int)(je[0] >> DYN_JCLS_SHIFT))) {
-        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
-        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
-        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
-        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
-        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, fi); break;
+        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
+        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
+        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
+        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
+        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
         default: break;
       }
     }

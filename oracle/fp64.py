@@ -78,9 +78,22 @@ def erf_pair_fp64(r, qq, beta):
     return e, f
 
 
-def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True):
+def schedule(lam):
+    """Staged decoupling (src/alchemical.rs:82-100): Coulomb is switched off
+    over λ ∈ [0, ½], then LJ over [½, 1].  Returns (coulomb_scale,
+    d coulomb_scale/dλ, LJ λ, d LJ λ/dλ)."""
+    lam = min(1.0, max(0.0, float(lam)))
+    if lam < 0.5:
+        return 1.0 - 2.0 * lam, -2.0, 0.0, 0.0
+    return 0.0, 0.0, 2.0 * lam - 1.0, 2.0
+
+
+def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True, alch=None, lam=0.0):
     """Forces (n_sites, 3) in site order (solute, then per water O, M, H0, H1)
-    and energies of all pairs inside the cutoff."""
+    and energies of all pairs inside the cutoff.  alch (bool per solute atom)
+    marks a decoupled molecule: pairs with exactly one atom in it take
+    soft-core LJ and scaled Coulomb at λ = lam (src/non_bonded.rs:300-345,
+    :883-941), and "dh_dl" is returned."""
     lj_rc = rc if lj_rc is None else lj_rc
     pos, q, sig, eps, mol, code = site_view(system)
     box = system.box.astype(F64)
@@ -122,12 +135,27 @@ def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True):
     e_c, f_c = coul_erfc_fp64(r, qq, beta)
     e_c = np.where(c_on, e_c, 0.0)
     f_c = np.where(c_on, f_c, 0.0)
+    dh = np.zeros_like(r)
+    if alch is not None and np.any(alch):
+        a_site = np.zeros(pos.shape[0], bool)
+        a_site[:nd] = np.asarray(alch, bool)
+        cross = a_site[i] ^ a_site[j]
+        cs, cds, ll, lld = schedule(lam)
+        e_sc, f_sc, dh_sc = soft_core_fp64(r, s_ab, e_ab, ll)
+        e_lj = np.where(cross & lj_on, e_sc, e_lj)
+        f_lj = np.where(cross & lj_on, f_sc, f_lj)
+        dh = np.where(cross & lj_on, dh_sc * lld, 0.0)
     s_lj = np.where(in_14, 0.5, 1.0)
     s_c = np.where(in_14, 1.0 / 1.2, 1.0)
     e_lj *= s_lj
     f_lj *= s_lj
+    dh *= s_lj
     e_c *= s_c
     f_c *= s_c
+    if alch is not None and np.any(alch):
+        dh = dh + np.where(cross, e_c * cds, 0.0)
+        e_c = np.where(cross, e_c * cs, e_c)
+        f_c = np.where(cross, f_c * cs, f_c)
     fmag = f_lj + f_c
     fvec = d * (fmag / r)[:, None]
     n = pos.shape[0]
@@ -143,6 +171,7 @@ def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True):
         "virial": float((fmag * r).sum()),
         "pair_count": int(interacting.sum()),
         "pair_count_lj": int(lj_on.sum()), "pair_count_coul": int(c_on.sum()),
+        "dh_dl": float(dh.sum()),
     }
 
 
@@ -387,20 +416,40 @@ def site_charges(system):
     return q
 
 
-def reference_energy_and_forces(system, rc, lj_rc, beta, K, order, mode="reference"):
+def reference_energy_and_forces(system, rc, lj_rc, beta, K, order, mode="reference", alch=None,
+                                lam=0.0):
     """Whole nonbonded evaluation in FP64: short range + corrections + SPME,
-    forces in site order, the engine's energy breakdown."""
-    sr = pair_sum_fp64(system, rc, lj_rc, beta)
+    forces in site order, the engine's energy breakdown.  With a decoupled
+    molecule (alch, bool per solute atom) the SPME sum is evaluated three
+    times, as the reference does (src/non_bonded.rs:709-742): all charges,
+    the environment only, the molecule only; the cross term is scaled by the
+    schedule's Coulomb factor and contributes its λ-derivative to dh_dl."""
+    sr = pair_sum_fp64(system, rc, lj_rc, beta, alch=alch, lam=lam)
     pos, q, _, _, mol, _ = site_view(system)
     box = system.box.astype(F64)
     F = sr["forces"].copy()
     out = {"e_lj": sr["e_lj"], "e_coul": sr["e_coul"], "virial_sr": sr["virial"],
-           "pair_count": sr["pair_count"]}
+           "pair_count": sr["pair_count"], "dh_dl": sr["dh_dl"]}
     e_rec = w_rec = 0.0
     e_self = 0.0
     if K is not None:
         qi = np.nonzero(q != 0)[0]
-        f_r, e_rec, w_rec = spme_fp64(pos[qi], q[qi], box, K, order, beta)
+        if alch is not None and np.any(alch):
+            a_site = np.zeros(pos.shape[0], bool)
+            a_site[:system.n_dyn] = np.asarray(alch, bool)
+            q_env = np.where(a_site, 0.0, q)
+            q_mol = np.where(a_site, q, 0.0)
+            f_all, e_all, w_all = spme_fp64(pos[qi], q[qi], box, K, order, beta)
+            f_env, e_env, w_env = spme_fp64(pos[qi], q_env[qi], box, K, order, beta)
+            f_mol, e_mol, w_mol = spme_fp64(pos[qi], q_mol[qi], box, K, order, beta)
+            cs, cds, _, _ = schedule(lam)
+            f_r = f_env + f_mol + cs * (f_all - f_env - f_mol)
+            e_cross = e_all - e_env - e_mol
+            e_rec = e_env + e_mol + cs * e_cross
+            w_rec = w_env + w_mol + cs * (w_all - w_env - w_mol)
+            out["dh_dl"] += cds * e_cross
+        else:
+            f_r, e_rec, w_rec = spme_fp64(pos[qi], q[qi], box, K, order, beta)
         F[qi] += f_r
         e_self = self_energy(q, beta)
     w_corr = 0.0

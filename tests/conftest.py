@@ -106,17 +106,32 @@ def engine_lj_table(system):
     return tab
 
 
-def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta):
+def engine_lj_se(system):
+    """(σ, ε) per type pair as the engine combines them for soft-core:
+    Lorentz–Berthelot in f32 over the solute types, water O and "none"."""
+    lt = np.asarray(system.lj_types, np.float32).reshape(-1, 2)
+    f = np.float32
+    o_sigma = f(1.777167268) * (f(2.0) / f(1.122462048309373))  # as the engine evaluates it
+    sig = np.concatenate([lt[:, 0], [o_sigma], [0.0]]).astype(np.float32)
+    eps = np.concatenate([lt[:, 1], [np.float32(0.2128008130)], [0.0]]).astype(np.float32)
+    s_ab = (np.float32(0.5) * (sig[:, None] + sig[None, :])).astype(np.float32)
+    e_ab = np.sqrt((eps[:, None] * eps[None, :]).astype(np.float32)).astype(np.float32)
+    return np.ascontiguousarray(np.stack([s_ab, e_ab], -1).reshape(-1, 2), np.float32)
+
+
+def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta, alch=False, lam=0.0):
     L = oracle_lib()
     tab = engine_lj_table(system)
+    tse = engine_lj_se(system)
     nt = system.lj_types.shape[0] + 2
     o = system.lj_types.shape[0]
     c6_oo, c12_oo = tab[o * nt + o]
     ns = len(dbg["slot_site"])
     fx = np.zeros((3, ns), np.int64)
-    L.orc_engine_tiles.argtypes = [C.c_void_p] * 8 + [C.c_uint64, C.c_uint32, C.c_void_p, C.c_int,
-                                                      C.c_void_p, C.c_float, C.c_float, C.c_float,
-                                                      C.c_float, C.c_float, C.c_void_p]
+    L.orc_engine_tiles.argtypes = [C.c_void_p] * 8 + [C.c_uint64, C.c_uint32, C.c_void_p,
+                                                      C.c_void_p, C.c_int, C.c_void_p, C.c_float,
+                                                      C.c_float, C.c_float, C.c_float, C.c_float,
+                                                      C.c_int, C.c_float, C.c_void_p]
     box = np.ascontiguousarray(system.box, np.float32)
     cm = np.ascontiguousarray(dbg["cmask"].astype(np.uint64))
     items = np.ascontiguousarray(dbg["items"], np.uint32)
@@ -124,8 +139,8 @@ def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta):
                        ptr(dbg["cl_class"]), ptr(dbg["row_ptr"]),
                        ptr(dbg["cj"]) if len(dbg["cj"]) else None,
                        ptr(cm) if len(cm) else None, ptr(items) if len(items) else None,
-                       len(items), ns, ptr(tab), nt, ptr(box), cutoff, lj_cutoff, beta,
-                       float(c6_oo), float(c12_oo), ptr(fx))
+                       len(items), ns, ptr(tab), ptr(tse), nt, ptr(box), cutoff, lj_cutoff, beta,
+                       float(c6_oo), float(c12_oo), 1 if alch else 0, float(lam), ptr(fx))
     return fx
 
 
