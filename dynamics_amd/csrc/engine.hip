@@ -1624,11 +1624,8 @@ extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
     eng->set_geometry();
     if (eng->pme_on) {
       if (eng->pme.ngrid == 0)
-      {
         eng->pme.init(eng->device, eng->stream, &eng->prof, eng->prm.grid, eng->box,
                       eng->prm.beta, (int)eng->prm.order);
-        eng->pme.clear_in_fft = true;
-      }
       else
         eng->pme.set_box(eng->box);
     }

@@ -50,11 +50,13 @@ struct PmeCore {
   PinnedBuf<double> h_part;
   unsigned conv_blocks = 0;
   DevBuf<double> res_dev;       // [energy, virial] reduced on the device
-  // When set, the first FFT pass clears the fixed-point grid as it reads it
-  // (the grid then stays zero between runs and no separate clear is
-  // needed); when not set, the grid is cleared before each spread and keeps
-  // the spread charges after the run.
-  bool clear_in_fft = false;
+  // Spreading by grid brick (8³ points): charges are counted per home brick
+  // (the brick of their base grid point), sorted by it, and each brick
+  // gathers the charges of the home bricks whose stencils can reach it.
+  uint32_t nbk[3] = {0, 0, 0};  // bricks per axis
+  DevBuf<uint32_t> brick_count, brick_start;  // per brick (+1)
+  DevBuf<uint2> charge_brick;   // per charge: home brick, rank within it
+  DevBuf<uint32_t> brick_items; // charges sorted by home brick
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
             float beta, int order);

@@ -194,7 +194,7 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 // the int64 fixed-point charge grid (z pass only).
 template <int AXIS, int SIGN, bool FROM_FX>
 __global__ void __launch_bounds__(kFftBlock)
-k_fft_pass(float2* __restrict__ data, int64_t* __restrict__ grid_fx, int clear_fx, PassGeom g,
+k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
            FftAxis ax, const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
   const int N = ax.N;
@@ -208,12 +208,8 @@ k_fft_pass(float2* __restrict__ data, int64_t* __restrict__ grid_fx, int clear_f
     split_e<AXIS>(e, nl, N, l, n);
     const size_t addr = pass_addr<AXIS>(g, line0 + l, n);
     float2 v;
-    if (FROM_FX) {
-      v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
-      if (clear_fx) grid_fx[addr] = 0;
-    } else {
-      v = data[addr];
-    }
+    if (FROM_FX) v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
+    else v = data[addr];
     A[l * N + n] = v;
   }
   __syncthreads();
@@ -334,35 +330,154 @@ __device__ __forceinline__ void frac_coord(float x, float sc, int K, int& k0, fl
   k0 = k;
 }
 
-template <int ORDER>
-__global__ void k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
-                             int64_t* __restrict__ grid) {
+constexpr int kBrick = 8;          // grid points per brick edge
+constexpr int kBrickBlock = 256;
+
+__device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk[3]) {
+  return ((uint32_t)(k0[0] / kBrick) * nbk[1] + (uint32_t)(k0[1] / kBrick)) * nbk[2] +
+         (uint32_t)(k0[2] / kBrick);
+}
+
+// Per charge: its home brick (the brick of its base grid point) and a rank
+// in it.  The rank order depends on atomic timing; the brick sums are
+// integers, so the spread grid does not.
+__global__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
+                          uint32_t nbx, uint32_t nby, uint32_t nbz,
+                          uint32_t* __restrict__ count, uint2* __restrict__ cb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = xyzq[i];
-  if (p.w == 0.0f) return;
-  int k0[3];
-  float th[3][ORDER], dth[ORDER];
   const float xs[3] = {p.x, p.y, p.z};
+  int k0[3];
   for (int a = 0; a < 3; ++a) {
     float w;
     frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
-    dyn_bspline<ORDER>(w, th[a], dth);
   }
-  for (int jx = 0; jx < ORDER; ++jx) {
-    const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
-    const float qx = p.w * th[0][jx];
-    for (int jy = 0; jy < ORDER; ++jy) {
-      const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
-      const float qxy = qx * th[1][jy];
-      int64_t* row = grid + ((size_t)gx * S.K[1] + gy) * S.K[2];
-      for (int jz = 0; jz < ORDER; ++jz) {
-        const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
-        const float v = qxy * th[2][jz];
-        atomicAdd(reinterpret_cast<unsigned long long*>(row + gz),
-                  (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
+  const uint32_t nb[3] = {nbx, nby, nbz};
+  const uint32_t b = brick_of(k0, nb);
+  cb[i] = make_uint2(b, atomicAdd(&count[b], 1u));
+}
+
+// Single-workgroup exclusive scan of the brick counts into start[0..n]; the
+// counts are cleared for the next binning pass.
+constexpr int kBrickScan = 1024;
+__global__ void __launch_bounds__(kBrickScan)
+k_pme_brick_scan(uint32_t* __restrict__ count, uint32_t n, uint32_t* __restrict__ start) {
+  __shared__ uint32_t wsum[kBrickScan / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t / 64;
+  const uint32_t per = (n + kBrickScan - 1) / kBrickScan;
+  const uint32_t b0 = t * per, e0 = min(n, b0 + per);
+  uint32_t s = 0;
+  for (uint32_t i = b0; i < e0; ++i) s += count[i];
+  uint32_t incl = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int k = 0; k < kBrickScan / 64; ++k) {
+    if (k < w) base += wsum[k];
+    tot += wsum[k];
+  }
+  uint32_t run = base + incl - s;
+  for (uint32_t i = b0; i < e0; ++i) {
+    const uint32_t c = count[i];
+    start[i] = run;
+    run += c;
+    count[i] = 0;
+  }
+  if (t == 0) start[n] = tot;
+}
+
+__global__ void k_pme_brick_scatter(const uint2* __restrict__ cb, uint32_t n,
+                                    const uint32_t* __restrict__ start,
+                                    uint32_t* __restrict__ items) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint2 v = cb[i];
+  const uint32_t k = start[v.x] + v.y;
+  if (k < n) items[k] = i;
+}
+
+// Home bricks along one axis whose charges can reach the points [lo, hi]:
+// a charge with base point k0 covers k0 − ORDER + 1 … k0 (mod K), so k0 runs
+// over lo … hi + ORDER − 1 (mod K).  At most 3 distinct bricks.
+template <int ORDER>
+__device__ __forceinline__ int home_bricks(int lo, int hi, int K, int out[3]) {
+  int n = 0;
+  const int len = min(hi - lo + ORDER, K);
+  for (int d = 0; d < len; ++d) {
+    const int b = ((lo + d) % K) / kBrick;
+    bool seen = false;
+    for (int q = 0; q < n; ++q) seen = seen || out[q] == b;
+    if (!seen && n < 3) out[n++] = b;
+  }
+  return n;
+}
+
+// One workgroup per brick: every charge of the reaching home bricks adds its
+// stencil points inside the brick to an LDS brick (int64 LDS adds of the
+// same fixed-point values the replay sums), which is then stored once.
+template <int ORDER>
+__global__ void __launch_bounds__(kBrickBlock)
+k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t nbx,
+             uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ start,
+             const uint32_t* __restrict__ items, int64_t* __restrict__ grid) {
+  __shared__ unsigned long long acc[kBrick * kBrick * kBrick];
+  const uint32_t b = blockIdx.x;
+  const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), bx = (int)(b / (nbz * nby));
+  const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
+  const int hi[3] = {min(lo[0] + kBrick, S.K[0]) - 1, min(lo[1] + kBrick, S.K[1]) - 1,
+                     min(lo[2] + kBrick, S.K[2]) - 1};
+  for (int k = threadIdx.x; k < kBrick * kBrick * kBrick; k += kBrickBlock) acc[k] = 0ull;
+  int hb[3][3], nh[3];
+  for (int a = 0; a < 3; ++a) nh[a] = home_bricks<ORDER>(lo[a], hi[a], S.K[a], hb[a]);
+  __syncthreads();
+  for (int ix = 0; ix < nh[0]; ++ix)
+    for (int iy = 0; iy < nh[1]; ++iy)
+      for (int iz = 0; iz < nh[2]; ++iz) {
+        const uint32_t h = ((uint32_t)hb[0][ix] * nby + (uint32_t)hb[1][iy]) * nbz + (uint32_t)hb[2][iz];
+        const uint32_t c0 = min(start[h], n), c1 = min(start[h + 1], n);
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kBrickBlock) {
+          const uint32_t ci = items[c];
+          if (ci >= n) continue;
+          const float4 p = xyzq[ci];
+          if (p.w == 0.0f) continue;
+          int k0[3];
+          float th[3][ORDER], dth[ORDER];
+          const float xs[3] = {p.x, p.y, p.z};
+          for (int a = 0; a < 3; ++a) {
+            float w;
+            frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
+            dyn_bspline<ORDER>(w, th[a], dth);
+          }
+          for (int jx = 0; jx < ORDER; ++jx) {
+            const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
+            if (gx < lo[0] || gx > hi[0]) continue;
+            const float qx = p.w * th[0][jx];
+            for (int jy = 0; jy < ORDER; ++jy) {
+              const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
+              if (gy < lo[1] || gy > hi[1]) continue;
+              const float qxy = qx * th[1][jy];
+              for (int jz = 0; jz < ORDER; ++jz) {
+                const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
+                if (gz < lo[2] || gz > hi[2]) continue;
+                const float v = qxy * th[2][jz];
+                const int li = ((gx - lo[0]) * kBrick + (gy - lo[1])) * kBrick + (gz - lo[2]);
+                atomicAdd(&acc[li], (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
+              }
+            }
+          }
+        }
       }
-    }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kBrick * kBrick * kBrick; k += kBrickBlock) {
+    const int x = lo[0] + k / (kBrick * kBrick), y = lo[1] + (k / kBrick) % kBrick,
+              z = lo[2] + k % kBrick;
+    if (x <= hi[0] && y <= hi[1] && z <= hi[2])
+      grid[((size_t)x * S.K[1] + y) * S.K[2] + z] = (int64_t)acc[k];
   }
 }
 
@@ -459,7 +574,12 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
   beta = b;
   ngrid = (size_t)K[0] * K[1] * K[2];
   grid_fx.reserve(ngrid);
-  DYN_HIP(hipMemsetAsync(grid_fx.p, 0, sizeof(int64_t) * ngrid, stream));
+  for (int a = 0; a < 3; ++a) nbk[a] = (K[a] + kBrick - 1) / kBrick;
+  const size_t nb = (size_t)nbk[0] * nbk[1] * nbk[2];
+  brick_count.reserve(nb + 1);
+  brick_start.reserve(nb + 1);
+  // the binning pass counts into zeroed counts; the scan clears them again
+  DYN_HIP(hipMemsetAsync(brick_count.p, 0, sizeof(uint32_t) * (nb + 1), stream));
   cgrid.reserve(ngrid);
   G.reserve(ngrid);
   for (int a = 0; a < 3; ++a) {
@@ -554,18 +674,32 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
     S.K[a] = (int)K[a];
     S.sc[a] = (float)((double)K[a] / (double)L[a]);
   }
-  if (!clear_in_fft) {
-    Profiler::Scope ps(*prof, "pme_zero");
-    DYN_HIP(hipMemsetAsync(grid_fx.p, 0, sizeof(int64_t) * ngrid, stream));
-  }
-  if (n) {
+  // every grid point belongs to exactly one brick and is stored by it, so
+  // the grid needs no clearing
+  {
     Profiler::Scope ps(*prof, "pme_spread");
+    const uint32_t nb = nbk[0] * nbk[1] * nbk[2];
+    if (n) {
+      charge_brick.reserve(n);
+      brick_items.reserve(n);
+      hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(n, 256)), dim3(256), 0, stream, xyzq, n, S,
+                         nbk[0], nbk[1], nbk[2], brick_count.p, charge_brick.p);
+      DYN_CHECK_LAUNCH("k_pme_bin");
+    }
+    hipLaunchKernelGGL(k_pme_brick_scan, dim3(1), dim3(kBrickScan), 0, stream, brick_count.p, nb,
+                       brick_start.p);
+    DYN_CHECK_LAUNCH("k_pme_brick_scan");
+    if (n) {
+      hipLaunchKernelGGL(k_pme_brick_scatter, dim3(grid_for(n, 256)), dim3(256), 0, stream,
+                         charge_brick.p, n, brick_start.p, brick_items.p);
+      DYN_CHECK_LAUNCH("k_pme_brick_scatter");
+    }
     if (order == 4)
-      hipLaunchKernelGGL(k_pme_spread<4>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, grid_fx.p);
+      hipLaunchKernelGGL(k_pme_spread<4>, dim3(nb), dim3(kBrickBlock), 0, stream, xyzq, n, S,
+                         nbk[0], nbk[1], nbk[2], brick_start.p, brick_items.p, grid_fx.p);
     else
-      hipLaunchKernelGGL(k_pme_spread<5>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, grid_fx.p);
+      hipLaunchKernelGGL(k_pme_spread<5>, dim3(nb), dim3(kBrickBlock), 0, stream, xyzq, n, S,
+                         nbk[0], nbk[1], nbk[2], brick_start.p, brick_items.p, grid_fx.p);
     DYN_CHECK_LAUNCH("k_pme_spread");
   }
   const PassGeom gz = pass_geom(K, 2), gy = pass_geom(K, 1), gx = pass_geom(K, 0);
@@ -575,13 +709,13 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   {
     Profiler::Scope ps(*prof, "pme_fft_z_fwd");
     hipLaunchKernelGGL((k_fft_pass<2, -1, true>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, grid_fx.p, clear_in_fft ? 1 : 0, gz, ax[2], tw[2].p);
+                       stream, cgrid.p, grid_fx.p, gz, ax[2], tw[2].p);
     DYN_CHECK_LAUNCH("k_fft_pass z fwd");
   }
   {
     Profiler::Scope ps(*prof, "pme_fft_y_fwd");
     hipLaunchKernelGGL((k_fft_pass<1, -1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (int64_t*)nullptr, 0, gy, ax[1], tw[1].p);
+                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
     DYN_CHECK_LAUNCH("k_fft_pass y fwd");
   }
   {
@@ -596,13 +730,13 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   {
     Profiler::Scope ps(*prof, "pme_fft_y_inv");
     hipLaunchKernelGGL((k_fft_pass<1, 1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (int64_t*)nullptr, 0, gy, ax[1], tw[1].p);
+                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
     DYN_CHECK_LAUNCH("k_fft_pass y inv");
   }
   {
     Profiler::Scope ps(*prof, "pme_fft_z_inv");
     hipLaunchKernelGGL((k_fft_pass<2, 1, false>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, (int64_t*)nullptr, 0, gz, ax[2], tw[2].p);
+                       stream, cgrid.p, (const int64_t*)nullptr, gz, ax[2], tw[2].p);
     DYN_CHECK_LAUNCH("k_fft_pass z inv");
   }
   {
