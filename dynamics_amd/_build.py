@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdyn_nb.so")
-SOURCES = ["compat.hip", "pme.hip", "engine.hip"]
+SOURCES = ["compat.hip", "pme.hip", "engine.hip", "md.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -57,6 +57,17 @@ class NbOpts(C.Structure):
     _fields_ = [("lj_cutoff", C.c_float), ("energy_all_pairs", C.c_uint32)]
 
 
+class MdParams(C.Structure):
+    _fields_ = [("dt", C.c_float), ("temperature", C.c_float), ("gamma", C.c_float),
+                ("shake_tol", C.c_float), ("seed", C.c_uint64)]
+
+
+class MdEnergies(C.Structure):
+    _fields_ = [("kinetic", C.c_double), ("bond", C.c_double), ("angle", C.c_double),
+                ("dihedral", C.c_double), ("temperature", C.c_double), ("dof", C.c_double),
+                ("steps", C.c_uint64)]
+
+
 class EngineParams(C.Structure):
     _fields_ = [("cutoff", C.c_float), ("lj_cutoff", C.c_float), ("beta", C.c_float),
                 ("grid", C.c_uint32 * 3), ("order", C.c_uint32), ("mode", C.c_uint32),
@@ -118,6 +129,15 @@ _sig("dyn_engine_set_positions", C.c_int, [_p, _p, _p, _p, _p, _p])
 _sig("dyn_engine_set_positions_device", C.c_int, [_p, C.c_void_p])
 _sig("dyn_engine_compute", C.c_int, [_p, C.c_uint32])
 _sig("dyn_engine_set_alchemical", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_double])
+_sig("dyn_md_create", C.c_int, [C.POINTER(_p), _p, C.POINTER(MdParams), _p,
+                                C.c_uint32, _p, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p,
+                                C.c_uint32, _p, _p])
+_sig("dyn_md_destroy", None, [_p])
+_sig("dyn_md_last_error", C.c_char_p, [_p])
+_sig("dyn_md_set_state", C.c_int, [_p, _p, _p, _p, _p])
+_sig("dyn_md_get_state", C.c_int, [_p, _p, _p, _p, _p])
+_sig("dyn_md_step", C.c_int, [_p, C.c_uint32])
+_sig("dyn_md_get_energies", C.c_int, [_p, C.POINTER(MdEnergies)])
 _sig("dyn_engine_sync", C.c_int, [_p])
 _sig("dyn_engine_get_forces", C.c_int, [_p, _p, _p])
 _sig("dyn_engine_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
@@ -141,6 +161,8 @@ EXPORTED = [
     "dyn_engine_create", "dyn_engine_destroy", "dyn_engine_last_error", "dyn_engine_set_box",
     "dyn_engine_set_positions", "dyn_engine_set_positions_device", "dyn_engine_compute",
     "dyn_engine_set_alchemical",
+    "dyn_md_create", "dyn_md_destroy", "dyn_md_last_error", "dyn_md_set_state",
+    "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
     "dyn_engine_profile",
@@ -419,3 +441,59 @@ class Engine:
         n = C.c_uint32()
         _check(lib.dyn_engine_profile_read(self.h, mx, names, ms, cnt, C.byref(n)), self._err)
         return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, mx))}
+
+
+class Md:
+    """Device-resident MD around an Engine (dyn_md_*): positions, velocities
+    and forces stay on the GPU between steps."""
+
+    def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0):
+        self.engine = engine
+        self.h = C.c_void_p()
+        p = MdParams(dt, temperature, gamma, shake_tol, seed)
+        arrs = [_c(ff.mass, np.float32),
+                _c(ff.bonds, np.uint32), _c(ff.bond_k_r0, np.float32),
+                _c(ff.angles, np.uint32), _c(ff.angle_k_t0, np.float32),
+                _c(ff.dihedrals, np.uint32), _c(ff.dihe_k_n_phase, np.float32),
+                _c(ff.constraints, np.uint32), _c(ff.constraint_r0, np.float32)]
+        self._keep = arrs
+        rc = lib.dyn_md_create(C.byref(self.h), engine.h, C.byref(p), _ptr(arrs[0]),
+                               len(ff.bonds), _ptr(arrs[1]), _ptr(arrs[2]),
+                               len(ff.angles), _ptr(arrs[3]), _ptr(arrs[4]),
+                               len(ff.dihedrals), _ptr(arrs[5]), _ptr(arrs[6]),
+                               len(ff.constraints), _ptr(arrs[7]), _ptr(arrs[8]))
+        _check(rc, lambda: lib.dyn_global_last_error().decode())
+        self.n_dyn, self.n_water = engine.n_dyn, engine.n_water
+
+    def close(self):
+        if self.h:
+            lib.dyn_md_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _err(self):
+        return lib.dyn_md_last_error(self.h).decode()
+
+    def set_state(self, pos_dyn, vel_dyn, pos_water, vel_water):
+        """pos_water (n_water, 4, 3) in O, M, H0, H1 order; vel_water
+        (n_water, 3, 3) in O, H0, H1 order."""
+        a = [_c(pos_dyn, np.float32), _c(vel_dyn, np.float32), _c(pos_water, np.float32),
+             _c(vel_water, np.float32)]
+        _check(lib.dyn_md_set_state(self.h, *[_ptr(x) for x in a]), self._err)
+
+    def state(self):
+        pd = np.zeros((self.n_dyn, 3), np.float32)
+        vd = np.zeros((self.n_dyn, 3), np.float32)
+        pw = np.zeros((self.n_water, 4, 3), np.float32)
+        vw = np.zeros((self.n_water, 3, 3), np.float32)
+        _check(lib.dyn_md_get_state(self.h, _ptr(pd), _ptr(vd), _ptr(pw), _ptr(vw)), self._err)
+        return pd, vd, pw, vw
+
+    def step(self, n=1):
+        _check(lib.dyn_md_step(self.h, int(n)), self._err)
+
+    def energies(self):
+        e = MdEnergies()
+        _check(lib.dyn_md_get_energies(self.h, C.byref(e)), self._err)
+        return {k: getattr(e, k) for k, _ in MdEnergies._fields_}

@@ -36,6 +36,7 @@
 #include "common.h"
 #include "dyn_math.h"
 #include "engine.h"
+#include "engine_internal.h"
 #include "pme.h"
 
 #pragma clang fp contract(off)
@@ -1039,6 +1040,15 @@ struct dyn_engine {
   DevBuf<double> pair_part, corr_part, eng_out;
   unsigned corr_blocks = 0;
   DevBuf<unsigned int> disp_bits;
+  // displacement readings of asynchronous computes (squared Å, bit pattern
+  // of a non-negative float), consumed when their event has completed
+  static constexpr int kDispRing = 4;
+  hipEvent_t disp_ev[kDispRing] = {};
+  PinnedBuf<float> h_disp;
+  uint64_t disp_gen[kDispRing] = {};
+  bool disp_pending[kDispRing] = {};
+  int disp_next = 0;
+  float disp_seen = 0.f;  // largest reading of the current list so far
   // PME
   bool pme_on = false;
   PmeCore pme;
@@ -1050,6 +1060,8 @@ struct dyn_engine {
   bool fx_fresh = false;
 
   ~dyn_engine() {
+    for (auto& e : disp_ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1235,12 +1247,27 @@ void dyn_engine::build_list() {
   }
   list_valid = true;
   ++n_builds;
+  disp_seen = 0.f;
 }
 
 void dyn_engine::enqueue(uint32_t flags) {
   DYN_REQUIRE(box_set, "box not set");
   if (!pos_set) throw Error(DYN_ERR_STATE, "positions not set");
   bool rebuild = (flags & DYN_COMPUTE_REBUILD) || !list_valid || prm.list_buffer <= 0.f;
+  const bool async = (flags & DYN_COMPUTE_ASYNC) != 0;
+  if (!rebuild && async) {
+    // Asynchronous computes never wait for the displacement: they use the
+    // newest readings of earlier computes on this list (a few steps old) and
+    // rebuild at 0.8 × buffer/2, which leaves the rest of the half buffer
+    // for the motion since those readings.
+    for (int k = 0; k < kDispRing; ++k)
+      if (disp_pending[k] && hipEventQuery(disp_ev[k]) == hipSuccess) {
+        disp_pending[k] = false;
+        if (disp_gen[k] == n_builds) disp_seen = std::max(disp_seen, h_disp.p[k]);
+      }
+    const float lag_half = 0.4f * prm.list_buffer;
+    if (!(disp_seen < lag_half * lag_half)) rebuild = true;
+  }
   if (!rebuild) {
     hipLaunchKernelGGL(k_eng_wrap, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, pos_dev,
                        q.p, n_sites, geom, wpos.p);
@@ -1248,12 +1275,23 @@ void dyn_engine::enqueue(uint32_t flags) {
     DYN_HIP(hipMemsetAsync(disp_bits.p, 0, 4, stream));
     hipLaunchKernelGGL(k_eng_max_disp, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
                        wpos.p, wpos_ref.p, n_sites, geom, disp_bits.p);
-    DYN_HIP(hipMemcpyAsync(h_misc.p + 6, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
-    DYN_HIP(hipStreamSynchronize(stream));
-    float d2;
-    memcpy(&d2, h_misc.p + 6, 4);
-    const float half = 0.5f * prm.list_buffer;
-    if (!(d2 < half * half)) rebuild = true;
+    if (async) {
+      const int k = disp_next;
+      disp_next = (disp_next + 1) % kDispRing;
+      if (!disp_pending[k]) {  // a slot still in flight is skipped, not overwritten
+        DYN_HIP(hipMemcpyAsync(h_disp.p + k, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
+        DYN_HIP(hipEventRecord(disp_ev[k], stream));
+        disp_pending[k] = true;
+        disp_gen[k] = n_builds;
+      }
+    } else {
+      DYN_HIP(hipMemcpyAsync(h_misc.p + 6, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
+      DYN_HIP(hipStreamSynchronize(stream));
+      float d2;
+      memcpy(&d2, h_misc.p + 6, 4);
+      const float half = 0.5f * prm.list_buffer;
+      if (!(d2 < half * half)) rebuild = true;
+    }
   }
   if (rebuild) {
     build_list();
@@ -1571,6 +1609,8 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->counters.reserve(8);
     E->status.reserve(1);
     E->disp_bits.reserve(1);
+    E->h_disp.reserve(dyn_engine::kDispRing);
+    for (auto& e : E->disp_ev) DYN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_b.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
@@ -1886,3 +1926,18 @@ extern "C" int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char
     }
   });
 }
+
+// ---------------------------------------------------------------------------
+// Internal accessors (engine_internal.h)
+// ---------------------------------------------------------------------------
+namespace dyn {
+hipStream_t engine_stream(const dyn_engine* eng) { return eng->stream; }
+int engine_device(const dyn_engine* eng) { return eng->device; }
+void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water) {
+  *n_dyn = eng->n_dyn;
+  *n_water = eng->n_water;
+}
+void engine_box(const dyn_engine* eng, float box[3]) {
+  for (int a = 0; a < 3; ++a) box[a] = eng->box[a];
+}
+}  // namespace dyn

@@ -1,0 +1,871 @@
+// md.hip — device-resident molecular-dynamics step around the nonbonded
+// engine: positions, velocities and forces stay in HBM; one step enqueues
+// integration kernels, the engine's evaluation and the bonded forces on the
+// engine's stream, with no host round trip.
+//
+// Reference semantics:
+//   Langevin-middle step        src/integrate.rs:100-190
+//   velocity Verlet (γ = 0)     src/integrate.rs:191-260
+//   kick / drift                src/integrate.rs:478-590
+//   rigid-water drift           src/solvent/opc_settle.rs:107-198
+//   M-site force projection     src/solvent/mod.rs:397-407 (C_H, C_O :89-90)
+//   Langevin velocity update    src/thermostat.rs:308-362
+//   SHAKE / RATTLE              src/bonded.rs:141-304
+//   bonded terms                src/bonded_forces.rs:11-200
+// Units as the reference: Å, ps, amu, kcal/mol; accelerations use
+// KCAL_TO_NATIVE = 418.4 (src/lib.rs:196).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+#include "dyn_math.h"
+#include "engine_internal.h"
+
+using namespace dyn;
+
+namespace {
+
+constexpr float kKcalToNative = 418.4f;
+constexpr float kKb = 0.83144626f;  // k_B in Å²/ps²/K per amu (src/thermostat.rs:19)
+constexpr float kMassO = 16.0f, kMassH = 1.008f;  // src/solvent/mod.rs:58-59
+constexpr float kMassW = kMassO + 2.0f * kMassH;
+constexpr float kOEpR = 0.15939833f;               // O–M distance (src/solvent/mod.rs:66)
+constexpr float kRa = 0.5395199719801114f;         // O to H–H midpoint (opc_settle.rs:41)
+constexpr float kCH = (kOEpR / kRa) / 2.0f;
+constexpr float kCO = 1.0f - 2.0f * kCH;
+constexpr float kMaxAccel = 1e5f;                  // Å/ps² (src/integrate.rs:26)
+constexpr float kEpsConstr = 1.0e-8f;              // src/bonded.rs:10
+constexpr int kShakeMaxIt = 100;                   // src/bonded.rs:16
+constexpr int kMaxClusterCons = 16;                // constraints of one cluster
+constexpr int kBlock = 256;
+
+struct V3 {
+  float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 ld(const float4& p) { return {p.x, p.y, p.z}; }
+
+struct Box {
+  float L[3];
+};
+// src/barostat.rs:141-170: dv − round(dv/ext)·ext, and rem_euclid wrap into [0, L).
+__device__ __forceinline__ V3 min_image(V3 d, const Box& b) {
+  return {dyn_min_image_ref(d.x, b.L[0]), dyn_min_image_ref(d.y, b.L[1]),
+          dyn_min_image_ref(d.z, b.L[2])};
+}
+__device__ __forceinline__ float wrap_coord(float x, float L) {
+  float r = fmodf(x, L);
+  if (r < 0.0f) r += L;
+  if (!(r < L)) r = 0.0f;
+  return r;
+}
+
+// Counter-based normal deviates: SplitMix64 finalizer over (seed, step,
+// index), Box–Muller on two 24-bit uniforms.
+__device__ __host__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float normal_dev(uint64_t seed, uint64_t step, uint64_t idx) {
+  const uint64_t h = mix64(mix64(mix64(seed) + step) + idx);
+  const float u1 = ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+  const float u2 = ((float)((h >> 16) & 0xFFFFFFull) + 0.5f) * (1.0f / 16777216.0f);
+  return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+}
+
+struct Fx {
+  const int64_t* nb;  // engine forces, site order, component-major
+  const int64_t* bd;  // bonded forces over solute atoms, component-major
+  uint32_t n_sites, n_dyn;
+};
+__device__ __forceinline__ V3 force_of(const Fx& F, uint32_t s) {
+  int64_t f[3];
+  for (int c = 0; c < 3; ++c) {
+    f[c] = F.nb[(size_t)c * F.n_sites + s];
+    if (s < F.n_dyn) f[c] += F.bd[(size_t)c * F.n_dyn + s];
+  }
+  return {(float)((double)f[0] * DYN_FX_INV_SCALE), (float)((double)f[1] * DYN_FX_INV_SCALE),
+          (float)((double)f[2] * DYN_FX_INV_SCALE)};
+}
+
+// Rotate r by angle |ω|·dt about ω (Rodrigues; series for tiny angles).
+__device__ __forceinline__ V3 rodrigues(V3 r, V3 w, float dt) {
+  const V3 wd = w * dt;
+  const float th = sqrtf(dot(wd, wd));
+  if (th < 1e-12f) {
+    const V3 wxr = cross(wd, r);
+    return r + wxr + cross(wd, wxr) * 0.5f;
+  }
+  const V3 n = wd * (1.0f / th);
+  const float c = cosf(th), s = sinf(th);
+  return r * c + cross(n, r) * s + n * (dot(n, r) * (1.0f - c));
+}
+
+// Rigid drift of one water over dt: centre-of-mass translation plus the
+// rotation whose angular velocity reproduces the molecule's angular
+// momentum; velocities become the rigid-body ones; O is wrapped, the H
+// sites follow it, and M is placed on the H–O–H bisector.
+__device__ void rigid_drift(float4* __restrict__ x, float4* __restrict__ v, uint32_t base,
+                            float dt, const Box& B) {
+  const V3 o = ld(x[base]);
+  const V3 h0 = o + min_image(ld(x[base + 2]) - o, B);
+  const V3 h1 = o + min_image(ld(x[base + 3]) - o, B);
+  const V3 vo = ld(v[base]), vh0 = ld(v[base + 2]), vh1 = ld(v[base + 3]);
+  const V3 rc = (o * kMassO + h0 * kMassH + h1 * kMassH) * (1.0f / kMassW);
+  const V3 vc = (vo * kMassO + vh0 * kMassH + vh1 * kMassH) * (1.0f / kMassW);
+  const V3 rO = o - rc, rH0 = h0 - rc, rH1 = h1 - rc;
+  const V3 uO = vo - vc, uH0 = vh0 - vc, uH1 = vh1 - vc;
+  const V3 Lm = cross(rO, uO) * kMassO + cross(rH0, uH0) * kMassH + cross(rH1, uH1) * kMassH;
+  float I[6] = {0, 0, 0, 0, 0, 0};  // xx, yy, zz, xy, xz, yz
+  const V3 rs[3] = {rO, rH0, rH1};
+  const float ms[3] = {kMassO, kMassH, kMassH};
+  for (int k = 0; k < 3; ++k) {
+    const V3 r = rs[k];
+    const float m = ms[k], r2 = dot(r, r);
+    I[0] += m * (r2 - r.x * r.x);
+    I[1] += m * (r2 - r.y * r.y);
+    I[2] += m * (r2 - r.z * r.z);
+    I[3] += -m * r.x * r.y;
+    I[4] += -m * r.x * r.z;
+    I[5] += -m * r.y * r.z;
+  }
+  const float ixx = I[0], iyy = I[1], izz = I[2], ixy = I[3], ixz = I[4], iyz = I[5];
+  const float det = ixx * (iyy * izz - iyz * iyz) - ixy * (ixy * izz - iyz * ixz) +
+                    ixz * (ixy * iyz - iyy * ixz);
+  V3 w = {0.f, 0.f, 0.f};
+  if (fabsf(det) >= 1.0e-12f) {
+    const float id = 1.0f / det;
+    const float a00 = (iyy * izz - iyz * iyz) * id, a01 = (ixz * iyz - ixy * izz) * id,
+                a02 = (ixy * iyz - ixz * iyy) * id, a11 = (ixx * izz - ixz * ixz) * id,
+                a12 = (ixz * ixy - ixx * iyz) * id, a22 = (ixx * iyy - ixy * ixy) * id;
+    w = {a00 * Lm.x + a01 * Lm.y + a02 * Lm.z, a01 * Lm.x + a11 * Lm.y + a12 * Lm.z,
+         a02 * Lm.x + a12 * Lm.y + a22 * Lm.z};
+  }
+  const V3 dlt = vc * dt;
+  const V3 rO2 = rodrigues(rO, w, dt), rH02 = rodrigues(rH0, w, dt), rH12 = rodrigues(rH1, w, dt);
+  const V3 no = rc + dlt + rO2, nh0 = rc + dlt + rH02, nh1 = rc + dlt + rH12;
+  const V3 wo = {wrap_coord(no.x, B.L[0]), wrap_coord(no.y, B.L[1]), wrap_coord(no.z, B.L[2])};
+  const V3 wh0 = wo + min_image(nh0 - wo, B), wh1 = wo + min_image(nh1 - wo, B);
+  const V3 bis = (wh0 - wo) + (wh1 - wo);
+  const float bl = sqrtf(dot(bis, bis));
+  const V3 wm = bl > 0.0f ? wo + bis * (kOEpR / bl) : wo;
+  x[base] = make_float4(wo.x, wo.y, wo.z, 0.f);
+  x[base + 1] = make_float4(wm.x, wm.y, wm.z, 0.f);
+  x[base + 2] = make_float4(wh0.x, wh0.y, wh0.z, 0.f);
+  x[base + 3] = make_float4(wh1.x, wh1.y, wh1.z, 0.f);
+  const V3 vO2 = vc + cross(w, rO2), vH02 = vc + cross(w, rH02), vH12 = vc + cross(w, rH12);
+  v[base] = make_float4(vO2.x, vO2.y, vO2.z, 0.f);
+  v[base + 2] = make_float4(vH02.x, vH02.y, vH02.z, 0.f);
+  v[base + 3] = make_float4(vH12.x, vH12.y, vH12.z, 0.f);
+  const V3 vm = (vH02 + vH12) * 0.5f;
+  v[base + 1] = make_float4(vm.x, vm.y, vm.z, 0.f);
+}
+
+struct IntArgs {
+  Fx F;
+  const float* acc_conv;  // per solute atom: KCAL_TO_NATIVE / mass
+  Box B;
+  uint32_t n_dyn, n_water;
+  float h_kick, h_drift;
+  int kick, drift;
+};
+
+// One item per solute atom, then one per water.  kick: v += a·h_kick from
+// the current forces (M-site force projected onto O and H for water; atom
+// accelerations clamped to MAX_ACCEL); drift: x += v·h_drift for atoms,
+// the rigid drift for water.
+__global__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.n_dyn) {
+    float4 vi = v[t];
+    if (A.kick) {
+      V3 a = force_of(A.F, t) * A.acc_conv[t];
+      const float a2 = dot(a, a);
+      if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
+      vi.x += a.x * A.h_kick;
+      vi.y += a.y * A.h_kick;
+      vi.z += a.z * A.h_kick;
+      v[t] = vi;
+    }
+    if (A.drift) {
+      float4 xi = x[t];
+      xi.x += vi.x * A.h_drift;
+      xi.y += vi.y * A.h_drift;
+      xi.z += vi.z * A.h_drift;
+      x[t] = xi;
+    }
+    return;
+  }
+  const uint32_t m = t - A.n_dyn;
+  if (m >= A.n_water) return;
+  const uint32_t base = A.n_dyn + 4u * m;
+  if (A.kick) {
+    const V3 fm = force_of(A.F, base + 1);
+    const V3 fo = force_of(A.F, base) + fm * kCO;
+    const V3 f0 = force_of(A.F, base + 2) + fm * kCH;
+    const V3 f1 = force_of(A.F, base + 3) + fm * kCH;
+    const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
+    const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
+    const uint32_t ss[3] = {base, base + 2, base + 3};
+    for (int k = 0; k < 3; ++k) {
+      float4 vv = v[ss[k]];
+      vv.x += fs[k].x * A.h_kick;
+      vv.y += fs[k].y * A.h_kick;
+      vv.z += fs[k].z * A.h_kick;
+      v[ss[k]] = vv;
+    }
+  }
+  if (A.drift) rigid_drift(x, v, base, A.h_drift, A.B);
+}
+
+// Langevin velocity update (Ornstein–Uhlenbeck): v ← c·v + σ·ξ per
+// component, σ² = k_B T (1 − c²)/m; solute atoms, water O and H.
+__global__ void k_md_langevin(float4* __restrict__ v, uint32_t n_dyn, uint32_t n_water, float c,
+                              const float* __restrict__ sigma_dyn, float sigma_o, float sigma_h,
+                              uint64_t seed, uint64_t step) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t sites[3];
+  float sig[3];
+  int n = 0;
+  if (t < n_dyn) {
+    sites[0] = t;
+    sig[0] = sigma_dyn[t];
+    n = 1;
+  } else if (t - n_dyn < n_water) {
+    const uint32_t base = n_dyn + 4u * (t - n_dyn);
+    sites[0] = base;
+    sites[1] = base + 2;
+    sites[2] = base + 3;
+    sig[0] = sigma_o;
+    sig[1] = sig[2] = sigma_h;
+    n = 3;
+  }
+  for (int k = 0; k < n; ++k) {
+    const uint32_t s = sites[k];
+    float4 vv = v[s];
+    vv.x = c * vv.x + sig[k] * normal_dev(seed, step, 3ull * s + 0);
+    vv.y = c * vv.y + sig[k] * normal_dev(seed, step, 3ull * s + 1);
+    vv.z = c * vv.z + sig[k] * normal_dev(seed, step, 3ull * s + 2);
+    v[s] = vv;
+  }
+}
+
+struct ConsArgs {
+  const uint32_t* cl_ptr;     // per cluster: range of constraint indices
+  const uint2* ij;            // per constraint
+  const float* r0sq;
+  const float* inv_m;         // 1/m_i + 1/m_j
+  const float* mass;          // per solute atom
+  uint32_t n_clusters;
+  float tol;
+};
+
+// SHAKE over one cluster of coupled constraints (one thread), in the
+// constraints' input order, until the largest correction is below tol.
+__global__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.n_clusters) return;
+  const uint32_t k0 = C.cl_ptr[c], k1 = C.cl_ptr[c + 1];
+  for (int it = 0; it < kShakeMaxIt; ++it) {
+    float max_corr = 0.0f;
+    for (uint32_t k = k0; k < k1; ++k) {
+      const uint2 p = C.ij[k];
+      const float4 a = x[p.x], b = x[p.y];
+      const V3 d = v3(b.x - a.x, b.y - a.y, b.z - a.z);
+      const float d2 = dot(d, d);
+      const float lam = (d2 - C.r0sq[k]) / (2.0f * C.inv_m[k] * fmaxf(d2, kEpsConstr));
+      const V3 corr = d * lam;
+      const float mi = 1.0f / C.mass[p.x], mj = 1.0f / C.mass[p.y];
+      x[p.x] = make_float4(a.x + corr.x * mi, a.y + corr.y * mi, a.z + corr.z * mi, a.w);
+      x[p.y] = make_float4(b.x - corr.x * mj, b.y - corr.y * mj, b.z - corr.z * mj, b.w);
+      max_corr = fmaxf(max_corr, sqrtf(dot(corr, corr)));
+    }
+    if (max_corr < C.tol) break;
+  }
+}
+
+// RATTLE: one pass over the cluster's constraints removing the relative
+// velocity along each bond.
+__global__ void k_md_rattle(ConsArgs C, const float4* __restrict__ x, float4* __restrict__ v) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C.n_clusters) return;
+  for (uint32_t k = C.cl_ptr[c]; k < C.cl_ptr[c + 1]; ++k) {
+    const uint2 p = C.ij[k];
+    const float4 a = x[p.x], b = x[p.y];
+    const float4 va = v[p.x], vb = v[p.y];
+    const V3 r = v3(b.x - a.x, b.y - a.y, b.z - a.z);
+    const V3 dv = v3(vb.x - va.x, vb.y - va.y, vb.z - va.z);
+    const float lam = dot(dv, r) / (C.inv_m[k] * fmaxf(dot(r, r), kEpsConstr));
+    const V3 corr = r * lam;
+    const float mi = 1.0f / C.mass[p.x], mj = 1.0f / C.mass[p.y];
+    v[p.x] = make_float4(va.x + corr.x * mi, va.y + corr.y * mi, va.z + corr.z * mi, va.w);
+    v[p.y] = make_float4(vb.x - corr.x * mj, vb.y - corr.y * mj, vb.z - corr.z * mj, vb.w);
+  }
+}
+
+struct BondedArgs {
+  const float4* x;
+  Box B;
+  uint32_t n_dyn;
+  uint32_t n_bond, n_angle, n_dihe;
+  const uint32_t* bond;   // 2 per term
+  const float2* bond_p;   // k, r0
+  const uint32_t* angle;  // 3 per term (vertex in the middle)
+  const float2* angle_p;  // k, θ0
+  const uint32_t* dihe;   // 4 per term
+  const float4* dihe_p;   // k, periodicity, phase, unused
+  int64_t* f;             // fixed point, component-major over solute atoms
+  double* partials;       // 3 per block: bond, angle, dihedral energy
+};
+
+__device__ __forceinline__ void add_f(int64_t* f, uint32_t nd, uint32_t i, V3 v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(f + i), (unsigned long long)dyn_fx(v.x));
+  atomicAdd(reinterpret_cast<unsigned long long*>(f + nd + i), (unsigned long long)dyn_fx(v.y));
+  atomicAdd(reinterpret_cast<unsigned long long*>(f + 2 * (size_t)nd + i), (unsigned long long)dyn_fx(v.z));
+}
+
+// Harmonic bonds and angles (U = k(r − r0)², k(θ − θ0)²) and periodic
+// dihedrals (U = k(1 + cos(nφ − γ)), φ the IUPAC torsion angle); one
+// thread per term, fixed-point force sums.
+__global__ void __launch_bounds__(kBlock) k_md_bonded(BondedArgs A) {
+  __shared__ double red[3][kBlock];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[3] = {0.0, 0.0, 0.0};
+  const uint32_t nd = A.n_dyn;
+  if (t < A.n_bond) {
+    const uint32_t i = A.bond[2 * t], j = A.bond[2 * t + 1];
+    const V3 d = min_image(ld(A.x[j]) - ld(A.x[i]), A.B);
+    const float r = sqrtf(dot(d, d));
+    const float2 p = A.bond_p[t];
+    const float dr = r - p.y;
+    const V3 fi = d * (2.0f * p.x * dr / fmaxf(r, 1e-8f));
+    add_f(A.f, nd, i, fi);
+    add_f(A.f, nd, j, fi * -1.0f);
+    e[0] = (double)(p.x * dr * dr);
+  } else if (t - A.n_bond < A.n_angle) {
+    const uint32_t q = t - A.n_bond;
+    const uint32_t i = A.angle[3 * q], j = A.angle[3 * q + 1], k = A.angle[3 * q + 2];
+    const V3 b0 = min_image(ld(A.x[i]) - ld(A.x[j]), A.B);
+    const V3 b2 = min_image(ld(A.x[k]) - ld(A.x[j]), A.B);
+    const float l0s = dot(b0, b0), l2s = dot(b2, b2);
+    if (l0s >= 1e-8f && l2s >= 1e-8f) {
+      const float l0 = sqrtf(l0s), l2 = sqrtf(l2s);
+      const V3 u0 = b0 * (1.0f / l0), u2 = b2 * (1.0f / l2);
+      const float ct = fminf(1.0f, fmaxf(-1.0f, dot(u0, u2)));
+      const float th = acosf(ct);
+      const float st = fmaxf(sqrtf(1.0f - ct * ct), 1e-8f);
+      const float2 p = A.angle_p[q];
+      const float dth = th - p.y;
+      const float dedt = 2.0f * p.x * dth;
+      const V3 g0 = (u0 * ct - u2) * (1.0f / (l0 * st));
+      const V3 g2 = (u2 * ct - u0) * (1.0f / (l2 * st));
+      const V3 f0 = g0 * -dedt, f2 = g2 * -dedt;
+      add_f(A.f, nd, i, f0);
+      add_f(A.f, nd, k, f2);
+      add_f(A.f, nd, j, (f0 + f2) * -1.0f);
+      e[1] = (double)(p.x * dth * dth);
+    }
+  } else if (t - A.n_bond - A.n_angle < A.n_dihe) {
+    const uint32_t q = t - A.n_bond - A.n_angle;
+    const uint32_t i = A.dihe[4 * q], j = A.dihe[4 * q + 1], k = A.dihe[4 * q + 2],
+                   l = A.dihe[4 * q + 3];
+    const V3 b1 = min_image(ld(A.x[j]) - ld(A.x[i]), A.B);
+    const V3 b2 = min_image(ld(A.x[k]) - ld(A.x[j]), A.B);
+    const V3 b3 = min_image(ld(A.x[l]) - ld(A.x[k]), A.B);
+    const V3 n1 = cross(b1, b2), n2 = cross(b2, b3);
+    const float n1s = dot(n1, n1), n2s = dot(n2, n2), b2s = dot(b2, b2);
+    const float b2l = sqrtf(b2s);
+    if (n1s >= 1e-6f && n2s >= 1e-6f && b2l >= 1e-6f) {
+      const float phi = atan2f(b2l * dot(b1, n2), dot(n1, n2));
+      const float4 p = A.dihe_p[q];
+      const float arg = p.y * phi - p.z;
+      const float dedphi = -p.x * p.y * sinf(arg);
+      // ∂φ/∂x_i = −|b2|/|n1|² n1, ∂φ/∂x_l = |b2|/|n2|² n2; the inner atoms
+      // take the lever-arm split of the two
+      const V3 gi = n1 * (-b2l / n1s), gl = n2 * (b2l / n2s);
+      const float ta = dot(b1, b2) / b2s, tb = dot(b3, b2) / b2s;
+      const V3 gj = gl * tb - gi * (1.0f + ta);
+      const V3 gk = gi * ta - gl * (1.0f + tb);
+      add_f(A.f, nd, i, gi * -dedphi);
+      add_f(A.f, nd, j, gj * -dedphi);
+      add_f(A.f, nd, k, gk * -dedphi);
+      add_f(A.f, nd, l, gl * -dedphi);
+      e[2] = (double)(p.x * (1.0f + cosf(arg)));
+    }
+  }
+  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = e[c];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 3; ++c) A.partials[3 * blockIdx.x + c] = red[c][0];
+}
+
+__global__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = 0;
+}
+
+// Kinetic energy (amu Å²/ps²) of solute atoms and water O/H, per block; and
+// the reduction of the bonded-energy partials.  out: [0] kinetic,
+// [1..3] bond, angle, dihedral.
+__global__ void __launch_bounds__(1024)
+k_md_energies(const float4* __restrict__ v, const float* __restrict__ mass, uint32_t n_dyn,
+              uint32_t n_water, const double* __restrict__ bpart, uint32_t n_bblk,
+              double* __restrict__ out) {
+  __shared__ double red[4][1024];
+  const uint32_t t = threadIdx.x;
+  double a[4] = {0, 0, 0, 0};
+  for (uint32_t i = t; i < n_dyn; i += 1024) {
+    const float4 u = v[i];
+    a[0] += 0.5 * (double)mass[i] * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+  }
+  for (uint32_t m = t; m < n_water; m += 1024) {
+    const uint32_t b = n_dyn + 4 * m;
+    const uint32_t ss[3] = {b, b + 2, b + 3};
+    const double ms[3] = {kMassO, kMassH, kMassH};
+    for (int k = 0; k < 3; ++k) {
+      const float4 u = v[ss[k]];
+      a[0] += 0.5 * ms[k] * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+    }
+  }
+  for (uint32_t r = t; r < n_bblk; r += 1024)
+    for (int c = 0; c < 3; ++c) a[1 + c] += bpart[3 * r + c];
+  for (int c = 0; c < 4; ++c) red[c][t] = a[c];
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (t < (uint32_t)s)
+      for (int c = 0; c < 4; ++c) red[c][t] += red[c][t + s];
+    __syncthreads();
+  }
+  if (t < 4) out[t] = red[t][0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handle
+// ---------------------------------------------------------------------------
+// An engine call's failure, with the engine's code and message.
+static void engine_call(int rc, dyn_engine* eng) {
+  if (rc != DYN_OK) throw Error(rc, fmt("engine: %s", dyn_engine_last_error(eng)));
+}
+
+struct dyn_md {
+  std::string err;
+  dyn_engine* eng = nullptr;
+  int device = 0;
+  hipStream_t stream = nullptr;  // the engine's stream
+  dyn_md_params prm{};
+  uint32_t n_dyn = 0, n_water = 0, n_sites = 0;
+  float box[3] = {0, 0, 0};
+  DevBuf<float4> x, v;
+  DevBuf<float> mass, acc_conv, lang_sigma;
+  float sigma_o = 0.f, sigma_h = 0.f, lang_c = 1.f;
+  uint32_t n_bond = 0, n_angle = 0, n_dihe = 0;
+  DevBuf<uint32_t> bond, angle, dihe;
+  DevBuf<float2> bond_p, angle_p;
+  DevBuf<float4> dihe_p;
+  DevBuf<int64_t> fb;
+  DevBuf<double> bpart;
+  unsigned bblocks = 1;
+  uint32_t n_clusters = 0, n_cons = 0;
+  DevBuf<uint32_t> cl_ptr;
+  DevBuf<uint2> cons_ij;
+  DevBuf<float> cons_r0sq, cons_invm;
+  DevBuf<double> eout;
+  PinnedBuf<double> h_out;
+  uint64_t step = 0;
+  bool state_set = false, forces_valid = false;
+
+  void forces();
+  void integrate(float hk, float hd, bool kick, bool drift);
+  void constrain_positions();
+  void constrain_velocities();
+  void one_step();
+};
+
+// The first evaluation after set_state is synchronous, so that a list that
+// outgrows its buffers is rebuilt at once; later ones are enqueued only (a
+// failure is reported at the next get_state / get_energies).
+void dyn_md::forces() {
+  engine_call(dyn_engine_compute(eng, DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u)),
+              eng);
+  hipLaunchKernelGGL(k_md_zero, dim3(grid_for(3 * (size_t)std::max(n_dyn, 1u), kBlock)), dim3(kBlock),
+                     0, stream, fb.p, 3 * (size_t)n_dyn);
+  DYN_CHECK_LAUNCH("k_md_zero");
+  if (n_bond + n_angle + n_dihe) {
+    BondedArgs A;
+    A.x = x.p;
+    for (int a = 0; a < 3; ++a) A.B.L[a] = box[a];
+    A.n_dyn = n_dyn;
+    A.n_bond = n_bond;
+    A.n_angle = n_angle;
+    A.n_dihe = n_dihe;
+    A.bond = bond.p;
+    A.bond_p = bond_p.p;
+    A.angle = angle.p;
+    A.angle_p = angle_p.p;
+    A.dihe = dihe.p;
+    A.dihe_p = dihe_p.p;
+    A.f = fb.p;
+    A.partials = bpart.p;
+    hipLaunchKernelGGL(k_md_bonded, dim3(bblocks), dim3(kBlock), 0, stream, A);
+    DYN_CHECK_LAUNCH("k_md_bonded");
+  }
+  forces_valid = true;
+}
+
+void dyn_md::integrate(float hk, float hd, bool kick, bool drift) {
+  IntArgs A;
+  A.F.nb = dyn_engine_forces_device(eng);
+  A.F.bd = fb.p;
+  A.F.n_sites = n_sites;
+  A.F.n_dyn = n_dyn;
+  A.acc_conv = acc_conv.p;
+  for (int a = 0; a < 3; ++a) A.B.L[a] = box[a];
+  A.n_dyn = n_dyn;
+  A.n_water = n_water;
+  A.h_kick = hk;
+  A.h_drift = hd;
+  A.kick = kick ? 1 : 0;
+  A.drift = drift ? 1 : 0;
+  hipLaunchKernelGGL(k_md_integrate, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
+                     stream, A, x.p, v.p);
+  DYN_CHECK_LAUNCH("k_md_integrate");
+}
+
+static ConsArgs cons_args(const dyn_md& M) {
+  ConsArgs C;
+  C.cl_ptr = M.cl_ptr.p;
+  C.ij = M.cons_ij.p;
+  C.r0sq = M.cons_r0sq.p;
+  C.inv_m = M.cons_invm.p;
+  C.mass = M.mass.p;
+  C.n_clusters = M.n_clusters;
+  C.tol = M.prm.shake_tol;
+  return C;
+}
+
+void dyn_md::constrain_positions() {
+  if (!n_clusters) return;
+  hipLaunchKernelGGL(k_md_shake, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
+                     cons_args(*this), x.p);
+  DYN_CHECK_LAUNCH("k_md_shake");
+}
+
+void dyn_md::constrain_velocities() {
+  if (!n_clusters) return;
+  hipLaunchKernelGGL(k_md_rattle, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
+                     cons_args(*this), (const float4*)x.p, v.p);
+  DYN_CHECK_LAUNCH("k_md_rattle");
+}
+
+// One step, as src/integrate.rs:100-190 (γ > 0, Langevin-middle) or
+// :191-260 (γ = 0, velocity Verlet without thermostat).
+void dyn_md::one_step() {
+  const float dt = prm.dt, h = 0.5f * dt;
+  if (!forces_valid) forces();
+  if (prm.gamma > 0.f) {
+    integrate(h, h, true, true);
+    constrain_positions();
+    constrain_velocities();
+    hipLaunchKernelGGL(k_md_langevin, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
+                       stream, v.p, n_dyn, n_water, lang_c, lang_sigma.p, sigma_o, sigma_h,
+                       prm.seed, step);
+    DYN_CHECK_LAUNCH("k_md_langevin");
+    constrain_velocities();
+    integrate(0.f, h, false, true);
+    constrain_positions();
+  } else {
+    integrate(h, dt, true, true);
+    constrain_positions();
+    constrain_velocities();
+  }
+  forces();
+  integrate(h, 0.f, true, false);
+  constrain_velocities();
+  ++step;
+}
+
+// ---------------------------------------------------------------------------
+// C entry points
+// ---------------------------------------------------------------------------
+#define MD_CHECK(m)                        \
+  if (!(m)) {                              \
+    set_global_error("null MD handle");    \
+    return DYN_ERR_ARG;                    \
+  }
+
+extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params* p,
+                             const float* mass_dyn, uint32_t n_bonds, const uint32_t* bond_ij,
+                             const float* bond_k_r0, uint32_t n_angles, const uint32_t* angle_ijk,
+                             const float* angle_k_theta0, uint32_t n_dihedrals,
+                             const uint32_t* dihedral_ijkl, const float* dihedral_k_n_phase,
+                             uint32_t n_constraints, const uint32_t* constraint_ij,
+                             const float* constraint_r0) {
+  try {
+    DYN_REQUIRE(out && eng && p, "null argument");
+    *out = nullptr;
+    DYN_REQUIRE(p->dt > 0.f && p->dt < 0.01f, "dt must be in (0, 0.01) ps");
+    DYN_REQUIRE(p->temperature >= 0.f && std::isfinite(p->temperature), "bad temperature");
+    DYN_REQUIRE(p->gamma >= 0.f && std::isfinite(p->gamma), "gamma must be ≥ 0");
+    DYN_REQUIRE(p->shake_tol > 0.f, "shake_tol must be > 0");
+    std::unique_ptr<dyn_md> M(new dyn_md());
+    M->eng = eng;
+    M->device = engine_device(eng);
+    M->stream = engine_stream(eng);
+    DeviceGuard g(M->device);
+    M->prm = *p;
+    engine_sizes(eng, &M->n_dyn, &M->n_water);
+    engine_box(eng, M->box);
+    DYN_REQUIRE(M->box[0] > 0.f, "set the engine's box first");
+    const uint32_t nd = M->n_dyn, nw = M->n_water, ns = nd + 4 * nw;
+    M->n_sites = ns;
+    DYN_REQUIRE(nd == 0 || mass_dyn, "mass_dyn is null");
+    std::vector<float> m(nd), conv(nd), sig(nd);
+    const float c = std::exp(-p->gamma * p->dt);
+    const float s2 = std::max(0.0f, 1.0f - c * c);
+    const float num = kKb * p->temperature * s2;
+    for (uint32_t i = 0; i < nd; ++i) {
+      DYN_REQUIRE(mass_dyn[i] > 0.f && std::isfinite(mass_dyn[i]), "atom %u: mass must be > 0", i);
+      m[i] = mass_dyn[i];
+      conv[i] = kKcalToNative / m[i];
+      sig[i] = std::sqrt(num / m[i]);
+    }
+    M->lang_c = c;
+    M->sigma_o = std::sqrt(num / kMassO);
+    M->sigma_h = std::sqrt(num / kMassH);
+    auto up = [&](auto& buf, const void* src, size_t count, size_t elem) {
+      buf.reserve(std::max<size_t>(count, 1));
+      if (count)
+        DYN_HIP(hipMemcpyAsync(buf.p, src, count * elem, hipMemcpyHostToDevice, M->stream));
+    };
+    up(M->mass, m.data(), nd, 4);
+    up(M->acc_conv, conv.data(), nd, 4);
+    up(M->lang_sigma, sig.data(), nd, 4);
+    auto check_idx = [&](const uint32_t* idx, uint32_t n, int per, const char* what) {
+      DYN_REQUIRE(n == 0 || idx, "%s indices are null", what);
+      for (uint64_t k = 0; k < (uint64_t)n * per; ++k)
+        DYN_REQUIRE(idx[k] < nd, "%s term %llu: atom %u out of range", what,
+                    (unsigned long long)(k / per), idx[k]);
+    };
+    check_idx(bond_ij, n_bonds, 2, "bond");
+    check_idx(angle_ijk, n_angles, 3, "angle");
+    check_idx(dihedral_ijkl, n_dihedrals, 4, "dihedral");
+    check_idx(constraint_ij, n_constraints, 2, "constraint");
+    DYN_REQUIRE(n_bonds == 0 || bond_k_r0, "bond parameters are null");
+    DYN_REQUIRE(n_angles == 0 || angle_k_theta0, "angle parameters are null");
+    DYN_REQUIRE(n_dihedrals == 0 || dihedral_k_n_phase, "dihedral parameters are null");
+    M->n_bond = n_bonds;
+    M->n_angle = n_angles;
+    M->n_dihe = n_dihedrals;
+    up(M->bond, bond_ij, 2 * (size_t)n_bonds, 4);
+    up(M->bond_p, bond_k_r0, n_bonds, 8);
+    up(M->angle, angle_ijk, 3 * (size_t)n_angles, 4);
+    up(M->angle_p, angle_k_theta0, n_angles, 8);
+    up(M->dihe, dihedral_ijkl, 4 * (size_t)n_dihedrals, 4);
+    std::vector<float4> dp(n_dihedrals);
+    for (uint32_t k = 0; k < n_dihedrals; ++k)
+      dp[k] = make_float4(dihedral_k_n_phase[3 * k], dihedral_k_n_phase[3 * k + 1],
+                          dihedral_k_n_phase[3 * k + 2], 0.f);
+    up(M->dihe_p, dp.data(), n_dihedrals, 16);
+    M->bblocks = grid_for(std::max<uint32_t>(n_bonds + n_angles + n_dihedrals, 1), kBlock);
+    M->bpart.reserve(3 * (size_t)M->bblocks);
+    DYN_HIP(hipMemsetAsync(M->bpart.p, 0, sizeof(double) * 3 * M->bblocks, M->stream));
+    M->fb.reserve(3 * (size_t)std::max(nd, 1u));
+    // constraint clusters: connected components of the constraint graph
+    std::vector<uint32_t> parent(nd);
+    std::iota(parent.begin(), parent.end(), 0u);
+    std::function<uint32_t(uint32_t)> find = [&](uint32_t a) {
+      while (parent[a] != a) a = parent[a] = parent[parent[a]];
+      return a;
+    };
+    for (uint32_t k = 0; k < n_constraints; ++k) {
+      DYN_REQUIRE(constraint_r0 && constraint_r0[k] > 0.f, "constraint %u: r0 must be > 0", k);
+      DYN_REQUIRE(constraint_ij[2 * k] != constraint_ij[2 * k + 1], "constraint %u joins an atom to itself", k);
+      parent[find(constraint_ij[2 * k])] = find(constraint_ij[2 * k + 1]);
+    }
+    std::vector<std::vector<uint32_t>> groups;
+    std::vector<int64_t> gid(nd, -1);
+    for (uint32_t k = 0; k < n_constraints; ++k) {
+      const uint32_t r = find(constraint_ij[2 * k]);
+      if (gid[r] < 0) {
+        gid[r] = (int64_t)groups.size();
+        groups.emplace_back();
+      }
+      groups[gid[r]].push_back(k);
+    }
+    std::vector<uint32_t> ptr(1, 0);
+    std::vector<uint2> ij;
+    std::vector<float> r0sq, invm;
+    for (auto& gq : groups) {
+      DYN_REQUIRE(gq.size() <= (size_t)kMaxClusterCons, "a constraint cluster has %zu constraints "
+                  "(at most %d: constrain bonds to hydrogen only)", gq.size(), kMaxClusterCons);
+      for (uint32_t k : gq) {
+        const uint32_t i = constraint_ij[2 * k], j = constraint_ij[2 * k + 1];
+        ij.push_back(make_uint2(i, j));
+        r0sq.push_back(constraint_r0[k] * constraint_r0[k]);
+        invm.push_back(1.0f / m[i] + 1.0f / m[j]);
+      }
+      ptr.push_back((uint32_t)ij.size());
+    }
+    M->n_clusters = (uint32_t)groups.size();
+    M->n_cons = n_constraints;
+    up(M->cl_ptr, ptr.data(), ptr.size(), 4);
+    up(M->cons_ij, ij.data(), ij.size(), 8);
+    up(M->cons_r0sq, r0sq.data(), r0sq.size(), 4);
+    up(M->cons_invm, invm.data(), invm.size(), 4);
+    M->x.reserve(ns);
+    M->v.reserve(ns);
+    M->eout.reserve(4);
+    M->h_out.reserve(4);
+    DYN_HIP(hipStreamSynchronize(M->stream));
+    *out = M.release();
+    return DYN_OK;
+  } catch (const Error& e) {
+    set_global_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_global_error(e.what());
+    return DYN_ERR_HIP;
+  }
+}
+
+extern "C" void dyn_md_destroy(dyn_md* md) {
+  if (!md) return;
+  try {
+    DeviceGuard g(md->device);
+    (void)hipStreamSynchronize(md->stream);
+    delete md;
+  } catch (...) {
+  }
+}
+
+extern "C" const char* dyn_md_last_error(const dyn_md* md) {
+  return md ? md->err.c_str() : dyn_global_last_error();
+}
+
+extern "C" int dyn_md_set_state(dyn_md* md, const float* pos_dyn, const float* vel_dyn,
+                                const float* pos_water, const float* vel_water) {
+  MD_CHECK(md);
+  DYN_API_BODY(md, {
+    DeviceGuard g(md->device);
+    const uint32_t nd = md->n_dyn, nw = md->n_water, ns = md->n_sites;
+    DYN_REQUIRE(nd == 0 || (pos_dyn && vel_dyn), "solute state is null");
+    DYN_REQUIRE(nw == 0 || (pos_water && vel_water), "water state is null");
+    std::vector<float4> hx(ns), hv(ns);
+    for (uint32_t i = 0; i < nd; ++i) {
+      hx[i] = make_float4(pos_dyn[3 * i], pos_dyn[3 * i + 1], pos_dyn[3 * i + 2], 0.f);
+      hv[i] = make_float4(vel_dyn[3 * i], vel_dyn[3 * i + 1], vel_dyn[3 * i + 2], 0.f);
+    }
+    for (uint32_t m = 0; m < nw; ++m) {
+      for (int k = 0; k < 4; ++k) {
+        const float* p = pos_water + 12 * (size_t)m + 3 * k;
+        hx[nd + 4 * m + k] = make_float4(p[0], p[1], p[2], 0.f);
+      }
+      // velocities of O, H0, H1; M takes the mean of the H velocities
+      const float* q = vel_water + 9 * (size_t)m;
+      hv[nd + 4 * m] = make_float4(q[0], q[1], q[2], 0.f);
+      hv[nd + 4 * m + 2] = make_float4(q[3], q[4], q[5], 0.f);
+      hv[nd + 4 * m + 3] = make_float4(q[6], q[7], q[8], 0.f);
+      hv[nd + 4 * m + 1] = make_float4(0.5f * (q[3] + q[6]), 0.5f * (q[4] + q[7]), 0.5f * (q[5] + q[8]), 0.f);
+    }
+    for (uint32_t s = 0; s < ns; ++s)
+      DYN_REQUIRE(std::isfinite(hx[s].x + hx[s].y + hx[s].z + hv[s].x + hv[s].y + hv[s].z),
+                  "non-finite state at site %u", s);
+    DYN_HIP(hipMemcpyAsync(md->x.p, hx.data(), 16 * (size_t)ns, hipMemcpyHostToDevice, md->stream));
+    DYN_HIP(hipMemcpyAsync(md->v.p, hv.data(), 16 * (size_t)ns, hipMemcpyHostToDevice, md->stream));
+    DYN_HIP(hipStreamSynchronize(md->stream));
+    engine_call(dyn_engine_set_positions_device(md->eng, reinterpret_cast<const float*>(md->x.p)),
+                md->eng);
+    md->state_set = true;
+    md->forces_valid = false;
+  });
+}
+
+extern "C" int dyn_md_step(dyn_md* md, uint32_t n_steps) {
+  MD_CHECK(md);
+  DYN_API_BODY(md, {
+    DeviceGuard g(md->device);
+    if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
+    for (uint32_t k = 0; k < n_steps; ++k) md->one_step();
+  });
+}
+
+extern "C" int dyn_md_get_state(dyn_md* md, float* pos_dyn, float* vel_dyn, float* pos_water,
+                                float* vel_water) {
+  MD_CHECK(md);
+  DYN_API_BODY(md, {
+    DeviceGuard g(md->device);
+    if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
+    engine_call(dyn_engine_sync(md->eng), md->eng);
+    const uint32_t nd = md->n_dyn, nw = md->n_water, ns = md->n_sites;
+    std::vector<float4> hx(ns), hv(ns);
+    DYN_HIP(hipMemcpyAsync(hx.data(), md->x.p, 16 * (size_t)ns, hipMemcpyDeviceToHost, md->stream));
+    DYN_HIP(hipMemcpyAsync(hv.data(), md->v.p, 16 * (size_t)ns, hipMemcpyDeviceToHost, md->stream));
+    DYN_HIP(hipStreamSynchronize(md->stream));
+    for (uint32_t i = 0; i < nd; ++i) {
+      if (pos_dyn) memcpy(pos_dyn + 3 * i, &hx[i], 12);
+      if (vel_dyn) memcpy(vel_dyn + 3 * i, &hv[i], 12);
+    }
+    for (uint32_t m = 0; m < nw; ++m) {
+      if (pos_water)
+        for (int k = 0; k < 4; ++k) memcpy(pos_water + 12 * (size_t)m + 3 * k, &hx[nd + 4 * m + k], 12);
+      if (vel_water) {
+        memcpy(vel_water + 9 * (size_t)m, &hv[nd + 4 * m], 12);
+        memcpy(vel_water + 9 * (size_t)m + 3, &hv[nd + 4 * m + 2], 12);
+        memcpy(vel_water + 9 * (size_t)m + 6, &hv[nd + 4 * m + 3], 12);
+      }
+    }
+  });
+}
+
+extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
+  MD_CHECK(md);
+  DYN_API_BODY(md, {
+    DeviceGuard g(md->device);
+    DYN_REQUIRE(out, "out is null");
+    if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
+    if (!md->forces_valid) md->forces();
+    hipLaunchKernelGGL(k_md_energies, dim3(1), dim3(1024), 0, md->stream, md->v.p, md->mass.p,
+                       md->n_dyn, md->n_water, md->bpart.p,
+                       (md->n_bond + md->n_angle + md->n_dihe) ? md->bblocks : 0u, md->eout.p);
+    DYN_CHECK_LAUNCH("k_md_energies");
+    DYN_HIP(hipMemcpyAsync(md->h_out.p, md->eout.p, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                           md->stream));
+    engine_call(dyn_engine_sync(md->eng), md->eng);
+    DYN_HIP(hipStreamSynchronize(md->stream));
+    dyn_md_energies E{};
+    E.kinetic = md->h_out.p[0] / (double)kKcalToNative;
+    E.bond = md->h_out.p[1];
+    E.angle = md->h_out.p[2];
+    E.dihedral = md->h_out.p[3];
+    // degrees of freedom: 3 per solute atom less one per constraint, 6 per
+    // rigid water
+    const double dof = 3.0 * md->n_dyn - (double)md->n_cons + 6.0 * md->n_water;
+    E.dof = dof;
+    E.temperature = dof > 0 ? 2.0 * md->h_out.p[0] / (dof * (double)kKb) : 0.0;
+    E.steps = md->step;
+    *out = E;
+  });
+}

@@ -147,6 +147,7 @@ class System:
     pairs_14: np.ndarray            # (k, 2) int32, i < j
     mol_start: np.ndarray           # molecule start indices over solute atoms
     name: str = ""
+    n_heavy: int = 0                # chain systems: heavy atoms come first, then H
 
     @property
     def n_dyn(self) -> int:
@@ -346,7 +347,7 @@ def solvated_chain(box: float, n_water: int, n_heavy: int, n_h: int, seed: int =
         lj_types=lj_types,
         pos_o=o.astype(F32), pos_m=m.astype(F32), pos_h0=h0.astype(F32), pos_h1=h1.astype(F32),
         bonds=bonds, excl_12_13=excl, pairs_14=p14,
-        mol_start=np.zeros(1, np.int64), name=name)
+        mol_start=np.zeros(1, np.int64), name=name, n_heavy=n_heavy)
 
 
 def exclusion_sets(n: int, bonds: np.ndarray):
@@ -408,3 +409,114 @@ def small_water(n_water: int = 512, box: float = 24.8, lattice_n: int = 8, seed:
 def small_solvated(seed: int = 2) -> System:
     """A 30-heavy + 40-H chain in 600 waters, 27 Å box (for parity tests)."""
     return solvated_chain(27.0, 600, 30, 40, seed=seed, lattice_n=10, name="chain70_opc600_box27")
+
+
+# --- a synthetic bonded force field and initial velocities (for MD steps) -------
+MASS_HEAVY = F32(12.011)
+MASS_H = F32(1.008)
+MASS_O = F32(16.0)
+KB_NATIVE = 0.83144626  # k_B in Å²/ps²/K per amu
+
+
+@dataclasses.dataclass
+class ForceField:
+    """Bonded terms and masses of the solute, in the reference's conventions
+    (src/bonded_forces.rs: U = k(r − r0)², k(θ − θ0)², k(1 + cos(nφ − γ)))."""
+
+    mass: np.ndarray              # (n_dyn,) amu
+    bonds: np.ndarray             # (nb, 2)
+    bond_k_r0: np.ndarray         # (nb, 2) kcal/mol/Å², Å
+    angles: np.ndarray            # (na, 3), vertex in the middle
+    angle_k_t0: np.ndarray        # (na, 2) kcal/mol/rad², rad
+    dihedrals: np.ndarray         # (nd, 4)
+    dihe_k_n_phase: np.ndarray    # (nd, 3) kcal/mol, periodicity, rad
+    constraints: np.ndarray       # (nc, 2)
+    constraint_r0: np.ndarray     # (nc,) Å
+
+
+def _angle(a, b, c, box):
+    u = _min_image(a - b, box)
+    v = _min_image(c - b, box)
+    return math.acos(max(-1.0, min(1.0, float(np.dot(u, v) / (np.linalg.norm(u) * np.linalg.norm(v))))))
+
+
+def chain_force_field(system: System, constrain_h: bool = True) -> ForceField:
+    """Harmonic bonds (k 310, r0 as built) and angles (k 50, θ0 as built) and a
+    threefold torsion (k 0.2) on every proper dihedral of the chain's bond
+    graph; heavy atoms 12.011 amu, hydrogens 1.008 amu; bonds to hydrogen
+    become constraints when constrain_h (and then carry no bond term)."""
+    nd = system.n_dyn
+    box = system.box.astype(np.float64)
+    pos = system.pos_dyn.astype(np.float64)
+    mass = np.full(nd, MASS_HEAVY, F32)
+    mass[system.n_heavy:] = MASS_H
+    is_h = np.zeros(nd, bool)
+    is_h[system.n_heavy:] = True
+    adj = [[] for _ in range(nd)]
+    for a, b in system.bonds:
+        adj[a].append(int(b))
+        adj[b].append(int(a))
+    bonds, bkr, cons, cr0 = [], [], [], []
+    for a, b in system.bonds:
+        a, b = int(a), int(b)
+        r0 = float(np.linalg.norm(_min_image(pos[b] - pos[a], box)))
+        if constrain_h and (is_h[a] or is_h[b]):
+            cons.append((a, b))
+            cr0.append(r0)
+        else:
+            bonds.append((a, b))
+            bkr.append((310.0, r0))
+    angles, akt = [], []
+    for j in range(nd):
+        nb = sorted(adj[j])
+        for x in range(len(nb)):
+            for y in range(x + 1, len(nb)):
+                i, k = nb[x], nb[y]
+                angles.append((i, j, k))
+                akt.append((50.0, _angle(pos[i], pos[j], pos[k], box)))
+    dih, dkp = [], []
+    for j, k in system.bonds:
+        j, k = int(j), int(k)
+        for i in adj[j]:
+            if i == k:
+                continue
+            for l in adj[k]:
+                if l in (j, i):
+                    continue
+                dih.append((i, j, k, l))
+                dkp.append((0.2, 3.0, 0.0))
+    arr = lambda v, w, dt: np.asarray(v, dt).reshape(-1, w)
+    return ForceField(mass=mass, bonds=arr(bonds, 2, np.uint32), bond_k_r0=arr(bkr, 2, F32),
+                      angles=arr(angles, 3, np.uint32), angle_k_t0=arr(akt, 2, F32),
+                      dihedrals=arr(dih, 4, np.uint32), dihe_k_n_phase=arr(dkp, 3, F32),
+                      constraints=arr(cons, 2, np.uint32), constraint_r0=np.asarray(cr0, F32))
+
+
+def water_force_field(system: System) -> ForceField:
+    """No solute: an empty force field."""
+    z = lambda w, dt: np.zeros((0, w), dt)
+    return ForceField(mass=np.zeros(0, F32), bonds=z(2, np.uint32), bond_k_r0=z(2, F32),
+                      angles=z(3, np.uint32), angle_k_t0=z(2, F32), dihedrals=z(4, np.uint32),
+                      dihe_k_n_phase=z(3, F32), constraints=z(2, np.uint32),
+                      constraint_r0=np.zeros(0, F32))
+
+
+def initial_velocities(system: System, ff: ForceField, temperature: float = 300.0, seed: int = 7):
+    """Maxwell–Boltzmann velocities: per solute atom, and per water one
+    centre-of-mass velocity shared by O, H0 and H1 (no initial rotation).
+    Returns (vel_dyn (n_dyn, 3), vel_water (n_water, 3, 3))."""
+    rng = SplitMix64(seed)
+    nd, nw = system.n_dyn, system.n_water
+    v = np.zeros((nd, 3), F32)
+    if nd:
+        sd = np.sqrt(KB_NATIVE * temperature / ff.mass.astype(np.float64))
+        v[:] = (rng.normal(3 * nd).reshape(nd, 3) * sd[:, None]).astype(F32)
+    mw = float(MASS_O + 2 * MASS_H)
+    vc = rng.normal(3 * nw).reshape(nw, 3) * math.sqrt(KB_NATIVE * temperature / mw)
+    vw = np.repeat(vc[:, None, :], 3, axis=1).astype(F32)
+    return v, vw
+
+
+def water_positions(system: System) -> np.ndarray:
+    """(n_water, 4, 3) positions in O, M, H0, H1 order."""
+    return np.stack([system.pos_o, system.pos_m, system.pos_h0, system.pos_h1], axis=1).astype(F32)

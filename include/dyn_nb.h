@@ -285,6 +285,68 @@ int dyn_engine_profile(dyn_engine* eng, int enable);
 int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char** names, double* ms,
                             uint64_t* counts, uint32_t* n);
 
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident MD step around an engine: positions, velocities and     */
+/* forces stay on the device; dyn_md_step enqueues whole steps (integration */
+/* kernels, the engine's evaluation, bonded forces) without host round      */
+/* trips.  Replaces MdState::step (src/integrate.rs:62-470) for the         */
+/* Langevin-middle integrator (src/integrate.rs:100-190) and, with          */
+/* gamma = 0, velocity Verlet without a thermostat (:191-260): rigid OPC    */
+/* water (src/solvent/opc_settle.rs:107-198) with the M-site force          */
+/* projected onto O and H (src/solvent/mod.rs:397-407), SHAKE/RATTLE        */
+/* constraints (src/bonded.rs:141-304), harmonic bonds and angles and       */
+/* periodic dihedrals (src/bonded_forces.rs:11-200).  Units: Å, ps, amu,    */
+/* kcal/mol.  Barostat, COM-motion removal and snapshots are not part of    */
+/* the step.                                                                */
+/* ------------------------------------------------------------------------ */
+typedef struct dyn_md dyn_md;
+
+typedef struct dyn_md_params {
+  float dt;           /* ps, in (0, 0.01) */
+  float temperature;  /* K, Langevin target */
+  float gamma;        /* 1/ps; 0 selects velocity Verlet without thermostat */
+  float shake_tol;    /* Å: SHAKE iterates until the largest correction is below it */
+  uint64_t seed;      /* Langevin noise (counter-based, per step and site) */
+} dyn_md_params;
+
+/* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],
+ * (k kcal/mol/Å², r0 Å)[2n], U = k (r − r0)²; angles: ijk[3n] (vertex j),
+ * (k kcal/mol/rad², θ0 rad)[2n], U = k (θ − θ0)²; dihedrals: ijkl[4n],
+ * (k kcal/mol, periodicity, phase rad)[3n], U = k (1 + cos(nφ − phase));
+ * constraints: ij[2n], r0[n] (coupled constraints form clusters of at most
+ * 16, as bonds to hydrogen do).  Indices are solute atoms.  The MD handle
+ * drives the engine (its positions become the MD state); destroy it before
+ * the engine. */
+int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params* p, const float* mass_dyn,
+                  uint32_t n_bonds, const uint32_t* bond_ij, const float* bond_k_r0,
+                  uint32_t n_angles, const uint32_t* angle_ijk, const float* angle_k_theta0,
+                  uint32_t n_dihedrals, const uint32_t* dihedral_ijkl,
+                  const float* dihedral_k_n_phase, uint32_t n_constraints,
+                  const uint32_t* constraint_ij, const float* constraint_r0);
+void dyn_md_destroy(dyn_md* md);
+const char* dyn_md_last_error(const dyn_md* md);
+
+/* pos_dyn/vel_dyn: 3·n_dyn; pos_water: n_water × (O, M, H0, H1) × 3;
+ * vel_water: n_water × (O, H0, H1) × 3. */
+int dyn_md_set_state(dyn_md* md, const float* pos_dyn, const float* vel_dyn,
+                     const float* pos_water, const float* vel_water);
+int dyn_md_get_state(dyn_md* md, float* pos_dyn, float* vel_dyn, float* pos_water,
+                     float* vel_water);
+
+/* Enqueues n_steps steps (asynchronous; get_state/get_energies wait). */
+int dyn_md_step(dyn_md* md, uint32_t n_steps);
+
+typedef struct dyn_md_energies {
+  double kinetic;       /* kcal/mol, solute atoms and water O/H */
+  double bond, angle, dihedral;  /* kcal/mol at the last force evaluation */
+  double temperature;   /* K from kinetic and dof */
+  double dof;           /* 3·n_dyn − constraints + 6·n_water */
+  uint64_t steps;
+} dyn_md_energies;
+
+int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,127 @@
+"""Device-resident MD step on the GPU against the FP64 restatement of the
+reference step: positions and velocities after 1 and 3 Langevin-middle steps
+of the solvated chain (bonded terms, SHAKE/RATTLE on bonds to hydrogen,
+rigid water); rigid geometry and constraints over many steps; energy
+conservation of the γ = 0 step; the Langevin temperature."""
+
+import numpy as np
+import pytest
+
+from dynamics_amd import api, systems
+from oracle import md_fp64
+
+pytestmark = pytest.mark.gpu
+
+BETA = float(np.float32(0.3123))
+F64 = np.float64
+
+
+def _setup(s, ff, mode=api.MODE_REFERENCE, K=(27, 27, 27), rc=10.0, gamma=1.0, dt=0.002,
+           buffer=1.0, seed=5):
+    eng = api.Engine(s, cutoff=rc, lj_cutoff=rc, beta=BETA, grid=K, order=4, mode=mode,
+                     list_buffer=buffer)
+    m = api.Md(eng, ff, dt=dt, temperature=300.0, gamma=gamma, shake_tol=1e-5, seed=seed)
+    v, vw = systems.initial_velocities(s, ff)
+    m.set_state(s.pos_dyn, v, systems.water_positions(s), vw)
+    return eng, m, v, vw
+
+
+def _min_image(d, box):
+    return d - box * np.round(d / box)
+
+
+def test_steps_vs_fp64(gpu):
+    s = systems.small_solvated()
+    ff = systems.chain_force_field(s)
+    eng, m, v, vw = _setup(s, ff)
+    model = md_fp64.Model(s, ff, 10.0, BETA, (27, 27, 27), dt=0.002, gamma=1.0, seed=5)
+    st = md_fp64.MdState(s.pos_dyn.astype(F64), v.astype(F64),
+                         systems.water_positions(s).astype(F64), vw.astype(F64))
+    box = s.box.astype(F64)
+    for n in (1, 2):
+        m.step(n)
+        for _ in range(n):
+            model.step(st)
+        pd, vd, pw, vwg = m.state()
+        dx_dyn = np.abs(pd - st.x_dyn).max()
+        dx_w = np.abs(_min_image(pw - st.x_w, box)).max()
+        dv = max(np.abs(vd - st.v_dyn).max(), np.abs(vwg - st.v_w).max())
+        print(f"after {st.step} steps: |dx| solute {dx_dyn:.2e} water {dx_w:.2e} A, "
+              f"|dv| {dv:.2e} A/ps", flush=True)
+        assert dx_dyn < 1e-4 and dx_w < 1e-4
+        assert dv < 2e-3
+    m.close()
+    eng.close()
+
+
+def test_geometry_and_constraints_hold(gpu):
+    s = systems.small_solvated()
+    ff = systems.chain_force_field(s)
+    eng, m, _, _ = _setup(s, ff)
+    m.step(200)
+    pd, vd, pw, vwg = m.state()
+    box = s.box.astype(F64)
+    o, mm, h0, h1 = (pw[:, k].astype(F64) for k in range(4))
+    oh0 = np.linalg.norm(_min_image(h0 - o, box), axis=1)
+    oh1 = np.linalg.norm(_min_image(h1 - o, box), axis=1)
+    om = np.linalg.norm(_min_image(mm - o, box), axis=1)
+    assert np.abs(oh0 - 0.87243313).max() < 2e-4 and np.abs(oh1 - 0.87243313).max() < 2e-4
+    assert np.abs(om - 0.15939833).max() < 1e-4
+    i, j = ff.constraints[:, 0].astype(int), ff.constraints[:, 1].astype(int)
+    r = np.linalg.norm(pd[j].astype(F64) - pd[i].astype(F64), axis=1)
+    assert np.abs(r - ff.constraint_r0).max() < 1e-4
+    E = m.energies()
+    assert E["steps"] == 200 and 150 < E["temperature"] < 450
+    assert np.isfinite(pd).all() and np.isfinite(vd).all()
+    m.close()
+    eng.close()
+
+
+def test_nve_energy_conservation(gpu):
+    """γ = 0 with standard-mode (conservative) forces: the total energy drifts
+    far less than the kinetic energy fluctuates."""
+    s = systems.small_water()
+    ff = systems.water_force_field(s)
+    K = api.pme_grid_n(s.box, 1.0)
+    eng, m, _, _ = _setup(s, ff, mode=api.MODE_STANDARD, K=K, gamma=0.0, dt=0.001)
+    tot, kin = [], []
+    for _ in range(20):
+        m.step(20)
+        e = m.energies()
+        pot = eng.energies()["total"]
+        kin.append(e["kinetic"])
+        tot.append(e["kinetic"] + pot)
+    tot, kin = np.array(tot), np.array(kin)
+    print(f"NVE: ptp(E) {np.ptp(tot):.4f}, ptp(KE) {np.ptp(kin):.4f} kcal/mol", flush=True)
+    assert np.ptp(tot) < 0.1 * np.ptp(kin)
+    m.close()
+    eng.close()
+
+
+def test_langevin_temperature(gpu):
+    s = systems.small_water()
+    ff = systems.water_force_field(s)
+    K = api.pme_grid_n(s.box, 1.0)
+    eng, m, _, _ = _setup(s, ff, K=K, gamma=10.0)
+    m.step(1000)
+    T = []
+    for _ in range(20):
+        m.step(50)
+        T.append(m.energies()["temperature"])
+    print(f"Langevin: mean T {np.mean(T):.1f} K", flush=True)
+    assert abs(np.mean(T) - 300.0) < 12.0
+    m.close()
+    eng.close()
+
+
+def test_md_errors(gpu):
+    s = systems.small_water()
+    ff = systems.water_force_field(s)
+    eng = api.Engine(s, cutoff=10.0, beta=BETA, grid=(25, 25, 25), order=4)
+    with pytest.raises(api.DynError):
+        api.Md(eng, ff, dt=0.05)
+    with pytest.raises(api.DynError):
+        api.Md(eng, ff, gamma=-1.0)
+    m = api.Md(eng, ff)
+    with pytest.raises(api.DynError):
+        m.step(1)  # state not set
