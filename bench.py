@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """Throughput of the nonbonded force path on one MI355X.
 
-Workload (one "step"): a full nonbonded evaluation of the solvated-chain
-system — 64 Å cubic box, 8,600 OPC waters around a 1,200-heavy-atom + 1,400-H
-random-walk chain (37,000 sites), 10 Å cutoff, β = 0.3123 Å⁻¹, SPME 64³ order
-4 — with positions already resident in HBM: cluster-pair list rebuilt every
-step, LJ + erfc Coulomb pair tiles, 1-4 / exclusion corrections, SPME.
-ns/day is quoted at dt = 2 fs per step (1 step/s = 0.1728 ns/day) for this
-force path; the integrator is not part of the timed step.
+Workload: the solvated-chain system — 64 Å cubic box, 8,600 OPC waters around
+a 1,200-heavy-atom + 1,400-H random-walk chain (37,000 sites), 10 Å cutoff,
+β = 0.3123 Å⁻¹, SPME 64³ order 4, reference semantics — with everything
+resident in HBM.
+
+--mode md (default): one step is a whole device-resident MD step at dt = 2 fs
+(Langevin-middle, γ = 1/ps, 300 K; rigid water; SHAKE/RATTLE on bonds to
+hydrogen; harmonic bonds and angles and threefold torsions on the chain):
+nonbonded forces with the pair list reused inside a list buffer (rebuilt
+when a site has moved 0.8 × buffer/2), bonded forces, integration.
+--mode eval: one step is one nonbonded evaluation with the list rebuilt every
+time (no integration).
+ns/day = steps/s × dt (1 step/s = 0.1728 ns/day at 2 fs).
 
 Prints one JSON line (rank 0) with the metric, the pair-kernel roofline from
 HIP events over the timed region, and the CPU baseline (the oracle's C++
@@ -38,9 +44,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall time of the CPU-baseline sample (0 skips it)")
-    ap.add_argument("--list-buffer", type=float, default=0.0,
-                    help="list buffer in Å (0: rebuild the list every step)")
+    ap.add_argument("--mode", choices=["md", "eval"], default="md",
+                    help="md: device-resident MD steps; eval: nonbonded evaluations only")
+    ap.add_argument("--list-buffer", type=float, default=None,
+                    help="list buffer in Å (md default 1.0; eval default 0: rebuild every step)")
     args = ap.parse_args()
+    if args.list_buffer is None:
+        args.list_buffer = 1.0 if args.mode == "md" else 0.0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -53,25 +63,52 @@ def main() -> int:
     K = (64, 64, 64)
     eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
                      mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
-    flags = api.COMPUTE_ALL | api.COMPUTE_ASYNC
-    if args.list_buffer <= 0:
-        flags |= api.COMPUTE_REBUILD
-    for _ in range(args.warmup):
-        eng.compute(flags)
-    eng.sync()
-    E = eng.energies()
-    n_both, n_lj, n_coul = E["pair_count_kind"]
-    n_pairs = E["pair_count"] + len(s.pairs_14)
+    # the built chain has close contacts: quench them first (untimed), so that
+    # both modes run on a physical configuration
+    ff = systems.chain_force_field(s)
+    pd0, pw0 = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=400)
+    s.pos_dyn = pd0
+    s.pos_o, s.pos_m, s.pos_h0, s.pos_h1 = (np.ascontiguousarray(pw0[:, k]) for k in range(4))
+    md = None
+    if args.mode == "md":
+        md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=1)
+        v, vw = systems.initial_velocities(s, ff, 300.0)
+        md.set_state(pd0, v, pw0, vw)
+
+        def run(n):
+            md.step(n)
+
+        def wait():
+            md.energies()
+    else:
+        eng.set_positions(s)
+        flags = api.COMPUTE_ALL | api.COMPUTE_ASYNC
+        if args.list_buffer <= 0:
+            flags |= api.COMPUTE_REBUILD
+
+        def run(n):
+            for _ in range(n):
+                eng.compute(flags)
+
+        def wait():
+            eng.sync()
+
+    run(args.warmup)
+    wait()
+    builds0 = eng.energies()["list_builds"]
 
     eng.profile(True)
-    eng.sync()
+    wait()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.compute(flags)
-    eng.sync()
+    run(args.steps)
+    wait()
     t1 = time.perf_counter()
     prof = eng.profile_read()
     eng.profile(False)
+    E = eng.energies()
+    n_both, n_lj, n_coul = E["pair_count_kind"]
+    n_pairs = E["pair_count"] + len(s.pairs_14)
+    md_energies = md.energies() if md is not None else None
 
     sec_per_step = (t1 - t0) / args.steps
     steps_per_s = 1.0 / sec_per_step
@@ -102,12 +139,17 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "chain2600_opc8600_box64: 37,000-site solvated chain, one nonbonded "
-                        "evaluation per step (list rebuild + LJ/erfc-Coulomb pairs + 1-4 "
-                        "corrections + SPME), integrator not included",
+            "workload": ("chain2600_opc8600_box64: 37,000-site solvated chain, one device-resident "
+                         "MD step per step (Langevin-middle, dt 2 fs, rigid water, SHAKE on H "
+                         "bonds, bonded terms, nonbonded LJ/erfc-Coulomb pairs + 1-4 corrections "
+                         "+ SPME)" if md is not None else
+                         "chain2600_opc8600_box64: 37,000-site solvated chain, one nonbonded "
+                         "evaluation per step (list rebuild + LJ/erfc-Coulomb pairs + 1-4 "
+                         "corrections + SPME), integrator not included"),
             "sites": s.n_sites, "cutoff_A": 10.0, "beta_inv_A": 0.3123, "pme_grid": list(K),
-            "pme_order": 4, "mode": "reference", "dt_fs": 2.0,
+            "pme_order": 4, "mode": "reference", "dt_fs": 2.0, "step": args.mode,
             "list": "rebuilt every step" if args.list_buffer <= 0 else f"buffer {args.list_buffer} A",
+            "list_builds_in_timed_steps": E["list_builds"] - builds0,
             "accumulation": "int64 fixed point (2^-32), f64 energy/virial",
         },
         "pair_interactions_per_s": n_pairs * steps_per_s,
@@ -127,6 +169,8 @@ def main() -> int:
         },
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
     }
+    if md_energies is not None:
+        out["md_temperature_K"] = md_energies["temperature"]
     if args.cpu_seconds > 0:
         try:
             from oracle import cpu_baseline

@@ -63,9 +63,9 @@ class MdParams(C.Structure):
 
 
 class MdEnergies(C.Structure):
-    _fields_ = [("kinetic", C.c_double), ("bond", C.c_double), ("angle", C.c_double),
-                ("dihedral", C.c_double), ("temperature", C.c_double), ("dof", C.c_double),
-                ("steps", C.c_uint64)]
+    _fields_ = [("kinetic", C.c_double), ("kinetic_rigid", C.c_double), ("bond", C.c_double),
+                ("angle", C.c_double), ("dihedral", C.c_double), ("temperature", C.c_double),
+                ("dof", C.c_double), ("steps", C.c_uint64)]
 
 
 class EngineParams(C.Structure):
@@ -462,7 +462,7 @@ class Md:
                                len(ff.angles), _ptr(arrs[3]), _ptr(arrs[4]),
                                len(ff.dihedrals), _ptr(arrs[5]), _ptr(arrs[6]),
                                len(ff.constraints), _ptr(arrs[7]), _ptr(arrs[8]))
-        _check(rc, lambda: lib.dyn_global_last_error().decode())
+        _check(rc, lib.dyn_global_last_error)
         self.n_dyn, self.n_water = engine.n_dyn, engine.n_water
 
     def close(self):
@@ -473,7 +473,7 @@ class Md:
     __del__ = close
 
     def _err(self):
-        return lib.dyn_md_last_error(self.h).decode()
+        return lib.dyn_md_last_error(self.h)
 
     def set_state(self, pos_dyn, vel_dyn, pos_water, vel_water):
         """pos_water (n_water, 4, 3) in O, M, H0, H1 order; vel_water
@@ -497,3 +497,17 @@ class Md:
         e = MdEnergies()
         _check(lib.dyn_md_get_energies(self.h, C.byref(e)), self._err)
         return {k: getattr(e, k) for k, _ in MdEnergies._fields_}
+
+
+def relax(engine, ff, pos_dyn, pos_water, steps=300, dt=0.002):
+    """Removes close contacts of a built structure by an overdamped quench:
+    MD at 0 K with friction 10⁴/ps (velocities are reset every step, so each
+    step moves atoms along their clamped accelerations, as steepest descent).
+    Returns relaxed (pos_dyn, pos_water)."""
+    m = Md(engine, ff, dt=dt, temperature=0.0, gamma=1.0e4, seed=0)
+    nd, nw = engine.n_dyn, engine.n_water
+    m.set_state(pos_dyn, np.zeros((nd, 3), np.float32), pos_water, np.zeros((nw, 3, 3), np.float32))
+    m.step(steps)
+    pd, _, pw, _ = m.state()
+    m.close()
+    return pd, pw

@@ -933,18 +933,47 @@ __global__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
 // are cleared here.
 constexpr int kRedBlock = 1024;
 constexpr int kEngOut = 20;
+constexpr int kRed1Grid = 128;   // first-stage workgroups (fixed: part of the sum's order)
+constexpr int kRed1Block = 256;
+
+// First stage of the pair-partial sums: workgroup b sums a fixed contiguous
+// share of the work items (kNumPart values each) into red1[b].
+__global__ void __launch_bounds__(kRed1Block)
+k_eng_energy_reduce1(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
+                     uint64_t stride, double* __restrict__ red1) {
+  __shared__ double red[kNumPart][kRed1Block / kWave];
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const uint32_t n_items = (uint32_t)min((uint64_t)*n_items_dev, stride);
+  const uint32_t per = (n_items + kRed1Grid - 1) / kRed1Grid;
+  const uint32_t r0 = blockIdx.x * per, r1 = min(n_items, r0 + per);
+  double a[kNumPart];
+  for (int v = 0; v < kNumPart; ++v) a[v] = 0.0;
+  for (uint32_t r = r0 + t; r < r1; r += kRed1Block)
+    for (int v = 0; v < kNumPart; ++v) a[v] += pp[(size_t)v * stride + r];
+  for (int v = 0; v < kNumPart; ++v) {
+    double x = a[v];
+    for (int m = 1; m < kWave; m <<= 1) x += __shfl_xor(x, m, kWave);
+    if (lane == 0) red[v][w] = x;
+  }
+  __syncthreads();
+  if (t < kNumPart) {
+    double x = 0.0;
+    for (int k = 0; k < kRed1Block / kWave; ++k) x += red[t][k];
+    red1[(size_t)blockIdx.x * kNumPart + t] = x;
+  }
+}
+
 __global__ void __launch_bounds__(kRedBlock)
-k_eng_energy_reduce(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
-                    uint64_t stride, const double* __restrict__ cp, uint32_t n_cblk,
-                    int pme_runs, double* __restrict__ out) {
+k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ cp,
+                    uint32_t n_cblk, int pme_runs, double* __restrict__ out) {
   constexpr int kVals = kNumPart + 6;
   __shared__ double red[kVals][kRedBlock / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   double a[kVals];
   for (int v = 0; v < kVals; ++v) a[v] = 0.0;
-  const uint32_t n_items = pp ? (uint32_t)min((uint64_t)*n_items_dev, stride) : 0u;
-  for (uint32_t r = t; r < n_items; r += kRedBlock)
-    for (int v = 0; v < kNumPart; ++v) a[v] += pp[(size_t)v * stride + r];
+  if (red1)
+    for (uint32_t r = t; r < (uint32_t)kRed1Grid; r += kRedBlock)
+      for (int v = 0; v < kNumPart; ++v) a[v] += red1[(size_t)r * kNumPart + v];
   for (uint32_t r = t; r < n_cblk; r += kRedBlock)
     for (int v = 0; v < 6; ++v) a[kNumPart + v] += cp[(size_t)r * 6 + v];
   for (int v = 0; v < kVals; ++v) {
@@ -1037,7 +1066,7 @@ struct dyn_engine {
   uint64_t n_builds = 0;
   // accumulators
   DevBuf<int64_t> fitem, fx_site, fx_out, fx_slot;  // fx_slot: inspection only
-  DevBuf<double> pair_part, corr_part, eng_out;
+  DevBuf<double> pair_part, corr_part, eng_out, red1;
   unsigned corr_blocks = 0;
   DevBuf<unsigned int> disp_bits;
   // displacement readings of asynchronous computes (squared Å, bit pattern
@@ -1394,9 +1423,13 @@ void dyn_engine::enqueue(uint32_t flags) {
                        fx_site.p, short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
                        cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
     DYN_CHECK_LAUNCH("k_eng_finalize");
+    if (short_on) {
+      hipLaunchKernelGGL(k_eng_energy_reduce1, dim3(kRed1Grid), dim3(kRed1Block), 0, stream,
+                         (const double*)pair_part.p, counters.p + 2, cap_chunks, red1.p);
+      DYN_CHECK_LAUNCH("k_eng_energy_reduce1");
+    }
     hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
-                       short_on ? pair_part.p : (const double*)nullptr, counters.p + 2, cap_chunks,
-                       corr_part.p,
+                       short_on ? (const double*)red1.p : (const double*)nullptr, corr_part.p,
                        corr_on ? corr_blocks : 0u, recip_on ? (recip_b ? 2 : 1) : 0, eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
@@ -1604,6 +1637,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->fx_out.reserve(3 * (size_t)ns);
     E->h_fx.reserve(3 * (size_t)ns);
     E->eng_out.reserve(kEngOut);
+    E->red1.reserve((size_t)kRed1Grid * kNumPart);
     E->h_eng.reserve(kEngOut);
     E->h_misc.reserve(8);
     E->counters.reserve(8);

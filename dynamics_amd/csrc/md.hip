@@ -425,39 +425,89 @@ __global__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
   if (i < n) f[i] = 0;
 }
 
-// Kinetic energy (amu Å²/ps²) of solute atoms and water O/H, per block; and
-// the reduction of the bonded-energy partials.  out: [0] kinetic,
-// [1..3] bond, angle, dihedral.
+// Kinetic energies (amu Å²/ps²) and the bonded-energy partials, reduced by
+// one workgroup.  out: [0] kinetic over solute atoms and water O/H site
+// velocities (the reference's measure, src/thermostat.rs:46-63), [1..3]
+// bond, angle, dihedral energy, [4] kinetic of the rigid-body motion (solute
+// atoms as they are; per water the centre-of-mass and rotational part,
+// ½ M v_com² + ½ ω·L with I ω = L about the centre of mass).
 __global__ void __launch_bounds__(1024)
-k_md_energies(const float4* __restrict__ v, const float* __restrict__ mass, uint32_t n_dyn,
-              uint32_t n_water, const double* __restrict__ bpart, uint32_t n_bblk,
-              double* __restrict__ out) {
-  __shared__ double red[4][1024];
+k_md_energies(const float4* __restrict__ x, const float4* __restrict__ v,
+              const float* __restrict__ mass, uint32_t n_dyn, uint32_t n_water, Box B,
+              const double* __restrict__ bpart, uint32_t n_bblk, double* __restrict__ out) {
+  __shared__ double red[5][1024];
   const uint32_t t = threadIdx.x;
-  double a[4] = {0, 0, 0, 0};
+  double a[5] = {0, 0, 0, 0, 0};
   for (uint32_t i = t; i < n_dyn; i += 1024) {
     const float4 u = v[i];
-    a[0] += 0.5 * (double)mass[i] * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+    const double k = 0.5 * (double)mass[i] * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+    a[0] += k;
+    a[4] += k;
   }
   for (uint32_t m = t; m < n_water; m += 1024) {
     const uint32_t b = n_dyn + 4 * m;
     const uint32_t ss[3] = {b, b + 2, b + 3};
     const double ms[3] = {kMassO, kMassH, kMassH};
+    double r[3][3], w[3][3], vc[3] = {0, 0, 0}, rc[3] = {0, 0, 0};
+    const float4 o = x[b];
     for (int k = 0; k < 3; ++k) {
-      const float4 u = v[ss[k]];
-      a[0] += 0.5 * ms[k] * ((double)u.x * u.x + (double)u.y * u.y + (double)u.z * u.z);
+      const float4 p = x[ss[k]], u = v[ss[k]];
+      const V3 d = min_image(v3(p.x - o.x, p.y - o.y, p.z - o.z), B);
+      r[k][0] = d.x;
+      r[k][1] = d.y;
+      r[k][2] = d.z;
+      w[k][0] = u.x;
+      w[k][1] = u.y;
+      w[k][2] = u.z;
+      a[0] += 0.5 * ms[k] * (w[k][0] * w[k][0] + w[k][1] * w[k][1] + w[k][2] * w[k][2]);
+      for (int c = 0; c < 3; ++c) {
+        vc[c] += ms[k] * w[k][c] / (double)kMassW;
+        rc[c] += ms[k] * r[k][c] / (double)kMassW;
+      }
     }
+    double L[3] = {0, 0, 0}, I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 3; ++k) {
+      double q[3], u[3];
+      for (int c = 0; c < 3; ++c) {
+        q[c] = r[k][c] - rc[c];
+        u[c] = w[k][c] - vc[c];
+      }
+      L[0] += ms[k] * (q[1] * u[2] - q[2] * u[1]);
+      L[1] += ms[k] * (q[2] * u[0] - q[0] * u[2]);
+      L[2] += ms[k] * (q[0] * u[1] - q[1] * u[0]);
+      const double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+      for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e) I[c][e] += ms[k] * ((c == e ? q2 : 0.0) - q[c] * q[e]);
+    }
+    const double det = I[0][0] * (I[1][1] * I[2][2] - I[1][2] * I[2][1]) -
+                       I[0][1] * (I[1][0] * I[2][2] - I[1][2] * I[2][0]) +
+                       I[0][2] * (I[1][0] * I[2][1] - I[1][1] * I[2][0]);
+    double krot = 0.0;
+    if (fabs(det) > 1e-18) {
+      const double inv[3][3] = {
+          {(I[1][1] * I[2][2] - I[1][2] * I[2][1]) / det, (I[0][2] * I[2][1] - I[0][1] * I[2][2]) / det,
+           (I[0][1] * I[1][2] - I[0][2] * I[1][1]) / det},
+          {(I[1][2] * I[2][0] - I[1][0] * I[2][2]) / det, (I[0][0] * I[2][2] - I[0][2] * I[2][0]) / det,
+           (I[0][2] * I[1][0] - I[0][0] * I[1][2]) / det},
+          {(I[1][0] * I[2][1] - I[1][1] * I[2][0]) / det, (I[0][1] * I[2][0] - I[0][0] * I[2][1]) / det,
+           (I[0][0] * I[1][1] - I[0][1] * I[1][0]) / det}};
+      for (int c = 0; c < 3; ++c) {
+        const double om = inv[c][0] * L[0] + inv[c][1] * L[1] + inv[c][2] * L[2];
+        krot += 0.5 * om * L[c];
+      }
+    }
+    a[4] += 0.5 * (double)kMassW * (vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2]) + krot;
   }
   for (uint32_t r = t; r < n_bblk; r += 1024)
     for (int c = 0; c < 3; ++c) a[1 + c] += bpart[3 * r + c];
-  for (int c = 0; c < 4; ++c) red[c][t] = a[c];
+  for (int c = 0; c < 5; ++c) red[c][t] = a[c];
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
-    if (t < (uint32_t)s)
-      for (int c = 0; c < 4; ++c) red[c][t] += red[c][t + s];
+  for (int s2 = 512; s2 > 0; s2 >>= 1) {
+    if (t < (uint32_t)s2)
+      for (int c = 0; c < 5; ++c) red[c][t] += red[c][t + s2];
     __syncthreads();
   }
-  if (t < 4) out[t] = red[t][0];
+  if (t < 5) out[t] = red[t][0];
 }
 
 }  // namespace
@@ -737,8 +787,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     up(M->cons_invm, invm.data(), invm.size(), 4);
     M->x.reserve(ns);
     M->v.reserve(ns);
-    M->eout.reserve(4);
-    M->h_out.reserve(4);
+    M->eout.reserve(8);
+    M->h_out.reserve(8);
     DYN_HIP(hipStreamSynchronize(M->stream));
     *out = M.release();
     return DYN_OK;
@@ -847,16 +897,19 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     DYN_REQUIRE(out, "out is null");
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
     if (!md->forces_valid) md->forces();
-    hipLaunchKernelGGL(k_md_energies, dim3(1), dim3(1024), 0, md->stream, md->v.p, md->mass.p,
-                       md->n_dyn, md->n_water, md->bpart.p,
+    Box B;
+    for (int a = 0; a < 3; ++a) B.L[a] = md->box[a];
+    hipLaunchKernelGGL(k_md_energies, dim3(1), dim3(1024), 0, md->stream, md->x.p, md->v.p,
+                       md->mass.p, md->n_dyn, md->n_water, B, md->bpart.p,
                        (md->n_bond + md->n_angle + md->n_dihe) ? md->bblocks : 0u, md->eout.p);
     DYN_CHECK_LAUNCH("k_md_energies");
-    DYN_HIP(hipMemcpyAsync(md->h_out.p, md->eout.p, 4 * sizeof(double), hipMemcpyDeviceToHost,
+    DYN_HIP(hipMemcpyAsync(md->h_out.p, md->eout.p, 5 * sizeof(double), hipMemcpyDeviceToHost,
                            md->stream));
     engine_call(dyn_engine_sync(md->eng), md->eng);
     DYN_HIP(hipStreamSynchronize(md->stream));
     dyn_md_energies E{};
     E.kinetic = md->h_out.p[0] / (double)kKcalToNative;
+    E.kinetic_rigid = md->h_out.p[4] / (double)kKcalToNative;
     E.bond = md->h_out.p[1];
     E.angle = md->h_out.p[2];
     E.dihedral = md->h_out.p[3];
@@ -864,7 +917,7 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     // rigid water
     const double dof = 3.0 * md->n_dyn - (double)md->n_cons + 6.0 * md->n_water;
     E.dof = dof;
-    E.temperature = dof > 0 ? 2.0 * md->h_out.p[0] / (dof * (double)kKb) : 0.0;
+    E.temperature = dof > 0 ? 2.0 * md->h_out.p[4] / (dof * (double)kKb) : 0.0;
     E.steps = md->step;
     *out = E;
   });

@@ -419,59 +419,73 @@ __device__ __forceinline__ int home_bricks(int lo, int hi, int K, int out[3]) {
 
 // One workgroup per brick: every charge of the reaching home bricks adds its
 // stencil points inside the brick to an LDS brick (int64 LDS adds of the
-// same fixed-point values the replay sums), which is then stored once.
+// same fixed-point values the replay sums), which is then stored once.  The
+// home bricks' charge lists are walked as one flattened index space.
 template <int ORDER>
 __global__ void __launch_bounds__(kBrickBlock)
 k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t nbx,
              uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ start,
              const uint32_t* __restrict__ items, int64_t* __restrict__ grid) {
   __shared__ unsigned long long acc[kBrick * kBrick * kBrick];
+  __shared__ uint32_t s_c0[27], s_beg[28];
   const uint32_t b = blockIdx.x;
   const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), bx = (int)(b / (nbz * nby));
   const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
   const int hi[3] = {min(lo[0] + kBrick, S.K[0]) - 1, min(lo[1] + kBrick, S.K[1]) - 1,
                      min(lo[2] + kBrick, S.K[2]) - 1};
   for (int k = threadIdx.x; k < kBrick * kBrick * kBrick; k += kBrickBlock) acc[k] = 0ull;
-  int hb[3][3], nh[3];
-  for (int a = 0; a < 3; ++a) nh[a] = home_bricks<ORDER>(lo[a], hi[a], S.K[a], hb[a]);
+  if (threadIdx.x == 0) {
+    int hb[3][3], nh[3];
+    for (int a = 0; a < 3; ++a) nh[a] = home_bricks<ORDER>(lo[a], hi[a], S.K[a], hb[a]);
+    uint32_t tot = 0;
+    int q = 0;
+    for (int ix = 0; ix < nh[0]; ++ix)
+      for (int iy = 0; iy < nh[1]; ++iy)
+        for (int iz = 0; iz < nh[2]; ++iz) {
+          const uint32_t h = ((uint32_t)hb[0][ix] * nby + (uint32_t)hb[1][iy]) * nbz + (uint32_t)hb[2][iz];
+          const uint32_t c0 = min(start[h], n), c1 = min(start[h + 1], n);
+          s_c0[q] = c0;
+          s_beg[q] = tot;
+          tot += c1 > c0 ? c1 - c0 : 0u;
+          ++q;
+        }
+    for (; q < 28; ++q) s_beg[q] = tot;
+  }
   __syncthreads();
-  for (int ix = 0; ix < nh[0]; ++ix)
-    for (int iy = 0; iy < nh[1]; ++iy)
-      for (int iz = 0; iz < nh[2]; ++iz) {
-        const uint32_t h = ((uint32_t)hb[0][ix] * nby + (uint32_t)hb[1][iy]) * nbz + (uint32_t)hb[2][iz];
-        const uint32_t c0 = min(start[h], n), c1 = min(start[h + 1], n);
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += kBrickBlock) {
-          const uint32_t ci = items[c];
-          if (ci >= n) continue;
-          const float4 p = xyzq[ci];
-          if (p.w == 0.0f) continue;
-          int k0[3];
-          float th[3][ORDER], dth[ORDER];
-          const float xs[3] = {p.x, p.y, p.z};
-          for (int a = 0; a < 3; ++a) {
-            float w;
-            frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
-            dyn_bspline<ORDER>(w, th[a], dth);
-          }
-          for (int jx = 0; jx < ORDER; ++jx) {
-            const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
-            if (gx < lo[0] || gx > hi[0]) continue;
-            const float qx = p.w * th[0][jx];
-            for (int jy = 0; jy < ORDER; ++jy) {
-              const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
-              if (gy < lo[1] || gy > hi[1]) continue;
-              const float qxy = qx * th[1][jy];
-              for (int jz = 0; jz < ORDER; ++jz) {
-                const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
-                if (gz < lo[2] || gz > hi[2]) continue;
-                const float v = qxy * th[2][jz];
-                const int li = ((gx - lo[0]) * kBrick + (gy - lo[1])) * kBrick + (gz - lo[2]);
-                atomicAdd(&acc[li], (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
-              }
-            }
-          }
+  const uint32_t total = s_beg[27];
+  for (uint32_t t = threadIdx.x; t < total; t += kBrickBlock) {
+    int q = 0;
+    while (q < 26 && s_beg[q + 1] <= t) ++q;
+    const uint32_t ci = items[s_c0[q] + (t - s_beg[q])];
+    if (ci >= n) continue;
+    const float4 p = xyzq[ci];
+    if (p.w == 0.0f) continue;
+    int k0[3];
+    float th[3][ORDER], dth[ORDER];
+    const float xs[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      float w;
+      frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
+      dyn_bspline<ORDER>(w, th[a], dth);
+    }
+    for (int jx = 0; jx < ORDER; ++jx) {
+      const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
+      if (gx < lo[0] || gx > hi[0]) continue;
+      const float qx = p.w * th[0][jx];
+      for (int jy = 0; jy < ORDER; ++jy) {
+        const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
+        if (gy < lo[1] || gy > hi[1]) continue;
+        const float qxy = qx * th[1][jy];
+        for (int jz = 0; jz < ORDER; ++jz) {
+          const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
+          if (gz < lo[2] || gz > hi[2]) continue;
+          const float v = qxy * th[2][jz];
+          const int li = ((gx - lo[0]) * kBrick + (gy - lo[1])) * kBrick + (gz - lo[2]);
+          atomicAdd(&acc[li], (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
         }
       }
+    }
+  }
   __syncthreads();
   for (int k = threadIdx.x; k < kBrick * kBrick * kBrick; k += kBrickBlock) {
     const int x = lo[0] + k / (kBrick * kBrick), y = lo[1] + (k / kBrick) % kBrick,

@@ -278,6 +278,25 @@ def solvated_chain(box: float, n_water: int, n_heavy: int, n_h: int, seed: int =
     two[rng.permutation(n_heavy)[:extra]] = True
     hpos = []
     hparent = []
+    from scipy.spatial import cKDTree  # local import: scipy only needed here
+    heavy_tree = cKDTree(_wrap(heavy, boxv), boxsize=boxv)
+
+    def clearance(cands, i):
+        """Smallest distance from the candidate H positions to atoms they do
+        not share a bond or angle with (heavy atoms beyond i ± 1, hydrogens
+        of other heavy atoms)."""
+        worst = np.inf
+        for q in cands:
+            for j in heavy_tree.query_ball_point(_wrap(q[None], boxv)[0], 3.0):
+                if abs(j - i) > 1:
+                    worst = min(worst, float(np.linalg.norm(_min_image(q - heavy[j], boxv))))
+            if hpos:
+                d = _min_image(np.asarray(hpos) - q, boxv)
+                other = np.asarray(hparent) != i
+                if other.any():
+                    worst = min(worst, float(np.sqrt(np.einsum("ij,ij->i", d, d)[other].min())))
+        return worst
+
     for i in range(n_heavy):
         a = heavy[i - 1] if i > 0 else heavy[i] - (heavy[i + 1] - heavy[i])
         c = heavy[i + 1] if i + 1 < n_heavy else heavy[i] - (heavy[i - 1] - heavy[i])
@@ -288,15 +307,25 @@ def solvated_chain(box: float, n_water: int, n_heavy: int, n_h: int, seed: int =
         nn = np.linalg.norm(n)
         n = n / nn if nn > 1e-6 else np.array([1.0, 0.0, 0.0])
         ang = math.radians(54.75)
-        d_plus = b * math.cos(ang) + n * math.sin(ang)
-        d_minus = b * math.cos(ang) - n * math.sin(ang)
-        if two[i]:
-            hpos += [heavy[i] + 1.09 * d_plus, heavy[i] + 1.09 * d_minus]
-            hparent += [i, i]
-        else:
-            pick = rng.uniform(1)[0] < 0.5
-            hpos.append(heavy[i] + 1.09 * (d_plus if pick else d_minus))
-            hparent.append(i)
+        m = np.cross(b, n)
+        pick = two[i] or rng.uniform(1)[0] < 0.5
+        # H directions at 54.75° from the bisector; rotated about it in 10°
+        # steps until they clear other atoms by 2.2 Å (else the best found)
+        best, best_c = None, -1.0
+        for k in range(36):
+            rot = math.radians(10.0 * k)
+            nr = n * math.cos(rot) + m * math.sin(rot)
+            d_plus = b * math.cos(ang) + nr * math.sin(ang)
+            d_minus = b * math.cos(ang) - nr * math.sin(ang)
+            cands = ([heavy[i] + 1.09 * d_plus, heavy[i] + 1.09 * d_minus] if two[i]
+                     else [heavy[i] + 1.09 * (d_plus if pick else d_minus)])
+            cl = clearance(cands, i)
+            if cl > best_c:
+                best, best_c = cands, cl
+            if cl >= 2.2:
+                break
+        hpos += best
+        hparent += [i] * len(best)
     hpos = np.array(hpos)
     # atom order: heavy atoms, then hydrogens
     pos_dyn = np.concatenate([heavy, hpos])

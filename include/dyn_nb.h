@@ -338,9 +338,12 @@ int dyn_md_get_state(dyn_md* md, float* pos_dyn, float* vel_dyn, float* pos_wate
 int dyn_md_step(dyn_md* md, uint32_t n_steps);
 
 typedef struct dyn_md_energies {
-  double kinetic;       /* kcal/mol, solute atoms and water O/H */
+  double kinetic;       /* kcal/mol over solute atoms and water O/H site velocities,
+                           as the reference measures it (src/thermostat.rs:46-63) */
+  double kinetic_rigid; /* kcal/mol of the rigid-body motion: solute atoms, and per
+                           water its centre-of-mass and rotational kinetic energy */
   double bond, angle, dihedral;  /* kcal/mol at the last force evaluation */
-  double temperature;   /* K from kinetic and dof */
+  double temperature;   /* K from kinetic_rigid and dof */
   double dof;           /* 3·n_dyn − constraints + 6·n_water */
   uint64_t steps;
 } dyn_md_energies;

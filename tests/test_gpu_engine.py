@@ -3,6 +3,7 @@
 the GPU-built list; forces within 1e-5 relative RMS and energies against the
 FP64 oracle of the same Hamiltonian, in reference and standard modes."""
 
+import copy
 import os
 import re
 
@@ -215,10 +216,31 @@ def _full_size_check(s, rc, K, mode=api.MODE_REFERENCE):
     assert net < 1e-2 * np.sqrt(len(F)), net
 
 
+_RELAXED = {}
+
+
+def relaxed_config1():
+    """The 37,000-site solvated chain after the benchmark's quench of its
+    built close contacts (the configuration the benchmark starts from)."""
+    if "c1" not in _RELAXED:
+        s = systems.config1()
+        ff = systems.chain_force_field(s)
+        eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=BETA, grid=(64, 64, 64), order=4,
+                         list_buffer=1.0)
+        pd, pw = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=400)
+        eng.close()
+        r = copy.copy(s)
+        r.pos_dyn = pd
+        r.pos_o, r.pos_m, r.pos_h0, r.pos_h1 = (np.ascontiguousarray(pw[:, k]) for k in range(4))
+        _RELAXED["c1"] = r
+    return _RELAXED["c1"]
+
+
 def test_bench_workload_vs_fp64(gpu):
     """The 37,000-site solvated chain at the benchmark's settings (rc 10 Å,
-    SPME 64³ order 4, reference mode, list rebuilt)."""
-    s = systems.config1()
+    SPME 64³ order 4, reference mode, list rebuilt), from the benchmark's
+    relaxed start."""
+    s = relaxed_config1()
     print(f"bench workload parity: sites={s.n_sites}", flush=True)
     _full_size_check(s, 10.0, (64, 64, 64))
 
@@ -267,7 +289,7 @@ def test_alchemical_vs_fp64(gpu, cases, lam):
 
 def test_alchemical_bench_system(gpu):
     """40 decoupled atoms of the 37,000-site solvated chain at λ = 0.6."""
-    s = systems.config1()
+    s = relaxed_config1()
     _alch_check(s, 0, 40, 0.6, (64, 64, 64))
 
 
