@@ -1965,6 +1965,7 @@ extern "C" int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char
 // Internal accessors (engine_internal.h)
 // ---------------------------------------------------------------------------
 namespace dyn {
+Profiler* engine_profiler(dyn_engine* eng) { return &eng->prof; }
 hipStream_t engine_stream(const dyn_engine* eng) { return eng->stream; }
 int engine_device(const dyn_engine* eng) { return eng->device; }
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water) {

@@ -8,6 +8,8 @@
 struct dyn_engine;
 
 namespace dyn {
+struct Profiler;
+Profiler* engine_profiler(dyn_engine* eng);
 hipStream_t engine_stream(const dyn_engine* eng);
 int engine_device(const dyn_engine* eng);
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water);

@@ -560,6 +560,7 @@ struct dyn_md {
 void dyn_md::forces() {
   engine_call(dyn_engine_compute(eng, DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u)),
               eng);
+  Profiler::Scope ps(*engine_profiler(eng), "md_bonded");
   hipLaunchKernelGGL(k_md_zero, dim3(grid_for(3 * (size_t)std::max(n_dyn, 1u), kBlock)), dim3(kBlock),
                      0, stream, fb.p, 3 * (size_t)n_dyn);
   DYN_CHECK_LAUNCH("k_md_zero");
@@ -586,6 +587,7 @@ void dyn_md::forces() {
 }
 
 void dyn_md::integrate(float hk, float hd, bool kick, bool drift) {
+  Profiler::Scope ps(*engine_profiler(eng), "md_integrate");
   IntArgs A;
   A.F.nb = dyn_engine_forces_device(eng);
   A.F.bd = fb.p;
@@ -618,6 +620,7 @@ static ConsArgs cons_args(const dyn_md& M) {
 
 void dyn_md::constrain_positions() {
   if (!n_clusters) return;
+  Profiler::Scope ps(*engine_profiler(eng), "md_constraints");
   hipLaunchKernelGGL(k_md_shake, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
                      cons_args(*this), x.p);
   DYN_CHECK_LAUNCH("k_md_shake");
@@ -625,6 +628,7 @@ void dyn_md::constrain_positions() {
 
 void dyn_md::constrain_velocities() {
   if (!n_clusters) return;
+  Profiler::Scope ps(*engine_profiler(eng), "md_constraints");
   hipLaunchKernelGGL(k_md_rattle, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
                      cons_args(*this), (const float4*)x.p, v.p);
   DYN_CHECK_LAUNCH("k_md_rattle");
@@ -639,10 +643,13 @@ void dyn_md::one_step() {
     integrate(h, h, true, true);
     constrain_positions();
     constrain_velocities();
-    hipLaunchKernelGGL(k_md_langevin, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
-                       stream, v.p, n_dyn, n_water, lang_c, lang_sigma.p, sigma_o, sigma_h,
-                       prm.seed, step);
-    DYN_CHECK_LAUNCH("k_md_langevin");
+    {
+      Profiler::Scope ps(*engine_profiler(eng), "md_langevin");
+      hipLaunchKernelGGL(k_md_langevin, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
+                         stream, v.p, n_dyn, n_water, lang_c, lang_sigma.p, sigma_o, sigma_h,
+                         prm.seed, step);
+      DYN_CHECK_LAUNCH("k_md_langevin");
+    }
     constrain_velocities();
     integrate(0.f, h, false, true);
     constrain_positions();
