@@ -30,6 +30,7 @@ COMPUTE_CORR = 4
 COMPUTE_ALL = 7
 COMPUTE_REBUILD = 8
 COMPUTE_ASYNC = 16
+COMPUTE_NO_ENERGY = 32
 
 
 class DynError(RuntimeError):

@@ -621,7 +621,9 @@ struct TileAcc {
 
 // ALCH: the group holds at least one lane across the decoupled molecule's
 // boundary (the alchemical variant returns the plain result on the others).
-template <int KIND, bool ALCH>
+// ENERGY: accumulate energies, virial, dH/dλ and pair counts (forces only
+// otherwise).
+template <int KIND, bool ALCH, bool ENERGY>
 __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4], int ti,
                                            const float4& pjv, int tj, bool active, TileAcc& T) {
   const float pj[4] = {pjv.x, pjv.y, pjv.z, pjv.w};
@@ -631,13 +633,14 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
     float dh;
     dyn_tile_pair_alch<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, A.lj_se, d, &s, &elj, &ec,
                              &r2, &hit, &dh);
-    T.dh += dh;
+    if (ENERGY) T.dh += dh;
   } else {
     dyn_tile_pair<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, d, &s, &elj, &ec, &r2, &hit);
   }
   T.fi[0] += s * d[0];
   T.fi[1] += s * d[1];
   T.fi[2] += s * d[2];
+  if (!ENERGY) return;
   const bool solute = KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_SQ;
   if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) {
     if (solute) T.e_lj_s += elj;
@@ -661,6 +664,7 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
 // energies, virial and pair counts as one partial per value; the grid has a
 // wave per work item, so the hardware's wave dispatch balances the load and
 // every sum keeps a fixed order.
+template <bool ENERGY>
 __global__ void __launch_bounds__(kPairBlock)
 k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -705,19 +709,19 @@ k_eng_pairs(PairArgs A) {
       const bool alch = A.P.alch_on && kind <= DYN_KIND_SQ &&
                         __ballot(((ti ^ tjc) & DYN_TYPE_ALCH) != 0) != 0ull;
       switch (kind) {
-        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false, ENERGY>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false, ENERGY>(A, pi, ti, pjc, tjc, active, T); break;
         case DYN_KIND_SS:
-          if (alch) tile_entry<DYN_KIND_SS, true>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SS, false>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SS, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SS, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
           break;
         case DYN_KIND_SO:
-          if (alch) tile_entry<DYN_KIND_SO, true>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SO, false>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SO, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SO, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
           break;
         case DYN_KIND_SQ:
-          if (alch) tile_entry<DYN_KIND_SQ, true>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SQ, false>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SQ, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SQ, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
           break;
         default: break;
       }
@@ -736,6 +740,7 @@ k_eng_pairs(PairArgs A) {
       o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
       o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
     }
+    if (!ENERGY) continue;
     double acc[6] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir, T.dh};
     for (int k = 0; k < 6; ++k) {
       double x = acc[k];
@@ -1019,6 +1024,7 @@ struct dyn_engine {
   int type_o = 0, type_none = 0;
   float box[3] = {0, 0, 0};
   bool box_set = false, pos_set = false, list_valid = false, computed = false;
+  bool energies_valid = false;  // the last compute evaluated energies
   double e_self = 0.0, q_total = 0.0;
   // topology
   DevBuf<float> q;
@@ -1347,6 +1353,7 @@ void dyn_engine::enqueue(uint32_t flags) {
   P.alch_on = alch_on ? 1 : 0;
   P.sch = dyn_schedule((float)lambda);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
+  const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
   const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
   const bool corr_on = (flags & DYN_COMPUTE_CORR) && n_corr;
   // with a decoupled molecule the SPME sum runs twice, with its charges as
@@ -1383,7 +1390,10 @@ void dyn_engine::enqueue(uint32_t flags) {
     A.status = status.p;
     // a wave per work item (the count is known on the device only)
     const unsigned blocks = (unsigned)std::max<uint64_t>(1, (cap_chunks + 3) / 4);
-    hipLaunchKernelGGL(k_eng_pairs, dim3(blocks), dim3(kPairBlock), 0, stream, A);
+    if (energy_on)
+      hipLaunchKernelGGL(k_eng_pairs<true>, dim3(blocks), dim3(kPairBlock), 0, stream, A);
+    else
+      hipLaunchKernelGGL(k_eng_pairs<false>, dim3(blocks), dim3(kPairBlock), 0, stream, A);
     DYN_CHECK_LAUNCH("k_eng_pairs");
   }
   if (corr_on) {
@@ -1423,17 +1433,19 @@ void dyn_engine::enqueue(uint32_t flags) {
                        fx_site.p, short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
                        cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
     DYN_CHECK_LAUNCH("k_eng_finalize");
-    if (short_on) {
+    if (short_on && energy_on) {
       hipLaunchKernelGGL(k_eng_energy_reduce1, dim3(kRed1Grid), dim3(kRed1Block), 0, stream,
                          (const double*)pair_part.p, counters.p + 2, cap_chunks, red1.p);
       DYN_CHECK_LAUNCH("k_eng_energy_reduce1");
     }
+    if (energy_on)
     hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
                        short_on ? (const double*)red1.p : (const double*)nullptr, corr_part.p,
                        corr_on ? corr_blocks : 0u, recip_on ? (recip_b ? 2 : 1) : 0, eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
   computed = true;
+  energies_valid = energy_on;
   fx_fresh = false;
 }
 
@@ -1851,6 +1863,8 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     DeviceGuard g(eng->device);
     DYN_REQUIRE(out, "out is null");
     if (!eng->computed) throw Error(DYN_ERR_STATE, "no compute yet");
+    if (!eng->energies_valid)
+      throw Error(DYN_ERR_STATE, "the last compute skipped energies (DYN_COMPUTE_NO_ENERGY)");
     eng->check_status();
     DYN_HIP(hipMemcpyAsync(eng->h_eng.p, eng->eng_out.p, kEngOut * sizeof(double),
                            hipMemcpyDeviceToHost, eng->stream));

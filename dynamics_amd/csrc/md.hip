@@ -547,19 +547,22 @@ struct dyn_md {
   uint64_t step = 0;
   bool state_set = false, forces_valid = false;
 
-  void forces();
+  void forces(bool energies);
   void integrate(float hk, float hd, bool kick, bool drift);
   void constrain_positions();
   void constrain_velocities();
-  void one_step();
+  void one_step(bool energies);
 };
 
 // The first evaluation after set_state is synchronous, so that a list that
 // outgrows its buffers is rebuilt at once; later ones are enqueued only (a
-// failure is reported at the next get_state / get_energies).
-void dyn_md::forces() {
-  engine_call(dyn_engine_compute(eng, DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u)),
-              eng);
+// failure is reported at the next get_state / get_energies).  Nonbonded
+// energies are evaluated only when asked (the last step of a dyn_md_step
+// call), the forces alone otherwise.
+void dyn_md::forces(bool energies) {
+  const uint32_t flags = DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u) |
+                         (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
+  engine_call(dyn_engine_compute(eng, flags), eng);
   Profiler::Scope ps(*engine_profiler(eng), "md_bonded");
   hipLaunchKernelGGL(k_md_zero, dim3(grid_for(3 * (size_t)std::max(n_dyn, 1u), kBlock)), dim3(kBlock),
                      0, stream, fb.p, 3 * (size_t)n_dyn);
@@ -636,9 +639,9 @@ void dyn_md::constrain_velocities() {
 
 // One step, as src/integrate.rs:100-190 (γ > 0, Langevin-middle) or
 // :191-260 (γ = 0, velocity Verlet without thermostat).
-void dyn_md::one_step() {
+void dyn_md::one_step(bool energies) {
   const float dt = prm.dt, h = 0.5f * dt;
-  if (!forces_valid) forces();
+  if (!forces_valid) forces(false);
   if (prm.gamma > 0.f) {
     integrate(h, h, true, true);
     constrain_positions();
@@ -658,7 +661,7 @@ void dyn_md::one_step() {
     constrain_positions();
     constrain_velocities();
   }
-  forces();
+  forces(energies);
   integrate(h, 0.f, true, false);
   constrain_velocities();
   ++step;
@@ -865,7 +868,7 @@ extern "C" int dyn_md_step(dyn_md* md, uint32_t n_steps) {
   DYN_API_BODY(md, {
     DeviceGuard g(md->device);
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
-    for (uint32_t k = 0; k < n_steps; ++k) md->one_step();
+    for (uint32_t k = 0; k < n_steps; ++k) md->one_step(k + 1 == n_steps);
   });
 }
 
@@ -903,7 +906,7 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     DeviceGuard g(md->device);
     DYN_REQUIRE(out, "out is null");
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
-    if (!md->forces_valid) md->forces();
+    if (!md->forces_valid) md->forces(true);
     Box B;
     for (int a = 0; a < 3; ++a) B.L[a] = md->box[a];
     hipLaunchKernelGGL(k_md_energies, dim3(1), dim3(1024), 0, md->stream, md->x.p, md->v.p,

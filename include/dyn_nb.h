@@ -215,6 +215,9 @@ int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device);
 #define DYN_COMPUTE_ALL 7u
 #define DYN_COMPUTE_REBUILD 8u   /* force a cluster-pair list rebuild      */
 #define DYN_COMPUTE_ASYNC 16u    /* enqueue only; dyn_engine_sync waits    */
+#define DYN_COMPUTE_NO_ENERGY 32u /* forces only: skip energies, virial and
+                                   * pair counts (dyn_engine_get_energies
+                                   * then fails until a compute without it) */
 
 /* Alchemical decoupling of one molecule, as the reference's
  * nonbonded_force_alchemical_kernel (src/cuda/cuda.cu:367-566) and its
@@ -334,7 +337,9 @@ int dyn_md_set_state(dyn_md* md, const float* pos_dyn, const float* vel_dyn,
 int dyn_md_get_state(dyn_md* md, float* pos_dyn, float* vel_dyn, float* pos_water,
                      float* vel_water);
 
-/* Enqueues n_steps steps (asynchronous; get_state/get_energies wait). */
+/* Enqueues n_steps steps (asynchronous; get_state/get_energies wait).  Only
+ * the last step's force evaluation computes nonbonded energies; the others
+ * run with DYN_COMPUTE_NO_ENERGY. */
 int dyn_md_step(dyn_md* md, uint32_t n_steps);
 
 typedef struct dyn_md_energies {

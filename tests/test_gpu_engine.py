@@ -136,6 +136,25 @@ def test_deterministic(gpu, cases):
     assert ea["total"] == eb["total"] and ea["virial_sr"] == eb["virial_sr"]
 
 
+@pytest.mark.parametrize("alch", [False, True])
+def test_no_energy_forces_identical(gpu, cases, alch):
+    """DYN_COMPUTE_NO_ENERGY gives the same fixed-point forces, bit for bit,
+    and get_energies refuses until a compute with energies."""
+    s = cases["solvated"]
+    eng = _engine(s, grid=(27, 27, 27), order=4)
+    if alch:
+        eng.set_alchemical(0, 40, 0.4)
+    eng.compute(api.COMPUTE_ALL)
+    a = eng.fixed_forces().copy()
+    ea = eng.energies()
+    eng.compute(api.COMPUTE_ALL | api.COMPUTE_NO_ENERGY)
+    assert np.array_equal(a, eng.fixed_forces())
+    with pytest.raises(api.DynError, match="skipped energies"):
+        eng.energies()
+    eng.compute(api.COMPUTE_ALL)
+    assert eng.energies() == ea
+
+
 def test_list_reuse_with_buffer(gpu, cases):
     """With a list buffer the list is reused while sites move less than half
     the buffer; the result equals a fresh build on the moved positions."""

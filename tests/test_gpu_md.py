@@ -50,6 +50,11 @@ def test_steps_vs_fp64(gpu):
               f"|dv| {dv:.2e} A/ps", flush=True)
         assert dx_dyn < 1e-4 and dx_w < 1e-4
         assert dv < 2e-3
+    # the last step of each call evaluates nonbonded energies: a fresh
+    # evaluation at the same positions agrees bit for bit
+    e_step = eng.energies()
+    eng.compute(api.COMPUTE_ALL)
+    assert eng.energies() == e_step
     m.close()
     eng.close()
 
