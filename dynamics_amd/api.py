@@ -144,6 +144,7 @@ _sig("dyn_engine_get_forces", C.c_int, [_p, _p, _p])
 _sig("dyn_engine_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
 _sig("dyn_engine_forces_device", C.c_void_p, [_p])
 _sig("dyn_engine_get_energies", C.c_int, [_p, C.POINTER(Energies)])
+_sig("dyn_engine_debug_item_partials", C.c_int, [_p, C.c_void_p, C.c_uint64])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
                                           _p, C.POINTER(C.c_uint64), _p])
@@ -166,6 +167,7 @@ EXPORTED = [
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
+    "dyn_engine_debug_item_partials",
     "dyn_engine_profile",
     "dyn_engine_profile_read",
 ]
@@ -430,6 +432,12 @@ class Engine:
         d["cmask"] = d["cmask"][:ne]
         d["items"] = d["items"][:ni]
         return d
+
+    def debug_item_partials(self, n_items):
+        """(n_items, 9) per work item sums of the last compute (see the C header)."""
+        out = np.zeros((9, n_items), np.float64)
+        _check(lib.dyn_engine_debug_item_partials(self.h, _ptr(out), int(n_items)), self._err)
+        return out.T.copy()
 
     def profile(self, enable=True):
         _check(lib.dyn_engine_profile(self.h, 1 if enable else 0), self._err)

@@ -693,7 +693,9 @@ k_eng_pairs(PairArgs A) {
     int tj = A.slot_type[js];
     int gj = A.slot_gid[js];
     for (uint32_t g = g0; g < g1; ++g) {
-      const int cls = (int)(__builtin_amdgcn_readfirstlane(v) >> DYN_JCLS_SHIFT);
+      // readfirstlane returns a signed int: shift the entry as unsigned (class
+      // 2 sets the top bit)
+      const int cls = (int)((uint32_t)__builtin_amdgcn_readfirstlane(v) >> DYN_JCLS_SHIFT);
       const float4 pjc = pj;
       const int tjc = tj, gjc = gj;
       if (g + 1 < g1) {
@@ -778,10 +780,12 @@ struct CorrArgs {
 };
 
 constexpr int kCorrBlock = 256;
+constexpr int kCorrVals = 7;  // per-block sums of the correction kernel
 __global__ void __launch_bounds__(kCorrBlock)
 k_eng_corr(CorrArgs A) {
   __shared__ double red[kCorrBlock];
-  double acc[6] = {0, 0, 0, 0, 0, 0};  // e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ
+  // e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ, 1-4 pairs within the cutoff
+  double acc[kCorrVals] = {0, 0, 0, 0, 0, 0, 0};
   for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += gridDim.x * kCorrBlock) {
     if (A.kind[k] == 0 && !A.recip_on) continue;
     const uint32_t i = A.ci[k], j = A.cj[k];
@@ -795,6 +799,7 @@ k_eng_corr(CorrArgs A) {
     const float qq = a.w * b.w;
     float s_sr = 0.f, s_rc = 0.f;
     if (A.kind[k] == 1) {
+      if (r2 < A.P.rc2) acc[6] += 1.0;
       // a 1-4 pair across the decoupled molecule's boundary is also an
       // alchemical pair (src/non_bonded.rs:581-599)
       const bool cross = A.P.alch_on && ((A.ltype[i] ^ A.ltype[j]) & DYN_TYPE_ALCH) != 0;
@@ -854,14 +859,14 @@ k_eng_corr(CorrArgs A) {
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_sites + j), (unsigned long long)(-q1));
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_sites + j), (unsigned long long)(-q2));
   }
-  for (int v = 0; v < 6; ++v) {
+  for (int v = 0; v < kCorrVals; ++v) {
     red[threadIdx.x] = acc[v];
     __syncthreads();
     for (int s = kCorrBlock / 2; s > 0; s >>= 1) {
       if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) A.partials[6 * blockIdx.x + v] = red[0];
+    if (threadIdx.x == 0) A.partials[kCorrVals * blockIdx.x + v] = red[0];
     __syncthreads();
   }
 }
@@ -971,7 +976,7 @@ k_eng_energy_reduce1(const double* __restrict__ pp, const uint32_t* __restrict__
 __global__ void __launch_bounds__(kRedBlock)
 k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ cp,
                     uint32_t n_cblk, int pme_runs, double* __restrict__ out) {
-  constexpr int kVals = kNumPart + 6;
+  constexpr int kVals = kNumPart + kCorrVals;
   __shared__ double red[kVals][kRedBlock / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   double a[kVals];
@@ -980,7 +985,7 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
     for (uint32_t r = t; r < (uint32_t)kRed1Grid; r += kRedBlock)
       for (int v = 0; v < kNumPart; ++v) a[v] += red1[(size_t)r * kNumPart + v];
   for (uint32_t r = t; r < n_cblk; r += kRedBlock)
-    for (int v = 0; v < 6; ++v) a[kNumPart + v] += cp[(size_t)r * 6 + v];
+    for (int v = 0; v < kCorrVals; ++v) a[kNumPart + v] += cp[(size_t)r * kCorrVals + v];
   for (int v = 0; v < kVals; ++v) {
     double x = a[v];
     for (int m = 1; m < kWave; m <<= 1) x += __shfl_xor(x, m, kWave);
@@ -992,7 +997,6 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
     for (int k = 0; k < kRedBlock / kWave; ++k) x += red[t][k];
     out[t] = t < kNumPart ? 0.5 * x : x;
   }
-  if (t == kVals) out[15] = 0.0;
   if (t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
 }
 
@@ -1639,7 +1643,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     up(E->qsite, hqs);
     E->n_corr = (uint32_t)ci.size();
     E->corr_blocks = grid_for(std::max<uint32_t>(E->n_corr, 1), kCorrBlock, 256);
-    E->corr_part.reserve(6 * (size_t)E->corr_blocks);
+    E->corr_part.reserve(kCorrVals * (size_t)E->corr_blocks);
     E->pos_own.reserve(ns);
     E->h_pos.reserve(ns);
     E->wpos.reserve(ns);
@@ -1871,7 +1875,8 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     DYN_HIP(hipStreamSynchronize(eng->stream));
     const double* v = eng->h_eng.p;
     // v[0..8]: e_lj_w, e_lj_s, e_c_w, e_c_s, virial_pairs, pair counts by
-    // kind, dH/dλ; v[9..14]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, dH/dλ;
+    // kind, dH/dλ; v[9..15]: e_lj14, e_c14, e_corr, vir_sr14, vir_recip_corr, dH/dλ,
+    // 1-4 pairs within the cutoff;
     // v[16..17], v[18..19]: energy and virial of the SPME evaluations
     double e_rec = v[16], w_rec = v[17], dh_rec = 0.0;
     if (eng->last_recip_b) {
@@ -1901,7 +1906,8 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     E.pair_count_kind[0] = (uint64_t)std::llround(v[5]);
     E.pair_count_kind[1] = (uint64_t)std::llround(v[6]);
     E.pair_count_kind[2] = (uint64_t)std::llround(v[7]);
-    E.pair_count = E.pair_count_kind[0] + E.pair_count_kind[1] + E.pair_count_kind[2];
+    E.pair_count = E.pair_count_kind[0] + E.pair_count_kind[1] + E.pair_count_kind[2] +
+                   (uint64_t)std::llround(v[15]);
     E.list_entries = eng->h_misc.p[2];
     E.list_builds = eng->n_builds;
     *out = E;
@@ -1945,6 +1951,25 @@ extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint3
                                hipMemcpyDeviceToHost, s));
     }
     DYN_HIP(hipStreamSynchronize(s));
+  });
+}
+
+extern "C" int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint64_t n_items) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DeviceGuard g(eng->device);
+    DYN_REQUIRE(out, "out is null");
+    if (!eng->energies_valid)
+      throw Error(DYN_ERR_STATE, "the last compute evaluated no pair energies");
+    eng->check_status();
+    const uint64_t nit = eng->h_misc.p[3];
+    DYN_REQUIRE(n_items == nit, "n_items %llu, the list has %llu work items",
+                (unsigned long long)n_items, (unsigned long long)nit);
+    for (int v = 0; v < kNumPart; ++v)
+      if (nit)
+        DYN_HIP(hipMemcpyAsync(out + (size_t)v * nit, eng->pair_part.p + (size_t)v * eng->cap_chunks,
+                               8 * nit, hipMemcpyDeviceToHost, eng->stream));
+    DYN_HIP(hipStreamSynchronize(eng->stream));
   });
 }
 

@@ -529,6 +529,7 @@ struct dyn_md {
   uint32_t n_dyn = 0, n_water = 0, n_sites = 0;
   float box[3] = {0, 0, 0};
   DevBuf<float4> x, v;
+  DevBuf<float4> x_ck, v_ck;  // state at the start of the current dyn_md_step call
   DevBuf<float> mass, acc_conv, lang_sigma;
   float sigma_o = 0.f, sigma_h = 0.f, lang_c = 1.f;
   uint32_t n_bond = 0, n_angle = 0, n_dihe = 0;
@@ -552,6 +553,7 @@ struct dyn_md {
   void constrain_positions();
   void constrain_velocities();
   void one_step(bool energies);
+  void run(uint32_t n);
 };
 
 // The first evaluation after set_state is synchronous, so that a list that
@@ -665,6 +667,31 @@ void dyn_md::one_step(bool energies) {
   integrate(h, 0.f, true, false);
   constrain_velocities();
   ++step;
+}
+
+// n steps, then a wait for them.  Later force evaluations are only enqueued,
+// so a cluster-pair list that outgrows its buffers is found at the wait; the
+// engine has enlarged them by then, and the steps run again from the state
+// they started from (the same random stream: the step index seeds it).
+void dyn_md::run(uint32_t n) {
+  if (!n) return;
+  const size_t bytes = 16 * (size_t)n_sites;
+  DYN_HIP(hipMemcpyAsync(x_ck.p, x.p, bytes, hipMemcpyDeviceToDevice, stream));
+  DYN_HIP(hipMemcpyAsync(v_ck.p, v.p, bytes, hipMemcpyDeviceToDevice, stream));
+  const uint64_t step0 = step;
+  for (int attempt = 0;; ++attempt) {
+    for (uint32_t k = 0; k < n; ++k) one_step(k + 1 == n);
+    try {
+      engine_call(dyn_engine_sync(eng), eng);
+      return;
+    } catch (const Error& e) {
+      if (e.code != DYN_ERR_CAPACITY || attempt >= 2) throw;
+    }
+    DYN_HIP(hipMemcpyAsync(x.p, x_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
+    DYN_HIP(hipMemcpyAsync(v.p, v_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
+    step = step0;
+    forces_valid = false;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -797,6 +824,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     up(M->cons_invm, invm.data(), invm.size(), 4);
     M->x.reserve(ns);
     M->v.reserve(ns);
+    M->x_ck.reserve(ns);
+    M->v_ck.reserve(ns);
     M->eout.reserve(8);
     M->h_out.reserve(8);
     DYN_HIP(hipStreamSynchronize(M->stream));
@@ -868,7 +897,7 @@ extern "C" int dyn_md_step(dyn_md* md, uint32_t n_steps) {
   DYN_API_BODY(md, {
     DeviceGuard g(md->device);
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
-    for (uint32_t k = 0; k < n_steps; ++k) md->one_step(k + 1 == n_steps);
+    md->run(n_steps);
   });
 }
 

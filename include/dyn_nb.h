@@ -256,11 +256,13 @@ typedef struct dyn_energies {
   double reference;     /* the reference's reported energy (see create)       */
   double virial_sr;     /* Σ r·F over real-space pairs                        */
   double virial_recip;  /* k-space virial + corrections                       */
-  double dh_dl;         /* reserved                                           */
-  uint64_t pair_count;  /* pairs inside the cutoff that interacted            */
+  double dh_dl;         /* dH/dλ of the alchemical decoupling (0 without)     */
+  uint64_t pair_count;  /* pairs inside the cutoff that interacted, 1-4 pairs
+                         * included (with DYN_COMPUTE_CORR)                   */
   uint64_t list_entries;/* cluster-pair entries in the current list           */
   uint64_t list_builds; /* list builds so far                                 */
-  uint64_t pair_count_kind[3]; /* of pair_count: LJ + Coulomb, LJ only, Coulomb only */
+  uint64_t pair_count_kind[3]; /* pairs of the pair kernel (pair_count less the
+                                * 1-4 pairs): LJ + Coulomb, LJ only, Coulomb only */
 } dyn_energies;
 
 int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out);
@@ -280,6 +282,11 @@ int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clus
                             int32_t* slot_type, int32_t* slot_gid, uint8_t* cl_class,
                             uint32_t* row_ptr, uint32_t* cj, uint64_t* cmask, uint32_t* items,
                             uint64_t* n_items, int64_t* fx_slot);
+/* Diagnostics (no counterpart in the reference): the pair kernel's per work
+ * item sums of the last compute, value-major out[9·n_items]: e_lj water,
+ * e_lj solute, e_coul water, e_coul solute, virial, pair counts (solute–
+ * solute, LJ, Coulomb), dH/dλ — every pair counted from both of its rows. */
+int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint64_t n_items);
 
 /* Per-kernel device time from HIP events recorded on the engine's stream
  * around every launch while profiling is on.  names/ms/counts receive up to
@@ -337,9 +344,12 @@ int dyn_md_set_state(dyn_md* md, const float* pos_dyn, const float* vel_dyn,
 int dyn_md_get_state(dyn_md* md, float* pos_dyn, float* vel_dyn, float* pos_water,
                      float* vel_water);
 
-/* Enqueues n_steps steps (asynchronous; get_state/get_energies wait).  Only
- * the last step's force evaluation computes nonbonded energies; the others
- * run with DYN_COMPUTE_NO_ENERGY. */
+/* Runs n_steps steps and waits for them.  Force evaluations after the first
+ * are enqueued without waiting; should the cluster-pair list outgrow its
+ * buffers meanwhile, the engine enlarges them and the call runs the steps
+ * again from the state it started from (same random stream).  Only the
+ * last step's force evaluation computes nonbonded energies; the others run
+ * with DYN_COMPUTE_NO_ENERGY. */
 int dyn_md_step(dyn_md* md, uint32_t n_steps);
 
 typedef struct dyn_md_energies {

@@ -118,7 +118,8 @@ void orc_ref_pairs(const float* pos, uint32_t n_dyn, uint32_t n_water, uint32_t 
 template <int KIND>
 static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slot_gid,
                  uint32_t n_slots, int type_none, uint32_t I, const uint32_t* jent, uint64_t mask,
-                 const DynPairParams& P, const float2* lj, const float2* lj_se, float fi[64][3]) {
+                 const DynPairParams& P, const float2* lj, const float2* lj_se, float fi[64][3],
+                 double* e) {
   for (int l = 0; l < 64; ++l) {
     const int ia = l & 7, jb = l >> 3;
     const uint32_t js = jent[jb] & DYN_JSLOT_MASK;
@@ -144,6 +145,15 @@ static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slo
     fi[l][0] += s * d[0];
     fi[l][1] += s * d[1];
     fi[l][2] += s * d[2];
+    if (e) {
+      // e: e_lj water, e_lj solute, e_coul water, e_coul solute, virial, dH/dλ
+      // (every pair twice: once from each of its two rows)
+      const bool solute = KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_SQ;
+      if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) e[solute ? 1 : 0] += elj;
+      if (KIND != DYN_KIND_SO && KIND != DYN_KIND_OO) e[solute ? 3 : 2] += ec;
+      e[4] += s * r2;
+      if (P.alch_on && KIND <= DYN_KIND_SQ) e[5] += dh;
+    }
   }
 }
 
@@ -156,7 +166,8 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
                       const uint64_t* cmask, const uint32_t* items, uint64_t n_items,
                       uint32_t n_slots, const float* lj_table, const float* lj_se_table,
                       int n_types, const float* box, float cutoff, float lj_cutoff, float beta,
-                      float c6_oo, float c12_oo, int alch_on, float lambda, int64_t* fx) {
+                      float c6_oo, float c12_oo, int alch_on, float lambda, int64_t* fx,
+                      double* energies, double* item_energies) {
   DynPairParams P;
   for (int k = 0; k < 3; ++k) {
     P.ext[k] = box[k];
@@ -175,24 +186,32 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
   const float2* lj = reinterpret_cast<const float2*>(lj_table);
   const float2* lse = reinterpret_cast<const float2*>(lj_se_table);
   memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
+  if (energies)
+    for (int k = 0; k < 6; ++k) energies[k] = 0.0;
   static float fi[64][3];
   for (uint64_t it = 0; it < n_items; ++it) {
     const uint32_t I = items[2 * it], g0 = items[2 * it + 1];
     const uint32_t g1 = std::min<uint32_t>(g0 + DYN_CHUNK, row_ptr[I + 1]);
     memset(fi, 0, sizeof fi);
     const int ci = cl_class[I];
+    double ie[6] = {0, 0, 0, 0, 0, 0};
+    double* e_acc = (energies || item_energies) ? ie : nullptr;
     for (uint32_t g = g0; g < g1; ++g) {
       const uint32_t* je = cj + (size_t)g * DYN_CL_SIZE;
       const uint64_t mask = cmask[g];
       const int tn = n_types - 1;
       switch (dyn_kind(ci, (int)(je[0] >> DYN_JCLS_SHIFT))) {
-        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
-        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
-        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
-        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
-        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi); break;
+        case DYN_KIND_QQ: tile<DYN_KIND_QQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi, e_acc); break;
+        case DYN_KIND_OO: tile<DYN_KIND_OO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi, e_acc); break;
+        case DYN_KIND_SS: tile<DYN_KIND_SS>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi, e_acc); break;
+        case DYN_KIND_SO: tile<DYN_KIND_SO>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi, e_acc); break;
+        case DYN_KIND_SQ: tile<DYN_KIND_SQ>(xyzq, slot_type, slot_gid, n_slots, tn, I, je, mask, P, lj, lse, fi, e_acc); break;
         default: break;
       }
+    }
+    for (int k = 0; k < 6; ++k) {
+      if (energies) energies[k] += ie[k];
+      if (item_energies) item_energies[6 * it + k] = ie[k];
     }
     float v[64][3];
     memcpy(v, fi, sizeof v);

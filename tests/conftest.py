@@ -119,7 +119,12 @@ def engine_lj_se(system):
     return np.ascontiguousarray(np.stack([s_ab, e_ab], -1).reshape(-1, 2), np.float32)
 
 
-def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta, alch=False, lam=0.0):
+def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta, alch=False, lam=0.0,
+                        energies=False, per_item=False):
+    """Host replay of the pair tiles over an engine list: (3, n_slots) int64
+    fixed-point slot forces, and with energies=True also the six sums
+    (e_lj water, e_lj solute, e_coul water, e_coul solute, virial, dH/dλ),
+    every pair counted from both of its rows (per_item: (n_items, 6) too)."""
     L = oracle_lib()
     tab = engine_lj_table(system)
     tse = engine_lj_se(system)
@@ -131,7 +136,10 @@ def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta, alch=False, lam=0.
     L.orc_engine_tiles.argtypes = [C.c_void_p] * 8 + [C.c_uint64, C.c_uint32, C.c_void_p,
                                                       C.c_void_p, C.c_int, C.c_void_p, C.c_float,
                                                       C.c_float, C.c_float, C.c_float, C.c_float,
-                                                      C.c_int, C.c_float, C.c_void_p]
+                                                      C.c_int, C.c_float, C.c_void_p, C.c_void_p,
+                                                      C.c_void_p]
+    e = np.zeros(6, np.float64)
+    ie = np.zeros((len(dbg["items"]), 6), np.float64)
     box = np.ascontiguousarray(system.box, np.float32)
     cm = np.ascontiguousarray(dbg["cmask"].astype(np.uint64))
     items = np.ascontiguousarray(dbg["items"], np.uint32)
@@ -140,8 +148,11 @@ def replay_engine_tiles(dbg, system, cutoff, lj_cutoff, beta, alch=False, lam=0.
                        ptr(dbg["cj"]) if len(dbg["cj"]) else None,
                        ptr(cm) if len(cm) else None, ptr(items) if len(items) else None,
                        len(items), ns, ptr(tab), ptr(tse), nt, ptr(box), cutoff, lj_cutoff, beta,
-                       float(c6_oo), float(c12_oo), 1 if alch else 0, float(lam), ptr(fx))
-    return fx
+                       float(c6_oo), float(c12_oo), 1 if alch else 0, float(lam), ptr(fx), ptr(e),
+                       ptr(ie) if per_item and len(ie) else None)
+    if per_item:
+        return fx, e, ie
+    return (fx, e) if energies else fx
 
 
 def replay_pme_spread(xyzq, K, box, order):
@@ -221,3 +232,43 @@ def normalize_null_entries(cj, n_slots):
     slot = cj & np.uint32(NULL_ENTRY)
     cj[slot >= n_slots] = (cj[slot >= n_slots] & ~np.uint32(NULL_ENTRY)) | np.uint32(NULL_ENTRY)
     return cj
+
+
+def host_engine_debug(system, cutoff, alch=None, chunk=16):
+    """The engine's list and slot arrays as the host restatement builds them,
+    in the layout of Engine.debug_export (slot positions wrapped with
+    np.mod, which the minimum image makes equivalent)."""
+    n, slot_site, cl_class, row_ptr, cj, cm = replay_engine_list(system, cutoff)
+    pos, q, _, _, mol, _ = system.site_arrays()
+    box = np.asarray(system.box, np.float32)
+    w = np.mod(pos.astype(np.float32), box).astype(np.float32)
+    nt = system.lj_types.shape[0]
+    tw = np.empty(len(q), np.int32)
+    tw[:system.n_dyn] = system.type_dyn
+    if alch is not None:
+        tw[:system.n_dyn] |= np.where(alch, 0x10000, 0).astype(np.int32)
+    tw[system.n_dyn:] = np.tile(np.array([nt, nt + 1, nt + 1, nt + 1], np.int32), system.n_water)
+    gid = np.where(mol >= 0, mol, -2 - np.arange(len(q))).astype(np.int32)
+    ns = 8 * n
+    xyzq = np.zeros((ns, 4), np.float32)
+    st = np.full(ns, nt + 1, np.int32)
+    sg = np.full(ns, -1, np.int32)
+    ok = slot_site >= 0
+    xyzq[ok, :3] = w[slot_site[ok]]
+    xyzq[ok, 3] = q[slot_site[ok]]
+    first = slot_site[(np.arange(ns) // 8) * 8]
+    xyzq[~ok, :3] = w[first[~ok]]
+    st[ok] = tw[slot_site[ok]]
+    sg[ok] = gid[slot_site[ok]]
+    items = [(I, g0) for I in range(n) for g0 in range(row_ptr[I], row_ptr[I + 1], chunk)]
+    items = np.asarray(items, np.uint32).reshape(-1, 2)
+    return {"slot_site": slot_site, "xyzq": xyzq, "slot_type": st, "slot_gid": sg,
+            "cl_class": cl_class, "row_ptr": row_ptr, "cj": cj, "cmask": cm, "items": items}
+
+
+def slot_forces_to_sites(fx, dbg, n_sites):
+    """(3, n_slots) fixed point → (n_sites, 3) float64 forces."""
+    F = np.zeros((n_sites, 3))
+    ok = dbg["slot_site"] >= 0
+    F[dbg["slot_site"][ok]] = fx[:, ok].T * 2.0 ** -32
+    return F

@@ -54,7 +54,10 @@ def test_steps_vs_fp64(gpu):
     # evaluation at the same positions agrees bit for bit
     e_step = eng.energies()
     eng.compute(api.COMPUTE_ALL)
-    assert eng.energies() == e_step
+    e_again = eng.energies()
+    for k in ("list_builds",):
+        e_step.pop(k), e_again.pop(k)
+    assert e_again == e_step
     m.close()
     eng.close()
 
