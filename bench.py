@@ -44,8 +44,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall time of the CPU-baseline sample (0 skips it)")
-    ap.add_argument("--mode", choices=["md", "eval"], default="md",
-                    help="md: device-resident MD steps; eval: nonbonded evaluations only")
+    ap.add_argument("--mode", choices=["md", "eval", "copy"], default="md",
+                    help="md: device-resident MD steps; eval: nonbonded evaluations only; "
+                         "copy: a plain 256 MiB device copy, three times (the known-size "
+                         "kernel that scales the HBM byte counters)")
     ap.add_argument("--list-buffer", type=float, default=None,
                     help="list buffer in Å (md default 1.0; eval default 0: rebuild every step)")
     args = ap.parse_args()
@@ -55,6 +57,17 @@ def main() -> int:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         return 2
+
+    if args.mode == "copy":
+        import torch
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"copied {4 * n} bytes, 3 times", flush=True)
+        return 0
 
     import numpy as np
     from dynamics_amd import api, systems

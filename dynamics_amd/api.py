@@ -31,6 +31,7 @@ COMPUTE_ALL = 7
 COMPUTE_REBUILD = 8
 COMPUTE_ASYNC = 16
 COMPUTE_NO_ENERGY = 32
+COMPUTE_RECIP_REUSE = 64
 
 
 class DynError(RuntimeError):
@@ -60,7 +61,7 @@ class NbOpts(C.Structure):
 
 class MdParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("temperature", C.c_float), ("gamma", C.c_float),
-                ("shake_tol", C.c_float), ("seed", C.c_uint64)]
+                ("shake_tol", C.c_float), ("seed", C.c_uint64), ("spme_ratio", C.c_uint32)]
 
 
 class MdEnergies(C.Structure):
@@ -456,10 +457,11 @@ class Md:
     """Device-resident MD around an Engine (dyn_md_*): positions, velocities
     and forces stay on the GPU between steps."""
 
-    def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0):
+    def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0,
+                 spme_ratio=1):
         self.engine = engine
         self.h = C.c_void_p()
-        p = MdParams(dt, temperature, gamma, shake_tol, seed)
+        p = MdParams(dt, temperature, gamma, shake_tol, seed, spme_ratio)
         arrs = [_c(ff.mass, np.float32),
                 _c(ff.bonds, np.uint32), _c(ff.bond_k_r0, np.float32),
                 _c(ff.angles, np.uint32), _c(ff.angle_k_t0, np.float32),

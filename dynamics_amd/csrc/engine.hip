@@ -878,12 +878,18 @@ k_eng_corr(CorrArgs A) {
 __global__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
                                 const uint32_t* __restrict__ qsite, uint32_t nq,
                                 float4* __restrict__ pme_xyzq, float4* __restrict__ pme_b,
-                                uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site) {
+                                uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site,
+                                int64_t* __restrict__ fx_recip) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n_sites) {
     fx_site[s] = 0;
     fx_site[(size_t)n_sites + s] = 0;
     fx_site[2 * (size_t)n_sites + s] = 0;
+    if (fx_recip) {
+      fx_recip[s] = 0;
+      fx_recip[(size_t)n_sites + s] = 0;
+      fx_recip[2 * (size_t)n_sites + s] = 0;
+    }
   }
   if (pme_xyzq && s < nq) {
     const uint32_t site = qsite[s];
@@ -908,9 +914,11 @@ __device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fit
   }
 }
 
-// Site-order forces: the site accumulators (corrections, SPME) plus the
-// pair kernel's work-item rows of the site's slot (when the pair kernel ran).
+// Site-order forces: the site accumulators (corrections), the SPME forces
+// (this evaluation's or the kept ones of an earlier one) and the pair
+// kernel's work-item rows of the site's slot (when the pair kernel ran).
 __global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
+                               const int64_t* __restrict__ fx_recip,
                                const int64_t* __restrict__ fitem,
                                const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
                                const uint32_t* __restrict__ slot_of, uint32_t n_sites,
@@ -918,6 +926,11 @@ __global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_sites) return;
   int64_t f[3] = {fx_site[s], fx_site[(size_t)n_sites + s], fx_site[2 * (size_t)n_sites + s]};
+  if (fx_recip) {
+    f[0] += fx_recip[s];
+    f[1] += fx_recip[(size_t)n_sites + s];
+    f[2] += fx_recip[2 * (size_t)n_sites + s];
+  }
   if (fitem) slot_item_forces(fitem, chunk_ptr, cap_chunks, min(slot_of[s], n_slots - 1), f);
   for (int c = 0; c < 3; ++c) out[(size_t)c * n_sites + s] = f[c];
 }
@@ -976,6 +989,8 @@ k_eng_energy_reduce1(const double* __restrict__ pp, const uint32_t* __restrict__
 __global__ void __launch_bounds__(kRedBlock)
 k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ cp,
                     uint32_t n_cblk, int pme_runs, double* __restrict__ out) {
+  // pme_runs < 0: the SPME sum was kept from an earlier evaluation, with its
+  // energies, virials and the reciprocal correction virial (out[13])
   constexpr int kVals = kNumPart + kCorrVals;
   __shared__ double red[kVals][kRedBlock / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
@@ -995,9 +1010,9 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
   if (t < kVals) {
     double x = 0.0;
     for (int k = 0; k < kRedBlock / kWave; ++k) x += red[t][k];
-    out[t] = t < kNumPart ? 0.5 * x : x;
+    if (!(pme_runs < 0 && t == 13)) out[t] = t < kNumPart ? 0.5 * x : x;
   }
-  if (t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
+  if (pme_runs >= 0 && t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
 }
 
 __global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
@@ -1092,6 +1107,8 @@ struct dyn_engine {
   bool pme_on = false;
   PmeCore pme;
   DevBuf<float4> pme_xyzq, pme_b;  // charge sets of the two SPME evaluations
+  DevBuf<int64_t> fx_recip;        // SPME forces of the last evaluation that ran it
+  bool recip_kept = false;         // fx_recip and its energies are those of this box
   // host copies
   PinnedBuf<int64_t> h_fx;
   PinnedBuf<double> h_eng;
@@ -1358,19 +1375,27 @@ void dyn_engine::enqueue(uint32_t flags) {
   P.sch = dyn_schedule((float)lambda);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
-  const bool recip_on = pme_on && (flags & DYN_COMPUTE_RECIP);
+  // the reference's SPME cadence: keep the last SPME forces, energy and
+  // virial (reference mode, no decoupled molecule, same box)
+  const bool reuse = pme_on && (flags & DYN_COMPUTE_RECIP_REUSE) && !(flags & DYN_COMPUTE_RECIP) &&
+                     recip_kept && !alch_on && prm.mode == DYN_MODE_REFERENCE;
+  const bool recip_on = pme_on && ((flags & DYN_COMPUTE_RECIP) ||
+                                   ((flags & DYN_COMPUTE_RECIP_REUSE) && !reuse));
   const bool corr_on = (flags & DYN_COMPUTE_CORR) && n_corr;
   // with a decoupled molecule the SPME sum runs twice, with its charges as
   // they are (A) and negated (B): ((1+s)A + (1−s)B)/2 keeps the
   // environment and molecule self terms and scales their cross term by s
   const bool recip_b = recip_on && alch_on;
-  last_recip_b = recip_b;
-  last_sch = P.sch;
+  if (!reuse) {
+    last_recip_b = recip_b;
+    last_sch = P.sch;
+  }
   {
     Profiler::Scope ps(prof, "site_prep");
     hipLaunchKernelGGL(k_eng_site_prep, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, wpos.p,
                        n_sites, qsite.p, n_q, recip_on ? pme_xyzq.p : (float4*)nullptr,
-                       recip_b ? pme_b.p : (float4*)nullptr, alch0, alch1, fx_site.p);
+                       recip_b ? pme_b.p : (float4*)nullptr, alch0, alch1, fx_site.p,
+                       recip_on ? fx_recip.p : (int64_t*)nullptr);
     DYN_CHECK_LAUNCH("k_eng_site_prep");
   }
   if (short_on) {
@@ -1422,19 +1447,20 @@ void dyn_engine::enqueue(uint32_t flags) {
   }
   if (recip_on) {
     if (!recip_b) {
-      pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 1.0f, eng_out.p + 16);
+      pme.run(n_q, pme_xyzq.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 1.0f, eng_out.p + 16);
     } else {
       const float s = P.sch.coulomb_scale;
-      pme.run(n_q, pme_xyzq.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 0.5f * (1.0f + s),
+      pme.run(n_q, pme_xyzq.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 0.5f * (1.0f + s),
               eng_out.p + 16);
-      pme.run(n_q, pme_b.p, qsite.p, fx_site.p, 1, n_sites, nullptr, 0.5f * (1.0f - s),
+      pme.run(n_q, pme_b.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 0.5f * (1.0f - s),
               eng_out.p + 18);
     }
   }
   {
     Profiler::Scope ps(prof, "finalize");
     hipLaunchKernelGGL(k_eng_finalize, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       fx_site.p, short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
+                       fx_site.p, (recip_on || reuse) ? fx_recip.p : (const int64_t*)nullptr,
+                       short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
                        cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
     DYN_CHECK_LAUNCH("k_eng_finalize");
     if (short_on && energy_on) {
@@ -1445,9 +1471,12 @@ void dyn_engine::enqueue(uint32_t flags) {
     if (energy_on)
     hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
                        short_on ? (const double*)red1.p : (const double*)nullptr, corr_part.p,
-                       corr_on ? corr_blocks : 0u, recip_on ? (recip_b ? 2 : 1) : 0, eng_out.p);
+                       corr_on ? corr_blocks : 0u, reuse ? -1 : recip_on ? (recip_b ? 2 : 1) : 0,
+                       eng_out.p);
     DYN_CHECK_LAUNCH("k_eng_energy_reduce");
   }
+  if (recip_on) recip_kept = true;
+  else if (!reuse) recip_kept = false;
   computed = true;
   energies_valid = energy_on;
   fx_fresh = false;
@@ -1651,6 +1680,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->slot_of.reserve(ns);
     E->fx_site.reserve(3 * (size_t)ns);
     E->fx_out.reserve(3 * (size_t)ns);
+    E->fx_recip.reserve(3 * (size_t)ns);
     E->h_fx.reserve(3 * (size_t)ns);
     E->eng_out.reserve(kEngOut);
     E->red1.reserve((size_t)kRed1Grid * kNumPart);
@@ -1711,6 +1741,7 @@ extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
     }
     for (int a = 0; a < 3; ++a) eng->box[a] = box[a];
     eng->box_set = true;
+    eng->recip_kept = false;
     eng->set_geometry();
     if (eng->pme_on) {
       if (eng->pme.ngrid == 0)

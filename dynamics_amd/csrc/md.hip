@@ -546,6 +546,7 @@ struct dyn_md {
   DevBuf<double> eout;
   PinnedBuf<double> h_out;
   uint64_t step = 0;
+  uint64_t n_eval = 0;  // force evaluations since set_state (SPME cadence)
   bool state_set = false, forces_valid = false;
 
   void forces(bool energies);
@@ -562,8 +563,12 @@ struct dyn_md {
 // energies are evaluated only when asked (the last step of a dyn_md_step
 // call), the forces alone otherwise.
 void dyn_md::forces(bool energies) {
-  const uint32_t flags = DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u) |
-                         (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
+  uint32_t flags = DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u) |
+                   (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
+  // the first evaluation after set_state (or a replay) always runs SPME
+  if (forces_valid && prm.spme_ratio > 1 && n_eval % prm.spme_ratio != 0)
+    flags = (flags & ~DYN_COMPUTE_RECIP) | DYN_COMPUTE_RECIP_REUSE;
+  ++n_eval;
   engine_call(dyn_engine_compute(eng, flags), eng);
   Profiler::Scope ps(*engine_profiler(eng), "md_bonded");
   hipLaunchKernelGGL(k_md_zero, dim3(grid_for(3 * (size_t)std::max(n_dyn, 1u), kBlock)), dim3(kBlock),
@@ -678,7 +683,7 @@ void dyn_md::run(uint32_t n) {
   const size_t bytes = 16 * (size_t)n_sites;
   DYN_HIP(hipMemcpyAsync(x_ck.p, x.p, bytes, hipMemcpyDeviceToDevice, stream));
   DYN_HIP(hipMemcpyAsync(v_ck.p, v.p, bytes, hipMemcpyDeviceToDevice, stream));
-  const uint64_t step0 = step;
+  const uint64_t step0 = step, eval0 = n_eval;
   for (int attempt = 0;; ++attempt) {
     for (uint32_t k = 0; k < n; ++k) one_step(k + 1 == n);
     try {
@@ -690,6 +695,7 @@ void dyn_md::run(uint32_t n) {
     DYN_HIP(hipMemcpyAsync(x.p, x_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
     DYN_HIP(hipMemcpyAsync(v.p, v_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
     step = step0;
+    n_eval = eval0;
     forces_valid = false;
   }
 }
@@ -717,6 +723,7 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     DYN_REQUIRE(p->temperature >= 0.f && std::isfinite(p->temperature), "bad temperature");
     DYN_REQUIRE(p->gamma >= 0.f && std::isfinite(p->gamma), "gamma must be ≥ 0");
     DYN_REQUIRE(p->shake_tol > 0.f, "shake_tol must be > 0");
+    DYN_REQUIRE(p->spme_ratio <= 64, "spme_ratio must be at most 64");
     std::unique_ptr<dyn_md> M(new dyn_md());
     M->eng = eng;
     M->device = engine_device(eng);
@@ -889,6 +896,7 @@ extern "C" int dyn_md_set_state(dyn_md* md, const float* pos_dyn, const float* v
                 md->eng);
     md->state_set = true;
     md->forces_valid = false;
+    md->n_eval = 0;
   });
 }
 

@@ -218,6 +218,13 @@ int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device);
 #define DYN_COMPUTE_NO_ENERGY 32u /* forces only: skip energies, virial and
                                    * pair counts (dyn_engine_get_energies
                                    * then fails until a compute without it) */
+/* In place of DYN_COMPUTE_RECIP: the reference's SPME cadence (SPME_RATIO,
+ * src/lib.rs:1282-1341) — keep the SPME forces, energy and virial of the last
+ * evaluation that ran SPME (with this box), without the 1-4 vacuum
+ * correction, which accompanies a fresh SPME sum only.  Runs SPME instead
+ * when nothing is kept, with a decoupled molecule (the reference evaluates
+ * SPME every step then) or in DYN_MODE_STANDARD. */
+#define DYN_COMPUTE_RECIP_REUSE 64u
 
 /* Alchemical decoupling of one molecule, as the reference's
  * nonbonded_force_alchemical_kernel (src/cuda/cuda.cu:367-566) and its
@@ -318,6 +325,9 @@ typedef struct dyn_md_params {
   float gamma;        /* 1/ps; 0 selects velocity Verlet without thermostat */
   float shake_tol;    /* Å: SHAKE iterates until the largest correction is below it */
   uint64_t seed;      /* Langevin noise (counter-based, per step and site) */
+  uint32_t spme_ratio;/* SPME every spme_ratio-th force evaluation, the kept
+                       * SPME forces in between (DYN_COMPUTE_RECIP_REUSE; the
+                       * reference's SPME_RATIO = 2); 0 or 1: every one */
 } dyn_md_params;
 
 /* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],

@@ -168,6 +168,36 @@ def test_no_energy_forces_identical(gpu, cases, alch):
     assert eb == ea
 
 
+def test_recip_reuse(gpu, cases):
+    """DYN_COMPUTE_RECIP_REUSE (the reference's SPME cadence): the kept SPME
+    forces of the last SPME evaluation are added to fresh short-range and
+    1-4 terms, without the 1-4 vacuum correction — bit for bit the sum of
+    the two separate evaluations; energy and virial of SPME are kept too."""
+    s = cases["solvated"]
+    eng = _engine(s, grid=(27, 27, 27), order=4, mode=api.MODE_REFERENCE)
+    eng.compute(api.COMPUTE_RECIP)
+    R = eng.fixed_forces().copy()
+    er = eng.energies()
+    eng.compute(api.COMPUTE_SHORT | api.COMPUTE_CORR | api.COMPUTE_RECIP_REUSE)
+    C = eng.fixed_forces().copy()
+    ec = eng.energies()
+    eng.compute(api.COMPUTE_SHORT | api.COMPUTE_CORR)
+    B = eng.fixed_forces().copy()
+    assert np.array_equal(C, B + R)
+    assert ec["recip"] == er["recip"] and ec["virial_recip"] == er["virial_recip"]
+    # nothing kept (the last compute ran no SPME): the flag runs SPME, with
+    # the 1-4 vacuum correction, as COMPUTE_ALL does
+    eng.compute(api.COMPUTE_SHORT | api.COMPUTE_CORR | api.COMPUTE_RECIP_REUSE)
+    D = eng.fixed_forces().copy()
+    eng.compute(api.COMPUTE_ALL)
+    assert np.array_equal(D, eng.fixed_forces())
+    # a new box drops what was kept
+    eng.set_box(s.box)
+    eng.compute(api.COMPUTE_SHORT | api.COMPUTE_CORR | api.COMPUTE_RECIP_REUSE)
+    assert np.array_equal(eng.fixed_forces(), D)
+    eng.close()
+
+
 def test_list_reuse_with_buffer(gpu, cases):
     """With a list buffer the list is reused while sites move less than half
     the buffer; the result equals a fresh build on the moved positions."""
