@@ -44,10 +44,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall time of the CPU-baseline sample (0 skips it)")
-    ap.add_argument("--mode", choices=["md", "eval", "copy"], default="md",
+    ap.add_argument("--mode", choices=["md", "eval", "copy", "batch-lambda", "batch-uneven"],
+                    default="md",
                     help="md: device-resident MD steps; eval: nonbonded evaluations only; "
                          "copy: a plain 256 MiB device copy, three times (the known-size "
-                         "kernel that scales the HBM byte counters)")
+                         "kernel that scales the HBM byte counters); batch-lambda: 16 "
+                         "alchemical λ-windows of the solvated chain in one batch; "
+                         "batch-uneven: 128 small solvated boxes of uneven sizes in one batch")
+    ap.add_argument("--systems", type=int, default=None,
+                    help="batch modes: number of systems (16 windows / 128 boxes)")
+    ap.add_argument("--relax-steps", type=int, default=400,
+                    help="untimed relaxation steps that quench the built chain's close contacts")
     ap.add_argument("--list-buffer", type=float, default=None,
                     help="list buffer in Å (md default 1.0; eval default 0: rebuild every step)")
     args = ap.parse_args()
@@ -69,6 +76,11 @@ def main() -> int:
         print(f"copied {4 * n} bytes, 3 times", flush=True)
         return 0
 
+    if args.mode == "batch-lambda":
+        return run_batch_lambda(args)
+    if args.mode == "batch-uneven":
+        return run_batch_uneven(args)
+
     import numpy as np
     from dynamics_amd import api, systems
 
@@ -79,7 +91,8 @@ def main() -> int:
     # the built chain has close contacts: quench them first (untimed), so that
     # both modes run on a physical configuration
     ff = systems.chain_force_field(s)
-    pd0, pw0 = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=400)
+    pd0, pw0 = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=args.relax_steps)
+    print(f"bench: relaxed ({args.relax_steps} steps)", file=sys.stderr, flush=True)
     s.pos_dyn = pd0
     s.pos_o, s.pos_m, s.pos_h0, s.pos_h1 = (np.ascontiguousarray(pw0[:, k]) for k in range(4))
     md = None
@@ -108,6 +121,7 @@ def main() -> int:
 
     run(args.warmup)
     wait()
+    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
     builds0 = eng.energies()["list_builds"]
 
     eng.profile(True)
@@ -210,6 +224,182 @@ def main() -> int:
             }
         except Exception as e:  # the baseline must never hide the GPU number
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(out))
+    return 0
+
+
+def _time_steps(step, wait, steps):
+    wait()
+    t0 = time.perf_counter()
+    step(steps)
+    wait()
+    return time.perf_counter() - t0
+
+
+def run_batch_lambda(args) -> int:
+    """16 alchemical λ-windows (λ = k/15) of the 37,000-site solvated chain,
+    40 decoupled solute atoms, Langevin-middle MD at dt 2 fs: one batch step
+    advances every window.  Compared with one window stepped alone."""
+    import numpy as np
+    from dynamics_amd import api, systems
+
+    n = args.systems or 16
+    s = systems.config1()
+    K = (64, 64, 64)
+    ff = systems.chain_force_field(s)
+
+    def engine():
+        return api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
+                          mode=api.MODE_REFERENCE, list_buffer=1.0)
+
+    e0 = engine()
+    pd0, pw0 = api.relax(e0, ff, s.pos_dyn, systems.water_positions(s), steps=args.relax_steps)
+    e0.close()
+    print("bench: relaxed", file=sys.stderr, flush=True)
+    v, vw = systems.initial_velocities(s, ff, 300.0)
+    lams = [k / max(n - 1, 1) for k in range(n)]
+    made = []
+    for k, lam in enumerate(lams):
+        e = engine()
+        e.set_alchemical(0, 40, lam)
+        m = api.Md(e, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=100 + k)
+        m.set_state(pd0, v, pw0, vw)
+        made.append((e, m))
+    # one window stepped alone (the plain single-system path)
+    e1, m1 = made[n // 2]
+    m1.step(args.warmup)
+    t_one = _time_steps(m1.step, lambda: m1.energies(), args.steps)
+    for e, m in made:
+        m.set_state(pd0, v, pw0, vw)
+    b = api.Batch([m for _, m in made])
+    b.step(args.warmup)
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    b.profile(True)
+    t_b = _time_steps(b.step, lambda: made[0][1].energies(), args.steps)
+    prof = b.profile_read()
+    b.close()
+    dh = [e.energies()["dh_dl"] for e, _ in made]
+    temps = [m.energies()["temperature"] for _, m in made]
+    one = args.steps * 0.1728 / t_one
+    agg = n * args.steps * 0.1728 / t_b
+    out = {
+        "metric": "aggregate ns/day of batched λ-windows, 1 GPU", "value": agg, "unit": "ns/day",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} alchemical windows of chain2600_opc8600_box64 (37,000 sites "
+                               "each), 40 decoupled solute atoms, λ = k/(n-1), soft-core α 0.5, "
+                               "Langevin-middle dt 2 fs, SPME 64^3 order 4 twice per window "
+                               "(the cross-term scaling), list buffer 1.0 A",
+                   "windows": n, "sites_per_window": s.n_sites},
+        "single_window_ns_per_day": one,
+        "batched_over_single": agg / one,
+        "dh_dl_range": [min(dh), max(dh)],
+        "temperature_K_range": [min(temps), max(temps)],
+        "kernels_ms_per_step": {k: v_[0] / args.steps for k, v_ in prof.items()},
+    }
+    for e, m in made:
+        m.close()
+        e.close()
+    print(json.dumps(out))
+    return 0
+
+
+def uneven_systems(n, seed=4):
+    """n solvated boxes with edges uniform in [24, 40] Å (≈0.0334 waters/Å³) and
+    20–60-atom chains (half heavy atoms), seeded."""
+    import numpy as np
+    from dynamics_amd import systems
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        L = float(rng.uniform(24.0, 40.0))
+        n_atoms = int(rng.integers(20, 61))
+        n_heavy = n_atoms // 2
+        lat = int(np.ceil((0.0334 * L ** 3) ** (1.0 / 3.0)))
+        n_water = int(round(0.0334 * L ** 3))
+        s = systems.solvated_chain(L, n_water, n_heavy, n_atoms - n_heavy, seed=1000 + k,
+                                   lattice_n=lat, name=f"uneven{k}")
+        out.append(s)
+    return out
+
+
+def run_batch_uneven(args) -> int:
+    """128 independent solvated boxes (edges 24–40 Å, 20–60-atom solutes, PME
+    grids at ~1 Å spacing), Langevin-middle MD: one batch step advances
+    every box.  Compared with the same boxes stepped one after the other."""
+    import numpy as np
+    from dynamics_amd import api, systems
+
+    n = args.systems or 128
+    S = uneven_systems(n)
+    print(f"bench: built {n} systems", file=sys.stderr, flush=True)
+
+    def make(s, gamma, temp, seed):
+        ff = systems.chain_force_field(s)
+        e = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=api.pme_grid_n(s.box, 1.0),
+                       order=4, mode=api.MODE_REFERENCE, list_buffer=1.0)
+        m = api.Md(e, ff, dt=0.002, temperature=temp, gamma=gamma, shake_tol=1e-5, seed=seed)
+        return e, m, ff
+
+    # relax every box (batched quench), then MD from the relaxed states
+    made = [make(s, 1.0e4, 0.0, 0) for s in S]
+    for (e, m, ff), s in zip(made, S):
+        m.set_state(s.pos_dyn, np.zeros((s.n_dyn, 3), np.float32), systems.water_positions(s),
+                    np.zeros((s.n_water, 3, 3), np.float32))
+    b = api.Batch([m for _, m, _ in made])
+    b.step(args.relax_steps)
+    b.close()
+    starts = [m.state() for _, m, _ in made]
+    for e, m, _ in made:
+        m.close()
+        e.close()
+    print("bench: relaxed", file=sys.stderr, flush=True)
+    made = [make(s, 1.0, 300.0, 200 + k) for k, s in enumerate(S)]
+    vel = [systems.initial_velocities(s, ff, 300.0) for s, (_, _, ff) in zip(S, made)]
+
+    def reset():
+        for (e, m, ff), st, (v, vw) in zip(made, starts, vel):
+            m.set_state(st[0], v, st[2], vw)
+
+    # the same boxes one after the other (plain single-system path)
+    reset()
+    for _, m, _ in made:
+        m.step(args.warmup)
+    t_seq = 0.0
+    for _, m, _ in made:
+        t_seq += _time_steps(m.step, lambda: m.energies(), args.steps)
+    reset()
+    b = api.Batch([m for _, m, _ in made])
+    b.step(args.warmup)
+    print("bench: warmup done", file=sys.stderr, flush=True)
+    b.profile(True)
+    t_b = _time_steps(b.step, lambda: made[0][1].energies(), args.steps)
+    prof = b.profile_read()
+    b.close()
+    temps = [m.energies()["temperature"] for _, m, _ in made]
+    sites = [s.n_sites for s in S]
+    seq = n * args.steps * 0.1728 / t_seq
+    agg = n * args.steps * 0.1728 / t_b
+    out = {
+        "metric": "aggregate ns/day of batched uneven systems, 1 GPU", "value": agg,
+        "unit": "ns/day", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{n} solvated boxes, edges U(24, 40) A, 20-60-atom chains, "
+                               "OPC water, rc 10 A, SPME ~1 A grids order 4, Langevin-middle "
+                               "dt 2 fs, list buffer 1.0 A",
+                   "systems": n, "sites_total": int(sum(sites)),
+                   "sites_range": [int(min(sites)), int(max(sites))]},
+        "sequential_ns_per_day": seq,
+        "batched_over_sequential": agg / seq,
+        "temperature_K_range": [min(temps), max(temps)],
+        "kernels_ms_per_step": {k: v_[0] / args.steps for k, v_ in prof.items()},
+    }
+    for e, m, _ in made:
+        m.close()
+        e.close()
     print(json.dumps(out))
     return 0
 

@@ -149,6 +149,14 @@ _sig("dyn_engine_debug_item_partials", C.c_int, [_p, C.c_void_p, C.c_uint64])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
                                           _p, C.POINTER(C.c_uint64), _p])
+_sig("dyn_batch_create", C.c_int, [C.POINTER(_p), C.c_uint32, C.POINTER(_p)])
+_sig("dyn_batch_destroy", None, [_p])
+_sig("dyn_batch_last_error", C.c_char_p, [_p])
+_sig("dyn_batch_step", C.c_int, [_p, C.c_uint32])
+_sig("dyn_batch_profile", C.c_int, [_p, C.c_int])
+_sig("dyn_batch_profile_read", C.c_int, [_p, C.c_uint32, C.POINTER(C.c_char_p),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint32)])
 _sig("dyn_engine_profile", C.c_int, [_p, C.c_int])
 _sig("dyn_engine_profile_read", C.c_int, [_p, C.c_uint32, C.POINTER(C.c_char_p),
                                           C.POINTER(C.c_double), C.POINTER(C.c_uint64),
@@ -171,6 +179,8 @@ EXPORTED = [
     "dyn_engine_debug_item_partials",
     "dyn_engine_profile",
     "dyn_engine_profile_read",
+    "dyn_batch_create", "dyn_batch_destroy", "dyn_batch_last_error", "dyn_batch_step",
+    "dyn_batch_profile", "dyn_batch_profile_read",
 ]
 
 
@@ -508,6 +518,43 @@ class Md:
         e = MdEnergies()
         _check(lib.dyn_md_get_energies(self.h, C.byref(e)), self._err)
         return {k: getattr(e, k) for k, _ in MdEnergies._fields_}
+
+
+class Batch:
+    """Independent systems stepped together (dyn_batch_*): every kernel of a
+    step is one launch for all members.  Members keep their own state;
+    Md.step refuses them while the batch lives."""
+
+    def __init__(self, mds):
+        self.mds = list(mds)
+        self.h = C.c_void_p()
+        arr = (_p * len(self.mds))(*[m.h for m in self.mds])
+        _check(lib.dyn_batch_create(C.byref(self.h), len(self.mds), arr), lib.dyn_global_last_error)
+
+    def close(self):
+        if self.h:
+            lib.dyn_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    __del__ = close
+
+    def _err(self):
+        return lib.dyn_batch_last_error(self.h)
+
+    def step(self, n=1):
+        _check(lib.dyn_batch_step(self.h, int(n)), self._err)
+
+    def profile(self, enable=True):
+        _check(lib.dyn_batch_profile(self.h, 1 if enable else 0), self._err)
+
+    def profile_read(self):
+        mx = 64
+        names = (C.c_char_p * mx)()
+        ms = (C.c_double * mx)()
+        cnt = (C.c_uint64 * mx)()
+        n = C.c_uint32()
+        _check(lib.dyn_batch_profile_read(self.h, mx, names, ms, cnt, C.byref(n)), self._err)
+        return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, mx))}
 
 
 def relax(engine, ff, pos_dyn, pos_water, steps=300, dt=0.002):

@@ -95,7 +95,7 @@ __device__ __forceinline__ uint32_t site_column(const ClassGrid& cg, float x, fl
 // Wrap every site into the box, find its x–y column and take a rank in it.
 // The rank order depends on atomic timing; the column sort that follows
 // orders each column by (z, site), so the clusters do not.
-__global__ void k_eng_bin_count(const float4* __restrict__ pos, const float* __restrict__ q,
+__device__ __forceinline__ void k_eng_bin_count(const float4* __restrict__ pos, const float* __restrict__ q,
                                 EngGeom g, float4* __restrict__ wpos, float4* __restrict__ wpos_ref,
                                 uint32_t* __restrict__ col_count, uint2* __restrict__ site_col) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,7 +115,7 @@ __global__ void k_eng_bin_count(const float4* __restrict__ pos, const float* __r
   site_col[s] = make_uint2(col, rank);
 }
 
-__global__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
+__device__ __forceinline__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
                                   const uint32_t* __restrict__ col_start, uint32_t n,
                                   uint32_t* __restrict__ items) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -131,7 +131,7 @@ __global__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
 // out_x[0..n] with out_x[n] = total, the totals to tot_x and max(a) to *max_a.
 constexpr int kScanBlock = 1024;
 template <int MODE>
-__global__ void __launch_bounds__(kScanBlock)
+__device__ __forceinline__ void
 k_eng_scan2(const uint32_t* __restrict__ in_a, const uint32_t* __restrict__ in_b, uint32_t n_host,
             const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ out_a,
             uint32_t* __restrict__ out_b, uint32_t* __restrict__ tot_a,
@@ -215,14 +215,15 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
 // cluster's bounding box.  The column's count is cleared for the next
 // binning pass (the binning kernel counts with atomics into zeroed counts).
 constexpr int kSortBlock = 256;
-__global__ void __launch_bounds__(kSortBlock)
+__device__ __forceinline__ void
 k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
               const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
               const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
               float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
-              unsigned int* __restrict__ status) {
+              unsigned int* __restrict__ status, uint32_t n_cols) {
   __shared__ unsigned long long key[kColCap];
   const int c = blockIdx.x;
+  if ((uint32_t)c >= n_cols) return;
   const uint32_t n = col_count[c];
   __syncthreads();
   if (threadIdx.x == 0 && n) col_count[c] = 0;
@@ -299,9 +300,11 @@ k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col
 }
 
 // Wrap all sites (list-reuse path).
-__global__ void k_eng_wrap(const float4* __restrict__ pos, const float* __restrict__ q,
-                           uint32_t n, EngGeom g, float4* __restrict__ wpos) {
+__device__ __forceinline__ void k_eng_wrap(const float4* __restrict__ pos, const float* __restrict__ q,
+                           uint32_t n, EngGeom g, float4* __restrict__ wpos,
+                           unsigned int* __restrict__ disp_bits) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) *disp_bits = 0u;  // k_eng_max_disp maximises into it next
   if (s >= n) return;
   const float4 p = pos[s];
   wpos[s] = make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
@@ -309,7 +312,7 @@ __global__ void k_eng_wrap(const float4* __restrict__ pos, const float* __restri
 }
 
 // Slot positions from the current wrapped positions (list-reuse path).
-__global__ void k_eng_slots(const int32_t* __restrict__ slot_site, const uint32_t* __restrict__ n_cl,
+__device__ __forceinline__ void k_eng_slots(const int32_t* __restrict__ slot_site, const uint32_t* __restrict__ n_cl,
                             const float4* __restrict__ wpos, float4* __restrict__ xyzq) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= *n_cl * DYN_CL_SIZE) return;
@@ -362,7 +365,7 @@ struct WalkLds {
 // j slot within rlist of at least one i slot it can interact with.  Each
 // class segment is padded to a multiple of 8 with the null slot.  The row
 // goes into its fixed-capacity slice of ent_tmp; counts are in groups of 8.
-__global__ void __launch_bounds__(kWave)
+__device__ __forceinline__ void
 k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
                 uint32_t* __restrict__ row_chunks) {
   __shared__ WalkLds L;
@@ -494,7 +497,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
 
 // Pass 1: compact the rows into cj (groups of 8 entries) and write the work
 // items (I, first group).
-__global__ void __launch_bounds__(kWave)
+__device__ __forceinline__ void
 k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ chunk_ptr, uint32_t* __restrict__ cj,
@@ -527,7 +530,7 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
 // in i slot ia and the group's j entry jb are a 1-2, 1-3 or 1-4 pair; the
 // lanes walk the i sites' exclusion lists.
 constexpr int kMaskCap = 2048;  // solute entries of one solute row
-__global__ void __launch_bounds__(kWave)
+__device__ __forceinline__ void
 k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ col_cl_start, unsigned long long* __restrict__ cmask) {
@@ -665,7 +668,7 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
 // wave per work item, so the hardware's wave dispatch balances the load and
 // every sum keeps a fixed order.
 template <bool ENERGY>
-__global__ void __launch_bounds__(kPairBlock)
+__device__ __forceinline__ void
 k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
   const int ia = lane & 7, jb = lane >> 3;
@@ -774,6 +777,7 @@ struct CorrArgs {
   uint32_t n;
   int mode;
   int recip_on;            // reciprocal corrections only accompany the SPME sum
+  uint32_t nblk;           // workgroups (one partial each)
   int64_t* fx;             // site forces, component-major
   uint32_t n_sites;
   double* partials;        // 6 per block: e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ
@@ -781,12 +785,13 @@ struct CorrArgs {
 
 constexpr int kCorrBlock = 256;
 constexpr int kCorrVals = 7;  // per-block sums of the correction kernel
-__global__ void __launch_bounds__(kCorrBlock)
+__device__ __forceinline__ void
 k_eng_corr(CorrArgs A) {
   __shared__ double red[kCorrBlock];
+  if (blockIdx.x >= A.nblk) return;  // partials exist for the system's own blocks
   // e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ, 1-4 pairs within the cutoff
   double acc[kCorrVals] = {0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += gridDim.x * kCorrBlock) {
+  for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += A.nblk * kCorrBlock) {
     if (A.kind[k] == 0 && !A.recip_on) continue;
     const uint32_t i = A.ci[k], j = A.cj[k];
     const float4 a = A.wpos[i], b = A.wpos[j];
@@ -875,7 +880,7 @@ k_eng_corr(CorrArgs A) {
 // gather the charged sites' (x, y, z, q) for the SPME spread.
 // With a decoupled molecule (sites [alch0, alch1)), pme_b receives a second
 // charge set with that molecule's charges negated.
-__global__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
+__device__ __forceinline__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
                                 const uint32_t* __restrict__ qsite, uint32_t nq,
                                 float4* __restrict__ pme_xyzq, float4* __restrict__ pme_b,
                                 uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site,
@@ -917,7 +922,7 @@ __device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fit
 // Site-order forces: the site accumulators (corrections), the SPME forces
 // (this evaluation's or the kept ones of an earlier one) and the pair
 // kernel's work-item rows of the site's slot (when the pair kernel ran).
-__global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
+__device__ __forceinline__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
                                const int64_t* __restrict__ fx_recip,
                                const int64_t* __restrict__ fitem,
                                const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
@@ -936,7 +941,7 @@ __global__ void k_eng_finalize(const int64_t* __restrict__ fx_site,
 }
 
 // Slot-order pair forces (component-major), for inspection.
-__global__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
+__device__ __forceinline__ void k_eng_slot_forces(const int64_t* __restrict__ fitem,
                                   const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
                                   const uint32_t* __restrict__ n_cl, uint32_t n_slots,
                                   int64_t* __restrict__ out) {
@@ -961,7 +966,7 @@ constexpr int kRed1Block = 256;
 
 // First stage of the pair-partial sums: workgroup b sums a fixed contiguous
 // share of the work items (kNumPart values each) into red1[b].
-__global__ void __launch_bounds__(kRed1Block)
+__device__ __forceinline__ void
 k_eng_energy_reduce1(const double* __restrict__ pp, const uint32_t* __restrict__ n_items_dev,
                      uint64_t stride, double* __restrict__ red1) {
   __shared__ double red[kNumPart][kRed1Block / kWave];
@@ -986,7 +991,7 @@ k_eng_energy_reduce1(const double* __restrict__ pp, const uint32_t* __restrict__
   }
 }
 
-__global__ void __launch_bounds__(kRedBlock)
+__device__ __forceinline__ void
 k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ cp,
                     uint32_t n_cblk, int pme_runs, double* __restrict__ out) {
   // pme_runs < 0: the SPME sum was kept from an earlier evaluation, with its
@@ -1015,7 +1020,7 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
   if (pme_runs >= 0 && t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
 }
 
-__global__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
+__device__ __forceinline__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
                                uint32_t n, EngGeom g, unsigned int* __restrict__ out_bits) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -1093,16 +1098,9 @@ struct dyn_engine {
   DevBuf<int64_t> fitem, fx_site, fx_out, fx_slot;  // fx_slot: inspection only
   DevBuf<double> pair_part, corr_part, eng_out, red1;
   unsigned corr_blocks = 0;
-  DevBuf<unsigned int> disp_bits;
-  // displacement readings of asynchronous computes (squared Å, bit pattern
-  // of a non-negative float), consumed when their event has completed
-  static constexpr int kDispRing = 4;
-  hipEvent_t disp_ev[kDispRing] = {};
-  PinnedBuf<float> h_disp;
-  uint64_t disp_gen[kDispRing] = {};
-  bool disp_pending[kDispRing] = {};
-  int disp_next = 0;
-  float disp_seen = 0.f;  // largest reading of the current list so far
+  // a pass over this engine alone: argument packs and displacement readings
+  ArgCache own_cache;
+  DispRing own_disp;
   // PME
   bool pme_on = false;
   PmeCore pme;
@@ -1115,19 +1113,48 @@ struct dyn_engine {
   PinnedBuf<uint32_t> h_misc;  // [0] status, [1..5] counters, [6] displacement bits
   bool fx_fresh = false;
 
+  hipStream_t own_stream = nullptr;  // the engine's stream (a batch may lend it another)
+
   ~dyn_engine() {
-    for (auto& e : disp_ev)
-      if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
   void set_geometry();
   void reserve_rows(uint32_t entries);
   void reserve_list(uint64_t groups, uint64_t items);
-  void build_list();
-  void enqueue(uint32_t flags);
   void check_status();
+  EngineSet self_set() {
+    EngineSet S;
+    S.e = {this};
+    S.R.stream = stream;
+    S.R.cache = &own_cache;
+    S.R.n = 1;
+    S.prof = &prof;
+    S.disp = &own_disp;
+    return S;
+  }
 };
+
+namespace dyn {
+
+void DispRing::init(uint32_t systems) {
+  if (systems == n && bits.p) return;
+  n = systems;
+  bits.reserve(std::max<uint32_t>(n, 1));
+  h.reserve((size_t)kRing * std::max<uint32_t>(n, 1));
+  h_sync.reserve(std::max<uint32_t>(n, 1));
+  for (auto& e : ev)
+    if (!e) DYN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  seen.assign(n, 0.f);
+  for (int k = 0; k < kRing; ++k) pending[k] = false;
+}
+
+DispRing::~DispRing() {
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+}  // namespace dyn
 
 void dyn_engine::set_geometry() {
   const double V = (double)box[0] * box[1] * box[2];
@@ -1232,259 +1259,408 @@ void dyn_engine::reserve_list(uint64_t groups, uint64_t n_items) {
   }
 }
 
-void dyn_engine::build_list() {
+namespace dyn {
+
+// Largest per-system grid of a launch.
+template <class Fn>
+static unsigned max_grid(const EngineSet& S, Fn&& fn) {
+  unsigned g = 1;
+  for (dyn_engine* e : S.e) g = std::max(g, (unsigned)fn(*e));
+  return g;
+}
+
+// Cluster-pair lists of every engine of the set.
+static void set_build_lists(EngineSet& S) {
+  auto& E = S.e;
+  const BatchRun& R = S.R;
+  Profiler& prof = *S.prof;
   {
     Profiler::Scope ps(prof, "nbl_bin");
-    const bool track = prm.list_buffer > 0.f;
-    hipLaunchKernelGGL(k_eng_bin_count, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       pos_dev, q.p, geom, wpos.p, track ? wpos_ref.p : (float4*)nullptr,
-                       col_count.p, site_col.p);
-    DYN_CHECK_LAUNCH("k_eng_bin_count");
-    hipLaunchKernelGGL(k_eng_scan2<0>, dim3(1), dim3(kScanBlock), 0, stream, col_count.p,
-                       (const uint32_t*)nullptr, n_cols, (const uint32_t*)nullptr, col_start.p,
-                       col_cl_start.p, (uint32_t*)nullptr, counters.p + 0, (uint32_t*)nullptr);
-    DYN_CHECK_LAUNCH("k_eng_scan2<0>");
-    hipLaunchKernelGGL(k_eng_bin_scatter, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       site_col.p, col_start.p, n_sites, items.p);
-    DYN_CHECK_LAUNCH("k_eng_bin_scatter");
+    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
+    launch<k_eng_bin_count>(R, "k_eng_bin_count", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      const bool track = e.prm.list_buffer > 0.f;
+      return args<k_eng_bin_count>(e.pos_dev, e.q.p, e.geom, e.wpos.p,
+                                   track ? e.wpos_ref.p : (float4*)nullptr, e.col_count.p,
+                                   e.site_col.p);
+    });
+    launch<k_eng_scan2<0>, kScanBlock>(R, "k_eng_scan2<0>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_scan2<0>>(e.col_count.p, (const uint32_t*)nullptr, e.n_cols,
+                                  (const uint32_t*)nullptr, e.col_start.p, e.col_cl_start.p,
+                                  (uint32_t*)nullptr, e.counters.p + 0, (uint32_t*)nullptr);
+    });
+    launch<k_eng_bin_scatter>(R, "k_eng_bin_scatter", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_bin_scatter>(e.site_col.p, e.col_start.p, e.n_sites, e.items.p);
+    });
   }
   {
     Profiler::Scope ps(prof, "nbl_colsort");
-    SlotArgs S{wpos.p, ltype.p, wmol.p, type_none, xyzq.p, slot_type.p, slot_gid.p, slot_site.p, slot_of.p};
-    hipLaunchKernelGGL(k_eng_colsort, dim3(n_cols), dim3(kSortBlock), 0, stream, col_count.p,
-                       col_start.p, items.p, col_cl_start.p, col_class.p, S, cl_class.p, bb_c.p,
-                       bb_h.p, n_sites, status.p);
-    DYN_CHECK_LAUNCH("k_eng_colsort");
+    const unsigned gc = max_grid(S, [](dyn_engine& e) { return e.n_cols; });
+    launch<k_eng_colsort, kSortBlock>(R, "k_eng_colsort", 0, gc, kSortBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      SlotArgs A;
+      std::memset(&A, 0, sizeof A);
+      A = SlotArgs{e.wpos.p, e.ltype.p, e.wmol.p, e.type_none, e.xyzq.p, e.slot_type.p,
+                   e.slot_gid.p, e.slot_site.p, e.slot_of.p};
+      return args<k_eng_colsort>(e.col_count.p, e.col_start.p, e.items.p, e.col_cl_start.p,
+                                 e.col_class.p, A, e.cl_class.p, e.bb_c.p, e.bb_h.p, e.n_sites,
+                                 e.status.p, e.n_cols);
+    });
   }
-  ListArgs A;
-  A.g = geom;
-  A.rlist = prm.cutoff + std::max(0.f, prm.list_buffer);
-  A.rlist2 = A.rlist * A.rlist;
-  A.n_cl = counters.p + 0;
-  A.bb_c = bb_c.p;
-  A.bb_h = bb_h.p;
-  A.col_cl_start = col_cl_start.p;
-  A.cl_class = cl_class.p;
-  A.slot_site = slot_site.p;
-  A.xyzq = xyzq.p;
-  A.slot_gid = slot_gid.p;
-  A.slot_of = slot_of.p;
-  A.ex_ptr = ex_ptr.p;
-  A.ex_idx = ex_idx.p;
-  A.n_dyn = n_dyn;
-  A.cap_entries = cap_entries;
-  A.cap_chunks = cap_chunks;
-  A.row_cap = row_cap;
-  A.null_slot = cap_slots;
-  A.longest = counters.p + 4;
-  A.status = status.p;
-  const unsigned list_grid = std::min<uint32_t>(cap_cl, 16384);
+  std::vector<ListArgs> LA(E.size());
+  for (size_t i = 0; i < E.size(); ++i) {
+    dyn_engine& e = *E[i];
+    ListArgs& A = LA[i];
+    std::memset(&A, 0, sizeof A);
+    A.g = e.geom;
+    A.rlist = e.prm.cutoff + std::max(0.f, e.prm.list_buffer);
+    A.rlist2 = A.rlist * A.rlist;
+    A.n_cl = e.counters.p + 0;
+    A.bb_c = e.bb_c.p;
+    A.bb_h = e.bb_h.p;
+    A.col_cl_start = e.col_cl_start.p;
+    A.cl_class = e.cl_class.p;
+    A.slot_site = e.slot_site.p;
+    A.xyzq = e.xyzq.p;
+    A.slot_gid = e.slot_gid.p;
+    A.slot_of = e.slot_of.p;
+    A.ex_ptr = e.ex_ptr.p;
+    A.ex_idx = e.ex_idx.p;
+    A.n_dyn = e.n_dyn;
+    A.cap_entries = e.cap_entries;
+    A.cap_chunks = e.cap_chunks;
+    A.row_cap = e.row_cap;
+    A.null_slot = e.cap_slots;
+    A.longest = e.counters.p + 4;
+    A.status = e.status.p;
+  }
+  const unsigned list_grid = max_grid(S, [](dyn_engine& e) { return std::min<uint32_t>(e.cap_cl, 16384); });
   {
     Profiler::Scope ps(prof, "nbl_list_walk");
-    hipLaunchKernelGGL(k_eng_list_walk, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
-                       row_cnt.p, row_chunks.p);
-    DYN_CHECK_LAUNCH("k_eng_list_walk");
-    hipLaunchKernelGGL(k_eng_scan2<1>, dim3(1), dim3(kScanBlock), 0, stream, row_cnt.p,
-                       row_chunks.p, 0u, counters.p + 0, row_ptr.p, chunk_ptr.p, counters.p + 1,
-                       counters.p + 2, counters.p + 3);
-    DYN_CHECK_LAUNCH("k_eng_scan2<1>");
+    launch<k_eng_list_walk, kWave>(R, "k_eng_list_walk", 0, list_grid, kWave, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p);
+    });
+    launch<k_eng_scan2<1>, kScanBlock>(R, "k_eng_scan2<1>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_scan2<1>>(e.row_cnt.p, e.row_chunks.p, 0u, e.counters.p + 0, e.row_ptr.p,
+                                  e.chunk_ptr.p, e.counters.p + 1, e.counters.p + 2,
+                                  e.counters.p + 3);
+    });
   }
   {
     Profiler::Scope ps(prof, "nbl_list_fill");
-    hipLaunchKernelGGL(k_eng_list_fill, dim3(list_grid), dim3(kWave), 0, stream, A, ent_tmp.p,
-                       row_cnt.p, row_ptr.p, chunk_ptr.p, cj.p, chunks.p);
-    DYN_CHECK_LAUNCH("k_eng_list_fill");
-    if (n_dyn) {
-      const uint32_t sol_rows = n_dyn / DYN_CL_SIZE + (uint32_t)(geom.cg[0].ncx * geom.cg[0].ncy) + 1;
-      hipLaunchKernelGGL(k_eng_list_mask, dim3(std::min<uint32_t>(sol_rows, 4096)), dim3(kWave), 0,
-                         stream, A, ent_tmp.p, row_cnt.p, row_ptr.p, col_cl_start.p, cmask.p);
-      DYN_CHECK_LAUNCH("k_eng_list_mask");
+    launch<k_eng_list_fill, kWave>(R, "k_eng_list_fill", 0, list_grid, kWave, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_list_fill>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_ptr.p, e.chunk_ptr.p,
+                                   e.cj.p, e.chunks.p);
+    });
+    bool any_solute = false;
+    for (dyn_engine* e : E) any_solute = any_solute || e->n_dyn > 0;
+    if (any_solute) {
+      const unsigned gm = max_grid(S, [](dyn_engine& e) {
+        const uint32_t sol_rows =
+            e.n_dyn / DYN_CL_SIZE + (uint32_t)(e.geom.cg[0].ncx * e.geom.cg[0].ncy) + 1;
+        return std::min<uint32_t>(sol_rows, 4096);
+      });
+      launch<k_eng_list_mask, kWave>(R, "k_eng_list_mask", 0, gm, kWave, 0, [&](uint32_t i) {
+        dyn_engine& e = *E[i];
+        return args<k_eng_list_mask>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_ptr.p,
+                                     e.col_cl_start.p, e.cmask.p);
+      });
     }
   }
-  list_valid = true;
-  ++n_builds;
-  disp_seen = 0.f;
+  for (dyn_engine* e : E) {
+    e->list_valid = true;
+    ++e->n_builds;
+  }
+  ++S.disp->builds;
+  std::fill(S.disp->seen.begin(), S.disp->seen.end(), 0.f);
 }
 
-void dyn_engine::enqueue(uint32_t flags) {
-  DYN_REQUIRE(box_set, "box not set");
-  if (!pos_set) throw Error(DYN_ERR_STATE, "positions not set");
-  bool rebuild = (flags & DYN_COMPUTE_REBUILD) || !list_valid || prm.list_buffer <= 0.f;
+void engine_set_enqueue(EngineSet& S, uint32_t flags) {
+  auto& E = S.e;
+  const uint32_t ns = (uint32_t)E.size();
+  const BatchRun& R = S.R;
+  Profiler& prof = *S.prof;
+  DispRing& D = *S.disp;
+  D.init(ns);
+  bool rebuild = (flags & DYN_COMPUTE_REBUILD) != 0;
+  for (dyn_engine* e : E) {
+    DYN_REQUIRE(e->box_set, "box not set");
+    if (!e->pos_set) throw Error(DYN_ERR_STATE, "positions not set");
+    if (!e->list_valid || e->prm.list_buffer <= 0.f) rebuild = true;
+  }
   const bool async = (flags & DYN_COMPUTE_ASYNC) != 0;
   if (!rebuild && async) {
     // Asynchronous computes never wait for the displacement: they use the
-    // newest readings of earlier computes on this list (a few steps old) and
-    // rebuild at 0.8 × buffer/2, which leaves the rest of the half buffer
-    // for the motion since those readings.
-    for (int k = 0; k < kDispRing; ++k)
-      if (disp_pending[k] && hipEventQuery(disp_ev[k]) == hipSuccess) {
-        disp_pending[k] = false;
-        if (disp_gen[k] == n_builds) disp_seen = std::max(disp_seen, h_disp.p[k]);
+    // newest readings of earlier computes on these lists (a few steps old)
+    // and rebuild at 0.8 × buffer/2, which leaves the rest of the half
+    // buffer for the motion since those readings.
+    for (int k = 0; k < DispRing::kRing; ++k)
+      if (D.pending[k] && hipEventQuery(D.ev[k]) == hipSuccess) {
+        D.pending[k] = false;
+        if (D.gen[k] == D.builds)
+          for (uint32_t i = 0; i < ns; ++i) {
+            float d2;
+            std::memcpy(&d2, D.h.p + (size_t)k * ns + i, 4);
+            D.seen[i] = std::max(D.seen[i], d2);
+          }
       }
-    const float lag_half = 0.4f * prm.list_buffer;
-    if (!(disp_seen < lag_half * lag_half)) rebuild = true;
+    for (uint32_t i = 0; i < ns; ++i) {
+      const float lag_half = 0.4f * E[i]->prm.list_buffer;
+      if (!(D.seen[i] < lag_half * lag_half)) rebuild = true;
+    }
   }
   if (!rebuild) {
-    hipLaunchKernelGGL(k_eng_wrap, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, pos_dev,
-                       q.p, n_sites, geom, wpos.p);
-    DYN_CHECK_LAUNCH("k_eng_wrap");
-    DYN_HIP(hipMemsetAsync(disp_bits.p, 0, 4, stream));
-    hipLaunchKernelGGL(k_eng_max_disp, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       wpos.p, wpos_ref.p, n_sites, geom, disp_bits.p);
+    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
+    launch<k_eng_wrap>(R, "k_eng_wrap", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_wrap>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p, D.bits.p + i);
+    });
+    launch<k_eng_max_disp>(R, "k_eng_max_disp", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_max_disp>(e.wpos.p, e.wpos_ref.p, e.n_sites, e.geom, D.bits.p + i);
+    });
     if (async) {
-      const int k = disp_next;
-      disp_next = (disp_next + 1) % kDispRing;
-      if (!disp_pending[k]) {  // a slot still in flight is skipped, not overwritten
-        DYN_HIP(hipMemcpyAsync(h_disp.p + k, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
-        DYN_HIP(hipEventRecord(disp_ev[k], stream));
-        disp_pending[k] = true;
-        disp_gen[k] = n_builds;
+      const int k = D.next;
+      D.next = (D.next + 1) % DispRing::kRing;
+      if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
+        DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
+                               hipMemcpyDeviceToHost, R.stream));
+        DYN_HIP(hipEventRecord(D.ev[k], R.stream));
+        D.pending[k] = true;
+        D.gen[k] = D.builds;
       }
     } else {
-      DYN_HIP(hipMemcpyAsync(h_misc.p + 6, disp_bits.p, 4, hipMemcpyDeviceToHost, stream));
-      DYN_HIP(hipStreamSynchronize(stream));
-      float d2;
-      memcpy(&d2, h_misc.p + 6, 4);
-      const float half = 0.5f * prm.list_buffer;
-      if (!(d2 < half * half)) rebuild = true;
+      DYN_HIP(hipMemcpyAsync(D.h_sync.p, D.bits.p, 4 * (size_t)ns, hipMemcpyDeviceToHost,
+                             R.stream));
+      DYN_HIP(hipStreamSynchronize(R.stream));
+      for (uint32_t i = 0; i < ns; ++i) {
+        float d2;
+        std::memcpy(&d2, D.h_sync.p + i, 4);
+        const float half = 0.5f * E[i]->prm.list_buffer;
+        if (!(d2 < half * half)) rebuild = true;
+      }
     }
   }
   if (rebuild) {
-    build_list();
+    set_build_lists(S);
   } else {
     Profiler::Scope ps(prof, "slots");
-    hipLaunchKernelGGL(k_eng_slots, dim3(grid_for(cap_slots, 256)), dim3(256), 0, stream,
-                       slot_site.p, counters.p + 0, wpos.p, xyzq.p);
-    DYN_CHECK_LAUNCH("k_eng_slots");
+    const unsigned gk = max_grid(S, [](dyn_engine& e) { return grid_for(e.cap_slots, 256); });
+    launch<k_eng_slots>(R, "k_eng_slots", 0, gk, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_slots>(e.slot_site.p, e.counters.p + 0, e.wpos.p, e.xyzq.p);
+    });
   }
-  DynPairParams P;
-  for (int a = 0; a < 3; ++a) {
-    P.ext[a] = box[a];
-    P.inv_ext[a] = 1.0f / box[a];
-  }
-  P.rc2 = prm.cutoff * prm.cutoff;
-  P.lj_rc2 = prm.lj_cutoff * prm.lj_cutoff;
-  P.beta = prm.beta;
-  P.beta2 = prm.beta * prm.beta;
-  P.two_b_sqrtpi = 2.0f * prm.beta * DYN_INV_SQRT_PI;
-  P.c6_oo = c6_oo;
-  P.c12_oo = c12_oo;
-  P.n_types = n_types;
-  const bool alch_on = alch1 > alch0;
-  P.alch_on = alch_on ? 1 : 0;
-  P.sch = dyn_schedule((float)lambda);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
-  // the reference's SPME cadence: keep the last SPME forces, energy and
-  // virial (reference mode, no decoupled molecule, same box)
-  const bool reuse = pme_on && (flags & DYN_COMPUTE_RECIP_REUSE) && !(flags & DYN_COMPUTE_RECIP) &&
-                     recip_kept && !alch_on && prm.mode == DYN_MODE_REFERENCE;
-  const bool recip_on = pme_on && ((flags & DYN_COMPUTE_RECIP) ||
-                                   ((flags & DYN_COMPUTE_RECIP_REUSE) && !reuse));
-  const bool corr_on = (flags & DYN_COMPUTE_CORR) && n_corr;
-  // with a decoupled molecule the SPME sum runs twice, with its charges as
-  // they are (A) and negated (B): ((1+s)A + (1−s)B)/2 keeps the
-  // environment and molecule self terms and scales their cross term by s
-  const bool recip_b = recip_on && alch_on;
-  if (!reuse) {
-    last_recip_b = recip_b;
-    last_sch = P.sch;
+  // per system: pair parameters and which reciprocal work it does
+  struct Plan {
+    DynPairParams P;
+    bool alch_on, reuse, recip_on, recip_b, corr_on;
+  };
+  std::vector<Plan> PL(ns);
+  bool any_recip = false, any_b = false, any_corr = false;
+  for (uint32_t i = 0; i < ns; ++i) {
+    dyn_engine& e = *E[i];
+    Plan& pl = PL[i];
+    DynPairParams& P = pl.P;
+    std::memset(&P, 0, sizeof P);
+    for (int a = 0; a < 3; ++a) {
+      P.ext[a] = e.box[a];
+      P.inv_ext[a] = 1.0f / e.box[a];
+    }
+    P.rc2 = e.prm.cutoff * e.prm.cutoff;
+    P.lj_rc2 = e.prm.lj_cutoff * e.prm.lj_cutoff;
+    P.beta = e.prm.beta;
+    P.beta2 = e.prm.beta * e.prm.beta;
+    P.two_b_sqrtpi = 2.0f * e.prm.beta * DYN_INV_SQRT_PI;
+    P.c6_oo = e.c6_oo;
+    P.c12_oo = e.c12_oo;
+    P.n_types = e.n_types;
+    pl.alch_on = e.alch1 > e.alch0;
+    P.alch_on = pl.alch_on ? 1 : 0;
+    P.sch = dyn_schedule((float)e.lambda);
+    // the reference's SPME cadence: keep the last SPME forces, energy and
+    // virial (reference mode, no decoupled molecule, same box)
+    pl.reuse = e.pme_on && (flags & DYN_COMPUTE_RECIP_REUSE) && !(flags & DYN_COMPUTE_RECIP) &&
+               e.recip_kept && !pl.alch_on && e.prm.mode == DYN_MODE_REFERENCE;
+    pl.recip_on = e.pme_on && ((flags & DYN_COMPUTE_RECIP) ||
+                               ((flags & DYN_COMPUTE_RECIP_REUSE) && !pl.reuse));
+    pl.corr_on = (flags & DYN_COMPUTE_CORR) && e.n_corr;
+    // with a decoupled molecule the SPME sum runs twice, with its charges as
+    // they are (A) and negated (B): ((1+s)A + (1−s)B)/2 keeps the
+    // environment and molecule self terms and scales their cross term by s
+    pl.recip_b = pl.recip_on && pl.alch_on;
+    if (!pl.reuse) {
+      e.last_recip_b = pl.recip_b;
+      e.last_sch = P.sch;
+    }
+    any_recip = any_recip || pl.recip_on;
+    any_b = any_b || pl.recip_b;
+    any_corr = any_corr || pl.corr_on;
   }
   {
     Profiler::Scope ps(prof, "site_prep");
-    hipLaunchKernelGGL(k_eng_site_prep, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream, wpos.p,
-                       n_sites, qsite.p, n_q, recip_on ? pme_xyzq.p : (float4*)nullptr,
-                       recip_b ? pme_b.p : (float4*)nullptr, alch0, alch1, fx_site.p,
-                       recip_on ? fx_recip.p : (int64_t*)nullptr);
-    DYN_CHECK_LAUNCH("k_eng_site_prep");
+    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
+    launch<k_eng_site_prep>(R, "k_eng_site_prep", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      const Plan& pl = PL[i];
+      return args<k_eng_site_prep>(e.wpos.p, e.n_sites, e.qsite.p, e.n_q,
+                                   pl.recip_on ? e.pme_xyzq.p : (float4*)nullptr,
+                                   pl.recip_b ? e.pme_b.p : (float4*)nullptr, e.alch0, e.alch1,
+                                   e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr);
+    });
   }
   if (short_on) {
     Profiler::Scope ps(prof, "pairs");
-    PairArgs A;
-    A.P = P;
-    A.xyzq = xyzq.p;
-    A.slot_type = slot_type.p;
-    A.slot_gid = slot_gid.p;
-    A.cl_class = cl_class.p;
-    A.row_ptr = row_ptr.p;
-    A.cj = cj.p;
-    A.cmask = cmask.p;
-    A.chunks = chunks.p;
-    A.n_chunks = counters.p + 2;
-    A.lj_table = lj_table.p;
-    A.lj_se = lj_se.p;
-    A.fitem = fitem.p;
-    A.partials = pair_part.p;
-    A.cap_chunks = cap_chunks;
-    A.status = status.p;
-    // a wave per work item (the count is known on the device only)
-    const unsigned blocks = (unsigned)std::max<uint64_t>(1, (cap_chunks + 3) / 4);
-    if (energy_on)
-      hipLaunchKernelGGL(k_eng_pairs<true>, dim3(blocks), dim3(kPairBlock), 0, stream, A);
-    else
-      hipLaunchKernelGGL(k_eng_pairs<false>, dim3(blocks), dim3(kPairBlock), 0, stream, A);
-    DYN_CHECK_LAUNCH("k_eng_pairs");
-  }
-  if (corr_on) {
-    Profiler::Scope ps(prof, "corr");
-    CorrArgs C;
-    C.P = P;
-    C.wpos = wpos.p;
-    C.ltype = ltype.p;
-    C.lj_table = lj_table.p;
-    C.lj_se = lj_se.p;
-    C.ci = corr_i.p;
-    C.cj = corr_j.p;
-    C.kind = corr_kind.p;
-    C.n = n_corr;
-    C.mode = (int)prm.mode;
-    C.recip_on = recip_on ? 1 : 0;
-    C.fx = fx_site.p;
-    C.n_sites = n_sites;
-    C.partials = corr_part.p;
-    hipLaunchKernelGGL(k_eng_corr, dim3(corr_blocks), dim3(kCorrBlock), 0, stream, C);
-    DYN_CHECK_LAUNCH("k_eng_corr");
-  }
-  if (recip_on) {
-    if (!recip_b) {
-      pme.run(n_q, pme_xyzq.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 1.0f, eng_out.p + 16);
-    } else {
-      const float s = P.sch.coulomb_scale;
-      pme.run(n_q, pme_xyzq.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 0.5f * (1.0f + s),
-              eng_out.p + 16);
-      pme.run(n_q, pme_b.p, qsite.p, fx_recip.p, 1, n_sites, nullptr, 0.5f * (1.0f - s),
-              eng_out.p + 18);
+    std::vector<PairArgs> PA(ns);
+    for (uint32_t i = 0; i < ns; ++i) {
+      dyn_engine& e = *E[i];
+      PairArgs& A = PA[i];
+      std::memset(&A, 0, sizeof A);
+      A.P = PL[i].P;
+      A.xyzq = e.xyzq.p;
+      A.slot_type = e.slot_type.p;
+      A.slot_gid = e.slot_gid.p;
+      A.cl_class = e.cl_class.p;
+      A.row_ptr = e.row_ptr.p;
+      A.cj = e.cj.p;
+      A.cmask = e.cmask.p;
+      A.chunks = e.chunks.p;
+      A.n_chunks = e.counters.p + 2;
+      A.lj_table = e.lj_table.p;
+      A.lj_se = e.lj_se.p;
+      A.fitem = e.fitem.p;
+      A.partials = e.pair_part.p;
+      A.cap_chunks = e.cap_chunks;
+      A.status = e.status.p;
     }
+    // a wave per work item (the count is known on the device only)
+    const unsigned gp = max_grid(S, [](dyn_engine& e) {
+      return (unsigned)std::max<uint64_t>(1, (e.cap_chunks + 3) / 4);
+    });
+    if (energy_on)
+      launch<k_eng_pairs<true>, kPairBlock>(R, "k_eng_pairs<true>", 0, gp, kPairBlock, 0,
+                                            [&](uint32_t i) { return args<k_eng_pairs<true>>(PA[i]); });
+    else
+      launch<k_eng_pairs<false>, kPairBlock>(R, "k_eng_pairs<false>", 0, gp, kPairBlock, 0,
+                                             [&](uint32_t i) { return args<k_eng_pairs<false>>(PA[i]); });
+  }
+  if (any_corr) {
+    Profiler::Scope ps(prof, "corr");
+    const unsigned gc = max_grid(S, [](dyn_engine& e) { return e.corr_blocks; });
+    launch<k_eng_corr, kCorrBlock>(R, "k_eng_corr", 0, gc, kCorrBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      CorrArgs C;
+      std::memset(&C, 0, sizeof C);
+      C.P = PL[i].P;
+      C.wpos = e.wpos.p;
+      C.ltype = e.ltype.p;
+      C.lj_table = e.lj_table.p;
+      C.lj_se = e.lj_se.p;
+      C.ci = e.corr_i.p;
+      C.cj = e.corr_j.p;
+      C.kind = e.corr_kind.p;
+      C.n = PL[i].corr_on ? e.n_corr : 0u;
+      C.mode = (int)e.prm.mode;
+      C.recip_on = PL[i].recip_on ? 1 : 0;
+      C.nblk = e.corr_blocks;
+      C.fx = e.fx_site.p;
+      C.n_sites = e.n_sites;
+      C.partials = e.corr_part.p;
+      return args<k_eng_corr>(C);
+    });
+  }
+  if (any_recip) {
+    std::vector<PmeRun> ra(ns), rb(ns);
+    for (uint32_t i = 0; i < ns; ++i) {
+      dyn_engine& e = *E[i];
+      const Plan& pl = PL[i];
+      if (!pl.recip_on) continue;
+      const float s = pl.recip_b ? pl.P.sch.coulomb_scale : 1.0f;
+      PmeRun r;
+      r.core = &e.pme;
+      r.n = e.n_q;
+      r.xyzq = e.pme_xyzq.p;
+      r.site = e.qsite.p;
+      r.fx = e.fx_recip.p;
+      r.comp_major = 1;
+      r.nsites = e.n_sites;
+      r.fscale = pl.recip_b ? 0.5f * (1.0f + s) : 1.0f;
+      r.res = e.eng_out.p + 16;
+      ra[i] = r;
+      if (pl.recip_b) {
+        r.xyzq = e.pme_b.p;
+        r.fscale = 0.5f * (1.0f - s);
+        r.res = e.eng_out.p + 18;
+        rb[i] = r;
+      }
+    }
+    pme_run(R, prof, ra.data(), 0);
+    if (any_b) pme_run(R, prof, rb.data(), 1);
   }
   {
     Profiler::Scope ps(prof, "finalize");
-    hipLaunchKernelGGL(k_eng_finalize, dim3(grid_for(n_sites, 256)), dim3(256), 0, stream,
-                       fx_site.p, (recip_on || reuse) ? fx_recip.p : (const int64_t*)nullptr,
-                       short_on ? fitem.p : (const int64_t*)nullptr, chunk_ptr.p,
-                       cap_chunks, slot_of.p, n_sites, cap_slots, fx_out.p);
-    DYN_CHECK_LAUNCH("k_eng_finalize");
-    if (short_on && energy_on) {
-      hipLaunchKernelGGL(k_eng_energy_reduce1, dim3(kRed1Grid), dim3(kRed1Block), 0, stream,
-                         (const double*)pair_part.p, counters.p + 2, cap_chunks, red1.p);
-      DYN_CHECK_LAUNCH("k_eng_energy_reduce1");
-    }
+    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
+    launch<k_eng_finalize>(R, "k_eng_finalize", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      const Plan& pl = PL[i];
+      return args<k_eng_finalize>(e.fx_site.p,
+                                  (pl.recip_on || pl.reuse) ? e.fx_recip.p : (const int64_t*)nullptr,
+                                  short_on ? e.fitem.p : (const int64_t*)nullptr, e.chunk_ptr.p,
+                                  e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p);
+    });
+    if (short_on && energy_on)
+      launch<k_eng_energy_reduce1, kRed1Block>(R, "k_eng_energy_reduce1", 0, kRed1Grid, kRed1Block,
+                                               0, [&](uint32_t i) {
+        dyn_engine& e = *E[i];
+        return args<k_eng_energy_reduce1>((const double*)e.pair_part.p, e.counters.p + 2,
+                                          e.cap_chunks, e.red1.p);
+      });
     if (energy_on)
-    hipLaunchKernelGGL(k_eng_energy_reduce, dim3(1), dim3(kRedBlock), 0, stream,
-                       short_on ? (const double*)red1.p : (const double*)nullptr, corr_part.p,
-                       corr_on ? corr_blocks : 0u, reuse ? -1 : recip_on ? (recip_b ? 2 : 1) : 0,
-                       eng_out.p);
-    DYN_CHECK_LAUNCH("k_eng_energy_reduce");
+      launch<k_eng_energy_reduce, kRedBlock>(R, "k_eng_energy_reduce", 0, 1, kRedBlock, 0,
+                                             [&](uint32_t i) {
+        dyn_engine& e = *E[i];
+        const Plan& pl = PL[i];
+        return args<k_eng_energy_reduce>(
+            short_on ? (const double*)e.red1.p : (const double*)nullptr, e.corr_part.p,
+            pl.corr_on ? e.corr_blocks : 0u,
+            pl.reuse ? -1 : pl.recip_on ? (pl.recip_b ? 2 : 1) : 0, e.eng_out.p);
+      });
   }
-  if (recip_on) recip_kept = true;
-  else if (!reuse) recip_kept = false;
-  computed = true;
-  energies_valid = energy_on;
-  fx_fresh = false;
+  for (uint32_t i = 0; i < ns; ++i) {
+    dyn_engine& e = *E[i];
+    if (PL[i].recip_on) e.recip_kept = true;
+    else if (!PL[i].reuse) e.recip_kept = false;
+    e.computed = true;
+    e.energies_valid = energy_on;
+    e.fx_fresh = false;
+  }
 }
 
-// Reads the status bits and counters (synchronizes).  On an overflow the
-// buffers grow to fit and the caller gets DYN_ERR_CAPACITY; the list is
-// rebuilt on the next call.
+void engine_set_check(EngineSet& S) {
+  const Error* first = nullptr;
+  Error keep(0, "");
+  for (dyn_engine* e : S.e) {
+    try {
+      e->check_status();
+    } catch (const Error& err) {
+      if (!first) {
+        keep = err;
+        first = &keep;
+      }
+    }
+  }
+  if (first) throw keep;
+}
+
+}  // namespace dyn
+
 void dyn_engine::check_status() {
   DYN_HIP(hipMemcpyAsync(h_misc.p, status.p, 4, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 20, hipMemcpyDeviceToHost, stream));
@@ -1544,6 +1720,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->device = device;
     DeviceGuard g(device);
     DYN_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+    E->own_stream = E->stream;
     E->prof.stream = E->stream;
     E->prm = p;
     E->n_dyn = n_dyn;
@@ -1688,9 +1865,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->h_misc.reserve(8);
     E->counters.reserve(8);
     E->status.reserve(1);
-    E->disp_bits.reserve(1);
-    E->h_disp.reserve(dyn_engine::kDispRing);
-    for (auto& e : E->disp_ev) DYN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    E->own_disp.init(1);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_b.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
@@ -1820,16 +1995,17 @@ extern "C" int dyn_engine_compute(dyn_engine* eng, uint32_t flags) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
     DeviceGuard g(eng->device);
+    EngineSet S = eng->self_set();
     if (flags & DYN_COMPUTE_ASYNC) {
-      eng->enqueue(flags);
+      engine_set_enqueue(S, flags);
     } else {
       // a list that outgrew its buffers is rebuilt once with enlarged ones
-      eng->enqueue(flags);
+      engine_set_enqueue(S, flags);
       try {
         eng->check_status();
       } catch (const Error& e) {
         if (e.code != DYN_ERR_CAPACITY || eng->list_valid) throw;
-        eng->enqueue(flags | DYN_COMPUTE_REBUILD);
+        engine_set_enqueue(S, flags | DYN_COMPUTE_REBUILD);
         eng->check_status();
       }
     }
@@ -1973,10 +2149,12 @@ extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint3
     if (items && nit) DYN_HIP(hipMemcpyAsync(items, eng->chunks.p, 8 * nit, hipMemcpyDeviceToHost, s));
     if (fx_slot) {
       eng->fx_slot.reserve(3 * (size_t)eng->cap_slots);
-      hipLaunchKernelGGL(k_eng_slot_forces, dim3(grid_for(eng->cap_slots, 256)), dim3(256), 0, s,
-                         eng->fitem.p, eng->chunk_ptr.p, eng->cap_chunks, eng->counters.p + 0,
-                         eng->cap_slots, eng->fx_slot.p);
-      DYN_CHECK_LAUNCH("k_eng_slot_forces");
+      EngineSet S = eng->self_set();
+      launch<k_eng_slot_forces>(S.R, "k_eng_slot_forces", 0, grid_for(eng->cap_slots, 256), 256, 0,
+                                [&](uint32_t) {
+        return args<k_eng_slot_forces>(eng->fitem.p, eng->chunk_ptr.p, eng->cap_chunks,
+                                       eng->counters.p + 0, eng->cap_slots, eng->fx_slot.p);
+      });
       for (int c = 0; c < 3; ++c)
         DYN_HIP(hipMemcpyAsync(fx_slot + c * ns, eng->fx_slot.p + (size_t)c * eng->cap_slots, 8 * ns,
                                hipMemcpyDeviceToHost, s));
@@ -2037,6 +2215,15 @@ extern "C" int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char
 namespace dyn {
 Profiler* engine_profiler(dyn_engine* eng) { return &eng->prof; }
 hipStream_t engine_stream(const dyn_engine* eng) { return eng->stream; }
+void engine_set_stream(dyn_engine* eng, hipStream_t s) {
+  eng->stream = s ? s : eng->own_stream;
+  eng->prof.stream = eng->stream;
+  eng->pme.stream = eng->stream;
+  eng->own_cache.invalidate();  // packs are uploaded on the stream in use
+  eng->pme.cache.invalidate();
+}
+const int64_t* engine_forces_device(const dyn_engine* eng) { return eng->fx_out.p; }
+EngineSet engine_self_set(dyn_engine* eng) { return eng->self_set(); }
 int engine_device(const dyn_engine* eng) { return eng->device; }
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water) {
   *n_dyn = eng->n_dyn;

@@ -190,7 +190,7 @@ struct IntArgs {
 // the current forces (M-site force projected onto O and H for water; atom
 // accelerations clamped to MAX_ACCEL); drift: x += v·h_drift for atoms,
 // the rigid drift for water.
-__global__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v) {
+__device__ __forceinline__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < A.n_dyn) {
     float4 vi = v[t];
@@ -236,7 +236,7 @@ __global__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __rest
 
 // Langevin velocity update (Ornstein–Uhlenbeck): v ← c·v + σ·ξ per
 // component, σ² = k_B T (1 − c²)/m; solute atoms, water O and H.
-__global__ void k_md_langevin(float4* __restrict__ v, uint32_t n_dyn, uint32_t n_water, float c,
+__device__ __forceinline__ void k_md_langevin(float4* __restrict__ v, uint32_t n_dyn, uint32_t n_water, float c,
                               const float* __restrict__ sigma_dyn, float sigma_o, float sigma_h,
                               uint64_t seed, uint64_t step) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,7 +278,7 @@ struct ConsArgs {
 
 // SHAKE over one cluster of coupled constraints (one thread), in the
 // constraints' input order, until the largest correction is below tol.
-__global__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
+__device__ __forceinline__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.n_clusters) return;
   const uint32_t k0 = C.cl_ptr[c], k1 = C.cl_ptr[c + 1];
@@ -302,7 +302,7 @@ __global__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
 
 // RATTLE: one pass over the cluster's constraints removing the relative
 // velocity along each bond.
-__global__ void k_md_rattle(ConsArgs C, const float4* __restrict__ x, float4* __restrict__ v) {
+__device__ __forceinline__ void k_md_rattle(ConsArgs C, const float4* __restrict__ x, float4* __restrict__ v) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= C.n_clusters) return;
   for (uint32_t k = C.cl_ptr[c]; k < C.cl_ptr[c + 1]; ++k) {
@@ -332,6 +332,7 @@ struct BondedArgs {
   const float4* dihe_p;   // k, periodicity, phase, unused
   int64_t* f;             // fixed point, component-major over solute atoms
   double* partials;       // 3 per block: bond, angle, dihedral energy
+  uint32_t nblk;          // workgroups of the system (one partial each)
 };
 
 __device__ __forceinline__ void add_f(int64_t* f, uint32_t nd, uint32_t i, V3 v) {
@@ -343,8 +344,9 @@ __device__ __forceinline__ void add_f(int64_t* f, uint32_t nd, uint32_t i, V3 v)
 // Harmonic bonds and angles (U = k(r − r0)², k(θ − θ0)²) and periodic
 // dihedrals (U = k(1 + cos(nφ − γ)), φ the IUPAC torsion angle); one
 // thread per term, fixed-point force sums.
-__global__ void __launch_bounds__(kBlock) k_md_bonded(BondedArgs A) {
+__device__ __forceinline__ void k_md_bonded(BondedArgs A) {
   __shared__ double red[3][kBlock];
+  if (blockIdx.x >= A.nblk) return;  // partials exist for the system's own blocks
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   double e[3] = {0.0, 0.0, 0.0};
   const uint32_t nd = A.n_dyn;
@@ -420,7 +422,7 @@ __global__ void __launch_bounds__(kBlock) k_md_bonded(BondedArgs A) {
     for (int c = 0; c < 3; ++c) A.partials[3 * blockIdx.x + c] = red[c][0];
 }
 
-__global__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
+__device__ __forceinline__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) f[i] = 0;
 }
@@ -431,7 +433,7 @@ __global__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
 // bond, angle, dihedral energy, [4] kinetic of the rigid-body motion (solute
 // atoms as they are; per water the centre-of-mass and rotational part,
 // ½ M v_com² + ½ ω·L with I ω = L about the centre of mass).
-__global__ void __launch_bounds__(1024)
+__device__ __forceinline__ void
 k_md_energies(const float4* __restrict__ x, const float4* __restrict__ v,
               const float* __restrict__ mass, uint32_t n_dyn, uint32_t n_water, Box B,
               const double* __restrict__ bpart, uint32_t n_bblk, double* __restrict__ out) {
@@ -524,7 +526,7 @@ struct dyn_md {
   std::string err;
   dyn_engine* eng = nullptr;
   int device = 0;
-  hipStream_t stream = nullptr;  // the engine's stream
+  hipStream_t stream = nullptr;  // the engine's stream (or the batch's)
   dyn_md_params prm{};
   uint32_t n_dyn = 0, n_water = 0, n_sites = 0;
   float box[3] = {0, 0, 0};
@@ -548,76 +550,126 @@ struct dyn_md {
   uint64_t step = 0;
   uint64_t n_eval = 0;  // force evaluations since set_state (SPME cadence)
   bool state_set = false, forces_valid = false;
-
-  void forces(bool energies);
-  void integrate(float hk, float hd, bool kick, bool drift);
-  void constrain_positions();
-  void constrain_velocities();
-  void one_step(bool energies);
-  void run(uint32_t n);
+  ArgCache own_cache;   // a pass over this MD alone
+  bool in_batch = false;
 };
 
-// The first evaluation after set_state is synchronous, so that a list that
-// outgrows its buffers is rebuilt at once; later ones are enqueued only (a
-// failure is reported at the next get_state / get_energies).  Nonbonded
-// energies are evaluated only when asked (the last step of a dyn_md_step
-// call), the forces alone otherwise.
-void dyn_md::forces(bool energies) {
-  uint32_t flags = DYN_COMPUTE_ALL | (forces_valid ? DYN_COMPUTE_ASYNC : 0u) |
-                   (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
-  // the first evaluation after set_state (or a replay) always runs SPME
-  if (forces_valid && prm.spme_ratio > 1 && n_eval % prm.spme_ratio != 0)
-    flags = (flags & ~DYN_COMPUTE_RECIP) | DYN_COMPUTE_RECIP_REUSE;
-  ++n_eval;
-  engine_call(dyn_engine_compute(eng, flags), eng);
-  Profiler::Scope ps(*engine_profiler(eng), "md_bonded");
-  hipLaunchKernelGGL(k_md_zero, dim3(grid_for(3 * (size_t)std::max(n_dyn, 1u), kBlock)), dim3(kBlock),
-                     0, stream, fb.p, 3 * (size_t)n_dyn);
-  DYN_CHECK_LAUNCH("k_md_zero");
-  if (n_bond + n_angle + n_dihe) {
-    BondedArgs A;
-    A.x = x.p;
-    for (int a = 0; a < 3; ++a) A.B.L[a] = box[a];
-    A.n_dyn = n_dyn;
-    A.n_bond = n_bond;
-    A.n_angle = n_angle;
-    A.n_dihe = n_dihe;
-    A.bond = bond.p;
-    A.bond_p = bond_p.p;
-    A.angle = angle.p;
-    A.angle_p = angle_p.p;
-    A.dihe = dihe.p;
-    A.dihe_p = dihe_p.p;
-    A.f = fb.p;
-    A.partials = bpart.p;
-    hipLaunchKernelGGL(k_md_bonded, dim3(bblocks), dim3(kBlock), 0, stream, A);
-    DYN_CHECK_LAUNCH("k_md_bonded");
+namespace dyn {
+
+// The MD handles of one pass (sharing a stream), with their engines.
+struct MdSet {
+  std::vector<dyn_md*> m;
+  EngineSet es;
+  BatchRun R;  // the MD kernels' launches (own argument packs)
+  Profiler* prof = nullptr;
+};
+
+}  // namespace dyn
+
+namespace {
+
+// One evaluation that waits: a list that outgrew its buffers is rebuilt once
+// with enlarged ones.
+void engine_set_compute_sync(EngineSet& S, uint32_t flags) {
+  engine_set_enqueue(S, flags);
+  try {
+    engine_set_check(S);
+  } catch (const Error& e) {
+    if (e.code != DYN_ERR_CAPACITY) throw;
+    engine_set_enqueue(S, flags | DYN_COMPUTE_REBUILD);
+    engine_set_check(S);
   }
-  forces_valid = true;
 }
 
-void dyn_md::integrate(float hk, float hd, bool kick, bool drift) {
-  Profiler::Scope ps(*engine_profiler(eng), "md_integrate");
-  IntArgs A;
-  A.F.nb = dyn_engine_forces_device(eng);
-  A.F.bd = fb.p;
-  A.F.n_sites = n_sites;
-  A.F.n_dyn = n_dyn;
-  A.acc_conv = acc_conv.p;
-  for (int a = 0; a < 3; ++a) A.B.L[a] = box[a];
-  A.n_dyn = n_dyn;
-  A.n_water = n_water;
-  A.h_kick = hk;
-  A.h_drift = hd;
-  A.kick = kick ? 1 : 0;
-  A.drift = drift ? 1 : 0;
-  hipLaunchKernelGGL(k_md_integrate, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
-                     stream, A, x.p, v.p);
-  DYN_CHECK_LAUNCH("k_md_integrate");
+// Nonbonded forces (all engines of the set) and bonded forces.  The first
+// evaluation after set_state is synchronous, so that a list that outgrows
+// its buffers is rebuilt at once; later ones are enqueued only (a failure
+// is found at the wait that ends the step call).  Nonbonded energies are
+// evaluated only when asked (the last step of a step call), the forces
+// alone otherwise.
+void md_forces(MdSet& S, bool energies) {
+  bool first = false, cadence_skip = true;
+  for (dyn_md* m : S.m) {
+    first = first || !m->forces_valid;
+    // the first evaluation after set_state (or a replay) always runs SPME
+    if (!(m->forces_valid && m->prm.spme_ratio > 1 && m->n_eval % m->prm.spme_ratio != 0))
+      cadence_skip = false;
+  }
+  uint32_t flags = DYN_COMPUTE_ALL | (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
+  if (cadence_skip) flags = (flags & ~DYN_COMPUTE_RECIP) | DYN_COMPUTE_RECIP_REUSE;
+  for (dyn_md* m : S.m) ++m->n_eval;
+  if (first)
+    engine_set_compute_sync(S.es, flags);
+  else
+    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC);
+  Profiler::Scope ps(*S.prof, "md_bonded");
+  const unsigned gz = [&] {
+    unsigned g = 1;
+    for (dyn_md* m : S.m) g = std::max(g, grid_for(3 * (size_t)std::max(m->n_dyn, 1u), kBlock));
+    return g;
+  }();
+  launch<k_md_zero>(S.R, "k_md_zero", 0, gz, kBlock, 0, [&](uint32_t i) {
+    dyn_md& m = *S.m[i];
+    return args<k_md_zero>(m.fb.p, 3 * (size_t)m.n_dyn);
+  });
+  bool any = false;
+  unsigned gb = 1;
+  for (dyn_md* m : S.m) {
+    any = any || (m->n_bond + m->n_angle + m->n_dihe) > 0;
+    gb = std::max(gb, m->bblocks);
+  }
+  if (any)
+    launch<k_md_bonded, kBlock>(S.R, "k_md_bonded", 0, gb, kBlock, 0, [&](uint32_t i) {
+      dyn_md& m = *S.m[i];
+      BondedArgs A;
+      std::memset(&A, 0, sizeof A);
+      A.x = m.x.p;
+      for (int a = 0; a < 3; ++a) A.B.L[a] = m.box[a];
+      A.n_dyn = m.n_dyn;
+      A.n_bond = m.n_bond;
+      A.n_angle = m.n_angle;
+      A.n_dihe = m.n_dihe;
+      A.bond = m.bond.p;
+      A.bond_p = m.bond_p.p;
+      A.angle = m.angle.p;
+      A.angle_p = m.angle_p.p;
+      A.dihe = m.dihe.p;
+      A.dihe_p = m.dihe_p.p;
+      A.f = m.fb.p;
+      A.partials = m.bpart.p;
+      A.nblk = m.bblocks;
+      return args<k_md_bonded>(A);
+    });
+  for (dyn_md* m : S.m) m->forces_valid = true;
 }
 
-static ConsArgs cons_args(const dyn_md& M) {
+void md_integrate(MdSet& S, int tag, float hk, float hd, bool kick, bool drift) {
+  Profiler::Scope ps(*S.prof, "md_integrate");
+  unsigned g = 1;
+  for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_dyn + m->n_water, kBlock));
+  launch<k_md_integrate>(S.R, "k_md_integrate", tag, g, kBlock, 0, [&](uint32_t i) {
+    dyn_md& m = *S.m[i];
+    IntArgs A;
+    std::memset(&A, 0, sizeof A);
+    A.F.nb = engine_forces_device(m.eng);
+    A.F.bd = m.fb.p;
+    A.F.n_sites = m.n_sites;
+    A.F.n_dyn = m.n_dyn;
+    A.acc_conv = m.acc_conv.p;
+    for (int a = 0; a < 3; ++a) A.B.L[a] = m.box[a];
+    A.n_dyn = m.n_dyn;
+    A.n_water = m.n_water;
+    A.h_kick = m.prm.dt * hk;
+    A.h_drift = m.prm.dt * hd;
+    A.kick = kick ? 1 : 0;
+    A.drift = drift ? 1 : 0;
+    return args<k_md_integrate>(A, m.x.p, m.v.p);
+  });
+}
+
+ConsArgs cons_args(const dyn_md& M) {
   ConsArgs C;
+  std::memset(&C, 0, sizeof C);
   C.cl_ptr = M.cl_ptr.p;
   C.ij = M.cons_ij.p;
   C.r0sq = M.cons_r0sq.p;
@@ -628,77 +680,117 @@ static ConsArgs cons_args(const dyn_md& M) {
   return C;
 }
 
-void dyn_md::constrain_positions() {
-  if (!n_clusters) return;
-  Profiler::Scope ps(*engine_profiler(eng), "md_constraints");
-  hipLaunchKernelGGL(k_md_shake, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
-                     cons_args(*this), x.p);
-  DYN_CHECK_LAUNCH("k_md_shake");
+unsigned cons_grid(const MdSet& S) {
+  unsigned g = 0;
+  for (dyn_md* m : S.m) g = std::max(g, m->n_clusters ? grid_for(m->n_clusters, 64) : 0u);
+  return g;
 }
 
-void dyn_md::constrain_velocities() {
-  if (!n_clusters) return;
-  Profiler::Scope ps(*engine_profiler(eng), "md_constraints");
-  hipLaunchKernelGGL(k_md_rattle, dim3(grid_for(n_clusters, 64)), dim3(64), 0, stream,
-                     cons_args(*this), (const float4*)x.p, v.p);
-  DYN_CHECK_LAUNCH("k_md_rattle");
+void md_constrain_positions(MdSet& S, int tag) {
+  const unsigned g = cons_grid(S);
+  if (!g) return;
+  Profiler::Scope ps(*S.prof, "md_constraints");
+  launch<k_md_shake, 64>(S.R, "k_md_shake", tag, g, 64, 0, [&](uint32_t i) {
+    return args<k_md_shake>(cons_args(*S.m[i]), S.m[i]->x.p);
+  });
 }
 
-// One step, as src/integrate.rs:100-190 (γ > 0, Langevin-middle) or
-// :191-260 (γ = 0, velocity Verlet without thermostat).
-void dyn_md::one_step(bool energies) {
-  const float dt = prm.dt, h = 0.5f * dt;
-  if (!forces_valid) forces(false);
-  if (prm.gamma > 0.f) {
-    integrate(h, h, true, true);
-    constrain_positions();
-    constrain_velocities();
+void md_constrain_velocities(MdSet& S, int tag) {
+  const unsigned g = cons_grid(S);
+  if (!g) return;
+  Profiler::Scope ps(*S.prof, "md_constraints");
+  launch<k_md_rattle, 64>(S.R, "k_md_rattle", tag, g, 64, 0, [&](uint32_t i) {
+    return args<k_md_rattle>(cons_args(*S.m[i]), (const float4*)S.m[i]->x.p, S.m[i]->v.p);
+  });
+}
+
+// One step of every MD of the set, as src/integrate.rs:100-190 (γ > 0,
+// Langevin-middle) or :191-260 (γ = 0, velocity Verlet without thermostat).
+// Step fractions are in units of each system's dt; the scheme (Langevin or
+// not) must be the same for the whole set.
+void md_one_step(MdSet& S, bool energies) {
+  bool any_invalid = false;
+  for (dyn_md* m : S.m) any_invalid = any_invalid || !m->forces_valid;
+  if (any_invalid) md_forces(S, false);
+  if (S.m[0]->prm.gamma > 0.f) {
+    md_integrate(S, 0, 0.5f, 0.5f, true, true);
+    md_constrain_positions(S, 0);
+    md_constrain_velocities(S, 0);
     {
-      Profiler::Scope ps(*engine_profiler(eng), "md_langevin");
-      hipLaunchKernelGGL(k_md_langevin, dim3(grid_for(n_dyn + n_water, kBlock)), dim3(kBlock), 0,
-                         stream, v.p, n_dyn, n_water, lang_c, lang_sigma.p, sigma_o, sigma_h,
-                         prm.seed, step);
-      DYN_CHECK_LAUNCH("k_md_langevin");
+      Profiler::Scope ps(*S.prof, "md_langevin");
+      unsigned g = 1;
+      for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_dyn + m->n_water, kBlock));
+      launch<k_md_langevin>(S.R, "k_md_langevin", 0, g, kBlock, 0, [&](uint32_t i) {
+        dyn_md& m = *S.m[i];
+        return args<k_md_langevin>(m.v.p, m.n_dyn, m.n_water, m.lang_c, m.lang_sigma.p, m.sigma_o,
+                                   m.sigma_h, m.prm.seed, m.step);
+      });
     }
-    constrain_velocities();
-    integrate(0.f, h, false, true);
-    constrain_positions();
+    md_constrain_velocities(S, 1);
+    md_integrate(S, 1, 0.f, 0.5f, false, true);
+    md_constrain_positions(S, 1);
   } else {
-    integrate(h, dt, true, true);
-    constrain_positions();
-    constrain_velocities();
+    md_integrate(S, 0, 0.5f, 1.0f, true, true);
+    md_constrain_positions(S, 0);
+    md_constrain_velocities(S, 0);
   }
-  forces(energies);
-  integrate(h, 0.f, true, false);
-  constrain_velocities();
-  ++step;
+  md_forces(S, energies);
+  md_integrate(S, 2, 0.5f, 0.f, true, false);
+  md_constrain_velocities(S, 2);
+  for (dyn_md* m : S.m) ++m->step;
 }
 
-// n steps, then a wait for them.  Later force evaluations are only enqueued,
-// so a cluster-pair list that outgrows its buffers is found at the wait; the
-// engine has enlarged them by then, and the steps run again from the state
-// they started from (the same random stream: the step index seeds it).
-void dyn_md::run(uint32_t n) {
+// n steps of every MD of the set, then a wait for them.  Later force
+// evaluations are only enqueued, so a cluster-pair list that outgrows its
+// buffers is found at the wait; the engine has enlarged them by then, and
+// the steps run again from the state they started from (the same random
+// stream: the step index seeds it).
+void md_run(MdSet& S, uint32_t n) {
   if (!n) return;
-  const size_t bytes = 16 * (size_t)n_sites;
-  DYN_HIP(hipMemcpyAsync(x_ck.p, x.p, bytes, hipMemcpyDeviceToDevice, stream));
-  DYN_HIP(hipMemcpyAsync(v_ck.p, v.p, bytes, hipMemcpyDeviceToDevice, stream));
-  const uint64_t step0 = step, eval0 = n_eval;
+  struct Ck {
+    uint64_t step, eval;
+  };
+  std::vector<Ck> ck(S.m.size());
+  for (size_t i = 0; i < S.m.size(); ++i) {
+    dyn_md& m = *S.m[i];
+    const size_t bytes = 16 * (size_t)m.n_sites;
+    DYN_HIP(hipMemcpyAsync(m.x_ck.p, m.x.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+    DYN_HIP(hipMemcpyAsync(m.v_ck.p, m.v.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+    ck[i] = {m.step, m.n_eval};
+  }
   for (int attempt = 0;; ++attempt) {
-    for (uint32_t k = 0; k < n; ++k) one_step(k + 1 == n);
+    for (uint32_t k = 0; k < n; ++k) md_one_step(S, k + 1 == n);
     try {
-      engine_call(dyn_engine_sync(eng), eng);
+      engine_set_check(S.es);
       return;
     } catch (const Error& e) {
       if (e.code != DYN_ERR_CAPACITY || attempt >= 2) throw;
     }
-    DYN_HIP(hipMemcpyAsync(x.p, x_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
-    DYN_HIP(hipMemcpyAsync(v.p, v_ck.p, bytes, hipMemcpyDeviceToDevice, stream));
-    step = step0;
-    n_eval = eval0;
-    forces_valid = false;
+    for (size_t i = 0; i < S.m.size(); ++i) {
+      dyn_md& m = *S.m[i];
+      const size_t bytes = 16 * (size_t)m.n_sites;
+      DYN_HIP(hipMemcpyAsync(m.x.p, m.x_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+      DYN_HIP(hipMemcpyAsync(m.v.p, m.v_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+      m.step = ck[i].step;
+      m.n_eval = ck[i].eval;
+      m.forces_valid = false;
+    }
   }
 }
+
+// A set of one: this MD and its engine.
+MdSet self_set(dyn_md* m) {
+  MdSet S;
+  S.m = {m};
+  S.es = engine_self_set(m->eng);
+  S.prof = engine_profiler(m->eng);
+  S.R.stream = m->stream;
+  S.R.cache = &m->own_cache;
+  S.R.n = 1;
+  return S;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C entry points
@@ -905,7 +997,9 @@ extern "C" int dyn_md_step(dyn_md* md, uint32_t n_steps) {
   DYN_API_BODY(md, {
     DeviceGuard g(md->device);
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
-    md->run(n_steps);
+    if (md->in_batch) throw Error(DYN_ERR_STATE, "this MD is stepped by a batch (dyn_batch_step)");
+    MdSet S = self_set(md);
+    md_run(S, n_steps);
   });
 }
 
@@ -943,13 +1037,16 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     DeviceGuard g(md->device);
     DYN_REQUIRE(out, "out is null");
     if (!md->state_set) throw Error(DYN_ERR_STATE, "state not set");
-    if (!md->forces_valid) md->forces(true);
+    MdSet S = self_set(md);
+    if (!md->forces_valid) md_forces(S, true);
     Box B;
     for (int a = 0; a < 3; ++a) B.L[a] = md->box[a];
-    hipLaunchKernelGGL(k_md_energies, dim3(1), dim3(1024), 0, md->stream, md->x.p, md->v.p,
-                       md->mass.p, md->n_dyn, md->n_water, B, md->bpart.p,
-                       (md->n_bond + md->n_angle + md->n_dihe) ? md->bblocks : 0u, md->eout.p);
-    DYN_CHECK_LAUNCH("k_md_energies");
+    launch<k_md_energies, 1024>(S.R, "k_md_energies", 0, 1, 1024, 0, [&](uint32_t) {
+      return args<k_md_energies>(md->x.p, md->v.p, md->mass.p, md->n_dyn, md->n_water, B,
+                                 md->bpart.p,
+                                 (md->n_bond + md->n_angle + md->n_dihe) ? md->bblocks : 0u,
+                                 md->eout.p);
+    });
     DYN_HIP(hipMemcpyAsync(md->h_out.p, md->eout.p, 5 * sizeof(double), hipMemcpyDeviceToHost,
                            md->stream));
     engine_call(dyn_engine_sync(md->eng), md->eng);
@@ -967,5 +1064,143 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     E.temperature = dof > 0 ? 2.0 * md->h_out.p[4] / (dof * (double)kKb) : 0.0;
     E.steps = md->step;
     *out = E;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Batch of independent systems: one launch per kernel for all of them
+// ---------------------------------------------------------------------------
+struct dyn_batch {
+  std::string err;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<dyn_md*> m;
+  ArgCache eng_cache, md_cache;
+  DispRing disp;
+  Profiler prof;
+
+  MdSet set() {
+    MdSet S;
+    S.m = m;
+    for (dyn_md* x : m) S.es.e.push_back(x->eng);
+    S.es.R.stream = stream;
+    S.es.R.cache = &eng_cache;
+    S.es.R.n = (uint32_t)m.size();
+    S.es.prof = &prof;
+    S.es.disp = &disp;
+    S.R.stream = stream;
+    S.R.cache = &md_cache;
+    S.R.n = (uint32_t)m.size();
+    S.prof = &prof;
+    return S;
+  }
+  // hand the members back to their own streams
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (dyn_md* x : m) {
+      engine_set_stream(x->eng, nullptr);
+      x->stream = engine_stream(x->eng);
+      x->in_batch = false;
+    }
+    m.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+  }
+};
+
+extern "C" int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds) {
+  try {
+    DYN_REQUIRE(out && mds && n > 0, "null argument or empty batch");
+    *out = nullptr;
+    std::unique_ptr<dyn_batch> B(new dyn_batch());
+    B->device = mds[0] ? mds[0]->device : 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      dyn_md* x = mds[i];
+      DYN_REQUIRE(x, "MD handle %u is null", i);
+      DYN_REQUIRE(!x->in_batch, "MD handle %u is in another batch", i);
+      DYN_REQUIRE(x->device == B->device, "MD handle %u is on device %d, not %d", i, x->device,
+                  B->device);
+      DYN_REQUIRE((x->prm.gamma > 0.f) == (mds[0]->prm.gamma > 0.f),
+                  "MD handle %u: Langevin and plain velocity Verlet cannot share a batch", i);
+      for (uint32_t j = 0; j < i; ++j)
+        DYN_REQUIRE(mds[j] != x && mds[j]->eng != x->eng, "MD handles %u and %u share an engine",
+                    j, i);
+    }
+    DeviceGuard g(B->device);
+    for (uint32_t i = 0; i < n; ++i) DYN_HIP(hipStreamSynchronize(mds[i]->stream));
+    DYN_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    B->prof.stream = B->stream;
+    B->disp.init(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      dyn_md* x = mds[i];
+      engine_set_stream(x->eng, B->stream);
+      x->stream = B->stream;
+      x->own_cache.invalidate();
+      x->in_batch = true;
+      B->m.push_back(x);
+    }
+    *out = B.release();
+    return DYN_OK;
+  } catch (const Error& e) {
+    set_global_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_global_error(e.what());
+    return DYN_ERR_HIP;
+  }
+}
+
+extern "C" void dyn_batch_destroy(dyn_batch* b) {
+  if (!b) return;
+  try {
+    DeviceGuard g(b->device);
+    b->release();
+    delete b;
+  } catch (...) {
+  }
+}
+
+extern "C" const char* dyn_batch_last_error(const dyn_batch* b) {
+  return b ? b->err.c_str() : "null batch handle";
+}
+
+extern "C" int dyn_batch_step(dyn_batch* b, uint32_t n_steps) {
+  if (!b) {
+    set_global_error("null batch handle");
+    return DYN_ERR_ARG;
+  }
+  DYN_API_BODY(b, {
+    DeviceGuard g(b->device);
+    for (size_t i = 0; i < b->m.size(); ++i)
+      if (!b->m[i]->state_set) throw Error(DYN_ERR_STATE, fmt("MD %zu: state not set", i));
+    MdSet S = b->set();
+    md_run(S, n_steps);
+  });
+}
+
+extern "C" int dyn_batch_profile(dyn_batch* b, int enable) {
+  if (!b) return DYN_ERR_ARG;
+  DYN_API_BODY(b, {
+    DeviceGuard g(b->device);
+    b->prof.reset();
+    b->prof.enabled = enable != 0;
+  });
+}
+
+extern "C" int dyn_batch_profile_read(dyn_batch* b, uint32_t max, const char** names, double* ms,
+                                      uint64_t* counts, uint32_t* n) {
+  if (!b) return DYN_ERR_ARG;
+  DYN_API_BODY(b, {
+    DeviceGuard g(b->device);
+    b->prof.drain();
+    const uint32_t k = (uint32_t)b->prof.order.size();
+    if (n) *n = k;
+    for (uint32_t i = 0; i < k && i < max; ++i) {
+      const auto& nm = b->prof.order[i];
+      auto it = b->prof.totals.find(nm);
+      if (names) names[i] = it->first.c_str();
+      if (ms) ms[i] = it->second.first;
+      if (counts) counts[i] = it->second.second;
+    }
   });
 }

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 
 namespace dyn {
@@ -72,6 +73,26 @@ struct PmeCore {
   void run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
            int fx_component_major, uint32_t fx_nsites, float* f_out, float fscale = 1.0f,
            double* res = nullptr);
+  ArgCache cache;  // device argument packs of run()
 };
+
+// One system's part of a batched SPME pass (the arguments of PmeCore::run;
+// core == nullptr: the system takes no part).
+struct PmeRun {
+  PmeCore* core = nullptr;
+  uint32_t n = 0;
+  const float4* xyzq = nullptr;
+  const uint32_t* site = nullptr;
+  int64_t* fx = nullptr;
+  int comp_major = 0;
+  uint32_t nsites = 0;
+  float* f_out = nullptr;
+  float fscale = 1.0f;
+  double* res = nullptr;  // [energy, virial]
+};
+
+// runs[0 .. R.n): one per system of the pass; tag separates passes that run
+// more than once per evaluation (their launch sites keep separate packs).
+void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag);
 
 }  // namespace dyn

@@ -193,12 +193,13 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 // One FFT pass along AXIS over complex data in place.  FROM_FX: the input is
 // the int64 fixed-point charge grid (z pass only).
 template <int AXIS, int SIGN, bool FROM_FX>
-__global__ void __launch_bounds__(kFftBlock)
+__device__ __forceinline__ void
 k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
            FftAxis ax, const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
+  if (line0 >= g.lines) return;  // beyond this system's lines
   const int nl = min(g.L, g.lines - line0);
   float2* A = smem;
   float2* B = smem + g.L * N;
@@ -228,13 +229,14 @@ struct ConvArgs {
 
 // x pass fused with the convolution: forward FFT along x, multiply by G(m)
 // (accumulating ½G|F[Q]|² and its virial weight in f64), inverse FFT along x.
-__global__ void __launch_bounds__(kFftBlock)
+__device__ __forceinline__ void
 k_fft_conv_x(float2* __restrict__ data, PassGeom g, FftAxis ax, const float2* __restrict__ tw,
              const float* __restrict__ G, ConvArgs C, double* __restrict__ partials) {
   extern __shared__ float2 smem[];
   __shared__ double red[2][kFftBlock];
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
+  if (line0 >= g.lines) return;  // beyond this system's lines (no partial either)
   const int nl = min(g.L, g.lines - line0);
   float2* A = smem;
   float2* B = smem + g.L * N;
@@ -291,9 +293,10 @@ k_fft_conv_x(float2* __restrict__ data, PassGeom g, FftAxis ax, const float2* __
   }
 }
 
-__global__ void k_pme_finalize(const double* __restrict__ partials, int nb,
+__device__ __forceinline__ void k_pme_finalize(const double* __restrict__ partials, int nb,
                                double* __restrict__ res) {
   __shared__ double red[2][256];
+  if (!res) return;  // a system that skips this pass
   double e = 0, w = 0;
   for (int b = threadIdx.x; b < nb; b += 256) {
     e += partials[2 * b];
@@ -341,7 +344,7 @@ __device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk
 // Per charge: its home brick (the brick of its base grid point) and a rank
 // in it.  The rank order depends on atomic timing; the brick sums are
 // integers, so the spread grid does not.
-__global__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
+__device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
                           uint32_t nbx, uint32_t nby, uint32_t nbz,
                           uint32_t* __restrict__ count, uint2* __restrict__ cb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -361,9 +364,10 @@ __global__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArg
 // Single-workgroup exclusive scan of the brick counts into start[0..n]; the
 // counts are cleared for the next binning pass.
 constexpr int kBrickScan = 1024;
-__global__ void __launch_bounds__(kBrickScan)
+__device__ __forceinline__ void
 k_pme_brick_scan(uint32_t* __restrict__ count, uint32_t n, uint32_t* __restrict__ start) {
   __shared__ uint32_t wsum[kBrickScan / 64];
+  if (!start) return;  // a system that skips this pass
   const int t = threadIdx.x, lane = t & 63, w = t / 64;
   const uint32_t per = (n + kBrickScan - 1) / kBrickScan;
   const uint32_t b0 = t * per, e0 = min(n, b0 + per);
@@ -391,7 +395,7 @@ k_pme_brick_scan(uint32_t* __restrict__ count, uint32_t n, uint32_t* __restrict_
   if (t == 0) start[n] = tot;
 }
 
-__global__ void k_pme_brick_scatter(const uint2* __restrict__ cb, uint32_t n,
+__device__ __forceinline__ void k_pme_brick_scatter(const uint2* __restrict__ cb, uint32_t n,
                                     const uint32_t* __restrict__ start,
                                     uint32_t* __restrict__ items) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,20 +405,25 @@ __global__ void k_pme_brick_scatter(const uint2* __restrict__ cb, uint32_t n,
   if (k < n) items[k] = i;
 }
 
-// Home bricks along one axis whose charges can reach the points [lo, hi]:
-// a charge with base point k0 covers k0 − ORDER + 1 … k0 (mod K), so k0 runs
-// over lo … hi + ORDER − 1 (mod K).  At most 3 distinct bricks.
+// Home bricks along one axis whose charges can reach the points [lo, hi]
+// (lo a brick start): a charge with base point k0 covers k0 − ORDER + 1 … k0
+// (mod K), so k0 runs over lo … hi + ORDER − 1 (mod K): the bricks of
+// lo … e1 (e1 < K), then, past the wrap, those of 0 … e2 that come before
+// lo's brick.  At most 3 bricks (ORDER ≤ 5, 8-point bricks).
+struct HomeRange {
+  int b0, n1, n;  // first brick, bricks before the wrap, bricks in all
+  __device__ int brick(int j) const { return j < n1 ? b0 + j : j - n1; }
+};
 template <int ORDER>
-__device__ __forceinline__ int home_bricks(int lo, int hi, int K, int out[3]) {
-  int n = 0;
+__device__ __forceinline__ HomeRange home_bricks(int lo, int hi, int K) {
   const int len = min(hi - lo + ORDER, K);
-  for (int d = 0; d < len; ++d) {
-    const int b = ((lo + d) % K) / kBrick;
-    bool seen = false;
-    for (int q = 0; q < n; ++q) seen = seen || out[q] == b;
-    if (!seen && n < 3) out[n++] = b;
-  }
-  return n;
+  const int last = lo + len - 1;
+  HomeRange r;
+  r.b0 = lo / kBrick;
+  r.n1 = min(last, K - 1) / kBrick - r.b0 + 1;
+  const int e2 = last - K;  // last point past the wrap (< 0: none)
+  r.n = r.n1 + (e2 >= 0 && r.b0 > 0 ? min(e2 / kBrick, r.b0 - 1) + 1 : 0);
+  return r;
 }
 
 // One workgroup per brick: every charge of the reaching home bricks adds its
@@ -422,27 +431,30 @@ __device__ __forceinline__ int home_bricks(int lo, int hi, int K, int out[3]) {
 // same fixed-point values the replay sums), which is then stored once.  The
 // home bricks' charge lists are walked as one flattened index space.
 template <int ORDER>
-__global__ void __launch_bounds__(kBrickBlock)
+__device__ __forceinline__ void
 k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t nbx,
              uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ start,
              const uint32_t* __restrict__ items, int64_t* __restrict__ grid) {
   __shared__ unsigned long long acc[kBrick * kBrick * kBrick];
   __shared__ uint32_t s_c0[27], s_beg[28];
   const uint32_t b = blockIdx.x;
+  if (b >= nbx * nby * nbz) return;  // beyond this system's bricks
   const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), bx = (int)(b / (nbz * nby));
   const int lo[3] = {bx * kBrick, by * kBrick, bz * kBrick};
   const int hi[3] = {min(lo[0] + kBrick, S.K[0]) - 1, min(lo[1] + kBrick, S.K[1]) - 1,
                      min(lo[2] + kBrick, S.K[2]) - 1};
   for (int k = threadIdx.x; k < kBrick * kBrick * kBrick; k += kBrickBlock) acc[k] = 0ull;
   if (threadIdx.x == 0) {
-    int hb[3][3], nh[3];
-    for (int a = 0; a < 3; ++a) nh[a] = home_bricks<ORDER>(lo[a], hi[a], S.K[a], hb[a]);
+    const HomeRange hx = home_bricks<ORDER>(lo[0], hi[0], S.K[0]);
+    const HomeRange hy = home_bricks<ORDER>(lo[1], hi[1], S.K[1]);
+    const HomeRange hz = home_bricks<ORDER>(lo[2], hi[2], S.K[2]);
     uint32_t tot = 0;
     int q = 0;
-    for (int ix = 0; ix < nh[0]; ++ix)
-      for (int iy = 0; iy < nh[1]; ++iy)
-        for (int iz = 0; iz < nh[2]; ++iz) {
-          const uint32_t h = ((uint32_t)hb[0][ix] * nby + (uint32_t)hb[1][iy]) * nbz + (uint32_t)hb[2][iz];
+    for (int ix = 0; ix < hx.n; ++ix)
+      for (int iy = 0; iy < hy.n; ++iy)
+        for (int iz = 0; iz < hz.n; ++iz) {
+          const uint32_t h = ((uint32_t)hx.brick(ix) * nby + (uint32_t)hy.brick(iy)) * nbz +
+                             (uint32_t)hz.brick(iz);
           const uint32_t c0 = min(start[h], n), c1 = min(start[h + 1], n);
           s_c0[q] = c0;
           s_beg[q] = tot;
@@ -496,7 +508,7 @@ k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t
 }
 
 template <int ORDER>
-__global__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
+__device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
                              const float2* __restrict__ phi, const uint32_t* __restrict__ site,
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
                              float* __restrict__ fout, float fscale) {
@@ -683,91 +695,178 @@ void PmeCore::set_box(const float l[3]) {
 
 void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
                   int comp_major, uint32_t nsites, float* f_out, float fscale, double* res) {
-  SplineArgs S;
-  for (int a = 0; a < 3; ++a) {
-    S.K[a] = (int)K[a];
-    S.sc[a] = (float)((double)K[a] / (double)L[a]);
+  PmeRun r;
+  r.core = this;
+  r.n = n;
+  r.xyzq = xyzq;
+  r.site = site;
+  r.fx = fx_sites;
+  r.comp_major = comp_major;
+  r.nsites = nsites;
+  r.f_out = f_out;
+  r.fscale = fscale;
+  r.res = res ? res : res_dev.p;
+  BatchRun R;
+  R.stream = stream;
+  R.cache = &cache;
+  R.n = 1;
+  pme_run(R, *prof, &r, 0);
+}
+
+// One SPME evaluation per system of the pass, each stage one launch for all
+// of them.  A system with a null core takes part in no stage (its packs
+// describe empty work: no charges, no bricks, no lines, no result).
+void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
+  const uint32_t ns = R.n;
+  std::vector<SplineArgs> S(ns);
+  std::vector<PassGeom> gz(ns), gy(ns), gx(ns);
+  unsigned g_n = 1, g_b = 1, bz = 1, by = 1, bx = 1;
+  size_t shz = 0, shy = 0, shx = 0;
+  bool any = false, ord[2] = {false, false};
+  auto shm = [](const PassGeom& g, int N) { return (size_t)2 * g.L * N * sizeof(float2); };
+  for (uint32_t i = 0; i < ns; ++i) {
+    const PmeRun& r = runs[i];
+    if (!r.core) {
+      std::memset(&S[i], 0, sizeof S[i]);
+      std::memset(&gz[i], 0, sizeof gz[i]);
+      gy[i] = gx[i] = gz[i];
+      continue;
+    }
+    const PmeCore& c = *r.core;
+    any = true;
+    ord[c.order == 5] = true;
+    for (int a = 0; a < 3; ++a) {
+      S[i].K[a] = (int)c.K[a];
+      S[i].sc[a] = (float)((double)c.K[a] / (double)c.L[a]);
+    }
+    r.core->charge_brick.reserve(std::max<uint32_t>(r.n, 1));
+    r.core->brick_items.reserve(std::max<uint32_t>(r.n, 1));
+    gz[i] = pass_geom(c.K, 2);
+    gy[i] = pass_geom(c.K, 1);
+    gx[i] = pass_geom(c.K, 0);
+    g_n = std::max(g_n, grid_for(r.n, 256));
+    g_b = std::max(g_b, c.nbk[0] * c.nbk[1] * c.nbk[2]);
+    bz = std::max(bz, (unsigned)((gz[i].lines + gz[i].L - 1) / gz[i].L));
+    by = std::max(by, (unsigned)((gy[i].lines + gy[i].L - 1) / gy[i].L));
+    bx = std::max(bx, (unsigned)((gx[i].lines + gx[i].L - 1) / gx[i].L));
+    shz = std::max(shz, shm(gz[i], (int)c.K[2]));
+    shy = std::max(shy, shm(gy[i], (int)c.K[1]));
+    shx = std::max(shx, shm(gx[i], (int)c.K[0]));
   }
+  if (!any) return;
+  auto on = [&](uint32_t i) { return runs[i].core != nullptr; };
+  auto nq = [&](uint32_t i) { return on(i) ? runs[i].n : 0u; };
+  auto nb = [&](uint32_t i, int a) { return on(i) ? runs[i].core->nbk[a] : 0u; };
+  auto cp = [&](uint32_t i) { return runs[i].core; };
   // every grid point belongs to exactly one brick and is stored by it, so
   // the grid needs no clearing
   {
-    Profiler::Scope ps(*prof, "pme_spread");
-    const uint32_t nb = nbk[0] * nbk[1] * nbk[2];
-    if (n) {
-      charge_brick.reserve(n);
-      brick_items.reserve(n);
-      hipLaunchKernelGGL(k_pme_bin, dim3(grid_for(n, 256)), dim3(256), 0, stream, xyzq, n, S,
-                         nbk[0], nbk[1], nbk[2], brick_count.p, charge_brick.p);
-      DYN_CHECK_LAUNCH("k_pme_bin");
-    }
-    hipLaunchKernelGGL(k_pme_brick_scan, dim3(1), dim3(kBrickScan), 0, stream, brick_count.p, nb,
-                       brick_start.p);
-    DYN_CHECK_LAUNCH("k_pme_brick_scan");
-    if (n) {
-      hipLaunchKernelGGL(k_pme_brick_scatter, dim3(grid_for(n, 256)), dim3(256), 0, stream,
-                         charge_brick.p, n, brick_start.p, brick_items.p);
-      DYN_CHECK_LAUNCH("k_pme_brick_scatter");
-    }
-    if (order == 4)
-      hipLaunchKernelGGL(k_pme_spread<4>, dim3(nb), dim3(kBrickBlock), 0, stream, xyzq, n, S,
-                         nbk[0], nbk[1], nbk[2], brick_start.p, brick_items.p, grid_fx.p);
-    else
-      hipLaunchKernelGGL(k_pme_spread<5>, dim3(nb), dim3(kBrickBlock), 0, stream, xyzq, n, S,
-                         nbk[0], nbk[1], nbk[2], brick_start.p, brick_items.p, grid_fx.p);
-    DYN_CHECK_LAUNCH("k_pme_spread");
+    Profiler::Scope ps(prof, "pme_spread");
+    launch<k_pme_bin>(R, "k_pme_bin", tag, g_n, 256, 0, [&](uint32_t i) {
+      return args<k_pme_bin>(runs[i].xyzq, nq(i), S[i], nb(i, 0), nb(i, 1), nb(i, 2),
+                             on(i) ? cp(i)->brick_count.p : nullptr,
+                             on(i) ? cp(i)->charge_brick.p : nullptr);
+    });
+    launch<k_pme_brick_scan, kBrickScan>(R, "k_pme_brick_scan", tag, 1, kBrickScan, 0,
+                                         [&](uint32_t i) {
+      return args<k_pme_brick_scan>(on(i) ? cp(i)->brick_count.p : nullptr,
+                                    nb(i, 0) * nb(i, 1) * nb(i, 2),
+                                    on(i) ? cp(i)->brick_start.p : nullptr);
+    });
+    launch<k_pme_brick_scatter>(R, "k_pme_brick_scatter", tag, g_n, 256, 0, [&](uint32_t i) {
+      return args<k_pme_brick_scatter>(on(i) ? cp(i)->charge_brick.p : nullptr, nq(i),
+                                       on(i) ? cp(i)->brick_start.p : nullptr,
+                                       on(i) ? cp(i)->brick_items.p : nullptr);
+    });
+    auto spread_args = [&](uint32_t i, int order, auto F) {
+      const bool mine = on(i) && cp(i)->order == order;
+      return args<F.value>(runs[i].xyzq, mine ? nq(i) : 0u, S[i], mine ? nb(i, 0) : 0u,
+                           mine ? nb(i, 1) : 0u, mine ? nb(i, 2) : 0u,
+                           mine ? cp(i)->brick_start.p : nullptr,
+                           mine ? cp(i)->brick_items.p : nullptr,
+                           mine ? cp(i)->grid_fx.p : nullptr);
+    };
+    if (ord[0])
+      launch<k_pme_spread<4>, kBrickBlock>(R, "k_pme_spread<4>", tag, g_b, kBrickBlock, 0,
+                                           [&](uint32_t i) {
+        return spread_args(i, 4, std::integral_constant<decltype(&k_pme_spread<4>), &k_pme_spread<4>>{});
+      });
+    if (ord[1])
+      launch<k_pme_spread<5>, kBrickBlock>(R, "k_pme_spread<5>", tag, g_b, kBrickBlock, 0,
+                                           [&](uint32_t i) {
+        return spread_args(i, 5, std::integral_constant<decltype(&k_pme_spread<5>), &k_pme_spread<5>>{});
+      });
   }
-  const PassGeom gz = pass_geom(K, 2), gy = pass_geom(K, 1), gx = pass_geom(K, 0);
-  auto shm = [&](const PassGeom& g, int N) { return (size_t)2 * g.L * N * sizeof(float2); };
-  const unsigned bz = (gz.lines + gz.L - 1) / gz.L, by = (gy.lines + gy.L - 1) / gy.L,
-                 bx = (gx.lines + gx.L - 1) / gx.L;
+  auto fft = [&](auto F, const char* name, const std::vector<PassGeom>& g, int axis, unsigned gx_,
+                 size_t shm_, bool from_fx) {
+    launch<decltype(F)::value, kFftBlock>(R, name, tag, gx_, kFftBlock, shm_, [&](uint32_t i) {
+      return args<decltype(F)::value>(on(i) ? cp(i)->cgrid.p : nullptr,
+                                      from_fx && on(i) ? cp(i)->grid_fx.p : nullptr, g[i],
+                                      on(i) ? cp(i)->ax[axis] : FftAxis{},
+                                      on(i) ? cp(i)->tw[axis].p : nullptr);
+    });
+  };
   {
-    Profiler::Scope ps(*prof, "pme_fft_z_fwd");
-    hipLaunchKernelGGL((k_fft_pass<2, -1, true>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, grid_fx.p, gz, ax[2], tw[2].p);
-    DYN_CHECK_LAUNCH("k_fft_pass z fwd");
-  }
-  {
-    Profiler::Scope ps(*prof, "pme_fft_y_fwd");
-    hipLaunchKernelGGL((k_fft_pass<1, -1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
-    DYN_CHECK_LAUNCH("k_fft_pass y fwd");
-  }
-  {
-    Profiler::Scope ps(*prof, "pme_fft_x_conv");
-    ConvArgs C;
-    for (int a = 0; a < 3; ++a) C.invL[a] = 1.0f / L[a];
-    C.pi2_over_b2 = M_PI * M_PI / ((double)beta * beta);
-    hipLaunchKernelGGL(k_fft_conv_x, dim3(bx), dim3(kFftBlock), shm(gx, K[0]), stream, cgrid.p,
-                       gx, ax[0], tw[0].p, G.p, C, partials.p);
-    DYN_CHECK_LAUNCH("k_fft_conv_x");
-  }
-  {
-    Profiler::Scope ps(*prof, "pme_fft_y_inv");
-    hipLaunchKernelGGL((k_fft_pass<1, 1, false>), dim3(by), dim3(kFftBlock), shm(gy, K[1]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gy, ax[1], tw[1].p);
-    DYN_CHECK_LAUNCH("k_fft_pass y inv");
+    Profiler::Scope ps(prof, "pme_fft_z_fwd");
+    fft(std::integral_constant<decltype(&k_fft_pass<2, -1, true>), &k_fft_pass<2, -1, true>>{},
+        "k_fft_pass z fwd", gz, 2, bz, shz, true);
   }
   {
-    Profiler::Scope ps(*prof, "pme_fft_z_inv");
-    hipLaunchKernelGGL((k_fft_pass<2, 1, false>), dim3(bz), dim3(kFftBlock), shm(gz, K[2]),
-                       stream, cgrid.p, (const int64_t*)nullptr, gz, ax[2], tw[2].p);
-    DYN_CHECK_LAUNCH("k_fft_pass z inv");
+    Profiler::Scope ps(prof, "pme_fft_y_fwd");
+    fft(std::integral_constant<decltype(&k_fft_pass<1, -1, false>), &k_fft_pass<1, -1, false>>{},
+        "k_fft_pass y fwd", gy, 1, by, shy, false);
   }
   {
-    Profiler::Scope ps(*prof, "pme_finalize");
-    hipLaunchKernelGGL(k_pme_finalize, dim3(1), dim3(256), 0, stream, partials.p,
-                       (int)conv_blocks, res ? res : res_dev.p);
-    DYN_CHECK_LAUNCH("k_pme_finalize");
+    Profiler::Scope ps(prof, "pme_fft_x_conv");
+    launch<k_fft_conv_x, kFftBlock>(R, "k_fft_conv_x", tag, bx, kFftBlock, shx, [&](uint32_t i) {
+      ConvArgs C;
+      std::memset(&C, 0, sizeof C);
+      if (on(i)) {
+        for (int a = 0; a < 3; ++a) C.invL[a] = 1.0f / cp(i)->L[a];
+        C.pi2_over_b2 = M_PI * M_PI / ((double)cp(i)->beta * cp(i)->beta);
+      }
+      return args<k_fft_conv_x>(on(i) ? cp(i)->cgrid.p : nullptr, gx[i],
+                                on(i) ? cp(i)->ax[0] : FftAxis{}, on(i) ? cp(i)->tw[0].p : nullptr,
+                                on(i) ? cp(i)->G.p : nullptr, C,
+                                on(i) ? cp(i)->partials.p : nullptr);
+    });
   }
-  if (n) {
-    Profiler::Scope ps(*prof, "pme_gather");
-    if (order == 4)
-      hipLaunchKernelGGL(k_pme_gather<4>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out, fscale);
-    else
-      hipLaunchKernelGGL(k_pme_gather<5>, dim3(grid_for(n, 128)), dim3(128), 0, stream, xyzq, n,
-                         S, cgrid.p, site, fx_sites, comp_major, nsites, f_out, fscale);
-    DYN_CHECK_LAUNCH("k_pme_gather");
+  {
+    Profiler::Scope ps(prof, "pme_fft_y_inv");
+    fft(std::integral_constant<decltype(&k_fft_pass<1, 1, false>), &k_fft_pass<1, 1, false>>{},
+        "k_fft_pass y inv", gy, 1, by, shy, false);
+  }
+  {
+    Profiler::Scope ps(prof, "pme_fft_z_inv");
+    fft(std::integral_constant<decltype(&k_fft_pass<2, 1, false>), &k_fft_pass<2, 1, false>>{},
+        "k_fft_pass z inv", gz, 2, bz, shz, false);
+  }
+  {
+    Profiler::Scope ps(prof, "pme_finalize");
+    launch<k_pme_finalize>(R, "k_pme_finalize", tag, 1, 256, 0, [&](uint32_t i) {
+      return args<k_pme_finalize>(on(i) ? cp(i)->partials.p : nullptr,
+                                  on(i) ? (int)cp(i)->conv_blocks : 0,
+                                  on(i) ? runs[i].res : nullptr);
+    });
+  }
+  {
+    Profiler::Scope ps(prof, "pme_gather");
+    auto gather_args = [&](uint32_t i, int order, auto F) {
+      const bool mine = on(i) && cp(i)->order == order;
+      return args<F.value>(runs[i].xyzq, mine ? nq(i) : 0u, S[i],
+                           mine ? (const float2*)cp(i)->cgrid.p : nullptr, runs[i].site,
+                           runs[i].fx, runs[i].comp_major, runs[i].nsites, runs[i].f_out,
+                           runs[i].fscale);
+    };
+    const unsigned gg = std::max(1u, (g_n * 256 + 127) / 128);
+    if (ord[0])
+      launch<k_pme_gather<4>, 128>(R, "k_pme_gather<4>", tag, gg, 128, 0, [&](uint32_t i) {
+        return gather_args(i, 4, std::integral_constant<decltype(&k_pme_gather<4>), &k_pme_gather<4>>{});
+      });
+    if (ord[1])
+      launch<k_pme_gather<5>, 128>(R, "k_pme_gather<5>", tag, gg, 128, 0, [&](uint32_t i) {
+        return gather_args(i, 5, std::integral_constant<decltype(&k_pme_gather<5>), &k_pme_gather<5>>{});
+      });
   }
 }
 

@@ -375,6 +375,29 @@ typedef struct dyn_md_energies {
 
 int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);
 
+/* ---- Batches of independent systems --------------------------------------
+ * Many MD handles (each with its own engine: independent systems of uneven
+ * sizes, or λ-windows of one system via dyn_engine_set_alchemical) advance
+ * together: every kernel of a step is one launch for all of them, on the
+ * batch's stream.  This is the batched mode the reference lacks (it steps
+ * one MdState at a time, src/integrate.rs:100-317).  The handles must share
+ * a device and the integrator kind (all Langevin or all plain velocity
+ * Verlet); while in the batch, dyn_md_step refuses them (get/set_state,
+ * get_energies and the engine queries work as before).  A list rebuild
+ * that one system needs rebuilds all.  Destroy the batch before its
+ * members. */
+typedef struct dyn_batch dyn_batch;
+int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds);
+void dyn_batch_destroy(dyn_batch* b);
+const char* dyn_batch_last_error(const dyn_batch* b);
+/* n_steps steps of every member, then a wait (replayed from the starting
+ * state after a list overflow, as dyn_md_step). */
+int dyn_batch_step(dyn_batch* b, uint32_t n_steps);
+/* Per-stage GPU time of the batch's steps (HIP events), as dyn_engine_profile. */
+int dyn_batch_profile(dyn_batch* b, int enable);
+int dyn_batch_profile_read(dyn_batch* b, uint32_t max, const char** names, double* ms,
+                           uint64_t* counts, uint32_t* n);
+
 #ifdef __cplusplus
 }
 #endif
