@@ -146,6 +146,7 @@ _sig("dyn_engine_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
 _sig("dyn_engine_forces_device", C.c_void_p, [_p])
 _sig("dyn_engine_get_energies", C.c_int, [_p, C.POINTER(Energies)])
 _sig("dyn_engine_debug_item_partials", C.c_int, [_p, C.c_void_p, C.c_uint64])
+_sig("dyn_debug_arg_uploads", C.c_uint64, [])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
                                           _p, C.POINTER(C.c_uint64), _p])
@@ -176,7 +177,7 @@ EXPORTED = [
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
-    "dyn_engine_debug_item_partials",
+    "dyn_engine_debug_item_partials", "dyn_debug_arg_uploads",
     "dyn_engine_profile",
     "dyn_engine_profile_read",
     "dyn_batch_create", "dyn_batch_destroy", "dyn_batch_last_error", "dyn_batch_step",
@@ -190,6 +191,11 @@ def _ptr(a):
 
 def _c(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
+
+
+def arg_uploads() -> int:
+    """Kernel argument-pack uploads so far in this process."""
+    return int(lib.dyn_debug_arg_uploads())
 
 
 def device_count() -> int:
@@ -451,7 +457,8 @@ class Engine:
         return out.T.copy()
 
     def profile(self, enable=True):
-        _check(lib.dyn_engine_profile(self.h, 1 if enable else 0), self._err)
+        """True/1: time every stage; 2: the pair kernel only; False/0: off."""
+        _check(lib.dyn_engine_profile(self.h, int(enable)), self._err)
 
     def profile_read(self):
         mx = 64
@@ -545,7 +552,7 @@ class Batch:
         _check(lib.dyn_batch_step(self.h, int(n)), self._err)
 
     def profile(self, enable=True):
-        _check(lib.dyn_batch_profile(self.h, 1 if enable else 0), self._err)
+        _check(lib.dyn_batch_profile(self.h, int(enable)), self._err)
 
     def profile_read(self):
         mx = 64

@@ -6,7 +6,7 @@
 // so a pass over n systems of uneven sizes is one launch whose grid is
 // (largest per-system grid) × n.  Bodies guard their own ranges (a block
 // beyond its system's work returns at once).  A single system is a batch
-// of one: there is one code path.
+// of one, launched through k_single with its pack as the kernel argument.
 //
 // Argument packs change rarely (buffer growth, a new box or λ), so each
 // launch site keeps its device array and uploads only when the packs
@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -93,6 +94,17 @@ __global__ void __launch_bounds__(LB) k_batched(const typename BodySig<F>::P* __
   call_body<F>(a[blockIdx.y], std::make_index_sequence<BodySig<F>::N>{});
 }
 
+// A single system: the pack travels as the kernel argument itself (kernarg
+// segment, no dependent load at kernel start).
+template <auto F, int LB>
+__global__ void __launch_bounds__(LB) k_single(const typename BodySig<F>::P a) {
+  call_body<F>(a, std::make_index_sequence<BodySig<F>::N>{});
+}
+
+// Uploads of argument packs so far in this process (steady-state steps
+// upload none).
+inline std::atomic<uint64_t> g_arg_uploads{0};
+
 // Device copies of the argument packs of each launch site.
 class ArgCache {
  public:
@@ -140,6 +152,7 @@ class ArgCache {
   static constexpr size_t kRing = 4u << 20;
 
   void upload(hipStream_t stream, char* dst, const void* src, size_t bytes) {
+    g_arg_uploads.fetch_add(1, std::memory_order_relaxed);
     if (bytes > kRing) {  // larger than the ring: a plain blocking copy
       DYN_HIP(hipStreamSynchronize(stream));
       DYN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
@@ -195,6 +208,14 @@ template <auto F, int LB = 256, class Fn>
 void launch(const BatchRun& R, const char* site, int tag, unsigned gx, unsigned block,
             size_t shm, Fn&& fn) {
   if (R.n == 0) return;
+  if (R.n == 1) {
+    const typename BodySig<F>::P p = fn(0u);
+    hipLaunchKernelGGL((k_single<F, LB>), dim3(std::max(gx, 1u)), dim3(block), shm, R.stream, p);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+      throw Error(DYN_ERR_HIP, fmt("launch of %s: %s", site, hipGetErrorString(e)));
+    return;
+  }
   std::vector<typename BodySig<F>::P> v;
   v.reserve(R.n);
   for (uint32_t i = 0; i < R.n; ++i) v.push_back(fn(i));

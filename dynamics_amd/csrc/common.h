@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -116,10 +117,12 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned max_blocks = 65535u 
 // and read() synchronizes and sums elapsed times per kernel name.
 struct Profiler {
   bool enabled = false;
+  const char* only = nullptr;  // when set, only scopes of this name are timed
   hipStream_t stream = nullptr;
   struct Rec {
     hipEvent_t a, b;
     const char* name;
+    double host_ms;  // host time spent issuing the scope's work
   };
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
@@ -139,16 +142,20 @@ struct Profiler {
   struct Scope {
     Profiler* p;
     Rec r;
-    Scope(Profiler& prof, const char* name) : p(prof.enabled ? &prof : nullptr) {
+    std::chrono::steady_clock::time_point t0;
+    Scope(Profiler& prof, const char* name)
+        : p(prof.enabled && (!prof.only || std::strcmp(prof.only, name) == 0) ? &prof : nullptr) {
       if (p) {
         r.name = name;
         r.a = p->ev();
         r.b = p->ev();
         DYN_HIP(hipEventRecord(r.a, p->stream));
+        t0 = std::chrono::steady_clock::now();
       }
     }
     ~Scope() noexcept(false) {
       if (p) {
+        r.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         DYN_HIP(hipEventRecord(r.b, p->stream));
         p->pending.push_back(r);
         if (p->pending.size() > 4096) p->drain();
@@ -168,10 +175,28 @@ struct Profiler {
       }
       totals[r.name].first += ms;
       totals[r.name].second += 1;
+      const std::string hn = std::string("host:") + r.name;
+      if (totals.find(hn) == totals.end()) {
+        totals[hn] = {0.0, 0};
+        order.push_back(hn);
+      }
+      totals[hn].first += r.host_ms;
+      totals[hn].second += 1;
       pool.push_back(r.a);
       pool.push_back(r.b);
     }
     pending.clear();
+  }
+  // A host-side duration (ms) under `name` (when profiling).
+  void add_host(const char* name, double ms) {
+    if (!enabled) return;
+    const std::string hn = std::string("host:") + name;
+    if (totals.find(hn) == totals.end()) {
+      totals[hn] = {0.0, 0};
+      order.push_back(hn);
+    }
+    totals[hn].first += ms;
+    totals[hn].second += 1;
   }
   void reset() {
     drain();

@@ -2182,12 +2182,15 @@ extern "C" int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint
   });
 }
 
+extern "C" uint64_t dyn_debug_arg_uploads(void) { return g_arg_uploads.load(); }
+
 extern "C" int dyn_engine_profile(dyn_engine* eng, int enable) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
     DeviceGuard g(eng->device);
     eng->prof.reset();
     eng->prof.enabled = enable != 0;
+    eng->prof.only = enable == 2 ? "pairs" : nullptr;
   });
 }
 

@@ -190,8 +190,10 @@ struct IntArgs {
 // the current forces (M-site force projected onto O and H for water; atom
 // accelerations clamped to MAX_ACCEL); drift: x += v·h_drift for atoms,
 // the rigid drift for water.
-__device__ __forceinline__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v) {
+__device__ __forceinline__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v,
+                                               uint64_t* __restrict__ step_inc) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (step_inc && t == 0) ++*step_inc;  // the step's last kick: the next step index
   if (t < A.n_dyn) {
     float4 vi = v[t];
     if (A.kick) {
@@ -238,7 +240,10 @@ __device__ __forceinline__ void k_md_integrate(IntArgs A, float4* __restrict__ x
 // component, σ² = k_B T (1 − c²)/m; solute atoms, water O and H.
 __device__ __forceinline__ void k_md_langevin(float4* __restrict__ v, uint32_t n_dyn, uint32_t n_water, float c,
                               const float* __restrict__ sigma_dyn, float sigma_o, float sigma_h,
-                              uint64_t seed, uint64_t step) {
+                              uint64_t seed, const uint64_t* __restrict__ step_dev) {
+  // the step index lives on the device (the step's last kick advances it),
+  // so the launch arguments stay the same from step to step
+  const uint64_t step = *step_dev;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t sites[3];
   float sig[3];
@@ -548,6 +553,7 @@ struct dyn_md {
   DevBuf<double> eout;
   PinnedBuf<double> h_out;
   uint64_t step = 0;
+  DevBuf<uint64_t> step_dev;  // the step index, as the Langevin kernel reads it
   uint64_t n_eval = 0;  // force evaluations since set_state (SPME cadence)
   bool state_set = false, forces_valid = false;
   ArgCache own_cache;   // a pass over this MD alone
@@ -663,7 +669,7 @@ void md_integrate(MdSet& S, int tag, float hk, float hd, bool kick, bool drift) 
     A.h_drift = m.prm.dt * hd;
     A.kick = kick ? 1 : 0;
     A.drift = drift ? 1 : 0;
-    return args<k_md_integrate>(A, m.x.p, m.v.p);
+    return args<k_md_integrate>(A, m.x.p, m.v.p, tag == 2 ? m.step_dev.p : (uint64_t*)nullptr);
   });
 }
 
@@ -723,7 +729,7 @@ void md_one_step(MdSet& S, bool energies) {
       launch<k_md_langevin>(S.R, "k_md_langevin", 0, g, kBlock, 0, [&](uint32_t i) {
         dyn_md& m = *S.m[i];
         return args<k_md_langevin>(m.v.p, m.n_dyn, m.n_water, m.lang_c, m.lang_sigma.p, m.sigma_o,
-                                   m.sigma_h, m.prm.seed, m.step);
+                                   m.sigma_h, m.prm.seed, (const uint64_t*)m.step_dev.p);
       });
     }
     md_constrain_velocities(S, 1);
@@ -759,9 +765,14 @@ void md_run(MdSet& S, uint32_t n) {
     ck[i] = {m.step, m.n_eval};
   }
   for (int attempt = 0;; ++attempt) {
+    const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t k = 0; k < n; ++k) md_one_step(S, k + 1 == n);
+    const auto t1 = std::chrono::steady_clock::now();
     try {
       engine_set_check(S.es);
+      const auto t2 = std::chrono::steady_clock::now();
+      S.prof->add_host("md_issue_steps", std::chrono::duration<double, std::milli>(t1 - t0).count());
+      S.prof->add_host("md_wait", std::chrono::duration<double, std::milli>(t2 - t1).count());
       return;
     } catch (const Error& e) {
       if (e.code != DYN_ERR_CAPACITY || attempt >= 2) throw;
@@ -774,6 +785,8 @@ void md_run(MdSet& S, uint32_t n) {
       m.step = ck[i].step;
       m.n_eval = ck[i].eval;
       m.forces_valid = false;
+      DYN_HIP(hipMemcpyAsync(m.step_dev.p, &ck[i].step, 8, hipMemcpyHostToDevice, S.R.stream));
+      DYN_HIP(hipStreamSynchronize(S.R.stream));
     }
   }
 }
@@ -925,6 +938,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     M->v.reserve(ns);
     M->x_ck.reserve(ns);
     M->v_ck.reserve(ns);
+    M->step_dev.reserve(1);
+    DYN_HIP(hipMemsetAsync(M->step_dev.p, 0, 8, M->stream));
     M->eout.reserve(8);
     M->h_out.reserve(8);
     DYN_HIP(hipStreamSynchronize(M->stream));
@@ -1184,6 +1199,7 @@ extern "C" int dyn_batch_profile(dyn_batch* b, int enable) {
     DeviceGuard g(b->device);
     b->prof.reset();
     b->prof.enabled = enable != 0;
+    b->prof.only = enable == 2 ? "pairs" : nullptr;
   });
 }
 

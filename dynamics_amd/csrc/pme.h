@@ -43,6 +43,7 @@ struct PmeCore {
   int order = 4;
   size_t ngrid = 0;
   FftAxis ax[3];
+  DevBuf<FftAxis> ax_dev;       // device copy of ax
   DevBuf<int64_t> grid_fx;      // K0·K1·K2 fixed-point charge grid
   DevBuf<float2> cgrid;         // complex working grid
   DevBuf<float> G;              // influence function

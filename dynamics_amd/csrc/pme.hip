@@ -195,8 +195,9 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 template <int AXIS, int SIGN, bool FROM_FX>
 __device__ __forceinline__ void
 k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
-           FftAxis ax, const float2* __restrict__ tw) {
+           const FftAxis* __restrict__ axp, const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
+  const FftAxis& ax = *axp;  // device copy: uniform loads, no private copy
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
   if (line0 >= g.lines) return;  // beyond this system's lines
@@ -230,10 +231,12 @@ struct ConvArgs {
 // x pass fused with the convolution: forward FFT along x, multiply by G(m)
 // (accumulating ½G|F[Q]|² and its virial weight in f64), inverse FFT along x.
 __device__ __forceinline__ void
-k_fft_conv_x(float2* __restrict__ data, PassGeom g, FftAxis ax, const float2* __restrict__ tw,
+k_fft_conv_x(float2* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
+             const float2* __restrict__ tw,
              const float* __restrict__ G, ConvArgs C, double* __restrict__ partials) {
   extern __shared__ float2 smem[];
   __shared__ double red[2][kFftBlock];
+  const FftAxis& ax = *axp;
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
   if (line0 >= g.lines) return;  // beyond this system's lines (no partial either)
@@ -594,6 +597,8 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
     K[a] = k[a];
     ax[a] = make_fft_axis((int)k[a]);
   }
+  ax_dev.reserve(3);
+  DYN_HIP(hipMemcpyAsync(ax_dev.p, ax, sizeof ax, hipMemcpyHostToDevice, s));
   DYN_REQUIRE(ord == 4 || ord == 5, "B-spline order must be 4 or 5 (got %d)", ord);
   DYN_REQUIRE(b > 0.f && std::isfinite(b), "beta must be > 0");
   order = ord;
@@ -802,7 +807,7 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
     launch<decltype(F)::value, kFftBlock>(R, name, tag, gx_, kFftBlock, shm_, [&](uint32_t i) {
       return args<decltype(F)::value>(on(i) ? cp(i)->cgrid.p : nullptr,
                                       from_fx && on(i) ? cp(i)->grid_fx.p : nullptr, g[i],
-                                      on(i) ? cp(i)->ax[axis] : FftAxis{},
+                                      on(i) ? cp(i)->ax_dev.p + axis : nullptr,
                                       on(i) ? cp(i)->tw[axis].p : nullptr);
     });
   };
@@ -826,7 +831,7 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
         C.pi2_over_b2 = M_PI * M_PI / ((double)cp(i)->beta * cp(i)->beta);
       }
       return args<k_fft_conv_x>(on(i) ? cp(i)->cgrid.p : nullptr, gx[i],
-                                on(i) ? cp(i)->ax[0] : FftAxis{}, on(i) ? cp(i)->tw[0].p : nullptr,
+                                on(i) ? cp(i)->ax_dev.p : nullptr, on(i) ? cp(i)->tw[0].p : nullptr,
                                 on(i) ? cp(i)->G.p : nullptr, C,
                                 on(i) ? cp(i)->partials.p : nullptr);
     });

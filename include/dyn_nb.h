@@ -294,10 +294,16 @@ int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clus
  * e_lj solute, e_coul water, e_coul solute, virial, pair counts (solute–
  * solute, LJ, Coulomb), dH/dλ — every pair counted from both of its rows. */
 int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint64_t n_items);
+/* Diagnostics (no counterpart in the reference): uploads of kernel argument
+ * packs so far in this process (packs are uploaded only when they change,
+ * so steady-state steps add none). */
+uint64_t dyn_debug_arg_uploads(void);
 
-/* Per-kernel device time from HIP events recorded on the engine's stream
- * around every launch while profiling is on.  names/ms/counts receive up to
- * max entries; *n the number of kernels seen. */
+/* Per-stage device time from HIP events recorded on the engine's stream
+ * around every stage while profiling is on (enable 1), or around the pair
+ * kernel only (enable 2: two events per evaluation); 0 turns it off.  Each
+ * stage also reports "host:<stage>", the host time spent issuing it.
+ * names/ms/counts receive up to max entries; *n the number of entries. */
 int dyn_engine_profile(dyn_engine* eng, int enable);
 int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char** names, double* ms,
                             uint64_t* counts, uint32_t* n);
