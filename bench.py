@@ -288,8 +288,12 @@ def run_batch_lambda(args) -> int:
     b = api.Batch([m for _, m in made])
     b.step(args.warmup)
     print("bench: warmup done", file=sys.stderr, flush=True)
-    b.profile(True)
+    up0 = api.arg_uploads()
     t_b = _time_steps(b.step, lambda: made[0][1].energies(), args.steps)
+    uploads = api.arg_uploads() - up0
+    n_prof = max(1, min(args.steps, 20))
+    b.profile(True)
+    b.step(n_prof)
     prof = b.profile_read()
     b.close()
     dh = [e.energies()["dh_dl"] for e, _ in made]
@@ -310,7 +314,11 @@ def run_batch_lambda(args) -> int:
         "batched_over_single": agg / one,
         "dh_dl_range": [min(dh), max(dh)],
         "temperature_K_range": [min(temps), max(temps)],
-        "kernels_ms_per_step": {k: v_[0] / args.steps for k, v_ in prof.items()},
+        "arg_uploads_in_timed_steps": uploads,
+        "kernels_ms_per_step": {k: v_[0] / n_prof for k, v_ in prof.items()
+                                if not k.startswith("host:")},
+        "host_ms_per_step": {k[5:]: v_[0] / n_prof for k, v_ in prof.items()
+                             if k.startswith("host:")},
     }
     for e, m in made:
         m.close()
@@ -388,8 +396,12 @@ def run_batch_uneven(args) -> int:
     b = api.Batch([m for _, m, _ in made])
     b.step(args.warmup)
     print("bench: warmup done", file=sys.stderr, flush=True)
-    b.profile(True)
+    up0 = api.arg_uploads()
     t_b = _time_steps(b.step, lambda: made[0][1].energies(), args.steps)
+    uploads = api.arg_uploads() - up0
+    n_prof = max(1, min(args.steps, 20))
+    b.profile(True)
+    b.step(n_prof)
     prof = b.profile_read()
     b.close()
     temps = [m.energies()["temperature"] for _, m, _ in made]
@@ -409,7 +421,11 @@ def run_batch_uneven(args) -> int:
         "sequential_ns_per_day": seq,
         "batched_over_sequential": agg / seq,
         "temperature_K_range": [min(temps), max(temps)],
-        "kernels_ms_per_step": {k: v_[0] / args.steps for k, v_ in prof.items()},
+        "arg_uploads_in_timed_steps": uploads,
+        "kernels_ms_per_step": {k: v_[0] / n_prof for k, v_ in prof.items()
+                                if not k.startswith("host:")},
+        "host_ms_per_step": {k[5:]: v_[0] / n_prof for k, v_ in prof.items()
+                             if k.startswith("host:")},
     }
     for e, m, _ in made:
         m.close()

@@ -1020,16 +1020,30 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
   if (pme_runs >= 0 && t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
 }
 
+// Largest squared displacement since the list build: one atomic per
+// workgroup (block-wide max first; 256 threads).  Squared distances are
+// non-negative floats, ordered like their bit patterns.
 __device__ __forceinline__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
                                uint32_t n, EngGeom g, unsigned int* __restrict__ out_bits) {
+  __shared__ unsigned int red[256 / kWave];
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const float4 a = wpos[s], b = ref[s];
-  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-  dx -= g.L[0] * rintf(dx * g.invL[0]);
-  dy -= g.L[1] * rintf(dy * g.invL[1]);
-  dz -= g.L[2] * rintf(dz * g.invL[2]);
-  atomicMax(out_bits, __float_as_uint(dx * dx + dy * dy + dz * dz));
+  unsigned int v = 0u;
+  if (s < n) {
+    const float4 a = wpos[s], b = ref[s];
+    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    dx -= g.L[0] * rintf(dx * g.invL[0]);
+    dy -= g.L[1] * rintf(dy * g.invL[1]);
+    dz -= g.L[2] * rintf(dz * g.invL[2]);
+    v = __float_as_uint(dx * dx + dy * dy + dz * dz);
+  }
+  for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int m = 0u;
+    for (int w = 0; w < 256 / kWave; ++w) m = max(m, red[w]);
+    if (m) atomicMax(out_bits, m);
+  }
 }
 
 }  // namespace
