@@ -32,6 +32,8 @@ COMPUTE_REBUILD = 8
 COMPUTE_ASYNC = 16
 COMPUTE_NO_ENERGY = 32
 COMPUTE_RECIP_REUSE = 64
+PREC_PAIR_F64 = 1
+PREC_PME_F64 = 2
 
 
 class DynError(RuntimeError):
@@ -73,7 +75,7 @@ class MdEnergies(C.Structure):
 class EngineParams(C.Structure):
     _fields_ = [("cutoff", C.c_float), ("lj_cutoff", C.c_float), ("beta", C.c_float),
                 ("grid", C.c_uint32 * 3), ("order", C.c_uint32), ("mode", C.c_uint32),
-                ("list_buffer", C.c_float)]
+                ("list_buffer", C.c_float), ("precision", C.c_uint32)]
 
 
 class Energies(C.Structure):
@@ -356,10 +358,12 @@ class Engine:
     """Native nonbonded engine over positions + topology."""
 
     def __init__(self, system, cutoff=10.0, lj_cutoff=None, beta=0.3123, grid=(0, 0, 0),
-                 order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0):
+                 order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0, precision=0):
+        """precision: PREC_PAIR_F64 | PREC_PME_F64 bits (0: FP32)."""
         s = system
         prm = EngineParams(cutoff, cutoff if lj_cutoff is None else lj_cutoff, beta,
-                           (C.c_uint32 * 3)(*[int(g) for g in grid]), order, mode, list_buffer)
+                           (C.c_uint32 * 3)(*[int(g) for g in grid]), order, mode, list_buffer,
+                           precision)
         self.h = C.c_void_p()
         q = _c(s.q_dyn, np.float32)
         t = _c(s.type_dyn, np.int32)

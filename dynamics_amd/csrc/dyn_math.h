@@ -36,6 +36,9 @@
 struct float2 {
   float x, y;
 };
+struct double2 {
+  double x, y;
+};
 #endif
 
 #if defined(__clang__)
@@ -219,6 +222,12 @@ DYN_HD int64_t dyn_fx_d(double f, double scale) {
   return dyn_d2bits(d) - (int64_t)0x4338000000000000LL;
 }
 
+// dyn_fx for a double value (FP64 pair option): same clamp, same units.
+DYN_HD int64_t dyn_fx64(double f) {
+  f = f < -(double)DYN_FX_CLAMP ? -(double)DYN_FX_CLAMP : (f > (double)DYN_FX_CLAMP ? (double)DYN_FX_CLAMP : f);
+  return dyn_fx_d(f, DYN_FX_SCALE);
+}
+
 // ---------------------------------------------------------------------------
 // Minimum image.
 // ---------------------------------------------------------------------------
@@ -380,20 +389,22 @@ DYN_HD void dyn_lj_soft_core(float r2, float sigma, float eps, float lambda,
 // th[j] is the weight for offset j (grid index k0 - order + 1 + j), dth its
 // derivative with respect to the fractional coordinate.
 // ---------------------------------------------------------------------------
-template <int ORDER>
-DYN_HD void dyn_bspline(float w, float* th, float* dth) {
+// R = float for the f32 paths (bit-identical on device and host), double for
+// the FP64 SPME option.
+template <int ORDER, class R = float>
+DYN_HD void dyn_bspline(R w, R* th, R* dth) {
   // order 2
-  float m[ORDER];
-  m[0] = 1.0f - w;
+  R m[ORDER];
+  m[0] = R(1) - w;
   m[1] = w;
   for (int k = 2; k < ORDER - 1; ++k) {
     // raise from order k to k+1
-    const float div = 1.0f / (float)k;
+    const R div = R(1) / (R)k;
     m[k] = div * w * m[k - 1];
     for (int j = 1; j < k; ++j) {
-      m[k - j] = div * ((w + (float)j) * m[k - j - 1] + ((float)(k - j + 1) - w) * m[k - j]);
+      m[k - j] = div * ((w + (R)j) * m[k - j - 1] + ((R)(k - j + 1) - w) * m[k - j]);
     }
-    m[0] = div * (1.0f - w) * m[0];
+    m[0] = div * (R(1) - w) * m[0];
   }
   // derivative of the order-ORDER spline from the order-(ORDER-1) values
   dth[0] = -m[0];
@@ -402,12 +413,12 @@ DYN_HD void dyn_bspline(float w, float* th, float* dth) {
   // final raise to ORDER
   {
     const int k = ORDER - 1;
-    const float div = 1.0f / (float)k;
+    const R div = R(1) / (R)k;
     m[k] = div * w * m[k - 1];
     for (int j = 1; j < k; ++j) {
-      m[k - j] = div * ((w + (float)j) * m[k - j - 1] + ((float)(k - j + 1) - w) * m[k - j]);
+      m[k - j] = div * ((w + (R)j) * m[k - j - 1] + ((R)(k - j + 1) - w) * m[k - j]);
     }
-    m[0] = div * (1.0f - w) * m[0];
+    m[0] = div * (R(1) - w) * m[0];
   }
   for (int j = 0; j < ORDER; ++j) th[j] = m[j];
 }

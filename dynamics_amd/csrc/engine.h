@@ -117,6 +117,51 @@ DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, 
   *hit = in ? 1 : 0;
 }
 
+// FP64 variant for the precision sweep: the same interactions from the same
+// f32 inputs (positions, charges, combined σ and ε), every operation in
+// double with the math library's erfc, exp and sqrt; c6/c12 from a double
+// table.  Not bit-replayed on the host: it is
+// checked against the FP64 oracle instead.
+template <int KIND>
+DYN_HD void dyn_tile_pair_f64(const float pi[4], int ti, const float pj[4], int tj, bool active,
+                              const DynPairParams& P, const double2* lj64, double d[3],
+                              double* s_out, double* e_lj, double* e_c, double* r2_out, int* hit) {
+  double r2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const double L = (double)P.ext[a];
+    double v = (double)pi[a] - (double)pj[a];
+    v -= L * rint(v / L);
+    d[a] = v;
+    r2 += v * v;
+  }
+  const bool in = active && r2 < (double)P.rc2 && r2 > 1e-16;
+  const double r2s = in ? r2 : 1.0;
+  const double rinv = 1.0 / sqrt(r2s);
+  double s = 0.0, elj = 0.0, ec = 0.0;
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) {
+    // the O–O pair too comes from the table (its f32 constants are rounded)
+    const double2 c = lj64[(ti & DYN_TYPE_MASK) * P.n_types + (tj & DYN_TYPE_MASK)];
+    const double c6 = c.x, c12 = c.y;
+    const double rinv2 = rinv * rinv, rinv6 = rinv2 * rinv2 * rinv2;
+    const bool lj_in = in && r2 <= (double)P.lj_rc2;
+    s = lj_in ? (12.0 * c12 * rinv6 - 6.0 * c6) * rinv6 * rinv2 : 0.0;
+    elj = lj_in ? (c12 * rinv6 - c6) * rinv6 : 0.0;
+  }
+  if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SQ || KIND == DYN_KIND_QQ) {
+    const double qq = (double)pi[3] * (double)pj[3];
+    const double beta = (double)P.beta, r = r2s * rinv;
+    const double g = erfc(beta * r) * rinv;
+    const double sc = qq * (g + 1.1283791670955126 * beta * exp(-beta * beta * r2s)) * rinv * rinv;
+    s += in ? sc : 0.0;
+    ec = in ? qq * g : 0.0;
+  }
+  *s_out = s;
+  *e_lj = elj;
+  *e_c = ec;
+  *r2_out = in ? r2 : 0.0;
+  *hit = in ? 1 : 0;
+}
+
 // Alchemical variant (src/cuda/cuda.cu:367-566): a lane whose two sites lie
 // on different sides of the decoupled molecule (exactly one carries
 // DYN_TYPE_ALCH) takes Beutler soft-core LJ at the schedule's LJ λ and

@@ -31,6 +31,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -607,6 +608,7 @@ struct PairArgs {
   const uint32_t* n_chunks;
   const float2* lj_table;    // (c6, c12) per type pair
   const float2* lj_se;       // (σ, ε) per type pair, for soft-core
+  const double2* lj_table64; // (c6, c12) in double, for the FP64 option
   int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
@@ -616,9 +618,10 @@ struct PairArgs {
 __device__ __forceinline__ float xor_sum(float v, int m) { return v + __shfl_xor(v, m, kWave); }
 __device__ __forceinline__ double xor_sum_d(double v, int m) { return v + __shfl_xor(v, m, kWave); }
 
-struct TileAcc {
-  float fi[3];
-  float e_lj_s, e_lj_w, e_c_s, e_c_w, vir, dh;
+template <class T>
+struct TileAccT {
+  T fi[3];
+  T e_lj_s, e_lj_w, e_c_s, e_c_w, vir, dh;
   uint32_t n_ss, n_lj, n_c;  // wave-uniform counts (popcounts)
 };
 
@@ -626,13 +629,18 @@ struct TileAcc {
 // boundary (the alchemical variant returns the plain result on the others).
 // ENERGY: accumulate energies, virial, dH/dλ and pair counts (forces only
 // otherwise).
-template <int KIND, bool ALCH, bool ENERGY>
+// F64: FP64 pair arithmetic (precision sweep; no alchemical groups).
+template <int KIND, bool ALCH, bool ENERGY, bool F64>
 __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4], int ti,
-                                           const float4& pjv, int tj, bool active, TileAcc& T) {
+                                           const float4& pjv, int tj, bool active,
+                                           TileAccT<std::conditional_t<F64, double, float>>& T) {
+  using R = std::conditional_t<F64, double, float>;
   const float pj[4] = {pjv.x, pjv.y, pjv.z, pjv.w};
-  float d[3], s, elj, ec, r2;
+  R d[3], s, elj, ec, r2;
   int hit;
-  if (ALCH) {
+  if constexpr (F64) {
+    dyn_tile_pair_f64<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table64, d, &s, &elj, &ec, &r2, &hit);
+  } else if constexpr (ALCH) {
     float dh;
     dyn_tile_pair_alch<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, A.lj_se, d, &s, &elj, &ec,
                              &r2, &hit, &dh);
@@ -667,9 +675,10 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
 // energies, virial and pair counts as one partial per value; the grid has a
 // wave per work item, so the hardware's wave dispatch balances the load and
 // every sum keeps a fixed order.
-template <bool ENERGY>
+template <bool ENERGY, bool F64>
 __device__ __forceinline__ void
 k_eng_pairs(PairArgs A) {
+  using R = std::conditional_t<F64, double, float>;
   const int lane = threadIdx.x & (kWave - 1);
   const int ia = lane & 7, jb = lane >> 3;
   if (*A.status) return;
@@ -686,9 +695,9 @@ k_eng_pairs(PairArgs A) {
     const int ti = A.slot_type[I * DYN_CL_SIZE + ia];
     const int gi = A.slot_gid[I * DYN_CL_SIZE + ia];
     const int ci = A.cl_class[I];
-    TileAcc T;
-    T.fi[0] = T.fi[1] = T.fi[2] = 0.f;
-    T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = T.dh = 0.f;
+    TileAccT<R> T;
+    T.fi[0] = T.fi[1] = T.fi[2] = R(0);
+    T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = T.dh = R(0);
     T.n_ss = T.n_lj = T.n_c = 0;
     uint32_t v = A.cj[(size_t)g0 * DYN_CL_SIZE + jb];
     uint32_t js = v & DYN_JSLOT_MASK;
@@ -711,39 +720,51 @@ k_eng_pairs(PairArgs A) {
       const int kind = dyn_kind(ci, cls);
       bool active = dyn_slot_pair_valid(gi, gjc);
       if (kind == DYN_KIND_SS) active = active && ((A.cmask[g] >> lane) & 1ull);
-      const bool alch = A.P.alch_on && kind <= DYN_KIND_SQ &&
+      const bool alch = !F64 && A.P.alch_on && kind <= DYN_KIND_SQ &&
                         __ballot(((ti ^ tjc) & DYN_TYPE_ALCH) != 0) != 0ull;
       switch (kind) {
-        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false, ENERGY>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false, ENERGY>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T); break;
         case DYN_KIND_SS:
-          if (alch) tile_entry<DYN_KIND_SS, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SS, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SS, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SS, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
           break;
         case DYN_KIND_SO:
-          if (alch) tile_entry<DYN_KIND_SO, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SO, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SO, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
           break;
         case DYN_KIND_SQ:
-          if (alch) tile_entry<DYN_KIND_SQ, true, ENERGY>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SQ, false, ENERGY>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SQ, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          else tile_entry<DYN_KIND_SQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
           break;
         default: break;
       }
     }
     // i forces: sum over the 8 j lanes of each i slot (lanes ia, ia+8, …, ia+56)
     for (int c = 0; c < 3; ++c) {
-      float f = T.fi[c];
-      f = xor_sum(f, 8);
-      f = xor_sum(f, 16);
-      f = xor_sum(f, 32);
+      R f = T.fi[c];
+      if constexpr (F64) {
+        f = xor_sum_d(f, 8);
+        f = xor_sum_d(f, 16);
+        f = xor_sum_d(f, 32);
+      } else {
+        f = xor_sum(f, 8);
+        f = xor_sum(f, 16);
+        f = xor_sum(f, 32);
+      }
       T.fi[c] = f;
     }
     if (jb == 0) {
       int64_t* o = A.fitem + (size_t)item * (3 * DYN_CL_SIZE);
-      o[ia] = dyn_fx(T.fi[0]);
-      o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
-      o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
+      if constexpr (F64) {
+        o[ia] = dyn_fx64(T.fi[0]);
+        o[DYN_CL_SIZE + ia] = dyn_fx64(T.fi[1]);
+        o[2 * DYN_CL_SIZE + ia] = dyn_fx64(T.fi[2]);
+      } else {
+        o[ia] = dyn_fx(T.fi[0]);
+        o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
+        o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
+      }
     }
     if (!ENERGY) continue;
     double acc[6] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir, T.dh};
@@ -1070,6 +1091,7 @@ struct dyn_engine {
   DevBuf<int32_t> wmol;
   DevBuf<float2> lj_table;   // (c6, c12) per type pair
   DevBuf<float2> lj_se;      // (σ, ε) per type pair (soft-core)
+  DevBuf<double2> lj_table64;  // (c6, c12) in double (FP64 pair option)
   std::vector<int32_t> h_type;  // per-site type words (host copy)
   // decoupled molecule: solute sites [alch0, alch1), λ
   uint32_t alch0 = 0, alch1 = 0;
@@ -1329,7 +1351,10 @@ static void set_build_lists(EngineSet& S) {
     ListArgs& A = LA[i];
     std::memset(&A, 0, sizeof A);
     A.g = e.geom;
-    A.rlist = e.prm.cutoff + std::max(0.f, e.prm.list_buffer);
+    // FP64 pair arithmetic decides the cutoff in double: 0.01 Å of list
+    // margin keeps every pair the f32 distance puts just outside
+    A.rlist = e.prm.cutoff + std::max(0.f, e.prm.list_buffer) +
+              ((e.prm.precision & DYN_PREC_PAIR_F64) ? 0.01f : 0.f);
     A.rlist2 = A.rlist * A.rlist;
     A.n_cl = e.counters.p + 0;
     A.bb_c = e.bb_c.p;
@@ -1549,6 +1574,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       A.n_chunks = e.counters.p + 2;
       A.lj_table = e.lj_table.p;
       A.lj_se = e.lj_se.p;
+      A.lj_table64 = e.lj_table64.p;
       A.fitem = e.fitem.p;
       A.partials = e.pair_part.p;
       A.cap_chunks = e.cap_chunks;
@@ -1558,12 +1584,21 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     const unsigned gp = max_grid(S, [](dyn_engine& e) {
       return (unsigned)std::max<uint64_t>(1, (e.cap_chunks + 3) / 4);
     });
-    if (energy_on)
-      launch<k_eng_pairs<true>, kPairBlock>(R, "k_eng_pairs<true>", 0, gp, kPairBlock, 0,
-                                            [&](uint32_t i) { return args<k_eng_pairs<true>>(PA[i]); });
+    // FP64 pair arithmetic when the first system asks for it (a set shares it)
+    const bool f64 = (E[0]->prm.precision & DYN_PREC_PAIR_F64) != 0;
+    auto pa = [&](uint32_t i) { return PA[i]; };
+    if (energy_on && !f64)
+      launch<k_eng_pairs<true, false>, kPairBlock>(R, "k_eng_pairs<true>", 0, gp, kPairBlock, 0,
+          [&](uint32_t i) { return args<k_eng_pairs<true, false>>(pa(i)); });
+    else if (!f64)
+      launch<k_eng_pairs<false, false>, kPairBlock>(R, "k_eng_pairs<false>", 0, gp, kPairBlock, 0,
+          [&](uint32_t i) { return args<k_eng_pairs<false, false>>(pa(i)); });
+    else if (energy_on)
+      launch<k_eng_pairs<true, true>, kPairBlock>(R, "k_eng_pairs<true, f64>", 0, gp, kPairBlock, 0,
+          [&](uint32_t i) { return args<k_eng_pairs<true, true>>(pa(i)); });
     else
-      launch<k_eng_pairs<false>, kPairBlock>(R, "k_eng_pairs<false>", 0, gp, kPairBlock, 0,
-                                             [&](uint32_t i) { return args<k_eng_pairs<false>>(PA[i]); });
+      launch<k_eng_pairs<false, true>, kPairBlock>(R, "k_eng_pairs<false, f64>", 0, gp, kPairBlock, 0,
+          [&](uint32_t i) { return args<k_eng_pairs<false, true>>(pa(i)); });
   }
   if (any_corr) {
     Profiler::Scope ps(prof, "corr");
@@ -1721,6 +1756,8 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_REQUIRE(p.beta > 0.f && p.beta * p.cutoff <= 4.0f,
                 "beta must be > 0 with beta·cutoff ≤ 4 (range of the erfc approximation)");
     DYN_REQUIRE(p.mode == DYN_MODE_REFERENCE || p.mode == DYN_MODE_STANDARD, "unknown mode");
+    DYN_REQUIRE((p.precision & ~(DYN_PREC_PAIR_F64 | DYN_PREC_PME_F64)) == 0,
+                "unknown precision bits 0x%x", p.precision);
     DYN_REQUIRE(p.list_buffer >= 0.f && p.list_buffer < 10.f, "list_buffer must be in [0, 10)");
     DYN_REQUIRE(n_dyn + 4ull * n_water > 0, "empty system");
     DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 30), "system too large");
@@ -1758,6 +1795,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     sig[E->type_o] = o_sigma;
     eps[E->type_o] = 0.2128008130f;
     std::vector<float2> tab((size_t)nt * nt), tse((size_t)nt * nt);
+    std::vector<double2> tab64((size_t)nt * nt);
     for (int a = 0; a < nt; ++a)
       for (int b = 0; b < nt; ++b) {
         // Lorentz–Berthelot in f32 (src/non_bonded.rs:1081-1086), then
@@ -1766,6 +1804,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
         const float e_ab = std::sqrt(eps[a] * eps[b]);
         const double s6 = std::pow((double)s_ab, 6.0);
         tab[(size_t)a * nt + b] = make_float2((float)(4.0 * e_ab * s6), (float)(4.0 * e_ab * s6 * s6));
+        tab64[(size_t)a * nt + b] = make_double2(4.0 * e_ab * s6, 4.0 * e_ab * s6 * s6);
         tse[(size_t)a * nt + b] = make_float2(s_ab, e_ab);
       }
     E->c6_oo = tab[(size_t)E->type_o * nt + E->type_o].x;
@@ -1853,6 +1892,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     up(E->ltype, ht);
     up(E->wmol, hw);
     up(E->lj_table, tab);
+    up(E->lj_table64, tab64);
     up(E->lj_se, tse);
     E->h_type = ht;
     up(E->ex_ptr, hptr);
@@ -1935,7 +1975,8 @@ extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
     if (eng->pme_on) {
       if (eng->pme.ngrid == 0)
         eng->pme.init(eng->device, eng->stream, &eng->prof, eng->prm.grid, eng->box,
-                      eng->prm.beta, (int)eng->prm.order);
+                      eng->prm.beta, (int)eng->prm.order,
+                      (eng->prm.precision & DYN_PREC_PME_F64) != 0);
       else
         eng->pme.set_box(eng->box);
     }
@@ -1986,6 +2027,8 @@ extern "C" int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32
     DYN_REQUIRE(std::isfinite(lambda), "lambda must be finite");
     DYN_REQUIRE((uint64_t)first + count <= eng->n_dyn, "alchemical atoms [%u, %u) beyond the %u "
                 "solute atoms", first, first + count, eng->n_dyn);
+    DYN_REQUIRE(count == 0 || !(eng->prm.precision & DYN_PREC_PAIR_F64),
+                "alchemical decoupling runs in FP32 pair arithmetic only");
     DYN_REQUIRE(count == 0 || eng->prm.mode == DYN_MODE_REFERENCE,
                 "alchemical decoupling needs DYN_MODE_REFERENCE");
     const uint32_t a0 = count ? first : 0, a1 = count ? first + count : 0;

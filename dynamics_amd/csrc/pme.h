@@ -44,10 +44,14 @@ struct PmeCore {
   size_t ngrid = 0;
   FftAxis ax[3];
   DevBuf<FftAxis> ax_dev;       // device copy of ax
+  bool f64 = false;             // FP64 option: the *64 buffers replace their f32 twins
   DevBuf<int64_t> grid_fx;      // K0·K1·K2 fixed-point charge grid
   DevBuf<float2> cgrid;         // complex working grid
   DevBuf<float> G;              // influence function
   DevBuf<float2> tw[3];         // e^{-2πi k/N} tables per axis
+  DevBuf<double2> cgrid64;
+  DevBuf<double> G64;
+  DevBuf<double2> tw64[3];
   DevBuf<double> partials;      // energy/virial partials of the convolution pass
   PinnedBuf<double> h_part;
   unsigned conv_blocks = 0;
@@ -61,7 +65,7 @@ struct PmeCore {
   DevBuf<uint32_t> brick_items; // charges sorted by home brick
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
-            float beta, int order);
+            float beta, int order, bool fp64 = false);
   void set_box(const float l[3]);  // recomputes G
   // Enqueue spread → FFT → convolve → inverse FFT → gather for n charges.
   // xyzq: device float4 array (positions wrapped or not; wrapped internally).

@@ -5,6 +5,8 @@
 // PmeRecip (used at src/non_bonded.rs:681-795, :850-875).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -61,56 +63,78 @@ namespace {
 constexpr int kFftBlock = 256;
 constexpr int kFftCap = 2048;  // complex elements per workgroup (two LDS buffers: 32 KB)
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// Complex arithmetic on float2 (the default SPME) or double2 (the FP64
+// option).
+template <class C>
+struct CxT;
+template <>
+struct CxT<float2> {
+  using R = float;
+  __device__ static float2 mk(float x, float y) { return make_float2(x, y); }
+  static constexpr float c3 = -0.5f, s3 = 0.866025403784438647f;
+  static constexpr float c51 = 0.309016994374947424f, c52 = -0.809016994374947424f;
+  static constexpr float s51 = 0.951056516295153572f, s52 = 0.587785252292473129f;
+};
+template <>
+struct CxT<double2> {
+  using R = double;
+  __device__ static double2 mk(double x, double y) { return make_double2(x, y); }
+  static constexpr double c3 = -0.5, s3 = 0.866025403784438647;
+  static constexpr double c51 = 0.309016994374947424, c52 = -0.809016994374947424;
+  static constexpr double s51 = 0.951056516295153572, s52 = 0.587785252292473129;
+};
+
+template <class C>
+__device__ __forceinline__ C cadd(C a, C b) { return CxT<C>::mk(a.x + b.x, a.y + b.y); }
+template <class C>
+__device__ __forceinline__ C csub(C a, C b) { return CxT<C>::mk(a.x - b.x, a.y - b.y); }
+template <class C>
+__device__ __forceinline__ C cscale(C a, typename CxT<C>::R s) { return CxT<C>::mk(a.x * s, a.y * s); }
 // multiply by SIGN·i
-template <int SIGN>
-__device__ __forceinline__ float2 cmul_si(float2 a) {
-  return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+template <int SIGN, class C>
+__device__ __forceinline__ C cmul_si(C a) {
+  return SIGN < 0 ? CxT<C>::mk(a.y, -a.x) : CxT<C>::mk(-a.y, a.x);
 }
 // v · w where w = tw (SIGN = −1) or conj(tw) (SIGN = +1)
-template <int SIGN>
-__device__ __forceinline__ float2 ctw(float2 v, float2 t) {
-  const float ti = SIGN < 0 ? t.y : -t.y;
-  return make_float2(v.x * t.x - v.y * ti, v.x * ti + v.y * t.x);
+template <int SIGN, class C>
+__device__ __forceinline__ C ctw(C v, C t) {
+  const typename CxT<C>::R ti = SIGN < 0 ? t.y : -t.y;
+  return CxT<C>::mk(v.x * t.x - v.y * ti, v.x * ti + v.y * t.x);
 }
 
-template <int SIGN>
-__device__ __forceinline__ void dft2(float2* v) {
-  const float2 a = v[0], b = v[1];
+template <int SIGN, class C>
+__device__ __forceinline__ void dft2(C* v) {
+  const C a = v[0], b = v[1];
   v[0] = cadd(a, b);
   v[1] = csub(a, b);
 }
-template <int SIGN>
-__device__ __forceinline__ void dft4(float2* v) {
-  const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-  const float2 s13 = cadd(v[1], v[3]), d13 = cmul_si<SIGN>(csub(v[1], v[3]));
+template <int SIGN, class C>
+__device__ __forceinline__ void dft4(C* v) {
+  const C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+  const C s13 = cadd(v[1], v[3]), d13 = cmul_si<SIGN>(csub(v[1], v[3]));
   v[0] = cadd(s02, s13);
   v[2] = csub(s02, s13);
   v[1] = cadd(d02, d13);
   v[3] = csub(d02, d13);
 }
-template <int SIGN>
-__device__ __forceinline__ void dft3(float2* v) {
-  const float c = -0.5f, s = 0.866025403784438647f;
-  const float2 t = cadd(v[1], v[2]);
-  const float2 d = cscale(cmul_si<SIGN>(csub(v[1], v[2])), s);
-  const float2 a = cadd(v[0], cscale(t, c));
+template <int SIGN, class C>
+__device__ __forceinline__ void dft3(C* v) {
+  const C t = cadd(v[1], v[2]);
+  const C d = cscale(cmul_si<SIGN>(csub(v[1], v[2])), CxT<C>::s3);
+  const C a = cadd(v[0], cscale(t, CxT<C>::c3));
   v[0] = cadd(v[0], t);
   v[1] = cadd(a, d);
   v[2] = csub(a, d);
 }
-template <int SIGN>
-__device__ __forceinline__ void dft5(float2* v) {
-  const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;
-  const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;
-  const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
-  const float2 d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
-  const float2 a1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
-  const float2 a2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
-  const float2 b1 = cmul_si<SIGN>(cadd(cscale(d1, s1), cscale(d2, s2)));
-  const float2 b2 = cmul_si<SIGN>(csub(cscale(d1, s2), cscale(d2, s1)));
+template <int SIGN, class C>
+__device__ __forceinline__ void dft5(C* v) {
+  const auto c1 = CxT<C>::c51, c2 = CxT<C>::c52, s1 = CxT<C>::s51, s2 = CxT<C>::s52;
+  const C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const C d1 = csub(v[1], v[4]), d2 = csub(v[2], v[3]);
+  const C a1 = cadd(v[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+  const C a2 = cadd(v[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+  const C b1 = cmul_si<SIGN>(cadd(cscale(d1, s1), cscale(d2, s2)));
+  const C b2 = cmul_si<SIGN>(csub(cscale(d1, s2), cscale(d2, s1)));
   v[0] = cadd(v[0], cadd(t1, t2));
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
@@ -120,9 +144,8 @@ __device__ __forceinline__ void dft5(float2* v) {
 
 // Stockham autosort FFT of nl lines of length N held in LDS as a[l·N + n].
 // Returns the buffer holding the (natural-order) result.
-template <int SIGN>
-__device__ float2* lds_fft(float2* a, float2* b, int nl, const FftAxis& ax,
-                           const float2* __restrict__ tw) {
+template <int SIGN, class C>
+__device__ C* lds_fft(C* a, C* b, int nl, const FftAxis& ax, const C* __restrict__ tw) {
   const int N = ax.N;
   int Ns = 1;
   for (int s = 0; s < ax.nstages; ++s) {
@@ -134,11 +157,11 @@ __device__ float2* lds_fft(float2* a, float2* b, int nl, const FftAxis& ax,
       const int l = i / NR;
       const int j = i - l * NR;
       const int k = j % Ns;
-      const float2* src = a + l * N;
-      float2* dst = b + l * N;
-      float2 v[5];
+      const C* src = a + l * N;
+      C* dst = b + l * N;
+      C v[5];
       for (int r = 0; r < R; ++r) {
-        float2 x = src[j + r * NR];
+        C x = src[j + r * NR];
         if (r) x = ctw<SIGN>(x, tw[(r * k * tw_step) % N]);
         v[r] = x;
       }
@@ -150,7 +173,7 @@ __device__ float2* lds_fft(float2* a, float2* b, int nl, const FftAxis& ax,
       for (int r = 0; r < R; ++r) dst[d + r * Ns] = v[r];
     }
     __syncthreads();
-    float2* t = a;
+    C* t = a;
     a = b;
     b = t;
     Ns *= R;
@@ -192,34 +215,36 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 
 // One FFT pass along AXIS over complex data in place.  FROM_FX: the input is
 // the int64 fixed-point charge grid (z pass only).
-template <int AXIS, int SIGN, bool FROM_FX>
+template <int AXIS, int SIGN, bool FROM_FX, class C>
 __device__ __forceinline__ void
-k_fft_pass(float2* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
-           const FftAxis* __restrict__ axp, const float2* __restrict__ tw) {
-  extern __shared__ float2 smem[];
+k_fft_pass(C* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
+           const FftAxis* __restrict__ axp, const C* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* smem = reinterpret_cast<C*>(smem_raw);
+  using R = typename CxT<C>::R;
   const FftAxis& ax = *axp;  // device copy: uniform loads, no private copy
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
   if (line0 >= g.lines) return;  // beyond this system's lines
   const int nl = min(g.L, g.lines - line0);
-  float2* A = smem;
-  float2* B = smem + g.L * N;
+  C* A = smem;
+  C* B = smem + g.L * N;
   const int tot = nl * N;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<AXIS>(e, nl, N, l, n);
     const size_t addr = pass_addr<AXIS>(g, line0 + l, n);
-    float2 v;
-    if (FROM_FX) v = make_float2((float)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), 0.f);
+    C v;
+    if (FROM_FX) v = CxT<C>::mk((R)((double)grid_fx[addr] * (1.0 / DYN_GRID_FX_SCALE)), R(0));
     else v = data[addr];
     A[l * N + n] = v;
   }
   __syncthreads();
-  float2* R = lds_fft<SIGN>(A, B, nl, ax, tw);
+  C* Rs = lds_fft<SIGN>(A, B, nl, ax, tw);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<AXIS>(e, nl, N, l, n);
-    data[pass_addr<AXIS>(g, line0 + l, n)] = R[l * N + n];
+    data[pass_addr<AXIS>(g, line0 + l, n)] = Rs[l * N + n];
   }
 }
 
@@ -230,19 +255,21 @@ struct ConvArgs {
 
 // x pass fused with the convolution: forward FFT along x, multiply by G(m)
 // (accumulating ½G|F[Q]|² and its virial weight in f64), inverse FFT along x.
+template <class CX>
 __device__ __forceinline__ void
-k_fft_conv_x(float2* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
-             const float2* __restrict__ tw,
-             const float* __restrict__ G, ConvArgs C, double* __restrict__ partials) {
-  extern __shared__ float2 smem[];
+k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
+             const CX* __restrict__ tw, const typename CxT<CX>::R* __restrict__ G, ConvArgs C,
+             double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  CX* smem = reinterpret_cast<CX*>(smem_raw);
   __shared__ double red[2][kFftBlock];
   const FftAxis& ax = *axp;
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
   if (line0 >= g.lines) return;  // beyond this system's lines (no partial either)
   const int nl = min(g.L, g.lines - line0);
-  float2* A = smem;
-  float2* B = smem + g.L * N;
+  CX* A = smem;
+  CX* B = smem + g.L * N;
   const int tot = nl * N;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
@@ -250,8 +277,8 @@ k_fft_conv_x(float2* __restrict__ data, PassGeom g, const FftAxis* __restrict__ 
     A[l * N + n] = data[pass_addr<0>(g, line0 + l, n)];
   }
   __syncthreads();
-  float2* R = lds_fft<-1>(A, B, nl, ax, tw);
-  float2* O = (R == A) ? B : A;
+  CX* R = lds_fft<-1>(A, B, nl, ax, tw);
+  CX* O = (R == A) ? B : A;
   double e_acc = 0.0, w_acc = 0.0;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
@@ -259,8 +286,8 @@ k_fft_conv_x(float2* __restrict__ data, PassGeom g, const FftAxis* __restrict__ 
     const int line = line0 + l;
     const int my = line / g.K[2], mz = line - my * g.K[2];
     const size_t gi = pass_addr<0>(g, line, n);
-    const float gv = G[gi];
-    float2 v = R[l * N + n];
+    const auto gv = G[gi];
+    CX v = R[l * N + n];
     const double en = 0.5 * (double)gv * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
     const int mxp = n <= g.K[0] / 2 ? n : n - g.K[0];
     const int myp = my <= g.K[1] / 2 ? my : my - g.K[1];
@@ -288,7 +315,7 @@ k_fft_conv_x(float2* __restrict__ data, PassGeom g, const FftAxis* __restrict__ 
     partials[2 * blockIdx.x] = red[0][0];
     partials[2 * blockIdx.x + 1] = red[1][0];
   }
-  float2* R2 = lds_fft<+1>(R, O, nl, ax, tw);
+  CX* R2 = lds_fft<+1>(R, O, nl, ax, tw);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<0>(e, nl, N, l, n);
@@ -323,13 +350,15 @@ __device__ __forceinline__ void k_pme_finalize(const double* __restrict__ partia
 
 struct SplineArgs {
   int K[3];
-  float sc[3];  // K/L
+  float sc[3];    // K/L
+  double scd[3];  // K/L in double (FP64 option)
 };
 
 // Fractional coordinate → base index (mod K) and in-cell weight.
-__device__ __forceinline__ void frac_coord(float x, float sc, int K, int& k0, float& w) {
-  const float u = x * sc;
-  const float fl = floorf(u);
+template <class R>
+__device__ __forceinline__ void frac_coord(R x, R sc, int K, int& k0, R& w) {
+  const R u = x * sc;
+  const R fl = floor(u);
   w = u - fl;
   int k = (int)fl % K;
   if (k < 0) k += K;
@@ -357,7 +386,7 @@ __device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint3
   int k0[3];
   for (int a = 0; a < 3; ++a) {
     float w;
-    frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
+    frac_coord<float>(xs[a], S.sc[a], S.K[a], k0[a], w);
   }
   const uint32_t nb[3] = {nbx, nby, nbz};
   const uint32_t b = brick_of(k0, nb);
@@ -433,7 +462,7 @@ __device__ __forceinline__ HomeRange home_bricks(int lo, int hi, int K) {
 // stencil points inside the brick to an LDS brick (int64 LDS adds of the
 // same fixed-point values the replay sums), which is then stored once.  The
 // home bricks' charge lists are walked as one flattened index space.
-template <int ORDER>
+template <int ORDER, class R>
 __device__ __forceinline__ void
 k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t nbx,
              uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ start,
@@ -476,25 +505,25 @@ k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t
     const float4 p = xyzq[ci];
     if (p.w == 0.0f) continue;
     int k0[3];
-    float th[3][ORDER], dth[ORDER];
-    const float xs[3] = {p.x, p.y, p.z};
+    R th[3][ORDER], dth[ORDER];
+    const R xs[3] = {(R)p.x, (R)p.y, (R)p.z};
     for (int a = 0; a < 3; ++a) {
-      float w;
-      frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
-      dyn_bspline<ORDER>(w, th[a], dth);
+      R w;
+      frac_coord<R>(xs[a], std::is_same<R, float>::value ? (R)S.sc[a] : (R)S.scd[a], S.K[a], k0[a], w);
+      dyn_bspline<ORDER, R>(w, th[a], dth);
     }
     for (int jx = 0; jx < ORDER; ++jx) {
       const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
       if (gx < lo[0] || gx > hi[0]) continue;
-      const float qx = p.w * th[0][jx];
+      const R qx = (R)p.w * th[0][jx];
       for (int jy = 0; jy < ORDER; ++jy) {
         const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
         if (gy < lo[1] || gy > hi[1]) continue;
-        const float qxy = qx * th[1][jy];
+        const R qxy = qx * th[1][jy];
         for (int jz = 0; jz < ORDER; ++jz) {
           const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
           if (gz < lo[2] || gz > hi[2]) continue;
-          const float v = qxy * th[2][jz];
+          const R v = qxy * th[2][jz];
           const int li = ((gx - lo[0]) * kBrick + (gy - lo[1])) * kBrick + (gz - lo[2]);
           atomicAdd(&acc[li], (unsigned long long)dyn_fx_d((double)v, DYN_GRID_FX_SCALE));
         }
@@ -510,61 +539,77 @@ k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t
   }
 }
 
-template <int ORDER>
+template <class R>
+__device__ __forceinline__ int64_t fx_of(R f) {
+  if constexpr (std::is_same<R, float>::value) return dyn_fx(f);
+  else return dyn_fx64(f);
+}
+
+template <int ORDER, class C>
 __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
-                             const float2* __restrict__ phi, const uint32_t* __restrict__ site,
+                             const C* __restrict__ phi, const uint32_t* __restrict__ site,
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
                              float* __restrict__ fout, float fscale) {
+  using R = typename CxT<C>::R;
+  constexpr bool F32 = std::is_same<R, float>::value;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = xyzq[i];
-  float f0 = 0.f, f1 = 0.f, f2 = 0.f;
+  R f0 = R(0), f1 = R(0), f2 = R(0);
   if (p.w != 0.0f) {
     int k0[3];
-    float th[3][ORDER], dth[3][ORDER];
-    const float xs[3] = {p.x, p.y, p.z};
+    R th[3][ORDER], dth[3][ORDER];
+    const R xs[3] = {(R)p.x, (R)p.y, (R)p.z};
+    R sc[3];
     for (int a = 0; a < 3; ++a) {
-      float w;
-      frac_coord(xs[a], S.sc[a], S.K[a], k0[a], w);
-      dyn_bspline<ORDER>(w, th[a], dth[a]);
+      sc[a] = F32 ? (R)S.sc[a] : (R)S.scd[a];
+      R w;
+      frac_coord<R>(xs[a], sc[a], S.K[a], k0[a], w);
+      dyn_bspline<ORDER, R>(w, th[a], dth[a]);
     }
-    float sx = 0.f, sy = 0.f, sz = 0.f;
+    R sx = R(0), sy = R(0), sz = R(0);
     for (int jx = 0; jx < ORDER; ++jx) {
       const int gx = (k0[0] + S.K[0] - ORDER + 1 + jx) % S.K[0];
       for (int jy = 0; jy < ORDER; ++jy) {
         const int gy = (k0[1] + S.K[1] - ORDER + 1 + jy) % S.K[1];
-        const float2* row = phi + ((size_t)gx * S.K[1] + gy) * S.K[2];
-        const float txy = th[0][jx] * th[1][jy];
-        const float dxy = dth[0][jx] * th[1][jy];
-        const float tdy = th[0][jx] * dth[1][jy];
+        const C* row = phi + ((size_t)gx * S.K[1] + gy) * S.K[2];
+        const R txy = th[0][jx] * th[1][jy];
+        const R dxy = dth[0][jx] * th[1][jy];
+        const R tdy = th[0][jx] * dth[1][jy];
         for (int jz = 0; jz < ORDER; ++jz) {
           const int gz = (k0[2] + S.K[2] - ORDER + 1 + jz) % S.K[2];
-          const float ph = row[gz].x;
-          sx = dyn_fmaf(ph, dxy * th[2][jz], sx);
-          sy = dyn_fmaf(ph, tdy * th[2][jz], sy);
-          sz = dyn_fmaf(ph, txy * dth[2][jz], sz);
+          const R ph = row[gz].x;
+          if constexpr (F32) {
+            sx = dyn_fmaf(ph, dxy * th[2][jz], sx);
+            sy = dyn_fmaf(ph, tdy * th[2][jz], sy);
+            sz = dyn_fmaf(ph, txy * dth[2][jz], sz);
+          } else {
+            sx = fma(ph, dxy * th[2][jz], sx);
+            sy = fma(ph, tdy * th[2][jz], sy);
+            sz = fma(ph, txy * dth[2][jz], sz);
+          }
         }
       }
     }
-    f0 = -p.w * S.sc[0] * sx * fscale;
-    f1 = -p.w * S.sc[1] * sy * fscale;
-    f2 = -p.w * S.sc[2] * sz * fscale;
+    f0 = -(R)p.w * sc[0] * sx * (R)fscale;
+    f1 = -(R)p.w * sc[1] * sy * (R)fscale;
+    f2 = -(R)p.w * sc[2] * sz * (R)fscale;
   }
   if (site) {
     const uint32_t s = site[i];
     if (comp_major) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + s), (unsigned long long)dyn_fx(f0));
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + nsites + s), (unsigned long long)dyn_fx(f1));
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 2 * (size_t)nsites + s), (unsigned long long)dyn_fx(f2));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + s), (unsigned long long)fx_of(f0));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + nsites + s), (unsigned long long)fx_of(f1));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 2 * (size_t)nsites + s), (unsigned long long)fx_of(f2));
     } else {
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s), (unsigned long long)dyn_fx(f0));
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 1), (unsigned long long)dyn_fx(f1));
-      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 2), (unsigned long long)dyn_fx(f2));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s), (unsigned long long)fx_of(f0));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 1), (unsigned long long)fx_of(f1));
+      atomicAdd(reinterpret_cast<unsigned long long*>(fx + 3 * (size_t)s + 2), (unsigned long long)fx_of(f2));
     }
   } else {
-    fout[3 * i] = f0;
-    fout[3 * i + 1] = f1;
-    fout[3 * i + 2] = f2;
+    fout[3 * i] = (float)f0;
+    fout[3 * i + 1] = (float)f1;
+    fout[3 * i + 2] = (float)f2;
   }
 }
 
@@ -586,7 +631,8 @@ PassGeom pass_geom(const uint32_t K[3], int axis) {
 namespace dyn {
 
 void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
-                   float b, int ord) {
+                   float b, int ord, bool fp64) {
+  f64 = fp64;
   device = dev;
   stream = s;
   prof = p;
@@ -611,16 +657,30 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
   brick_start.reserve(nb + 1);
   // the binning pass counts into zeroed counts; the scan clears them again
   DYN_HIP(hipMemsetAsync(brick_count.p, 0, sizeof(uint32_t) * (nb + 1), stream));
-  cgrid.reserve(ngrid);
-  G.reserve(ngrid);
+  if (f64) {
+    cgrid64.reserve(ngrid);
+    G64.reserve(ngrid);
+  } else {
+    cgrid.reserve(ngrid);
+    G.reserve(ngrid);
+  }
   for (int a = 0; a < 3; ++a) {
     std::vector<float2> t(K[a]);
+    std::vector<double2> t64(K[a]);
     for (uint32_t m = 0; m < K[a]; ++m) {
       const double ang = -2.0 * M_PI * (double)m / (double)K[a];
       t[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+      t64[m] = make_double2(std::cos(ang), std::sin(ang));
     }
-    tw[a].reserve(K[a]);
-    DYN_HIP(hipMemcpyAsync(tw[a].p, t.data(), sizeof(float2) * K[a], hipMemcpyHostToDevice, stream));
+    if (f64) {
+      tw64[a].reserve(K[a]);
+      DYN_HIP(hipMemcpyAsync(tw64[a].p, t64.data(), sizeof(double2) * K[a], hipMemcpyHostToDevice,
+                             stream));
+    } else {
+      tw[a].reserve(K[a]);
+      DYN_HIP(hipMemcpyAsync(tw[a].p, t.data(), sizeof(float2) * K[a], hipMemcpyHostToDevice,
+                             stream));
+    }
     DYN_HIP(hipStreamSynchronize(stream));
   }
   const PassGeom gx = pass_geom(K, 0);
@@ -675,7 +735,7 @@ void PmeCore::set_box(const float l[3]) {
   }
   const double V = (double)L[0] * L[1] * L[2];
   const double b2 = (double)beta * beta;
-  std::vector<float> g(ngrid);
+  std::vector<double> g(ngrid);
   for (uint32_t mx = 0; mx < K[0]; ++mx) {
     const int mxp = mx <= K[0] / 2 ? (int)mx : (int)mx - (int)K[0];
     const double fx = mxp / (double)L[0];
@@ -690,12 +750,18 @@ void PmeCore::set_box(const float l[3]) {
         if (msq > 0)
           val = std::exp(-M_PI * M_PI * msq / b2) / (M_PI * V * msq) * bsq[0][mx] * bsq[1][my] *
                 bsq[2][mz];
-        g[((size_t)mx * K[1] + my) * K[2] + mz] = (float)val;
+        g[((size_t)mx * K[1] + my) * K[2] + mz] = val;
       }
     }
   }
-  DYN_HIP(hipMemcpyAsync(G.p, g.data(), sizeof(float) * ngrid, hipMemcpyHostToDevice, stream));
-  DYN_HIP(hipStreamSynchronize(stream));
+  if (f64) {
+    DYN_HIP(hipMemcpyAsync(G64.p, g.data(), sizeof(double) * ngrid, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+  } else {
+    std::vector<float> gf(g.begin(), g.end());
+    DYN_HIP(hipMemcpyAsync(G.p, gf.data(), sizeof(float) * ngrid, hipMemcpyHostToDevice, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+  }
 }
 
 void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
@@ -719,50 +785,57 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
 }
 
 // One SPME evaluation per system of the pass, each stage one launch for all
-// of them.  A system with a null core takes part in no stage (its packs
-// describe empty work: no charges, no bricks, no lines, no result).
+// of them (per precision and B-spline order present).  A system with a null
+// core takes part in no stage (its packs describe empty work: no charges, no
+// bricks, no lines, no result); neither does it in the launches of the other
+// precision or order.
 void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
   const uint32_t ns = R.n;
   std::vector<SplineArgs> S(ns);
   std::vector<PassGeom> gz(ns), gy(ns), gx(ns);
   unsigned g_n = 1, g_b = 1, bz = 1, by = 1, bx = 1;
   size_t shz = 0, shy = 0, shx = 0;
-  bool any = false, ord[2] = {false, false};
-  auto shm = [](const PassGeom& g, int N) { return (size_t)2 * g.L * N * sizeof(float2); };
+  bool any = false, ord[2][2] = {{false, false}, {false, false}}, prec[2] = {false, false};
+  auto shm = [](const PassGeom& g, int N, size_t elem) { return (size_t)2 * g.L * N * elem; };
   for (uint32_t i = 0; i < ns; ++i) {
     const PmeRun& r = runs[i];
-    if (!r.core) {
-      std::memset(&S[i], 0, sizeof S[i]);
-      std::memset(&gz[i], 0, sizeof gz[i]);
-      gy[i] = gx[i] = gz[i];
-      continue;
-    }
+    std::memset(&S[i], 0, sizeof S[i]);
+    std::memset(&gz[i], 0, sizeof gz[i]);
+    gy[i] = gx[i] = gz[i];
+    if (!r.core) continue;
     const PmeCore& c = *r.core;
     any = true;
-    ord[c.order == 5] = true;
+    ord[c.f64][c.order == 5] = true;
+    prec[c.f64] = true;
     for (int a = 0; a < 3; ++a) {
       S[i].K[a] = (int)c.K[a];
       S[i].sc[a] = (float)((double)c.K[a] / (double)c.L[a]);
+      S[i].scd[a] = (double)c.K[a] / (double)c.L[a];
     }
     r.core->charge_brick.reserve(std::max<uint32_t>(r.n, 1));
     r.core->brick_items.reserve(std::max<uint32_t>(r.n, 1));
     gz[i] = pass_geom(c.K, 2);
     gy[i] = pass_geom(c.K, 1);
     gx[i] = pass_geom(c.K, 0);
+    const size_t el = c.f64 ? sizeof(double2) : sizeof(float2);
     g_n = std::max(g_n, grid_for(r.n, 256));
     g_b = std::max(g_b, c.nbk[0] * c.nbk[1] * c.nbk[2]);
     bz = std::max(bz, (unsigned)((gz[i].lines + gz[i].L - 1) / gz[i].L));
     by = std::max(by, (unsigned)((gy[i].lines + gy[i].L - 1) / gy[i].L));
     bx = std::max(bx, (unsigned)((gx[i].lines + gx[i].L - 1) / gx[i].L));
-    shz = std::max(shz, shm(gz[i], (int)c.K[2]));
-    shy = std::max(shy, shm(gy[i], (int)c.K[1]));
-    shx = std::max(shx, shm(gx[i], (int)c.K[0]));
+    shz = std::max(shz, shm(gz[i], (int)c.K[2], el));
+    shy = std::max(shy, shm(gy[i], (int)c.K[1], el));
+    shx = std::max(shx, shm(gx[i], (int)c.K[0], el));
   }
   if (!any) return;
   auto on = [&](uint32_t i) { return runs[i].core != nullptr; };
   auto nq = [&](uint32_t i) { return on(i) ? runs[i].n : 0u; };
   auto nb = [&](uint32_t i, int a) { return on(i) ? runs[i].core->nbk[a] : 0u; };
   auto cp = [&](uint32_t i) { return runs[i].core; };
+  // this system in the launch of precision F64 (and order ORDER when given)
+  auto mine = [&](uint32_t i, bool f64, int order) {
+    return on(i) && cp(i)->f64 == f64 && (order == 0 || cp(i)->order == order);
+  };
   // every grid point belongs to exactly one brick and is stored by it, so
   // the grid needs no clearing
   {
@@ -783,68 +856,112 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
                                        on(i) ? cp(i)->brick_start.p : nullptr,
                                        on(i) ? cp(i)->brick_items.p : nullptr);
     });
-    auto spread_args = [&](uint32_t i, int order, auto F) {
-      const bool mine = on(i) && cp(i)->order == order;
-      return args<F.value>(runs[i].xyzq, mine ? nq(i) : 0u, S[i], mine ? nb(i, 0) : 0u,
-                           mine ? nb(i, 1) : 0u, mine ? nb(i, 2) : 0u,
-                           mine ? cp(i)->brick_start.p : nullptr,
-                           mine ? cp(i)->brick_items.p : nullptr,
-                           mine ? cp(i)->grid_fx.p : nullptr);
+    auto spread = [&](auto F, const char* name, bool f64, int order) {
+      launch<decltype(F)::value, kBrickBlock>(R, name, tag, g_b, kBrickBlock, 0, [&](uint32_t i) {
+        const bool m = mine(i, f64, order);
+        return args<decltype(F)::value>(runs[i].xyzq, m ? nq(i) : 0u, S[i], m ? nb(i, 0) : 0u,
+                                        m ? nb(i, 1) : 0u, m ? nb(i, 2) : 0u,
+                                        m ? cp(i)->brick_start.p : nullptr,
+                                        m ? cp(i)->brick_items.p : nullptr,
+                                        m ? cp(i)->grid_fx.p : nullptr);
+      });
     };
-    if (ord[0])
-      launch<k_pme_spread<4>, kBrickBlock>(R, "k_pme_spread<4>", tag, g_b, kBrickBlock, 0,
-                                           [&](uint32_t i) {
-        return spread_args(i, 4, std::integral_constant<decltype(&k_pme_spread<4>), &k_pme_spread<4>>{});
-      });
-    if (ord[1])
-      launch<k_pme_spread<5>, kBrickBlock>(R, "k_pme_spread<5>", tag, g_b, kBrickBlock, 0,
-                                           [&](uint32_t i) {
-        return spread_args(i, 5, std::integral_constant<decltype(&k_pme_spread<5>), &k_pme_spread<5>>{});
-      });
+#define DYN_SPREAD(O, T, F64)                                                                    \
+  if (ord[F64][O == 5])                                                                          \
+    spread(std::integral_constant<decltype(&k_pme_spread<O, T>), &k_pme_spread<O, T>>{},         \
+           "k_pme_spread<" #O ", " #T ">", F64, O);
+    DYN_SPREAD(4, float, 0)
+    DYN_SPREAD(5, float, 0)
+    DYN_SPREAD(4, double, 1)
+    DYN_SPREAD(5, double, 1)
+#undef DYN_SPREAD
   }
-  auto fft = [&](auto F, const char* name, const std::vector<PassGeom>& g, int axis, unsigned gx_,
-                 size_t shm_, bool from_fx) {
+  // FFT stage over the systems of one precision: C = float2 or double2
+  auto fft = [&](auto F, auto Ctag, const char* name, const std::vector<PassGeom>& g, int axis,
+                 unsigned gx_, size_t shm_, bool from_fx) {
+    using C = typename decltype(Ctag)::type;
+    constexpr bool F64 = std::is_same<C, double2>::value;
     launch<decltype(F)::value, kFftBlock>(R, name, tag, gx_, kFftBlock, shm_, [&](uint32_t i) {
-      return args<decltype(F)::value>(on(i) ? cp(i)->cgrid.p : nullptr,
-                                      from_fx && on(i) ? cp(i)->grid_fx.p : nullptr, g[i],
-                                      on(i) ? cp(i)->ax_dev.p + axis : nullptr,
-                                      on(i) ? cp(i)->tw[axis].p : nullptr);
+      const bool m = mine(i, F64, 0);
+      C* grid = nullptr;
+      const C* tw = nullptr;
+      if (m) {
+        if constexpr (F64) {
+          grid = cp(i)->cgrid64.p;
+          tw = cp(i)->tw64[axis].p;
+        } else {
+          grid = cp(i)->cgrid.p;
+          tw = cp(i)->tw[axis].p;
+        }
+      }
+      PassGeom ge = g[i];
+      if (!m) ge.lines = 0;
+      return args<decltype(F)::value>(grid, from_fx && m ? cp(i)->grid_fx.p : nullptr, ge,
+                                      m ? cp(i)->ax_dev.p + axis : nullptr, tw);
     });
   };
-  {
-    Profiler::Scope ps(prof, "pme_fft_z_fwd");
-    fft(std::integral_constant<decltype(&k_fft_pass<2, -1, true>), &k_fft_pass<2, -1, true>>{},
-        "k_fft_pass z fwd", gz, 2, bz, shz, true);
-  }
-  {
-    Profiler::Scope ps(prof, "pme_fft_y_fwd");
-    fft(std::integral_constant<decltype(&k_fft_pass<1, -1, false>), &k_fft_pass<1, -1, false>>{},
-        "k_fft_pass y fwd", gy, 1, by, shy, false);
-  }
-  {
-    Profiler::Scope ps(prof, "pme_fft_x_conv");
-    launch<k_fft_conv_x, kFftBlock>(R, "k_fft_conv_x", tag, bx, kFftBlock, shx, [&](uint32_t i) {
-      ConvArgs C;
-      std::memset(&C, 0, sizeof C);
-      if (on(i)) {
-        for (int a = 0; a < 3; ++a) C.invL[a] = 1.0f / cp(i)->L[a];
-        C.pi2_over_b2 = M_PI * M_PI / ((double)cp(i)->beta * cp(i)->beta);
+  auto conv = [&](auto F, auto Ctag, const char* name) {
+    using C = typename decltype(Ctag)::type;
+    constexpr bool F64 = std::is_same<C, double2>::value;
+    launch<decltype(F)::value, kFftBlock>(R, name, tag, bx, kFftBlock, shx, [&](uint32_t i) {
+      const bool m = mine(i, F64, 0);
+      ConvArgs Cv;
+      std::memset(&Cv, 0, sizeof Cv);
+      C* grid = nullptr;
+      const C* tw = nullptr;
+      const typename CxT<C>::R* G = nullptr;
+      PassGeom ge = gx[i];
+      if (m) {
+        for (int a = 0; a < 3; ++a) Cv.invL[a] = 1.0f / cp(i)->L[a];
+        Cv.pi2_over_b2 = M_PI * M_PI / ((double)cp(i)->beta * cp(i)->beta);
+        if constexpr (F64) {
+          grid = cp(i)->cgrid64.p;
+          tw = cp(i)->tw64[0].p;
+          G = cp(i)->G64.p;
+        } else {
+          grid = cp(i)->cgrid.p;
+          tw = cp(i)->tw[0].p;
+          G = cp(i)->G.p;
+        }
+      } else {
+        ge.lines = 0;
       }
-      return args<k_fft_conv_x>(on(i) ? cp(i)->cgrid.p : nullptr, gx[i],
-                                on(i) ? cp(i)->ax_dev.p : nullptr, on(i) ? cp(i)->tw[0].p : nullptr,
-                                on(i) ? cp(i)->G.p : nullptr, C,
-                                on(i) ? cp(i)->partials.p : nullptr);
+      return args<decltype(F)::value>(grid, ge, m ? cp(i)->ax_dev.p : nullptr, tw, G, Cv,
+                                      m ? cp(i)->partials.p : nullptr);
     });
-  }
-  {
-    Profiler::Scope ps(prof, "pme_fft_y_inv");
-    fft(std::integral_constant<decltype(&k_fft_pass<1, 1, false>), &k_fft_pass<1, 1, false>>{},
-        "k_fft_pass y inv", gy, 1, by, shy, false);
-  }
-  {
-    Profiler::Scope ps(prof, "pme_fft_z_inv");
-    fft(std::integral_constant<decltype(&k_fft_pass<2, 1, false>), &k_fft_pass<2, 1, false>>{},
-        "k_fft_pass z inv", gz, 2, bz, shz, false);
+  };
+#define DYN_IC(F) std::integral_constant<decltype(&F), &F>{}
+  struct TF {
+    using type = float2;
+  };
+  struct TD {
+    using type = double2;
+  };
+  for (int pr = 0; pr < 2; ++pr) {
+    if (!prec[pr]) continue;
+    if (pr == 0) {
+      { Profiler::Scope ps(prof, "pme_fft_z_fwd");
+        fft(DYN_IC((k_fft_pass<2, -1, true, float2>)), TF{}, "k_fft_pass z fwd", gz, 2, bz, shz, true); }
+      { Profiler::Scope ps(prof, "pme_fft_y_fwd");
+        fft(DYN_IC((k_fft_pass<1, -1, false, float2>)), TF{}, "k_fft_pass y fwd", gy, 1, by, shy, false); }
+      { Profiler::Scope ps(prof, "pme_fft_x_conv");
+        conv(DYN_IC(k_fft_conv_x<float2>), TF{}, "k_fft_conv_x"); }
+      { Profiler::Scope ps(prof, "pme_fft_y_inv");
+        fft(DYN_IC((k_fft_pass<1, 1, false, float2>)), TF{}, "k_fft_pass y inv", gy, 1, by, shy, false); }
+      { Profiler::Scope ps(prof, "pme_fft_z_inv");
+        fft(DYN_IC((k_fft_pass<2, 1, false, float2>)), TF{}, "k_fft_pass z inv", gz, 2, bz, shz, false); }
+    } else {
+      { Profiler::Scope ps(prof, "pme_fft_z_fwd");
+        fft(DYN_IC((k_fft_pass<2, -1, true, double2>)), TD{}, "k_fft_pass z fwd f64", gz, 2, bz, shz, true); }
+      { Profiler::Scope ps(prof, "pme_fft_y_fwd");
+        fft(DYN_IC((k_fft_pass<1, -1, false, double2>)), TD{}, "k_fft_pass y fwd f64", gy, 1, by, shy, false); }
+      { Profiler::Scope ps(prof, "pme_fft_x_conv");
+        conv(DYN_IC(k_fft_conv_x<double2>), TD{}, "k_fft_conv_x f64"); }
+      { Profiler::Scope ps(prof, "pme_fft_y_inv");
+        fft(DYN_IC((k_fft_pass<1, 1, false, double2>)), TD{}, "k_fft_pass y inv f64", gy, 1, by, shy, false); }
+      { Profiler::Scope ps(prof, "pme_fft_z_inv");
+        fft(DYN_IC((k_fft_pass<2, 1, false, double2>)), TD{}, "k_fft_pass z inv f64", gz, 2, bz, shz, false); }
+    }
   }
   {
     Profiler::Scope ps(prof, "pme_finalize");
@@ -856,23 +973,29 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
   }
   {
     Profiler::Scope ps(prof, "pme_gather");
-    auto gather_args = [&](uint32_t i, int order, auto F) {
-      const bool mine = on(i) && cp(i)->order == order;
-      return args<F.value>(runs[i].xyzq, mine ? nq(i) : 0u, S[i],
-                           mine ? (const float2*)cp(i)->cgrid.p : nullptr, runs[i].site,
-                           runs[i].fx, runs[i].comp_major, runs[i].nsites, runs[i].f_out,
-                           runs[i].fscale);
-    };
     const unsigned gg = std::max(1u, (g_n * 256 + 127) / 128);
-    if (ord[0])
-      launch<k_pme_gather<4>, 128>(R, "k_pme_gather<4>", tag, gg, 128, 0, [&](uint32_t i) {
-        return gather_args(i, 4, std::integral_constant<decltype(&k_pme_gather<4>), &k_pme_gather<4>>{});
+    auto gather = [&](auto F, auto Ctag, int order) {
+      using C = typename decltype(Ctag)::type;
+      constexpr bool F64 = std::is_same<C, double2>::value;
+      launch<decltype(F)::value, 128>(R, "k_pme_gather", tag * 4 + (F64 ? 2 : 0) + (order == 5),
+                                      gg, 128, 0, [&](uint32_t i) {
+        const bool m = mine(i, F64, order);
+        const C* phi = nullptr;
+        if (m) {
+          if constexpr (F64) phi = cp(i)->cgrid64.p;
+          else phi = cp(i)->cgrid.p;
+        }
+        return args<decltype(F)::value>(runs[i].xyzq, m ? nq(i) : 0u, S[i], phi, runs[i].site,
+                                        runs[i].fx, runs[i].comp_major, runs[i].nsites,
+                                        runs[i].f_out, runs[i].fscale);
       });
-    if (ord[1])
-      launch<k_pme_gather<5>, 128>(R, "k_pme_gather<5>", tag, gg, 128, 0, [&](uint32_t i) {
-        return gather_args(i, 5, std::integral_constant<decltype(&k_pme_gather<5>), &k_pme_gather<5>>{});
-      });
+    };
+    if (ord[0][0]) gather(DYN_IC((k_pme_gather<4, float2>)), TF{}, 4);
+    if (ord[0][1]) gather(DYN_IC((k_pme_gather<5, float2>)), TF{}, 5);
+    if (ord[1][0]) gather(DYN_IC((k_pme_gather<4, double2>)), TD{}, 4);
+    if (ord[1][1]) gather(DYN_IC((k_pme_gather<5, double2>)), TD{}, 5);
   }
+#undef DYN_IC
 }
 
 }  // namespace dyn
@@ -910,7 +1033,7 @@ extern "C" int dyn_pme_create(dyn_pme** out, int device, uint32_t nx, uint32_t n
     p->prof.stream = p->stream;
     const uint32_t K[3] = {nx, ny, nz};
     const float L[3] = {lx, ly, lz};
-    p->core.init(device, p->stream, &p->prof, K, L, alpha, (int)order);
+    p->core.init(device, p->stream, &p->prof, K, L, alpha, (int)order, false);
     *out = p.release();
     return DYN_OK;
   } catch (const Error& e) {

@@ -173,7 +173,16 @@ typedef struct dyn_engine_params {
   uint32_t order;      /* B-spline order, 4 or 5                             */
   uint32_t mode;       /* DYN_MODE_REFERENCE or DYN_MODE_STANDARD            */
   float list_buffer;   /* extra list radius (Å) for list reuse across calls  */
+  uint32_t precision;  /* 0: FP32 arithmetic; DYN_PREC_* bits select FP64   */
 } dyn_engine_params;
+
+/* Precision options (the precision sweep; the default FP32 path is the one
+ * the reference's kernels run): FP64 pair arithmetic in the pair kernel
+ * (no alchemical decoupling), and FP64 SPME (B-spline weights, FFT grid,
+ * influence function and gather in double; the spread grid keeps its 2^-32
+ * fixed point).  Forces accumulate in the same fixed point either way. */
+#define DYN_PREC_PAIR_F64 1u
+#define DYN_PREC_PME_F64 2u
 
 /* Topology: n_dyn solute atoms (charges, LJ type per atom, n_types (σ, ε)
  * pairs combined by Lorentz–Berthelot), n_water OPC waters (parameters of
