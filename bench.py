@@ -44,13 +44,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall time of the CPU-baseline sample (0 skips it)")
-    ap.add_argument("--mode", choices=["md", "eval", "copy", "batch-lambda", "batch-uneven"],
+    ap.add_argument("--mode", choices=["md", "eval", "copy", "batch-lambda", "batch-uneven",
+                                       "precision"],
                     default="md",
                     help="md: device-resident MD steps; eval: nonbonded evaluations only; "
                          "copy: a plain 256 MiB device copy, three times (the known-size "
                          "kernel that scales the HBM byte counters); batch-lambda: 16 "
                          "alchemical λ-windows of the solvated chain in one batch; "
-                         "batch-uneven: 128 small solvated boxes of uneven sizes in one batch")
+                         "batch-uneven: 128 small solvated boxes of uneven sizes in one batch; "
+                         "precision: the 128 A water box with the pair kernel and SPME each in "
+                         "FP32 and FP64, errors against the FP64 oracle")
     ap.add_argument("--systems", type=int, default=None,
                     help="batch modes: number of systems (16 windows / 128 boxes)")
     ap.add_argument("--relax-steps", type=int, default=400,
@@ -80,6 +83,8 @@ def main() -> int:
         return run_batch_lambda(args)
     if args.mode == "batch-uneven":
         return run_batch_uneven(args)
+    if args.mode == "precision":
+        return run_precision(args)
 
     import numpy as np
     from dynamics_amd import api, systems
@@ -430,6 +435,66 @@ def run_batch_uneven(args) -> int:
     for e, m, _ in made:
         m.close()
         e.close()
+    print(json.dumps(out))
+    return 0
+
+
+def run_precision(args) -> int:
+    """The 128 Å water box (69,900 OPC waters, 279,600 sites; rc 9 Å, SPME 128³
+    order 4): one nonbonded evaluation per step with the pair kernel and the
+    SPME each in FP32 or FP64; force and energy errors against the FP64
+    oracle of the same Hamiltonian, and the time per evaluation."""
+    import numpy as np
+    from dynamics_amd import api, systems
+    from oracle import fp64
+
+    s = systems.config2()
+    K = (128, 128, 128)
+    beta = float(np.float32(0.3123))
+    ref = fp64.reference_energy_and_forces(s, 9.0, 9.0, beta, K, 4)
+    print("bench: FP64 oracle done", file=sys.stderr, flush=True)
+    nrm = float(np.linalg.norm(ref["forces"]))
+    rows = []
+    for prec, label in ((0, "pair f32, spme f32"), (api.PREC_PAIR_F64, "pair f64, spme f32"),
+                        (api.PREC_PME_F64, "pair f32, spme f64"),
+                        (api.PREC_PAIR_F64 | api.PREC_PME_F64, "pair f64, spme f64")):
+        eng = api.Engine(s, cutoff=9.0, lj_cutoff=9.0, beta=beta, grid=K, order=4,
+                         mode=api.MODE_REFERENCE, list_buffer=1.0, precision=prec)
+        eng.set_positions(s)
+        eng.compute(api.COMPUTE_ALL | api.COMPUTE_REBUILD)
+        F = eng.site_forces()
+        E = eng.energies()
+        err = float(np.linalg.norm(F - ref["forces"]) / nrm)
+        e_err = abs(E["total"] - ref["e_total"]) / (abs(ref["e_lj"]) + abs(ref["e_coul"]) +
+                                                     abs(ref["e_recip"]) + abs(ref["e_self"]))
+        for _ in range(args.warmup):
+            eng.compute(api.COMPUTE_ALL | api.COMPUTE_ASYNC)
+        eng.profile(True)
+        eng.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.compute(api.COMPUTE_ALL | api.COMPUTE_ASYNC)
+        eng.sync()
+        t = (time.perf_counter() - t0) / args.steps
+        prof = eng.profile_read()
+        eng.close()
+        rows.append({"precision": label, "force_rel_rms_vs_fp64": err, "energy_rel_err": e_err,
+                     "ms_per_eval": t * 1e3,
+                     "pairs_ms": prof.get("pairs", (0.0, 1))[0] / max(prof.get("pairs", (0, 1))[1], 1),
+                     "spme_ms": sum(v[0] for k, v in prof.items()
+                                    if k.startswith("pme_")) / args.steps})
+        print(f"bench: {label}: {err:.2e}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "force/energy error vs FP64 and time per evaluation by precision, 1 GPU",
+        "value": rows[0]["force_rel_rms_vs_fp64"], "unit": "relative RMS force error (FP32 path)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": rows[0]["ms_per_eval"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": "opc69900_box128: 69,900 OPC waters (279,600 sites), rc 9 A, "
+                               "SPME 128^3 order 4, reference mode, one evaluation per step "
+                               "with a 1 A list buffer"},
+        "precision_sweep": rows,
+    }
     print(json.dumps(out))
     return 0
 
