@@ -133,6 +133,8 @@ _sig("dyn_engine_set_positions", C.c_int, [_p, _p, _p, _p, _p, _p])
 _sig("dyn_engine_set_positions_device", C.c_int, [_p, C.c_void_p])
 _sig("dyn_engine_compute", C.c_int, [_p, C.c_uint32])
 _sig("dyn_engine_set_alchemical", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_double])
+_sig("dyn_engine_set_molecules", C.c_int, [_p, C.c_uint32, _p])
+_sig("dyn_engine_get_mol_energies", C.c_int, [_p, _f64p, C.c_uint32])
 _sig("dyn_md_create", C.c_int, [C.POINTER(_p), _p, C.POINTER(MdParams), _p,
                                 C.c_uint32, _p, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p,
                                 C.c_uint32, _p, _p])
@@ -174,7 +176,7 @@ EXPORTED = [
     "dyn_pme_get_fixed_grid", "dyn_pme_grid_n",
     "dyn_engine_create", "dyn_engine_destroy", "dyn_engine_last_error", "dyn_engine_set_box",
     "dyn_engine_set_positions", "dyn_engine_set_positions_device", "dyn_engine_compute",
-    "dyn_engine_set_alchemical",
+    "dyn_engine_set_alchemical", "dyn_engine_set_molecules", "dyn_engine_get_mol_energies",
     "dyn_md_create", "dyn_md_destroy", "dyn_md_last_error", "dyn_md_set_state",
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
@@ -407,6 +409,21 @@ class Engine:
     def set_alchemical(self, first, count, lam):
         """Decouple solute atoms [first, first + count) at λ = lam (count 0: off)."""
         _check(lib.dyn_engine_set_alchemical(self.h, int(first), int(count), float(lam)), self._err)
+
+    def set_molecules(self, mol_start):
+        """Molecule-pair energy matrix over the solute cut at mol_start (first
+        atom of each molecule; empty: off)."""
+        ms = _c(mol_start, np.uint32)
+        self.n_mol = int(ms.size)
+        _check(lib.dyn_engine_set_molecules(self.h, self.n_mol, _ptr(ms) if ms.size else None),
+               self._err)
+
+    def mol_energies(self):
+        """(n_mol, n_mol) solute molecule-pair energies of the last compute, kcal/mol."""
+        n = getattr(self, "n_mol", 0)
+        out = np.zeros((max(n, 1), max(n, 1)), np.float64)
+        _check(lib.dyn_engine_get_mol_energies(self.h, out, n), self._err)
+        return out
 
     def sync(self):
         _check(lib.dyn_engine_sync(self.h), self._err)

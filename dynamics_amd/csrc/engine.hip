@@ -609,6 +609,9 @@ struct PairArgs {
   const float2* lj_table;    // (c6, c12) per type pair
   const float2* lj_se;       // (σ, ε) per type pair, for soft-core
   const double2* lj_table64; // (c6, c12) in double, for the FP64 option
+  int64_t* mol_e;            // n_mol² molecule-pair energies (fixed point), or null
+  const int32_t* mol_of;     // per solute atom: its molecule
+  uint32_t n_mol;
   int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
@@ -625,6 +628,45 @@ struct TileAccT {
   uint32_t n_ss, n_lj, n_c;  // wave-uniform counts (popcounts)
 };
 
+__device__ __forceinline__ void wave_add_fx(int64_t* __restrict__ M, int key, double e);
+
+// Molecule-pair energy of one tile entry (src/non_bonded.rs:350-361): the
+// pair energy of every solute–solute lane goes to M[m_i·n + m_j] in 2⁻³²
+// kcal/mol fixed point.  The full list visits each pair from both rows, so
+// each visit adds its own direction and a diagonal entry takes half.  A wave
+// whose live lanes share one entry adds it once (the common case: one
+// molecule, or one molecule pair per tile); otherwise lane by lane.
+__device__ __forceinline__ void mol_energy_add(int64_t* __restrict__ M,
+                                               const int32_t* __restrict__ mol_of, uint32_t n_mol,
+                                               int gi, int gj, double e) {
+  int key = -1;
+  if (e != 0.0) {
+    const int mi = mol_of[-2 - gi], mj = mol_of[-2 - gj];  // solute slots: gid = −2 − site
+    key = mi * (int)n_mol + mj;
+    if (mi == mj) e *= 0.5;
+  }
+  wave_add_fx(M, key, e);
+}
+
+// Adds e to M[key] (2⁻³² fixed point) for every lane with key ≥ 0: one
+// atomic for the wave when all such lanes share the key.  All lanes of the
+// wave must be active.
+__device__ __forceinline__ void wave_add_fx(int64_t* __restrict__ M, int key, double e) {
+  const unsigned long long live = __ballot(key >= 0);
+  if (!live) return;
+  const int kf = __shfl(key, __ffsll((long long)live) - 1, kWave);
+  if (__ballot(key >= 0 && key != kf) == 0ull) {
+    double x = key >= 0 ? e : 0.0;
+    for (int m = 1; m < kWave; m <<= 1) x += __shfl_xor(x, m, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(M + kf),
+                (unsigned long long)dyn_fx64(x));
+  } else if (key >= 0) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(M + key),
+              (unsigned long long)dyn_fx64(e));
+  }
+}
+
 // ALCH: the group holds at least one lane across the decoupled molecule's
 // boundary (the alchemical variant returns the plain result on the others).
 // ENERGY: accumulate energies, virial, dH/dλ and pair counts (forces only
@@ -633,7 +675,8 @@ struct TileAccT {
 template <int KIND, bool ALCH, bool ENERGY, bool F64>
 __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4], int ti,
                                            const float4& pjv, int tj, bool active,
-                                           TileAccT<std::conditional_t<F64, double, float>>& T) {
+                                           TileAccT<std::conditional_t<F64, double, float>>& T,
+                                           int gi, int gj) {
   using R = std::conditional_t<F64, double, float>;
   const float pj[4] = {pjv.x, pjv.y, pjv.z, pjv.w};
   R d[3], s, elj, ec, r2;
@@ -662,6 +705,8 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
     else T.e_c_w += ec;
   }
   T.vir += s * r2;
+  if constexpr (KIND == DYN_KIND_SS)
+    if (A.mol_e) mol_energy_add(A.mol_e, A.mol_of, A.n_mol, gi, gj, (double)elj + (double)ec);
   const uint32_t h = (uint32_t)__popcll(__ballot(hit != 0));
   if (KIND == DYN_KIND_SS) T.n_ss += h;
   else if (KIND == DYN_KIND_SO || KIND == DYN_KIND_OO) T.n_lj += h;
@@ -723,19 +768,19 @@ k_eng_pairs(PairArgs A) {
       const bool alch = !F64 && A.P.alch_on && kind <= DYN_KIND_SQ &&
                         __ballot(((ti ^ tjc) & DYN_TYPE_ALCH) != 0) != 0ull;
       switch (kind) {
-        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T); break;
-        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T); break;
+        case DYN_KIND_QQ: tile_entry<DYN_KIND_QQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc); break;
+        case DYN_KIND_OO: tile_entry<DYN_KIND_OO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc); break;
         case DYN_KIND_SS:
-          if (alch) tile_entry<DYN_KIND_SS, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SS, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SS, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
+          else tile_entry<DYN_KIND_SS, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
           break;
         case DYN_KIND_SO:
-          if (alch) tile_entry<DYN_KIND_SO, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SO, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
+          else tile_entry<DYN_KIND_SO, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
           break;
         case DYN_KIND_SQ:
-          if (alch) tile_entry<DYN_KIND_SQ, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
-          else tile_entry<DYN_KIND_SQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T);
+          if (alch) tile_entry<DYN_KIND_SQ, true, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
+          else tile_entry<DYN_KIND_SQ, false, ENERGY, F64>(A, pi, ti, pjc, tjc, active, T, gi, gjc);
           break;
         default: break;
       }
@@ -799,6 +844,9 @@ struct CorrArgs {
   int mode;
   int recip_on;            // reciprocal corrections only accompany the SPME sum
   uint32_t nblk;           // workgroups (one partial each)
+  int64_t* mol_e;          // n_mol² molecule-pair energies, or null
+  const int32_t* mol_of;
+  uint32_t n_mol;
   int64_t* fx;             // site forces, component-major
   uint32_t n_sites;
   double* partials;        // 6 per block: e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ
@@ -812,6 +860,10 @@ k_eng_corr(CorrArgs A) {
   if (blockIdx.x >= A.nblk) return;  // partials exist for the system's own blocks
   // e_lj14, e_c14, e_corr, vir_sr, vir_recip, dH/dλ, 1-4 pairs within the cutoff
   double acc[kCorrVals] = {0, 0, 0, 0, 0, 0, 0};
+  // molecule-pair matrix: a running entry per thread (1-4 pairs lie within
+  // one molecule), flushed when the entry changes and once at the end
+  int mkey = -1;
+  double me = 0.0;
   for (uint32_t k = blockIdx.x * kCorrBlock + threadIdx.x; k < A.n; k += A.nblk * kCorrBlock) {
     if (A.kind[k] == 0 && !A.recip_on) continue;
     const uint32_t i = A.ci[k], j = A.cj[k];
@@ -824,6 +876,7 @@ k_eng_corr(CorrArgs A) {
     const float rinv = dyn_rsqrtf(r2);
     const float qq = a.w * b.w;
     float s_sr = 0.f, s_rc = 0.f;
+    double e_pair = 0.0;  // the 1-4 pair's energy (molecule-pair matrix)
     if (A.kind[k] == 1) {
       if (r2 < A.P.rc2) acc[6] += 1.0;
       // a 1-4 pair across the decoupled molecule's boundary is also an
@@ -843,6 +896,7 @@ k_eng_corr(CorrArgs A) {
         }
         s_sr += DYN_SCALE_LJ_14 * sl;
         acc[0] += (double)(DYN_SCALE_LJ_14 * el);
+        e_pair += (double)(DYN_SCALE_LJ_14 * el);
       }
       if (r2 < A.P.rc2) {
         float sc, ec;
@@ -856,6 +910,7 @@ k_eng_corr(CorrArgs A) {
         }
         s_sr += sc;
         acc[1] += (double)ec;
+        e_pair += (double)ec;
       }
       if (!A.recip_on) {
         // no reciprocal sum, nothing to correct
@@ -876,6 +931,24 @@ k_eng_corr(CorrArgs A) {
     }
     acc[3] += (double)(s_sr * r2);
     acc[4] += (double)(s_rc * r2);
+    if (A.mol_e && e_pair != 0.0) {
+      const int mi = A.mol_of[i], mj = A.mol_of[j];
+      if (mi != mj) {  // both orders (src/non_bonded.rs:350-361)
+        const uint64_t v = (uint64_t)dyn_fx64(e_pair);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.mol_e + mi * (int)A.n_mol + mj), v);
+        atomicAdd(reinterpret_cast<unsigned long long*>(A.mol_e + mj * (int)A.n_mol + mi), v);
+      } else {
+        const int key = mi * (int)A.n_mol + mi;
+        if (key != mkey) {
+          if (mkey >= 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(A.mol_e + mkey),
+                      (unsigned long long)dyn_fx64(me));
+          mkey = key;
+          me = 0.0;
+        }
+        me += e_pair;
+      }
+    }
     const float s = s_sr + s_rc;
     const int64_t q0 = dyn_fx(s * d0), q1 = dyn_fx(s * d1), q2 = dyn_fx(s * d2);
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + i), (unsigned long long)q0);
@@ -885,6 +958,7 @@ k_eng_corr(CorrArgs A) {
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + A.n_sites + j), (unsigned long long)(-q1));
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_sites + j), (unsigned long long)(-q2));
   }
+  if (A.mol_e) wave_add_fx(A.mol_e, mkey, me);
   for (int v = 0; v < kCorrVals; ++v) {
     red[threadIdx.x] = acc[v];
     __syncthreads();
@@ -905,8 +979,11 @@ __device__ __forceinline__ void k_eng_site_prep(const float4* __restrict__ wpos,
                                 const uint32_t* __restrict__ qsite, uint32_t nq,
                                 float4* __restrict__ pme_xyzq, float4* __restrict__ pme_b,
                                 uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site,
-                                int64_t* __restrict__ fx_recip) {
+                                int64_t* __restrict__ fx_recip, int64_t* __restrict__ mol_e,
+                                uint32_t n_mol2) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mol_e)
+    for (uint32_t k = s; k < n_mol2; k += gridDim.x * blockDim.x) mol_e[k] = 0;
   if (s < n_sites) {
     fx_site[s] = 0;
     fx_site[(size_t)n_sites + s] = 0;
@@ -1105,6 +1182,12 @@ struct dyn_engine {
   DevBuf<uint32_t> qsite;
   uint32_t n_q = 0;
   float c6_oo = 0.f, c12_oo = 0.f;
+  // molecule-pair energy matrix (n_mol² fixed-point sums; n_mol = 0: off)
+  uint32_t n_mol = 0;
+  bool mol_valid = false;  // mol_e holds the last compute's matrix
+  DevBuf<int32_t> mol_of;
+  DevBuf<int64_t> mol_e;
+  PinnedBuf<int64_t> h_mol;
   // positions
   DevBuf<float4> pos_own;
   const float4* pos_dev = nullptr;
@@ -1552,7 +1635,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       return args<k_eng_site_prep>(e.wpos.p, e.n_sites, e.qsite.p, e.n_q,
                                    pl.recip_on ? e.pme_xyzq.p : (float4*)nullptr,
                                    pl.recip_b ? e.pme_b.p : (float4*)nullptr, e.alch0, e.alch1,
-                                   e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr);
+                                   e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr,
+                                   energy_on && e.n_mol ? e.mol_e.p : (int64_t*)nullptr,
+                                   e.n_mol * e.n_mol);
     });
   }
   if (short_on) {
@@ -1575,6 +1660,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       A.lj_table = e.lj_table.p;
       A.lj_se = e.lj_se.p;
       A.lj_table64 = e.lj_table64.p;
+      A.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
+      A.mol_of = e.mol_of.p;
+      A.n_mol = e.n_mol;
       A.fitem = e.fitem.p;
       A.partials = e.pair_part.p;
       A.cap_chunks = e.cap_chunks;
@@ -1622,6 +1710,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       C.fx = e.fx_site.p;
       C.n_sites = e.n_sites;
       C.partials = e.corr_part.p;
+      C.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
+      C.mol_of = e.mol_of.p;
+      C.n_mol = e.n_mol;
       return args<k_eng_corr>(C);
     });
   }
@@ -1688,6 +1779,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     else if (!PL[i].reuse) e.recip_kept = false;
     e.computed = true;
     e.energies_valid = energy_on;
+    e.mol_valid = energy_on && e.n_mol;
     e.fx_fresh = false;
   }
 }
@@ -2045,6 +2137,62 @@ extern "C" int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32
       eng->list_valid = false;  // slot types are refreshed by a rebuild
     }
     eng->lambda = lambda;
+  });
+}
+
+extern "C" int dyn_engine_set_molecules(dyn_engine* eng, uint32_t n_mol,
+                                        const uint32_t* mol_start) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DeviceGuard g(eng->device);
+    DYN_REQUIRE(n_mol <= DYN_MAX_MOLECULES, "%u molecules (at most %u)", n_mol,
+                (unsigned)DYN_MAX_MOLECULES);
+    DYN_REQUIRE(n_mol == 0 || mol_start, "mol_start is null");
+    // the reference's checks (src/lib.rs:1522-1566)
+    if (n_mol) {
+      DYN_REQUIRE(eng->n_dyn > 0, "molecules given, but there are no solute atoms");
+      DYN_REQUIRE(mol_start[0] == 0, "first molecule starts at atom %u, but it must start at 0",
+                  mol_start[0]);
+      for (uint32_t m = 1; m < n_mol; ++m) {
+        DYN_REQUIRE(mol_start[m] < eng->n_dyn, "molecule %u starts at atom %u, but valid atom "
+                    "indices are 0..%u", m, mol_start[m], eng->n_dyn - 1);
+        DYN_REQUIRE(mol_start[m] > mol_start[m - 1], "molecule starts must be strictly "
+                    "increasing; molecule %u starts at %u after %u", m, mol_start[m],
+                    mol_start[m - 1]);
+      }
+    }
+    DYN_HIP(hipStreamSynchronize(eng->stream));  // no evaluation still reads the old arrays
+    eng->n_mol = n_mol;
+    eng->mol_valid = false;
+    if (n_mol) {
+      std::vector<int32_t> of(eng->n_dyn);
+      for (uint32_t m = 0; m < n_mol; ++m) {
+        const uint32_t end = m + 1 < n_mol ? mol_start[m + 1] : eng->n_dyn;
+        for (uint32_t a = mol_start[m]; a < end; ++a) of[a] = (int32_t)m;
+      }
+      eng->mol_of.reserve(eng->n_dyn);
+      eng->mol_e.reserve((size_t)n_mol * n_mol);
+      eng->h_mol.reserve((size_t)n_mol * n_mol);
+      DYN_HIP(hipMemcpy(eng->mol_of.p, of.data(), sizeof(int32_t) * eng->n_dyn,
+                        hipMemcpyHostToDevice));
+    }
+  });
+}
+
+extern "C" int dyn_engine_get_mol_energies(dyn_engine* eng, double* out, uint32_t n_mol) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DeviceGuard g(eng->device);
+    DYN_REQUIRE(eng->n_mol, "no molecules set (dyn_engine_set_molecules)");
+    DYN_REQUIRE(out && n_mol == eng->n_mol, "need the %u × %u matrix", eng->n_mol, eng->n_mol);
+    if (!eng->mol_valid)
+      throw Error(DYN_ERR_STATE, "no compute with energies since the molecules were set");
+    eng->check_status();
+    const size_t nn = (size_t)n_mol * n_mol;
+    DYN_HIP(hipMemcpyAsync(eng->h_mol.p, eng->mol_e.p, sizeof(int64_t) * nn,
+                           hipMemcpyDeviceToHost, eng->stream));
+    DYN_HIP(hipStreamSynchronize(eng->stream));
+    for (size_t k = 0; k < nn; ++k) out[k] = (double)eng->h_mol.p[k] * DYN_FX_INV_SCALE;
   });
 }
 

@@ -283,6 +283,23 @@ typedef struct dyn_energies {
 
 int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out);
 
+/* Molecule-pair energies, as the reference's energy_between_mols
+ * (src/non_bonded.rs:350-361, summed into potential_energy_between_mols at
+ * :529-533; its CUDA path returns none, src/gpu_interface.rs:629).  The
+ * solute is cut into n_mol molecules: mol_start[m] is the first solute atom
+ * of molecule m, starting at 0 and strictly increasing (the reference's
+ * checks, src/lib.rs:1522-1566); n_mol = 0 switches the matrix off.  Every
+ * compute with energies then also sums the short-range pair energy (LJ +
+ * erfc Coulomb, 1-4 pairs scaled) of each solute–solute pair into
+ * M[m_i·n_mol + m_j] and, for m_i ≠ m_j, M[m_j·n_mol + m_i] — a pair
+ * inside one molecule counts once on the diagonal.  Sums are 2⁻³² kcal/mol
+ * fixed point, independent of the order of the additions. */
+#define DYN_MAX_MOLECULES 4096u
+int dyn_engine_set_molecules(dyn_engine* eng, uint32_t n_mol, const uint32_t* mol_start);
+/* out[n_mol·n_mol] (kcal/mol, row-major) of the last compute, which must have
+ * computed energies after the molecules were set. */
+int dyn_engine_get_mol_energies(dyn_engine* eng, double* out, uint32_t n_mol);
+
 /* Diagnostics (no counterpart in the reference): copies of the current
  * cluster layout and list for host replay checks.  Sizes come first: call
  * with null arrays to read n_slots, n_clusters, n_entries (groups of 8 j

@@ -134,7 +134,7 @@ static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slo
     const int gj = null_j ? DYN_GID_PAD : slot_gid[js];
     bool active = dyn_slot_pair_valid(slot_gid[I * DYN_CL_SIZE + ia], gj);
     if (KIND == DYN_KIND_SS) active = active && ((mask >> l) & 1ull);
-    float d[3], s, elj, ec, r2, dh;
+    float d[3], s, elj, ec, r2, dh = 0.f;
     int hit;
     // the alchemical variant equals the plain one on every lane that does
     // not cross the decoupled molecule's boundary

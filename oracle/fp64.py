@@ -21,6 +21,13 @@
                      vacuum Coulomb within 1 %, F = −dE/dx).
   ewald_recip_fp64   direct reciprocal-space Ewald sum (independent of any
                      grid), with forces and the k-space virial.
+  mol_energies       (pair_sum_fp64 with mol_start) the molecule-pair energy
+                     matrix of solute–solute pairs (src/non_bonded.rs:
+                     350-361).  Pinned by separability: for molecules A, B
+                     the diagonal M_AA is the pair energy of A alone and
+                     M_AB = E(A ∪ B) − E(A) − E(B), each E from the sum over
+                     a sub-system holding only those atoms
+                     (tests/test_oracle_mol_energies.py).
   corr_14_vacuum     the reference's force-only 1-4 correction
                      (src/non_bonded.rs:763-790).
   erf_pair_fp64      qq·erf(βr)/r pair term for exclusion corrections.
@@ -88,12 +95,16 @@ def schedule(lam):
     return 0.0, 0.0, 2.0 * lam - 1.0, 2.0
 
 
-def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True, alch=None, lam=0.0):
+def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True, alch=None, lam=0.0,
+                  mol_start=None):
     """Forces (n_sites, 3) in site order (solute, then per water O, M, H0, H1)
     and energies of all pairs inside the cutoff.  alch (bool per solute atom)
     marks a decoupled molecule: pairs with exactly one atom in it take
     soft-core LJ and scaled Coulomb at λ = lam (src/non_bonded.rs:300-345,
-    :883-941), and "dh_dl" is returned."""
+    :883-941), and "dh_dl" is returned.  mol_start (first solute atom of
+    each molecule) adds "mol_energies": M[m_i, m_j] sums the energy of every
+    solute–solute pair, both orders for m_i ≠ m_j, once on the diagonal
+    (src/non_bonded.rs:350-361)."""
     lj_rc = rc if lj_rc is None else lj_rc
     pos, q, sig, eps, mol, code = site_view(system)
     box = system.box.astype(F64)
@@ -164,7 +175,21 @@ def pair_sum_fp64(system, rc=10.0, lj_rc=None, beta=0.3123, include_14=True, alc
     np.add.at(F, j, -fvec)
     solute = (i < nd) | (j < nd)
     interacting = lj_on | c_on
+    extra = {}
+    if mol_start is not None:
+        starts = np.asarray(mol_start, np.int64)
+        nm = len(starts)
+        M = np.zeros((nm, nm))
+        ss = (i < nd) & (j < nd)
+        mi = np.searchsorted(starts, i[ss], side="right") - 1
+        mj = np.searchsorted(starts, j[ss], side="right") - 1
+        e = (e_lj + e_c)[ss]
+        np.add.at(M, (mi, mj), e)
+        off = mi != mj
+        np.add.at(M, (mj[off], mi[off]), e[off])
+        extra["mol_energies"] = M
     return {
+        **extra,
         "forces": F,
         "e_lj": float(e_lj.sum()), "e_coul": float(e_c.sum()),
         "e_lj_solute": float(e_lj[solute].sum()), "e_coul_solute": float(e_c[solute].sum()),
