@@ -63,19 +63,24 @@ class NbOpts(C.Structure):
 
 class MdParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("temperature", C.c_float), ("gamma", C.c_float),
-                ("shake_tol", C.c_float), ("seed", C.c_uint64), ("spme_ratio", C.c_uint32)]
+                ("shake_tol", C.c_float), ("seed", C.c_uint64), ("spme_ratio", C.c_uint32),
+                ("nstpcouple", C.c_uint32), ("pressure_target", C.c_float), ("tau_p", C.c_float),
+                ("compressibility", C.c_double)]
 
 
 class MdEnergies(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("kinetic_rigid", C.c_double), ("bond", C.c_double),
                 ("angle", C.c_double), ("dihedral", C.c_double), ("temperature", C.c_double),
-                ("dof", C.c_double), ("steps", C.c_uint64)]
+                ("dof", C.c_double), ("steps", C.c_uint64), ("pressure", C.c_double),
+                ("virial", C.c_double), ("kinetic_com", C.c_double), ("volume", C.c_double),
+                ("box", C.c_double * 3), ("pcouple_steps", C.c_uint64)]
 
 
 class EngineParams(C.Structure):
     _fields_ = [("cutoff", C.c_float), ("lj_cutoff", C.c_float), ("beta", C.c_float),
                 ("grid", C.c_uint32 * 3), ("order", C.c_uint32), ("mode", C.c_uint32),
-                ("list_buffer", C.c_float), ("precision", C.c_uint32)]
+                ("list_buffer", C.c_float), ("precision", C.c_uint32),
+                ("mesh_spacing", C.c_float)]
 
 
 class Energies(C.Structure):
@@ -134,6 +139,7 @@ _sig("dyn_engine_set_positions_device", C.c_int, [_p, C.c_void_p])
 _sig("dyn_engine_compute", C.c_int, [_p, C.c_uint32])
 _sig("dyn_engine_set_alchemical", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_double])
 _sig("dyn_engine_set_molecules", C.c_int, [_p, C.c_uint32, _p])
+_sig("dyn_engine_get_pme_grid", C.c_int, [_p, C.POINTER(C.c_uint32)])
 _sig("dyn_engine_get_mol_energies", C.c_int, [_p, _f64p, C.c_uint32])
 _sig("dyn_md_create", C.c_int, [C.POINTER(_p), _p, C.POINTER(MdParams), _p,
                                 C.c_uint32, _p, _p, C.c_uint32, _p, _p, C.c_uint32, _p, _p,
@@ -177,6 +183,7 @@ EXPORTED = [
     "dyn_engine_create", "dyn_engine_destroy", "dyn_engine_last_error", "dyn_engine_set_box",
     "dyn_engine_set_positions", "dyn_engine_set_positions_device", "dyn_engine_compute",
     "dyn_engine_set_alchemical", "dyn_engine_set_molecules", "dyn_engine_get_mol_energies",
+    "dyn_engine_get_pme_grid",
     "dyn_md_create", "dyn_md_destroy", "dyn_md_last_error", "dyn_md_set_state",
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
@@ -360,12 +367,14 @@ class Engine:
     """Native nonbonded engine over positions + topology."""
 
     def __init__(self, system, cutoff=10.0, lj_cutoff=None, beta=0.3123, grid=(0, 0, 0),
-                 order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0, precision=0):
-        """precision: PREC_PAIR_F64 | PREC_PME_F64 bits (0: FP32)."""
+                 order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0, precision=0,
+                 mesh_spacing=0.0):
+        """precision: PREC_PAIR_F64 | PREC_PME_F64 bits (0: FP32).  mesh_spacing
+        > 0 (with grid 0): the SPME grid follows the box (regen_pme)."""
         s = system
         prm = EngineParams(cutoff, cutoff if lj_cutoff is None else lj_cutoff, beta,
                            (C.c_uint32 * 3)(*[int(g) for g in grid]), order, mode, list_buffer,
-                           precision)
+                           precision, mesh_spacing)
         self.h = C.c_void_p()
         q = _c(s.q_dyn, np.float32)
         t = _c(s.type_dyn, np.int32)
@@ -409,6 +418,11 @@ class Engine:
     def set_alchemical(self, first, count, lam):
         """Decouple solute atoms [first, first + count) at λ = lam (count 0: off)."""
         _check(lib.dyn_engine_set_alchemical(self.h, int(first), int(count), float(lam)), self._err)
+
+    def pme_grid(self):
+        out = (C.c_uint32 * 3)()
+        _check(lib.dyn_engine_get_pme_grid(self.h, out), self._err)
+        return tuple(int(v) for v in out)
 
     def set_molecules(self, mol_start):
         """Molecule-pair energy matrix over the solute cut at mol_start (first
@@ -496,10 +510,14 @@ class Md:
     and forces stay on the GPU between steps."""
 
     def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0,
-                 spme_ratio=1):
+                 spme_ratio=1, nstpcouple=0, pressure_target=1.0, tau_p=5.0,
+                 compressibility=4.5e-5):
+        """nstpcouple > 0 couples the pressure (C-rescale) every nstpcouple
+        steps to pressure_target bar with time constant tau_p ps."""
         self.engine = engine
         self.h = C.c_void_p()
-        p = MdParams(dt, temperature, gamma, shake_tol, seed, spme_ratio)
+        p = MdParams(dt, temperature, gamma, shake_tol, seed, spme_ratio, nstpcouple,
+                     pressure_target, tau_p, compressibility)
         arrs = [_c(ff.mass, np.float32),
                 _c(ff.bonds, np.uint32), _c(ff.bond_k_r0, np.float32),
                 _c(ff.angles, np.uint32), _c(ff.angle_k_t0, np.float32),
@@ -545,7 +563,9 @@ class Md:
     def energies(self):
         e = MdEnergies()
         _check(lib.dyn_md_get_energies(self.h, C.byref(e)), self._err)
-        return {k: getattr(e, k) for k, _ in MdEnergies._fields_}
+        d = {k: getattr(e, k) for k, _ in MdEnergies._fields_}
+        d["box"] = tuple(float(v) for v in e.box)
+        return d
 
 
 class Batch:

@@ -1851,6 +1851,9 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_REQUIRE((p.precision & ~(DYN_PREC_PAIR_F64 | DYN_PREC_PME_F64)) == 0,
                 "unknown precision bits 0x%x", p.precision);
     DYN_REQUIRE(p.list_buffer >= 0.f && p.list_buffer < 10.f, "list_buffer must be in [0, 10)");
+    DYN_REQUIRE(p.mesh_spacing >= 0.f && std::isfinite(p.mesh_spacing), "bad mesh_spacing");
+    DYN_REQUIRE(p.mesh_spacing == 0.f || (p.grid[0] == 0 && p.grid[1] == 0 && p.grid[2] == 0),
+                "give either a PME grid or a mesh spacing");
     DYN_REQUIRE(n_dyn + 4ull * n_water > 0, "empty system");
     DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 30), "system too large");
     DYN_REQUIRE(n_dyn == 0 || (q_dyn && type_dyn), "null solute arrays");
@@ -2014,7 +2017,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->own_disp.init(1);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_b.reserve(std::max<uint32_t>(E->n_q, 1));
-    E->pme_on = p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0;
+    E->pme_on = (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0) || p.mesh_spacing > 0.f;
     DYN_HIP(hipMemsetAsync(E->counters.p, 0, 32, E->stream));
     DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
     DYN_HIP(hipMemsetAsync(E->status.p, 0, 4, E->stream));
@@ -2060,15 +2063,22 @@ extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
       DYN_REQUIRE(eng->prm.cutoff + eng->prm.list_buffer < 0.5f * box[a],
                   "cutoff + list_buffer must be below half the box (minimum image)");
     }
+    uint32_t K[3] = {eng->prm.grid[0], eng->prm.grid[1], eng->prm.grid[2]};
+    if (eng->prm.mesh_spacing > 0.f &&
+        dyn_pme_grid_n(box[0], box[1], box[2], eng->prm.mesh_spacing, K) != DYN_OK)
+      throw Error(DYN_ERR_ARG, dyn_global_last_error());
     for (int a = 0; a < 3; ++a) eng->box[a] = box[a];
     eng->box_set = true;
     eng->recip_kept = false;
     eng->set_geometry();
     if (eng->pme_on) {
-      if (eng->pme.ngrid == 0)
-        eng->pme.init(eng->device, eng->stream, &eng->prof, eng->prm.grid, eng->box,
-                      eng->prm.beta, (int)eng->prm.order,
-                      (eng->prm.precision & DYN_PREC_PME_F64) != 0);
+      // a grid of another size is planned afresh (regen_pme); the same grid
+      // only takes the new influence function
+      const bool same = eng->pme.ngrid != 0 && K[0] == eng->pme.K[0] && K[1] == eng->pme.K[1] &&
+                        K[2] == eng->pme.K[2];
+      if (!same)
+        eng->pme.init(eng->device, eng->stream, &eng->prof, K, eng->box, eng->prm.beta,
+                      (int)eng->prm.order, (eng->prm.precision & DYN_PREC_PME_F64) != 0);
       else
         eng->pme.set_box(eng->box);
     }
@@ -2108,6 +2118,14 @@ extern "C" int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyz
     DYN_REQUIRE(((uintptr_t)xyzw_device & 15) == 0, "device positions must be 16-byte aligned");
     eng->pos_dev = reinterpret_cast<const float4*>(xyzw_device);
     eng->pos_set = true;
+  });
+}
+
+extern "C" int dyn_engine_get_pme_grid(dyn_engine* eng, uint32_t out[3]) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    DYN_REQUIRE(out, "out is null");
+    for (int a = 0; a < 3; ++a) out[a] = eng->pme_on ? eng->pme.K[a] : 0u;
   });
 }
 

@@ -12,6 +12,7 @@
 //   Langevin velocity update    src/thermostat.rs:308-362
 //   SHAKE / RATTLE              src/bonded.rs:141-304
 //   bonded terms                src/bonded_forces.rs:11-200
+//   pressure, C-rescale         src/barostat.rs:324-413, :419-431
 // Units as the reference: Å, ps, amu, kcal/mol; accelerations use
 // KCAL_TO_NATIVE = 418.4 (src/lib.rs:196).
 #include <hip/hip_runtime.h>
@@ -88,6 +89,16 @@ __device__ __forceinline__ float normal_dev(uint64_t seed, uint64_t step, uint64
   const float u2 = ((float)((h >> 16) & 0xFFFFFFull) + 0.5f) * (1.0f / 16777216.0f);
   return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
 }
+
+// The same deviate in double on the host (the barostat's noise).
+inline double normal_host(uint64_t seed, uint64_t step, uint64_t idx) {
+  const uint64_t h = mix64(mix64(mix64(seed) + step) + idx);
+  const double u1 = ((double)(h >> 40) + 0.5) * (1.0 / 16777216.0);
+  const double u2 = ((double)((h >> 16) & 0xFFFFFFull) + 0.5) * (1.0 / 16777216.0);
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+// Stream index of the barostat's deviate (beyond every site's 3 indices).
+constexpr uint64_t kBarostatIndex = 0xB0B0000000000000ull;
 
 struct Fx {
   const int64_t* nb;  // engine forces, site order, component-major
@@ -336,7 +347,7 @@ struct BondedArgs {
   const uint32_t* dihe;   // 4 per term
   const float4* dihe_p;   // k, periodicity, phase, unused
   int64_t* f;             // fixed point, component-major over solute atoms
-  double* partials;       // 3 per block: bond, angle, dihedral energy
+  double* partials;       // 4 per block: bond, angle, dihedral energy, virial
   uint32_t nblk;          // workgroups of the system (one partial each)
 };
 
@@ -348,12 +359,14 @@ __device__ __forceinline__ void add_f(int64_t* f, uint32_t nd, uint32_t i, V3 v)
 
 // Harmonic bonds and angles (U = k(r − r0)², k(θ − θ0)²) and periodic
 // dihedrals (U = k(1 + cos(nφ − γ)), φ the IUPAC torsion angle); one
-// thread per term, fixed-point force sums.
+// thread per term, fixed-point force sums.  The virial Σ r_a·F_a of each
+// term is taken relative to one of its atoms (src/bonded.rs:41-46, :79-83,
+// :124-130; the same sign convention for every term).
 __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
-  __shared__ double red[3][kBlock];
+  __shared__ double red[4][kBlock];
   if (blockIdx.x >= A.nblk) return;  // partials exist for the system's own blocks
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  double e[3] = {0.0, 0.0, 0.0};
+  double e[4] = {0.0, 0.0, 0.0, 0.0};
   const uint32_t nd = A.n_dyn;
   if (t < A.n_bond) {
     const uint32_t i = A.bond[2 * t], j = A.bond[2 * t + 1];
@@ -365,6 +378,7 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
     add_f(A.f, nd, i, fi);
     add_f(A.f, nd, j, fi * -1.0f);
     e[0] = (double)(p.x * dr * dr);
+    e[3] = -(double)dot(d, fi);  // (r_i − r_j)·F_i
   } else if (t - A.n_bond < A.n_angle) {
     const uint32_t q = t - A.n_bond;
     const uint32_t i = A.angle[3 * q], j = A.angle[3 * q + 1], k = A.angle[3 * q + 2];
@@ -387,6 +401,7 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
       add_f(A.f, nd, k, f2);
       add_f(A.f, nd, j, (f0 + f2) * -1.0f);
       e[1] = (double)(p.x * dth * dth);
+      e[3] = (double)dot(b0, f0) + (double)dot(b2, f2);
     }
   } else if (t - A.n_bond - A.n_angle < A.n_dihe) {
     const uint32_t q = t - A.n_bond - A.n_angle;
@@ -414,17 +429,20 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
       add_f(A.f, nd, k, gk * -dedphi);
       add_f(A.f, nd, l, gl * -dedphi);
       e[2] = (double)(p.x * (1.0f + cosf(arg)));
+      // relative to atom j: r_i − r_j = −b1, r_k − r_j = b2, r_l − r_j = b2 + b3
+      e[3] = -(double)dot(b1, gi * -dedphi) + (double)dot(b2, gk * -dedphi) +
+             (double)dot(b2 + b3, gl * -dedphi);
     }
   }
-  for (int c = 0; c < 3; ++c) red[c][threadIdx.x] = e[c];
+  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = e[c];
   __syncthreads();
   for (int s = kBlock / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s)
-      for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int c = 0; c < 3; ++c) A.partials[3 * blockIdx.x + c] = red[c][0];
+    for (int c = 0; c < 4; ++c) A.partials[4 * blockIdx.x + c] = red[c][0];
 }
 
 __device__ __forceinline__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
@@ -506,7 +524,7 @@ k_md_energies(const float4* __restrict__ x, const float4* __restrict__ v,
     a[4] += 0.5 * (double)kMassW * (vc[0] * vc[0] + vc[1] * vc[1] + vc[2] * vc[2]) + krot;
   }
   for (uint32_t r = t; r < n_bblk; r += 1024)
-    for (int c = 0; c < 3; ++c) a[1 + c] += bpart[3 * r + c];
+    for (int c = 0; c < 3; ++c) a[1 + c] += bpart[4 * r + c];
   for (int c = 0; c < 5; ++c) red[c][t] = a[c];
   __syncthreads();
   for (int s2 = 512; s2 > 0; s2 >>= 1) {
@@ -515,6 +533,161 @@ k_md_energies(const float4* __restrict__ x, const float4* __restrict__ v,
     __syncthreads();
   }
   if (t < 5) out[t] = red[t][0];
+}
+
+// Pressure groups: every water (O, H0, H1; M is a virtual site), every
+// cluster of solute atoms joined by constraints, every other solute atom
+// alone.  Forces inside a group (the rigid water's, the constraints') cancel
+// in the group virial, so no constraint virial is needed.
+struct GroupArgs {
+  const uint32_t* grp_ptr;    // solute groups: range of grp_atoms
+  const uint32_t* grp_atoms;
+  const float* mass;
+  uint32_t n_sgroups, n_water, n_dyn;
+  Box B;
+};
+
+__device__ __forceinline__ void force_of_d(const Fx& F, uint32_t s, double f[3]) {
+  for (int c = 0; c < 3; ++c) {
+    int64_t v = F.nb[(size_t)c * F.n_sites + s];
+    if (s < F.n_dyn) v += F.bd[(size_t)c * F.n_dyn + s];
+    f[c] = (double)v * DYN_FX_INV_SCALE;
+  }
+}
+
+// Members of group g: site indices and masses; returns the member count.
+__device__ __forceinline__ int group_sites(const GroupArgs& G, uint32_t g,
+                                           uint32_t site[kMaxClusterCons + 1],
+                                           float mass[kMaxClusterCons + 1]) {
+  if (g < G.n_sgroups) {
+    const uint32_t a0 = G.grp_ptr[g], a1 = G.grp_ptr[g + 1];
+    int n = 0;
+    for (uint32_t a = a0; a < a1 && n <= kMaxClusterCons; ++a, ++n) {
+      site[n] = G.grp_atoms[a];
+      mass[n] = G.mass[site[n]];
+    }
+    return n;
+  }
+  const uint32_t base = G.n_dyn + 4u * (g - G.n_sgroups);
+  site[0] = base;
+  site[1] = base + 2;
+  site[2] = base + 3;
+  mass[0] = kMassO;
+  mass[1] = mass[2] = kMassH;
+  return 3;
+}
+
+// Mass-weighted mean of the members' offsets from the first member
+// (minimum image): the centre of mass is x[site[0]] + dr.
+__device__ __forceinline__ double group_com(const GroupArgs& G, const float4* __restrict__ x,
+                                            const uint32_t* site, const float* mass, int n,
+                                            double dr[3]) {
+  const float4 ref = x[site[0]];
+  double M = 0.0;
+  dr[0] = dr[1] = dr[2] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const float4 p = x[site[k]];
+    const V3 d = min_image(v3(p.x - ref.x, p.y - ref.y, p.z - ref.z), G.B);
+    M += (double)mass[k];
+    dr[0] += (double)mass[k] * d.x;
+    dr[1] += (double)mass[k] * d.y;
+    dr[2] += (double)mass[k] * d.z;
+  }
+  for (int c = 0; c < 3; ++c) dr[c] /= M;
+  return M;
+}
+
+// Per group: kinetic energy of the centre of mass ½M|V|² (amu Å²/ps²) and
+// the intra-group virial Σ (r_a − R)·F_a (kcal/mol; a water's M-site force
+// projected onto O and the H sites as the kick does); one partial pair per
+// workgroup.
+__device__ __forceinline__ void k_md_press(GroupArgs G, const float4* __restrict__ x,
+                                           const float4* __restrict__ v, Fx F,
+                                           double* __restrict__ partials, uint32_t nblk) {
+  __shared__ double red[2][kBlock];
+  if (blockIdx.x >= nblk) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  double kin = 0.0, cvir = 0.0;
+  if (g < G.n_sgroups + G.n_water) {
+    uint32_t site[kMaxClusterCons + 1];
+    float m[kMaxClusterCons + 1];
+    const int n = group_sites(G, g, site, m);
+    double dr[3];
+    const double M = group_com(G, x, site, m, n, dr);
+    double pv[3] = {0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+      const float4 u = v[site[k]];
+      pv[0] += (double)m[k] * u.x;
+      pv[1] += (double)m[k] * u.y;
+      pv[2] += (double)m[k] * u.z;
+    }
+    kin = 0.5 * (pv[0] * pv[0] + pv[1] * pv[1] + pv[2] * pv[2]) / M;
+    if (n > 1) {
+      const bool water = g >= G.n_sgroups;
+      double fm[3] = {0, 0, 0};
+      if (water) force_of_d(F, site[0] + 1, fm);
+      const float4 ref = x[site[0]];
+      for (int k = 0; k < n; ++k) {
+        const float4 p = x[site[k]];
+        const V3 d = min_image(v3(p.x - ref.x, p.y - ref.y, p.z - ref.z), G.B);
+        double f[3];
+        force_of_d(F, site[k], f);
+        if (water) {
+          const double c = k == 0 ? (double)kCO : (double)kCH;
+          for (int e = 0; e < 3; ++e) f[e] += c * fm[e];
+        }
+        cvir += ((double)d.x - dr[0]) * f[0] + ((double)d.y - dr[1]) * f[1] +
+                ((double)d.z - dr[2]) * f[2];
+      }
+    }
+  }
+  red[0][threadIdx.x] = kin;
+  red[1][threadIdx.x] = cvir;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = red[0][0];
+    partials[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+// Isotropic rescaling by μ about the origin: every group's centre of mass
+// moves to μR (its members keep their relative positions; a water's M site
+// moves with it) and every velocity is multiplied by 1/μ.  dm = μ − 1.
+__device__ __forceinline__ void k_md_rescale(GroupArgs G, float4* __restrict__ x,
+                                             float4* __restrict__ v, float dm, float inv_mu) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G.n_sgroups + G.n_water) return;
+  uint32_t site[kMaxClusterCons + 1];
+  float m[kMaxClusterCons + 1];
+  const int n = group_sites(G, g, site, m);
+  double dr[3];
+  (void)group_com(G, x, site, m, n, dr);
+  const float4 ref = x[site[0]];
+  const float sx = dm * (float)((double)ref.x + dr[0]);
+  const float sy = dm * (float)((double)ref.y + dr[1]);
+  const float sz = dm * (float)((double)ref.z + dr[2]);
+  const bool water = g >= G.n_sgroups;
+  const int nm = water ? 4 : n;
+  for (int k = 0; k < nm; ++k) {
+    // water: O, M, H0, H1 (M moves with the molecule)
+    const uint32_t s = water ? G.n_dyn + 4u * (g - G.n_sgroups) + (uint32_t)k : site[k];
+    float4 p = x[s], u = v[s];
+    p.x += sx;
+    p.y += sy;
+    p.z += sz;
+    u.x *= inv_mu;
+    u.y *= inv_mu;
+    u.z *= inv_mu;
+    x[s] = p;
+    v[s] = u;
+  }
 }
 
 }  // namespace
@@ -556,6 +729,15 @@ struct dyn_md {
   DevBuf<uint64_t> step_dev;  // the step index, as the Langevin kernel reads it
   uint64_t n_eval = 0;  // force evaluations since set_state (SPME cadence)
   bool state_set = false, forces_valid = false;
+  // pressure groups (solute constraint clusters and lone atoms; waters
+  // follow) and the pressure of the last barostat step
+  uint32_t n_sgroups = 0;
+  DevBuf<uint32_t> grp_ptr, grp_atoms;
+  unsigned pblocks = 1;
+  DevBuf<double> ppart;
+  PinnedBuf<double> h_ppart, h_bpart;
+  double pressure = std::nan(""), virial = std::nan(""), kin_com = std::nan("");
+  uint64_t n_pcouple = 0;  // barostat rescalings so far
   ArgCache own_cache;   // a pass over this MD alone
   bool in_batch = false;
 };
@@ -710,6 +892,114 @@ void md_constrain_velocities(MdSet& S, int tag) {
   });
 }
 
+GroupArgs group_args(const dyn_md& M) {
+  GroupArgs G;
+  std::memset(&G, 0, sizeof G);
+  G.grp_ptr = M.grp_ptr.p;
+  G.grp_atoms = M.grp_atoms.p;
+  G.mass = M.mass.p;
+  G.n_sgroups = M.n_sgroups;
+  G.n_water = M.n_water;
+  G.n_dyn = M.n_dyn;
+  for (int a = 0; a < 3; ++a) G.B.L[a] = M.box[a];
+  return G;
+}
+
+// Enqueue the pressure parts of every MD of the set (the forces of the last
+// evaluation, the current velocities) and their copies to the host.
+void md_pressure_enqueue(MdSet& S) {
+  Profiler::Scope ps(*S.prof, "md_pressure");
+  unsigned g = 1;
+  for (dyn_md* m : S.m) g = std::max(g, m->pblocks);
+  launch<k_md_press, kBlock>(S.R, "k_md_press", 0, g, kBlock, 0, [&](uint32_t i) {
+    dyn_md& m = *S.m[i];
+    Fx F;
+    F.nb = engine_forces_device(m.eng);
+    F.bd = m.fb.p;
+    F.n_sites = m.n_sites;
+    F.n_dyn = m.n_dyn;
+    return args<k_md_press>(group_args(m), (const float4*)m.x.p, (const float4*)m.v.p, F,
+                            m.ppart.p, m.pblocks);
+  });
+  for (dyn_md* m : S.m) {
+    DYN_HIP(hipMemcpyAsync(m->h_ppart.p, m->ppart.p, sizeof(double) * 2 * m->pblocks,
+                           hipMemcpyDeviceToHost, S.R.stream));
+    DYN_HIP(hipMemcpyAsync(m->h_bpart.p, m->bpart.p, sizeof(double) * 4 * m->bblocks,
+                           hipMemcpyDeviceToHost, S.R.stream));
+  }
+}
+
+constexpr double kBarPerKcalMolA3 = 69476.95457055373;  // src/barostat.rs:22
+constexpr double kKbBarA3 = 138.0649;                   // k_B in bar·Å³/K, src/barostat.rs:26
+
+// Pressure (bar) from the copies md_pressure_enqueue made, after the wait:
+// P = (2 K_com + W)/(3V) over the pressure groups (src/barostat.rs:419-431
+// with the group virial W = W_nonbonded + W_bonded − Σ_g Σ_a (r_a − R_g)·F_a).
+double md_pressure_finish(dyn_md& m) {
+  double kin = 0.0, cvir = 0.0, wb = 0.0;
+  for (unsigned b = 0; b < m.pblocks; ++b) {
+    kin += m.h_ppart.p[2 * b];
+    cvir += m.h_ppart.p[2 * b + 1];
+  }
+  if (m.n_bond + m.n_angle + m.n_dihe)
+    for (unsigned b = 0; b < m.bblocks; ++b) wb += m.h_bpart.p[4 * b + 3];
+  dyn_energies E{};
+  engine_call(dyn_engine_get_energies(m.eng, &E), m.eng);
+  const double V = (double)m.box[0] * m.box[1] * m.box[2];
+  m.kin_com = kin / (double)kKcalToNative;
+  m.virial = E.virial_sr + E.virial_recip + wb - cvir;
+  m.pressure = (2.0 * m.kin_com + m.virial) / (3.0 * V) * kBarPerKcalMolA3;
+  return m.pressure;
+}
+
+// C-rescale length factor μ (Barostat::scale_factor, src/barostat.rs:324-351):
+// ΔlnV = (κ/τ)(P − P₀)Δt + √(2κ k_B T Δt/(τV))·ξ, clamped to ±0.1, μ = e^{ΔlnV/3};
+// Δt = nstpcouple·dt, ξ the step's deviate of the counter-based stream.
+double c_rescale_mu(const dyn_md& m, double p_bar, uint64_t step) {
+  const dyn_md_params& P = m.prm;
+  const double dt = (double)P.dt * P.nstpcouple;
+  const double V = (double)m.box[0] * m.box[1] * m.box[2];
+  const double det = (P.compressibility / (double)P.tau_p) * (p_bar - (double)P.pressure_target) * dt;
+  const double sig = std::sqrt(2.0 * P.compressibility * kKbBarA3 * (double)P.temperature * dt /
+                               ((double)P.tau_p * V));
+  const double xi = normal_host(P.seed, step, kBarostatIndex);
+  const double dlnv = std::min(0.1, std::max(-0.1, det + sig * xi));
+  return std::exp(dlnv / 3.0);
+}
+
+// Pressure coupling of every MD of the set at the end of a step's force
+// evaluation (energies on): measure, wait, rescale positions, velocities and
+// box (the engine takes the new box: a list rebuild and, with a mesh
+// spacing, a grid re-plan at the next evaluation — regen_pme).
+void md_barostat(MdSet& S) {
+  md_pressure_enqueue(S);
+  engine_set_check(S.es);  // waits; a list that overflowed throws (the call replays)
+  std::vector<double> mu(S.m.size(), 1.0);
+  for (size_t i = 0; i < S.m.size(); ++i) {
+    dyn_md& m = *S.m[i];
+    const double p = md_pressure_finish(m);
+    if (std::isfinite(p)) mu[i] = c_rescale_mu(m, p, m.step);
+    if (!(std::isfinite(mu[i]) && mu[i] > 0.0) || std::fabs(mu[i] - 1.0) < 1e-12) mu[i] = 1.0;
+  }
+  Profiler::Scope ps(*S.prof, "md_rescale");
+  unsigned g = 1;
+  for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_sgroups + m->n_water, kBlock));
+  launch<k_md_rescale>(S.R, "k_md_rescale", 0, g, kBlock, 0, [&](uint32_t i) {
+    dyn_md& m = *S.m[i];
+    return args<k_md_rescale>(group_args(m), m.x.p, m.v.p, (float)(mu[i] - 1.0),
+                              (float)(1.0 / mu[i]));
+  });
+  for (size_t i = 0; i < S.m.size(); ++i) {
+    dyn_md& m = *S.m[i];
+    if (mu[i] == 1.0) continue;
+    float nb[3];
+    for (int a = 0; a < 3; ++a) nb[a] = (float)((double)m.box[a] * mu[i]);
+    engine_call(dyn_engine_set_box(m.eng, nb), m.eng);
+    for (int a = 0; a < 3; ++a) m.box[a] = nb[a];
+    ++m.n_pcouple;
+  }
+}
+
 // One step of every MD of the set, as src/integrate.rs:100-190 (γ > 0,
 // Langevin-middle) or :191-260 (γ = 0, velocity Verlet without thermostat).
 // Step fractions are in units of each system's dt; the scheme (Langevin or
@@ -718,6 +1008,11 @@ void md_one_step(MdSet& S, bool energies) {
   bool any_invalid = false;
   for (dyn_md* m : S.m) any_invalid = any_invalid || !m->forces_valid;
   if (any_invalid) md_forces(S, false);
+  // pressure coupling after this step's force evaluation (the set shares
+  // nstpcouple and the step index)
+  const uint32_t nst = S.m[0]->prm.nstpcouple;
+  const bool couple = nst > 0 && (S.m[0]->step + 1) % nst == 0;
+  energies = energies || couple;
   if (S.m[0]->prm.gamma > 0.f) {
     md_integrate(S, 0, 0.5f, 0.5f, true, true);
     md_constrain_positions(S, 0);
@@ -735,14 +1030,19 @@ void md_one_step(MdSet& S, bool energies) {
     md_constrain_velocities(S, 1);
     md_integrate(S, 1, 0.f, 0.5f, false, true);
     md_constrain_positions(S, 1);
+    md_forces(S, energies);
+    if (couple) md_barostat(S);  // before the last half-kick (src/integrate.rs:172-192)
+    md_integrate(S, 2, 0.5f, 0.f, true, false);
+    md_constrain_velocities(S, 2);
   } else {
     md_integrate(S, 0, 0.5f, 1.0f, true, true);
     md_constrain_positions(S, 0);
     md_constrain_velocities(S, 0);
+    md_forces(S, energies);
+    md_integrate(S, 2, 0.5f, 0.f, true, false);
+    md_constrain_velocities(S, 2);
+    if (couple) md_barostat(S);  // at the end of the step (src/integrate.rs:241-310)
   }
-  md_forces(S, energies);
-  md_integrate(S, 2, 0.5f, 0.f, true, false);
-  md_constrain_velocities(S, 2);
   for (dyn_md* m : S.m) ++m->step;
 }
 
@@ -754,7 +1054,8 @@ void md_one_step(MdSet& S, bool energies) {
 void md_run(MdSet& S, uint32_t n) {
   if (!n) return;
   struct Ck {
-    uint64_t step, eval;
+    uint64_t step, eval, pcouple;
+    float box[3];
   };
   std::vector<Ck> ck(S.m.size());
   for (size_t i = 0; i < S.m.size(); ++i) {
@@ -762,13 +1063,14 @@ void md_run(MdSet& S, uint32_t n) {
     const size_t bytes = 16 * (size_t)m.n_sites;
     DYN_HIP(hipMemcpyAsync(m.x_ck.p, m.x.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
     DYN_HIP(hipMemcpyAsync(m.v_ck.p, m.v.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
-    ck[i] = {m.step, m.n_eval};
+    ck[i] = {m.step, m.n_eval, m.n_pcouple, {m.box[0], m.box[1], m.box[2]}};
   }
   for (int attempt = 0;; ++attempt) {
-    const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t k = 0; k < n; ++k) md_one_step(S, k + 1 == n);
-    const auto t1 = std::chrono::steady_clock::now();
     try {
+      const auto t0 = std::chrono::steady_clock::now();
+      // a barostat step waits mid-call, and may find an overflow there
+      for (uint32_t k = 0; k < n; ++k) md_one_step(S, k + 1 == n);
+      const auto t1 = std::chrono::steady_clock::now();
       engine_set_check(S.es);
       const auto t2 = std::chrono::steady_clock::now();
       S.prof->add_host("md_issue_steps", std::chrono::duration<double, std::milli>(t1 - t0).count());
@@ -785,6 +1087,11 @@ void md_run(MdSet& S, uint32_t n) {
       m.step = ck[i].step;
       m.n_eval = ck[i].eval;
       m.forces_valid = false;
+      if (m.n_pcouple != ck[i].pcouple) {  // the box of the start
+        engine_call(dyn_engine_set_box(m.eng, ck[i].box), m.eng);
+        for (int a = 0; a < 3; ++a) m.box[a] = ck[i].box[a];
+        m.n_pcouple = ck[i].pcouple;
+      }
       DYN_HIP(hipMemcpyAsync(m.step_dev.p, &ck[i].step, 8, hipMemcpyHostToDevice, S.R.stream));
       DYN_HIP(hipStreamSynchronize(S.R.stream));
     }
@@ -828,6 +1135,12 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     DYN_REQUIRE(p->temperature >= 0.f && std::isfinite(p->temperature), "bad temperature");
     DYN_REQUIRE(p->gamma >= 0.f && std::isfinite(p->gamma), "gamma must be ≥ 0");
     DYN_REQUIRE(p->shake_tol > 0.f, "shake_tol must be > 0");
+    if (p->nstpcouple) {
+      DYN_REQUIRE(p->tau_p > 0.f && std::isfinite(p->tau_p), "tau_p must be > 0");
+      DYN_REQUIRE(p->compressibility > 0.0 && std::isfinite(p->compressibility),
+                  "compressibility must be > 0");
+      DYN_REQUIRE(std::isfinite(p->pressure_target), "bad pressure_target");
+    }
     DYN_REQUIRE(p->spme_ratio <= 64, "spme_ratio must be at most 64");
     std::unique_ptr<dyn_md> M(new dyn_md());
     M->eng = eng;
@@ -889,8 +1202,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
                           dihedral_k_n_phase[3 * k + 2], 0.f);
     up(M->dihe_p, dp.data(), n_dihedrals, 16);
     M->bblocks = grid_for(std::max<uint32_t>(n_bonds + n_angles + n_dihedrals, 1), kBlock);
-    M->bpart.reserve(3 * (size_t)M->bblocks);
-    DYN_HIP(hipMemsetAsync(M->bpart.p, 0, sizeof(double) * 3 * M->bblocks, M->stream));
+    M->bpart.reserve(4 * (size_t)M->bblocks);
+    DYN_HIP(hipMemsetAsync(M->bpart.p, 0, sizeof(double) * 4 * M->bblocks, M->stream));
     M->fb.reserve(3 * (size_t)std::max(nd, 1u));
     // constraint clusters: connected components of the constraint graph
     std::vector<uint32_t> parent(nd);
@@ -930,6 +1243,33 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     }
     M->n_clusters = (uint32_t)groups.size();
     M->n_cons = n_constraints;
+    // pressure groups: the atoms of each constraint cluster, then every atom
+    // in no constraint alone
+    {
+      std::vector<uint32_t> gptr(1, 0), gat;
+      std::vector<std::vector<uint32_t>> members(groups.size());
+      std::vector<char> in_cons(nd, 0);
+      for (uint32_t k = 0; k < n_constraints; ++k)
+        in_cons[constraint_ij[2 * k]] = in_cons[constraint_ij[2 * k + 1]] = 1;
+      for (uint32_t a = 0; a < nd; ++a)
+        if (in_cons[a]) members[gid[find(a)]].push_back(a);
+      for (auto& mem : members) {
+        gat.insert(gat.end(), mem.begin(), mem.end());
+        gptr.push_back((uint32_t)gat.size());
+      }
+      for (uint32_t a = 0; a < nd; ++a)
+        if (!in_cons[a]) {
+          gat.push_back(a);
+          gptr.push_back((uint32_t)gat.size());
+        }
+      M->n_sgroups = (uint32_t)gptr.size() - 1;
+      up(M->grp_ptr, gptr.data(), gptr.size(), 4);
+      up(M->grp_atoms, gat.data(), gat.size(), 4);
+      M->pblocks = grid_for(std::max<uint32_t>(M->n_sgroups + nw, 1), kBlock);
+      M->ppart.reserve(2 * (size_t)M->pblocks);
+      M->h_ppart.reserve(2 * (size_t)M->pblocks);
+      M->h_bpart.reserve(4 * (size_t)M->bblocks);
+    }
     up(M->cl_ptr, ptr.data(), ptr.size(), 4);
     up(M->cons_ij, ij.data(), ij.size(), 8);
     up(M->cons_r0sq, r0sq.data(), r0sq.size(), 4);
@@ -1078,6 +1418,22 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     E.dof = dof;
     E.temperature = dof > 0 ? 2.0 * md->h_out.p[4] / (dof * (double)kKb) : 0.0;
     E.steps = md->step;
+    // pressure of the current velocities and the last evaluation's forces
+    md_pressure_enqueue(S);
+    engine_call(dyn_engine_sync(md->eng), md->eng);
+    DYN_HIP(hipStreamSynchronize(md->stream));
+    try {
+      E.pressure = md_pressure_finish(*md);
+    } catch (const Error& e) {
+      if (e.code != DYN_ERR_STATE) throw;  // the last evaluation skipped energies
+      md->pressure = md->virial = md->kin_com = std::nan("");
+      E.pressure = md->pressure;
+    }
+    E.virial = md->virial;
+    E.kinetic_com = md->kin_com;
+    for (int a = 0; a < 3; ++a) E.box[a] = md->box[a];
+    E.volume = (double)md->box[0] * md->box[1] * md->box[2];
+    E.pcouple_steps = md->n_pcouple;
     *out = E;
   });
 }
@@ -1137,6 +1493,11 @@ extern "C" int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds)
                   B->device);
       DYN_REQUIRE((x->prm.gamma > 0.f) == (mds[0]->prm.gamma > 0.f),
                   "MD handle %u: Langevin and plain velocity Verlet cannot share a batch", i);
+      DYN_REQUIRE(x->prm.nstpcouple == mds[0]->prm.nstpcouple,
+                  "MD handle %u: members of a batch share nstpcouple", i);
+      DYN_REQUIRE(!x->prm.nstpcouple || x->step == mds[0]->step, "MD handle %u: members of a "
+                  "batch with pressure coupling share the step count (%llu vs %llu)", i,
+                  (unsigned long long)x->step, (unsigned long long)mds[0]->step);
       for (uint32_t j = 0; j < i; ++j)
         DYN_REQUIRE(mds[j] != x && mds[j]->eng != x->eng, "MD handles %u and %u share an engine",
                     j, i);

@@ -48,6 +48,7 @@ struct PmeCore {
   DevBuf<int64_t> grid_fx;      // K0·K1·K2 fixed-point charge grid
   DevBuf<float2> cgrid;         // complex working grid
   DevBuf<float> G;              // influence function
+  DevBuf<double> bsq;           // 1/|b_a(m)|² per axis (K0 + K1 + K2 values)
   DevBuf<float2> tw[3];         // e^{-2πi k/N} tables per axis
   DevBuf<double2> cgrid64;
   DevBuf<double> G64;
@@ -66,7 +67,8 @@ struct PmeCore {
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
             float beta, int order, bool fp64 = false);
-  void set_box(const float l[3]);  // recomputes G
+  void set_box(const float l[3]);  // recomputes G (on the device, asynchronously)
+  void set_moduli();                // bsq for the current K and order
   // Enqueue spread → FFT → convolve → inverse FFT → gather for n charges.
   // xyzq: device float4 array (positions wrapped or not; wrapped internally).
   // If site != null, forces are atomically added (fixed point, 2^32) into

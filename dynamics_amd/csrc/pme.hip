@@ -687,35 +687,48 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
   conv_blocks = (unsigned)((gx.lines + gx.L - 1) / gx.L);
   partials.reserve(2 * (size_t)conv_blocks);
   res_dev.reserve(2);
+  set_moduli();
   set_box(l);
 }
 
-void PmeCore::set_box(const float l[3]) {
-  for (int a = 0; a < 3; ++a) {
-    DYN_REQUIRE(l[a] > 0.f && std::isfinite(l[a]), "box length must be > 0");
-    L[a] = l[a];
-  }
-  // |b(m)|² per axis (Essmann eq. 4.4), from the order-n spline at integers.
-  std::vector<double> bsq[3];
+// Influence function at every grid point (set_box), in double:
+// G(m) = exp(−π²|m|²/β²)/(π V |m|²) · Π_a |b_a(m_a)|², m_a = m'_a/L_a with
+// m'_a the signed index; G(0) = 0.  bsq: 1/|b_a|² per axis, axes concatenated.
+template <class R>
+__global__ void __launch_bounds__(256)
+k_pme_influence(R* __restrict__ G, const double* __restrict__ bsq, uint32_t K0, uint32_t K1,
+                uint32_t K2, double il0, double il1, double il2, double inv_piv, double pi2_b2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)K0 * K1 * K2) return;
+  const uint32_t mz = (uint32_t)(i % K2);
+  const size_t r = i / K2;
+  const uint32_t my = (uint32_t)(r % K1), mx = (uint32_t)(r / K1);
+  const double fx = (mx <= K0 / 2 ? (double)mx : (double)mx - K0) * il0;
+  const double fy = (my <= K1 / 2 ? (double)my : (double)my - K1) * il1;
+  const double fz = (mz <= K2 / 2 ? (double)mz : (double)mz - K2) * il2;
+  const double msq = fx * fx + fy * fy + fz * fz;
+  double val = 0.0;
+  if (msq > 0)
+    val = exp(-pi2_b2 * msq) * inv_piv / msq * bsq[mx] * bsq[K0 + my] * bsq[K0 + K1 + mz];
+  G[i] = (R)val;
+}
+
+// |b(m)|⁻² per axis (Essmann eq. 4.4) from the order-n cardinal B-spline at
+// the integers; a vanishing modulus (odd order at m = K/2) takes the mean of
+// its neighbours.
+void PmeCore::set_moduli() {
   std::vector<double> Mn(order + 1, 0.0);
-  {
-    float th[5], dth[5];
-    if (order == 4) dyn_bspline<4>(0.0f, th, dth);
-    else dyn_bspline<5>(0.0f, th, dth);
-    // th[j] = M_n(n-1-j) at w = 0 → M_n(k) = th[n-1-k]
-    for (int k = 1; k <= order - 1; ++k) Mn[k] = th[order - 1 - k];
-    // exact rational values (tests pin these against the closed form)
-    if (order == 4) {
-      Mn[1] = 1.0 / 6;
-      Mn[2] = 4.0 / 6;
-      Mn[3] = 1.0 / 6;
-    } else {
-      Mn[1] = 1.0 / 24;
-      Mn[2] = 11.0 / 24;
-      Mn[3] = 11.0 / 24;
-      Mn[4] = 1.0 / 24;
-    }
+  if (order == 4) {
+    Mn[1] = 1.0 / 6;
+    Mn[2] = 4.0 / 6;
+    Mn[3] = 1.0 / 6;
+  } else {
+    Mn[1] = 1.0 / 24;
+    Mn[2] = 11.0 / 24;
+    Mn[3] = 11.0 / 24;
+    Mn[4] = 1.0 / 24;
   }
+  std::vector<double> all;
   for (int a = 0; a < 3; ++a) {
     const int Ka = (int)K[a];
     std::vector<double> mod(Ka);
@@ -730,38 +743,34 @@ void PmeCore::set_box(const float l[3]) {
     }
     for (int m = 0; m < Ka; ++m)
       if (mod[m] < 1e-7) mod[m] = 0.5 * (mod[(m - 1 + Ka) % Ka] + mod[(m + 1) % Ka]);
-    bsq[a].resize(Ka);
-    for (int m = 0; m < Ka; ++m) bsq[a][m] = 1.0 / mod[m];
+    for (int m = 0; m < Ka; ++m) all.push_back(1.0 / mod[m]);
+  }
+  bsq.reserve(all.size());
+  DYN_HIP(hipMemcpyAsync(bsq.p, all.data(), sizeof(double) * all.size(), hipMemcpyHostToDevice,
+                         stream));
+  DYN_HIP(hipStreamSynchronize(stream));
+}
+
+// A new box: the influence function is recomputed on the device (the grid
+// and its FFT plan stay; the reference rebuilds its PmeRecip instead,
+// src/non_bonded.rs:850-875).
+void PmeCore::set_box(const float l[3]) {
+  for (int a = 0; a < 3; ++a) {
+    DYN_REQUIRE(l[a] > 0.f && std::isfinite(l[a]), "box length must be > 0");
+    L[a] = l[a];
   }
   const double V = (double)L[0] * L[1] * L[2];
   const double b2 = (double)beta * beta;
-  std::vector<double> g(ngrid);
-  for (uint32_t mx = 0; mx < K[0]; ++mx) {
-    const int mxp = mx <= K[0] / 2 ? (int)mx : (int)mx - (int)K[0];
-    const double fx = mxp / (double)L[0];
-    for (uint32_t my = 0; my < K[1]; ++my) {
-      const int myp = my <= K[1] / 2 ? (int)my : (int)my - (int)K[1];
-      const double fy = myp / (double)L[1];
-      for (uint32_t mz = 0; mz < K[2]; ++mz) {
-        const int mzp = mz <= K[2] / 2 ? (int)mz : (int)mz - (int)K[2];
-        const double fz = mzp / (double)L[2];
-        const double msq = fx * fx + fy * fy + fz * fz;
-        double val = 0.0;
-        if (msq > 0)
-          val = std::exp(-M_PI * M_PI * msq / b2) / (M_PI * V * msq) * bsq[0][mx] * bsq[1][my] *
-                bsq[2][mz];
-        g[((size_t)mx * K[1] + my) * K[2] + mz] = val;
-      }
-    }
-  }
-  if (f64) {
-    DYN_HIP(hipMemcpyAsync(G64.p, g.data(), sizeof(double) * ngrid, hipMemcpyHostToDevice, stream));
-    DYN_HIP(hipStreamSynchronize(stream));
-  } else {
-    std::vector<float> gf(g.begin(), g.end());
-    DYN_HIP(hipMemcpyAsync(G.p, gf.data(), sizeof(float) * ngrid, hipMemcpyHostToDevice, stream));
-    DYN_HIP(hipStreamSynchronize(stream));
-  }
+  const unsigned grid = (unsigned)((ngrid + 255) / 256);
+  const double il0 = 1.0 / (double)L[0], il1 = 1.0 / (double)L[1], il2 = 1.0 / (double)L[2];
+  if (f64)
+    hipLaunchKernelGGL((k_pme_influence<double>), dim3(grid), dim3(256), 0, stream, G64.p, bsq.p,
+                       K[0], K[1], K[2], il0, il1, il2, 1.0 / (M_PI * V), M_PI * M_PI / b2);
+  else
+    hipLaunchKernelGGL((k_pme_influence<float>), dim3(grid), dim3(256), 0, stream, G.p, bsq.p,
+                       K[0], K[1], K[2], il0, il1, il2, 1.0 / (M_PI * V), M_PI * M_PI / b2);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(DYN_ERR_HIP, fmt("launch of k_pme_influence: %s", hipGetErrorString(e)));
 }
 
 void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,

@@ -174,6 +174,10 @@ typedef struct dyn_engine_params {
   uint32_t mode;       /* DYN_MODE_REFERENCE or DYN_MODE_STANDARD            */
   float list_buffer;   /* extra list radius (Å) for list reuse across calls  */
   uint32_t precision;  /* 0: FP32 arithmetic; DYN_PREC_* bits select FP64   */
+  float mesh_spacing;  /* > 0: the PME grid follows the box, chosen by
+                        * dyn_pme_grid_n at every set_box (the reference's
+                        * regen_pme, src/non_bonded.rs:850-875); grid must
+                        * then be 0.  0: the fixed grid above              */
 } dyn_engine_params;
 
 /* Precision options (the precision sweep; the default FP32 path is the one
@@ -205,7 +209,9 @@ int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_params* par
 void dyn_engine_destroy(dyn_engine* eng);
 const char* dyn_engine_last_error(const dyn_engine* eng);
 
-/* Box edge lengths (orthorhombic, origin at 0). */
+/* Box edge lengths (orthorhombic, origin at 0).  A new box recomputes the
+ * SPME influence function on the device; with mesh_spacing, a box whose
+ * grid size differs re-plans the grid. */
 int dyn_engine_set_box(dyn_engine* eng, const float box[3]);
 
 /* Positions in the reference's layout (host pointers, 3 floats per site). */
@@ -245,6 +251,9 @@ int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device);
  * decoupling off.  lambda is clamped to [0, 1].  A changed atom range
  * rebuilds the list at the next compute.  DYN_MODE_REFERENCE only.
  * dyn_energies.dh_dl returns dH/dλ. */
+/* The SPME grid in use (0, 0, 0 without reciprocal space). */
+int dyn_engine_get_pme_grid(dyn_engine* eng, uint32_t out[3]);
+
 int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32_t count, double lambda);
 
 /* One nonbonded evaluation.  Rebuilds the cluster-pair list when asked, on
@@ -360,6 +369,26 @@ typedef struct dyn_md_params {
   uint32_t spme_ratio;/* SPME every spme_ratio-th force evaluation, the kept
                        * SPME forces in between (DYN_COMPUTE_RECIP_REUSE; the
                        * reference's SPME_RATIO = 2); 0 or 1: every one */
+  /* Isotropic C-rescale pressure coupling (Bernetti & Bussi, J. Chem. Phys.
+   * 153, 114107 (2020)), the reference's Barostat (src/barostat.rs:324-413,
+   * switched off there at :367-368; BarostatCfg defaults :28-50: 1 bar,
+   * 5 ps, 4.5e-5 /bar).  Every nstpcouple-th step (0: off) the pressure
+   * P = (2 K_com + W)/(3V) is measured where the reference measures it
+   * (src/integrate.rs:172-190, :241-310) and the box, every pressure
+   * group's centre of mass and every velocity (by 1/μ) are rescaled by
+   * μ = exp(ΔlnV/3), ΔlnV = (κ/τ)(P − P₀)·nstpcouple·dt + √(2κ k_B T
+   * nstpcouple·dt/(τV))·ξ clamped to ±0.1 (scale_factor, :324-351), ξ from
+   * the seed's counter-based stream.  Pressure groups: each water, each
+   * cluster of solute atoms joined by constraints, each other atom; K_com
+   * is their centre-of-mass kinetic energy and W their virial (nonbonded +
+   * bonded − Σ (r − R_group)·F), in which constraint and rigid-water forces
+   * cancel.  The box is scaled about the origin (the reference scales about
+   * its centre: the same up to a uniform translation).  Needs the engine's
+   * list to stay valid as the box changes: every rescaling rebuilds it. */
+  uint32_t nstpcouple;
+  float pressure_target;  /* bar */
+  float tau_p;            /* ps */
+  double compressibility; /* 1/bar */
 } dyn_md_params;
 
 /* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],
@@ -403,6 +432,14 @@ typedef struct dyn_md_energies {
   double temperature;   /* K from kinetic_rigid and dof */
   double dof;           /* 3·n_dyn − constraints + 6·n_water */
   uint64_t steps;
+  double pressure;      /* bar, from the current velocities and the last force
+                           evaluation's forces and virial (NaN when that
+                           evaluation skipped energies) */
+  double virial;        /* kcal/mol, the pressure groups' virial W */
+  double kinetic_com;   /* kcal/mol, the pressure groups' centre-of-mass K */
+  double volume;        /* Å³ */
+  double box[3];        /* Å */
+  uint64_t pcouple_steps; /* box rescalings so far */
 } dyn_md_energies;
 
 int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);

@@ -12,12 +12,20 @@ Follows:
   bonded terms                src/bonded_forces.rs:11-200 (harmonic bond
                               and angle; periodic dihedral written with the
                               IUPAC torsion angle)
+  pressure, C-rescale         src/barostat.rs:324-351 (scale factor),
+                              :419-431 (P = (2K + W)/3V), measured where
+                              src/integrate.rs:172-192 / :241-310 do; with
+                              pressure groups (each water, each constraint
+                              cluster, each other atom): K of the groups'
+                              centres of mass, W = W_atomic − Σ (r − R_g)·F
 The Langevin noise uses the same counter-based deviates as the device
 (SplitMix64 finalizer over seed, step and index; Box–Muller), evaluated in
 float64.  Pinned by: the SplitMix64 known answer (tests/test_oracle_md.py),
 −∇U finite differences for the bonded forces, rigid-body invariants of the
 water drift (geometry, linear and angular momentum), constraint residuals,
-and energy conservation of the γ = 0 step.
+and energy conservation of the γ = 0 step.  The group virial is pinned by
+W = −dU/dλ, U's derivative when every group's centre of mass and the box
+scale by λ (tests/test_oracle_pressure.py).
 """
 
 from __future__ import annotations
@@ -30,6 +38,9 @@ import numpy as np
 from . import fp64
 
 KCAL_TO_NATIVE = 418.4
+BAR_PER_KCAL_MOL_A3 = 69476.95457055373   # src/barostat.rs:22
+KB_BAR_A3 = 138.0649                      # src/barostat.rs:26
+BAROSTAT_INDEX = 0xB0B0000000000000       # the barostat's deviate in the step's stream
 KB = 0.83144626
 M_O, M_H = 16.0, 1.008
 M_W = M_O + 2 * M_H
@@ -69,13 +80,15 @@ def min_image(d, box):
     return d - np.sign(q) * np.floor(np.abs(q) + 0.5) * box
 
 
-def bonded_fp64(pos, ff, box):
+def bonded_fp64(pos, ff, box, virial=False):
     """Forces (n_dyn, 3) and energies (bond, angle, dihedral) of the bonded
-    terms."""
+    terms; with virial=True also W = Σ_a r_a·F_a per term, positions taken
+    relative to one of its atoms."""
     pos = np.asarray(pos, np.float64)
     box = np.asarray(box, np.float64)
     F = np.zeros_like(pos)
     e = [0.0, 0.0, 0.0]
+    w = 0.0
     if len(ff.bonds):
         i, j = ff.bonds[:, 0].astype(np.int64), ff.bonds[:, 1].astype(np.int64)
         k, r0 = ff.bond_k_r0[:, 0].astype(np.float64), ff.bond_k_r0[:, 1].astype(np.float64)
@@ -85,6 +98,7 @@ def bonded_fp64(pos, ff, box):
         np.add.at(F, i, fi)
         np.add.at(F, j, -fi)
         e[0] = float((k * (r - r0) ** 2).sum())
+        w += -float((d * fi).sum())
     if len(ff.angles):
         i, j, l = (ff.angles[:, c].astype(np.int64) for c in range(3))
         k, t0 = ff.angle_k_t0[:, 0].astype(np.float64), ff.angle_k_t0[:, 1].astype(np.float64)
@@ -104,6 +118,7 @@ def bonded_fp64(pos, ff, box):
         np.add.at(F, l, f2)
         np.add.at(F, j, -(f0 + f2))
         e[1] = float((k * (th - t0) ** 2).sum())
+        w += float((b0 * f0).sum() + (b2 * f2).sum())
     if len(ff.dihedrals):
         i, j, k, l = (ff.dihedrals[:, c].astype(np.int64) for c in range(4))
         kk = ff.dihe_k_n_phase[:, 0].astype(np.float64)
@@ -127,6 +142,10 @@ def bonded_fp64(pos, ff, box):
         for idx, gv in ((i, gi), (j, gj), (k, gk), (l, gl)):
             np.add.at(F, idx, -gv * dedphi[:, None])
         e[2] = float((kk * (1 + np.cos(arg))).sum())
+        fi_, fk_, fl_ = (-gv * dedphi[:, None] for gv in (gi, gk, gl))
+        w += float((-b1 * fi_).sum() + (b2 * fk_).sum() + ((b2 + b3) * fl_).sum())
+    if virial:
+        return F, e, w
     return F, e
 
 
@@ -150,7 +169,8 @@ class Model:
     """The forces and parameters of one MD setup (FP64)."""
 
     def __init__(self, system, ff, rc, beta, K, order=4, mode="reference", dt=0.002,
-                 temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0):
+                 temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0, nstpcouple=0,
+                 pressure_target=1.0, tau_p=5.0, compressibility=4.5e-5):
         self.system, self.ff = system, ff
         self.rc, self.beta, self.K, self.order, self.mode = rc, beta, K, order, mode
         self.dt, self.T, self.gamma, self.tol, self.seed = dt, temperature, gamma, shake_tol, seed
@@ -172,19 +192,117 @@ class Model:
         for k, (a, b) in enumerate(ff.constraints):
             groups.setdefault(find(int(a)), []).append(k)
         self.clusters = list(groups.values())
+        # pressure groups of the solute: atoms of each constraint cluster, then
+        # every atom in no constraint
+        in_cons = np.zeros(self.nd, bool)
+        if len(ff.constraints):
+            in_cons[np.asarray(ff.constraints, np.int64).reshape(-1)] = True
+        comp = {}
+        for a in range(self.nd):
+            if in_cons[a]:
+                comp.setdefault(find(a), []).append(a)
+        order_roots = []
+        for k in range(len(ff.constraints)):
+            r = find(int(ff.constraints[k][0]))
+            if r not in order_roots:
+                order_roots.append(r)
+        self.pgroups = [comp[r] for r in order_roots] + [[a] for a in range(self.nd) if not in_cons[a]]
+        self.nst, self.p0, self.tau_p, self.kappa = nstpcouple, pressure_target, tau_p, compressibility
         self.f = None
         self.energy = None
+        self.pressure_log = []
 
     def forces(self, st):
         s = _system_at(self.system, st)
+        s.box = self.box.copy()
         r = fp64.reference_energy_and_forces(s, self.rc, self.rc, self.beta, self.K, self.order,
                                              mode=self.mode)
         F = r["forces"].copy()
-        fb, eb = bonded_fp64(st.x_dyn, self.ff, self.box)
+        fb, eb, wb = bonded_fp64(st.x_dyn, self.ff, self.box, virial=True)
         F[:self.nd] += fb
         self.f = F
+        self.w_atomic = r["virial_sr"] + r["virial_recip"] + wb
         self.energy = {"nonbonded": r["e_total"], "bond": eb[0], "angle": eb[1], "dihedral": eb[2]}
         return F
+
+    def _groups(self, st):
+        """(positions (n,3), velocities, masses, forces) per pressure group;
+        water: O, H0, H1 with the M-site force projected as the kick does."""
+        nd = self.nd
+        out = []
+        for g in self.pgroups:
+            out.append((st.x_dyn[g], st.v_dyn[g], self.mass[g], self.f[g]))
+        if self.nw:
+            fw = self.f[nd:].reshape(self.nw, 4, 3)
+            fo = fw[:, 0] + C_O * fw[:, 1]
+            f0 = fw[:, 2] + C_H * fw[:, 1]
+            f1 = fw[:, 3] + C_H * fw[:, 1]
+            xw = np.stack([st.x_w[:, 0], st.x_w[:, 2], st.x_w[:, 3]], axis=1)
+            fws = np.stack([fo, f0, f1], axis=1)
+            out.append(("water", xw, st.v_w, np.array([M_O, M_H, M_H]), fws))
+        return out
+
+    def pressure(self, st):
+        """(P bar, W group virial kcal/mol, K_com kcal/mol) from the last
+        forces and the current velocities."""
+        kin = 0.0
+        cvir = 0.0
+        for item in self._groups(st):
+            if isinstance(item[0], str):
+                _, xw, vw, mw, fws = item
+                d = min_image(xw - xw[:, :1], self.box)
+                M = mw.sum()
+                R = (d * mw[None, :, None]).sum(1) / M
+                V = (vw * mw[None, :, None]).sum(1) / M
+                kin += 0.5 * M * float((V * V).sum())
+                cvir += float(((d - R[:, None]) * fws).sum())
+                continue
+            x, v, m, f = item
+            d = min_image(x - x[:1], self.box)
+            M = m.sum()
+            R = (d * m[:, None]).sum(0) / M
+            V = (v * m[:, None]).sum(0) / M
+            kin += 0.5 * M * float(V @ V)
+            cvir += float(((d - R) * f).sum())
+        kin /= KCAL_TO_NATIVE
+        W = self.w_atomic - cvir
+        P = (2 * kin + W) / (3 * float(np.prod(self.box))) * BAR_PER_KCAL_MOL_A3
+        return P, W, kin
+
+    def c_rescale_mu(self, P, step):
+        """Barostat::scale_factor (src/barostat.rs:324-351) over nstpcouple·dt."""
+        dt = self.dt * self.nst
+        V = float(np.prod(self.box))
+        det = (self.kappa / self.tau_p) * (P - self.p0) * dt
+        sig = np.sqrt(2 * self.kappa * KB_BAR_A3 * self.T * dt / (self.tau_p * V))
+        xi = float(normal(self.seed, step, BAROSTAT_INDEX))
+        return float(np.exp(np.clip(det + sig * xi, -0.1, 0.1) / 3))
+
+    def rescale(self, st, mu):
+        """Every pressure group's centre of mass to μR (about the origin),
+        velocities by 1/μ, the box by μ."""
+        for g in self.pgroups:
+            x = st.x_dyn[g]
+            d = min_image(x - x[:1], self.box)
+            m = self.mass[g]
+            R = x[0] + (d * m[:, None]).sum(0) / m.sum()
+            st.x_dyn[g] = x + (mu - 1) * R
+        st.v_dyn = st.v_dyn / mu
+        if self.nw:
+            mw = np.array([M_O, M_H, M_H])
+            xw = np.stack([st.x_w[:, 0], st.x_w[:, 2], st.x_w[:, 3]], axis=1)
+            d = min_image(xw - xw[:, :1], self.box)
+            R = xw[:, 0] + (d * mw[None, :, None]).sum(1) / mw.sum()
+            st.x_w = st.x_w + (mu - 1) * R[:, None, :]
+            st.v_w = st.v_w / mu
+        self.box = self.box * mu
+
+    def barostat(self, st):
+        P, W, kin = self.pressure(st)
+        mu = self.c_rescale_mu(P, st.step)
+        self.pressure_log.append((st.step, P, W, kin, mu))
+        if abs(mu - 1) >= 1e-12:
+            self.rescale(st, mu)
 
     def kick(self, st, h):
         F = self.f
@@ -278,9 +396,14 @@ class Model:
             self.drift(st, self.dt)
             self.shake(st)
             self.rattle(st)
+        couple = self.nst > 0 and (st.step + 1) % self.nst == 0
         self.forces(st)
+        if couple and self.gamma > 0:
+            self.barostat(st)
         self.kick(st, h)
         self.rattle(st)
+        if couple and self.gamma <= 0:
+            self.barostat(st)
         st.step += 1
         return st
 
