@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "common.h"
+#include "wave.h"
 #include "dyn_math.h"
 #include "engine.h"
 #include "engine_internal.h"
@@ -100,20 +101,24 @@ __device__ __forceinline__ void k_eng_bin_count(const float4* __restrict__ pos, 
                                 EngGeom g, float4* __restrict__ wpos, float4* __restrict__ wpos_ref,
                                 uint32_t* __restrict__ col_count, uint2* __restrict__ site_col) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= g.n_dyn + 4u * g.n_water) return;
-  const float4 p = pos[s];
-  const float x = wrap1(p.x, g.L[0], g.invL[0]);
-  const float y = wrap1(p.y, g.L[1], g.invL[1]);
-  const float z = wrap1(p.z, g.L[2], g.invL[2]);
-  const float4 w = make_float4(x, y, z, q[s]);
-  wpos[s] = w;
-  if (wpos_ref) wpos_ref[s] = w;
-  const bool solute = s < g.n_dyn;
-  const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
-  const ClassGrid cg = solute ? g.cg[0] : (oxy ? g.cg[1] : g.cg[2]);
-  const uint32_t col = (uint32_t)cg.col_off + site_column(cg, x, y);
-  const uint32_t rank = atomicAdd(&col_count[col], 1u);
-  site_col[s] = make_uint2(col, rank);
+  const bool valid = s < g.n_dyn + 4u * g.n_water;
+  uint32_t col = 0;
+  if (valid) {
+    const float4 p = pos[s];
+    const float x = wrap1(p.x, g.L[0], g.invL[0]);
+    const float y = wrap1(p.y, g.L[1], g.invL[1]);
+    const float z = wrap1(p.z, g.L[2], g.invL[2]);
+    const float4 w = make_float4(x, y, z, q[s]);
+    wpos[s] = w;
+    if (wpos_ref) wpos_ref[s] = w;
+    const bool solute = s < g.n_dyn;
+    const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
+    const ClassGrid cg = solute ? g.cg[0] : (oxy ? g.cg[1] : g.cg[2]);
+    col = (uint32_t)cg.col_off + site_column(cg, x, y);
+  }
+  // one atomic per distinct column of the wave
+  const uint32_t rank = wave_rank(col, valid, col_count);
+  if (valid) site_col[s] = make_uint2(col, rank);
 }
 
 __device__ __forceinline__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
@@ -300,16 +305,38 @@ k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col
   }
 }
 
-// Wrap all sites (list-reuse path).
-__device__ __forceinline__ void k_eng_wrap(const float4* __restrict__ pos, const float* __restrict__ q,
-                           uint32_t n, EngGeom g, float4* __restrict__ wpos,
-                           unsigned int* __restrict__ disp_bits) {
+// Wrap every site into the box and maximise its squared displacement since
+// the list build into *disp_bits (one atomic per workgroup; squared
+// distances are non-negative floats, ordered like their bit patterns).  The
+// previous evaluation's finalize cleared *disp_bits.
+__device__ __forceinline__ void k_eng_wrap_disp(const float4* __restrict__ pos,
+                                                const float* __restrict__ q, uint32_t n, EngGeom g,
+                                                float4* __restrict__ wpos,
+                                                const float4* __restrict__ ref,
+                                                unsigned int* __restrict__ disp_bits) {
+  __shared__ unsigned int red[256 / kWave];
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) *disp_bits = 0u;  // k_eng_max_disp maximises into it next
-  if (s >= n) return;
-  const float4 p = pos[s];
-  wpos[s] = make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
-                        wrap1(p.z, g.L[2], g.invL[2]), q[s]);
+  unsigned int v = 0u;
+  if (s < n) {
+    const float4 p = pos[s];
+    const float4 w = make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
+                                 wrap1(p.z, g.L[2], g.invL[2]), q[s]);
+    wpos[s] = w;
+    const float4 b = ref[s];
+    float dx = w.x - b.x, dy = w.y - b.y, dz = w.z - b.z;
+    dx -= g.L[0] * rintf(dx * g.invL[0]);
+    dy -= g.L[1] * rintf(dy * g.invL[1]);
+    dz -= g.L[2] * rintf(dz * g.invL[2]);
+    v = __float_as_uint(dx * dx + dy * dy + dz * dz);
+  }
+  for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int mx = 0u;
+    for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
+    if (mx) atomicMax(disp_bits, mx);
+  }
 }
 
 // Slot positions from the current wrapped positions (list-reuse path).
@@ -1002,6 +1029,23 @@ __device__ __forceinline__ void k_eng_site_prep(const float4* __restrict__ wpos,
   }
 }
 
+// List-reuse path: the slot positions and the site set-up in one launch
+// (grid over the larger of the two ranges).
+__device__ __forceinline__ void k_eng_slots_prep(const int32_t* __restrict__ slot_site,
+                                                 const uint32_t* __restrict__ n_cl,
+                                                 float4* __restrict__ xyzq,
+                                                 const float4* __restrict__ wpos, uint32_t n_sites,
+                                                 const uint32_t* __restrict__ qsite, uint32_t nq,
+                                                 float4* __restrict__ pme_xyzq,
+                                                 float4* __restrict__ pme_b, uint32_t alch0,
+                                                 uint32_t alch1, int64_t* __restrict__ fx_site,
+                                                 int64_t* __restrict__ fx_recip,
+                                                 int64_t* __restrict__ mol_e, uint32_t n_mol2) {
+  k_eng_slots(slot_site, n_cl, wpos, xyzq);
+  k_eng_site_prep(wpos, n_sites, qsite, nq, pme_xyzq, pme_b, alch0, alch1, fx_site, fx_recip,
+                  mol_e, n_mol2);
+}
+
 // Sum of the work-item force rows of slot k's cluster row (fixed point).
 __device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fitem,
                                                  const uint32_t* __restrict__ chunk_ptr,
@@ -1025,8 +1069,10 @@ __device__ __forceinline__ void k_eng_finalize(const int64_t* __restrict__ fx_si
                                const int64_t* __restrict__ fitem,
                                const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
                                const uint32_t* __restrict__ slot_of, uint32_t n_sites,
-                               uint32_t n_slots, int64_t* __restrict__ out) {
+                               uint32_t n_slots, int64_t* __restrict__ out,
+                               unsigned int* __restrict__ disp_bits) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) *disp_bits = 0u;  // for the next evaluation's displacement reading
   if (s >= n_sites) return;
   int64_t f[3] = {fx_site[s], fx_site[(size_t)n_sites + s], fx_site[2 * (size_t)n_sites + s]};
   if (fx_recip) {
@@ -1116,32 +1162,6 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
     if (!(pme_runs < 0 && t == 13)) out[t] = t < kNumPart ? 0.5 * x : x;
   }
   if (pme_runs >= 0 && t >= 16 && t < kEngOut && (t - 16) / 2 >= pme_runs) out[t] = 0.0;
-}
-
-// Largest squared displacement since the list build: one atomic per
-// workgroup (block-wide max first; 256 threads).  Squared distances are
-// non-negative floats, ordered like their bit patterns.
-__device__ __forceinline__ void k_eng_max_disp(const float4* __restrict__ wpos, const float4* __restrict__ ref,
-                               uint32_t n, EngGeom g, unsigned int* __restrict__ out_bits) {
-  __shared__ unsigned int red[256 / kWave];
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned int v = 0u;
-  if (s < n) {
-    const float4 a = wpos[s], b = ref[s];
-    float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-    dx -= g.L[0] * rintf(dx * g.invL[0]);
-    dy -= g.L[1] * rintf(dy * g.invL[1]);
-    dz -= g.L[2] * rintf(dz * g.invL[2]);
-    v = __float_as_uint(dx * dx + dy * dy + dz * dz);
-  }
-  for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int m = 0u;
-    for (int w = 0; w < 256 / kWave; ++w) m = max(m, red[w]);
-    if (m) atomicMax(out_bits, m);
-  }
 }
 
 }  // namespace
@@ -1260,6 +1280,8 @@ void DispRing::init(uint32_t systems) {
   if (systems == n && bits.p) return;
   n = systems;
   bits.reserve(std::max<uint32_t>(n, 1));
+  // zero readings; every evaluation's finalize clears its system's again
+  DYN_HIP(hipMemset(bits.p, 0, sizeof(unsigned int) * std::max<uint32_t>(n, 1)));
   h.reserve((size_t)kRing * std::max<uint32_t>(n, 1));
   h_sync.reserve(std::max<uint32_t>(n, 1));
   for (auto& e : ev)
@@ -1538,13 +1560,10 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
   }
   if (!rebuild) {
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_wrap>(R, "k_eng_wrap", 0, gs, 256, 0, [&](uint32_t i) {
+    launch<k_eng_wrap_disp>(R, "k_eng_wrap_disp", 0, gs, 256, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_wrap>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p, D.bits.p + i);
-    });
-    launch<k_eng_max_disp>(R, "k_eng_max_disp", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_max_disp>(e.wpos.p, e.wpos_ref.p, e.n_sites, e.geom, D.bits.p + i);
+      return args<k_eng_wrap_disp>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p,
+                                   (const float4*)e.wpos_ref.p, D.bits.p + i);
     });
     if (async) {
       const int k = D.next;
@@ -1568,16 +1587,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       }
     }
   }
-  if (rebuild) {
-    set_build_lists(S);
-  } else {
-    Profiler::Scope ps(prof, "slots");
-    const unsigned gk = max_grid(S, [](dyn_engine& e) { return grid_for(e.cap_slots, 256); });
-    launch<k_eng_slots>(R, "k_eng_slots", 0, gk, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_slots>(e.slot_site.p, e.counters.p + 0, e.wpos.p, e.xyzq.p);
-    });
-  }
+  if (rebuild) set_build_lists(S);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
   // per system: pair parameters and which reciprocal work it does
@@ -1626,7 +1636,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     any_b = any_b || pl.recip_b;
     any_corr = any_corr || pl.corr_on;
   }
-  {
+  if (rebuild) {
     Profiler::Scope ps(prof, "site_prep");
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
     launch<k_eng_site_prep>(R, "k_eng_site_prep", 0, gs, 256, 0, [&](uint32_t i) {
@@ -1638,6 +1648,23 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
                                    e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr,
                                    energy_on && e.n_mol ? e.mol_e.p : (int64_t*)nullptr,
                                    e.n_mol * e.n_mol);
+    });
+  } else {
+    // list reuse: the slots' positions and the site set-up in one launch
+    Profiler::Scope ps(prof, "slots_prep");
+    const unsigned gs = max_grid(S, [](dyn_engine& e) {
+      return std::max(grid_for(e.n_sites, 256), grid_for(e.cap_slots, 256));
+    });
+    launch<k_eng_slots_prep>(R, "k_eng_slots_prep", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      const Plan& pl = PL[i];
+      return args<k_eng_slots_prep>(e.slot_site.p, e.counters.p + 0, e.xyzq.p, e.wpos.p,
+                                    e.n_sites, e.qsite.p, e.n_q,
+                                    pl.recip_on ? e.pme_xyzq.p : (float4*)nullptr,
+                                    pl.recip_b ? e.pme_b.p : (float4*)nullptr, e.alch0, e.alch1,
+                                    e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr,
+                                    energy_on && e.n_mol ? e.mol_e.p : (int64_t*)nullptr,
+                                    e.n_mol * e.n_mol);
     });
   }
   if (short_on) {
@@ -1753,7 +1780,8 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       return args<k_eng_finalize>(e.fx_site.p,
                                   (pl.recip_on || pl.reuse) ? e.fx_recip.p : (const int64_t*)nullptr,
                                   short_on ? e.fitem.p : (const int64_t*)nullptr, e.chunk_ptr.p,
-                                  e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p);
+                                  e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p,
+                                  D.bits.p + i);
     });
     if (short_on && energy_on)
       launch<k_eng_energy_reduce1, kRed1Block>(R, "k_eng_energy_reduce1", 0, kRed1Grid, kRed1Block,

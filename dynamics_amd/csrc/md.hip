@@ -188,98 +188,59 @@ __device__ void rigid_drift(float4* __restrict__ x, float4* __restrict__ v, uint
   v[base + 1] = make_float4(vm.x, vm.y, vm.z, 0.f);
 }
 
-struct IntArgs {
-  Fx F;
-  const float* acc_conv;  // per solute atom: KCAL_TO_NATIVE / mass
-  Box B;
-  uint32_t n_dyn, n_water;
-  float h_kick, h_drift;
-  int kick, drift;
-};
-
-// One item per solute atom, then one per water.  kick: v += a·h_kick from
-// the current forces (M-site force projected onto O and H for water; atom
-// accelerations clamped to MAX_ACCEL); drift: x += v·h_drift for atoms,
-// the rigid drift for water.
-__device__ __forceinline__ void k_md_integrate(IntArgs A, float4* __restrict__ x, float4* __restrict__ v,
-                                               uint64_t* __restrict__ step_inc) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (step_inc && t == 0) ++*step_inc;  // the step's last kick: the next step index
-  if (t < A.n_dyn) {
-    float4 vi = v[t];
-    if (A.kick) {
-      V3 a = force_of(A.F, t) * A.acc_conv[t];
-      const float a2 = dot(a, a);
-      if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
-      vi.x += a.x * A.h_kick;
-      vi.y += a.y * A.h_kick;
-      vi.z += a.z * A.h_kick;
-      v[t] = vi;
-    }
-    if (A.drift) {
-      float4 xi = x[t];
-      xi.x += vi.x * A.h_drift;
-      xi.y += vi.y * A.h_drift;
-      xi.z += vi.z * A.h_drift;
-      x[t] = xi;
-    }
-    return;
-  }
-  const uint32_t m = t - A.n_dyn;
-  if (m >= A.n_water) return;
-  const uint32_t base = A.n_dyn + 4u * m;
-  if (A.kick) {
-    const V3 fm = force_of(A.F, base + 1);
-    const V3 fo = force_of(A.F, base) + fm * kCO;
-    const V3 f0 = force_of(A.F, base + 2) + fm * kCH;
-    const V3 f1 = force_of(A.F, base + 3) + fm * kCH;
-    const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
-    const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
-    const uint32_t ss[3] = {base, base + 2, base + 3};
-    for (int k = 0; k < 3; ++k) {
-      float4 vv = v[ss[k]];
-      vv.x += fs[k].x * A.h_kick;
-      vv.y += fs[k].y * A.h_kick;
-      vv.z += fs[k].z * A.h_kick;
-      v[ss[k]] = vv;
-    }
-  }
-  if (A.drift) rigid_drift(x, v, base, A.h_drift, A.B);
+// Kick of solute atom t: v += a·h from the current forces, the acceleration
+// clamped to MAX_ACCEL.
+__device__ __forceinline__ void kick_atom(const Fx& F, const float* __restrict__ acc_conv,
+                                          uint32_t t, float h, float4* __restrict__ v) {
+  float4 vi = v[t];
+  V3 a = force_of(F, t) * acc_conv[t];
+  const float a2 = dot(a, a);
+  if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
+  vi.x += a.x * h;
+  vi.y += a.y * h;
+  vi.z += a.z * h;
+  v[t] = vi;
 }
 
-// Langevin velocity update (Ornstein–Uhlenbeck): v ← c·v + σ·ξ per
-// component, σ² = k_B T (1 − c²)/m; solute atoms, water O and H.
-__device__ __forceinline__ void k_md_langevin(float4* __restrict__ v, uint32_t n_dyn, uint32_t n_water, float c,
-                              const float* __restrict__ sigma_dyn, float sigma_o, float sigma_h,
-                              uint64_t seed, const uint64_t* __restrict__ step_dev) {
-  // the step index lives on the device (the step's last kick advances it),
-  // so the launch arguments stay the same from step to step
-  const uint64_t step = *step_dev;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t sites[3];
-  float sig[3];
-  int n = 0;
-  if (t < n_dyn) {
-    sites[0] = t;
-    sig[0] = sigma_dyn[t];
-    n = 1;
-  } else if (t - n_dyn < n_water) {
-    const uint32_t base = n_dyn + 4u * (t - n_dyn);
-    sites[0] = base;
-    sites[1] = base + 2;
-    sites[2] = base + 3;
-    sig[0] = sigma_o;
-    sig[1] = sig[2] = sigma_h;
-    n = 3;
+__device__ __forceinline__ void drift_atom(uint32_t t, float h, float4* __restrict__ x,
+                                           const float4* __restrict__ v) {
+  const float4 vi = v[t];
+  float4 xi = x[t];
+  xi.x += vi.x * h;
+  xi.y += vi.y * h;
+  xi.z += vi.z * h;
+  x[t] = xi;
+}
+
+// Kick of the water whose O is site base: the M-site force projected onto
+// O and the H sites.
+__device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h,
+                                           float4* __restrict__ v) {
+  const V3 fm = force_of(F, base + 1);
+  const V3 fo = force_of(F, base) + fm * kCO;
+  const V3 f0 = force_of(F, base + 2) + fm * kCH;
+  const V3 f1 = force_of(F, base + 3) + fm * kCH;
+  const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
+  const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
+  const uint32_t ss[3] = {base, base + 2, base + 3};
+  for (int k = 0; k < 3; ++k) {
+    float4 vv = v[ss[k]];
+    vv.x += fs[k].x * h;
+    vv.y += fs[k].y * h;
+    vv.z += fs[k].z * h;
+    v[ss[k]] = vv;
   }
-  for (int k = 0; k < n; ++k) {
-    const uint32_t s = sites[k];
-    float4 vv = v[s];
-    vv.x = c * vv.x + sig[k] * normal_dev(seed, step, 3ull * s + 0);
-    vv.y = c * vv.y + sig[k] * normal_dev(seed, step, 3ull * s + 1);
-    vv.z = c * vv.z + sig[k] * normal_dev(seed, step, 3ull * s + 2);
-    v[s] = vv;
-  }
+}
+
+// Langevin velocity update (Ornstein–Uhlenbeck) of site s: v ← c·v + σ·ξ per
+// component, σ² = k_B T (1 − c²)/m.
+__device__ __forceinline__ void langevin_site(float4* __restrict__ v, uint32_t s, float c,
+                                              float sig, uint64_t seed, uint64_t step) {
+  float4 vv = v[s];
+  vv.x = c * vv.x + sig * normal_dev(seed, step, 3ull * s + 0);
+  vv.y = c * vv.y + sig * normal_dev(seed, step, 3ull * s + 1);
+  vv.z = c * vv.z + sig * normal_dev(seed, step, 3ull * s + 2);
+  v[s] = vv;
 }
 
 struct ConsArgs {
@@ -292,11 +253,10 @@ struct ConsArgs {
   float tol;
 };
 
-// SHAKE over one cluster of coupled constraints (one thread), in the
+// SHAKE over cluster c of coupled constraints (one thread), in the
 // constraints' input order, until the largest correction is below tol.
-__device__ __forceinline__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C.n_clusters) return;
+__device__ __forceinline__ void shake_cluster(const ConsArgs& C, uint32_t c,
+                                              float4* __restrict__ x) {
   const uint32_t k0 = C.cl_ptr[c], k1 = C.cl_ptr[c + 1];
   for (int it = 0; it < kShakeMaxIt; ++it) {
     float max_corr = 0.0f;
@@ -316,11 +276,11 @@ __device__ __forceinline__ void k_md_shake(ConsArgs C, float4* __restrict__ x) {
   }
 }
 
-// RATTLE: one pass over the cluster's constraints removing the relative
+// RATTLE: one pass over cluster c's constraints removing the relative
 // velocity along each bond.
-__device__ __forceinline__ void k_md_rattle(ConsArgs C, const float4* __restrict__ x, float4* __restrict__ v) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= C.n_clusters) return;
+__device__ __forceinline__ void rattle_cluster(const ConsArgs& C, uint32_t c,
+                                               const float4* __restrict__ x,
+                                               float4* __restrict__ v) {
   for (uint32_t k = C.cl_ptr[c]; k < C.cl_ptr[c + 1]; ++k) {
     const uint2 p = C.ij[k];
     const float4 a = x[p.x], b = x[p.y];
@@ -690,6 +650,96 @@ __device__ __forceinline__ void k_md_rescale(GroupArgs G, float4* __restrict__ x
   }
 }
 
+// The integration around one force evaluation, one thread per pressure
+// group (a cluster of constrained solute atoms, a lone atom or a water), so
+// each group's kick, drift, constraints and thermostat run back to back in
+// one launch.  Solute groups [0, n_clusters) are the constraint clusters.
+struct StepArgs {
+  Fx F;
+  const float* acc_conv;  // per solute atom: KCAL_TO_NATIVE / mass
+  GroupArgs G;
+  ConsArgs C;
+  float h;                // dt/2
+  float dt;
+  int langevin;           // Langevin-middle (else velocity Verlet)
+  float lang_c;           // e^{−γ dt}
+  const float* sigma_dyn; // per solute atom: Langevin σ
+  float sigma_o, sigma_h;
+  uint64_t seed;
+  int64_t* fb;            // bonded force sums to clear for the evaluation
+};
+
+// Before the force evaluation.  Langevin-middle (src/integrate.rs:100-160):
+// kick ½ + drift ½, SHAKE, RATTLE, Langevin, RATTLE, drift ½, SHAKE.
+// Velocity Verlet (:213-230): kick ½ + drift 1, SHAKE, RATTLE.  Water: the
+// same steps with the rigid drift and no constraints.  Clears the group's
+// bonded force sums.
+__device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
+                                            float4* __restrict__ v,
+                                            const uint64_t* __restrict__ step_dev) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.G.n_sgroups + A.G.n_water) return;
+  const float hd = A.langevin ? A.h : A.dt;
+  if (g < A.G.n_sgroups) {
+    const uint32_t a0 = A.G.grp_ptr[g], a1 = A.G.grp_ptr[g + 1];
+    const bool cons = g < A.C.n_clusters;
+    for (uint32_t a = a0; a < a1; ++a) {
+      const uint32_t t = A.G.grp_atoms[a];
+      kick_atom(A.F, A.acc_conv, t, A.h, v);
+      drift_atom(t, hd, x, v);
+    }
+    if (cons) {
+      shake_cluster(A.C, g, x);
+      rattle_cluster(A.C, g, x, v);
+    }
+    if (A.langevin) {
+      const uint64_t step = *step_dev;
+      for (uint32_t a = a0; a < a1; ++a) {
+        const uint32_t t = A.G.grp_atoms[a];
+        langevin_site(v, t, A.lang_c, A.sigma_dyn[t], A.seed, step);
+      }
+      if (cons) rattle_cluster(A.C, g, x, v);
+      for (uint32_t a = a0; a < a1; ++a) drift_atom(A.G.grp_atoms[a], A.h, x, v);
+      if (cons) shake_cluster(A.C, g, x);
+    }
+    const uint32_t nd = A.G.n_dyn;
+    for (uint32_t a = a0; a < a1; ++a) {
+      const uint32_t t = A.G.grp_atoms[a];
+      A.fb[t] = 0;
+      A.fb[nd + t] = 0;
+      A.fb[2 * (size_t)nd + t] = 0;
+    }
+    return;
+  }
+  const uint32_t base = A.G.n_dyn + 4u * (g - A.G.n_sgroups);
+  kick_water(A.F, base, A.h, v);
+  rigid_drift(x, v, base, hd, A.G.B);
+  if (A.langevin) {
+    const uint64_t step = *step_dev;
+    langevin_site(v, base, A.lang_c, A.sigma_o, A.seed, step);
+    langevin_site(v, base + 2, A.lang_c, A.sigma_h, A.seed, step);
+    langevin_site(v, base + 3, A.lang_c, A.sigma_h, A.seed, step);
+    rigid_drift(x, v, base, A.h, A.G.B);
+  }
+}
+
+// After the force evaluation: kick ½ with the new forces, RATTLE; the step's
+// last launch advances the device step index.
+__device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
+                                            float4* __restrict__ v,
+                                            uint64_t* __restrict__ step_inc) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) ++*step_inc;
+  if (g >= A.G.n_sgroups + A.G.n_water) return;
+  if (g < A.G.n_sgroups) {
+    for (uint32_t a = A.G.grp_ptr[g]; a < A.G.grp_ptr[g + 1]; ++a)
+      kick_atom(A.F, A.acc_conv, A.G.grp_atoms[a], A.h, v);
+    if (g < A.C.n_clusters) rattle_cluster(A.C, g, x, v);
+    return;
+  }
+  kick_water(A.F, A.G.n_dyn + 4u * (g - A.G.n_sgroups), A.h, v);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -775,7 +825,7 @@ void engine_set_compute_sync(EngineSet& S, uint32_t flags) {
 // is found at the wait that ends the step call).  Nonbonded energies are
 // evaluated only when asked (the last step of a step call), the forces
 // alone otherwise.
-void md_forces(MdSet& S, bool energies) {
+void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
   bool first = false, cadence_skip = true;
   for (dyn_md* m : S.m) {
     first = first || !m->forces_valid;
@@ -791,15 +841,17 @@ void md_forces(MdSet& S, bool energies) {
   else
     engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC);
   Profiler::Scope ps(*S.prof, "md_bonded");
-  const unsigned gz = [&] {
-    unsigned g = 1;
-    for (dyn_md* m : S.m) g = std::max(g, grid_for(3 * (size_t)std::max(m->n_dyn, 1u), kBlock));
-    return g;
-  }();
-  launch<k_md_zero>(S.R, "k_md_zero", 0, gz, kBlock, 0, [&](uint32_t i) {
-    dyn_md& m = *S.m[i];
-    return args<k_md_zero>(m.fb.p, 3 * (size_t)m.n_dyn);
-  });
+  if (fb_clear) {  // else the step's first launch has cleared the sums
+    const unsigned gz = [&] {
+      unsigned g = 1;
+      for (dyn_md* m : S.m) g = std::max(g, grid_for(3 * (size_t)std::max(m->n_dyn, 1u), kBlock));
+      return g;
+    }();
+    launch<k_md_zero>(S.R, "k_md_zero", 0, gz, kBlock, 0, [&](uint32_t i) {
+      dyn_md& m = *S.m[i];
+      return args<k_md_zero>(m.fb.p, 3 * (size_t)m.n_dyn);
+    });
+  }
   bool any = false;
   unsigned gb = 1;
   for (dyn_md* m : S.m) {
@@ -831,30 +883,6 @@ void md_forces(MdSet& S, bool energies) {
   for (dyn_md* m : S.m) m->forces_valid = true;
 }
 
-void md_integrate(MdSet& S, int tag, float hk, float hd, bool kick, bool drift) {
-  Profiler::Scope ps(*S.prof, "md_integrate");
-  unsigned g = 1;
-  for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_dyn + m->n_water, kBlock));
-  launch<k_md_integrate>(S.R, "k_md_integrate", tag, g, kBlock, 0, [&](uint32_t i) {
-    dyn_md& m = *S.m[i];
-    IntArgs A;
-    std::memset(&A, 0, sizeof A);
-    A.F.nb = engine_forces_device(m.eng);
-    A.F.bd = m.fb.p;
-    A.F.n_sites = m.n_sites;
-    A.F.n_dyn = m.n_dyn;
-    A.acc_conv = m.acc_conv.p;
-    for (int a = 0; a < 3; ++a) A.B.L[a] = m.box[a];
-    A.n_dyn = m.n_dyn;
-    A.n_water = m.n_water;
-    A.h_kick = m.prm.dt * hk;
-    A.h_drift = m.prm.dt * hd;
-    A.kick = kick ? 1 : 0;
-    A.drift = drift ? 1 : 0;
-    return args<k_md_integrate>(A, m.x.p, m.v.p, tag == 2 ? m.step_dev.p : (uint64_t*)nullptr);
-  });
-}
-
 ConsArgs cons_args(const dyn_md& M) {
   ConsArgs C;
   std::memset(&C, 0, sizeof C);
@@ -866,30 +894,6 @@ ConsArgs cons_args(const dyn_md& M) {
   C.n_clusters = M.n_clusters;
   C.tol = M.prm.shake_tol;
   return C;
-}
-
-unsigned cons_grid(const MdSet& S) {
-  unsigned g = 0;
-  for (dyn_md* m : S.m) g = std::max(g, m->n_clusters ? grid_for(m->n_clusters, 64) : 0u);
-  return g;
-}
-
-void md_constrain_positions(MdSet& S, int tag) {
-  const unsigned g = cons_grid(S);
-  if (!g) return;
-  Profiler::Scope ps(*S.prof, "md_constraints");
-  launch<k_md_shake, 64>(S.R, "k_md_shake", tag, g, 64, 0, [&](uint32_t i) {
-    return args<k_md_shake>(cons_args(*S.m[i]), S.m[i]->x.p);
-  });
-}
-
-void md_constrain_velocities(MdSet& S, int tag) {
-  const unsigned g = cons_grid(S);
-  if (!g) return;
-  Profiler::Scope ps(*S.prof, "md_constraints");
-  launch<k_md_rattle, 64>(S.R, "k_md_rattle", tag, g, 64, 0, [&](uint32_t i) {
-    return args<k_md_rattle>(cons_args(*S.m[i]), (const float4*)S.m[i]->x.p, S.m[i]->v.p);
-  });
 }
 
 GroupArgs group_args(const dyn_md& M) {
@@ -967,6 +971,34 @@ double c_rescale_mu(const dyn_md& m, double p_bar, uint64_t step) {
   return std::exp(dlnv / 3.0);
 }
 
+StepArgs step_args(const dyn_md& m) {
+  StepArgs A;
+  std::memset(&A, 0, sizeof A);
+  A.F.nb = engine_forces_device(m.eng);
+  A.F.bd = m.fb.p;
+  A.F.n_sites = m.n_sites;
+  A.F.n_dyn = m.n_dyn;
+  A.acc_conv = m.acc_conv.p;
+  A.G = group_args(m);
+  A.C = cons_args(m);
+  A.h = 0.5f * m.prm.dt;
+  A.dt = m.prm.dt;
+  A.langevin = m.prm.gamma > 0.f ? 1 : 0;
+  A.lang_c = m.lang_c;
+  A.sigma_dyn = m.lang_sigma.p;
+  A.sigma_o = m.sigma_o;
+  A.sigma_h = m.sigma_h;
+  A.seed = m.prm.seed;
+  A.fb = m.fb.p;
+  return A;
+}
+
+unsigned group_grid(const MdSet& S) {
+  unsigned g = 1;
+  for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_sgroups + m->n_water, kBlock));
+  return g;
+}
+
 // Pressure coupling of every MD of the set at the end of a step's force
 // evaluation (energies on): measure, wait, rescale positions, velocities and
 // box (the engine takes the new box: a list rebuild and, with a mesh
@@ -982,9 +1014,7 @@ void md_barostat(MdSet& S) {
     if (!(std::isfinite(mu[i]) && mu[i] > 0.0) || std::fabs(mu[i] - 1.0) < 1e-12) mu[i] = 1.0;
   }
   Profiler::Scope ps(*S.prof, "md_rescale");
-  unsigned g = 1;
-  for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_sgroups + m->n_water, kBlock));
-  launch<k_md_rescale>(S.R, "k_md_rescale", 0, g, kBlock, 0, [&](uint32_t i) {
+  launch<k_md_rescale>(S.R, "k_md_rescale", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
     dyn_md& m = *S.m[i];
     return args<k_md_rescale>(group_args(m), m.x.p, m.v.p, (float)(mu[i] - 1.0),
                               (float)(1.0 / mu[i]));
@@ -1001,9 +1031,11 @@ void md_barostat(MdSet& S) {
 }
 
 // One step of every MD of the set, as src/integrate.rs:100-190 (γ > 0,
-// Langevin-middle) or :191-260 (γ = 0, velocity Verlet without thermostat).
-// Step fractions are in units of each system's dt; the scheme (Langevin or
-// not) must be the same for the whole set.
+// Langevin-middle) or :191-260 (γ = 0, velocity Verlet without thermostat):
+// the integration before the force evaluation in one launch, the last
+// half-kick after it in another.  Step fractions are in units of each
+// system's dt; the scheme (Langevin or not) must be the same for the whole
+// set.
 void md_one_step(MdSet& S, bool energies) {
   bool any_invalid = false;
   for (dyn_md* m : S.m) any_invalid = any_invalid || !m->forces_valid;
@@ -1012,37 +1044,25 @@ void md_one_step(MdSet& S, bool energies) {
   // nstpcouple and the step index)
   const uint32_t nst = S.m[0]->prm.nstpcouple;
   const bool couple = nst > 0 && (S.m[0]->step + 1) % nst == 0;
+  const bool langevin = S.m[0]->prm.gamma > 0.f;
   energies = energies || couple;
-  if (S.m[0]->prm.gamma > 0.f) {
-    md_integrate(S, 0, 0.5f, 0.5f, true, true);
-    md_constrain_positions(S, 0);
-    md_constrain_velocities(S, 0);
-    {
-      Profiler::Scope ps(*S.prof, "md_langevin");
-      unsigned g = 1;
-      for (dyn_md* m : S.m) g = std::max(g, grid_for(m->n_dyn + m->n_water, kBlock));
-      launch<k_md_langevin>(S.R, "k_md_langevin", 0, g, kBlock, 0, [&](uint32_t i) {
-        dyn_md& m = *S.m[i];
-        return args<k_md_langevin>(m.v.p, m.n_dyn, m.n_water, m.lang_c, m.lang_sigma.p, m.sigma_o,
-                                   m.sigma_h, m.prm.seed, (const uint64_t*)m.step_dev.p);
-      });
-    }
-    md_constrain_velocities(S, 1);
-    md_integrate(S, 1, 0.f, 0.5f, false, true);
-    md_constrain_positions(S, 1);
-    md_forces(S, energies);
-    if (couple) md_barostat(S);  // before the last half-kick (src/integrate.rs:172-192)
-    md_integrate(S, 2, 0.5f, 0.f, true, false);
-    md_constrain_velocities(S, 2);
-  } else {
-    md_integrate(S, 0, 0.5f, 1.0f, true, true);
-    md_constrain_positions(S, 0);
-    md_constrain_velocities(S, 0);
-    md_forces(S, energies);
-    md_integrate(S, 2, 0.5f, 0.f, true, false);
-    md_constrain_velocities(S, 2);
-    if (couple) md_barostat(S);  // at the end of the step (src/integrate.rs:241-310)
+  {
+    Profiler::Scope ps(*S.prof, "md_step_a");
+    launch<k_md_step_a>(S.R, "k_md_step_a", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+      dyn_md& m = *S.m[i];
+      return args<k_md_step_a>(step_args(m), m.x.p, m.v.p, (const uint64_t*)m.step_dev.p);
+    });
   }
+  md_forces(S, energies, /*fb_clear=*/false);
+  if (couple && langevin) md_barostat(S);  // before the last half-kick (src/integrate.rs:172-192)
+  {
+    Profiler::Scope ps(*S.prof, "md_step_b");
+    launch<k_md_step_b>(S.R, "k_md_step_b", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+      dyn_md& m = *S.m[i];
+      return args<k_md_step_b>(step_args(m), m.x.p, m.v.p, m.step_dev.p);
+    });
+  }
+  if (couple && !langevin) md_barostat(S);  // at the end of the step (src/integrate.rs:241-310)
   for (dyn_md* m : S.m) ++m->step;
 }
 

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "common.h"
+#include "wave.h"
 #include "dyn_math.h"
 #include "pme.h"
 
@@ -142,11 +143,61 @@ __device__ __forceinline__ void dft5(C* v) {
   v[3] = csub(a2, b2);
 }
 
+// One radix-R Stockham stage with compile-time sizes (line length N,
+// sub-transform length NS): the index arithmetic folds to shifts and masks.
+template <int SIGN, int N, int R, int NS, class C>
+__device__ __forceinline__ void fft_stage(const C* __restrict__ a, C* __restrict__ b, int nl,
+                                          const C* __restrict__ tw) {
+  constexpr int NR = N / R, TWS = N / (NS * R);
+  const int total = nl * NR;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int l = i / NR;
+    const int j = i - l * NR;
+    const int k = j % NS;
+    const C* src = a + l * N;
+    C* dst = b + l * N;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C x = src[j + r * NR];
+      if (r) x = ctw<SIGN>(x, tw[(r * k * TWS) % N]);
+      v[r] = x;
+    }
+    if constexpr (R == 4) dft4<SIGN>(v);
+    else dft2<SIGN>(v);
+    const int d = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[d + r * NS] = v[r];
+  }
+  __syncthreads();
+}
+
+// The same transform as lds_fft for the power-of-two lengths of common grids
+// (radix 4 first, then 2, the plan of make_fft_axis).
+template <int SIGN, int N, class C>
+__device__ __forceinline__ C* lds_fft_pow2(C* a, C* b, int nl, const C* __restrict__ tw) {
+  static_assert(N == 32 || N == 64 || N == 128, "fixed plans for 32, 64 and 128 points");
+  fft_stage<SIGN, N, 4, 1>(a, b, nl, tw);
+  fft_stage<SIGN, N, 4, 4>(b, a, nl, tw);
+  if constexpr (N == 32) {
+    fft_stage<SIGN, N, 2, 16>(a, b, nl, tw);
+    return b;
+  } else {
+    fft_stage<SIGN, N, 4, 16>(a, b, nl, tw);
+    if constexpr (N == 64) return b;
+    fft_stage<SIGN, N, 2, 64>(b, a, nl, tw);
+    return a;
+  }
+}
+
 // Stockham autosort FFT of nl lines of length N held in LDS as a[l·N + n].
 // Returns the buffer holding the (natural-order) result.
 template <int SIGN, class C>
 __device__ C* lds_fft(C* a, C* b, int nl, const FftAxis& ax, const C* __restrict__ tw) {
   const int N = ax.N;
+  if (N == 64) return lds_fft_pow2<SIGN, 64>(a, b, nl, tw);
+  if (N == 128) return lds_fft_pow2<SIGN, 128>(a, b, nl, tw);
+  if (N == 32) return lds_fft_pow2<SIGN, 32>(a, b, nl, tw);
   int Ns = 1;
   for (int s = 0; s < ax.nstages; ++s) {
     const int R = ax.radix[s];
@@ -380,17 +431,21 @@ __device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint3
                           uint32_t nbx, uint32_t nby, uint32_t nbz,
                           uint32_t* __restrict__ count, uint2* __restrict__ cb) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float4 p = xyzq[i];
-  const float xs[3] = {p.x, p.y, p.z};
-  int k0[3];
-  for (int a = 0; a < 3; ++a) {
-    float w;
-    frac_coord<float>(xs[a], S.sc[a], S.K[a], k0[a], w);
+  uint32_t b = 0xFFFFFFFFu;  // no brick (beyond n)
+  if (i < n) {
+    const float4 p = xyzq[i];
+    const float xs[3] = {p.x, p.y, p.z};
+    int k0[3];
+    for (int a = 0; a < 3; ++a) {
+      float w;
+      frac_coord<float>(xs[a], S.sc[a], S.K[a], k0[a], w);
+    }
+    const uint32_t nb[3] = {nbx, nby, nbz};
+    b = brick_of(k0, nb);
   }
-  const uint32_t nb[3] = {nbx, nby, nbz};
-  const uint32_t b = brick_of(k0, nb);
-  cb[i] = make_uint2(b, atomicAdd(&count[b], 1u));
+  // one atomic per distinct brick of the wave (neighbouring charges share bricks)
+  const uint32_t rank = wave_rank(b, i < n, count);
+  if (i < n) cb[i] = make_uint2(b, rank);
 }
 
 // Single-workgroup exclusive scan of the brick counts into start[0..n]; the
