@@ -771,23 +771,29 @@ k_eng_pairs(PairArgs A) {
     T.fi[0] = T.fi[1] = T.fi[2] = R(0);
     T.e_lj_s = T.e_lj_w = T.e_c_s = T.e_c_w = T.vir = T.dh = R(0);
     T.n_ss = T.n_lj = T.n_c = 0;
+    // two-deep software pipeline: while group g is evaluated, the j data of
+    // group g + 1 and the entry of group g + 2 are in flight.  The loads are
+    // issued on every iteration (the last ones repeat the row's last group)
+    // so the wait counts stay exact.
+    const uint32_t glast = g1 - 1;
     uint32_t v = A.cj[(size_t)g0 * DYN_CL_SIZE + jb];
-    uint32_t js = v & DYN_JSLOT_MASK;
-    float4 pj = A.xyzq[js];
-    int tj = A.slot_type[js];
-    int gj = A.slot_gid[js];
+    uint32_t vn = A.cj[(size_t)min(g0 + 1, glast) * DYN_CL_SIZE + jb];
+    float4 pj = A.xyzq[v & DYN_JSLOT_MASK];
+    int tj = A.slot_type[v & DYN_JSLOT_MASK];
+    int gj = A.slot_gid[v & DYN_JSLOT_MASK];
     for (uint32_t g = g0; g < g1; ++g) {
       // readfirstlane returns a signed int: shift the entry as unsigned (class
       // 2 sets the top bit)
       const int cls = (int)((uint32_t)__builtin_amdgcn_readfirstlane(v) >> DYN_JCLS_SHIFT);
       const float4 pjc = pj;
       const int tjc = tj, gjc = gj;
-      if (g + 1 < g1) {
-        v = A.cj[(size_t)(g + 1) * DYN_CL_SIZE + jb];
-        js = v & DYN_JSLOT_MASK;
-        pj = A.xyzq[js];
-        tj = A.slot_type[js];
-        gj = A.slot_gid[js];
+      {
+        const uint32_t jsn = vn & DYN_JSLOT_MASK;
+        pj = A.xyzq[jsn];
+        tj = A.slot_type[jsn];
+        gj = A.slot_gid[jsn];
+        v = vn;
+        vn = A.cj[(size_t)min(g + 2, glast) * DYN_CL_SIZE + jb];
       }
       const int kind = dyn_kind(ci, cls);
       bool active = dyn_slot_pair_valid(gi, gjc);
@@ -986,15 +992,17 @@ k_eng_corr(CorrArgs A) {
     atomicAdd(reinterpret_cast<unsigned long long*>(A.fx + 2 * (size_t)A.n_sites + j), (unsigned long long)(-q2));
   }
   if (A.mol_e) wave_add_fx(A.mol_e, mkey, me);
-  for (int v = 0; v < kCorrVals; ++v) {
-    red[threadIdx.x] = acc[v];
-    __syncthreads();
-    for (int s = kCorrBlock / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) A.partials[kCorrVals * blockIdx.x + v] = red[0];
-    __syncthreads();
+  if (!A.partials) return;  // forces only
+  // wave sums by shuffles, then the block's waves in a fixed order
+  for (int v = 0; v < kCorrVals; ++v)
+    for (int m = kWave / 2; m >= 1; m >>= 1) acc[v] += __shfl_xor(acc[v], m, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0)
+    for (int v = 0; v < kCorrVals; ++v) red[v * (kCorrBlock / kWave) + threadIdx.x / kWave] = acc[v];
+  __syncthreads();
+  if (threadIdx.x < kCorrVals) {
+    double t = 0.0;
+    for (int w = 0; w < kCorrBlock / kWave; ++w) t += red[threadIdx.x * (kCorrBlock / kWave) + w];
+    A.partials[kCorrVals * blockIdx.x + threadIdx.x] = t;
   }
 }
 
@@ -1736,7 +1744,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       C.nblk = e.corr_blocks;
       C.fx = e.fx_site.p;
       C.n_sites = e.n_sites;
-      C.partials = e.corr_part.p;
+      C.partials = energy_on ? e.corr_part.p : nullptr;
       C.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
       C.mol_of = e.mol_of.p;
       C.n_mol = e.n_mol;

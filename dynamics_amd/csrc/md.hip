@@ -323,7 +323,7 @@ __device__ __forceinline__ void add_f(int64_t* f, uint32_t nd, uint32_t i, V3 v)
 // term is taken relative to one of its atoms (src/bonded.rs:41-46, :79-83,
 // :124-130; the same sign convention for every term).
 __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
-  __shared__ double red[4][kBlock];
+  __shared__ double red[4][kBlock / 64];
   if (blockIdx.x >= A.nblk) return;  // partials exist for the system's own blocks
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   double e[4] = {0.0, 0.0, 0.0, 0.0};
@@ -394,15 +394,18 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
              (double)dot(b2 + b3, gl * -dedphi);
     }
   }
-  for (int c = 0; c < 4; ++c) red[c][threadIdx.x] = e[c];
+  if (!A.partials) return;  // forces only
+  // wave sums by shuffles, then the block's waves in a fixed order
+  for (int c = 0; c < 4; ++c)
+    for (int m = 32; m >= 1; m >>= 1) e[c] += __shfl_xor(e[c], m, 64);
+  if ((threadIdx.x & 63) == 0)
+    for (int c = 0; c < 4; ++c) red[c][threadIdx.x / 64] = e[c];
   __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s)
-      for (int c = 0; c < 4; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-    __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += red[threadIdx.x][w];
+    A.partials[4 * blockIdx.x + threadIdx.x] = t;
   }
-  if (threadIdx.x == 0)
-    for (int c = 0; c < 4; ++c) A.partials[4 * blockIdx.x + c] = red[c][0];
 }
 
 __device__ __forceinline__ void k_md_zero(int64_t* __restrict__ f, size_t n) {
@@ -876,7 +879,7 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
       A.dihe = m.dihe.p;
       A.dihe_p = m.dihe_p.p;
       A.f = m.fb.p;
-      A.partials = m.bpart.p;
+      A.partials = energies ? m.bpart.p : nullptr;
       A.nblk = m.bblocks;
       return args<k_md_bonded>(A);
     });
