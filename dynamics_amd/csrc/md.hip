@@ -69,14 +69,23 @@ __device__ __forceinline__ V3 min_image(V3 d, const Box& b) {
           dyn_min_image_ref(d.z, b.L[2])};
 }
 __device__ __forceinline__ float wrap_coord(float x, float L) {
-  float r = fmodf(x, L);
-  if (r < 0.0f) r += L;
+  // a coordinate within one box length of the box: one exact add or
+  // subtract (the same result as fmod); anything farther takes fmod
+  float r = x;
+  if (r >= L) r -= L;
+  else if (r < 0.0f) r += L;
+  if (!(r >= 0.0f && r < L)) {
+    r = fmodf(x, L);
+    if (r < 0.0f) r += L;
+  }
   if (!(r < L)) r = 0.0f;
   return r;
 }
 
 // Counter-based normal deviates: SplitMix64 finalizer over (seed, step,
-// index), Box–Muller on two 24-bit uniforms.
+// index), Box–Muller on two 24-bit uniforms, with the hardware log2, square
+// root and cosine (v_cos_f32 takes its argument in turns): a thermostat's
+// noise needs the distribution, not the last bits.
 __device__ __host__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -87,7 +96,9 @@ __device__ __forceinline__ float normal_dev(uint64_t seed, uint64_t step, uint64
   const uint64_t h = mix64(mix64(mix64(seed) + step) + idx);
   const float u1 = ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
   const float u2 = ((float)((h >> 16) & 0xFFFFFFull) + 0.5f) * (1.0f / 16777216.0f);
-  return sqrtf(-2.0f * logf(u1)) * cosf(6.283185307179586f * u2);
+  // −2 ln u1 = −2 ln 2 · log2 u1
+  return __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1)) *
+         __builtin_amdgcn_cosf(u2);
 }
 
 // The same deviate in double on the host (the barostat's noise).
@@ -124,7 +135,18 @@ __device__ __forceinline__ V3 rodrigues(V3 r, V3 w, float dt) {
     return r + wxr + cross(wd, wxr) * 0.5f;
   }
   const V3 n = wd * (1.0f / th);
-  const float c = cosf(th), s = sinf(th);
+  float c, s;
+  if (th < 0.5f) {
+    // Taylor series to th⁹ / th⁸: below 3e-10 absolute for th < 0.5
+    const float t2 = th * th;
+    s = th * (1.0f - t2 * (1.0f / 6.0f) * (1.0f - t2 * (1.0f / 20.0f) *
+                                                   (1.0f - t2 * (1.0f / 42.0f) * (1.0f - t2 * (1.0f / 72.0f)))));
+    c = 1.0f - t2 * 0.5f * (1.0f - t2 * (1.0f / 12.0f) *
+                                      (1.0f - t2 * (1.0f / 30.0f) * (1.0f - t2 * (1.0f / 56.0f))));
+  } else {
+    c = cosf(th);
+    s = sinf(th);
+  }
   return r * c + cross(n, r) * s + n * (dot(n, r) * (1.0f - c));
 }
 
@@ -132,12 +154,16 @@ __device__ __forceinline__ V3 rodrigues(V3 r, V3 w, float dt) {
 // rotation whose angular velocity reproduces the molecule's angular
 // momentum; velocities become the rigid-body ones; O is wrapped, the H
 // sites follow it, and M is placed on the H–O–H bisector.
-__device__ void rigid_drift(float4* __restrict__ x, float4* __restrict__ v, uint32_t base,
-                            float dt, const Box& B) {
-  const V3 o = ld(x[base]);
-  const V3 h0 = o + min_image(ld(x[base + 2]) - o, B);
-  const V3 h1 = o + min_image(ld(x[base + 3]) - o, B);
-  const V3 vo = ld(v[base]), vh0 = ld(v[base + 2]), vh1 = ld(v[base + 3]);
+// One water's sites in registers: O, M, H0, H1 positions and velocities.
+struct WaterRegs {
+  float4 x[4], v[4];
+};
+
+__device__ __forceinline__ void rigid_drift(WaterRegs& W, float dt, const Box& B) {
+  const V3 o = ld(W.x[0]);
+  const V3 h0 = o + min_image(ld(W.x[2]) - o, B);
+  const V3 h1 = o + min_image(ld(W.x[3]) - o, B);
+  const V3 vo = ld(W.v[0]), vh0 = ld(W.v[2]), vh1 = ld(W.v[3]);
   const V3 rc = (o * kMassO + h0 * kMassH + h1 * kMassH) * (1.0f / kMassW);
   const V3 vc = (vo * kMassO + vh0 * kMassH + vh1 * kMassH) * (1.0f / kMassW);
   const V3 rO = o - rc, rH0 = h0 - rc, rH1 = h1 - rc;
@@ -176,16 +202,16 @@ __device__ void rigid_drift(float4* __restrict__ x, float4* __restrict__ v, uint
   const V3 bis = (wh0 - wo) + (wh1 - wo);
   const float bl = sqrtf(dot(bis, bis));
   const V3 wm = bl > 0.0f ? wo + bis * (kOEpR / bl) : wo;
-  x[base] = make_float4(wo.x, wo.y, wo.z, 0.f);
-  x[base + 1] = make_float4(wm.x, wm.y, wm.z, 0.f);
-  x[base + 2] = make_float4(wh0.x, wh0.y, wh0.z, 0.f);
-  x[base + 3] = make_float4(wh1.x, wh1.y, wh1.z, 0.f);
+  W.x[0] = make_float4(wo.x, wo.y, wo.z, 0.f);
+  W.x[1] = make_float4(wm.x, wm.y, wm.z, 0.f);
+  W.x[2] = make_float4(wh0.x, wh0.y, wh0.z, 0.f);
+  W.x[3] = make_float4(wh1.x, wh1.y, wh1.z, 0.f);
   const V3 vO2 = vc + cross(w, rO2), vH02 = vc + cross(w, rH02), vH12 = vc + cross(w, rH12);
-  v[base] = make_float4(vO2.x, vO2.y, vO2.z, 0.f);
-  v[base + 2] = make_float4(vH02.x, vH02.y, vH02.z, 0.f);
-  v[base + 3] = make_float4(vH12.x, vH12.y, vH12.z, 0.f);
+  W.v[0] = make_float4(vO2.x, vO2.y, vO2.z, 0.f);
+  W.v[2] = make_float4(vH02.x, vH02.y, vH02.z, 0.f);
+  W.v[3] = make_float4(vH12.x, vH12.y, vH12.z, 0.f);
   const V3 vm = (vH02 + vH12) * 0.5f;
-  v[base + 1] = make_float4(vm.x, vm.y, vm.z, 0.f);
+  W.v[1] = make_float4(vm.x, vm.y, vm.z, 0.f);
 }
 
 // Kick of solute atom t: v += a·h from the current forces, the acceleration
@@ -229,6 +255,24 @@ __device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h,
     vv.y += fs[k].y * h;
     vv.z += fs[k].z * h;
     v[ss[k]] = vv;
+  }
+}
+
+// The same kick on a water held in registers (O at site base).
+__device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h, WaterRegs& W) {
+  const V3 fm = force_of(F, base + 1);
+  const V3 fo = force_of(F, base) + fm * kCO;
+  const V3 f0 = force_of(F, base + 2) + fm * kCH;
+  const V3 f1 = force_of(F, base + 3) + fm * kCH;
+  const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
+  const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
+  const int ks[3] = {0, 2, 3};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float4& vv = W.v[ks[k]];
+    vv.x += fs[k].x * h;
+    vv.y += fs[k].y * h;
+    vv.z += fs[k].z * h;
   }
 }
 
@@ -293,6 +337,97 @@ __device__ __forceinline__ void rattle_cluster(const ConsArgs& C, uint32_t c,
     v[p.x] = make_float4(va.x + corr.x * mi, va.y + corr.y * mi, va.z + corr.z * mi, va.w);
     v[p.y] = make_float4(vb.x - corr.x * mj, vb.y - corr.y * mj, vb.z - corr.z * mj, vb.w);
   }
+}
+
+// A solute pressure group the step kernels keep in registers: a constraint
+// cluster that is a star of at most 4 constraints around one atom (bonds to
+// hydrogen) — atom[0] the centre, atom[k+1] the other atom of constraint k
+// (cluster order), bit k of flip set when constraint k lists the spoke first
+// — or a lone atom (n = 0).  natoms = 0: another cluster (the general path).
+constexpr int kStarMax = 4;
+struct StarDesc {
+  uint32_t atom[kStarMax + 1];
+  uint32_t n, flip, natoms;
+};
+
+// SHAKE and RATTLE of a star cluster held in registers: the same operations,
+// in the same order, as shake_cluster / rattle_cluster.
+__device__ __forceinline__ void shake_star(const ConsArgs& C, uint32_t c, const StarDesc& D,
+                                           const float (&im)[kStarMax + 1],
+                                           float4 (&X)[kStarMax + 1]) {
+  const uint32_t k0 = C.cl_ptr[c];
+  for (int it = 0; it < kShakeMaxIt; ++it) {
+    float max_corr = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kStarMax; ++k) {
+      if (k < (int)D.n) {
+        const bool f = (D.flip >> k) & 1u;
+        const float4 a = f ? X[k + 1] : X[0], b = f ? X[0] : X[k + 1];
+        const float mi = f ? im[k + 1] : im[0], mj = f ? im[0] : im[k + 1];
+        const V3 d = v3(b.x - a.x, b.y - a.y, b.z - a.z);
+        const float d2 = dot(d, d);
+        const float lam = (d2 - C.r0sq[k0 + k]) / (2.0f * C.inv_m[k0 + k] * fmaxf(d2, kEpsConstr));
+        const V3 corr = d * lam;
+        const float4 na = make_float4(a.x + corr.x * mi, a.y + corr.y * mi, a.z + corr.z * mi, a.w);
+        const float4 nb = make_float4(b.x - corr.x * mj, b.y - corr.y * mj, b.z - corr.z * mj, b.w);
+        X[0] = f ? nb : na;
+        X[k + 1] = f ? na : nb;
+        max_corr = fmaxf(max_corr, sqrtf(dot(corr, corr)));
+      }
+    }
+    if (max_corr < C.tol) break;
+  }
+}
+
+__device__ __forceinline__ void rattle_star(const ConsArgs& C, uint32_t c, const StarDesc& D,
+                                            const float (&im)[kStarMax + 1],
+                                            const float4 (&X)[kStarMax + 1],
+                                            float4 (&V)[kStarMax + 1]) {
+  const uint32_t k0 = C.cl_ptr[c];
+#pragma unroll
+  for (int k = 0; k < kStarMax; ++k) {
+    if (k < (int)D.n) {
+      const bool f = (D.flip >> k) & 1u;
+      const float4 a = f ? X[k + 1] : X[0], b = f ? X[0] : X[k + 1];
+      const float4 va = f ? V[k + 1] : V[0], vb = f ? V[0] : V[k + 1];
+      const float mi = f ? im[k + 1] : im[0], mj = f ? im[0] : im[k + 1];
+      const V3 r = v3(b.x - a.x, b.y - a.y, b.z - a.z);
+      const V3 dv = v3(vb.x - va.x, vb.y - va.y, vb.z - va.z);
+      const float lam = dot(dv, r) / (C.inv_m[k0 + k] * fmaxf(dot(r, r), kEpsConstr));
+      const V3 corr = r * lam;
+      const float4 na = make_float4(va.x + corr.x * mi, va.y + corr.y * mi, va.z + corr.z * mi, va.w);
+      const float4 nb = make_float4(vb.x - corr.x * mj, vb.y - corr.y * mj, vb.z - corr.z * mj, vb.w);
+      V[0] = f ? nb : na;
+      V[k + 1] = f ? na : nb;
+    }
+  }
+}
+
+// Kick of a solute atom's velocity held in a register.
+__device__ __forceinline__ float4 kick_value(const Fx& F, const float* __restrict__ acc_conv,
+                                             uint32_t t, float h, float4 vi) {
+  V3 a = force_of(F, t) * acc_conv[t];
+  const float a2 = dot(a, a);
+  if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
+  vi.x += a.x * h;
+  vi.y += a.y * h;
+  vi.z += a.z * h;
+  return vi;
+}
+
+__device__ __forceinline__ float4 drift_value(float4 xi, float4 vi, float h) {
+  xi.x += vi.x * h;
+  xi.y += vi.y * h;
+  xi.z += vi.z * h;
+  return xi;
+}
+
+__device__ __forceinline__ float4 langevin_value(float4 vv, uint32_t s, float c, float sig,
+                                                 uint64_t seed, uint64_t step) {
+  vv.x = c * vv.x + sig * normal_dev(seed, step, 3ull * s + 0);
+  vv.y = c * vv.y + sig * normal_dev(seed, step, 3ull * s + 1);
+  vv.z = c * vv.z + sig * normal_dev(seed, step, 3ull * s + 2);
+  return vv;
 }
 
 struct BondedArgs {
@@ -670,6 +805,7 @@ struct StepArgs {
   float sigma_o, sigma_h;
   uint64_t seed;
   int64_t* fb;            // bonded force sums to clear for the evaluation
+  const StarDesc* stars;  // per solute pressure group
 };
 
 // Before the force evaluation.  Langevin-middle (src/integrate.rs:100-160):
@@ -683,6 +819,48 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= A.G.n_sgroups + A.G.n_water) return;
   const float hd = A.langevin ? A.h : A.dt;
+  if (g < A.G.n_sgroups && A.stars[g].natoms) {
+    // a star cluster or a lone atom: the whole sequence in registers
+    const StarDesc D = A.stars[g];
+    const bool cons = D.n > 0;
+    float4 X[kStarMax + 1], V[kStarMax + 1];
+    float im[kStarMax + 1];
+#pragma unroll
+    for (int k = 0; k <= kStarMax; ++k) {
+      const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
+      X[k] = x[t];
+      V[k] = kick_value(A.F, A.acc_conv, t, A.h, v[t]);
+      X[k] = drift_value(X[k], V[k], hd);
+      im[k] = 1.0f / A.C.mass[t];
+    }
+    if (cons) {
+      shake_star(A.C, g, D, im, X);
+      rattle_star(A.C, g, D, im, X, V);
+    }
+    if (A.langevin) {
+      const uint64_t step = *step_dev;
+#pragma unroll
+      for (int k = 0; k <= kStarMax; ++k)
+        if (k <= (int)D.n)
+          V[k] = langevin_value(V[k], D.atom[k], A.lang_c, A.sigma_dyn[D.atom[k]], A.seed, step);
+      if (cons) rattle_star(A.C, g, D, im, X, V);
+#pragma unroll
+      for (int k = 0; k <= kStarMax; ++k) X[k] = drift_value(X[k], V[k], A.h);
+      if (cons) shake_star(A.C, g, D, im, X);
+    }
+    const uint32_t nd = A.G.n_dyn;
+#pragma unroll
+    for (int k = 0; k <= kStarMax; ++k)
+      if (k <= (int)D.n) {
+        const uint32_t t = D.atom[k];
+        x[t] = X[k];
+        v[t] = V[k];
+        A.fb[t] = 0;
+        A.fb[nd + t] = 0;
+        A.fb[2 * (size_t)nd + t] = 0;
+      }
+    return;
+  }
   if (g < A.G.n_sgroups) {
     const uint32_t a0 = A.G.grp_ptr[g], a1 = A.G.grp_ptr[g + 1];
     const bool cons = g < A.C.n_clusters;
@@ -715,14 +893,25 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
     return;
   }
   const uint32_t base = A.G.n_dyn + 4u * (g - A.G.n_sgroups);
-  kick_water(A.F, base, A.h, v);
-  rigid_drift(x, v, base, hd, A.G.B);
+  WaterRegs W;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    W.x[k] = x[base + k];
+    W.v[k] = v[base + k];
+  }
+  kick_water(A.F, base, A.h, W);
+  rigid_drift(W, hd, A.G.B);
   if (A.langevin) {
     const uint64_t step = *step_dev;
-    langevin_site(v, base, A.lang_c, A.sigma_o, A.seed, step);
-    langevin_site(v, base + 2, A.lang_c, A.sigma_h, A.seed, step);
-    langevin_site(v, base + 3, A.lang_c, A.sigma_h, A.seed, step);
-    rigid_drift(x, v, base, A.h, A.G.B);
+    W.v[0] = langevin_value(W.v[0], base, A.lang_c, A.sigma_o, A.seed, step);
+    W.v[2] = langevin_value(W.v[2], base + 2, A.lang_c, A.sigma_h, A.seed, step);
+    W.v[3] = langevin_value(W.v[3], base + 3, A.lang_c, A.sigma_h, A.seed, step);
+    rigid_drift(W, A.h, A.G.B);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    x[base + k] = W.x[k];
+    v[base + k] = W.v[k];
   }
 }
 
@@ -734,6 +923,23 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g == 0) ++*step_inc;
   if (g >= A.G.n_sgroups + A.G.n_water) return;
+  if (g < A.G.n_sgroups && A.stars[g].natoms) {
+    const StarDesc D = A.stars[g];
+    float4 X[kStarMax + 1], V[kStarMax + 1];
+    float im[kStarMax + 1];
+#pragma unroll
+    for (int k = 0; k <= kStarMax; ++k) {
+      const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
+      X[k] = x[t];
+      V[k] = kick_value(A.F, A.acc_conv, t, A.h, v[t]);
+      im[k] = 1.0f / A.C.mass[t];
+    }
+    if (D.n) rattle_star(A.C, g, D, im, X, V);
+#pragma unroll
+    for (int k = 0; k <= kStarMax; ++k)
+      if (k <= (int)D.n) v[D.atom[k]] = V[k];
+    return;
+  }
   if (g < A.G.n_sgroups) {
     for (uint32_t a = A.G.grp_ptr[g]; a < A.G.grp_ptr[g + 1]; ++a)
       kick_atom(A.F, A.acc_conv, A.G.grp_atoms[a], A.h, v);
@@ -773,6 +979,7 @@ struct dyn_md {
   DevBuf<double> bpart;
   unsigned bblocks = 1;
   uint32_t n_clusters = 0, n_cons = 0;
+  DevBuf<StarDesc> stars;     // per cluster (n = 0: not a star)
   DevBuf<uint32_t> cl_ptr;
   DevBuf<uint2> cons_ij;
   DevBuf<float> cons_r0sq, cons_invm;
@@ -993,6 +1200,7 @@ StepArgs step_args(const dyn_md& m) {
   A.sigma_h = m.sigma_h;
   A.seed = m.prm.seed;
   A.fb = m.fb.p;
+  A.stars = m.stars.p;
   return A;
 }
 
@@ -1266,6 +1474,43 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     }
     M->n_clusters = (uint32_t)groups.size();
     M->n_cons = n_constraints;
+    // star clusters (≤ 4 constraints sharing one atom) and lone atoms run in
+    // registers; the descriptors of the lone atoms are added with the
+    // pressure groups below
+    std::vector<StarDesc> st(groups.size());
+    {
+      for (size_t c = 0; c < groups.size(); ++c) {
+        StarDesc& D = st[c];
+        std::memset(&D, 0, sizeof D);
+        const auto& gq = groups[c];
+        if (gq.size() > (size_t)kStarMax) continue;
+        const uint32_t a = constraint_ij[2 * gq[0]], b = constraint_ij[2 * gq[0] + 1];
+        for (uint32_t center : {a, b}) {
+          bool ok = true;
+          for (uint32_t k : gq)
+            ok = ok && (constraint_ij[2 * k] == center || constraint_ij[2 * k + 1] == center);
+          // the spokes must be distinct atoms (no constraint listed twice)
+          for (size_t u = 0; ok && u < gq.size(); ++u)
+            for (size_t w = 0; w < u; ++w) {
+              const uint32_t su = constraint_ij[2 * gq[u]] == center ? constraint_ij[2 * gq[u] + 1]
+                                                                     : constraint_ij[2 * gq[u]];
+              const uint32_t sw = constraint_ij[2 * gq[w]] == center ? constraint_ij[2 * gq[w] + 1]
+                                                                     : constraint_ij[2 * gq[w]];
+              ok = ok && su != sw;
+            }
+          if (!ok) continue;
+          D.atom[0] = center;
+          for (size_t k = 0; k < gq.size(); ++k) {
+            const uint32_t i = constraint_ij[2 * gq[k]], j = constraint_ij[2 * gq[k] + 1];
+            D.atom[k + 1] = i == center ? j : i;
+            if (i != center) D.flip |= 1u << k;
+          }
+          D.n = (uint32_t)gq.size();
+          D.natoms = D.n + 1;
+          break;
+        }
+      }
+    }
     // pressure groups: the atoms of each constraint cluster, then every atom
     // in no constraint alone
     {
@@ -1284,8 +1529,15 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
         if (!in_cons[a]) {
           gat.push_back(a);
           gptr.push_back((uint32_t)gat.size());
+          StarDesc D;
+          std::memset(&D, 0, sizeof D);
+          D.atom[0] = a;
+          D.natoms = 1;
+          st.push_back(D);
         }
       M->n_sgroups = (uint32_t)gptr.size() - 1;
+      if (st.empty()) st.resize(1);
+      up(M->stars, st.data(), st.size(), sizeof(StarDesc));
       up(M->grp_ptr, gptr.data(), gptr.size(), 4);
       up(M->grp_atoms, gat.data(), gat.size(), 4);
       M->pblocks = grid_for(std::max<uint32_t>(M->n_sgroups + nw, 1), kBlock);
