@@ -374,30 +374,6 @@ k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
   }
 }
 
-__device__ __forceinline__ void k_pme_finalize(const double* __restrict__ partials, int nb,
-                               double* __restrict__ res) {
-  __shared__ double red[2][256];
-  if (!res) return;  // a system that skips this pass
-  double e = 0, w = 0;
-  for (int b = threadIdx.x; b < nb; b += 256) {
-    e += partials[2 * b];
-    w += partials[2 * b + 1];
-  }
-  red[0][threadIdx.x] = e;
-  red[1][threadIdx.x] = w;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    res[0] = red[0][0];
-    res[1] = red[1][0];
-  }
-}
 
 struct SplineArgs {
   int K[3];
@@ -600,13 +576,44 @@ __device__ __forceinline__ int64_t fx_of(R f) {
   else return dyn_fx64(f);
 }
 
+constexpr int kGatherBlock = 128;
 template <int ORDER, class C>
 __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
                              const C* __restrict__ phi, const uint32_t* __restrict__ site,
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
-                             float* __restrict__ fout, float fscale) {
+                             float* __restrict__ fout, float fscale,
+                             const double* __restrict__ partials, int nbp,
+                             double* __restrict__ res) {
   using R = typename CxT<C>::R;
   constexpr bool F32 = std::is_same<R, float>::value;
+  if (res && blockIdx.x == 0) {
+    // the first workgroup also sums the convolution's energy and virial
+    // partials (fixed order: strided per thread, then waves, then lanes)
+    __shared__ double red[2][kGatherBlock / 64];
+    double e = 0.0, w = 0.0;
+    for (int b = threadIdx.x; b < nbp; b += kGatherBlock) {
+      e += partials[2 * b];
+      w += partials[2 * b + 1];
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+      e += __shfl_xor(e, m, 64);
+      w += __shfl_xor(w, m, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x / 64] = e;
+      red[1][threadIdx.x / 64] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double te = 0.0, tw = 0.0;
+      for (int k = 0; k < kGatherBlock / 64; ++k) {
+        te += red[0][k];
+        tw += red[1][k];
+      }
+      res[0] = te;
+      res[1] = tw;
+    }
+  }
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 p = xyzq[i];
@@ -1028,21 +1035,14 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
     }
   }
   {
-    Profiler::Scope ps(prof, "pme_finalize");
-    launch<k_pme_finalize>(R, "k_pme_finalize", tag, 1, 256, 0, [&](uint32_t i) {
-      return args<k_pme_finalize>(on(i) ? cp(i)->partials.p : nullptr,
-                                  on(i) ? (int)cp(i)->conv_blocks : 0,
-                                  on(i) ? runs[i].res : nullptr);
-    });
-  }
-  {
     Profiler::Scope ps(prof, "pme_gather");
-    const unsigned gg = std::max(1u, (g_n * 256 + 127) / 128);
+    const unsigned gg = std::max(1u, (g_n * 256 + kGatherBlock - 1) / kGatherBlock);
     auto gather = [&](auto F, auto Ctag, int order) {
       using C = typename decltype(Ctag)::type;
       constexpr bool F64 = std::is_same<C, double2>::value;
-      launch<decltype(F)::value, 128>(R, "k_pme_gather", tag * 4 + (F64 ? 2 : 0) + (order == 5),
-                                      gg, 128, 0, [&](uint32_t i) {
+      launch<decltype(F)::value, kGatherBlock>(R, "k_pme_gather",
+                                               tag * 4 + (F64 ? 2 : 0) + (order == 5), gg,
+                                               kGatherBlock, 0, [&](uint32_t i) {
         const bool m = mine(i, F64, order);
         const C* phi = nullptr;
         if (m) {
@@ -1051,7 +1051,10 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
         }
         return args<decltype(F)::value>(runs[i].xyzq, m ? nq(i) : 0u, S[i], phi, runs[i].site,
                                         runs[i].fx, runs[i].comp_major, runs[i].nsites,
-                                        runs[i].f_out, runs[i].fscale);
+                                        runs[i].f_out, runs[i].fscale,
+                                        m ? (const double*)cp(i)->partials.p : nullptr,
+                                        m ? (int)cp(i)->conv_blocks : 0,
+                                        m ? runs[i].res : nullptr);
       });
     };
     if (ord[0][0]) gather(DYN_IC((k_pme_gather<4, float2>)), TF{}, 4);
