@@ -54,7 +54,8 @@ constexpr int kMaxColsPerClass = 4096;
 constexpr int kNumPart = 9;           // f64 partials per work item of the pair kernel
 
 // status bits
-constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u;
+constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u,
+                   kStPme = 32u;  // an SPME brick list overflowed
 
 struct ClassGrid {
   int ncx, ncy;
@@ -1061,7 +1062,26 @@ __device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fit
   const uint32_t I = k / DYN_CL_SIZE, ia = k % DYN_CL_SIZE;
   const uint64_t c0 = min((uint64_t)chunk_ptr[I], cap_chunks);
   const uint64_t c1 = min((uint64_t)chunk_ptr[I + 1], cap_chunks);
-  for (uint64_t c = c0; c < c1; ++c) {
+  uint64_t c = c0;
+  // four rows per round: their twelve loads are independent and issued
+  // together (integer sums: the order does not change the result)
+  for (; c + 4 <= c1; c += 4) {
+    int64_t v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t* o = fitem + (c + u) * (3 * DYN_CL_SIZE);
+      v[u][0] = o[ia];
+      v[u][1] = o[DYN_CL_SIZE + ia];
+      v[u][2] = o[2 * DYN_CL_SIZE + ia];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      f[0] += v[u][0];
+      f[1] += v[u][1];
+      f[2] += v[u][2];
+    }
+  }
+  for (; c < c1; ++c) {
     const int64_t* o = fitem + c * (3 * DYN_CL_SIZE);
     f[0] += o[ia];
     f[1] += o[DYN_CL_SIZE + ia];
@@ -1764,10 +1784,12 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       r.xyzq = e.pme_xyzq.p;
       r.site = e.qsite.p;
       r.fx = e.fx_recip.p;
-      r.comp_major = 1;
+      r.comp_major = 3;  // component-major, one charge per site
       r.nsites = e.n_sites;
       r.fscale = pl.recip_b ? 0.5f * (1.0f + s) : 1.0f;
       r.res = e.eng_out.p + 16;
+      r.status = e.status.p;
+      r.status_bit = kStPme;
       ra[i] = r;
       if (pl.recip_b) {
         r.xyzq = e.pme_b.p;
@@ -1847,6 +1869,12 @@ void dyn_engine::check_status() {
   DYN_HIP(hipMemsetAsync(status.p, 0, 4, stream));
   DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
   list_valid = false;
+  if (st & kStPme) {
+    pme.grow();
+    if (st == kStPme)
+      throw Error(DYN_ERR_CAPACITY, fmt("an SPME brick list overflowed; lists enlarged to %u "
+                                        "charges per brick", pme.cap_brick));
+  }
   if (st & kStColumn)
     throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
   if (st & kStMask)

@@ -1046,11 +1046,6 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
   uint32_t flags = DYN_COMPUTE_ALL | (energies ? 0u : DYN_COMPUTE_NO_ENERGY);
   if (cadence_skip) flags = (flags & ~DYN_COMPUTE_RECIP) | DYN_COMPUTE_RECIP_REUSE;
   for (dyn_md* m : S.m) ++m->n_eval;
-  if (first)
-    engine_set_compute_sync(S.es, flags);
-  else
-    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC);
-  Profiler::Scope ps(*S.prof, "md_bonded");
   if (fb_clear) {  // else the step's first launch has cleared the sums
     const unsigned gz = [&] {
       unsigned g = 1;
@@ -1068,7 +1063,12 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
     any = any || (m->n_bond + m->n_angle + m->n_dihe) > 0;
     gb = std::max(gb, m->bblocks);
   }
-  if (any)
+  if (first)
+    engine_set_compute_sync(S.es, flags);
+  else
+    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC);
+  if (any) {
+    Profiler::Scope ps(*S.prof, "md_bonded");
     launch<k_md_bonded, kBlock>(S.R, "k_md_bonded", 0, gb, kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
       BondedArgs A;
@@ -1090,6 +1090,7 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
       A.nblk = m.bblocks;
       return args<k_md_bonded>(A);
     });
+  }
   for (dyn_md* m : S.m) m->forces_valid = true;
 }
 

@@ -57,13 +57,17 @@ struct PmeCore {
   PinnedBuf<double> h_part;
   unsigned conv_blocks = 0;
   DevBuf<double> res_dev;       // [energy, virial] reduced on the device
-  // Spreading by grid brick (8³ points): charges are counted per home brick
-  // (the brick of their base grid point), sorted by it, and each brick
-  // gathers the charges of the home bricks whose stencils can reach it.
+  // Spreading by grid brick (8³ points): each charge is listed under its
+  // home brick (the brick of its base grid point) in a fixed-capacity list,
+  // and each brick gathers the charges of the home bricks whose stencils can
+  // reach it.
   uint32_t nbk[3] = {0, 0, 0};  // bricks per axis
-  DevBuf<uint32_t> brick_count, brick_start;  // per brick (+1)
-  DevBuf<uint2> charge_brick;   // per charge: home brick, rank within it
-  DevBuf<uint32_t> brick_items; // charges sorted by home brick
+  DevBuf<uint32_t> brick_count;  // per brick: charges listed (zeroed after use)
+  DevBuf<uint32_t> brick_items;  // cap_brick slots per brick
+  uint32_t cap_brick = 0;
+  DevBuf<unsigned int> status;   // standalone runs: 1 = a brick list overflowed
+  void ensure_cap(uint32_t n);
+  void grow();
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
             float beta, int order, bool fp64 = false);
@@ -91,11 +95,14 @@ struct PmeRun {
   const float4* xyzq = nullptr;
   const uint32_t* site = nullptr;
   int64_t* fx = nullptr;
-  int comp_major = 0;
+  int comp_major = 0;  // bit 0: component-major fx; bit 1: each site has one charge
+                       // (plain adds instead of atomics)
   uint32_t nsites = 0;
   float* f_out = nullptr;
   float fscale = 1.0f;
   double* res = nullptr;  // [energy, virial]
+  unsigned int* status = nullptr;  // a full brick list sets status_bit here (null: the core's)
+  unsigned int status_bit = 0;
 };
 
 // runs[0 .. R.n): one per system of the pass; tag separates passes that run

@@ -401,11 +401,15 @@ __device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk
 }
 
 // Per charge: its home brick (the brick of its base grid point) and a rank
-// in it.  The rank order depends on atomic timing; the brick sums are
-// integers, so the spread grid does not.
+// in it; the charge goes to slot rank of the brick's fixed-capacity list
+// (cap slots per brick).  A full brick sets *status |= status_bit (the caller
+// enlarges the lists and evaluates again).  The rank order depends on atomic
+// timing; the brick sums are integers, so the spread grid does not.
 __device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
                           uint32_t nbx, uint32_t nby, uint32_t nbz,
-                          uint32_t* __restrict__ count, uint2* __restrict__ cb) {
+                          uint32_t* __restrict__ count, uint32_t* __restrict__ items,
+                          uint32_t cap, unsigned int* __restrict__ status,
+                          unsigned int status_bit) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t b = 0xFFFFFFFFu;  // no brick (beyond n)
   if (i < n) {
@@ -421,51 +425,10 @@ __device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint3
   }
   // one atomic per distinct brick of the wave (neighbouring charges share bricks)
   const uint32_t rank = wave_rank(b, i < n, count);
-  if (i < n) cb[i] = make_uint2(b, rank);
-}
-
-// Single-workgroup exclusive scan of the brick counts into start[0..n]; the
-// counts are cleared for the next binning pass.
-constexpr int kBrickScan = 1024;
-__device__ __forceinline__ void
-k_pme_brick_scan(uint32_t* __restrict__ count, uint32_t n, uint32_t* __restrict__ start) {
-  __shared__ uint32_t wsum[kBrickScan / 64];
-  if (!start) return;  // a system that skips this pass
-  const int t = threadIdx.x, lane = t & 63, w = t / 64;
-  const uint32_t per = (n + kBrickScan - 1) / kBrickScan;
-  const uint32_t b0 = t * per, e0 = min(n, b0 + per);
-  uint32_t s = 0;
-  for (uint32_t i = b0; i < e0; ++i) s += count[i];
-  uint32_t incl = s;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t v = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += v;
+  if (i < n) {
+    if (rank < cap) items[(size_t)b * cap + rank] = i;
+    else atomicOr(status, status_bit);
   }
-  if (lane == 63) wsum[w] = incl;
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-  for (int k = 0; k < kBrickScan / 64; ++k) {
-    if (k < w) base += wsum[k];
-    tot += wsum[k];
-  }
-  uint32_t run = base + incl - s;
-  for (uint32_t i = b0; i < e0; ++i) {
-    const uint32_t c = count[i];
-    start[i] = run;
-    run += c;
-    count[i] = 0;
-  }
-  if (t == 0) start[n] = tot;
-}
-
-__device__ __forceinline__ void k_pme_brick_scatter(const uint2* __restrict__ cb, uint32_t n,
-                                    const uint32_t* __restrict__ start,
-                                    uint32_t* __restrict__ items) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint2 v = cb[i];
-  const uint32_t k = start[v.x] + v.y;
-  if (k < n) items[k] = i;
 }
 
 // Home bricks along one axis whose charges can reach the points [lo, hi]
@@ -496,8 +459,8 @@ __device__ __forceinline__ HomeRange home_bricks(int lo, int hi, int K) {
 template <int ORDER, class R>
 __device__ __forceinline__ void
 k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t nbx,
-             uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ start,
-             const uint32_t* __restrict__ items, int64_t* __restrict__ grid) {
+             uint32_t nby, uint32_t nbz, const uint32_t* __restrict__ count,
+             const uint32_t* __restrict__ items, uint32_t cap, int64_t* __restrict__ grid) {
   __shared__ unsigned long long acc[kBrick * kBrick * kBrick];
   __shared__ uint32_t s_c0[27], s_beg[28];
   const uint32_t b = blockIdx.x;
@@ -518,10 +481,9 @@ k_pme_spread(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S, uint32_t
         for (int iz = 0; iz < hz.n; ++iz) {
           const uint32_t h = ((uint32_t)hx.brick(ix) * nby + (uint32_t)hy.brick(iy)) * nbz +
                              (uint32_t)hz.brick(iz);
-          const uint32_t c0 = min(start[h], n), c1 = min(start[h + 1], n);
-          s_c0[q] = c0;
+          s_c0[q] = h * cap;
           s_beg[q] = tot;
-          tot += c1 > c0 ? c1 - c0 : 0u;
+          tot += min(count[h], cap);
           ++q;
         }
     for (; q < 28; ++q) s_beg[q] = tot;
@@ -583,9 +545,14 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
                              float* __restrict__ fout, float fscale,
                              const double* __restrict__ partials, int nbp,
-                             double* __restrict__ res) {
+                             double* __restrict__ res, uint32_t* __restrict__ brick_count,
+                             uint32_t nbk) {
   using R = typename CxT<C>::R;
   constexpr bool F32 = std::is_same<R, float>::value;
+  // the spread has read the brick counts: clear them for the next binning
+  if (brick_count)
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nbk; k += gridDim.x * blockDim.x)
+      brick_count[k] = 0u;
   if (res && blockIdx.x == 0) {
     // the first workgroup also sums the convolution's energy and virial
     // partials (fixed order: strided per thread, then waves, then lanes)
@@ -659,7 +626,13 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
   }
   if (site) {
     const uint32_t s = site[i];
-    if (comp_major) {
+    if (comp_major & 2) {
+      // every site has one charge: no other thread of this pass touches it
+      // (a second pass over the same sites is a later launch)
+      fx[s] += fx_of(f0);
+      fx[nsites + s] += fx_of(f1);
+      fx[2 * (size_t)nsites + s] += fx_of(f2);
+    } else if (comp_major) {
       atomicAdd(reinterpret_cast<unsigned long long*>(fx + s), (unsigned long long)fx_of(f0));
       atomicAdd(reinterpret_cast<unsigned long long*>(fx + nsites + s), (unsigned long long)fx_of(f1));
       atomicAdd(reinterpret_cast<unsigned long long*>(fx + 2 * (size_t)nsites + s), (unsigned long long)fx_of(f2));
@@ -716,9 +689,11 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
   for (int a = 0; a < 3; ++a) nbk[a] = (K[a] + kBrick - 1) / kBrick;
   const size_t nb = (size_t)nbk[0] * nbk[1] * nbk[2];
   brick_count.reserve(nb + 1);
-  brick_start.reserve(nb + 1);
-  // the binning pass counts into zeroed counts; the scan clears them again
+  cap_brick = 0;
+  status.reserve(1);
+  // the binning pass counts into zeroed counts; the gather clears them again
   DYN_HIP(hipMemsetAsync(brick_count.p, 0, sizeof(uint32_t) * (nb + 1), stream));
+  DYN_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned int), stream));
   if (f64) {
     cgrid64.reserve(ngrid);
     G64.reserve(ngrid);
@@ -835,6 +810,21 @@ void PmeCore::set_box(const float l[3]) {
   if (e != hipSuccess) throw Error(DYN_ERR_HIP, fmt("launch of k_pme_influence: %s", hipGetErrorString(e)));
 }
 
+// Brick lists for n charges: three times the mean per brick plus 64, in
+// multiples of 32 (grown further when a brick overflows).
+void PmeCore::ensure_cap(uint32_t n) {
+  const size_t nb = (size_t)nbk[0] * nbk[1] * nbk[2];
+  const double mean = (double)n * (double)(kBrick * kBrick * kBrick) / (double)ngrid;
+  const uint32_t want = ((uint32_t)(3.0 * mean) + 64 + 31) & ~31u;
+  if (want > cap_brick) cap_brick = want;
+  brick_items.reserve(nb * cap_brick);
+}
+
+void PmeCore::grow() {
+  cap_brick = std::max<uint32_t>(64, 2 * cap_brick);
+  brick_items.reserve((size_t)nbk[0] * nbk[1] * nbk[2] * cap_brick);
+}
+
 void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t* fx_sites,
                   int comp_major, uint32_t nsites, float* f_out, float fscale, double* res) {
   PmeRun r;
@@ -852,7 +842,17 @@ void PmeCore::run(uint32_t n, const float4* xyzq, const uint32_t* site, int64_t*
   R.stream = stream;
   R.cache = &cache;
   R.n = 1;
-  pme_run(R, *prof, &r, 0);
+  // a brick list that overflowed: enlarge and evaluate again
+  for (int attempt = 0;; ++attempt) {
+    pme_run(R, *prof, &r, 0);
+    unsigned int st = 0;
+    DYN_HIP(hipMemcpyAsync(&st, status.p, sizeof st, hipMemcpyDeviceToHost, stream));
+    DYN_HIP(hipStreamSynchronize(stream));
+    if (!st) break;
+    DYN_HIP(hipMemsetAsync(status.p, 0, sizeof st, stream));
+    if (attempt >= 4) throw Error(DYN_ERR_CAPACITY, "SPME brick lists keep overflowing");
+    grow();
+  }
 }
 
 // One SPME evaluation per system of the pass, each stage one launch for all
@@ -883,8 +883,7 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
       S[i].sc[a] = (float)((double)c.K[a] / (double)c.L[a]);
       S[i].scd[a] = (double)c.K[a] / (double)c.L[a];
     }
-    r.core->charge_brick.reserve(std::max<uint32_t>(r.n, 1));
-    r.core->brick_items.reserve(std::max<uint32_t>(r.n, 1));
+    r.core->ensure_cap(r.n);
     gz[i] = pass_geom(c.K, 2);
     gy[i] = pass_geom(c.K, 1);
     gx[i] = pass_geom(c.K, 0);
@@ -912,28 +911,21 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
   {
     Profiler::Scope ps(prof, "pme_spread");
     launch<k_pme_bin>(R, "k_pme_bin", tag, g_n, 256, 0, [&](uint32_t i) {
+      const bool o = on(i);
       return args<k_pme_bin>(runs[i].xyzq, nq(i), S[i], nb(i, 0), nb(i, 1), nb(i, 2),
-                             on(i) ? cp(i)->brick_count.p : nullptr,
-                             on(i) ? cp(i)->charge_brick.p : nullptr);
-    });
-    launch<k_pme_brick_scan, kBrickScan>(R, "k_pme_brick_scan", tag, 1, kBrickScan, 0,
-                                         [&](uint32_t i) {
-      return args<k_pme_brick_scan>(on(i) ? cp(i)->brick_count.p : nullptr,
-                                    nb(i, 0) * nb(i, 1) * nb(i, 2),
-                                    on(i) ? cp(i)->brick_start.p : nullptr);
-    });
-    launch<k_pme_brick_scatter>(R, "k_pme_brick_scatter", tag, g_n, 256, 0, [&](uint32_t i) {
-      return args<k_pme_brick_scatter>(on(i) ? cp(i)->charge_brick.p : nullptr, nq(i),
-                                       on(i) ? cp(i)->brick_start.p : nullptr,
-                                       on(i) ? cp(i)->brick_items.p : nullptr);
+                             o ? cp(i)->brick_count.p : nullptr,
+                             o ? cp(i)->brick_items.p : nullptr, o ? cp(i)->cap_brick : 0u,
+                             o ? (runs[i].status ? runs[i].status : cp(i)->status.p) : nullptr,
+                             o ? (runs[i].status ? runs[i].status_bit : 1u) : 0u);
     });
     auto spread = [&](auto F, const char* name, bool f64, int order) {
       launch<decltype(F)::value, kBrickBlock>(R, name, tag, g_b, kBrickBlock, 0, [&](uint32_t i) {
         const bool m = mine(i, f64, order);
         return args<decltype(F)::value>(runs[i].xyzq, m ? nq(i) : 0u, S[i], m ? nb(i, 0) : 0u,
                                         m ? nb(i, 1) : 0u, m ? nb(i, 2) : 0u,
-                                        m ? cp(i)->brick_start.p : nullptr,
+                                        m ? cp(i)->brick_count.p : nullptr,
                                         m ? cp(i)->brick_items.p : nullptr,
+                                        m ? cp(i)->cap_brick : 0u,
                                         m ? cp(i)->grid_fx.p : nullptr);
       });
     };
@@ -1054,7 +1046,9 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
                                         runs[i].f_out, runs[i].fscale,
                                         m ? (const double*)cp(i)->partials.p : nullptr,
                                         m ? (int)cp(i)->conv_blocks : 0,
-                                        m ? runs[i].res : nullptr);
+                                        m ? runs[i].res : nullptr,
+                                        m ? cp(i)->brick_count.p : nullptr,
+                                        m ? nb(i, 0) * nb(i, 1) * nb(i, 2) : 0u);
       });
     };
     if (ord[0][0]) gather(DYN_IC((k_pme_gather<4, float2>)), TF{}, 4);
