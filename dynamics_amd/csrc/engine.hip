@@ -1055,6 +1055,92 @@ __device__ __forceinline__ void k_eng_slots_prep(const int32_t* __restrict__ slo
                   mol_e, n_mol2);
 }
 
+// List-reuse evaluation with a lagged displacement check: wrapping and the
+// displacement reading, the slot positions and the site set-up in one
+// launch.  Slots and charges take their wrapped positions from pos directly
+// (the same wrap1 of the same input as wpos).
+__device__ __forceinline__ float4 wrapped(const float4* __restrict__ pos,
+                                          const float* __restrict__ q, const EngGeom& g,
+                                          uint32_t s) {
+  const float4 p = pos[s];
+  return make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
+                     wrap1(p.z, g.L[2], g.invL[2]), q[s]);
+}
+
+struct PrepArgs {
+  const float4* pos;
+  const float* q;
+  EngGeom g;
+  float4* wpos;
+  const float4* wpos_ref;
+  unsigned int* disp_bits;
+  const int32_t* slot_site;
+  const uint32_t* n_cl;
+  float4* xyzq;
+  const uint32_t* qsite;
+  uint32_t nq;
+  float4* pme_xyzq;   // null: no SPME this evaluation
+  float4* pme_b;      // null: one SPME evaluation
+  uint32_t alch0, alch1;
+  int64_t* fx_site;
+  int64_t* fx_recip;  // null: SPME forces kept
+  int64_t* mol_e;
+  uint32_t n_mol2;
+};
+
+__device__ __forceinline__ void k_eng_prep(PrepArgs A) {
+  __shared__ unsigned int red[256 / kWave];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = A.g.n_dyn + 4u * A.g.n_water;
+  unsigned int v = 0u;
+  if (t < n) {
+    const float4 w = wrapped(A.pos, A.q, A.g, t);
+    A.wpos[t] = w;
+    const float4 b = A.wpos_ref[t];
+    float dx = w.x - b.x, dy = w.y - b.y, dz = w.z - b.z;
+    dx -= A.g.L[0] * rintf(dx * A.g.invL[0]);
+    dy -= A.g.L[1] * rintf(dy * A.g.invL[1]);
+    dz -= A.g.L[2] * rintf(dz * A.g.invL[2]);
+    v = __float_as_uint(dx * dx + dy * dy + dz * dz);
+    A.fx_site[t] = 0;
+    A.fx_site[(size_t)n + t] = 0;
+    A.fx_site[2 * (size_t)n + t] = 0;
+    if (A.fx_recip) {
+      A.fx_recip[t] = 0;
+      A.fx_recip[(size_t)n + t] = 0;
+      A.fx_recip[2 * (size_t)n + t] = 0;
+    }
+  }
+  if (A.mol_e)
+    for (uint32_t k = t; k < A.n_mol2; k += gridDim.x * blockDim.x) A.mol_e[k] = 0;
+  if (A.pme_xyzq && t < A.nq) {
+    const uint32_t site = A.qsite[t];
+    const float4 p = wrapped(A.pos, A.q, A.g, site);
+    A.pme_xyzq[t] = p;
+    if (A.pme_b)
+      A.pme_b[t] = make_float4(p.x, p.y, p.z, site >= A.alch0 && site < A.alch1 ? -p.w : p.w);
+  }
+  if (t < *A.n_cl * DYN_CL_SIZE) {
+    const int32_t s = A.slot_site[t];
+    if (s >= 0) {
+      A.xyzq[t] = wrapped(A.pos, A.q, A.g, (uint32_t)s);
+    } else {
+      float4 p = wrapped(A.pos, A.q, A.g,
+                         (uint32_t)A.slot_site[t & ~(uint32_t)(DYN_CL_SIZE - 1)]);
+      p.w = 0.0f;
+      A.xyzq[t] = p;
+    }
+  }
+  for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int mx = 0u;
+    for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
+    if (mx) atomicMax(A.disp_bits, mx);
+  }
+}
+
 // Sum of the work-item force rows of slot k's cluster row (fixed point).
 __device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fitem,
                                                  const uint32_t* __restrict__ chunk_ptr,
@@ -1586,36 +1672,6 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       if (!(D.seen[i] < lag_half * lag_half)) rebuild = true;
     }
   }
-  if (!rebuild) {
-    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_wrap_disp>(R, "k_eng_wrap_disp", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_wrap_disp>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p,
-                                   (const float4*)e.wpos_ref.p, D.bits.p + i);
-    });
-    if (async) {
-      const int k = D.next;
-      D.next = (D.next + 1) % DispRing::kRing;
-      if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
-        DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
-                               hipMemcpyDeviceToHost, R.stream));
-        DYN_HIP(hipEventRecord(D.ev[k], R.stream));
-        D.pending[k] = true;
-        D.gen[k] = D.builds;
-      }
-    } else {
-      DYN_HIP(hipMemcpyAsync(D.h_sync.p, D.bits.p, 4 * (size_t)ns, hipMemcpyDeviceToHost,
-                             R.stream));
-      DYN_HIP(hipStreamSynchronize(R.stream));
-      for (uint32_t i = 0; i < ns; ++i) {
-        float d2;
-        std::memcpy(&d2, D.h_sync.p + i, 4);
-        const float half = 0.5f * E[i]->prm.list_buffer;
-        if (!(d2 < half * half)) rebuild = true;
-      }
-    }
-  }
-  if (rebuild) set_build_lists(S);
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
   // per system: pair parameters and which reciprocal work it does
@@ -1664,7 +1720,82 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     any_b = any_b || pl.recip_b;
     any_corr = any_corr || pl.corr_on;
   }
-  if (rebuild) {
+  // steady state (list reuse with the lagged check): one preparation launch
+  const bool fused = !rebuild && async;
+  if (fused) {
+    Profiler::Scope ps(prof, "prep");
+    const unsigned gp = max_grid(S, [](dyn_engine& e) {
+      return std::max(grid_for(e.n_sites, 256), grid_for(e.cap_slots, 256));
+    });
+    launch<k_eng_prep>(R, "k_eng_prep", 0, gp, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      const Plan& pl = PL[i];
+      PrepArgs A;
+      std::memset(&A, 0, sizeof A);
+      A.pos = e.pos_dev;
+      A.q = e.q.p;
+      A.g = e.geom;
+      A.wpos = e.wpos.p;
+      A.wpos_ref = e.wpos_ref.p;
+      A.disp_bits = D.bits.p + i;
+      A.slot_site = e.slot_site.p;
+      A.n_cl = e.counters.p + 0;
+      A.xyzq = e.xyzq.p;
+      A.qsite = e.qsite.p;
+      A.nq = e.n_q;
+      A.pme_xyzq = pl.recip_on ? e.pme_xyzq.p : nullptr;
+      A.pme_b = pl.recip_b ? e.pme_b.p : nullptr;
+      A.alch0 = e.alch0;
+      A.alch1 = e.alch1;
+      A.fx_site = e.fx_site.p;
+      A.fx_recip = pl.recip_on ? e.fx_recip.p : nullptr;
+      A.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
+      A.n_mol2 = e.n_mol * e.n_mol;
+      return args<k_eng_prep>(A);
+    });
+    const int k = D.next;
+    D.next = (D.next + 1) % DispRing::kRing;
+    if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
+      DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
+                             hipMemcpyDeviceToHost, R.stream));
+      DYN_HIP(hipEventRecord(D.ev[k], R.stream));
+      D.pending[k] = true;
+      D.gen[k] = D.builds;
+    }
+  }
+  if (!rebuild && !fused) {
+    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
+    launch<k_eng_wrap_disp>(R, "k_eng_wrap_disp", 0, gs, 256, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_wrap_disp>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p,
+                                   (const float4*)e.wpos_ref.p, D.bits.p + i);
+    });
+    if (async) {
+      const int k = D.next;
+      D.next = (D.next + 1) % DispRing::kRing;
+      if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
+        DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
+                               hipMemcpyDeviceToHost, R.stream));
+        DYN_HIP(hipEventRecord(D.ev[k], R.stream));
+        D.pending[k] = true;
+        D.gen[k] = D.builds;
+      }
+    } else {
+      DYN_HIP(hipMemcpyAsync(D.h_sync.p, D.bits.p, 4 * (size_t)ns, hipMemcpyDeviceToHost,
+                             R.stream));
+      DYN_HIP(hipStreamSynchronize(R.stream));
+      for (uint32_t i = 0; i < ns; ++i) {
+        float d2;
+        std::memcpy(&d2, D.h_sync.p + i, 4);
+        const float half = 0.5f * E[i]->prm.list_buffer;
+        if (!(d2 < half * half)) rebuild = true;
+      }
+    }
+  }
+  if (rebuild) set_build_lists(S);
+  if (fused) {
+    // done by the preparation launch
+  } else if (rebuild) {
     Profiler::Scope ps(prof, "site_prep");
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
     launch<k_eng_site_prep>(R, "k_eng_site_prep", 0, gs, 256, 0, [&](uint32_t i) {

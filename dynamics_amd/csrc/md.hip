@@ -440,7 +440,7 @@ struct BondedArgs {
   const uint32_t* angle;  // 3 per term (vertex in the middle)
   const float2* angle_p;  // k, θ0
   const uint32_t* dihe;   // 4 per term
-  const float4* dihe_p;   // k, periodicity, phase, unused
+  const float4* dihe_p;   // k, periodicity (an integer 0..6), cos(phase), sin(phase)
   int64_t* f;             // fixed point, component-major over solute atoms
   double* partials;       // 4 per block: bond, angle, dihedral energy, virial
   uint32_t nblk;          // workgroups of the system (one partial each)
@@ -509,10 +509,23 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
     const float n1s = dot(n1, n1), n2s = dot(n2, n2), b2s = dot(b2, b2);
     const float b2l = sqrtf(b2s);
     if (n1s >= 1e-6f && n2s >= 1e-6f && b2l >= 1e-6f) {
-      const float phi = atan2f(b2l * dot(b1, n2), dot(n1, n2));
+      // cos φ, sin φ of the IUPAC torsion φ = atan2(|b2| b1·n2, n1·n2); then
+      // cos nφ, sin nφ by the angle-addition recurrence and the phase by one
+      // more rotation: cos(nφ − γ), sin(nφ − γ) without trigonometric calls
       const float4 p = A.dihe_p[q];
-      const float arg = p.y * phi - p.z;
-      const float dedphi = -p.x * p.y * sinf(arg);
+      const float y = b2l * dot(b1, n2), x = dot(n1, n2);
+      const float rinv = 1.0f / sqrtf(x * x + y * y);
+      const float c1 = x * rinv, s1 = y * rinv;
+      float cn = 1.0f, sn = 0.0f;
+      const int nper = (int)p.y;
+      for (int k = 0; k < nper; ++k) {
+        const float c2 = cn * c1 - sn * s1;
+        sn = sn * c1 + cn * s1;
+        cn = c2;
+      }
+      const float cos_arg = cn * p.z + sn * p.w;   // cos(nφ − γ)
+      const float sin_arg = sn * p.z - cn * p.w;   // sin(nφ − γ)
+      const float dedphi = -p.x * p.y * sin_arg;
       // ∂φ/∂x_i = −|b2|/|n1|² n1, ∂φ/∂x_l = |b2|/|n2|² n2; the inner atoms
       // take the lever-arm split of the two
       const V3 gi = n1 * (-b2l / n1s), gl = n2 * (b2l / n2s);
@@ -523,7 +536,7 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
       add_f(A.f, nd, j, gj * -dedphi);
       add_f(A.f, nd, k, gk * -dedphi);
       add_f(A.f, nd, l, gl * -dedphi);
-      e[2] = (double)(p.x * (1.0f + cosf(arg)));
+      e[2] = (double)(p.x * (1.0f + cos_arg));
       // relative to atom j: r_i − r_j = −b1, r_k − r_j = b2, r_l − r_j = b2 + b3
       e[3] = -(double)dot(b1, gi * -dedphi) + (double)dot(b2, gk * -dedphi) +
              (double)dot(b2 + b3, gl * -dedphi);
@@ -1429,9 +1442,13 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     up(M->angle_p, angle_k_theta0, n_angles, 8);
     up(M->dihe, dihedral_ijkl, 4 * (size_t)n_dihedrals, 4);
     std::vector<float4> dp(n_dihedrals);
-    for (uint32_t k = 0; k < n_dihedrals; ++k)
-      dp[k] = make_float4(dihedral_k_n_phase[3 * k], dihedral_k_n_phase[3 * k + 1],
-                          dihedral_k_n_phase[3 * k + 2], 0.f);
+    for (uint32_t k = 0; k < n_dihedrals; ++k) {
+      const float per = dihedral_k_n_phase[3 * k + 1];
+      DYN_REQUIRE(per >= 0.f && per <= 6.f && per == std::floor(per),
+                  "dihedral %u: periodicity %g is not an integer in [0, 6]", k, (double)per);
+      const double ph = (double)dihedral_k_n_phase[3 * k + 2];
+      dp[k] = make_float4(dihedral_k_n_phase[3 * k], per, (float)std::cos(ph), (float)std::sin(ph));
+    }
     up(M->dihe_p, dp.data(), n_dihedrals, 16);
     M->bblocks = grid_for(std::max<uint32_t>(n_bonds + n_angles + n_dihedrals, 1), kBlock);
     M->bpart.reserve(4 * (size_t)M->bblocks);

@@ -394,7 +394,8 @@ typedef struct dyn_md_params {
 /* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],
  * (k kcal/mol/Å², r0 Å)[2n], U = k (r − r0)²; angles: ijk[3n] (vertex j),
  * (k kcal/mol/rad², θ0 rad)[2n], U = k (θ − θ0)²; dihedrals: ijkl[4n],
- * (k kcal/mol, periodicity, phase rad)[3n], U = k (1 + cos(nφ − phase));
+ * (k kcal/mol, periodicity, phase rad)[3n], U = k (1 + cos(nφ − phase)), the
+ * periodicity an integer in [0, 6];
  * constraints: ij[2n], r0[n] (coupled constraints form clusters of at most
  * 16, as bonds to hydrogen do).  Indices are solute atoms.  The MD handle
  * drives the engine (its positions become the MD state); destroy it before
