@@ -191,13 +191,12 @@ __device__ __forceinline__ C* lds_fft_pow2(C* a, C* b, int nl, const C* __restri
 }
 
 // Stockham autosort FFT of nl lines of length N held in LDS as a[l·N + n].
-// Returns the buffer holding the (natural-order) result.
-template <int SIGN, class C>
+// Returns the buffer holding the (natural-order) result.  NF: a line length
+// every system of the launch shares (32, 64 or 128: the fixed plan), or 0.
+template <int SIGN, int NF, class C>
 __device__ C* lds_fft(C* a, C* b, int nl, const FftAxis& ax, const C* __restrict__ tw) {
+  if constexpr (NF != 0) return lds_fft_pow2<SIGN, NF>(a, b, nl, tw);
   const int N = ax.N;
-  if (N == 64) return lds_fft_pow2<SIGN, 64>(a, b, nl, tw);
-  if (N == 128) return lds_fft_pow2<SIGN, 128>(a, b, nl, tw);
-  if (N == 32) return lds_fft_pow2<SIGN, 32>(a, b, nl, tw);
   int Ns = 1;
   for (int s = 0; s < ax.nstages; ++s) {
     const int R = ax.radix[s];
@@ -266,7 +265,7 @@ __device__ __forceinline__ void split_e(int e, int L, int N, int& l, int& n) {
 
 // One FFT pass along AXIS over complex data in place.  FROM_FX: the input is
 // the int64 fixed-point charge grid (z pass only).
-template <int AXIS, int SIGN, bool FROM_FX, class C>
+template <int AXIS, int SIGN, bool FROM_FX, class C, int NF>
 __device__ __forceinline__ void
 k_fft_pass(C* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g,
            const FftAxis* __restrict__ axp, const C* __restrict__ tw) {
@@ -291,7 +290,7 @@ k_fft_pass(C* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g
     A[l * N + n] = v;
   }
   __syncthreads();
-  C* Rs = lds_fft<SIGN>(A, B, nl, ax, tw);
+  C* Rs = lds_fft<SIGN, NF>(A, B, nl, ax, tw);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<AXIS>(e, nl, N, l, n);
@@ -306,7 +305,7 @@ struct ConvArgs {
 
 // x pass fused with the convolution: forward FFT along x, multiply by G(m)
 // (accumulating ½G|F[Q]|² and its virial weight in f64), inverse FFT along x.
-template <class CX>
+template <class CX, int NF>
 __device__ __forceinline__ void
 k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
              const CX* __restrict__ tw, const typename CxT<CX>::R* __restrict__ G, ConvArgs C,
@@ -328,7 +327,7 @@ k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
     A[l * N + n] = data[pass_addr<0>(g, line0 + l, n)];
   }
   __syncthreads();
-  CX* R = lds_fft<-1>(A, B, nl, ax, tw);
+  CX* R = lds_fft<-1, NF>(A, B, nl, ax, tw);
   CX* O = (R == A) ? B : A;
   double e_acc = 0.0, w_acc = 0.0;
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -366,7 +365,7 @@ k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
     partials[2 * blockIdx.x] = red[0][0];
     partials[2 * blockIdx.x + 1] = red[1][0];
   }
-  CX* R2 = lds_fft<+1>(R, O, nl, ax, tw);
+  CX* R2 = lds_fft<+1, NF>(R, O, nl, ax, tw);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<0>(e, nl, N, l, n);
@@ -1000,32 +999,57 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
   struct TD {
     using type = double2;
   };
+  // a line length shared by every single-precision system of the pass
+  // along an axis (32, 64, 128) selects the fixed-plan FFT kernels
+  auto shared_n = [&](int axis) {
+    int n = -1;
+    for (uint32_t i = 0; i < ns; ++i) {
+      if (!mine(i, false, 0)) continue;
+      const int k = (int)cp(i)->K[axis];
+      if (n < 0) n = k;
+      else if (n != k) return 0;
+    }
+    return (n == 32 || n == 64 || n == 128) ? n : 0;
+  };
+#define DYN_FFT_F32(AXIS, SIGN, FX, NAME, G, B, SH)                                               \
+  switch (shared_n(AXIS)) {                                                                       \
+    case 32: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 32>)), TF{}, NAME, G, AXIS, B, SH, FX); break; \
+    case 64: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 64>)), TF{}, NAME, G, AXIS, B, SH, FX); break; \
+    case 128: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 128>)), TF{}, NAME, G, AXIS, B, SH, FX); break; \
+    default: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 0>)), TF{}, NAME, G, AXIS, B, SH, FX); \
+  }
   for (int pr = 0; pr < 2; ++pr) {
     if (!prec[pr]) continue;
     if (pr == 0) {
       { Profiler::Scope ps(prof, "pme_fft_z_fwd");
-        fft(DYN_IC((k_fft_pass<2, -1, true, float2>)), TF{}, "k_fft_pass z fwd", gz, 2, bz, shz, true); }
+        DYN_FFT_F32(2, -1, true, "k_fft_pass z fwd", gz, bz, shz) }
       { Profiler::Scope ps(prof, "pme_fft_y_fwd");
-        fft(DYN_IC((k_fft_pass<1, -1, false, float2>)), TF{}, "k_fft_pass y fwd", gy, 1, by, shy, false); }
+        DYN_FFT_F32(1, -1, false, "k_fft_pass y fwd", gy, by, shy) }
       { Profiler::Scope ps(prof, "pme_fft_x_conv");
-        conv(DYN_IC(k_fft_conv_x<float2>), TF{}, "k_fft_conv_x"); }
+        switch (shared_n(0)) {
+          case 32: conv(DYN_IC((k_fft_conv_x<float2, 32>)), TF{}, "k_fft_conv_x"); break;
+          case 64: conv(DYN_IC((k_fft_conv_x<float2, 64>)), TF{}, "k_fft_conv_x"); break;
+          case 128: conv(DYN_IC((k_fft_conv_x<float2, 128>)), TF{}, "k_fft_conv_x"); break;
+          default: conv(DYN_IC((k_fft_conv_x<float2, 0>)), TF{}, "k_fft_conv_x");
+        } }
       { Profiler::Scope ps(prof, "pme_fft_y_inv");
-        fft(DYN_IC((k_fft_pass<1, 1, false, float2>)), TF{}, "k_fft_pass y inv", gy, 1, by, shy, false); }
+        DYN_FFT_F32(1, 1, false, "k_fft_pass y inv", gy, by, shy) }
       { Profiler::Scope ps(prof, "pme_fft_z_inv");
-        fft(DYN_IC((k_fft_pass<2, 1, false, float2>)), TF{}, "k_fft_pass z inv", gz, 2, bz, shz, false); }
+        DYN_FFT_F32(2, 1, false, "k_fft_pass z inv", gz, bz, shz) }
     } else {
       { Profiler::Scope ps(prof, "pme_fft_z_fwd");
-        fft(DYN_IC((k_fft_pass<2, -1, true, double2>)), TD{}, "k_fft_pass z fwd f64", gz, 2, bz, shz, true); }
+        fft(DYN_IC((k_fft_pass<2, -1, true, double2, 0>)), TD{}, "k_fft_pass z fwd f64", gz, 2, bz, shz, true); }
       { Profiler::Scope ps(prof, "pme_fft_y_fwd");
-        fft(DYN_IC((k_fft_pass<1, -1, false, double2>)), TD{}, "k_fft_pass y fwd f64", gy, 1, by, shy, false); }
+        fft(DYN_IC((k_fft_pass<1, -1, false, double2, 0>)), TD{}, "k_fft_pass y fwd f64", gy, 1, by, shy, false); }
       { Profiler::Scope ps(prof, "pme_fft_x_conv");
-        conv(DYN_IC(k_fft_conv_x<double2>), TD{}, "k_fft_conv_x f64"); }
+        conv(DYN_IC((k_fft_conv_x<double2, 0>)), TD{}, "k_fft_conv_x f64"); }
       { Profiler::Scope ps(prof, "pme_fft_y_inv");
-        fft(DYN_IC((k_fft_pass<1, 1, false, double2>)), TD{}, "k_fft_pass y inv f64", gy, 1, by, shy, false); }
+        fft(DYN_IC((k_fft_pass<1, 1, false, double2, 0>)), TD{}, "k_fft_pass y inv f64", gy, 1, by, shy, false); }
       { Profiler::Scope ps(prof, "pme_fft_z_inv");
-        fft(DYN_IC((k_fft_pass<2, 1, false, double2>)), TD{}, "k_fft_pass z inv f64", gz, 2, bz, shz, false); }
+        fft(DYN_IC((k_fft_pass<2, 1, false, double2, 0>)), TD{}, "k_fft_pass z inv f64", gz, 2, bz, shz, false); }
     }
   }
+#undef DYN_FFT_F32
   {
     Profiler::Scope ps(prof, "pme_gather");
     const unsigned gg = std::max(1u, (g_n * 256 + kGatherBlock - 1) / kGatherBlock);
