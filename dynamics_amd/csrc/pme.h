@@ -53,9 +53,8 @@ struct PmeCore {
   DevBuf<double2> cgrid64;
   DevBuf<double> G64;
   DevBuf<double2> tw64[3];
-  DevBuf<double> partials;      // energy/virial partials of the convolution pass
+  DevBuf<int64_t> partials;     // energy/virial partials of the convolution pass (2^-32 fixed point)
   PinnedBuf<double> h_part;
-  unsigned conv_blocks = 0;
   DevBuf<double> res_dev;       // [energy, virial] reduced on the device
   // Spreading by grid brick (8³ points): each charge is listed under its
   // home brick (the brick of its base grid point) in a fixed-capacity list,

@@ -172,6 +172,50 @@ __device__ __forceinline__ void fft_stage(const C* __restrict__ a, C* __restrict
   __syncthreads();
 }
 
+// x / d for 0 ≤ x < 2²² by a float reciprocal and one correction step.
+__device__ __forceinline__ int fast_div(int x, int d, float rd) {
+  int q = (int)((float)x * rd);
+  const int rem = x - q * d;
+  if (rem < 0) --q;
+  else if (rem >= d) ++q;
+  return q;
+}
+
+// One radix-R Stockham stage with the line length and sub-transform length
+// known at run time (the general grids): the radix is a template parameter so
+// the butterfly stays in registers, and the divisions use fast_div.  The
+// twiddle index r·k·(N/(Ns·R)) stays below N.
+template <int SIGN, int R, class C>
+__device__ __forceinline__ void fft_stage_rt(const C* __restrict__ a, C* __restrict__ b, int nl,
+                                             int N, int Ns, const C* __restrict__ tw) {
+  const int NR = N / R, tws = N / (Ns * R);
+  const float rNR = 1.0f / (float)NR, rNs = 1.0f / (float)Ns;
+  const int total = nl * NR;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int l = fast_div(i, NR, rNR);
+    const int j = i - l * NR;
+    const int q = fast_div(j, Ns, rNs);
+    const int k = j - q * Ns;
+    const C* src = a + l * N;
+    C* dst = b + l * N;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C x = src[j + r * NR];
+      if (r) x = ctw<SIGN>(x, tw[r * k * tws]);
+      v[r] = x;
+    }
+    if constexpr (R == 4) dft4<SIGN>(v);
+    else if constexpr (R == 2) dft2<SIGN>(v);
+    else if constexpr (R == 3) dft3<SIGN>(v);
+    else dft5<SIGN>(v);
+    const int d = q * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[d + r * Ns] = v[r];
+  }
+  __syncthreads();
+}
+
 // The same transform as lds_fft for the power-of-two lengths of common grids
 // (radix 4 first, then 2, the plan of make_fft_axis).
 template <int SIGN, int N, class C>
@@ -200,29 +244,10 @@ __device__ C* lds_fft(C* a, C* b, int nl, const FftAxis& ax, const C* __restrict
   int Ns = 1;
   for (int s = 0; s < ax.nstages; ++s) {
     const int R = ax.radix[s];
-    const int NR = N / R;
-    const int tw_step = N / (Ns * R);
-    const int total = nl * NR;
-    for (int i = threadIdx.x; i < total; i += blockDim.x) {
-      const int l = i / NR;
-      const int j = i - l * NR;
-      const int k = j % Ns;
-      const C* src = a + l * N;
-      C* dst = b + l * N;
-      C v[5];
-      for (int r = 0; r < R; ++r) {
-        C x = src[j + r * NR];
-        if (r) x = ctw<SIGN>(x, tw[(r * k * tw_step) % N]);
-        v[r] = x;
-      }
-      if (R == 4) dft4<SIGN>(v);
-      else if (R == 2) dft2<SIGN>(v);
-      else if (R == 3) dft3<SIGN>(v);
-      else dft5<SIGN>(v);
-      const int d = (j / Ns) * Ns * R + k;
-      for (int r = 0; r < R; ++r) dst[d + r * Ns] = v[r];
-    }
-    __syncthreads();
+    if (R == 4) fft_stage_rt<SIGN, 4>(a, b, nl, N, Ns, tw);
+    else if (R == 2) fft_stage_rt<SIGN, 2>(a, b, nl, N, Ns, tw);
+    else if (R == 3) fft_stage_rt<SIGN, 3>(a, b, nl, N, Ns, tw);
+    else fft_stage_rt<SIGN, 5>(a, b, nl, N, Ns, tw);
     C* t = a;
     a = b;
     b = t;
@@ -309,14 +334,13 @@ template <class CX, int NF>
 __device__ __forceinline__ void
 k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
              const CX* __restrict__ tw, const typename CxT<CX>::R* __restrict__ G, ConvArgs C,
-             double* __restrict__ partials) {
+             int64_t* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   CX* smem = reinterpret_cast<CX*>(smem_raw);
-  __shared__ double red[2][kFftBlock];
   const FftAxis& ax = *axp;
   const int N = ax.N;
   const int line0 = blockIdx.x * g.L;
-  if (line0 >= g.lines) return;  // beyond this system's lines (no partial either)
+  if (line0 >= g.lines) return;  // beyond this system's lines
   const int nl = min(g.L, g.lines - line0);
   CX* A = smem;
   CX* B = smem + g.L * N;
@@ -329,42 +353,71 @@ k_fft_conv_x(CX* __restrict__ data, PassGeom g, const FftAxis* __restrict__ axp,
   __syncthreads();
   CX* R = lds_fft<-1, NF>(A, B, nl, ax, tw);
   CX* O = (R == A) ? B : A;
-  double e_acc = 0.0, w_acc = 0.0;
+  // the influence function of the block's points into the free LDS buffer
+  // (coalesced reads), for the energy sums and the product
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
     split_e<0>(e, nl, N, l, n);
-    const int line = line0 + l;
-    const int my = line / g.K[2], mz = line - my * g.K[2];
-    const size_t gi = pass_addr<0>(g, line, n);
-    const auto gv = G[gi];
+    O[l * N + n].x = G[pass_addr<0>(g, line0 + l, n)];
+  }
+  __syncthreads();
+  // energy and virial weight per line (a wave per line, lanes over its
+  // points, shuffle sums), each line's pair rounded to 2^-32 fixed point and
+  // summed as integers: the workgroup's partial does not depend on how many
+  // lines it holds, so the total is the same for any grouping of the lines
+  {
+    __shared__ long long red[2][kFftBlock / 64];
+    long long qe = 0, qw = 0;
+    const int wv = threadIdx.x / 64, lane = threadIdx.x & 63;
+    for (int l = wv; l < nl; l += kFftBlock / 64) {
+      const int line = line0 + l;
+      const int my = line / g.K[2], mz = line - my * g.K[2];
+      const int myp = my <= g.K[1] / 2 ? my : my - g.K[1];
+      const int mzp = mz <= g.K[2] / 2 ? mz : mz - g.K[2];
+      const double fy = myp * (double)C.invL[1], fz = mzp * (double)C.invL[2];
+      double e_acc = 0.0, w_acc = 0.0;
+      for (int n = lane; n < N; n += 64) {
+        const auto gv = O[l * N + n].x;
+        const CX v = R[l * N + n];
+        const double en = 0.5 * (double)gv * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+        const int mxp = n <= g.K[0] / 2 ? n : n - g.K[0];
+        const double fx = mxp * (double)C.invL[0];
+        const double msq = fx * fx + fy * fy + fz * fz;
+        e_acc += en;
+        w_acc += en * (1.0 - 2.0 * C.pi2_over_b2 * msq);
+      }
+      for (int m = 32; m >= 1; m >>= 1) {
+        e_acc += __shfl_xor(e_acc, m, 64);
+        w_acc += __shfl_xor(w_acc, m, 64);
+      }
+      qe += (long long)dyn_fx64(e_acc);
+      qw += (long long)dyn_fx64(w_acc);
+    }
+    if (lane == 0) {
+      red[0][wv] = qe;
+      red[1][wv] = qw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long te = 0, tw2 = 0;
+      for (int k = 0; k < kFftBlock / 64; ++k) {
+        te += red[0][k];
+        tw2 += red[1][k];
+      }
+      partials[2 * blockIdx.x] = te;
+      partials[2 * blockIdx.x + 1] = tw2;
+    }
+  }
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    int l, n;
+    split_e<0>(e, nl, N, l, n);
+    const auto gv = O[l * N + n].x;
     CX v = R[l * N + n];
-    const double en = 0.5 * (double)gv * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
-    const int mxp = n <= g.K[0] / 2 ? n : n - g.K[0];
-    const int myp = my <= g.K[1] / 2 ? my : my - g.K[1];
-    const int mzp = mz <= g.K[2] / 2 ? mz : mz - g.K[2];
-    const double fx = mxp * (double)C.invL[0], fy = myp * (double)C.invL[1],
-                 fz = mzp * (double)C.invL[2];
-    const double msq = fx * fx + fy * fy + fz * fz;
-    e_acc += en;
-    w_acc += en * (1.0 - 2.0 * C.pi2_over_b2 * msq);
     v.x *= gv;
     v.y *= gv;
     R[l * N + n] = v;
   }
-  red[0][threadIdx.x] = e_acc;
-  red[1][threadIdx.x] = w_acc;
   __syncthreads();
-  for (int s = kFftBlock / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      red[0][threadIdx.x] += red[0][threadIdx.x + s];
-      red[1][threadIdx.x] += red[1][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = red[0][0];
-    partials[2 * blockIdx.x + 1] = red[1][0];
-  }
   CX* R2 = lds_fft<+1, NF>(R, O, nl, ax, tw);
   for (int e = threadIdx.x; e < tot; e += blockDim.x) {
     int l, n;
@@ -543,9 +596,11 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
                              const C* __restrict__ phi, const uint32_t* __restrict__ site,
                              int64_t* __restrict__ fx, int comp_major, uint32_t nsites,
                              float* __restrict__ fout, float fscale,
-                             const double* __restrict__ partials, int nbp,
+                             const int64_t* __restrict__ partials, int nbp,
                              double* __restrict__ res, uint32_t* __restrict__ brick_count,
                              uint32_t nbk) {
+  // partials: the convolution's per-workgroup energy and virial sums in
+  // 2^-32 fixed point (nbp workgroups)
   using R = typename CxT<C>::R;
   constexpr bool F32 = std::is_same<R, float>::value;
   // the spread has read the brick counts: clear them for the next binning
@@ -555,11 +610,11 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
   if (res && blockIdx.x == 0) {
     // the first workgroup also sums the convolution's energy and virial
     // partials (fixed order: strided per thread, then waves, then lanes)
-    __shared__ double red[2][kGatherBlock / 64];
-    double e = 0.0, w = 0.0;
+    __shared__ long long red[2][kGatherBlock / 64];
+    long long e = 0, w = 0;
     for (int b = threadIdx.x; b < nbp; b += kGatherBlock) {
-      e += partials[2 * b];
-      w += partials[2 * b + 1];
+      e += (long long)partials[2 * b];
+      w += (long long)partials[2 * b + 1];
     }
     for (int m = 32; m >= 1; m >>= 1) {
       e += __shfl_xor(e, m, 64);
@@ -571,13 +626,13 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double te = 0.0, tw = 0.0;
+      long long te = 0, tw = 0;
       for (int k = 0; k < kGatherBlock / 64; ++k) {
         te += red[0][k];
         tw += red[1][k];
       }
-      res[0] = te;
-      res[1] = tw;
+      res[0] = (double)te * DYN_FX_INV_SCALE;
+      res[1] = (double)tw * DYN_FX_INV_SCALE;
     }
   }
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -647,13 +702,17 @@ __device__ __forceinline__ void k_pme_gather(const float4* __restrict__ xyzq, ui
   }
 }
 
-PassGeom pass_geom(const uint32_t K[3], int axis) {
+// Lines per workgroup of an FFT pass: about 512 points per workgroup (two
+// per thread), at most kFftCap/N lines, and few enough lines that the pass
+// over all nsys systems of a launch still has ≥ 512 workgroups.
+PassGeom pass_geom(const uint32_t K[3], int axis, uint32_t nsys) {
   PassGeom g;
   for (int a = 0; a < 3; ++a) g.K[a] = (int)K[a];
   const int N = (int)K[axis];
   g.lines = (int)(K[0] * K[1] * K[2] / K[axis]);
-  // aim for ≥ 512 workgroups, at most kFftCap/N lines per group
-  int L = std::max(1, std::min(kFftCap / N, g.lines / 512));
+  const int l_work = std::max(1, 512 / N);
+  const int l_par = (int)std::max<int64_t>(1, (int64_t)g.lines * std::max<uint32_t>(nsys, 1) / 512);
+  int L = std::max(1, std::min(kFftCap / N, std::min(l_work, l_par)));
   int p2 = 1;
   while (p2 * 2 <= L) p2 *= 2;
   g.L = p2;
@@ -719,9 +778,9 @@ void PmeCore::init(int dev, hipStream_t s, Profiler* p, const uint32_t k[3], con
     }
     DYN_HIP(hipStreamSynchronize(stream));
   }
-  const PassGeom gx = pass_geom(K, 0);
-  conv_blocks = (unsigned)((gx.lines + gx.L - 1) / gx.L);
-  partials.reserve(2 * (size_t)conv_blocks);
+  // energy and virial partials: one fixed-point pair per x-pass workgroup
+  // (at most one per line)
+  partials.reserve(2 * (size_t)K[1] * K[2]);
   res_dev.reserve(2);
   set_moduli();
   set_box(l);
@@ -883,9 +942,9 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
       S[i].scd[a] = (double)c.K[a] / (double)c.L[a];
     }
     r.core->ensure_cap(r.n);
-    gz[i] = pass_geom(c.K, 2);
-    gy[i] = pass_geom(c.K, 1);
-    gx[i] = pass_geom(c.K, 0);
+    gz[i] = pass_geom(c.K, 2, ns);
+    gy[i] = pass_geom(c.K, 1, ns);
+    gx[i] = pass_geom(c.K, 0, ns);
     const size_t el = c.f64 ? sizeof(double2) : sizeof(float2);
     g_n = std::max(g_n, grid_for(r.n, 256));
     g_b = std::max(g_b, c.nbk[0] * c.nbk[1] * c.nbk[2]);
@@ -1068,8 +1127,8 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
         return args<decltype(F)::value>(runs[i].xyzq, m ? nq(i) : 0u, S[i], phi, runs[i].site,
                                         runs[i].fx, runs[i].comp_major, runs[i].nsites,
                                         runs[i].f_out, runs[i].fscale,
-                                        m ? (const double*)cp(i)->partials.p : nullptr,
-                                        m ? (int)cp(i)->conv_blocks : 0,
+                                        m ? (const int64_t*)cp(i)->partials.p : nullptr,
+                                        m ? (gx[i].lines + gx[i].L - 1) / gx[i].L : 0,
                                         m ? runs[i].res : nullptr,
                                         m ? cp(i)->brick_count.p : nullptr,
                                         m ? nb(i, 0) * nb(i, 1) * nb(i, 2) : 0u);
