@@ -214,18 +214,27 @@ __device__ __forceinline__ void rigid_drift(WaterRegs& W, float dt, const Box& B
   W.v[1] = make_float4(vm.x, vm.y, vm.z, 0.f);
 }
 
-// Kick of solute atom t: v += a·h from the current forces, the acceleration
-// clamped to MAX_ACCEL.
-__device__ __forceinline__ void kick_atom(const Fx& F, const float* __restrict__ acc_conv,
-                                          uint32_t t, float h, float4* __restrict__ v) {
-  float4 vi = v[t];
+// Acceleration of solute atom t from the current forces, clamped to
+// MAX_ACCEL.
+__device__ __forceinline__ V3 accel_atom(const Fx& F, const float* __restrict__ acc_conv,
+                                         uint32_t t) {
   V3 a = force_of(F, t) * acc_conv[t];
   const float a2 = dot(a, a);
   if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
+  return a;
+}
+
+__device__ __forceinline__ float4 kick_by(float4 vi, V3 a, float h) {
   vi.x += a.x * h;
   vi.y += a.y * h;
   vi.z += a.z * h;
-  v[t] = vi;
+  return vi;
+}
+
+// Kick of solute atom t: v += a·h from the current forces.
+__device__ __forceinline__ void kick_atom(const Fx& F, const float* __restrict__ acc_conv,
+                                          uint32_t t, float h, float4* __restrict__ v) {
+  v[t] = kick_by(v[t], accel_atom(F, acc_conv, t), h);
 }
 
 __device__ __forceinline__ void drift_atom(uint32_t t, float h, float4* __restrict__ x,
@@ -238,42 +247,34 @@ __device__ __forceinline__ void drift_atom(uint32_t t, float h, float4* __restri
   x[t] = xi;
 }
 
-// Kick of the water whose O is site base: the M-site force projected onto
-// O and the H sites.
-__device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h,
-                                           float4* __restrict__ v) {
+// Accelerations of the O and H sites of the water whose O is site base,
+// with the M-site force projected onto them.
+struct WaterAcc {
+  V3 a[3];  // O, H0, H1
+};
+__device__ __forceinline__ WaterAcc water_accel(const Fx& F, uint32_t base) {
   const V3 fm = force_of(F, base + 1);
   const V3 fo = force_of(F, base) + fm * kCO;
   const V3 f0 = force_of(F, base + 2) + fm * kCH;
   const V3 f1 = force_of(F, base + 3) + fm * kCH;
   const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
-  const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
-  const uint32_t ss[3] = {base, base + 2, base + 3};
-  for (int k = 0; k < 3; ++k) {
-    float4 vv = v[ss[k]];
-    vv.x += fs[k].x * h;
-    vv.y += fs[k].y * h;
-    vv.z += fs[k].z * h;
-    v[ss[k]] = vv;
-  }
+  return {{fo * co, f0 * ch, f1 * ch}};
 }
 
-// The same kick on a water held in registers (O at site base).
-__device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h, WaterRegs& W) {
-  const V3 fm = force_of(F, base + 1);
-  const V3 fo = force_of(F, base) + fm * kCO;
-  const V3 f0 = force_of(F, base + 2) + fm * kCH;
-  const V3 f1 = force_of(F, base + 3) + fm * kCH;
-  const float co = kKcalToNative / kMassO, ch = kKcalToNative / kMassH;
-  const V3 fs[3] = {fo * co, f0 * ch, f1 * ch};
+// Kick of the water whose O is site base (M carries no mass).
+__device__ __forceinline__ void kick_water(const Fx& F, uint32_t base, float h,
+                                           float4* __restrict__ v) {
+  const WaterAcc wa = water_accel(F, base);
+  const uint32_t ss[3] = {base, base + 2, base + 3};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[ss[k]] = kick_by(v[ss[k]], wa.a[k], h);
+}
+
+// The same kick on a water held in registers.
+__device__ __forceinline__ void kick_water(const WaterAcc& wa, float h, WaterRegs& W) {
   const int ks[3] = {0, 2, 3};
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    float4& vv = W.v[ks[k]];
-    vv.x += fs[k].x * h;
-    vv.y += fs[k].y * h;
-    vv.z += fs[k].z * h;
-  }
+  for (int k = 0; k < 3; ++k) W.v[ks[k]] = kick_by(W.v[ks[k]], wa.a[k], h);
 }
 
 // Langevin velocity update (Ornstein–Uhlenbeck) of site s: v ← c·v + σ·ξ per
@@ -401,18 +402,6 @@ __device__ __forceinline__ void rattle_star(const ConsArgs& C, uint32_t c, const
       V[k + 1] = f ? na : nb;
     }
   }
-}
-
-// Kick of a solute atom's velocity held in a register.
-__device__ __forceinline__ float4 kick_value(const Fx& F, const float* __restrict__ acc_conv,
-                                             uint32_t t, float h, float4 vi) {
-  V3 a = force_of(F, t) * acc_conv[t];
-  const float a2 = dot(a, a);
-  if (a2 > kMaxAccel * kMaxAccel) a = a * (kMaxAccel / sqrtf(a2));
-  vi.x += a.x * h;
-  vi.y += a.y * h;
-  vi.z += a.z * h;
-  return vi;
 }
 
 __device__ __forceinline__ float4 drift_value(float4 xi, float4 vi, float h) {
@@ -825,33 +814,45 @@ struct StepArgs {
 // kick ½ + drift ½, SHAKE, RATTLE, Langevin, RATTLE, drift ½, SHAKE.
 // Velocity Verlet (:213-230): kick ½ + drift 1, SHAKE, RATTLE.  Water: the
 // same steps with the rigid drift and no constraints.  Clears the group's
-// bonded force sums.
-__device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
-                                            float4* __restrict__ v,
-                                            const uint64_t* __restrict__ step_dev) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.G.n_sgroups + A.G.n_water) return;
+// bonded force sums.  PREV: the group first finishes the previous step
+// (k_md_step_b's kick ½ and RATTLE, with the same forces) in registers, so
+// two steps meet in one launch; the operations and their order are those of
+// the two separate launches.  step: the index of the step that begins here
+// (it seeds the Langevin noise).
+template <bool PREV>
+__device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict__ x,
+                                           float4* __restrict__ v, uint32_t g, uint64_t step) {
   const float hd = A.langevin ? A.h : A.dt;
   if (g < A.G.n_sgroups && A.stars[g].natoms) {
     // a star cluster or a lone atom: the whole sequence in registers
     const StarDesc D = A.stars[g];
     const bool cons = D.n > 0;
     float4 X[kStarMax + 1], V[kStarMax + 1];
+    V3 acc[kStarMax + 1];
     float im[kStarMax + 1];
 #pragma unroll
     for (int k = 0; k <= kStarMax; ++k) {
       const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
       X[k] = x[t];
-      V[k] = kick_value(A.F, A.acc_conv, t, A.h, v[t]);
-      X[k] = drift_value(X[k], V[k], hd);
+      V[k] = v[t];
+      acc[k] = accel_atom(A.F, A.acc_conv, t);
       im[k] = 1.0f / A.C.mass[t];
+    }
+    if (PREV) {
+#pragma unroll
+      for (int k = 0; k <= kStarMax; ++k) V[k] = kick_by(V[k], acc[k], A.h);
+      if (cons) rattle_star(A.C, g, D, im, X, V);
+    }
+#pragma unroll
+    for (int k = 0; k <= kStarMax; ++k) {
+      V[k] = kick_by(V[k], acc[k], A.h);
+      X[k] = drift_value(X[k], V[k], hd);
     }
     if (cons) {
       shake_star(A.C, g, D, im, X);
       rattle_star(A.C, g, D, im, X, V);
     }
     if (A.langevin) {
-      const uint64_t step = *step_dev;
 #pragma unroll
       for (int k = 0; k <= kStarMax; ++k)
         if (k <= (int)D.n)
@@ -877,6 +878,10 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
   if (g < A.G.n_sgroups) {
     const uint32_t a0 = A.G.grp_ptr[g], a1 = A.G.grp_ptr[g + 1];
     const bool cons = g < A.C.n_clusters;
+    if (PREV) {
+      for (uint32_t a = a0; a < a1; ++a) kick_atom(A.F, A.acc_conv, A.G.grp_atoms[a], A.h, v);
+      if (cons) rattle_cluster(A.C, g, x, v);
+    }
     for (uint32_t a = a0; a < a1; ++a) {
       const uint32_t t = A.G.grp_atoms[a];
       kick_atom(A.F, A.acc_conv, t, A.h, v);
@@ -887,7 +892,6 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
       rattle_cluster(A.C, g, x, v);
     }
     if (A.langevin) {
-      const uint64_t step = *step_dev;
       for (uint32_t a = a0; a < a1; ++a) {
         const uint32_t t = A.G.grp_atoms[a];
         langevin_site(v, t, A.lang_c, A.sigma_dyn[t], A.seed, step);
@@ -912,10 +916,11 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
     W.x[k] = x[base + k];
     W.v[k] = v[base + k];
   }
-  kick_water(A.F, base, A.h, W);
+  const WaterAcc wa = water_accel(A.F, base);
+  if (PREV) kick_water(wa, A.h, W);
+  kick_water(wa, A.h, W);
   rigid_drift(W, hd, A.G.B);
   if (A.langevin) {
-    const uint64_t step = *step_dev;
     W.v[0] = langevin_value(W.v[0], base, A.lang_c, A.sigma_o, A.seed, step);
     W.v[2] = langevin_value(W.v[2], base + 2, A.lang_c, A.sigma_h, A.seed, step);
     W.v[3] = langevin_value(W.v[3], base + 3, A.lang_c, A.sigma_h, A.seed, step);
@@ -925,6 +930,38 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
   for (int k = 0; k < 4; ++k) {
     x[base + k] = W.x[k];
     v[base + k] = W.v[k];
+  }
+}
+
+__device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
+                                            float4* __restrict__ v,
+                                            const uint64_t* __restrict__ step_dev) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.G.n_sgroups + A.G.n_water) return;
+  step_group<false>(A, x, v, g, *step_dev);
+}
+
+// The end of one step and the beginning of the next in one launch (no
+// energies or pressure are observed between them).  step_dev[0] is the
+// index of the step that ends here; the launch advances it once every
+// workgroup has read it: each workgroup counts itself in step_dev[1] when
+// done, and the last one stores the new index and clears the count.
+__device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
+                                             float4* __restrict__ v,
+                                             uint64_t* __restrict__ step_dev) {
+  __shared__ uint64_t s_step;
+  if (threadIdx.x == 0) s_step = step_dev[0];
+  __syncthreads();
+  const uint64_t step = s_step;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < A.G.n_sgroups + A.G.n_water) step_group<true>(A, x, v, g, step + 1);
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned* done = reinterpret_cast<unsigned*>(step_dev + 1);
+    if (atomicAdd(done, 1u) == gridDim.x - 1) {
+      step_dev[0] = step + 1;
+      *done = 0u;
+    }
   }
 }
 
@@ -944,7 +981,7 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
     for (int k = 0; k <= kStarMax; ++k) {
       const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
       X[k] = x[t];
-      V[k] = kick_value(A.F, A.acc_conv, t, A.h, v[t]);
+      V[k] = kick_by(v[t], accel_atom(A.F, A.acc_conv, t), A.h);
       im[k] = 1.0f / A.C.mass[t];
     }
     if (D.n) rattle_star(A.C, g, D, im, X, V);
@@ -999,7 +1036,7 @@ struct dyn_md {
   DevBuf<double> eout;
   PinnedBuf<double> h_out;
   uint64_t step = 0;
-  DevBuf<uint64_t> step_dev;  // the step index, as the Langevin kernel reads it
+  DevBuf<uint64_t> step_dev;  // the step index (the Langevin noise's), a workgroup count
   uint64_t n_eval = 0;  // force evaluations since set_state (SPME cadence)
   bool state_set = false, forces_valid = false;
   // pressure groups (solute constraint clusters and lone atoms; waters
@@ -1260,8 +1297,11 @@ void md_barostat(MdSet& S) {
 // the integration before the force evaluation in one launch, the last
 // half-kick after it in another.  Step fractions are in units of each
 // system's dt; the scheme (Langevin or not) must be the same for the whole
-// set.
-void md_one_step(MdSet& S, bool energies) {
+// set.  begun: the previous call's last launch has already done this step's
+// first part.  may_join: nothing observes the state between this step and
+// the next, so this step's last launch may also do the next step's first
+// part (k_md_step_ba); returns whether it did.
+bool md_one_step(MdSet& S, bool energies, bool begun, bool may_join) {
   bool any_invalid = false;
   for (dyn_md* m : S.m) any_invalid = any_invalid || !m->forces_valid;
   if (any_invalid) md_forces(S, false);
@@ -1271,7 +1311,7 @@ void md_one_step(MdSet& S, bool energies) {
   const bool couple = nst > 0 && (S.m[0]->step + 1) % nst == 0;
   const bool langevin = S.m[0]->prm.gamma > 0.f;
   energies = energies || couple;
-  {
+  if (!begun) {
     Profiler::Scope ps(*S.prof, "md_step_a");
     launch<k_md_step_a>(S.R, "k_md_step_a", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
@@ -1280,7 +1320,14 @@ void md_one_step(MdSet& S, bool energies) {
   }
   md_forces(S, energies, /*fb_clear=*/false);
   if (couple && langevin) md_barostat(S);  // before the last half-kick (src/integrate.rs:172-192)
-  {
+  const bool join = may_join && !(couple && !langevin);
+  if (join) {
+    Profiler::Scope ps(*S.prof, "md_step_ba");
+    launch<k_md_step_ba>(S.R, "k_md_step_ba", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+      dyn_md& m = *S.m[i];
+      return args<k_md_step_ba>(step_args(m), m.x.p, m.v.p, m.step_dev.p);
+    });
+  } else {
     Profiler::Scope ps(*S.prof, "md_step_b");
     launch<k_md_step_b>(S.R, "k_md_step_b", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
@@ -1289,6 +1336,7 @@ void md_one_step(MdSet& S, bool energies) {
   }
   if (couple && !langevin) md_barostat(S);  // at the end of the step (src/integrate.rs:241-310)
   for (dyn_md* m : S.m) ++m->step;
+  return join;
 }
 
 // n steps of every MD of the set, then a wait for them.  Later force
@@ -1314,7 +1362,8 @@ void md_run(MdSet& S, uint32_t n) {
     try {
       const auto t0 = std::chrono::steady_clock::now();
       // a barostat step waits mid-call, and may find an overflow there
-      for (uint32_t k = 0; k < n; ++k) md_one_step(S, k + 1 == n);
+      bool begun = false;
+      for (uint32_t k = 0; k < n; ++k) begun = md_one_step(S, k + 1 == n, begun, k + 1 < n);
       const auto t1 = std::chrono::steady_clock::now();
       engine_set_check(S.es);
       const auto t2 = std::chrono::steady_clock::now();
@@ -1571,8 +1620,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     M->v.reserve(ns);
     M->x_ck.reserve(ns);
     M->v_ck.reserve(ns);
-    M->step_dev.reserve(1);
-    DYN_HIP(hipMemsetAsync(M->step_dev.p, 0, 8, M->stream));
+    M->step_dev.reserve(2);  // the step index, the workgroup count of k_md_step_ba
+    DYN_HIP(hipMemsetAsync(M->step_dev.p, 0, 16, M->stream));
     M->eout.reserve(8);
     M->h_out.reserve(8);
     DYN_HIP(hipStreamSynchronize(M->stream));

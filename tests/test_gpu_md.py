@@ -171,3 +171,43 @@ def test_spme_cadence(gpu):
     # kept steps lack the 1-4 vacuum correction (as the reference's do): up
     # to a few kcal/mol/Å on charged chain atoms, a few 0.01 Å in 4 steps
     assert 0.0 < dx < 5e-2
+
+
+@pytest.mark.parametrize("gamma,nst", [(1.0, 0), (0.0, 0), (1.0, 2), (0.0, 2)])
+def test_joined_steps_bit_exact(gpu, gamma, nst):
+    """Inside a call of several steps, one launch ends a step and begins the
+    next (k_md_step_ba); one-step calls never join.  Both give the same
+    trajectory bit for bit: the Langevin noise of each step keeps its index,
+    and a pressure-coupling step of velocity Verlet (the barostat after the
+    last half-kick) is not joined."""
+    s = systems.small_solvated()
+    ff = systems.chain_force_field(s)
+    eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=BETA, grid=(27, 27, 27), order=4,
+                     mode=api.MODE_REFERENCE, list_buffer=1.0)
+    pd0, pw0 = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=100)
+    eng.close()
+    out = []
+    for split in ((5,), (1, 1, 1, 1, 1), (2, 3)):
+        # a 2 Å buffer: no list rebuild inside the 5 steps, whichever call
+        # reads the displacements
+        eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=BETA, grid=(27, 27, 27), order=4,
+                         mode=api.MODE_REFERENCE, list_buffer=2.0)
+        m = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=gamma, shake_tol=1e-5, seed=5,
+                   nstpcouple=nst)
+        v, vw = systems.initial_velocities(s, ff)
+        m.set_state(pd0, v, pw0, vw)
+        for n in split:
+            m.step(n)
+        e = m.energies()
+        out.append((m.state(), eng.energies()["list_builds"], e["pcouple_steps"], e["box"]))
+        m.close()
+        eng.close()
+    ref = out[0]
+    print(f"gamma {gamma} nstpcouple {nst}: list builds {[o[1] for o in out]}, "
+          f"pressure couplings {[o[2] for o in out]}", flush=True)
+    for o in out[1:]:
+        assert o[1] == ref[1] and o[2] == ref[2] and o[3] == ref[3]
+        for a, b in zip(o[0], ref[0]):
+            assert np.array_equal(a, b)
+    if nst:
+        assert ref[2] == 2
