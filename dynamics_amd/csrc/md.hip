@@ -64,6 +64,13 @@ struct Box {
   float L[3];
 };
 // src/barostat.rs:141-170: dv − round(dv/ext)·ext, and rem_euclid wrap into [0, L).
+// Reciprocal, square root and reciprocal square root by the hardware
+// instructions (about 1 ulp): the integrator's constraint, rigid-body and
+// bonded arithmetic needs no correctly rounded division.
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
 __device__ __forceinline__ V3 min_image(V3 d, const Box& b) {
   return {dyn_min_image_ref(d.x, b.L[0]), dyn_min_image_ref(d.y, b.L[1]),
           dyn_min_image_ref(d.z, b.L[2])};
@@ -129,12 +136,12 @@ __device__ __forceinline__ V3 force_of(const Fx& F, uint32_t s) {
 // Rotate r by angle |ω|·dt about ω (Rodrigues; series for tiny angles).
 __device__ __forceinline__ V3 rodrigues(V3 r, V3 w, float dt) {
   const V3 wd = w * dt;
-  const float th = sqrtf(dot(wd, wd));
+  const float th = hw_sqrt(dot(wd, wd));
   if (th < 1e-12f) {
     const V3 wxr = cross(wd, r);
     return r + wxr + cross(wd, wxr) * 0.5f;
   }
-  const V3 n = wd * (1.0f / th);
+  const V3 n = wd * hw_rcp(th);
   float c, s;
   if (th < 0.5f) {
     // Taylor series to th⁹ / th⁸: below 3e-10 absolute for th < 0.5
@@ -187,7 +194,7 @@ __device__ __forceinline__ void rigid_drift(WaterRegs& W, float dt, const Box& B
                     ixz * (ixy * iyz - iyy * ixz);
   V3 w = {0.f, 0.f, 0.f};
   if (fabsf(det) >= 1.0e-12f) {
-    const float id = 1.0f / det;
+    const float id = hw_rcp(det);
     const float a00 = (iyy * izz - iyz * iyz) * id, a01 = (ixz * iyz - ixy * izz) * id,
                 a02 = (ixy * iyz - ixz * iyy) * id, a11 = (ixx * izz - ixz * ixz) * id,
                 a12 = (ixz * ixy - ixx * iyz) * id, a22 = (ixx * iyy - ixy * ixy) * id;
@@ -200,8 +207,8 @@ __device__ __forceinline__ void rigid_drift(WaterRegs& W, float dt, const Box& B
   const V3 wo = {wrap_coord(no.x, B.L[0]), wrap_coord(no.y, B.L[1]), wrap_coord(no.z, B.L[2])};
   const V3 wh0 = wo + min_image(nh0 - wo, B), wh1 = wo + min_image(nh1 - wo, B);
   const V3 bis = (wh0 - wo) + (wh1 - wo);
-  const float bl = sqrtf(dot(bis, bis));
-  const V3 wm = bl > 0.0f ? wo + bis * (kOEpR / bl) : wo;
+  const float bl2 = dot(bis, bis);
+  const V3 wm = bl2 > 0.0f ? wo + bis * (kOEpR * hw_rsqrt(bl2)) : wo;
   W.x[0] = make_float4(wo.x, wo.y, wo.z, 0.f);
   W.x[1] = make_float4(wm.x, wm.y, wm.z, 0.f);
   W.x[2] = make_float4(wh0.x, wh0.y, wh0.z, 0.f);
@@ -310,14 +317,14 @@ __device__ __forceinline__ void shake_cluster(const ConsArgs& C, uint32_t c,
       const float4 a = x[p.x], b = x[p.y];
       const V3 d = v3(b.x - a.x, b.y - a.y, b.z - a.z);
       const float d2 = dot(d, d);
-      const float lam = (d2 - C.r0sq[k]) / (2.0f * C.inv_m[k] * fmaxf(d2, kEpsConstr));
+      const float lam = (d2 - C.r0sq[k]) * hw_rcp(2.0f * C.inv_m[k] * fmaxf(d2, kEpsConstr));
       const V3 corr = d * lam;
-      const float mi = 1.0f / C.mass[p.x], mj = 1.0f / C.mass[p.y];
+      const float mi = hw_rcp(C.mass[p.x]), mj = hw_rcp(C.mass[p.y]);
       x[p.x] = make_float4(a.x + corr.x * mi, a.y + corr.y * mi, a.z + corr.z * mi, a.w);
       x[p.y] = make_float4(b.x - corr.x * mj, b.y - corr.y * mj, b.z - corr.z * mj, b.w);
-      max_corr = fmaxf(max_corr, sqrtf(dot(corr, corr)));
+      max_corr = fmaxf(max_corr, dot(corr, corr));
     }
-    if (max_corr < C.tol) break;
+    if (max_corr < C.tol * C.tol) break;
   }
 }
 
@@ -332,9 +339,9 @@ __device__ __forceinline__ void rattle_cluster(const ConsArgs& C, uint32_t c,
     const float4 va = v[p.x], vb = v[p.y];
     const V3 r = v3(b.x - a.x, b.y - a.y, b.z - a.z);
     const V3 dv = v3(vb.x - va.x, vb.y - va.y, vb.z - va.z);
-    const float lam = dot(dv, r) / (C.inv_m[k] * fmaxf(dot(r, r), kEpsConstr));
+    const float lam = dot(dv, r) * hw_rcp(C.inv_m[k] * fmaxf(dot(r, r), kEpsConstr));
     const V3 corr = r * lam;
-    const float mi = 1.0f / C.mass[p.x], mj = 1.0f / C.mass[p.y];
+    const float mi = hw_rcp(C.mass[p.x]), mj = hw_rcp(C.mass[p.y]);
     v[p.x] = make_float4(va.x + corr.x * mi, va.y + corr.y * mi, va.z + corr.z * mi, va.w);
     v[p.y] = make_float4(vb.x - corr.x * mj, vb.y - corr.y * mj, vb.z - corr.z * mj, vb.w);
   }
@@ -367,16 +374,16 @@ __device__ __forceinline__ void shake_star(const ConsArgs& C, uint32_t c, const 
         const float mi = f ? im[k + 1] : im[0], mj = f ? im[0] : im[k + 1];
         const V3 d = v3(b.x - a.x, b.y - a.y, b.z - a.z);
         const float d2 = dot(d, d);
-        const float lam = (d2 - C.r0sq[k0 + k]) / (2.0f * C.inv_m[k0 + k] * fmaxf(d2, kEpsConstr));
+        const float lam = (d2 - C.r0sq[k0 + k]) * hw_rcp(2.0f * C.inv_m[k0 + k] * fmaxf(d2, kEpsConstr));
         const V3 corr = d * lam;
         const float4 na = make_float4(a.x + corr.x * mi, a.y + corr.y * mi, a.z + corr.z * mi, a.w);
         const float4 nb = make_float4(b.x - corr.x * mj, b.y - corr.y * mj, b.z - corr.z * mj, b.w);
         X[0] = f ? nb : na;
         X[k + 1] = f ? na : nb;
-        max_corr = fmaxf(max_corr, sqrtf(dot(corr, corr)));
+        max_corr = fmaxf(max_corr, dot(corr, corr));
       }
     }
-    if (max_corr < C.tol) break;
+    if (max_corr < C.tol * C.tol) break;
   }
 }
 
@@ -394,7 +401,7 @@ __device__ __forceinline__ void rattle_star(const ConsArgs& C, uint32_t c, const
       const float mi = f ? im[k + 1] : im[0], mj = f ? im[0] : im[k + 1];
       const V3 r = v3(b.x - a.x, b.y - a.y, b.z - a.z);
       const V3 dv = v3(vb.x - va.x, vb.y - va.y, vb.z - va.z);
-      const float lam = dot(dv, r) / (C.inv_m[k0 + k] * fmaxf(dot(r, r), kEpsConstr));
+      const float lam = dot(dv, r) * hw_rcp(C.inv_m[k0 + k] * fmaxf(dot(r, r), kEpsConstr));
       const V3 corr = r * lam;
       const float4 na = make_float4(va.x + corr.x * mi, va.y + corr.y * mi, va.z + corr.z * mi, va.w);
       const float4 nb = make_float4(vb.x - corr.x * mj, vb.y - corr.y * mj, vb.z - corr.z * mj, vb.w);
@@ -455,10 +462,10 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
   if (t < A.n_bond) {
     const uint32_t i = A.bond[2 * t], j = A.bond[2 * t + 1];
     const V3 d = min_image(ld(A.x[j]) - ld(A.x[i]), A.B);
-    const float r = sqrtf(dot(d, d));
+    const float r = hw_sqrt(dot(d, d));
     const float2 p = A.bond_p[t];
     const float dr = r - p.y;
-    const V3 fi = d * (2.0f * p.x * dr / fmaxf(r, 1e-8f));
+    const V3 fi = d * (2.0f * p.x * dr * hw_rcp(fmaxf(r, 1e-8f)));
     add_f(A.f, nd, i, fi);
     add_f(A.f, nd, j, fi * -1.0f);
     e[0] = (double)(p.x * dr * dr);
@@ -470,16 +477,16 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
     const V3 b2 = min_image(ld(A.x[k]) - ld(A.x[j]), A.B);
     const float l0s = dot(b0, b0), l2s = dot(b2, b2);
     if (l0s >= 1e-8f && l2s >= 1e-8f) {
-      const float l0 = sqrtf(l0s), l2 = sqrtf(l2s);
-      const V3 u0 = b0 * (1.0f / l0), u2 = b2 * (1.0f / l2);
+      const float il0 = hw_rsqrt(l0s), il2 = hw_rsqrt(l2s);
+      const V3 u0 = b0 * il0, u2 = b2 * il2;
       const float ct = fminf(1.0f, fmaxf(-1.0f, dot(u0, u2)));
       const float th = acosf(ct);
-      const float st = fmaxf(sqrtf(1.0f - ct * ct), 1e-8f);
+      const float ist = hw_rcp(fmaxf(hw_sqrt(1.0f - ct * ct), 1e-8f));
       const float2 p = A.angle_p[q];
       const float dth = th - p.y;
       const float dedt = 2.0f * p.x * dth;
-      const V3 g0 = (u0 * ct - u2) * (1.0f / (l0 * st));
-      const V3 g2 = (u2 * ct - u0) * (1.0f / (l2 * st));
+      const V3 g0 = (u0 * ct - u2) * (il0 * ist);
+      const V3 g2 = (u2 * ct - u0) * (il2 * ist);
       const V3 f0 = g0 * -dedt, f2 = g2 * -dedt;
       add_f(A.f, nd, i, f0);
       add_f(A.f, nd, k, f2);
@@ -496,14 +503,14 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
     const V3 b3 = min_image(ld(A.x[l]) - ld(A.x[k]), A.B);
     const V3 n1 = cross(b1, b2), n2 = cross(b2, b3);
     const float n1s = dot(n1, n1), n2s = dot(n2, n2), b2s = dot(b2, b2);
-    const float b2l = sqrtf(b2s);
+    const float b2l = hw_sqrt(b2s);
     if (n1s >= 1e-6f && n2s >= 1e-6f && b2l >= 1e-6f) {
       // cos φ, sin φ of the IUPAC torsion φ = atan2(|b2| b1·n2, n1·n2); then
       // cos nφ, sin nφ by the angle-addition recurrence and the phase by one
       // more rotation: cos(nφ − γ), sin(nφ − γ) without trigonometric calls
       const float4 p = A.dihe_p[q];
       const float y = b2l * dot(b1, n2), x = dot(n1, n2);
-      const float rinv = 1.0f / sqrtf(x * x + y * y);
+      const float rinv = hw_rsqrt(x * x + y * y);
       const float c1 = x * rinv, s1 = y * rinv;
       float cn = 1.0f, sn = 0.0f;
       const int nper = (int)p.y;
@@ -517,8 +524,9 @@ __device__ __forceinline__ void k_md_bonded(BondedArgs A) {
       const float dedphi = -p.x * p.y * sin_arg;
       // ∂φ/∂x_i = −|b2|/|n1|² n1, ∂φ/∂x_l = |b2|/|n2|² n2; the inner atoms
       // take the lever-arm split of the two
-      const V3 gi = n1 * (-b2l / n1s), gl = n2 * (b2l / n2s);
-      const float ta = dot(b1, b2) / b2s, tb = dot(b3, b2) / b2s;
+      const V3 gi = n1 * (-b2l * hw_rcp(n1s)), gl = n2 * (b2l * hw_rcp(n2s));
+      const float ib2s = hw_rcp(b2s);
+      const float ta = dot(b1, b2) * ib2s, tb = dot(b3, b2) * ib2s;
       const V3 gj = gl * tb - gi * (1.0f + ta);
       const V3 gk = gi * ta - gl * (1.0f + tb);
       add_f(A.f, nd, i, gi * -dedphi);
@@ -808,6 +816,7 @@ struct StepArgs {
   uint64_t seed;
   int64_t* fb;            // bonded force sums to clear for the evaluation
   const StarDesc* stars;  // per solute pressure group
+  uint32_t water0;        // thread of the first water (n_sgroups rounded up to a wave)
 };
 
 // Before the force evaluation.  Langevin-middle (src/integrate.rs:100-160):
@@ -817,11 +826,14 @@ struct StepArgs {
 // bonded force sums.  PREV: the group first finishes the previous step
 // (k_md_step_b's kick ½ and RATTLE, with the same forces) in registers, so
 // two steps meet in one launch; the operations and their order are those of
-// the two separate launches.  step: the index of the step that begins here
-// (it seeds the Langevin noise).
+// the two separate launches.  The index of the step that begins here (it
+// seeds the Langevin noise) is *step_dev + PREV.  Thread g < n_sgroups
+// holds a solute group, thread water0 + w water w (water0 rounds n_sgroups
+// up to a whole wave, so no wave runs both paths).
 template <bool PREV>
 __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict__ x,
-                                           float4* __restrict__ v, uint32_t g, uint64_t step) {
+                                           float4* __restrict__ v, uint32_t g,
+                                           const uint64_t* __restrict__ step_dev) {
   const float hd = A.langevin ? A.h : A.dt;
   if (g < A.G.n_sgroups && A.stars[g].natoms) {
     // a star cluster or a lone atom: the whole sequence in registers
@@ -829,14 +841,15 @@ __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict
     const bool cons = D.n > 0;
     float4 X[kStarMax + 1], V[kStarMax + 1];
     V3 acc[kStarMax + 1];
-    float im[kStarMax + 1];
+    float im[kStarMax + 1], sg[kStarMax + 1];
 #pragma unroll
     for (int k = 0; k <= kStarMax; ++k) {
       const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
       X[k] = x[t];
       V[k] = v[t];
       acc[k] = accel_atom(A.F, A.acc_conv, t);
-      im[k] = 1.0f / A.C.mass[t];
+      im[k] = hw_rcp(A.C.mass[t]);
+      sg[k] = A.langevin ? A.sigma_dyn[t] : 0.f;
     }
     if (PREV) {
 #pragma unroll
@@ -853,10 +866,10 @@ __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict
       rattle_star(A.C, g, D, im, X, V);
     }
     if (A.langevin) {
+      const uint64_t step = *step_dev + PREV;
 #pragma unroll
       for (int k = 0; k <= kStarMax; ++k)
-        if (k <= (int)D.n)
-          V[k] = langevin_value(V[k], D.atom[k], A.lang_c, A.sigma_dyn[D.atom[k]], A.seed, step);
+        if (k <= (int)D.n) V[k] = langevin_value(V[k], D.atom[k], A.lang_c, sg[k], A.seed, step);
       if (cons) rattle_star(A.C, g, D, im, X, V);
 #pragma unroll
       for (int k = 0; k <= kStarMax; ++k) X[k] = drift_value(X[k], V[k], A.h);
@@ -892,6 +905,7 @@ __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict
       rattle_cluster(A.C, g, x, v);
     }
     if (A.langevin) {
+      const uint64_t step = *step_dev + PREV;
       for (uint32_t a = a0; a < a1; ++a) {
         const uint32_t t = A.G.grp_atoms[a];
         langevin_site(v, t, A.lang_c, A.sigma_dyn[t], A.seed, step);
@@ -909,7 +923,8 @@ __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict
     }
     return;
   }
-  const uint32_t base = A.G.n_dyn + 4u * (g - A.G.n_sgroups);
+  if (g < A.water0) return;
+  const uint32_t base = A.G.n_dyn + 4u * (g - A.water0);
   WaterRegs W;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -921,6 +936,7 @@ __device__ __forceinline__ void step_group(const StepArgs& A, float4* __restrict
   kick_water(wa, A.h, W);
   rigid_drift(W, hd, A.G.B);
   if (A.langevin) {
+    const uint64_t step = *step_dev + PREV;
     W.v[0] = langevin_value(W.v[0], base, A.lang_c, A.sigma_o, A.seed, step);
     W.v[2] = langevin_value(W.v[2], base + 2, A.lang_c, A.sigma_h, A.seed, step);
     W.v[3] = langevin_value(W.v[3], base + 3, A.lang_c, A.sigma_h, A.seed, step);
@@ -937,29 +953,27 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
                                             float4* __restrict__ v,
                                             const uint64_t* __restrict__ step_dev) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.G.n_sgroups + A.G.n_water) return;
-  step_group<false>(A, x, v, g, *step_dev);
+  if (g >= A.water0 + A.G.n_water) return;
+  step_group<false>(A, x, v, g, step_dev);
 }
 
 // The end of one step and the beginning of the next in one launch (no
 // energies or pressure are observed between them).  step_dev[0] is the
 // index of the step that ends here; the launch advances it once every
-// workgroup has read it: each workgroup counts itself in step_dev[1] when
-// done, and the last one stores the new index and clears the count.
+// workgroup has used it: each workgroup counts itself in step_dev[1] when
+// its threads are done, and the last one stores the new index and clears
+// the count.
 __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
                                              float4* __restrict__ v,
                                              uint64_t* __restrict__ step_dev) {
-  __shared__ uint64_t s_step;
-  if (threadIdx.x == 0) s_step = step_dev[0];
-  __syncthreads();
-  const uint64_t step = s_step;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < A.G.n_sgroups + A.G.n_water) step_group<true>(A, x, v, g, step + 1);
+  if (g < A.water0 + A.G.n_water) step_group<true>(A, x, v, g, step_dev);
+  __syncthreads();
   if (threadIdx.x == 0) {
     __threadfence();
     unsigned* done = reinterpret_cast<unsigned*>(step_dev + 1);
     if (atomicAdd(done, 1u) == gridDim.x - 1) {
-      step_dev[0] = step + 1;
+      step_dev[0] = step_dev[0] + 1;
       *done = 0u;
     }
   }
@@ -972,7 +986,7 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
                                             uint64_t* __restrict__ step_inc) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g == 0) ++*step_inc;
-  if (g >= A.G.n_sgroups + A.G.n_water) return;
+  if (g >= A.water0 + A.G.n_water) return;
   if (g < A.G.n_sgroups && A.stars[g].natoms) {
     const StarDesc D = A.stars[g];
     float4 X[kStarMax + 1], V[kStarMax + 1];
@@ -982,7 +996,7 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
       const uint32_t t = D.atom[k <= (int)D.n ? k : 0];
       X[k] = x[t];
       V[k] = kick_by(v[t], accel_atom(A.F, A.acc_conv, t), A.h);
-      im[k] = 1.0f / A.C.mass[t];
+      im[k] = hw_rcp(A.C.mass[t]);
     }
     if (D.n) rattle_star(A.C, g, D, im, X, V);
 #pragma unroll
@@ -996,7 +1010,7 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
     if (g < A.C.n_clusters) rattle_cluster(A.C, g, x, v);
     return;
   }
-  kick_water(A.F, A.G.n_dyn + 4u * (g - A.G.n_sgroups), A.h, v);
+  if (g >= A.water0) kick_water(A.F, A.G.n_dyn + 4u * (g - A.water0), A.h, v);
 }
 
 }  // namespace
@@ -1232,6 +1246,16 @@ double c_rescale_mu(const dyn_md& m, double p_bar, uint64_t step) {
   return std::exp(dlnv / 3.0);
 }
 
+// Thread of the first water in the step kernels: the solute groups rounded
+// up to a whole wave.
+uint32_t step_water0(const dyn_md& m) { return (m.n_sgroups + 63u) / 64u * 64u; }
+
+unsigned step_grid(const MdSet& S) {
+  unsigned g = 1;
+  for (dyn_md* m : S.m) g = std::max(g, grid_for(step_water0(*m) + m->n_water, kBlock));
+  return g;
+}
+
 StepArgs step_args(const dyn_md& m) {
   StepArgs A;
   std::memset(&A, 0, sizeof A);
@@ -1252,6 +1276,7 @@ StepArgs step_args(const dyn_md& m) {
   A.seed = m.prm.seed;
   A.fb = m.fb.p;
   A.stars = m.stars.p;
+  A.water0 = step_water0(m);
   return A;
 }
 
@@ -1313,7 +1338,7 @@ bool md_one_step(MdSet& S, bool energies, bool begun, bool may_join) {
   energies = energies || couple;
   if (!begun) {
     Profiler::Scope ps(*S.prof, "md_step_a");
-    launch<k_md_step_a>(S.R, "k_md_step_a", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+    launch<k_md_step_a>(S.R, "k_md_step_a", 0, step_grid(S), kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
       return args<k_md_step_a>(step_args(m), m.x.p, m.v.p, (const uint64_t*)m.step_dev.p);
     });
@@ -1323,13 +1348,13 @@ bool md_one_step(MdSet& S, bool energies, bool begun, bool may_join) {
   const bool join = may_join && !(couple && !langevin);
   if (join) {
     Profiler::Scope ps(*S.prof, "md_step_ba");
-    launch<k_md_step_ba>(S.R, "k_md_step_ba", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+    launch<k_md_step_ba>(S.R, "k_md_step_ba", 0, step_grid(S), kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
       return args<k_md_step_ba>(step_args(m), m.x.p, m.v.p, m.step_dev.p);
     });
   } else {
     Profiler::Scope ps(*S.prof, "md_step_b");
-    launch<k_md_step_b>(S.R, "k_md_step_b", 0, group_grid(S), kBlock, 0, [&](uint32_t i) {
+    launch<k_md_step_b>(S.R, "k_md_step_b", 0, step_grid(S), kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
       return args<k_md_step_b>(step_args(m), m.x.p, m.v.p, m.step_dev.p);
     });
