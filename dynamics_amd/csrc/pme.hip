@@ -256,6 +256,47 @@ __device__ C* lds_fft(C* a, C* b, int nl, const FftAxis& ax, const C* __restrict
   return a;
 }
 
+// The stage of fft_stage on nl lines stored interleaved (element n of line l
+// at a[n·NL + l]): consecutive threads take consecutive lines, so a wave's
+// LDS accesses are contiguous.  Every line sees the arithmetic of fft_stage.
+template <int SIGN, int N, int R, int NS, int NL, class C>
+__device__ __forceinline__ void fft_stage_t(const C* __restrict__ a, C* __restrict__ b,
+                                            const C* __restrict__ tw) {
+  constexpr int NR = N / R, TWS = N / (NS * R);
+  for (int i = threadIdx.x; i < NL * NR; i += blockDim.x) {
+    const int j = i / NL;
+    const int l = i - j * NL;
+    const int k = j % NS;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      C x = a[(j + r * NR) * NL + l];
+      if (r) x = ctw<SIGN>(x, tw[(r * k * TWS) % N]);
+      v[r] = x;
+    }
+    if constexpr (R == 4) dft4<SIGN>(v);
+    else dft2<SIGN>(v);
+    const int d = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[(d + r * NS) * NL + l] = v[r];
+  }
+  __syncthreads();
+}
+
+// lds_fft_pow2 on NL interleaved lines.
+template <int SIGN, int N, int NL, class C>
+__device__ __forceinline__ C* lds_fft_pow2_t(C* a, C* b, const C* __restrict__ tw) {
+  static_assert(N == 32 || N == 64, "plane passes for 32 and 64 points");
+  fft_stage_t<SIGN, N, 4, 1, NL>(a, b, tw);
+  fft_stage_t<SIGN, N, 4, 4, NL>(b, a, tw);
+  if constexpr (N == 32) {
+    fft_stage_t<SIGN, N, 2, 16, NL>(a, b, tw);
+  } else {
+    fft_stage_t<SIGN, N, 4, 16, NL>(a, b, tw);
+  }
+  return b;
+}
+
 struct PassGeom {
   int K[3];
   int lines;      // total lines in this pass
@@ -321,6 +362,42 @@ k_fft_pass(C* __restrict__ data, const int64_t* __restrict__ grid_fx, PassGeom g
     split_e<AXIS>(e, nl, N, l, n);
     data[pass_addr<AXIS>(g, line0 + l, n)] = Rs[l * N + n];
   }
+}
+
+// The z and y passes of one direction in one launch (single precision,
+// fixed plans, NY·NZ ≤ kPlaneCap): a workgroup holds the x plane of its
+// block index in LDS and transforms its z lines then its y lines (forward,
+// from the fixed-point charge grid) or its y lines then its z lines
+// (inverse).  Every line sees the stages, twiddles and order of the
+// separate passes, so the result is the same to the bit.
+constexpr int kPlaneCap = 4096;
+constexpr int kPlaneBlock = 512;
+template <int SIGN, bool FROM_FX, int NY, int NZ>
+__device__ __forceinline__ void k_fft_plane(float2* __restrict__ data,
+                                            const int64_t* __restrict__ grid_fx, int nx,
+                                            const float2* __restrict__ twy,
+                                            const float2* __restrict__ twz) {
+  static_assert(NY * NZ <= kPlaneCap, "plane exceeds the LDS buffers");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  float2* A = reinterpret_cast<float2*>(smem_raw);
+  float2* B = A + NY * NZ;
+  const int x = blockIdx.x;
+  if (x >= nx) return;  // beyond this system's planes
+  const size_t base = (size_t)x * NY * NZ;
+  for (int e = threadIdx.x; e < NY * NZ; e += blockDim.x) {
+    if (FROM_FX) A[e] = make_float2((float)((double)grid_fx[base + e] * (1.0 / DYN_GRID_FX_SCALE)), 0.0f);
+    else A[e] = data[base + e];
+  }
+  __syncthreads();
+  float2* out;
+  if (SIGN < 0) {
+    float2* rz = lds_fft_pow2<SIGN, NZ>(A, B, NY, twz);  // NY contiguous z lines
+    out = lds_fft_pow2_t<SIGN, NY, NZ>(rz, rz == A ? B : A, twy);
+  } else {
+    float2* ry = lds_fft_pow2_t<SIGN, NY, NZ>(A, B, twy);
+    out = lds_fft_pow2<SIGN, NZ>(ry, ry == A ? B : A, NY, twz);
+  }
+  for (int e = threadIdx.x; e < NY * NZ; e += blockDim.x) data[base + e] = out[e];
 }
 
 struct ConvArgs {
@@ -1077,13 +1154,42 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
     case 128: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 128>)), TF{}, NAME, G, AXIS, B, SH, FX); break; \
     default: fft(DYN_IC((k_fft_pass<AXIS, SIGN, FX, float2, 0>)), TF{}, NAME, G, AXIS, B, SH, FX); \
   }
+  // the z and y passes of each direction as one plane launch when every
+  // single-precision system shares fixed-plan y and z lengths
+  auto plane = [&](auto F, const char* name, bool from_fx, int ny, int nz) {
+    unsigned gp = 1;
+    for (uint32_t i = 0; i < ns; ++i)
+      if (mine(i, false, 0)) gp = std::max(gp, cp(i)->K[0]);
+    launch<decltype(F)::value, kPlaneBlock>(R, name, tag, gp, kPlaneBlock,
+                                            (size_t)2 * ny * nz * sizeof(float2), [&](uint32_t i) {
+      const bool m = mine(i, false, 0);
+      return args<decltype(F)::value>(m ? cp(i)->cgrid.p : nullptr,
+                                      from_fx && m ? cp(i)->grid_fx.p : nullptr,
+                                      m ? (int)cp(i)->K[0] : 0, m ? cp(i)->tw[1].p : nullptr,
+                                      m ? cp(i)->tw[2].p : nullptr);
+    });
+  };
+#define DYN_PLANE(SIGN, FX, NAME)                                                                  \
+  switch (ny * 1000 + nz) {                                                                       \
+    case 32032: plane(DYN_IC((k_fft_plane<SIGN, FX, 32, 32>)), NAME, FX, 32, 32); break;           \
+    case 32064: plane(DYN_IC((k_fft_plane<SIGN, FX, 32, 64>)), NAME, FX, 32, 64); break;           \
+    case 64032: plane(DYN_IC((k_fft_plane<SIGN, FX, 64, 32>)), NAME, FX, 64, 32); break;           \
+    default: plane(DYN_IC((k_fft_plane<SIGN, FX, 64, 64>)), NAME, FX, 64, 64);                     \
+  }
   for (int pr = 0; pr < 2; ++pr) {
     if (!prec[pr]) continue;
     if (pr == 0) {
-      { Profiler::Scope ps(prof, "pme_fft_z_fwd");
-        DYN_FFT_F32(2, -1, true, "k_fft_pass z fwd", gz, bz, shz) }
-      { Profiler::Scope ps(prof, "pme_fft_y_fwd");
-        DYN_FFT_F32(1, -1, false, "k_fft_pass y fwd", gy, by, shy) }
+      const int ny = shared_n(1), nz = shared_n(2);
+      const bool planes = ny > 0 && nz > 0 && ny <= 64 && nz <= 64;
+      if (planes) {
+        Profiler::Scope ps(prof, "pme_fft_zy_fwd");
+        DYN_PLANE(-1, true, "k_fft_plane fwd")
+      } else {
+        { Profiler::Scope ps(prof, "pme_fft_z_fwd");
+          DYN_FFT_F32(2, -1, true, "k_fft_pass z fwd", gz, bz, shz) }
+        { Profiler::Scope ps(prof, "pme_fft_y_fwd");
+          DYN_FFT_F32(1, -1, false, "k_fft_pass y fwd", gy, by, shy) }
+      }
       { Profiler::Scope ps(prof, "pme_fft_x_conv");
         switch (shared_n(0)) {
           case 32: conv(DYN_IC((k_fft_conv_x<float2, 32>)), TF{}, "k_fft_conv_x"); break;
@@ -1091,10 +1197,15 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
           case 128: conv(DYN_IC((k_fft_conv_x<float2, 128>)), TF{}, "k_fft_conv_x"); break;
           default: conv(DYN_IC((k_fft_conv_x<float2, 0>)), TF{}, "k_fft_conv_x");
         } }
-      { Profiler::Scope ps(prof, "pme_fft_y_inv");
-        DYN_FFT_F32(1, 1, false, "k_fft_pass y inv", gy, by, shy) }
-      { Profiler::Scope ps(prof, "pme_fft_z_inv");
-        DYN_FFT_F32(2, 1, false, "k_fft_pass z inv", gz, bz, shz) }
+      if (planes) {
+        Profiler::Scope ps(prof, "pme_fft_yz_inv");
+        DYN_PLANE(1, false, "k_fft_plane inv")
+      } else {
+        { Profiler::Scope ps(prof, "pme_fft_y_inv");
+          DYN_FFT_F32(1, 1, false, "k_fft_pass y inv", gy, by, shy) }
+        { Profiler::Scope ps(prof, "pme_fft_z_inv");
+          DYN_FFT_F32(2, 1, false, "k_fft_pass z inv", gz, bz, shz) }
+      }
     } else {
       { Profiler::Scope ps(prof, "pme_fft_z_fwd");
         fft(DYN_IC((k_fft_pass<2, -1, true, double2, 0>)), TD{}, "k_fft_pass z fwd f64", gz, 2, bz, shz, true); }
@@ -1109,6 +1220,7 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
     }
   }
 #undef DYN_FFT_F32
+#undef DYN_PLANE
   {
     Profiler::Scope ps(prof, "pme_gather");
     const unsigned gg = std::max(1u, (g_n * 256 + kGatherBlock - 1) / kGatherBlock);
