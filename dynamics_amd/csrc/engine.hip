@@ -1086,6 +1086,8 @@ struct PrepArgs {
   int64_t* fx_recip;  // null: SPME forces kept
   int64_t* mol_e;
   uint32_t n_mol2;
+  BinArgs bin;        // SPME binning of the charges written here
+  int bin_on;
 };
 
 __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
@@ -1113,12 +1115,18 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
   }
   if (A.mol_e)
     for (uint32_t k = t; k < A.n_mol2; k += gridDim.x * blockDim.x) A.mol_e[k] = 0;
-  if (A.pme_xyzq && t < A.nq) {
-    const uint32_t site = A.qsite[t];
-    const float4 p = wrapped(A.pos, A.q, A.g, site);
-    A.pme_xyzq[t] = p;
-    if (A.pme_b)
-      A.pme_b[t] = make_float4(p.x, p.y, p.z, site >= A.alch0 && site < A.alch1 ? -p.w : p.w);
+  if (A.pme_xyzq) {
+    const bool qv = t < A.nq;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (qv) {
+      const uint32_t site = A.qsite[t];
+      p = wrapped(A.pos, A.q, A.g, site);
+      A.pme_xyzq[t] = p;
+      if (A.pme_b)
+        A.pme_b[t] = make_float4(p.x, p.y, p.z, site >= A.alch0 && site < A.alch1 ? -p.w : p.w);
+    }
+    // every lane of the wave takes part in the binning
+    if (A.bin_on) pme_bin_charge(A.bin, p, qv, t);
   }
   if (t < *A.n_cl * DYN_CL_SIZE) {
     const int32_t s = A.slot_site[t];
@@ -1751,6 +1759,11 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       A.fx_recip = pl.recip_on ? e.fx_recip.p : nullptr;
       A.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
       A.n_mol2 = e.n_mol * e.n_mol;
+      if (pl.recip_on) {  // the SPME charges are listed under their bricks here
+        e.pme.ensure_cap(e.n_q);
+        A.bin = e.pme.bin_args(e.status.p, kStPme);
+        A.bin_on = 1;
+      }
       return args<k_eng_prep>(A);
     });
     const int k = D.next;
@@ -1921,7 +1934,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       r.res = e.eng_out.p + 16;
       r.status = e.status.p;
       r.status_bit = kStPme;
+      r.binned = fused;  // the preparation launch has listed the charges
       ra[i] = r;
+      r.binned = false;
       if (pl.recip_b) {
         r.xyzq = e.pme_b.p;
         r.fscale = 0.5f * (1.0f - s);

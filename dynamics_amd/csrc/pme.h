@@ -18,8 +18,63 @@
 
 #include "batch.h"
 #include "common.h"
+#include "wave.h"
 
 namespace dyn {
+
+constexpr int kBrick = 8;  // grid points per brick edge
+
+// Fractional coordinate → base index (mod K) and in-cell weight.
+template <class R>
+__device__ __forceinline__ void frac_coord(R x, R sc, int K, int& k0, R& w) {
+  const R u = x * sc;
+  const R fl = floor(u);
+  w = u - fl;
+  int k = (int)fl % K;
+  if (k < 0) k += K;
+  k0 = k;
+}
+
+__device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk[3]) {
+  return ((uint32_t)(k0[0] / kBrick) * nbk[1] + (uint32_t)(k0[1] / kBrick)) * nbk[2] +
+         (uint32_t)(k0[2] / kBrick);
+}
+
+// What listing charges under their home bricks needs: grid size and K/L
+// per axis, bricks per axis, the per-brick counts and fixed-capacity lists,
+// and where an overflow is flagged.
+struct BinArgs {
+  int K[3];
+  float sc[3];
+  uint32_t nb[3];
+  uint32_t* count;
+  uint32_t* items;
+  uint32_t cap;
+  unsigned int* status;
+  unsigned int status_bit;
+};
+
+// Lists charge i (position p) under its home brick.  Every lane of the wave
+// must call it (valid marks the lanes with a charge): one atomic per
+// distinct brick of the wave.
+__device__ __forceinline__ void pme_bin_charge(const BinArgs& B, float4 p, bool valid,
+                                               uint32_t i) {
+  uint32_t b = 0xFFFFFFFFu;  // no brick
+  if (valid) {
+    const float xs[3] = {p.x, p.y, p.z};
+    int k0[3];
+    for (int a = 0; a < 3; ++a) {
+      float w;
+      frac_coord<float>(xs[a], B.sc[a], B.K[a], k0[a], w);
+    }
+    b = brick_of(k0, B.nb);
+  }
+  const uint32_t rank = wave_rank(b, valid, B.count);
+  if (valid) {
+    if (rank < B.cap) B.items[(size_t)b * B.cap + rank] = i;
+    else atomicOr(B.status, B.status_bit);
+  }
+}
 
 struct FftAxis {
   int N;
@@ -67,6 +122,8 @@ struct PmeCore {
   DevBuf<unsigned int> status;   // standalone runs: 1 = a brick list overflowed
   void ensure_cap(uint32_t n);
   void grow();
+  // the binning arguments of the current grid and lists (ensure_cap first)
+  BinArgs bin_args(unsigned int* status, unsigned int status_bit) const;
 
   void init(int device, hipStream_t s, Profiler* p, const uint32_t k[3], const float l[3],
             float beta, int order, bool fp64 = false);
@@ -102,6 +159,7 @@ struct PmeRun {
   double* res = nullptr;  // [energy, virial]
   unsigned int* status = nullptr;  // a full brick list sets status_bit here (null: the core's)
   unsigned int status_bit = 0;
+  bool binned = false;  // the charges are already listed (bin_args, after ensure_cap(n))
 };
 
 // runs[0 .. R.n): one per system of the pass; tag separates passes that run

@@ -510,54 +510,16 @@ struct SplineArgs {
   double scd[3];  // K/L in double (FP64 option)
 };
 
-// Fractional coordinate → base index (mod K) and in-cell weight.
-template <class R>
-__device__ __forceinline__ void frac_coord(R x, R sc, int K, int& k0, R& w) {
-  const R u = x * sc;
-  const R fl = floor(u);
-  w = u - fl;
-  int k = (int)fl % K;
-  if (k < 0) k += K;
-  k0 = k;
-}
-
-constexpr int kBrick = 8;          // grid points per brick edge
 constexpr int kBrickBlock = 256;
-
-__device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk[3]) {
-  return ((uint32_t)(k0[0] / kBrick) * nbk[1] + (uint32_t)(k0[1] / kBrick)) * nbk[2] +
-         (uint32_t)(k0[2] / kBrick);
-}
 
 // Per charge: its home brick (the brick of its base grid point) and a rank
 // in it; the charge goes to slot rank of the brick's fixed-capacity list
 // (cap slots per brick).  A full brick sets *status |= status_bit (the caller
 // enlarges the lists and evaluates again).  The rank order depends on atomic
 // timing; the brick sums are integers, so the spread grid does not.
-__device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, SplineArgs S,
-                          uint32_t nbx, uint32_t nby, uint32_t nbz,
-                          uint32_t* __restrict__ count, uint32_t* __restrict__ items,
-                          uint32_t cap, unsigned int* __restrict__ status,
-                          unsigned int status_bit) {
+__device__ __forceinline__ void k_pme_bin(const float4* __restrict__ xyzq, uint32_t n, BinArgs B) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t b = 0xFFFFFFFFu;  // no brick (beyond n)
-  if (i < n) {
-    const float4 p = xyzq[i];
-    const float xs[3] = {p.x, p.y, p.z};
-    int k0[3];
-    for (int a = 0; a < 3; ++a) {
-      float w;
-      frac_coord<float>(xs[a], S.sc[a], S.K[a], k0[a], w);
-    }
-    const uint32_t nb[3] = {nbx, nby, nbz};
-    b = brick_of(k0, nb);
-  }
-  // one atomic per distinct brick of the wave (neighbouring charges share bricks)
-  const uint32_t rank = wave_rank(b, i < n, count);
-  if (i < n) {
-    if (rank < cap) items[(size_t)b * cap + rank] = i;
-    else atomicOr(status, status_bit);
-  }
+  pme_bin_charge(B, i < n ? xyzq[i] : make_float4(0.f, 0.f, 0.f, 0.f), i < n, i);
 }
 
 // Home bricks along one axis whose charges can reach the points [lo, hi]
@@ -955,6 +917,21 @@ void PmeCore::ensure_cap(uint32_t n) {
   brick_items.reserve(nb * cap_brick);
 }
 
+BinArgs PmeCore::bin_args(unsigned int* st, unsigned int bit) const {
+  BinArgs B;
+  std::memset(&B, 0, sizeof B);
+  for (int a = 0; a < 3; ++a) {
+    B.K[a] = (int)K[a];
+    B.sc[a] = (float)((double)K[a] / (double)L[a]);
+    B.nb[a] = nbk[a];
+  }
+  B.count = brick_count.p;
+  B.items = brick_items.p;
+  B.cap = cap_brick;
+  B.status = st ? st : status.p;
+  B.status_bit = st ? bit : 1u;
+  return B;
+}
 void PmeCore::grow() {
   cap_brick = std::max<uint32_t>(64, 2 * cap_brick);
   brick_items.reserve((size_t)nbk[0] * nbk[1] * nbk[2] * cap_brick);
@@ -1045,14 +1022,16 @@ void pme_run(const BatchRun& R, Profiler& prof, const PmeRun* runs, int tag) {
   // the grid needs no clearing
   {
     Profiler::Scope ps(prof, "pme_spread");
-    launch<k_pme_bin>(R, "k_pme_bin", tag, g_n, 256, 0, [&](uint32_t i) {
-      const bool o = on(i);
-      return args<k_pme_bin>(runs[i].xyzq, nq(i), S[i], nb(i, 0), nb(i, 1), nb(i, 2),
-                             o ? cp(i)->brick_count.p : nullptr,
-                             o ? cp(i)->brick_items.p : nullptr, o ? cp(i)->cap_brick : 0u,
-                             o ? (runs[i].status ? runs[i].status : cp(i)->status.p) : nullptr,
-                             o ? (runs[i].status ? runs[i].status_bit : 1u) : 0u);
-    });
+    bool bin_any = false;
+    for (uint32_t i = 0; i < ns; ++i) bin_any = bin_any || (on(i) && !runs[i].binned);
+    if (bin_any)
+      launch<k_pme_bin>(R, "k_pme_bin", tag, g_n, 256, 0, [&](uint32_t i) {
+        const bool o = on(i) && !runs[i].binned;
+        BinArgs B;
+        std::memset(&B, 0, sizeof B);
+        if (o) B = cp(i)->bin_args(runs[i].status, runs[i].status_bit);
+        return args<k_pme_bin>(runs[i].xyzq, o ? nq(i) : 0u, B);
+      });
     auto spread = [&](auto F, const char* name, bool f64, int order) {
       launch<decltype(F)::value, kBrickBlock>(R, name, tag, g_b, kBrickBlock, 0, [&](uint32_t i) {
         const bool m = mine(i, f64, order);
