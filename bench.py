@@ -37,6 +37,27 @@ FP32_PEAK_TFLOPS = 157.3
 FLOPS_LJ_COUL, FLOPS_LJ, FLOPS_COUL = 63, 50, 51
 
 
+# list buffer of the MD modes (Å): the list is rebuilt when a site has moved
+# 0.3 × buffer since the last build (readings at most two steps old)
+MD_BUFFER = 2.5
+
+
+def equilibrate(eng, ff, s, pd, pw, steps, seed=7):
+    """Untimed Langevin MD at 300 K with γ = 10/ps from the relaxed
+    structure (Maxwell–Boltzmann start), so that the timed steps run on a
+    thermal configuration.  Returns the state (pos_dyn, vel_dyn, pos_water,
+    vel_water)."""
+    from dynamics_amd import api, systems
+    v, vw = systems.initial_velocities(s, ff, 300.0)
+    m = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=10.0, shake_tol=1e-5, seed=seed)
+    m.set_state(pd, v, pw, vw)
+    if steps:
+        m.step(steps)
+    st = m.state()
+    m.close()
+    return st
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,11 +79,16 @@ def main() -> int:
                     help="batch modes: number of systems (16 windows / 128 boxes)")
     ap.add_argument("--relax-steps", type=int, default=400,
                     help="untimed relaxation steps that quench the built chain's close contacts")
+    ap.add_argument("--equil-steps", type=int, default=2000,
+                    help="untimed Langevin steps (γ = 10/ps, 300 K) that thermalise the "
+                         "relaxed structure before the timed MD")
     ap.add_argument("--list-buffer", type=float, default=None,
-                    help="list buffer in Å (md default 1.0; eval default 0: rebuild every step)")
+                    help="list buffer in Å (MD modes default %.1f; eval default 0: rebuild "
+                         "every step)" % MD_BUFFER)
     args = ap.parse_args()
     if args.list_buffer is None:
-        args.list_buffer = 1.0 if args.mode == "md" else 0.0
+        # the smallest uneven box (24 A) allows rc + buffer < 12 A only
+        args.list_buffer = {"eval": 0.0, "batch-uneven": 1.5}.get(args.mode, MD_BUFFER)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
@@ -102,9 +128,10 @@ def main() -> int:
     s.pos_o, s.pos_m, s.pos_h0, s.pos_h1 = (np.ascontiguousarray(pw0[:, k]) for k in range(4))
     md = None
     if args.mode == "md":
+        st = equilibrate(eng, ff, s, pd0, pw0, args.equil_steps)
+        print(f"bench: equilibrated ({args.equil_steps} steps)", file=sys.stderr, flush=True)
         md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=1)
-        v, vw = systems.initial_velocities(s, ff, 300.0)
-        md.set_state(pd0, v, pw0, vw)
+        md.set_state(*st)
 
         def run(n):
             md.step(n)
@@ -191,6 +218,9 @@ def main() -> int:
             "pme_order": 4, "mode": "reference", "dt_fs": 2.0, "step": args.mode,
             "list": "rebuilt every step" if args.list_buffer <= 0 else f"buffer {args.list_buffer} A",
             "list_builds_in_timed_steps": E["list_builds"] - builds0,
+            "start": (f"relaxed ({args.relax_steps} steps), then {args.equil_steps} untimed "
+                      "Langevin steps at 300 K, γ 10/ps" if md is not None else
+                      f"relaxed ({args.relax_steps} steps)"),
             "accumulation": "int64 fixed point (2^-32), f64 energy/virial",
         },
         "pair_interactions_per_s": n_pairs * steps_per_s,
@@ -269,13 +299,13 @@ def run_batch_lambda(args) -> int:
 
     def engine():
         return api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
-                          mode=api.MODE_REFERENCE, list_buffer=1.0)
+                          mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
 
     e0 = engine()
     pd0, pw0 = api.relax(e0, ff, s.pos_dyn, systems.water_positions(s), steps=args.relax_steps)
+    pd0, v, pw0, vw = equilibrate(e0, ff, s, pd0, pw0, args.equil_steps)
     e0.close()
-    print("bench: relaxed", file=sys.stderr, flush=True)
-    v, vw = systems.initial_velocities(s, ff, 300.0)
+    print("bench: relaxed and equilibrated", file=sys.stderr, flush=True)
     lams = [k / max(n - 1, 1) for k in range(n)]
     made = []
     for k, lam in enumerate(lams):
@@ -313,7 +343,8 @@ def run_batch_lambda(args) -> int:
         "config": {"workload": f"{n} alchemical windows of chain2600_opc8600_box64 (37,000 sites "
                                "each), 40 decoupled solute atoms, λ = k/(n-1), soft-core α 0.5, "
                                "Langevin-middle dt 2 fs, SPME 64^3 order 4 twice per window "
-                               "(the cross-term scaling), list buffer 1.0 A",
+                               f"(the cross-term scaling), list buffer {args.list_buffer:.1f} A, "
+                               "thermalised start",
                    "windows": n, "sites_per_window": s.n_sites},
         "single_window_ns_per_day": one,
         "batched_over_single": agg / one,
@@ -366,7 +397,7 @@ def run_batch_uneven(args) -> int:
     def make(s, gamma, temp, seed):
         ff = systems.chain_force_field(s)
         e = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=api.pme_grid_n(s.box, 1.0),
-                       order=4, mode=api.MODE_REFERENCE, list_buffer=1.0)
+                       order=4, mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
         m = api.Md(e, ff, dt=0.002, temperature=temp, gamma=gamma, shake_tol=1e-5, seed=seed)
         return e, m, ff
 
@@ -382,9 +413,21 @@ def run_batch_uneven(args) -> int:
     for e, m, _ in made:
         m.close()
         e.close()
-    print("bench: relaxed", file=sys.stderr, flush=True)
+    # thermalise every box (batched Langevin, γ = 10/ps, 300 K), untimed
+    made = [make(s, 10.0, 300.0, 300 + k) for k, s in enumerate(S)]
+    for (e, m, ff), s, st in zip(made, S, starts):
+        v, vw = systems.initial_velocities(s, ff, 300.0)
+        m.set_state(st[0], v, st[2], vw)
+    b = api.Batch([m for _, m, _ in made])
+    b.step(args.equil_steps)
+    b.close()
+    starts = [m.state() for _, m, _ in made]
+    for e, m, _ in made:
+        m.close()
+        e.close()
+    print("bench: relaxed and equilibrated", file=sys.stderr, flush=True)
     made = [make(s, 1.0, 300.0, 200 + k) for k, s in enumerate(S)]
-    vel = [systems.initial_velocities(s, ff, 300.0) for s, (_, _, ff) in zip(S, made)]
+    vel = [(st[1], st[3]) for st in starts]
 
     def reset():
         for (e, m, ff), st, (v, vw) in zip(made, starts, vel):
@@ -420,7 +463,7 @@ def run_batch_uneven(args) -> int:
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{n} solvated boxes, edges U(24, 40) A, 20-60-atom chains, "
                                "OPC water, rc 10 A, SPME ~1 A grids order 4, Langevin-middle "
-                               "dt 2 fs, list buffer 1.0 A",
+                               f"dt 2 fs, list buffer {args.list_buffer:.1f} A, thermalised start",
                    "systems": n, "sites_total": int(sum(sites)),
                    "sites_range": [int(min(sites)), int(max(sites))]},
         "sequential_ns_per_day": seq,

@@ -55,7 +55,8 @@ constexpr int kNumPart = 9;           // f64 partials per work item of the pair 
 
 // status bits
 constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u,
-                   kStPme = 32u;  // an SPME brick list overflowed
+                   kStPme = 32u,   // an SPME brick list overflowed
+                   kStDisp = 64u;  // a site moved half the list buffer: the list was stale
 
 struct ClassGrid {
   int ncx, ncy;
@@ -314,7 +315,9 @@ __device__ __forceinline__ void k_eng_wrap_disp(const float4* __restrict__ pos,
                                                 const float* __restrict__ q, uint32_t n, EngGeom g,
                                                 float4* __restrict__ wpos,
                                                 const float4* __restrict__ ref,
-                                                unsigned int* __restrict__ disp_bits) {
+                                                unsigned int* __restrict__ disp_bits,
+                                                unsigned int* __restrict__ status,
+                                                unsigned int disp_limit) {
   __shared__ unsigned int red[256 / kWave];
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int v = 0u;
@@ -337,6 +340,7 @@ __device__ __forceinline__ void k_eng_wrap_disp(const float4* __restrict__ pos,
     unsigned int mx = 0u;
     for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
     if (mx) atomicMax(disp_bits, mx);
+    if (mx >= disp_limit) atomicOr(status, kStDisp);
   }
 }
 
@@ -1088,6 +1092,8 @@ struct PrepArgs {
   uint32_t n_mol2;
   BinArgs bin;        // SPME binning of the charges written here
   int bin_on;
+  unsigned int* status;
+  unsigned int disp_limit;  // bits of (buffer/2)²: a larger displacement flags kStDisp
 };
 
 __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
@@ -1146,6 +1152,7 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
     unsigned int mx = 0u;
     for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
     if (mx) atomicMax(A.disp_bits, mx);
+    if (mx >= A.disp_limit) atomicOr(A.status, kStDisp);
   }
 }
 
@@ -1660,23 +1667,57 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     if (!e->list_valid || e->prm.list_buffer <= 0.f) rebuild = true;
   }
   const bool async = (flags & DYN_COMPUTE_ASYNC) != 0;
-  if (!rebuild && async) {
-    // Asynchronous computes never wait for the displacement: they use the
-    // newest readings of earlier computes on these lists (a few steps old)
-    // and rebuild at 0.8 × buffer/2, which leaves the rest of the half
-    // buffer for the motion since those readings.
-    for (int k = 0; k < DispRing::kRing; ++k)
-      if (D.pending[k] && hipEventQuery(D.ev[k]) == hipSuccess) {
-        D.pending[k] = false;
-        if (D.gen[k] == D.builds)
-          for (uint32_t i = 0; i < ns; ++i) {
-            float d2;
-            std::memcpy(&d2, D.h.p + (size_t)k * ns + i, 4);
-            D.seen[i] = std::max(D.seen[i], d2);
-          }
+  // the displacement reading of ring slot k, taken into the current lists' maximum
+  auto consume = [&](int k) {
+    D.pending[k] = false;
+    if (D.gen[k] == D.builds)
+      for (uint32_t i = 0; i < ns; ++i) {
+        float d2;
+        std::memcpy(&d2, D.h.p + (size_t)k * ns + i, 4);
+        D.seen[i] = std::max(D.seen[i], d2);
       }
+  };
+  // an asynchronous evaluation's reading: copied to the host behind an event
+  auto record = [&]() {
+    const int k = D.next;
+    D.next = (D.next + 1) % DispRing::kRing;
+    if (D.pending[k]) {
+      DYN_HIP(hipEventSynchronize(D.ev[k]));
+      consume(k);
+    }
+    DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
+                           hipMemcpyDeviceToHost, R.stream));
+    DYN_HIP(hipEventRecord(D.ev[k], R.stream));
+    D.pending[k] = true;
+    D.gen[k] = D.builds;
+    D.eval_of[k] = D.issued++;
+  };
+  // bits of (buffer/2)²: the device flags an evaluation whose sites moved
+  // that far since the list was built (kStDisp), and the call repeats
+  auto disp_limit = [&](const dyn_engine& e) {
+    const float half = 0.5f * e.prm.list_buffer;
+    const float h2 = half * half;
+    unsigned int b;
+    std::memcpy(&b, &h2, 4);
+    return b;
+  };
+  if (!rebuild && async) {
+    // Asynchronous computes do not wait for their own displacement: they
+    // decide from the readings of earlier computes on these lists, at most
+    // kLag evaluations old (the host waits for those), and rebuild at
+    // 0.6 × buffer/2, which leaves the rest of the half buffer for the
+    // motion since (the fastest of many light sites move ~0.1 Å per 2 fs
+    // step).  Should a site still cover the half buffer first, the device
+    // flags the evaluation (kStDisp) and the call repeats.
+    for (int k = 0; k < DispRing::kRing; ++k)
+      if (D.pending[k] && D.eval_of[k] + DispRing::kLag <= D.issued) {
+        DYN_HIP(hipEventSynchronize(D.ev[k]));
+        consume(k);
+      }
+    for (int k = 0; k < DispRing::kRing; ++k)
+      if (D.pending[k] && hipEventQuery(D.ev[k]) == hipSuccess) consume(k);
     for (uint32_t i = 0; i < ns; ++i) {
-      const float lag_half = 0.4f * E[i]->prm.list_buffer;
+      const float lag_half = 0.3f * E[i]->prm.list_buffer;
       if (!(D.seen[i] < lag_half * lag_half)) rebuild = true;
     }
   }
@@ -1764,35 +1805,24 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
         A.bin = e.pme.bin_args(e.status.p, kStPme);
         A.bin_on = 1;
       }
+      A.status = e.status.p;
+      A.disp_limit = disp_limit(e);
       return args<k_eng_prep>(A);
     });
-    const int k = D.next;
-    D.next = (D.next + 1) % DispRing::kRing;
-    if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
-      DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
-                             hipMemcpyDeviceToHost, R.stream));
-      DYN_HIP(hipEventRecord(D.ev[k], R.stream));
-      D.pending[k] = true;
-      D.gen[k] = D.builds;
-    }
+    record();
   }
   if (!rebuild && !fused) {
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
     launch<k_eng_wrap_disp>(R, "k_eng_wrap_disp", 0, gs, 256, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
+      // a synchronous evaluation reads its own displacement and rebuilds
+      // in time: no flag
       return args<k_eng_wrap_disp>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p,
-                                   (const float4*)e.wpos_ref.p, D.bits.p + i);
+                                   (const float4*)e.wpos_ref.p, D.bits.p + i, e.status.p,
+                                   async ? disp_limit(e) : 0xFFFFFFFFu);
     });
     if (async) {
-      const int k = D.next;
-      D.next = (D.next + 1) % DispRing::kRing;
-      if (!D.pending[k]) {  // a slot still in flight is skipped, not overwritten
-        DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
-                               hipMemcpyDeviceToHost, R.stream));
-        DYN_HIP(hipEventRecord(D.ev[k], R.stream));
-        D.pending[k] = true;
-        D.gen[k] = D.builds;
-      }
+      record();
     } else {
       DYN_HIP(hipMemcpyAsync(D.h_sync.p, D.bits.p, 4 * (size_t)ns, hipMemcpyDeviceToHost,
                              R.stream));
@@ -2021,6 +2051,9 @@ void dyn_engine::check_status() {
       throw Error(DYN_ERR_CAPACITY, fmt("an SPME brick list overflowed; lists enlarged to %u "
                                         "charges per brick", pme.cap_brick));
   }
+  if (st == kStDisp)
+    throw Error(DYN_ERR_CAPACITY, "a site moved half the list buffer before the list was "
+                                  "rebuilt; the list is rebuilt at the next call, call again");
   if (st & kStColumn)
     throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
   if (st & kStMask)

@@ -21,12 +21,17 @@ namespace dyn {
 // completed) and a copy for synchronous ones.
 struct DispRing {
   static constexpr int kRing = 4;
+  // an asynchronous evaluation decides with every reading at least kLag
+  // evaluations old in hand (the host waits for them)
+  static constexpr int kLag = 2;
   uint32_t n = 0;
   DevBuf<unsigned int> bits;
   PinnedBuf<unsigned int> h;       // kRing × n
   PinnedBuf<unsigned int> h_sync;  // n
   hipEvent_t ev[kRing] = {};
   uint64_t gen[kRing] = {};
+  uint64_t eval_of[kRing] = {};    // the asynchronous evaluation of each reading
+  uint64_t issued = 0;             // asynchronous readings recorded
   bool pending[kRing] = {};
   int next = 0;
   uint64_t builds = 0;             // list builds of the set (reading generation)

@@ -1074,6 +1074,7 @@ struct MdSet {
   EngineSet es;
   BatchRun R;  // the MD kernels' launches (own argument packs)
   Profiler* prof = nullptr;
+  bool strict = false;  // every evaluation waits for its own displacement reading
 };
 
 }  // namespace dyn
@@ -1130,7 +1131,7 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
   if (first)
     engine_set_compute_sync(S.es, flags);
   else
-    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC);
+    engine_set_enqueue(S.es, flags | (S.strict ? 0u : DYN_COMPUTE_ASYNC));
   if (any) {
     Profiler::Scope ps(*S.prof, "md_bonded");
     launch<k_md_bonded, kBlock>(S.R, "k_md_bonded", 0, gb, kBlock, 0, [&](uint32_t i) {
@@ -1384,6 +1385,9 @@ void md_run(MdSet& S, uint32_t n) {
     ck[i] = {m.step, m.n_eval, m.n_pcouple, {m.box[0], m.box[1], m.box[2]}};
   }
   for (int attempt = 0;; ++attempt) {
+    // a repeat (after a buffer overflow or a stale list) checks every
+    // evaluation's displacement before its pair kernel
+    S.strict = attempt > 0;
     try {
       const auto t0 = std::chrono::steady_clock::now();
       // a barostat step waits mid-call, and may find an overflow there

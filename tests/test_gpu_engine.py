@@ -387,3 +387,30 @@ def test_engine_errors(gpu, cases):
     bad.pos_o[0, 0] = np.nan
     with pytest.raises(api.DynError):
         eng.set_positions(bad)
+
+
+def test_stale_list_flagged(gpu, cases):
+    """An asynchronous compute decides on a rebuild from earlier
+    displacement readings.  If a site has already moved half the list
+    buffer, the device flags the evaluation: the wait reports a capacity
+    error (call again), and the next compute rebuilds and equals a fresh
+    build on the moved positions bit for bit."""
+    s = cases["water512"]
+    eng = _engine(s, list_buffer=1.0)
+    eng.compute(api.COMPUTE_ALL)
+    assert eng.energies()["list_builds"] == 1
+    moved = copy.deepcopy(s)
+    for name in ["pos_o", "pos_m", "pos_h0", "pos_h1"]:
+        getattr(moved, name)[3] += np.float32(0.6)
+    eng.set_positions(moved)
+    eng.compute(api.COMPUTE_ALL | api.COMPUTE_ASYNC)
+    with pytest.raises(api.DynError) as err:
+        eng.sync()
+    assert err.value.code == api.DYN_ERR_CAPACITY
+    eng.compute(api.COMPUTE_ALL)
+    assert eng.energies()["list_builds"] == 2
+    fresh = _engine(moved, list_buffer=1.0)
+    fresh.compute(api.COMPUTE_ALL)
+    assert np.array_equal(eng.fixed_forces(), fresh.fixed_forces())
+    eng.close()
+    fresh.close()
