@@ -563,11 +563,21 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
 // in i slot ia and the group's j entry jb are a 1-2, 1-3 or 1-4 pair; the
 // lanes walk the i sites' exclusion lists.
 constexpr int kMaskCap = 2048;  // solute entries of one solute row
+constexpr int kMaskHash = 2 * kMaskCap;
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t mask_hash(uint32_t slot) {
+  return (slot * 2654435761u) >> (32 - 12);  // 12 bits: kMaskHash = 4096
+}
+static_assert(kMaskHash == 4096, "mask_hash takes 12 bits");
 __device__ __forceinline__ void
 k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ col_cl_start, unsigned long long* __restrict__ cmask) {
-  __shared__ uint32_t s_ent[kMaskCap];
+  // the row's solute entries as an LDS hash table (slot → entry index,
+  // linear probing, at most half full), so each exclusion finds its entry
+  // in a few probes
+  __shared__ uint32_t h_key[kMaskHash];
+  __shared__ uint16_t h_val[kMaskHash];
   __shared__ unsigned long long s_mask[kMaskCap / DYN_CL_SIZE];
   __shared__ uint32_t s_beg[DYN_CL_SIZE + 1], s_ex0[DYN_CL_SIZE];
   const uint32_t n_sol = col_cl_start[A.g.cg[1].col_off];
@@ -588,8 +598,16 @@ k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
       if (lane == 0) atomicOr(A.status, kStMask);
       continue;
     }
-    for (uint32_t k = lane; k < nss; k += kWave) s_ent[k] = ent[k] & DYN_JSLOT_MASK;
+    for (uint32_t h = lane; h < (uint32_t)kMaskHash; h += kWave) h_key[h] = kHashEmpty;
     for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) s_mask[g] = ~0ull;
+    __syncthreads();
+    for (uint32_t k = lane; k < nss; k += kWave) {
+      const uint32_t slot = ent[k] & DYN_JSLOT_MASK;
+      if (slot == A.null_slot) continue;  // padding: no exclusion partner
+      uint32_t h = mask_hash(slot);
+      while (atomicCAS(&h_key[h], kHashEmpty, slot) != kHashEmpty) h = (h + 1) & (kMaskHash - 1);
+      h_val[h] = (uint16_t)k;
+    }
     uint32_t len = 0, ex0 = 0;
     if (lane < DYN_CL_SIZE) {
       const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
@@ -615,11 +633,15 @@ k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
       while (ia + 1 < DYN_CL_SIZE && s_beg[ia + 1] <= t) ++ia;
       const uint32_t sb = A.ex_idx[s_ex0[ia] + (t - s_beg[ia])];
       const uint32_t slot = A.slot_of[sb];
-      for (uint32_t k = 0; k < nss; ++k)
-        if (s_ent[k] == slot) {
+      for (uint32_t h = mask_hash(slot);; h = (h + 1) & (kMaskHash - 1)) {
+        const uint32_t key = h_key[h];
+        if (key == slot) {
+          const uint32_t k = h_val[h];
           atomicAnd(&s_mask[k / DYN_CL_SIZE], ~(1ull << (ia + DYN_CL_SIZE * (k % DYN_CL_SIZE))));
           break;
         }
+        if (key == kHashEmpty) break;  // the partner is not in this row
+      }
     }
     __syncthreads();
     for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) cmask[base + g] = s_mask[g];
