@@ -1726,11 +1726,12 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
   if (!rebuild && async) {
     // Asynchronous computes do not wait for their own displacement: they
     // decide from the readings of earlier computes on these lists, at most
-    // kLag evaluations old (the host waits for those), and rebuild at
-    // 0.6 × buffer/2, which leaves the rest of the half buffer for the
-    // motion since (the fastest of many light sites move ~0.1 Å per 2 fs
-    // step).  Should a site still cover the half buffer first, the device
-    // flags the evaluation (kStDisp) and the call repeats.
+    // kLag evaluations old (the host waits for those), and rebuild when a
+    // site has moved D.factor × buffer (0.25: half of the half buffer), which
+    // leaves the rest of the half buffer for the motion since (the fastest
+    // of many light sites move ~0.1 Å per 2 fs step).  Should a site still
+    // cover the half buffer first, the device flags the evaluation
+    // (kStDisp) and the call repeats.
     for (int k = 0; k < DispRing::kRing; ++k)
       if (D.pending[k] && D.eval_of[k] + DispRing::kLag <= D.issued) {
         DYN_HIP(hipEventSynchronize(D.ev[k]));
@@ -1739,7 +1740,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     for (int k = 0; k < DispRing::kRing; ++k)
       if (D.pending[k] && hipEventQuery(D.ev[k]) == hipSuccess) consume(k);
     for (uint32_t i = 0; i < ns; ++i) {
-      const float lag_half = 0.3f * E[i]->prm.list_buffer;
+      const float lag_half = D.factor * E[i]->prm.list_buffer;
       if (!(D.seen[i] < lag_half * lag_half)) rebuild = true;
     }
   }

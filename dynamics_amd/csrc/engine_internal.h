@@ -24,6 +24,9 @@ struct DispRing {
   // an asynchronous evaluation decides with every reading at least kLag
   // evaluations old in hand (the host waits for them)
   static constexpr int kLag = 2;
+  // rebuild when a site has moved factor × buffer (a repeat after a
+  // flagged evaluation lowers it)
+  float factor = 0.25f;
   uint32_t n = 0;
   DevBuf<unsigned int> bits;
   PinnedBuf<unsigned int> h;       // kRing × n

@@ -1385,9 +1385,11 @@ void md_run(MdSet& S, uint32_t n) {
     ck[i] = {m.step, m.n_eval, m.n_pcouple, {m.box[0], m.box[1], m.box[2]}};
   }
   for (int attempt = 0;; ++attempt) {
-    // a repeat (after a buffer overflow or a stale list) checks every
-    // evaluation's displacement before its pair kernel
-    S.strict = attempt > 0;
+    // a repeat (after a buffer overflow or a stale list) rebuilds at half
+    // the usual displacement; a second one checks every evaluation's
+    // displacement before its pair kernel
+    S.es.disp->factor = attempt == 1 ? 0.15f : 0.25f;
+    S.strict = attempt > 1;
     try {
       const auto t0 = std::chrono::steady_clock::now();
       // a barostat step waits mid-call, and may find an overflow there
@@ -1398,9 +1400,13 @@ void md_run(MdSet& S, uint32_t n) {
       const auto t2 = std::chrono::steady_clock::now();
       S.prof->add_host("md_issue_steps", std::chrono::duration<double, std::milli>(t1 - t0).count());
       S.prof->add_host("md_wait", std::chrono::duration<double, std::milli>(t2 - t1).count());
+      S.es.disp->factor = 0.25f;
       return;
     } catch (const Error& e) {
-      if (e.code != DYN_ERR_CAPACITY || attempt >= 2) throw;
+      if (e.code != DYN_ERR_CAPACITY || attempt >= 2) {
+        S.es.disp->factor = 0.25f;
+        throw;
+      }
     }
     for (size_t i = 0; i < S.m.size(); ++i) {
       dyn_md& m = *S.m[i];
