@@ -38,7 +38,7 @@ FLOPS_LJ_COUL, FLOPS_LJ, FLOPS_COUL = 63, 50, 51
 
 
 # list buffer of the MD modes (Å): the list is rebuilt when a site has moved
-# 0.3 × buffer since the last build (readings at most two steps old)
+# 0.25 × buffer since the last build (readings at most two steps old)
 MD_BUFFER = 2.5
 
 

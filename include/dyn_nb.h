@@ -261,7 +261,7 @@ int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32_t count, d
  * list_buffer/2 since the last build.  With DYN_COMPUTE_ASYNC the decision
  * comes from the displacement readings of earlier evaluations (at most two
  * evaluations old; the call waits for older ones) and the list is rebuilt
- * at 0.3 × list_buffer; should a site have reached list_buffer/2 before,
+ * at 0.25 × list_buffer; should a site have reached list_buffer/2 before,
  * the evaluation is flagged on the device and its forces are not valid:
  * dyn_engine_sync then returns DYN_ERR_CAPACITY, the list is rebuilt at the
  * next compute, and the caller evaluates again. */
