@@ -171,8 +171,6 @@ def setup_pairs(system, nb, alch_atoms=None):
         s_w = (F32(0.5) * (sig[ii] + O_SIGMA).astype(F32)).astype(F32)
         e_w = np.sqrt((eps[ii] * O_EPS).astype(F32)).astype(F32)
         qsite = np.array([0.0, Q_EP, Q_H, Q_H], F32)
-        for site in (1, 2, 3, 4):
-            pass
         # interleave the 4 sites per (atom, water) as the reference does
         n = ii.size
         rep = lambda a: np.repeat(a, 4)
