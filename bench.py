@@ -364,23 +364,9 @@ def run_batch_lambda(args) -> int:
 
 
 def uneven_systems(n, seed=4):
-    """n solvated boxes with edges uniform in [24, 40] Å (≈0.0334 waters/Å³) and
-    20–60-atom chains (half heavy atoms), seeded."""
-    import numpy as np
+    """The batched uneven-systems workload (systems.config4)."""
     from dynamics_amd import systems
-
-    rng = np.random.default_rng(seed)
-    out = []
-    for k in range(n):
-        L = float(rng.uniform(24.0, 40.0))
-        n_atoms = int(rng.integers(20, 61))
-        n_heavy = n_atoms // 2
-        lat = int(np.ceil((0.0334 * L ** 3) ** (1.0 / 3.0))) + 1  # room for the chain
-        n_water = int(round(0.0334 * L ** 3))
-        s = systems.solvated_chain(L, n_water, n_heavy, n_atoms - n_heavy, seed=1000 + k,
-                                   lattice_n=lat, name=f"uneven{k}")
-        out.append(s)
-    return out
+    return systems.config4(n, seed)
 
 
 def run_batch_uneven(args) -> int:

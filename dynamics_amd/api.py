@@ -20,6 +20,7 @@ DYN_ERR_HIP = 2
 DYN_ERR_STATE = 3
 DYN_ERR_CAPACITY = 4
 DYN_ERR_NOMEM = 5
+DYN_ERR_LIMIT = 6
 
 MODE_REFERENCE = 0
 MODE_STANDARD = 1
@@ -73,7 +74,10 @@ class MdEnergies(C.Structure):
                 ("angle", C.c_double), ("dihedral", C.c_double), ("temperature", C.c_double),
                 ("dof", C.c_double), ("steps", C.c_uint64), ("pressure", C.c_double),
                 ("virial", C.c_double), ("kinetic_com", C.c_double), ("volume", C.c_double),
-                ("box", C.c_double * 3), ("pcouple_steps", C.c_uint64)]
+                ("box", C.c_double * 3), ("pcouple_steps", C.c_uint64),
+                ("potential_nonbonded", C.c_double), ("potential_bonded", C.c_double),
+                ("potential", C.c_double), ("dh_dl", C.c_double), ("density", C.c_double),
+                ("repeats", C.c_uint64)]
 
 
 class EngineParams(C.Structure):
@@ -157,6 +161,7 @@ _sig("dyn_engine_forces_device", C.c_void_p, [_p])
 _sig("dyn_engine_get_energies", C.c_int, [_p, C.POINTER(Energies)])
 _sig("dyn_engine_debug_item_partials", C.c_int, [_p, C.c_void_p, C.c_uint64])
 _sig("dyn_debug_arg_uploads", C.c_uint64, [])
+_sig("dyn_engine_debug_inject_status", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_uint32])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
                                           _p, C.POINTER(C.c_uint64), _p])
@@ -188,7 +193,7 @@ EXPORTED = [
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
-    "dyn_engine_debug_item_partials", "dyn_debug_arg_uploads",
+    "dyn_engine_debug_item_partials", "dyn_debug_arg_uploads", "dyn_engine_debug_inject_status",
     "dyn_engine_profile",
     "dyn_engine_profile_read",
     "dyn_batch_create", "dyn_batch_destroy", "dyn_batch_last_error", "dyn_batch_step",
@@ -484,6 +489,13 @@ class Engine:
         d["cmask"] = d["cmask"][:ne]
         d["items"] = d["items"][:ni]
         return d
+
+    def debug_inject_status(self, bits, skip=0, count=1):
+        """Test hook: after `skip` evaluations, the next `count` get `bits`
+        ORed into the device status word (64 = stale list) before the pair
+        kernel, as the device's own checks would set them."""
+        _check(lib.dyn_engine_debug_inject_status(self.h, int(bits), int(skip), int(count)),
+               self._err)
 
     def debug_item_partials(self, n_items):
         """(n_items, 9) per work item sums of the last compute (see the C header)."""

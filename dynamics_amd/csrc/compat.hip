@@ -374,7 +374,6 @@ DynRefParams make_ref_params(const dyn_nb_ctx* c, const float ext[3], bool coul_
 
 extern "C" int dyn_nb_create(dyn_nb_ctx** out, int device, uint32_t n_dyn, uint32_t n_water,
                              float cutoff_ewald, float alpha_ewald, const dyn_nb_opts* opts) {
-  dyn_nb_ctx* holder = nullptr;
   try {
     DYN_REQUIRE(out != nullptr, "out is null");
     *out = nullptr;
@@ -387,7 +386,6 @@ extern "C" int dyn_nb_create(dyn_nb_ctx** out, int device, uint32_t n_dyn, uint3
     DYN_HIP(hipGetDeviceCount(&ndev));
     DYN_REQUIRE(device >= 0 && device < ndev, "device %d not present (%d devices)", device, ndev);
     std::unique_ptr<dyn_nb_ctx> c(new dyn_nb_ctx());
-    holder = c.get();
     c->device = device;
     DeviceGuard g(device);
     DYN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

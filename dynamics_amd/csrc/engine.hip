@@ -1320,6 +1320,10 @@ k_eng_energy_reduce(const double* __restrict__ red1, const double* __restrict__ 
 // ---------------------------------------------------------------------------
 // Handle
 // ---------------------------------------------------------------------------
+// Test hook (dyn_engine_debug_inject_status): sets status bits as a device
+// check would, so the error and repeat paths run without a real failure.
+__global__ void k_dbg_or_status(unsigned int* status, unsigned int bits) { atomicOr(status, bits); }
+
 struct dyn_engine {
   std::string err;
   int device = 0;
@@ -1404,6 +1408,18 @@ struct dyn_engine {
   bool fx_fresh = false;
 
   hipStream_t own_stream = nullptr;  // the engine's stream (a batch may lend it another)
+  // the forces, SPME forces and energies of the last evaluation, kept at the
+  // start of an MD call so that a repeat or a failed call restores them
+  DevBuf<int64_t> fx_out_ck, fx_recip_ck;
+  DevBuf<double> eng_ck;
+  struct {
+    bool recip_kept, last_recip_b, computed, energies_valid;
+    DynSchedule last_sch;
+  } ck{};
+  // test hook: after dbg_skip more evaluations, the next dbg_count ones have
+  // dbg_bits ORed into the status word before the pair kernel
+  unsigned dbg_bits = 0;
+  uint32_t dbg_skip = 0, dbg_count = 0;
 
   ~dyn_engine() {
     if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -1892,6 +1908,16 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
                                     e.n_mol * e.n_mol);
     });
   }
+  for (dyn_engine* e : E) {
+    if (!e->dbg_count) continue;
+    if (e->dbg_skip) {
+      --e->dbg_skip;
+      continue;
+    }
+    --e->dbg_count;
+    hipLaunchKernelGGL(k_dbg_or_status, dim3(1), dim3(1), 0, R.stream, e->status.p, e->dbg_bits);
+    DYN_CHECK_LAUNCH("k_dbg_or_status");
+  }
   if (short_on) {
     Profiler::Scope ps(prof, "pairs");
     std::vector<PairArgs> PA(ns);
@@ -2074,14 +2100,15 @@ void dyn_engine::check_status() {
       throw Error(DYN_ERR_CAPACITY, fmt("an SPME brick list overflowed; lists enlarged to %u "
                                         "charges per brick", pme.cap_brick));
   }
-  if (st == kStDisp)
+  // fixed limits of the design: calling again does not help
+  if (st & kStColumn)
+    throw Error(DYN_ERR_LIMIT, fmt("a cluster column holds more than %d sites", kColCap));
+  if (st & kStMask)
+    throw Error(DYN_ERR_LIMIT, fmt("a solute cluster has more than %d solute neighbours "
+                                   "(reduce list_buffer)", kMaskCap));
+  if ((st & ~kStPme) == kStDisp)
     throw Error(DYN_ERR_CAPACITY, "a site moved half the list buffer before the list was "
                                   "rebuilt; the list is rebuilt at the next call, call again");
-  if (st & kStColumn)
-    throw Error(DYN_ERR_CAPACITY, fmt("a cluster column holds more than %d sites", kColCap));
-  if (st & kStMask)
-    throw Error(DYN_ERR_CAPACITY, fmt("a solute cluster has more than %d solute neighbours "
-                                      "(reduce list_buffer)", kMaskCap));
   if (st & kStRow) {
     const uint32_t longest = h_misc.p[5];
     DYN_HIP(hipMemsetAsync(counters.p + 4, 0, 4, stream));
@@ -2724,4 +2751,53 @@ void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water) {
 void engine_box(const dyn_engine* eng, float box[3]) {
   for (int a = 0; a < 3; ++a) box[a] = eng->box[a];
 }
+int engine_mode(const dyn_engine* eng) { return (int)eng->prm.mode; }
+
+// Keep the last evaluation's outputs (site forces, SPME forces, energies)
+// on the engine's stream; engine_restore puts them back and marks the list
+// for a rebuild (it was built for positions the caller is discarding).
+void engine_checkpoint(dyn_engine* e) {
+  const size_t nf = 3 * (size_t)e->n_sites;
+  e->fx_out_ck.reserve(nf);
+  e->eng_ck.reserve(kEngOut);
+  DYN_HIP(hipMemcpyAsync(e->fx_out_ck.p, e->fx_out.p, 8 * nf, hipMemcpyDeviceToDevice, e->stream));
+  DYN_HIP(hipMemcpyAsync(e->eng_ck.p, e->eng_out.p, 8 * kEngOut, hipMemcpyDeviceToDevice, e->stream));
+  if (e->pme_on) {
+    e->fx_recip_ck.reserve(nf);
+    DYN_HIP(hipMemcpyAsync(e->fx_recip_ck.p, e->fx_recip.p, 8 * nf, hipMemcpyDeviceToDevice,
+                           e->stream));
+  }
+  e->ck.recip_kept = e->recip_kept;
+  e->ck.last_recip_b = e->last_recip_b;
+  e->ck.last_sch = e->last_sch;
+  e->ck.computed = e->computed;
+  e->ck.energies_valid = e->energies_valid;
+}
+
+void engine_restore(dyn_engine* e) {
+  const size_t nf = 3 * (size_t)e->n_sites;
+  DYN_HIP(hipMemcpyAsync(e->fx_out.p, e->fx_out_ck.p, 8 * nf, hipMemcpyDeviceToDevice, e->stream));
+  DYN_HIP(hipMemcpyAsync(e->eng_out.p, e->eng_ck.p, 8 * kEngOut, hipMemcpyDeviceToDevice, e->stream));
+  if (e->pme_on)
+    DYN_HIP(hipMemcpyAsync(e->fx_recip.p, e->fx_recip_ck.p, 8 * nf, hipMemcpyDeviceToDevice,
+                           e->stream));
+  e->recip_kept = e->ck.recip_kept;
+  e->last_recip_b = e->ck.last_recip_b;
+  e->last_sch = e->ck.last_sch;
+  e->computed = e->ck.computed;
+  e->energies_valid = e->ck.energies_valid;
+  e->mol_valid = false;
+  e->fx_fresh = false;
+  e->list_valid = false;
+}
 }  // namespace dyn
+
+extern "C" int dyn_engine_debug_inject_status(dyn_engine* eng, uint32_t bits, uint32_t skip,
+                                              uint32_t count) {
+  ENG_CHECK(eng);
+  DYN_API_BODY(eng, {
+    eng->dbg_bits = bits;
+    eng->dbg_skip = skip;
+    eng->dbg_count = count;
+  });
+}

@@ -70,5 +70,10 @@ void engine_set_stream(dyn_engine* eng, hipStream_t s);  // null: its own stream
 int engine_device(const dyn_engine* eng);
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water);
 void engine_box(const dyn_engine* eng, float box[3]);
+int engine_mode(const dyn_engine* eng);  // DYN_MODE_*
 const int64_t* engine_forces_device(const dyn_engine* eng);
+// Keep / put back the last evaluation's forces, SPME forces and energies
+// (enqueued on the engine's stream); a restore also invalidates the list.
+void engine_checkpoint(dyn_engine* eng);
+void engine_restore(dyn_engine* eng);
 }  // namespace dyn

@@ -1033,6 +1033,8 @@ struct dyn_md {
   float box[3] = {0, 0, 0};
   DevBuf<float4> x, v;
   DevBuf<float4> x_ck, v_ck;  // state at the start of the current dyn_md_step call
+  DevBuf<int64_t> fb_ck;      // bonded forces and partials at that start
+  DevBuf<double> bpart_ck;
   DevBuf<float> mass, acc_conv, lang_sigma;
   float sigma_o = 0.f, sigma_h = 0.f, lang_c = 1.f;
   uint32_t n_bond = 0, n_angle = 0, n_dihe = 0;
@@ -1062,6 +1064,9 @@ struct dyn_md {
   PinnedBuf<double> h_ppart, h_bpart;
   double pressure = std::nan(""), virial = std::nan(""), kin_com = std::nan("");
   uint64_t n_pcouple = 0;  // barostat rescalings so far
+  uint64_t n_repeats = 0;  // step calls run again from their start (list overflow, stale list)
+  double mass_total = 0.0;  // amu: solute atoms and the waters' O and H
+  bool eng_mode_reference = true;
   ArgCache own_cache;   // a pass over this MD alone
   bool in_batch = false;
 };
@@ -1367,27 +1372,75 @@ bool md_one_step(MdSet& S, bool energies, bool begun, bool may_join) {
 
 // n steps of every MD of the set, then a wait for them.  Later force
 // evaluations are only enqueued, so a cluster-pair list that outgrows its
-// buffers is found at the wait; the engine has enlarged them by then, and
-// the steps run again from the state they started from (the same random
-// stream: the step index seeds it).
-void md_run(MdSet& S, uint32_t n) {
-  if (!n) return;
-  struct Ck {
-    uint64_t step, eval, pcouple;
-    float box[3];
-  };
-  std::vector<Ck> ck(S.m.size());
+// buffers (or a stale list) is found at the wait; the engine has enlarged
+// the buffers by then, and the steps run again from the state they started
+// from: positions, velocities, step counters, box, and the start's forces
+// and energies (so a repeat keeps the SPME cadence of the original call and
+// the same random stream, which the step index seeds).  A call that gives
+// up restores that start state too before it reports the error.
+namespace {
+struct MdCk {
+  uint64_t step, eval, pcouple;
+  float box[3];
+  bool forces_valid;
+};
+
+void md_checkpoint(MdSet& S, std::vector<MdCk>& ck) {
+  ck.resize(S.m.size());
   for (size_t i = 0; i < S.m.size(); ++i) {
     dyn_md& m = *S.m[i];
     const size_t bytes = 16 * (size_t)m.n_sites;
     DYN_HIP(hipMemcpyAsync(m.x_ck.p, m.x.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
     DYN_HIP(hipMemcpyAsync(m.v_ck.p, m.v.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
-    ck[i] = {m.step, m.n_eval, m.n_pcouple, {m.box[0], m.box[1], m.box[2]}};
+    ck[i] = {m.step, m.n_eval, m.n_pcouple, {m.box[0], m.box[1], m.box[2]}, m.forces_valid};
+    if (m.forces_valid) {
+      DYN_HIP(hipMemcpyAsync(m.fb_ck.p, m.fb.p, 24 * (size_t)std::max(m.n_dyn, 1u),
+                             hipMemcpyDeviceToDevice, S.R.stream));
+      DYN_HIP(hipMemcpyAsync(m.bpart_ck.p, m.bpart.p, 32 * (size_t)m.bblocks,
+                             hipMemcpyDeviceToDevice, S.R.stream));
+      engine_checkpoint(m.eng);  // on the same stream as the set's work
+    }
   }
+}
+
+// Waits for whatever the failed attempt left in flight, then puts every MD
+// of the set back to its checkpoint.
+void md_restore(MdSet& S, const std::vector<MdCk>& ck) {
+  (void)hipStreamSynchronize(S.R.stream);
+  for (size_t i = 0; i < S.m.size(); ++i) {
+    dyn_md& m = *S.m[i];
+    const size_t bytes = 16 * (size_t)m.n_sites;
+    DYN_HIP(hipMemcpyAsync(m.x.p, m.x_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+    DYN_HIP(hipMemcpyAsync(m.v.p, m.v_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
+    m.step = ck[i].step;
+    m.n_eval = ck[i].eval;
+    if (m.n_pcouple != ck[i].pcouple) {  // the box of the start
+      engine_call(dyn_engine_set_box(m.eng, ck[i].box), m.eng);
+      for (int a = 0; a < 3; ++a) m.box[a] = ck[i].box[a];
+      m.n_pcouple = ck[i].pcouple;
+    }
+    m.forces_valid = ck[i].forces_valid;
+    if (m.forces_valid) {
+      DYN_HIP(hipMemcpyAsync(m.fb.p, m.fb_ck.p, 24 * (size_t)std::max(m.n_dyn, 1u),
+                             hipMemcpyDeviceToDevice, S.R.stream));
+      DYN_HIP(hipMemcpyAsync(m.bpart.p, m.bpart_ck.p, 32 * (size_t)m.bblocks,
+                             hipMemcpyDeviceToDevice, S.R.stream));
+      engine_restore(m.eng);
+    }
+    DYN_HIP(hipMemcpyAsync(m.step_dev.p, &ck[i].step, 8, hipMemcpyHostToDevice, S.R.stream));
+    DYN_HIP(hipStreamSynchronize(S.R.stream));
+  }
+}
+}  // namespace
+
+void md_run(MdSet& S, uint32_t n) {
+  if (!n) return;
+  std::vector<MdCk> ck;
+  md_checkpoint(S, ck);
   for (int attempt = 0;; ++attempt) {
-    // a repeat (after a buffer overflow or a stale list) rebuilds at half
-    // the usual displacement; a second one checks every evaluation's
-    // displacement before its pair kernel
+    // a repeat (after a buffer overflow or a stale list) rebuilds at a lower
+    // displacement; a second one checks every evaluation's displacement
+    // before its pair kernel
     S.es.disp->factor = attempt == 1 ? 0.15f : 0.25f;
     S.strict = attempt > 1;
     try {
@@ -1401,29 +1454,22 @@ void md_run(MdSet& S, uint32_t n) {
       S.prof->add_host("md_issue_steps", std::chrono::duration<double, std::milli>(t1 - t0).count());
       S.prof->add_host("md_wait", std::chrono::duration<double, std::milli>(t2 - t1).count());
       S.es.disp->factor = 0.25f;
+      S.strict = false;
+      for (dyn_md* m : S.m) m->n_repeats += (uint64_t)attempt;
       return;
     } catch (const Error& e) {
       if (e.code != DYN_ERR_CAPACITY || attempt >= 2) {
         S.es.disp->factor = 0.25f;
+        S.strict = false;
+        try {
+          md_restore(S, ck);
+        } catch (const Error&) {
+          // the original error is the one to report
+        }
         throw;
       }
     }
-    for (size_t i = 0; i < S.m.size(); ++i) {
-      dyn_md& m = *S.m[i];
-      const size_t bytes = 16 * (size_t)m.n_sites;
-      DYN_HIP(hipMemcpyAsync(m.x.p, m.x_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
-      DYN_HIP(hipMemcpyAsync(m.v.p, m.v_ck.p, bytes, hipMemcpyDeviceToDevice, S.R.stream));
-      m.step = ck[i].step;
-      m.n_eval = ck[i].eval;
-      m.forces_valid = false;
-      if (m.n_pcouple != ck[i].pcouple) {  // the box of the start
-        engine_call(dyn_engine_set_box(m.eng, ck[i].box), m.eng);
-        for (int a = 0; a < 3; ++a) m.box[a] = ck[i].box[a];
-        m.n_pcouple = ck[i].pcouple;
-      }
-      DYN_HIP(hipMemcpyAsync(m.step_dev.p, &ck[i].step, 8, hipMemcpyHostToDevice, S.R.stream));
-      DYN_HIP(hipStreamSynchronize(S.R.stream));
-    }
+    md_restore(S, ck);
   }
 }
 
@@ -1493,6 +1539,9 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
       conv[i] = kKcalToNative / m[i];
       sig[i] = std::sqrt(num / m[i]);
     }
+    M->mass_total = (double)kMassW * nw;
+    for (uint32_t i = 0; i < nd; ++i) M->mass_total += (double)m[i];
+    M->eng_mode_reference = engine_mode(eng) == DYN_MODE_REFERENCE;
     M->lang_c = c;
     M->sigma_o = std::sqrt(num / kMassO);
     M->sigma_h = std::sqrt(num / kMassH);
@@ -1654,6 +1703,8 @@ extern "C" int dyn_md_create(dyn_md** out, dyn_engine* eng, const dyn_md_params*
     M->x.reserve(ns);
     M->v.reserve(ns);
     M->x_ck.reserve(ns);
+    M->fb_ck.reserve(3 * (size_t)std::max(nd, 1u));
+    M->bpart_ck.reserve(4 * (size_t)M->bblocks);
     M->v_ck.reserve(ns);
     M->step_dev.reserve(2);  // the step index, the workgroup count of k_md_step_ba
     DYN_HIP(hipMemsetAsync(M->step_dev.p, 0, 16, M->stream));
@@ -1811,6 +1862,22 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     for (int a = 0; a < 3; ++a) E.box[a] = md->box[a];
     E.volume = (double)md->box[0] * md->box[1] * md->box[2];
     E.pcouple_steps = md->n_pcouple;
+    E.potential_bonded = E.bond + E.angle + E.dihedral;
+    {
+      dyn_energies EE{};
+      const int rc = dyn_engine_get_energies(md->eng, &EE);
+      if (rc == DYN_OK) {
+        E.potential_nonbonded = md->eng_mode_reference ? EE.reference : EE.total;
+        E.dh_dl = EE.dh_dl;
+      } else if (rc == DYN_ERR_STATE) {  // the last evaluation skipped energies
+        E.potential_nonbonded = E.dh_dl = std::nan("");
+      } else {
+        engine_call(rc, md->eng);
+      }
+    }
+    E.potential = E.potential_nonbonded + E.potential_bonded;
+    E.density = md->mass_total / E.volume;
+    E.repeats = md->n_repeats;
     *out = E;
   });
 }

@@ -431,6 +431,32 @@ def config2(seed: int = 0) -> System:
     return water_box(69900, 128.0, 42, seed=seed, jitter=0.3, name="opc69900_box128")
 
 
+def config4(n: int = 128, seed: int = 4) -> list:
+    """n solvated boxes with edges uniform in [24, 40] Å (≈0.0334 waters/Å³,
+    ≈460–2,140 waters each) around 20–60-atom chains (half heavy atoms),
+    seeded: the batched uneven-systems workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        L = float(rng.uniform(24.0, 40.0))
+        n_atoms = int(rng.integers(20, 61))
+        n_heavy = n_atoms // 2
+        lat = int(np.ceil((0.0334 * L ** 3) ** (1.0 / 3.0))) + 1  # room for the chain
+        n_water = int(round(0.0334 * L ** 3))
+        out.append(solvated_chain(L, n_water, n_heavy, n_atoms - n_heavy, seed=1000 + k,
+                                  lattice_n=lat, name=f"uneven{k}"))
+    return out
+
+
+def config3_lambdas(n: int = 16) -> list:
+    """λ of the n alchemical windows of config 1: k/(n−1); the decoupled set is
+    solute atoms [0, 40)."""
+    return [k / max(n - 1, 1) for k in range(n)]
+
+
+CONFIG3_ALCH = (0, 40)
+
+
 def small_water(n_water: int = 512, box: float = 24.8, lattice_n: int = 8, seed: int = 1) -> System:
     return water_box(n_water, box, lattice_n, seed=seed, name=f"opc{n_water}_box{box}")
 

@@ -39,6 +39,9 @@ extern "C" {
 #define DYN_ERR_STATE 3     /* call order (e.g. compute before set_pairs)    */
 #define DYN_ERR_CAPACITY 4  /* caller buffer too small; required size given  */
 #define DYN_ERR_NOMEM 5     /* device allocation failed                      */
+#define DYN_ERR_LIMIT 6     /* a fixed limit of the design was exceeded (a
+                               cluster column or a solute row too dense);
+                               calling again does not help                   */
 
 /* Version of the library (major*10000 + minor*100 + patch). */
 int dyn_version(void);
@@ -335,6 +338,14 @@ int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint32_t* n_clus
  * e_lj solute, e_coul water, e_coul solute, virial, pair counts (solute–
  * solute, LJ, Coulomb), dH/dλ — every pair counted from both of its rows. */
 int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint64_t n_items);
+/* Test hook (no counterpart in the reference): after `skip` more force
+ * evaluations of this engine, the next `count` evaluations get `bits` ORed
+ * into the engine's device status word before the pair kernel, as the
+ * device's own checks would set them (64: a site moved half the list buffer,
+ * the evaluation is void and the call repeats).  Lets tests run the error
+ * and repeat paths of dyn_engine_compute, dyn_md_step and dyn_batch_step
+ * without a real failure.  count = 0 switches it off. */
+int dyn_engine_debug_inject_status(dyn_engine* eng, uint32_t bits, uint32_t skip, uint32_t count);
 /* Diagnostics (no counterpart in the reference): uploads of kernel argument
  * packs so far in this process (packs are uploaded only when they change,
  * so steady-state steps add none). */
@@ -447,6 +458,21 @@ typedef struct dyn_md_energies {
   double volume;        /* Å³ */
   double box[3];        /* Å */
   uint64_t pcouple_steps; /* box rescalings so far */
+  /* The reference's SnapshotEnergyData quantities (src/snapshot.rs:61-92),
+   * from the last force evaluation (NaN when it skipped energies):
+   * energy_potential_nonbonded = the engine's reported energy (its
+   * `reference` field in reference mode, `total` in standard mode),
+   * energy_potential_bonded = bond + angle + dihedral, energy_potential =
+   * their sum, dh_dl of the decoupling, and density = total mass / volume in
+   * amu/Å³ (energy_kinetic is `kinetic`, temperature, pressure and volume
+   * are above; energy_potential_between_mols is dyn_engine_get_mol_energies). */
+  double potential_nonbonded;
+  double potential_bonded;
+  double potential;
+  double dh_dl;
+  double density;
+  uint64_t repeats;     /* step calls run again from their start (list
+                           overflow or stale list) since create */
 } dyn_md_energies;
 
 int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);
