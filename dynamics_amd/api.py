@@ -77,7 +77,7 @@ class MdEnergies(C.Structure):
                 ("box", C.c_double * 3), ("pcouple_steps", C.c_uint64),
                 ("potential_nonbonded", C.c_double), ("potential_bonded", C.c_double),
                 ("potential", C.c_double), ("dh_dl", C.c_double), ("density", C.c_double),
-                ("repeats", C.c_uint64)]
+                ("repeats", C.c_uint64), ("graph_steps", C.c_uint64)]
 
 
 class EngineParams(C.Structure):
@@ -293,13 +293,15 @@ class ForcesPositsGpu:
         return out.reshape(-1, 3)
 
     def correct_14(self, i14, j14, pos_dyn, q_dyn, extent, f_dyn, virial=0.0):
+        """The 1-4 correction (src/non_bonded.rs:763-790) added into f_dyn;
+        returns virial + its virial."""
         i14 = _c(i14, np.uint32)
         j14 = _c(j14, np.uint32)
         pd = _c(pos_dyn, np.float32)
         qd = _c(q_dyn, np.float32)
         w = C.c_double(virial)
         rc = lib.dyn_nb_correct_14(self.h, len(i14), _ptr(i14), _ptr(j14), _ptr(pd), _ptr(qd),
-                                   _c(extent, np.float32), f_dyn, C.byref(w))
+                _c(extent, np.float32), f_dyn, C.byref(w))
         _check(rc, self._err)
         return w.value
 
@@ -492,8 +494,9 @@ class Engine:
 
     def debug_inject_status(self, bits, skip=0, count=1):
         """Test hook: after `skip` evaluations, the next `count` get `bits`
-        ORed into the device status word (64 = stale list) before the pair
-        kernel, as the device's own checks would set them."""
+        ORed into the device status word (8 = the list's work items outgrew
+        their buffer) before the pair kernel, as the device's own checks
+        would set them."""
         _check(lib.dyn_engine_debug_inject_status(self.h, int(bits), int(skip), int(count)),
                self._err)
 

@@ -105,6 +105,19 @@ __global__ void __launch_bounds__(LB) k_single(const typename BodySig<F>::P a) {
 // upload none).
 inline std::atomic<uint64_t> g_arg_uploads{0};
 
+// Raised when work cannot be recorded into a graph (the caller stops the
+// capture and enqueues the work directly).
+struct CaptureRefused : Error {
+  explicit CaptureRefused(const std::string& m) : Error(DYN_ERR_STATE, m) {}
+};
+
+// Stream ordering between two streams (fork / join), skipped by dry runs.
+inline void stream_order(hipStream_t from, hipStream_t to, hipEvent_t ev) {
+  if (g_launch_mode == kLaunchDry || from == to) return;
+  DYN_HIP(hipEventRecord(ev, from));
+  DYN_HIP(hipStreamWaitEvent(to, ev, 0));
+}
+
 // Device copies of the argument packs of each launch site.
 class ArgCache {
  public:
@@ -117,33 +130,59 @@ class ArgCache {
   }
 
   // Device array of the packs in v (site: a string literal naming the launch
-  // site, tag: a second key for sites run more than once per pass).
+  // site, tag: a second key for sites run more than once per pass).  A site
+  // keeps up to kVersions device copies (steps that alternate between packs,
+  // such as the SPME cadence, find theirs uploaded); reusing a copy for other
+  // packs advances the layout generation, since a captured graph may hold it.
   template <class P>
   const P* put(hipStream_t stream, const char* site, int tag, const std::vector<P>& v) {
     static_assert(std::is_trivially_copyable<P>::value, "argument packs are plain data");
-    Slot& s = slots_[{site, tag}];
+    Site& st = sites_[{site, tag}];
     const size_t bytes = v.size() * sizeof(P);
-    if (s.dev.n < bytes) {
-      s.dev.reserve(bytes);
-      s.last.clear();
+    ++st.clock;
+    for (Copy& c : st.copies)
+      if (c.last.size() == bytes && std::memcmp(c.last.data(), v.data(), bytes) == 0) {
+        c.used = st.clock;
+        return reinterpret_cast<const P*>(c.dev.p);
+      }
+    // a graph would keep copying from a staging region later calls reuse
+    if (g_launch_mode == kLaunchCapture)
+      throw CaptureRefused(std::string("argument packs of ") + site + " changed during capture");
+    Copy* c = nullptr;
+    if (st.copies.size() < kVersions) {
+      st.copies.emplace_back();
+      c = &st.copies.back();
+    } else {
+      c = &st.copies[0];
+      for (Copy& x : st.copies)
+        if (x.used < c->used) c = &x;
+      g_layout_gen.fetch_add(1, std::memory_order_relaxed);
     }
-    if (s.last.size() != bytes || std::memcmp(s.last.data(), v.data(), bytes) != 0) {
-      upload(stream, s.dev.p, v.data(), bytes);
-      s.last.assign(reinterpret_cast<const char*>(v.data()),
-                    reinterpret_cast<const char*>(v.data()) + bytes);
-    }
-    return reinterpret_cast<const P*>(s.dev.p);
+    c->dev.reserve(bytes);
+    upload(stream, c->dev.p, v.data(), bytes);
+    c->last.assign(reinterpret_cast<const char*>(v.data()),
+                   reinterpret_cast<const char*>(v.data()) + bytes);
+    c->used = st.clock;
+    return reinterpret_cast<const P*>(c->dev.p);
   }
 
   // Forget every device copy (the next launches upload again).
   void invalidate() {
-    for (auto& kv : slots_) kv.second.last.clear();
+    for (auto& kv : sites_)
+      for (Copy& c : kv.second.copies) c.last.clear();
+    g_layout_gen.fetch_add(1, std::memory_order_relaxed);
   }
 
  private:
-  struct Slot {
+  static constexpr size_t kVersions = 4;
+  struct Copy {
     DevBuf<char> dev;
     std::vector<char> last;
+    uint64_t used = 0;
+  };
+  struct Site {
+    std::deque<Copy> copies;  // stable addresses as copies are added
+    uint64_t clock = 0;
   };
   struct Region {
     size_t lo, hi;
@@ -188,7 +227,7 @@ class ArgCache {
     head_ = hi;
   }
 
-  std::map<std::pair<const char*, int>, Slot> slots_;
+  std::map<std::pair<const char*, int>, Site> sites_;
   PinnedBuf<char> ring_;
   size_t head_ = 0;
   std::deque<Region> inflight_;
@@ -207,7 +246,7 @@ struct BatchRun {
 template <auto F, int LB = 256, class Fn>
 void launch(const BatchRun& R, const char* site, int tag, unsigned gx, unsigned block,
             size_t shm, Fn&& fn) {
-  if (R.n == 0) return;
+  if (R.n == 0 || g_launch_mode == kLaunchDry) return;
   if (R.n == 1) {
     const typename BodySig<F>::P p = fn(0u);
     hipLaunchKernelGGL((k_single<F, LB>), dim3(std::max(gx, 1u)), dim3(block), shm, R.stream, p);

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -57,6 +58,13 @@ inline std::string fmt(const char* f, ...) {
     if (!(cond)) throw ::dyn::Error(DYN_ERR_ARG, ::dyn::fmt(__VA_ARGS__));         \
   } while (0)
 
+// Device memory layout generation: every release of a device buffer (also
+// the one that precedes a larger allocation) advances it, so a captured
+// graph (whose kernels hold buffer addresses) knows whether its addresses
+// are still those of live buffers.  A new buffer leaves it: no graph can
+// hold an address that did not exist.
+inline std::atomic<uint64_t> g_layout_gen{1};
+
 // Device buffer owned by a handle.  Size in elements; zero-size buffers hold
 // no allocation (the pointer is null and no kernel may touch it).
 template <typename T>
@@ -68,7 +76,10 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      g_layout_gen.fetch_add(1, std::memory_order_relaxed);
+    }
     p = nullptr;
     n = 0;
   }
@@ -112,9 +123,20 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned max_blocks = 65535u 
   return (unsigned)g;
 }
 
+// What enqueueing does with work on this thread: kLaunchDirect launches it;
+// kLaunchCapture launches it into a stream capture (a graph's recording);
+// kLaunchDry runs only the host side of the enqueueing code (its
+// bookkeeping) while a graph holding the same work is launched instead.
+enum LaunchMode { kLaunchDirect = 0, kLaunchDry = 1, kLaunchCapture = 2 };
+inline thread_local int g_launch_mode = kLaunchDirect;
+
 // Per-kernel timing with HIP events on the handle's stream.  Off by default;
 // when on, every launch wrapped in Profiler::Scope records a start/stop pair,
-// and read() synchronizes and sums elapsed times per kernel name.
+// and read() synchronizes and sums elapsed times per kernel name.  Scopes
+// recorded into a graph become event nodes of the graph: their records go to
+// capture_sink (kept with the graph), and every launch of the graph adds them
+// to the pending ones again (their events then hold that launch's times,
+// read once the work has completed).
 struct Profiler {
   bool enabled = false;
   const char* only = nullptr;  // when set, only scopes of this name are timed
@@ -123,8 +145,11 @@ struct Profiler {
     hipEvent_t a, b;
     const char* name;
     double host_ms;  // host time spent issuing the scope's work
+    bool keep = false;  // events owned by a graph (not returned to the pool)
   };
   std::vector<Rec> pending;
+  std::vector<Rec>* capture_sink = nullptr;
+  std::vector<hipEvent_t> graph_events;  // events recorded into graphs
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, uint64_t>> totals;
   std::vector<std::string> order;
@@ -144,7 +169,10 @@ struct Profiler {
     Rec r;
     std::chrono::steady_clock::time_point t0;
     Scope(Profiler& prof, const char* name)
-        : p(prof.enabled && (!prof.only || std::strcmp(prof.only, name) == 0) ? &prof : nullptr) {
+        : p(prof.enabled && g_launch_mode != kLaunchDry &&
+                    (!prof.only || std::strcmp(prof.only, name) == 0)
+                ? &prof
+                : nullptr) {
       if (p) {
         r.name = name;
         r.a = p->ev();
@@ -157,8 +185,15 @@ struct Profiler {
       if (p) {
         r.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         DYN_HIP(hipEventRecord(r.b, p->stream));
-        p->pending.push_back(r);
-        if (p->pending.size() > 4096) p->drain();
+        if (g_launch_mode == kLaunchCapture && p->capture_sink) {
+          r.keep = true;  // a graph holds the events: they stay until the profiler goes
+          p->graph_events.push_back(r.a);
+          p->graph_events.push_back(r.b);
+          p->capture_sink->push_back(r);
+        } else {
+          p->pending.push_back(r);
+          if (p->pending.size() > 4096) p->drain();
+        }
       }
     }
   };
@@ -182,8 +217,10 @@ struct Profiler {
       }
       totals[hn].first += r.host_ms;
       totals[hn].second += 1;
-      pool.push_back(r.a);
-      pool.push_back(r.b);
+      if (!r.keep) {
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+      }
     }
     pending.clear();
   }
@@ -204,11 +241,13 @@ struct Profiler {
     order.clear();
   }
   ~Profiler() {
-    for (auto& r : pending) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
-    }
+    for (auto& r : pending)
+      if (!r.keep) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+      }
     for (auto e : pool) (void)hipEventDestroy(e);
+    for (auto e : graph_events) (void)hipEventDestroy(e);
   }
 };
 

@@ -55,8 +55,7 @@ constexpr int kNumPart = 9;           // f64 partials per work item of the pair 
 
 // status bits
 constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u,
-                   kStPme = 32u,   // an SPME brick list overflowed
-                   kStDisp = 64u;  // a site moved half the list buffer: the list was stale
+                   kStPme = 32u;   // an SPME brick list overflowed
 
 struct ClassGrid {
   int ncx, ncy;
@@ -72,6 +71,22 @@ struct EngGeom {
   int n_cols;
   uint32_t n_dyn, n_water;
 };
+
+// Whether this evaluation rebuilds the pair list.  The host asks for a build
+// (force: the first evaluation, a new box, positions set from the host, no
+// list buffer), or the preparation kernel of this evaluation has found a
+// site that moved half the list buffer since the last build (the largest
+// squared displacement, as float bits, in *disp_bits).  Every build kernel
+// reads the same word the preparation kernel completed, so they all agree;
+// the finalize kernel clears it for the next evaluation.
+struct BuildGate {
+  const unsigned int* disp_bits;
+  unsigned int limit;  // bits of (buffer/2)² less a rounding margin
+  int force;
+};
+__device__ __forceinline__ bool build_on(const BuildGate& G) {
+  return G.force != 0 || *G.disp_bits >= G.limit;
+}
 
 __device__ __forceinline__ float wrap1(float x, float L, float invL) {
   float w = x - L * floorf(x * invL);
@@ -96,36 +111,38 @@ __device__ __forceinline__ uint32_t site_column(const ClassGrid& cg, float x, fl
   return (uint32_t)(cx * cg.ncy + cy);
 }
 
-// Wrap every site into the box, find its x–y column and take a rank in it.
-// The rank order depends on atomic timing; the column sort that follows
-// orders each column by (z, site), so the clusters do not.
-__device__ __forceinline__ void k_eng_bin_count(const float4* __restrict__ pos, const float* __restrict__ q,
-                                EngGeom g, float4* __restrict__ wpos, float4* __restrict__ wpos_ref,
-                                uint32_t* __restrict__ col_count, uint2* __restrict__ site_col) {
+// Find every site's x–y column (from the positions the preparation kernel
+// wrapped) and take a rank in it; the positions become the reference of
+// the displacement readings.  The rank order depends on atomic timing; the
+// column sort that follows orders each column by (z, site), so the
+// clusters do not.
+__device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __restrict__ wpos,
+                                                EngGeom g, float4* __restrict__ wpos_ref,
+                                                uint32_t* __restrict__ col_count,
+                                                uint2* __restrict__ site_col,
+                                                unsigned int* __restrict__ n_builds) {
+  if (!build_on(G)) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) ++*n_builds;
   const bool valid = s < g.n_dyn + 4u * g.n_water;
   uint32_t col = 0;
   if (valid) {
-    const float4 p = pos[s];
-    const float x = wrap1(p.x, g.L[0], g.invL[0]);
-    const float y = wrap1(p.y, g.L[1], g.invL[1]);
-    const float z = wrap1(p.z, g.L[2], g.invL[2]);
-    const float4 w = make_float4(x, y, z, q[s]);
-    wpos[s] = w;
-    if (wpos_ref) wpos_ref[s] = w;
+    const float4 w = wpos[s];
+    wpos_ref[s] = w;
     const bool solute = s < g.n_dyn;
     const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
     const ClassGrid cg = solute ? g.cg[0] : (oxy ? g.cg[1] : g.cg[2]);
-    col = (uint32_t)cg.col_off + site_column(cg, x, y);
+    col = (uint32_t)cg.col_off + site_column(cg, w.x, w.y);
   }
   // one atomic per distinct column of the wave
   const uint32_t rank = wave_rank(col, valid, col_count);
   if (valid) site_col[s] = make_uint2(col, rank);
 }
 
-__device__ __forceinline__ void k_eng_bin_scatter(const uint2* __restrict__ site_col,
+__device__ __forceinline__ void k_eng_bin_scatter(BuildGate G, const uint2* __restrict__ site_col,
                                   const uint32_t* __restrict__ col_start, uint32_t n,
                                   uint32_t* __restrict__ items) {
+  if (!build_on(G)) return;
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint2 cr = site_col[s];
@@ -140,11 +157,12 @@ __device__ __forceinline__ void k_eng_bin_scatter(const uint2* __restrict__ site
 constexpr int kScanBlock = 1024;
 template <int MODE>
 __device__ __forceinline__ void
-k_eng_scan2(const uint32_t* __restrict__ in_a, const uint32_t* __restrict__ in_b, uint32_t n_host,
-            const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ out_a,
+k_eng_scan2(BuildGate G, const uint32_t* __restrict__ in_a, const uint32_t* __restrict__ in_b,
+            uint32_t n_host, const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ out_a,
             uint32_t* __restrict__ out_b, uint32_t* __restrict__ tot_a,
             uint32_t* __restrict__ tot_b, uint32_t* __restrict__ max_a) {
   __shared__ uint32_t wa[kScanBlock / kWave], wb[kScanBlock / kWave], wm[kScanBlock / kWave];
+  if (!build_on(G)) return;
   const uint32_t n = n_dev ? *n_dev : n_host;
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
@@ -224,12 +242,13 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
 // binning pass (the binning kernel counts with atomics into zeroed counts).
 constexpr int kSortBlock = 256;
 __device__ __forceinline__ void
-k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
+k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
               const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
               const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
               float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
               unsigned int* __restrict__ status, uint32_t n_cols) {
   __shared__ unsigned long long key[kColCap];
+  if (!build_on(G)) return;
   const int c = blockIdx.x;
   if ((uint32_t)c >= n_cols) return;
   const uint32_t n = col_count[c];
@@ -307,59 +326,8 @@ k_eng_colsort(uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col
   }
 }
 
-// Wrap every site into the box and maximise its squared displacement since
-// the list build into *disp_bits (one atomic per workgroup; squared
-// distances are non-negative floats, ordered like their bit patterns).  The
-// previous evaluation's finalize cleared *disp_bits.
-__device__ __forceinline__ void k_eng_wrap_disp(const float4* __restrict__ pos,
-                                                const float* __restrict__ q, uint32_t n, EngGeom g,
-                                                float4* __restrict__ wpos,
-                                                const float4* __restrict__ ref,
-                                                unsigned int* __restrict__ disp_bits,
-                                                unsigned int* __restrict__ status,
-                                                unsigned int disp_limit) {
-  __shared__ unsigned int red[256 / kWave];
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned int v = 0u;
-  if (s < n) {
-    const float4 p = pos[s];
-    const float4 w = make_float4(wrap1(p.x, g.L[0], g.invL[0]), wrap1(p.y, g.L[1], g.invL[1]),
-                                 wrap1(p.z, g.L[2], g.invL[2]), q[s]);
-    wpos[s] = w;
-    const float4 b = ref[s];
-    float dx = w.x - b.x, dy = w.y - b.y, dz = w.z - b.z;
-    dx -= g.L[0] * rintf(dx * g.invL[0]);
-    dy -= g.L[1] * rintf(dy * g.invL[1]);
-    dz -= g.L[2] * rintf(dz * g.invL[2]);
-    v = __float_as_uint(dx * dx + dy * dy + dz * dz);
-  }
-  for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int mx = 0u;
-    for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
-    if (mx) atomicMax(disp_bits, mx);
-    if (mx >= disp_limit) atomicOr(status, kStDisp);
-  }
-}
-
-// Slot positions from the current wrapped positions (list-reuse path).
-__device__ __forceinline__ void k_eng_slots(const int32_t* __restrict__ slot_site, const uint32_t* __restrict__ n_cl,
-                            const float4* __restrict__ wpos, float4* __restrict__ xyzq) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= *n_cl * DYN_CL_SIZE) return;
-  const int32_t s = slot_site[k];
-  if (s >= 0) {
-    xyzq[k] = wpos[s];
-  } else {
-    float4 p = wpos[slot_site[k & ~(uint32_t)(DYN_CL_SIZE - 1)]];
-    p.w = 0.0f;
-    xyzq[k] = p;
-  }
-}
-
 struct ListArgs {
+  BuildGate gate;
   EngGeom g;
   float rlist2, rlist;
   const uint32_t* n_cl;
@@ -383,28 +351,27 @@ struct ListArgs {
 
 // LDS of the row walk: a batch of up to 64 columns (first cluster, range in
 // the flattened candidate sequence), then the batch's candidate clusters
-// that passed the box test and their slots' positions and groups.
+// that passed the box test.
 struct WalkLds {
   uint32_t js[kWave], beg[kWave], end[kWave];
   uint32_t okJ[kWave];
-  float4 jp[kWave * DYN_CL_SIZE];
-  int32_t jg[kWave * DYN_CL_SIZE];
 };
 
 // Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
 // order (class, column x, column y, cluster), keep those whose bounding box
 // is within rlist of I's box (periodic), test their 8 slots against I's 8
-// slots (the same f32 min image and r² as the pair kernel), and append every
-// j slot within rlist of at least one i slot it can interact with.  Each
+// slots (the same f32 min image and r² as the pair kernel; eight candidates
+// per round, a lane per j slot), and append every j slot within rlist of at
+// least one i slot it can interact with.  Each
 // class segment is padded to a multiple of 8 with the null slot.  The row
 // goes into its fixed-capacity slice of ent_tmp; counts are in groups of 8.
 __device__ __forceinline__ void
 k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
                 uint32_t* __restrict__ row_chunks) {
   __shared__ WalkLds L;
+  if (!build_on(A.gate)) return;
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x;
-  const int ia = lane & (DYN_CL_SIZE - 1), jb = lane / DYN_CL_SIZE;
   const unsigned long long lt = (1ull << lane) - 1ull;
   // a column is skipped when its x–y rectangle is beyond rlist of I's box;
   // the margin keeps the test conservative against rounding of the binning
@@ -413,8 +380,13 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
     uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
     const int ci = A.cl_class[I];
     const float4 c = A.bb_c[I], h = A.bb_h[I];
-    const float4 pi = A.xyzq[I * DYN_CL_SIZE + ia];
-    const int gi = A.slot_gid[I * DYN_CL_SIZE + ia];
+    float4 pI[DYN_CL_SIZE];
+    int gI[DYN_CL_SIZE];
+#pragma unroll
+    for (int u = 0; u < DYN_CL_SIZE; ++u) {
+      pI[u] = A.xyzq[I * DYN_CL_SIZE + u];
+      gI[u] = A.slot_gid[I * DYN_CL_SIZE + u];
+    }
     uint32_t n_out = 0;
     for (int cls = 0; cls < 3; ++cls) {
       if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
@@ -487,25 +459,32 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
           const uint32_t n_ok = (uint32_t)__popcll(bal);
           if (ok) L.okJ[__popcll(bal & lt)] = J;
           __syncthreads();
-          for (uint32_t u = lane; u < n_ok * DYN_CL_SIZE; u += kWave) {
-            const uint32_t slot = L.okJ[u / DYN_CL_SIZE] * DYN_CL_SIZE + (u % DYN_CL_SIZE);
-            L.jp[u] = A.xyzq[slot];
-            L.jg[u] = A.slot_gid[slot];
-          }
-          __syncthreads();
-          for (uint32_t kk = 0; kk < n_ok; ++kk) {
-            const float4 pj = L.jp[kk * DYN_CL_SIZE + jb];
-            const int gj = L.jg[kk * DYN_CL_SIZE + jb];
-            const float d0 = dyn_min_image_rint(pi.x - pj.x, A.g.L[0], A.g.invL[0]);
-            const float d1 = dyn_min_image_rint(pi.y - pj.y, A.g.L[1], A.g.invL[1]);
-            const float d2 = dyn_min_image_rint(pi.z - pj.z, A.g.L[2], A.g.invL[2]);
-            const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-            const unsigned long long hb = __ballot(r2 < A.rlist2 && dyn_slot_pair_valid(gi, gj));
-            const bool keep = lane < DYN_CL_SIZE && ((hb >> (DYN_CL_SIZE * lane)) & 0xffull) != 0ull;
+          // eight candidate clusters per round: lane jb + 8·c tests j slot
+          // jb of candidate c against all eight i slots (held in registers),
+          // so the kept slots come out candidate by candidate, each in slot
+          // order, as a one-candidate-at-a-time walk would write them
+          for (uint32_t k0 = 0; k0 < n_ok; k0 += kWave / DYN_CL_SIZE) {
+            const uint32_t kk = k0 + (uint32_t)(lane / DYN_CL_SIZE);
+            const int jj = lane & (DYN_CL_SIZE - 1);
+            bool keep = false;
+            uint32_t J = 0;
+            if (kk < n_ok) {
+              J = L.okJ[kk];
+              const float4 pj = A.xyzq[J * DYN_CL_SIZE + jj];
+              const int gj = A.slot_gid[J * DYN_CL_SIZE + jj];
+#pragma unroll
+              for (int u = 0; u < DYN_CL_SIZE; ++u) {
+                const float d0 = dyn_min_image_rint(pI[u].x - pj.x, A.g.L[0], A.g.invL[0]);
+                const float d1 = dyn_min_image_rint(pI[u].y - pj.y, A.g.L[1], A.g.invL[1]);
+                const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
+                const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+                keep = keep || (r2 < A.rlist2 && dyn_slot_pair_valid(gI[u], gj));
+              }
+            }
             const unsigned long long kb = __ballot(keep);
             if (keep) {
               const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
-              if (pos < A.row_cap) ent[pos] = (L.okJ[kk] * DYN_CL_SIZE + lane) | tag;
+              if (pos < A.row_cap) ent[pos] = (J * DYN_CL_SIZE + jj) | tag;
             }
             n_out += (uint32_t)__popcll(kb);
           }
@@ -535,6 +514,7 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ chunk_ptr, uint32_t* __restrict__ cj,
                 uint2* __restrict__ chunks) {
+  if (!build_on(A.gate)) return;
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x;
   for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
@@ -580,6 +560,7 @@ k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
   __shared__ uint16_t h_val[kMaskHash];
   __shared__ unsigned long long s_mask[kMaskCap / DYN_CL_SIZE];
   __shared__ uint32_t s_beg[DYN_CL_SIZE + 1], s_ex0[DYN_CL_SIZE];
+  if (!build_on(A.gate)) return;
   const uint32_t n_sol = col_cl_start[A.g.cg[1].col_off];
   const int lane = threadIdx.x;
   for (uint32_t I = blockIdx.x; I < n_sol; I += gridDim.x) {
@@ -1037,54 +1018,14 @@ k_eng_corr(CorrArgs A) {
 // gather the charged sites' (x, y, z, q) for the SPME spread.
 // With a decoupled molecule (sites [alch0, alch1)), pme_b receives a second
 // charge set with that molecule's charges negated.
-__device__ __forceinline__ void k_eng_site_prep(const float4* __restrict__ wpos, uint32_t n_sites,
-                                const uint32_t* __restrict__ qsite, uint32_t nq,
-                                float4* __restrict__ pme_xyzq, float4* __restrict__ pme_b,
-                                uint32_t alch0, uint32_t alch1, int64_t* __restrict__ fx_site,
-                                int64_t* __restrict__ fx_recip, int64_t* __restrict__ mol_e,
-                                uint32_t n_mol2) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (mol_e)
-    for (uint32_t k = s; k < n_mol2; k += gridDim.x * blockDim.x) mol_e[k] = 0;
-  if (s < n_sites) {
-    fx_site[s] = 0;
-    fx_site[(size_t)n_sites + s] = 0;
-    fx_site[2 * (size_t)n_sites + s] = 0;
-    if (fx_recip) {
-      fx_recip[s] = 0;
-      fx_recip[(size_t)n_sites + s] = 0;
-      fx_recip[2 * (size_t)n_sites + s] = 0;
-    }
-  }
-  if (pme_xyzq && s < nq) {
-    const uint32_t site = qsite[s];
-    const float4 p = wpos[site];
-    pme_xyzq[s] = p;
-    if (pme_b) pme_b[s] = make_float4(p.x, p.y, p.z, site >= alch0 && site < alch1 ? -p.w : p.w);
-  }
-}
-
-// List-reuse path: the slot positions and the site set-up in one launch
-// (grid over the larger of the two ranges).
-__device__ __forceinline__ void k_eng_slots_prep(const int32_t* __restrict__ slot_site,
-                                                 const uint32_t* __restrict__ n_cl,
-                                                 float4* __restrict__ xyzq,
-                                                 const float4* __restrict__ wpos, uint32_t n_sites,
-                                                 const uint32_t* __restrict__ qsite, uint32_t nq,
-                                                 float4* __restrict__ pme_xyzq,
-                                                 float4* __restrict__ pme_b, uint32_t alch0,
-                                                 uint32_t alch1, int64_t* __restrict__ fx_site,
-                                                 int64_t* __restrict__ fx_recip,
-                                                 int64_t* __restrict__ mol_e, uint32_t n_mol2) {
-  k_eng_slots(slot_site, n_cl, wpos, xyzq);
-  k_eng_site_prep(wpos, n_sites, qsite, nq, pme_xyzq, pme_b, alch0, alch1, fx_site, fx_recip,
-                  mol_e, n_mol2);
-}
-
-// List-reuse evaluation with a lagged displacement check: wrapping and the
-// displacement reading, the slot positions and the site set-up in one
-// launch.  Slots and charges take their wrapped positions from pos directly
-// (the same wrap1 of the same input as wpos).
+// The first launch of every evaluation: wrapping, the displacement reading
+// that decides on a list build (the largest squared displacement of any
+// site since the last build, one atomic per workgroup; squared distances
+// are non-negative floats, ordered like their bit patterns), the slot
+// positions of the current list (a build rewrites them), the SPME charges
+// and their binning, and clearing of the accumulators.  Slots and charges
+// take their wrapped positions from pos directly (the same wrap1 of the
+// same input as wpos).
 __device__ __forceinline__ float4 wrapped(const float4* __restrict__ pos,
                                           const float* __restrict__ q, const EngGeom& g,
                                           uint32_t s) {
@@ -1114,8 +1055,6 @@ struct PrepArgs {
   uint32_t n_mol2;
   BinArgs bin;        // SPME binning of the charges written here
   int bin_on;
-  unsigned int* status;
-  unsigned int disp_limit;  // bits of (buffer/2)²: a larger displacement flags kStDisp
 };
 
 __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
@@ -1174,7 +1113,6 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
     unsigned int mx = 0u;
     for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
     if (mx) atomicMax(A.disp_bits, mx);
-    if (mx >= A.disp_limit) atomicOr(A.status, kStDisp);
   }
 }
 
@@ -1387,14 +1325,16 @@ struct dyn_engine {
   // [4] longest row that overflowed its slice (entries)
   DevBuf<uint32_t> counters;
   DevBuf<unsigned int> status;
-  uint64_t n_builds = 0;
+  // [0] largest squared displacement since the last build (float bits; the
+  // preparation kernel maximises it, the finalize kernel clears it), [1] list
+  // builds so far
+  DevBuf<unsigned int> dflags;
   // accumulators
   DevBuf<int64_t> fitem, fx_site, fx_out, fx_slot;  // fx_slot: inspection only
   DevBuf<double> pair_part, corr_part, eng_out, red1;
   unsigned corr_blocks = 0;
-  // a pass over this engine alone: argument packs and displacement readings
-  ArgCache own_cache;
-  DispRing own_disp;
+  // a pass over this engine alone: argument packs of both streams
+  ArgCache own_cache, side_cache;
   // PME
   bool pme_on = false;
   PmeCore pme;
@@ -1408,6 +1348,8 @@ struct dyn_engine {
   bool fx_fresh = false;
 
   hipStream_t own_stream = nullptr;  // the engine's stream (a batch may lend it another)
+  hipStream_t side_stream = nullptr;  // its second stream (SPME), same priority
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // the forces, SPME forces and energies of the last evaluation, kept at the
   // start of an MD call so that a repeat or a failed call restores them
   DevBuf<int64_t> fx_out_ck, fx_recip_ck;
@@ -1422,6 +1364,9 @@ struct dyn_engine {
   uint32_t dbg_skip = 0, dbg_count = 0;
 
   ~dyn_engine() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
@@ -1435,34 +1380,18 @@ struct dyn_engine {
     S.R.stream = stream;
     S.R.cache = &own_cache;
     S.R.n = 1;
+    // fully profiled evaluations stay on one stream (their per-kernel times)
+    if ((!prof.enabled || prof.only) && stream == own_stream) {
+      S.side.stream = side_stream;
+      S.side.cache = &side_cache;
+      S.side.n = 1;
+      S.ev_fork = ev_fork;
+      S.ev_join = ev_join;
+    }
     S.prof = &prof;
-    S.disp = &own_disp;
     return S;
   }
 };
-
-namespace dyn {
-
-void DispRing::init(uint32_t systems) {
-  if (systems == n && bits.p) return;
-  n = systems;
-  bits.reserve(std::max<uint32_t>(n, 1));
-  // zero readings; every evaluation's finalize clears its system's again
-  DYN_HIP(hipMemset(bits.p, 0, sizeof(unsigned int) * std::max<uint32_t>(n, 1)));
-  h.reserve((size_t)kRing * std::max<uint32_t>(n, 1));
-  h_sync.reserve(std::max<uint32_t>(n, 1));
-  for (auto& e : ev)
-    if (!e) DYN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  seen.assign(n, 0.f);
-  for (int k = 0; k < kRing; ++k) pending[k] = false;
-}
-
-DispRing::~DispRing() {
-  for (auto& e : ev)
-    if (e) (void)hipEventDestroy(e);
-}
-
-}  // namespace dyn
 
 void dyn_engine::set_geometry() {
   const double V = (double)box[0] * box[1] * box[2];
@@ -1541,6 +1470,9 @@ void dyn_engine::set_geometry() {
   // the binning kernel counts into zeroed column counts; the column sort
   // clears them again after use
   DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
+  // no clusters until the next build: the preparation kernel gathers slot
+  // positions only for the clusters of a built list
+  if (counters.p) DYN_HIP(hipMemsetAsync(counters.p, 0, 4, stream));
   DYN_HIP(hipStreamSynchronize(stream));
   list_valid = false;
 }
@@ -1577,8 +1509,29 @@ static unsigned max_grid(const EngineSet& S, Fn&& fn) {
   return g;
 }
 
-// Cluster-pair lists of every engine of the set.
-static void set_build_lists(EngineSet& S) {
+// Bits of the squared displacement at which an engine rebuilds its list:
+// half the list buffer (every pair now within the cutoff was then within
+// the list radius), less a margin for the rounding of the f32 readings.
+static unsigned int build_limit(const dyn_engine& e) {
+  const float half = 0.5f * e.prm.list_buffer * (1.0f - 1e-4f);
+  const float h2 = half * half;
+  unsigned int b;
+  std::memcpy(&b, &h2, 4);
+  return b;
+}
+
+// The gate of engine e's build kernels in this evaluation.
+static BuildGate build_gate(const dyn_engine& e, bool force) {
+  BuildGate G;
+  G.disp_bits = e.dflags.p + 0;
+  G.limit = build_limit(e);
+  G.force = (force || e.prm.list_buffer <= 0.f) ? 1 : 0;
+  return G;
+}
+
+// Cluster-pair list build of every engine of the set; the kernels of an
+// engine whose gate is closed return at once.
+static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
   auto& E = S.e;
   const BatchRun& R = S.R;
   Profiler& prof = *S.prof;
@@ -1587,20 +1540,18 @@ static void set_build_lists(EngineSet& S) {
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
     launch<k_eng_bin_count>(R, "k_eng_bin_count", 0, gs, 256, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      const bool track = e.prm.list_buffer > 0.f;
-      return args<k_eng_bin_count>(e.pos_dev, e.q.p, e.geom, e.wpos.p,
-                                   track ? e.wpos_ref.p : (float4*)nullptr, e.col_count.p,
-                                   e.site_col.p);
+      return args<k_eng_bin_count>(gate[i], e.wpos.p, e.geom, e.wpos_ref.p, e.col_count.p,
+                                   e.site_col.p, e.dflags.p + 1);
     });
     launch<k_eng_scan2<0>, kScanBlock>(R, "k_eng_scan2<0>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_scan2<0>>(e.col_count.p, (const uint32_t*)nullptr, e.n_cols,
+      return args<k_eng_scan2<0>>(gate[i], e.col_count.p, (const uint32_t*)nullptr, e.n_cols,
                                   (const uint32_t*)nullptr, e.col_start.p, e.col_cl_start.p,
                                   (uint32_t*)nullptr, e.counters.p + 0, (uint32_t*)nullptr);
     });
     launch<k_eng_bin_scatter>(R, "k_eng_bin_scatter", 0, gs, 256, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_bin_scatter>(e.site_col.p, e.col_start.p, e.n_sites, e.items.p);
+      return args<k_eng_bin_scatter>(gate[i], e.site_col.p, e.col_start.p, e.n_sites, e.items.p);
     });
   }
   {
@@ -1612,9 +1563,9 @@ static void set_build_lists(EngineSet& S) {
       std::memset(&A, 0, sizeof A);
       A = SlotArgs{e.wpos.p, e.ltype.p, e.wmol.p, e.type_none, e.xyzq.p, e.slot_type.p,
                    e.slot_gid.p, e.slot_site.p, e.slot_of.p};
-      return args<k_eng_colsort>(e.col_count.p, e.col_start.p, e.items.p, e.col_cl_start.p,
-                                 e.col_class.p, A, e.cl_class.p, e.bb_c.p, e.bb_h.p, e.n_sites,
-                                 e.status.p, e.n_cols);
+      return args<k_eng_colsort>(gate[i], e.col_count.p, e.col_start.p, e.items.p,
+                                 e.col_cl_start.p, e.col_class.p, A, e.cl_class.p, e.bb_c.p,
+                                 e.bb_h.p, e.n_sites, e.status.p, e.n_cols);
     });
   }
   std::vector<ListArgs> LA(E.size());
@@ -1622,6 +1573,7 @@ static void set_build_lists(EngineSet& S) {
     dyn_engine& e = *E[i];
     ListArgs& A = LA[i];
     std::memset(&A, 0, sizeof A);
+    A.gate = gate[i];
     A.g = e.geom;
     // FP64 pair arithmetic decides the cutoff in double: 0.01 Å of list
     // margin keeps every pair the f32 distance puts just outside
@@ -1656,8 +1608,8 @@ static void set_build_lists(EngineSet& S) {
     });
     launch<k_eng_scan2<1>, kScanBlock>(R, "k_eng_scan2<1>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_scan2<1>>(e.row_cnt.p, e.row_chunks.p, 0u, e.counters.p + 0, e.row_ptr.p,
-                                  e.chunk_ptr.p, e.counters.p + 1, e.counters.p + 2,
+      return args<k_eng_scan2<1>>(gate[i], e.row_cnt.p, e.row_chunks.p, 0u, e.counters.p + 0,
+                                  e.row_ptr.p, e.chunk_ptr.p, e.counters.p + 1, e.counters.p + 2,
                                   e.counters.p + 3);
     });
   }
@@ -1683,82 +1635,24 @@ static void set_build_lists(EngineSet& S) {
       });
     }
   }
-  for (dyn_engine* e : E) {
-    e->list_valid = true;
-    ++e->n_builds;
-  }
-  ++S.disp->builds;
-  std::fill(S.disp->seen.begin(), S.disp->seen.end(), 0.f);
 }
 
-void engine_set_enqueue(EngineSet& S, uint32_t flags) {
+void engine_set_enqueue(EngineSet& S, uint32_t flags,
+                        const std::function<void(const BatchRun&)>& side_work) {
   auto& E = S.e;
   const uint32_t ns = (uint32_t)E.size();
   const BatchRun& R = S.R;
+  const BatchRun& RS = S.side.stream ? S.side : S.R;
   Profiler& prof = *S.prof;
-  DispRing& D = *S.disp;
-  D.init(ns);
-  bool rebuild = (flags & DYN_COMPUTE_REBUILD) != 0;
-  for (dyn_engine* e : E) {
+  // host-forced builds: asked for, or no list (never built, a new box,
+  // positions set from the host, a failed evaluation); the device decides
+  // for the others from their displacement readings
+  std::vector<BuildGate> gate(ns);
+  for (uint32_t i = 0; i < ns; ++i) {
+    dyn_engine* e = E[i];
     DYN_REQUIRE(e->box_set, "box not set");
     if (!e->pos_set) throw Error(DYN_ERR_STATE, "positions not set");
-    if (!e->list_valid || e->prm.list_buffer <= 0.f) rebuild = true;
-  }
-  const bool async = (flags & DYN_COMPUTE_ASYNC) != 0;
-  // the displacement reading of ring slot k, taken into the current lists' maximum
-  auto consume = [&](int k) {
-    D.pending[k] = false;
-    if (D.gen[k] == D.builds)
-      for (uint32_t i = 0; i < ns; ++i) {
-        float d2;
-        std::memcpy(&d2, D.h.p + (size_t)k * ns + i, 4);
-        D.seen[i] = std::max(D.seen[i], d2);
-      }
-  };
-  // an asynchronous evaluation's reading: copied to the host behind an event
-  auto record = [&]() {
-    const int k = D.next;
-    D.next = (D.next + 1) % DispRing::kRing;
-    if (D.pending[k]) {
-      DYN_HIP(hipEventSynchronize(D.ev[k]));
-      consume(k);
-    }
-    DYN_HIP(hipMemcpyAsync(D.h.p + (size_t)k * ns, D.bits.p, 4 * (size_t)ns,
-                           hipMemcpyDeviceToHost, R.stream));
-    DYN_HIP(hipEventRecord(D.ev[k], R.stream));
-    D.pending[k] = true;
-    D.gen[k] = D.builds;
-    D.eval_of[k] = D.issued++;
-  };
-  // bits of (buffer/2)²: the device flags an evaluation whose sites moved
-  // that far since the list was built (kStDisp), and the call repeats
-  auto disp_limit = [&](const dyn_engine& e) {
-    const float half = 0.5f * e.prm.list_buffer;
-    const float h2 = half * half;
-    unsigned int b;
-    std::memcpy(&b, &h2, 4);
-    return b;
-  };
-  if (!rebuild && async) {
-    // Asynchronous computes do not wait for their own displacement: they
-    // decide from the readings of earlier computes on these lists, at most
-    // kLag evaluations old (the host waits for those), and rebuild when a
-    // site has moved D.factor × buffer (0.25: half of the half buffer), which
-    // leaves the rest of the half buffer for the motion since (the fastest
-    // of many light sites move ~0.1 Å per 2 fs step).  Should a site still
-    // cover the half buffer first, the device flags the evaluation
-    // (kStDisp) and the call repeats.
-    for (int k = 0; k < DispRing::kRing; ++k)
-      if (D.pending[k] && D.eval_of[k] + DispRing::kLag <= D.issued) {
-        DYN_HIP(hipEventSynchronize(D.ev[k]));
-        consume(k);
-      }
-    for (int k = 0; k < DispRing::kRing; ++k)
-      if (D.pending[k] && hipEventQuery(D.ev[k]) == hipSuccess) consume(k);
-    for (uint32_t i = 0; i < ns; ++i) {
-      const float lag_half = D.factor * E[i]->prm.list_buffer;
-      if (!(D.seen[i] < lag_half * lag_half)) rebuild = true;
-    }
+    gate[i] = build_gate(*e, (flags & DYN_COMPUTE_REBUILD) || !e->list_valid);
   }
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
@@ -1808,9 +1702,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     any_b = any_b || pl.recip_b;
     any_corr = any_corr || pl.corr_on;
   }
-  // steady state (list reuse with the lagged check): one preparation launch
-  const bool fused = !rebuild && async;
-  if (fused) {
+  {
     Profiler::Scope ps(prof, "prep");
     const unsigned gp = max_grid(S, [](dyn_engine& e) {
       return std::max(grid_for(e.n_sites, 256), grid_for(e.cap_slots, 256));
@@ -1825,7 +1717,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       A.g = e.geom;
       A.wpos = e.wpos.p;
       A.wpos_ref = e.wpos_ref.p;
-      A.disp_bits = D.bits.p + i;
+      A.disp_bits = e.dflags.p + 0;
       A.slot_site = e.slot_site.p;
       A.n_cl = e.counters.p + 0;
       A.xyzq = e.xyzq.p;
@@ -1844,71 +1736,49 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
         A.bin = e.pme.bin_args(e.status.p, kStPme);
         A.bin_on = 1;
       }
-      A.status = e.status.p;
-      A.disp_limit = disp_limit(e);
       return args<k_eng_prep>(A);
     });
-    record();
   }
-  if (!rebuild && !fused) {
-    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_wrap_disp>(R, "k_eng_wrap_disp", 0, gs, 256, 0, [&](uint32_t i) {
+  // the second stream: SPME (spread → FFT → convolution → inverse FFT →
+  // gather) and the caller's side work, overlapping the list build, the
+  // pair kernel and the corrections on the main stream
+  stream_order(R.stream, RS.stream, S.ev_fork);
+  if (any_recip) {
+    std::vector<PmeRun> ra(ns), rb(ns);
+    for (uint32_t i = 0; i < ns; ++i) {
       dyn_engine& e = *E[i];
-      // a synchronous evaluation reads its own displacement and rebuilds
-      // in time: no flag
-      return args<k_eng_wrap_disp>(e.pos_dev, e.q.p, e.n_sites, e.geom, e.wpos.p,
-                                   (const float4*)e.wpos_ref.p, D.bits.p + i, e.status.p,
-                                   async ? disp_limit(e) : 0xFFFFFFFFu);
-    });
-    if (async) {
-      record();
-    } else {
-      DYN_HIP(hipMemcpyAsync(D.h_sync.p, D.bits.p, 4 * (size_t)ns, hipMemcpyDeviceToHost,
-                             R.stream));
-      DYN_HIP(hipStreamSynchronize(R.stream));
-      for (uint32_t i = 0; i < ns; ++i) {
-        float d2;
-        std::memcpy(&d2, D.h_sync.p + i, 4);
-        const float half = 0.5f * E[i]->prm.list_buffer;
-        if (!(d2 < half * half)) rebuild = true;
+      const Plan& pl = PL[i];
+      if (!pl.recip_on) continue;
+      const float s = pl.recip_b ? pl.P.sch.coulomb_scale : 1.0f;
+      PmeRun r;
+      r.core = &e.pme;
+      r.n = e.n_q;
+      r.xyzq = e.pme_xyzq.p;
+      r.site = e.qsite.p;
+      r.fx = e.fx_recip.p;
+      r.comp_major = 3;  // component-major, one charge per site
+      r.nsites = e.n_sites;
+      r.fscale = pl.recip_b ? 0.5f * (1.0f + s) : 1.0f;
+      r.res = e.eng_out.p + 16;
+      r.status = e.status.p;
+      r.status_bit = kStPme;
+      r.binned = true;  // the preparation launch has listed the charges
+      ra[i] = r;
+      r.binned = false;
+      if (pl.recip_b) {
+        r.xyzq = e.pme_b.p;
+        r.fscale = 0.5f * (1.0f - s);
+        r.res = e.eng_out.p + 18;
+        rb[i] = r;
       }
     }
+    pme_run(RS, prof, ra.data(), 0);
+    if (any_b) pme_run(RS, prof, rb.data(), 1);
   }
-  if (rebuild) set_build_lists(S);
-  if (fused) {
-    // done by the preparation launch
-  } else if (rebuild) {
-    Profiler::Scope ps(prof, "site_prep");
-    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_site_prep>(R, "k_eng_site_prep", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      const Plan& pl = PL[i];
-      return args<k_eng_site_prep>(e.wpos.p, e.n_sites, e.qsite.p, e.n_q,
-                                   pl.recip_on ? e.pme_xyzq.p : (float4*)nullptr,
-                                   pl.recip_b ? e.pme_b.p : (float4*)nullptr, e.alch0, e.alch1,
-                                   e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr,
-                                   energy_on && e.n_mol ? e.mol_e.p : (int64_t*)nullptr,
-                                   e.n_mol * e.n_mol);
-    });
-  } else {
-    // list reuse: the slots' positions and the site set-up in one launch
-    Profiler::Scope ps(prof, "slots_prep");
-    const unsigned gs = max_grid(S, [](dyn_engine& e) {
-      return std::max(grid_for(e.n_sites, 256), grid_for(e.cap_slots, 256));
-    });
-    launch<k_eng_slots_prep>(R, "k_eng_slots_prep", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      const Plan& pl = PL[i];
-      return args<k_eng_slots_prep>(e.slot_site.p, e.counters.p + 0, e.xyzq.p, e.wpos.p,
-                                    e.n_sites, e.qsite.p, e.n_q,
-                                    pl.recip_on ? e.pme_xyzq.p : (float4*)nullptr,
-                                    pl.recip_b ? e.pme_b.p : (float4*)nullptr, e.alch0, e.alch1,
-                                    e.fx_site.p, pl.recip_on ? e.fx_recip.p : (int64_t*)nullptr,
-                                    energy_on && e.n_mol ? e.mol_e.p : (int64_t*)nullptr,
-                                    e.n_mol * e.n_mol);
-    });
-  }
-  for (dyn_engine* e : E) {
+  if (side_work) side_work(RS);
+  set_build_lists(S, gate);
+  for (uint32_t i = 0; i < ns; ++i) {
+    dyn_engine* e = E[i];
     if (!e->dbg_count) continue;
     if (e->dbg_skip) {
       --e->dbg_skip;
@@ -1994,38 +1864,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
       return args<k_eng_corr>(C);
     });
   }
-  if (any_recip) {
-    std::vector<PmeRun> ra(ns), rb(ns);
-    for (uint32_t i = 0; i < ns; ++i) {
-      dyn_engine& e = *E[i];
-      const Plan& pl = PL[i];
-      if (!pl.recip_on) continue;
-      const float s = pl.recip_b ? pl.P.sch.coulomb_scale : 1.0f;
-      PmeRun r;
-      r.core = &e.pme;
-      r.n = e.n_q;
-      r.xyzq = e.pme_xyzq.p;
-      r.site = e.qsite.p;
-      r.fx = e.fx_recip.p;
-      r.comp_major = 3;  // component-major, one charge per site
-      r.nsites = e.n_sites;
-      r.fscale = pl.recip_b ? 0.5f * (1.0f + s) : 1.0f;
-      r.res = e.eng_out.p + 16;
-      r.status = e.status.p;
-      r.status_bit = kStPme;
-      r.binned = fused;  // the preparation launch has listed the charges
-      ra[i] = r;
-      r.binned = false;
-      if (pl.recip_b) {
-        r.xyzq = e.pme_b.p;
-        r.fscale = 0.5f * (1.0f - s);
-        r.res = e.eng_out.p + 18;
-        rb[i] = r;
-      }
-    }
-    pme_run(R, prof, ra.data(), 0);
-    if (any_b) pme_run(R, prof, rb.data(), 1);
-  }
+  stream_order(RS.stream, R.stream, S.ev_join);
   {
     Profiler::Scope ps(prof, "finalize");
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
@@ -2036,7 +1875,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
                                   (pl.recip_on || pl.reuse) ? e.fx_recip.p : (const int64_t*)nullptr,
                                   short_on ? e.fitem.p : (const int64_t*)nullptr, e.chunk_ptr.p,
                                   e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p,
-                                  D.bits.p + i);
+                                  e.dflags.p + 0);
     });
     if (short_on && energy_on)
       launch<k_eng_energy_reduce1, kRed1Block>(R, "k_eng_energy_reduce1", 0, kRed1Grid, kRed1Block,
@@ -2061,6 +1900,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags) {
     if (PL[i].recip_on) e.recip_kept = true;
     else if (!PL[i].reuse) e.recip_kept = false;
     e.computed = true;
+    e.list_valid = true;  // every later evaluation keeps it valid on the device
     e.energies_valid = energy_on;
     e.mol_valid = energy_on && e.n_mol;
     e.fx_fresh = false;
@@ -2088,6 +1928,7 @@ void engine_set_check(EngineSet& S) {
 void dyn_engine::check_status() {
   DYN_HIP(hipMemcpyAsync(h_misc.p, status.p, 4, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 20, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 6, dflags.p + 1, 4, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipStreamSynchronize(stream));
   const uint32_t st = h_misc.p[0];
   if (!st) return;
@@ -2106,9 +1947,6 @@ void dyn_engine::check_status() {
   if (st & kStMask)
     throw Error(DYN_ERR_LIMIT, fmt("a solute cluster has more than %d solute neighbours "
                                    "(reduce list_buffer)", kMaskCap));
-  if ((st & ~kStPme) == kStDisp)
-    throw Error(DYN_ERR_CAPACITY, "a site moved half the list buffer before the list was "
-                                  "rebuilt; the list is rebuilt at the next call, call again");
   if (st & kStRow) {
     const uint32_t longest = h_misc.p[5];
     DYN_HIP(hipMemsetAsync(counters.p + 4, 0, 4, stream));
@@ -2160,6 +1998,9 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DeviceGuard g(device);
     DYN_HIP(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
     E->own_stream = E->stream;
+    DYN_HIP(hipStreamCreateWithFlags(&E->side_stream, hipStreamNonBlocking));
+    DYN_HIP(hipEventCreateWithFlags(&E->ev_fork, hipEventDisableTiming));
+    DYN_HIP(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
     E->prof.stream = E->stream;
     E->prm = p;
     E->n_dyn = n_dyn;
@@ -2307,13 +2148,19 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->h_misc.reserve(8);
     E->counters.reserve(8);
     E->status.reserve(1);
-    E->own_disp.init(1);
+    E->dflags.reserve(4);
+    // checkpoint copies of an MD call (allocated here: a step call allocates
+    // nothing, so captured graphs of its steps stay current)
+    E->fx_out_ck.reserve(3 * (size_t)ns);
+    E->fx_recip_ck.reserve(3 * (size_t)ns);
+    E->eng_ck.reserve(kEngOut);
     E->pme_xyzq.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_b.reserve(std::max<uint32_t>(E->n_q, 1));
     E->pme_on = (p.grid[0] > 0 && p.grid[1] > 0 && p.grid[2] > 0) || p.mesh_spacing > 0.f;
     DYN_HIP(hipMemsetAsync(E->counters.p, 0, 32, E->stream));
     DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
     DYN_HIP(hipMemsetAsync(E->status.p, 0, 4, E->stream));
+    DYN_HIP(hipMemsetAsync(E->dflags.p, 0, 16, E->stream));
     DYN_HIP(hipStreamSynchronize(E->stream));
     *out = E.release();
     return DYN_OK;
@@ -2349,6 +2196,7 @@ extern "C" const char* dyn_engine_last_error(const dyn_engine* eng) {
 extern "C" int dyn_engine_set_box(dyn_engine* eng, const float box[3]) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
+    g_layout_gen.fetch_add(1);  // captured graphs hold the old arguments
     DeviceGuard g(eng->device);
     DYN_REQUIRE(box, "box is null");
     for (int a = 0; a < 3; ++a) {
@@ -2407,6 +2255,7 @@ extern "C" int dyn_engine_set_positions(dyn_engine* eng, const float* pos_dyn,
 extern "C" int dyn_engine_set_positions_device(dyn_engine* eng, const float* xyzw_device) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
+    g_layout_gen.fetch_add(1);  // captured graphs hold the old arguments
     DYN_REQUIRE(xyzw_device, "null device pointer");
     DYN_REQUIRE(((uintptr_t)xyzw_device & 15) == 0, "device positions must be 16-byte aligned");
     eng->pos_dev = reinterpret_cast<const float4*>(xyzw_device);
@@ -2426,6 +2275,7 @@ extern "C" int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32
                                          double lambda) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
+    g_layout_gen.fetch_add(1);  // captured graphs hold the old arguments
     DeviceGuard g(eng->device);
     DYN_REQUIRE(std::isfinite(lambda), "lambda must be finite");
     DYN_REQUIRE((uint64_t)first + count <= eng->n_dyn, "alchemical atoms [%u, %u) beyond the %u "
@@ -2455,6 +2305,7 @@ extern "C" int dyn_engine_set_molecules(dyn_engine* eng, uint32_t n_mol,
                                         const uint32_t* mol_start) {
   ENG_CHECK(eng);
   DYN_API_BODY(eng, {
+    g_layout_gen.fetch_add(1);  // captured graphs hold the old arguments
     DeviceGuard g(eng->device);
     DYN_REQUIRE(n_mol <= DYN_MAX_MOLECULES, "%u molecules (at most %u)", n_mol,
                 (unsigned)DYN_MAX_MOLECULES);
@@ -2632,7 +2483,7 @@ extern "C" int dyn_engine_get_energies(dyn_engine* eng, dyn_energies* out) {
     E.pair_count = E.pair_count_kind[0] + E.pair_count_kind[1] + E.pair_count_kind[2] +
                    (uint64_t)std::llround(v[15]);
     E.list_entries = eng->h_misc.p[2];
-    E.list_builds = eng->n_builds;
+    E.list_builds = eng->h_misc.p[6];
     *out = E;
   });
 }
@@ -2705,8 +2556,13 @@ extern "C" int dyn_engine_profile(dyn_engine* eng, int enable) {
   DYN_API_BODY(eng, {
     DeviceGuard g(eng->device);
     eng->prof.reset();
-    eng->prof.enabled = enable != 0;
-    eng->prof.only = enable == 2 ? "pairs" : nullptr;
+    const bool on = enable != 0;
+    const char* only = enable == 2 ? "pairs" : nullptr;
+    // captured graphs hold the timing events of their mode: a new mode makes
+    // new graphs (the same mode again only clears the totals)
+    if (on != eng->prof.enabled || only != eng->prof.only) g_layout_gen.fetch_add(1);
+    eng->prof.enabled = on;
+    eng->prof.only = only;
   });
 }
 
@@ -2734,7 +2590,12 @@ extern "C" int dyn_engine_profile_read(dyn_engine* eng, uint32_t max, const char
 namespace dyn {
 Profiler* engine_profiler(dyn_engine* eng) { return &eng->prof; }
 hipStream_t engine_stream(const dyn_engine* eng) { return eng->stream; }
+const unsigned int* engine_status_device(const dyn_engine* eng) { return eng->status.p; }
+bool engine_graph_safe(const dyn_engine* eng) {
+  return !eng->dbg_count && (!eng->prof.enabled || eng->prof.only);
+}
 void engine_set_stream(dyn_engine* eng, hipStream_t s) {
+  g_layout_gen.fetch_add(1);
   eng->stream = s ? s : eng->own_stream;
   eng->prof.stream = eng->stream;
   eng->pme.stream = eng->stream;

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -817,6 +818,10 @@ struct StepArgs {
   int64_t* fb;            // bonded force sums to clear for the evaluation
   const StarDesc* stars;  // per solute pressure group
   uint32_t water0;        // thread of the first water (n_sgroups rounded up to a wave)
+  // the engine's status word: after a failed evaluation (a list or SPME
+  // buffer overflowed) the remaining steps of the call leave the state as it
+  // is, and the call is repeated from its start
+  const unsigned int* status;
 };
 
 // Before the force evaluation.  Langevin-middle (src/integrate.rs:100-160):
@@ -953,7 +958,7 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
                                             float4* __restrict__ v,
                                             const uint64_t* __restrict__ step_dev) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= A.water0 + A.G.n_water) return;
+  if (*A.status || g >= A.water0 + A.G.n_water) return;
   step_group<false>(A, x, v, g, step_dev);
 }
 
@@ -966,6 +971,7 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
 __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
                                              float4* __restrict__ v,
                                              uint64_t* __restrict__ step_dev) {
+  if (*A.status) return;  // every workgroup reads the same word: none counts itself
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < A.water0 + A.G.n_water) step_group<true>(A, x, v, g, step_dev);
   __syncthreads();
@@ -984,6 +990,7 @@ __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
 __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
                                             float4* __restrict__ v,
                                             uint64_t* __restrict__ step_inc) {
+  if (*A.status) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g == 0) ++*step_inc;
   if (g >= A.water0 + A.G.n_water) return;
@@ -1022,6 +1029,38 @@ __device__ __forceinline__ void k_md_step_b(StepArgs A, float4* __restrict__ x,
 static void engine_call(int rc, dyn_engine* eng) {
   if (rc != DYN_OK) throw Error(rc, fmt("engine: %s", dyn_engine_last_error(eng)));
 }
+
+namespace dyn {
+// Graphs of runs of steady MD steps (begun by the previous step's last
+// launch, no energies, joined to the next step), one per SPME cadence phase
+// of the members.  A graph holds the kernel arguments and buffer addresses
+// of its capture, so it is used only while the device layout generation is
+// the one it was captured at; a run is captured the second time it is met
+// at one generation (the first, direct run has uploaded every argument pack
+// and sized every buffer).
+struct StepGraphs {
+  struct Entry {
+    hipGraphExec_t exec = nullptr;
+    uint64_t gen = 0;       // layout generation of exec
+    uint64_t seen_gen = 0;  // generation of the last direct run
+    std::vector<Profiler::Rec> recs;  // timed scopes recorded into the graph
+  };
+  std::map<std::vector<uint32_t>, Entry> cache;
+  bool disabled = false;  // a capture was refused: direct launches from then on
+  uint64_t launches = 0;  // graph launches so far
+  void drop(Entry& e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    e.exec = nullptr;
+    e.recs.clear();
+  }
+  StepGraphs() = default;
+  StepGraphs(const StepGraphs&) = delete;
+  StepGraphs& operator=(const StepGraphs&) = delete;
+  ~StepGraphs() {
+    for (auto& kv : cache) drop(kv.second);
+  }
+};
+}  // namespace dyn
 
 struct dyn_md {
   std::string err;
@@ -1064,10 +1103,12 @@ struct dyn_md {
   PinnedBuf<double> h_ppart, h_bpart;
   double pressure = std::nan(""), virial = std::nan(""), kin_com = std::nan("");
   uint64_t n_pcouple = 0;  // barostat rescalings so far
-  uint64_t n_repeats = 0;  // step calls run again from their start (list overflow, stale list)
+  uint64_t n_repeats = 0;  // step calls run again from their start (a buffer outgrew its capacity)
+  uint64_t graph_steps = 0;  // steps run inside graph launches
   double mass_total = 0.0;  // amu: solute atoms and the waters' O and H
   bool eng_mode_reference = true;
-  ArgCache own_cache;   // a pass over this MD alone
+  ArgCache own_cache, side_cache;  // a pass over this MD alone (main, second stream)
+  StepGraphs graphs;
   bool in_batch = false;
 };
 
@@ -1077,9 +1118,10 @@ namespace dyn {
 struct MdSet {
   std::vector<dyn_md*> m;
   EngineSet es;
-  BatchRun R;  // the MD kernels' launches (own argument packs)
+  BatchRun R;     // the MD kernels' launches (own argument packs)
+  BatchRun side;  // MD work on the engines' second stream (bonded forces)
   Profiler* prof = nullptr;
-  bool strict = false;  // every evaluation waits for its own displacement reading
+  StepGraphs* graphs = nullptr;
 };
 
 }  // namespace dyn
@@ -1133,13 +1175,12 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
     any = any || (m->n_bond + m->n_angle + m->n_dihe) > 0;
     gb = std::max(gb, m->bblocks);
   }
-  if (first)
-    engine_set_compute_sync(S.es, flags);
-  else
-    engine_set_enqueue(S.es, flags | (S.strict ? 0u : DYN_COMPUTE_ASYNC));
-  if (any) {
+  // bonded forces on the engines' second stream, beside the pair kernel
+  auto bonded = [&](const BatchRun& RS) {
+    if (!any) return;
+    const BatchRun& RB = RS.stream == S.side.stream ? S.side : S.R;
     Profiler::Scope ps(*S.prof, "md_bonded");
-    launch<k_md_bonded, kBlock>(S.R, "k_md_bonded", 0, gb, kBlock, 0, [&](uint32_t i) {
+    launch<k_md_bonded, kBlock>(RB, "k_md_bonded", 0, gb, kBlock, 0, [&](uint32_t i) {
       dyn_md& m = *S.m[i];
       BondedArgs A;
       std::memset(&A, 0, sizeof A);
@@ -1160,6 +1201,12 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
       A.nblk = m.bblocks;
       return args<k_md_bonded>(A);
     });
+  };
+  if (first) {
+    engine_set_compute_sync(S.es, flags);
+    bonded(S.R);
+  } else {
+    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC, bonded);
   }
   for (dyn_md* m : S.m) m->forces_valid = true;
 }
@@ -1283,6 +1330,7 @@ StepArgs step_args(const dyn_md& m) {
   A.fb = m.fb.p;
   A.stars = m.stars.p;
   A.water0 = step_water0(m);
+  A.status = engine_status_device(m.eng);
   return A;
 }
 
@@ -1433,34 +1481,133 @@ void md_restore(MdSet& S, const std::vector<MdCk>& ck) {
 }
 }  // namespace
 
+// Steady steps of a call from step k on: steps the previous step's last
+// launch has begun, that are not the call's last and couple no pressure.
+uint32_t steady_steps(const MdSet& S, uint32_t k, uint32_t n, bool begun) {
+  if (!begun) return 0;
+  const uint32_t nst = S.m[0]->prm.nstpcouple;
+  uint32_t j = k;
+  for (; j + 1 < n; ++j) {
+    const uint64_t step = S.m[0]->step + (j - k);
+    if (nst > 0 && (step + 1) % nst == 0) break;
+  }
+  return j - k;
+}
+
+constexpr uint32_t kGraphSteps = 16;  // steps per graph (a multiple of every SPME ratio in use)
+
+bool graphs_usable(const MdSet& S) {
+  // a profile of every kernel runs them one by one; timing one kernel (the
+  // pair kernel) records its events into the graphs
+  if (!S.graphs || S.graphs->disabled || (S.prof->enabled && !S.prof->only)) return false;
+  for (dyn_md* m : S.m)
+    if (!engine_graph_safe(m->eng) || !m->forces_valid) return false;
+  return true;
+}
+
+// kGraphSteps steady steps of every MD of the set: as one graph launch when
+// a graph of this run is current, else directly (and captured the second
+// time the run is met at the same layout).
+void md_graph_block(MdSet& S) {
+  StepGraphs& C = *S.graphs;
+  std::vector<uint32_t> key;
+  for (dyn_md* m : S.m)
+    key.push_back(m->prm.spme_ratio > 1 ? (uint32_t)(m->n_eval % m->prm.spme_ratio) : 0u);
+  StepGraphs::Entry& E = C.cache[key];
+  const uint64_t gen = g_layout_gen.load();
+  auto steps = [&]() {
+    for (uint32_t k = 0; k < kGraphSteps; ++k) md_one_step(S, false, true, true);
+  };
+  if (E.exec && E.gen == gen) {
+    // the graph enqueues the work; the host keeps its bookkeeping
+    g_launch_mode = kLaunchDry;
+    try {
+      steps();
+    } catch (...) {
+      g_launch_mode = kLaunchDirect;
+      throw;
+    }
+    g_launch_mode = kLaunchDirect;
+    DYN_HIP(hipGraphLaunch(E.exec, S.R.stream));
+    ++C.launches;
+    for (dyn_md* m : S.m) m->graph_steps += kGraphSteps;
+    for (const auto& r : E.recs) S.prof->pending.push_back(r);
+    return;
+  }
+  C.drop(E);
+  if (E.seen_gen != gen) {
+    // the direct run may itself upload argument packs (new device copies):
+    // the layout it leaves is the one a capture would find
+    steps();
+    E.seen_gen = g_layout_gen.load();
+    return;
+  }
+  DYN_HIP(hipStreamBeginCapture(S.R.stream, hipStreamCaptureModeThreadLocal));
+  g_launch_mode = kLaunchCapture;
+  S.prof->capture_sink = &E.recs;
+  hipGraph_t graph = nullptr;
+  try {
+    steps();
+  } catch (...) {
+    S.prof->capture_sink = nullptr;
+    g_launch_mode = kLaunchDirect;
+    (void)hipStreamEndCapture(S.R.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    C.disabled = true;
+    throw CaptureRefused("the steps could not be recorded into a graph");
+  }
+  S.prof->capture_sink = nullptr;
+  g_launch_mode = kLaunchDirect;
+  DYN_HIP(hipStreamEndCapture(S.R.stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess || g_layout_gen.load() != gen) {
+    // nothing of the recorded steps has run: the call repeats without graphs
+    if (exec) (void)hipGraphExecDestroy(exec);
+    C.disabled = true;
+    throw CaptureRefused("a graph of the steps could not be made");
+  }
+  E.exec = exec;
+  E.gen = gen;
+  DYN_HIP(hipGraphLaunch(E.exec, S.R.stream));
+  ++C.launches;
+  for (dyn_md* m : S.m) m->graph_steps += kGraphSteps;
+  for (const auto& r : E.recs) S.prof->pending.push_back(r);
+}
+
 void md_run(MdSet& S, uint32_t n) {
   if (!n) return;
   std::vector<MdCk> ck;
   md_checkpoint(S, ck);
   for (int attempt = 0;; ++attempt) {
-    // a repeat (after a buffer overflow or a stale list) rebuilds at a lower
-    // displacement; a second one checks every evaluation's displacement
-    // before its pair kernel
-    S.es.disp->factor = attempt == 1 ? 0.15f : 0.25f;
-    S.strict = attempt > 1;
     try {
       const auto t0 = std::chrono::steady_clock::now();
       // a barostat step waits mid-call, and may find an overflow there
       bool begun = false;
-      for (uint32_t k = 0; k < n; ++k) begun = md_one_step(S, k + 1 == n, begun, k + 1 < n);
+      uint32_t k = 0;
+      while (k < n) {
+        if (steady_steps(S, k, n, begun) >= kGraphSteps && graphs_usable(S)) {
+          md_graph_block(S);
+          k += kGraphSteps;
+          continue;
+        }
+        begun = md_one_step(S, k + 1 == n, begun, k + 1 < n);
+        ++k;
+      }
       const auto t1 = std::chrono::steady_clock::now();
       engine_set_check(S.es);
       const auto t2 = std::chrono::steady_clock::now();
       S.prof->add_host("md_issue_steps", std::chrono::duration<double, std::milli>(t1 - t0).count());
       S.prof->add_host("md_wait", std::chrono::duration<double, std::milli>(t2 - t1).count());
-      S.es.disp->factor = 0.25f;
-      S.strict = false;
       for (dyn_md* m : S.m) m->n_repeats += (uint64_t)attempt;
       return;
+    } catch (const CaptureRefused&) {
+      // graphs are off for this set from now on; the call runs again
     } catch (const Error& e) {
+      // a buffer that overflowed has been enlarged: the call runs again (twice
+      // at most); any other failure ends it
       if (e.code != DYN_ERR_CAPACITY || attempt >= 2) {
-        S.es.disp->factor = 0.25f;
-        S.strict = false;
         try {
           md_restore(S, ck);
         } catch (const Error&) {
@@ -1482,6 +1629,10 @@ MdSet self_set(dyn_md* m) {
   S.R.stream = m->stream;
   S.R.cache = &m->own_cache;
   S.R.n = 1;
+  S.side.stream = S.es.side.stream;
+  S.side.cache = &m->side_cache;
+  S.side.n = 1;
+  S.graphs = &m->graphs;
   return S;
 }
 
@@ -1878,6 +2029,7 @@ extern "C" int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out) {
     E.potential = E.potential_nonbonded + E.potential_bonded;
     E.density = md->mass_total / E.volume;
     E.repeats = md->n_repeats;
+    E.graph_steps = md->graph_steps;
     *out = E;
   });
 }
@@ -1890,23 +2042,29 @@ struct dyn_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   std::vector<dyn_md*> m;
-  ArgCache eng_cache, md_cache;
-  DispRing disp;
+  hipStream_t side_stream = nullptr;  // second stream (SPME, bonded), same priority
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ArgCache eng_cache, md_cache, eng_side_cache, md_side_cache;
   Profiler prof;
+  StepGraphs graphs;
 
   MdSet set() {
     MdSet S;
+    const uint32_t n = (uint32_t)m.size();
     S.m = m;
     for (dyn_md* x : m) S.es.e.push_back(x->eng);
-    S.es.R.stream = stream;
-    S.es.R.cache = &eng_cache;
-    S.es.R.n = (uint32_t)m.size();
+    S.es.R = BatchRun{stream, &eng_cache, n};
     S.es.prof = &prof;
-    S.es.disp = &disp;
-    S.R.stream = stream;
-    S.R.cache = &md_cache;
-    S.R.n = (uint32_t)m.size();
+    S.R = BatchRun{stream, &md_cache, n};
+    // fully profiled steps stay on one stream (their per-kernel times)
+    if (!prof.enabled || prof.only) {
+      S.es.side = BatchRun{side_stream, &eng_side_cache, n};
+      S.es.ev_fork = ev_fork;
+      S.es.ev_join = ev_join;
+      S.side = BatchRun{side_stream, &md_side_cache, n};
+    }
     S.prof = &prof;
+    S.graphs = &graphs;
     return S;
   }
   // hand the members back to their own streams
@@ -1918,8 +2076,13 @@ struct dyn_batch {
       x->in_batch = false;
     }
     m.clear();
+    if (side_stream) (void)hipStreamSynchronize(side_stream);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side_stream) (void)hipStreamDestroy(side_stream);
     if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
+    stream = side_stream = nullptr;
+    ev_fork = ev_join = nullptr;
   }
 };
 
@@ -1949,8 +2112,10 @@ extern "C" int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds)
     DeviceGuard g(B->device);
     for (uint32_t i = 0; i < n; ++i) DYN_HIP(hipStreamSynchronize(mds[i]->stream));
     DYN_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    DYN_HIP(hipStreamCreateWithFlags(&B->side_stream, hipStreamNonBlocking));
+    DYN_HIP(hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming));
+    DYN_HIP(hipEventCreateWithFlags(&B->ev_join, hipEventDisableTiming));
     B->prof.stream = B->stream;
-    B->disp.init(n);
     for (uint32_t i = 0; i < n; ++i) {
       dyn_md* x = mds[i];
       engine_set_stream(x->eng, B->stream);
@@ -2003,8 +2168,11 @@ extern "C" int dyn_batch_profile(dyn_batch* b, int enable) {
   DYN_API_BODY(b, {
     DeviceGuard g(b->device);
     b->prof.reset();
-    b->prof.enabled = enable != 0;
-    b->prof.only = enable == 2 ? "pairs" : nullptr;
+    const bool on = enable != 0;
+    const char* only = enable == 2 ? "pairs" : nullptr;
+    if (on != b->prof.enabled || only != b->prof.only) g_layout_gen.fetch_add(1);
+    b->prof.enabled = on;
+    b->prof.only = only;
   });
 }
 

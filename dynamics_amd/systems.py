@@ -575,3 +575,14 @@ def initial_velocities(system: System, ff: ForceField, temperature: float = 300.
 def water_positions(system: System) -> np.ndarray:
     """(n_water, 4, 3) positions in O, M, H0, H1 order."""
     return np.stack([system.pos_o, system.pos_m, system.pos_h0, system.pos_h1], axis=1).astype(F32)
+
+
+def with_positions(system: System, pos_dyn, pos_water) -> System:
+    """A copy of the system at other positions: solute (n_dyn, 3) and water
+    (n_water, 4, 3) in O, M, H0, H1 order (as an MD state holds them)."""
+    pw = np.asarray(pos_water, F32).reshape(-1, 4, 3)
+    return dataclasses.replace(system, pos_dyn=np.ascontiguousarray(pos_dyn, F32).reshape(-1, 3),
+                               pos_o=np.ascontiguousarray(pw[:, 0]),
+                               pos_m=np.ascontiguousarray(pw[:, 1]),
+                               pos_h0=np.ascontiguousarray(pw[:, 2]),
+                               pos_h1=np.ascontiguousarray(pw[:, 3]))

@@ -471,8 +471,9 @@ typedef struct dyn_md_energies {
   double potential;
   double dh_dl;
   double density;
-  uint64_t repeats;     /* step calls run again from their start (list
-                           overflow or stale list) since create */
+  uint64_t repeats;     /* step calls run again from their start (a list or
+                           SPME buffer outgrew its capacity) since create */
+  uint64_t graph_steps; /* steps run inside launches of captured graphs */
 } dyn_md_energies;
 
 int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);

@@ -357,11 +357,29 @@ struct CpuRefStats {
   int steps;
 };
 
+// Evaluations over a trajectory at the reference's cadence: frame k (sites
+// ordered solute, then per water O, M, H0, H1; 3 floats each) replaces the
+// positions of step k.  The neighbor lists and the flattened pair list are
+// rebuilt only when the largest displacement since the last build, of a
+// solute atom (plain difference) or a water O (minimum image of the wrapped
+// position), reaches half the skin (src/neighbors.rs:56-83, :159-169); SPME
+// runs on steps divisible by spme_ratio and its forces, energy and virial
+// are reused in between (src/lib.rs:1282-1341; with spme_ratio 1 every
+// step).  With frames == NULL every step evaluates the input positions and
+// rebuilds (the cost of one full evaluation).  rebuilds_out counts builds.
+int cpuref_run_traj(const Input* in, int steps, const float* frames, int spme_ratio, int threads,
+                    CpuRefStats* st, double* forces_out, int* rebuilds_out);
+
 // Runs `steps` evaluations (neighbors + setup_pairs + short range + SPME each)
 // and fills per-component wall times summed over the steps.  forces_out
 // (optional) receives the last step's forces, sites ordered solute, then per
 // water O, M, H0, H1.
 int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double* forces_out) {
+  return cpuref_run_traj(in, steps, nullptr, 1, threads, st, forces_out, nullptr);
+}
+
+int cpuref_run_traj(const Input* in, int steps, const float* frames, int spme_ratio, int threads,
+                    CpuRefStats* st, double* forces_out, int* rebuilds_out) {
   using clk = std::chrono::steady_clock;
   auto us = [](clk::time_point a, clk::time_point b) {
     return std::chrono::duration<double, std::micro>(b - a).count();
@@ -392,8 +410,20 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
   const float thr = rl * rl;
   const float two_b_sqrtpi = 2.0f * in->alpha * 0.5641895835477563f;
   std::vector<double> F(3 * nsites);
+  std::vector<Pair> pairs;
+  std::vector<V3> apos_ref, wpos_ref;  // positions at the last build
+  const float half_skin_sq = (0.5f * in->skin) * (0.5f * in->skin);
+  int rebuilds = 0;
+  std::vector<double> f_rec_kept(3 * nsites, 0.0);
+  double e_rec_kept = 0.0, w_rec_kept = 0.0;
+  double e_rec = 0.0, w_rec = 0.0;
+  const int ratio = std::max(1, spme_ratio);
   for (int step = 0; step < steps; ++step) {
     auto t0 = clk::now();
+    if (frames) {
+      const float* fr = frames + (size_t)step * 3 * nsites;
+      for (size_t k = 0; k < nsites; ++k) pos[k] = {fr[3 * k], fr[3 * k + 1], fr[3 * k + 2]};
+    }
     std::vector<V3> apos(pos.begin(), pos.begin() + ns);
     std::vector<V3> wpos(nw);
     for (uint32_t m = 0; m < nw; ++m) {
@@ -406,12 +436,32 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
       }
       wpos[m] = o;
     }
-    auto nss = build_neighbors(apos, apos, true, in->ext, thr, threads);
-    auto nsw = build_neighbors(apos, wpos, false, in->ext, thr, threads);
-    auto nww = build_neighbors(wpos, wpos, true, in->ext, thr, threads);
-    auto t1 = clk::now();
-    auto pairs = setup_pairs(*in, nss, nsw, nww, ex, s14);
-    auto t2 = clk::now();
+    bool rebuild = !frames || step == 0;
+    if (!rebuild) {
+      float mx = 0.f;
+      for (uint32_t i = 0; i < ns; ++i) {
+        const V3 d{apos[i].x - apos_ref[i].x, apos[i].y - apos_ref[i].y, apos[i].z - apos_ref[i].z};
+        mx = std::max(mx, norm2(d));
+      }
+      for (uint32_t m = 0; m < nw; ++m) {
+        const V3 d = min_image(V3{wpos[m].x - wpos_ref[m].x, wpos[m].y - wpos_ref[m].y,
+                                  wpos[m].z - wpos_ref[m].z}, in->ext);
+        mx = std::max(mx, norm2(d));
+      }
+      rebuild = mx >= half_skin_sq;
+    }
+    auto t1 = t0, t2 = t0;
+    if (rebuild) {
+      auto nss = build_neighbors(apos, apos, true, in->ext, thr, threads);
+      auto nsw = build_neighbors(apos, wpos, false, in->ext, thr, threads);
+      auto nww = build_neighbors(wpos, wpos, true, in->ext, thr, threads);
+      t1 = clk::now();
+      pairs = setup_pairs(*in, nss, nsw, nww, ex, s14);
+      t2 = clk::now();
+      apos_ref = apos;
+      wpos_ref = wpos;
+      ++rebuilds;
+    }
     // calc_force_cpu
     const int T = std::max(1, threads);
     std::vector<std::vector<double>> bufs(T, std::vector<double>(3 * nsites, 0.0));
@@ -486,6 +536,25 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
       nh += hits[t];
     }
     auto t3 = clk::now();
+    if (step % ratio != 0) {
+      // the reference's cached SPME forces, energy and virial
+      for (size_t k = 0; k < F.size(); ++k) F[k] += f_rec_kept[k];
+      e_rec = e_rec_kept;
+      w_rec = w_rec_kept;
+      auto t4 = clk::now();
+      st->us_neighbors += us(t0, t1);
+      st->us_setup_pairs += us(t1, t2);
+      st->us_short += us(t2, t3);
+      st->us_spme += us(t3, t4);
+      st->energy_short = e_sr;
+      st->virial_short = w_sr;
+      st->energy_recip = e_rec;
+      st->virial_recip = w_rec;
+      st->n_list_pairs = pairs.size();
+      st->n_interacting = nh;
+      st->steps = step + 1;
+      continue;
+    }
     // SPME over charged sites, wrapped
     std::vector<V3> qpos;
     std::vector<float> qv;
@@ -504,8 +573,9 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
         qidx.push_back(s);
       }
     std::vector<double> fr;
-    double e_rec, w_rec;
     spme.run(qpos, qv, fr, &e_rec, &w_rec, threads);
+    std::vector<double> before;
+    if (ratio > 1) before = F;
     for (size_t k = 0; k < qidx.size(); ++k)
       for (int a = 0; a < 3; ++a) F[3 * qidx[k] + a] += fr[3 * k + a];
     // 1-4 vacuum correction of handle_spme_recip (src/non_bonded.rs:763-790)
@@ -525,6 +595,12 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
       F[3 * j + 2] -= df.z;
       w_rec += (double)(d.x * df.x + d.y * df.y + d.z * df.z);
     }
+    if (ratio > 1) {
+      // what the reference caches: SPME with its 1-4 correction
+      for (size_t k = 0; k < F.size(); ++k) f_rec_kept[k] = F[k] - before[k];
+      e_rec_kept = e_rec;
+      w_rec_kept = w_rec;
+    }
     auto t4 = clk::now();
     st->us_neighbors += us(t0, t1);
     st->us_setup_pairs += us(t1, t2);
@@ -539,6 +615,7 @@ int cpuref_run(const Input* in, int steps, int threads, CpuRefStats* st, double*
     st->steps = step + 1;
   }
   if (forces_out) memcpy(forces_out, F.data(), sizeof(double) * F.size());
+  if (rebuilds_out) *rebuilds_out = rebuilds;
   return 0;
 }
 

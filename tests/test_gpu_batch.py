@@ -2,8 +2,9 @@
 launch for all members.  With the list rebuilt every step (the same lists
 either way) a batch reproduces each member's own run bit for bit — systems
 of uneven sizes and PME grids, alchemical λ-windows (two SPME evaluations
-each) mixed with plain systems; with a list buffer it stays within f32
-rounding of them."""
+each) mixed with plain systems.  Each member decides on its own list builds (on the
+device), so with a list buffer too a batch reproduces its members' own runs
+bit for bit, also when the steps run as captured graphs."""
 
 import numpy as np
 import pytest
@@ -40,15 +41,23 @@ def _make(s, alch, lam, buffer, seed):
     return eng, m
 
 
-def _run(buffer, steps, batched):
+def _run(buffer, steps, batched, profile=False):
+    """steps: an int, or a sequence of step-call lengths."""
+    calls = [steps] if isinstance(steps, int) else list(steps)
     made = [_make(s, a, lam, buffer, seed=11 + k) for k, (s, a, lam) in enumerate(_members())]
     if batched:
         b = api.Batch([m for _, m in made])
-        b.step(steps)
+        if profile:
+            b.profile(True)
+        for n in calls:
+            b.step(n)
         b.close()
     else:
-        for _, m in made:
-            m.step(steps)
+        for e, m in made:
+            if profile:
+                e.profile(True)
+            for n in calls:
+                m.step(n)
     out = [(m.state(), m.energies(), e.energies()) for e, m in made]
     for e, m in made:
         m.close()
@@ -68,13 +77,30 @@ def test_batch_matches_members_bit_exact(gpu):
             assert ea[key] == eb[key], (k, key, ea[key], eb[key])
 
 
+def _same_states(a, b):
+    for k, ((sa, ma, ea), (sb, mb, eb)) in enumerate(zip(a, b)):
+        for x, y in zip(sa, sb):
+            assert np.array_equal(x, y), f"member {k}: states differ"
+        assert ea["list_builds"] == eb["list_builds"], k
+        assert ea["total"] == eb["total"], k
+
+
 def test_batch_with_list_buffer(gpu):
-    print("batch of 5 with a 1 A list buffer, 20 steps", flush=True)
-    a = _run(1.0, 20, batched=False)
-    b = _run(1.0, 20, batched=True)
-    for k, ((sa, _, _), (sb, _, _)) in enumerate(zip(a, b)):
-        dx = max(np.abs(sa[0] - sb[0]).max(initial=0.0), np.abs(sa[2] - sb[2]).max())
-        assert dx < 1e-3, (k, dx)
+    """With a 1 Å list buffer and three calls of 20 steps (the second records
+    a graph of 16 steady steps, the third replays it), the batch and each
+    member alone take the same list builds and reach the same state bit for
+    bit; so does a profiled run (direct launches on one stream)."""
+    print("batch of 5 with a 1 A list buffer, 3 x 20 steps", flush=True)
+    a = _run(1.0, (20, 20, 20), batched=False)
+    b = _run(1.0, (20, 20, 20), batched=True)
+    _same_states(a, b)
+    assert [m["graph_steps"] for _, m, _ in a] == [32] * len(a)
+    assert [m["graph_steps"] for _, m, _ in b] == [32] * len(b)
+    assert all(e["list_builds"] > 1 for _, _, e in a)
+    print("the same members stepped with direct launches on one stream", flush=True)
+    c = _run(1.0, (20, 20, 20), batched=True, profile=True)
+    _same_states(a, c)
+    assert all(m["graph_steps"] == 0 for _, m, _ in c)
 
 
 def test_batch_errors(gpu):

@@ -1,6 +1,7 @@
 """The repeat path of dyn_md_step, run deterministically through the test
-hook dyn_engine_debug_inject_status (it sets the stale-list status bit as the
-device's own displacement check would):
+hook dyn_engine_debug_inject_status (it sets the status bit of a pair list
+whose work items outgrew their buffer, as the list build itself would; the
+buffers are then enlarged and the call runs again):
 
 * one flagged evaluation inside a call: the call repeats from its start and
   returns OK with the expected step count, one recorded repeat, and a state
@@ -22,7 +23,7 @@ from dynamics_amd import api, systems
 pytestmark = pytest.mark.gpu
 
 BETA = float(np.float32(0.3123))
-STALE = 64  # the status bit of a site that moved half the list buffer
+OVERFLOW = 8  # the status bit of a pair list whose work items outgrew their buffer
 
 
 def _setup(s, ff, pd, pw, v, vw, **kw):
@@ -59,14 +60,14 @@ def test_flagged_evaluation_repeats_call(gpu, start):
     eng_b, m_b = _setup(s, ff, pd, pw, v, vw)
     m_b.step(1)
     assert m_b.energies()["repeats"] == 0
-    print("inject: stale-list flag on the 3rd evaluation of a 6-step call", flush=True)
-    eng_b.debug_inject_status(STALE, skip=2, count=1)
+    print("inject: overflow flag on the 3rd evaluation of a 6-step call", flush=True)
+    eng_b.debug_inject_status(OVERFLOW, skip=2, count=1)
     m_b.step(6)
     E = m_b.energies()
     assert E["steps"] == 7
     assert E["repeats"] == 1
-    # the repeat rebuilt its list at other steps: only list-order rounding differs
-    _state_close(m_a.state(), m_b.state(), 1e-4)
+    # the repeat rebuilt its list at another step: only list-order rounding differs
+    _state_close(m_a.state(), m_b.state(), 2e-3)
     # the final forces are those of the final positions
     pd_b, _, pw_b, _ = m_b.state()
     s2 = dataclasses.replace(s, pos_dyn=pd_b, pos_o=pw_b[:, 0].copy(), pos_m=pw_b[:, 1].copy(),
@@ -88,8 +89,8 @@ def test_give_up_restores_start_state(gpu, start):
     m.step(2)
     before = m.state()
     e_before = m.energies()
-    print("inject: stale-list flag on every evaluation (the call gives up)", flush=True)
-    eng.debug_inject_status(STALE, skip=0, count=1000)
+    print("inject: overflow flag on every evaluation (the call gives up)", flush=True)
+    eng.debug_inject_status(OVERFLOW, skip=0, count=1000)
     with pytest.raises(api.DynError) as ei:
         m.step(4)
     assert ei.value.code == api.DYN_ERR_CAPACITY
@@ -108,7 +109,7 @@ def test_give_up_restores_start_state(gpu, start):
     eng_r, m_r = _setup(s, ff, pd, pw, v, vw)
     m_r.step(2)
     m_r.step(4)
-    _state_close(m.state(), m_r.state(), 1e-4)
+    _state_close(m.state(), m_r.state(), 2e-3)
     assert m.energies()["steps"] == 6
     for h in (m, m_r):
         h.close()
@@ -124,14 +125,14 @@ def test_repeat_across_pressure_coupling(gpu, start):
     m_a.step(5)
     eng_b, m_b = _setup(s, ff, pd, pw, v, vw, **kw)
     m_b.step(1)
-    print("inject: stale-list flag after the first coupling of a 5-step call", flush=True)
-    eng_b.debug_inject_status(STALE, skip=3, count=1)
+    print("inject: overflow flag after the first coupling of a 5-step call", flush=True)
+    eng_b.debug_inject_status(OVERFLOW, skip=3, count=1)
     m_b.step(5)
     Ea, Eb = m_a.energies(), m_b.energies()
     assert Eb["repeats"] == 1
     assert Eb["pcouple_steps"] == Ea["pcouple_steps"] == 3
     np.testing.assert_allclose(Eb["box"], Ea["box"], rtol=1e-6)
-    _state_close(m_a.state(), m_b.state(), 1e-4)
+    _state_close(m_a.state(), m_b.state(), 2e-3)
     for h in (m_a, m_b):
         h.close()
     for e in (eng_a, eng_b):

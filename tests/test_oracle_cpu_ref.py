@@ -71,3 +71,35 @@ def test_cpu_ref_planted_sign_fails(solvated):
     bad[:s.n_dyn] -= 2 * f14
     err = np.linalg.norm(bad - ref["forces"]) / np.linalg.norm(ref["forces"])
     assert err > 1e-4
+
+
+def _frames(s, shifts):
+    pos, _, _, _, _, _ = s.site_arrays()
+    return np.stack([pos.astype(np.float32) + np.float32(d) for d in shifts])
+
+
+def test_cpu_ref_trajectory_cadence(solvated):
+    """The trajectory mode (the timed baseline at the reference's cadence):
+    a list kept across frames that moved less than half the skin gives the
+    forces of a fresh evaluation (to summation order), a move of half the
+    skin rebuilds, and with SPME every second step the second of two equal
+    frames reuses the first one's SPME forces exactly."""
+    s, _, F, _, _ = solvated
+    kw = dict(threads=4, cutoff=RC, lj_cutoff=RC, skin=4.0, alpha=BETA, K=K, order=4)
+    # a uniform shift of every site keeps all pair distances: list kept
+    out = cpu_baseline.run_trajectory(s, _frames(s, [0.0, 0.6, 1.1]), spme_ratio=1, **kw)
+    assert out["rebuilds"] == 1 and out["steps"] == 3  # |(1.1, 1.1, 1.1)| < skin/2
+    out = cpu_baseline.run_trajectory(s, _frames(s, [0.0, 0.6, 1.2]), spme_ratio=1, **kw)
+    assert out["rebuilds"] == 2  # |(1.2, 1.2, 1.2)| ≥ skin/2
+    # forces of the last frame, list reused and SPME kept from the first
+    L = cpu_baseline._lib()
+    inp, keep = cpu_baseline._input(s, RC, RC, 4.0, BETA, K, 4)
+    fr = np.ascontiguousarray(_frames(s, [0.0, 0.0]))
+    st = cpu_baseline.Stats()
+    G = np.zeros_like(F)
+    rb = cpu_baseline.C.c_int()
+    assert L.cpuref_run_traj(cpu_baseline.C.byref(inp), 2, fr.ctypes.data_as(cpu_baseline.C.c_void_p),
+                             2, 4, cpu_baseline.C.byref(st), G.ctypes.data_as(cpu_baseline.C.c_void_p),
+                             cpu_baseline.C.byref(rb)) == 0
+    assert rb.value == 1
+    assert np.linalg.norm(G - F) <= 1e-12 * np.linalg.norm(F)
