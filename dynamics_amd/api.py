@@ -121,6 +121,7 @@ _sig("dyn_nb_compute", C.c_int, [_p, _p, _p, _p, _p, _p, _f32p, C.c_uint8, C.c_u
                                  C.c_double, _p, _p, _p, _p, _p])
 _sig("dyn_nb_get_fixed_forces", C.c_int, [_p, _i64p, C.c_uint64])
 _sig("dyn_nb_correct_14", C.c_int, [_p, C.c_uint32, _p, _p, _p, _p, _f32p, _f64p, _p])
+_sig("dyn_nb_correct_14_noalloc", C.c_int, [_p, C.c_uint32, _p, _p, _p, _p, _f32p, _f64p, _p])
 _sig("dyn_nb_build_neighbors", C.c_int, [_p, _p, C.c_uint32, _p, C.c_uint32, _p, _f32p,
                                          C.c_uint8, C.c_float, _u32p, _p, C.c_uint64,
                                          C.POINTER(C.c_uint64)])
@@ -182,7 +183,8 @@ _sig("dyn_engine_profile_read", C.c_int, [_p, C.c_uint32, C.POINTER(C.c_char_p),
 EXPORTED = [
     "dyn_version", "dyn_device_count", "dyn_global_last_error",
     "dyn_nb_create", "dyn_nb_destroy", "dyn_nb_last_error", "dyn_nb_set_pairs",
-    "dyn_nb_compute", "dyn_nb_get_fixed_forces", "dyn_nb_correct_14", "dyn_nb_build_neighbors",
+    "dyn_nb_compute", "dyn_nb_get_fixed_forces", "dyn_nb_correct_14", "dyn_nb_correct_14_noalloc",
+    "dyn_nb_build_neighbors",
     "dyn_pme_create", "dyn_pme_destroy", "dyn_pme_last_error", "dyn_pme_forces_and_virial",
     "dyn_pme_get_fixed_grid", "dyn_pme_grid_n",
     "dyn_engine_create", "dyn_engine_destroy", "dyn_engine_last_error", "dyn_engine_set_box",
@@ -292,15 +294,16 @@ class ForcesPositsGpu:
         _check(lib.dyn_nb_get_fixed_forces(self.h, out, n), self._err)
         return out.reshape(-1, 3)
 
-    def correct_14(self, i14, j14, pos_dyn, q_dyn, extent, f_dyn, virial=0.0):
+    def correct_14(self, i14, j14, pos_dyn, q_dyn, extent, f_dyn, virial=0.0, noalloc=False):
         """The 1-4 correction (src/non_bonded.rs:763-790) added into f_dyn;
-        returns virial + its virial."""
+        returns virial + its virial.  noalloc: the allocation-free entry."""
         i14 = _c(i14, np.uint32)
         j14 = _c(j14, np.uint32)
         pd = _c(pos_dyn, np.float32)
         qd = _c(q_dyn, np.float32)
         w = C.c_double(virial)
-        rc = lib.dyn_nb_correct_14(self.h, len(i14), _ptr(i14), _ptr(j14), _ptr(pd), _ptr(qd),
+        fn = lib.dyn_nb_correct_14_noalloc if noalloc else lib.dyn_nb_correct_14
+        rc = fn(self.h, len(i14), _ptr(i14), _ptr(j14), _ptr(pd), _ptr(qd),
                 _c(extent, np.float32), f_dyn, C.byref(w))
         _check(rc, self._err)
         return w.value

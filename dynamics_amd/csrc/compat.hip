@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -56,6 +57,47 @@ __device__ double block_sum_f64(double v, double* sh) {
   const double r = sh[0];
   __syncthreads();
   return r;
+}
+
+// ---- scratch of the allocation-free 1-4 correction ------------------------
+// dyn_nb_correct_14_noalloc stages its inputs, fixed-point sums and partials
+// here instead of in buffers the context allocates.  Each device holds its
+// own copy for the life of the process; host code uses the address from
+// hipGetSymbolAddress (c14_scratch_ptr), never the symbol itself.
+constexpr uint32_t kC14MaxAtoms = 1u << 16;  // solute atoms of one call
+constexpr uint32_t kC14MaxPairs = 1u << 18;  // 1-4 pairs of one call
+constexpr unsigned kC14MaxBlocks = 1024u;    // grid of k_corr14_vacuum (grid_for(n, 256, 1024))
+constexpr size_t c14_align(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kC14OffFx = 0;                                                      // int64 [3·atoms]
+constexpr size_t kC14OffF = kC14OffFx + c14_align(8ull * 3 * kC14MaxAtoms);          // double [3·atoms]
+constexpr size_t kC14OffPart = kC14OffF + c14_align(8ull * 3 * kC14MaxAtoms);        // double [blocks + 1]
+constexpr size_t kC14OffPos = kC14OffPart + c14_align(8ull * (kC14MaxBlocks + 1));   // float [3·atoms]
+constexpr size_t kC14OffQ = kC14OffPos + c14_align(4ull * 3 * kC14MaxAtoms);         // float [atoms]
+constexpr size_t kC14OffI = kC14OffQ + c14_align(4ull * kC14MaxAtoms);               // uint32 [pairs]
+constexpr size_t kC14OffJ = kC14OffI + c14_align(4ull * kC14MaxPairs);               // uint32 [pairs]
+constexpr size_t kC14ScratchBytes = kC14OffJ + c14_align(4ull * kC14MaxPairs);       // 6.3 MB
+
+// Zero n int64 values (a fill kernel, not a memset: callers may capture it).
+__global__ void k_fill_zero_i64(int64_t* __restrict__ p, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = 0;
+}
+
+// f[k] += fx[k]·2⁻³² (the caller's forces, staged in f) and w_out = the sum
+// of the per-block partials in block order: the arithmetic and order of the
+// host-side finish of dyn_nb_correct_14.
+__global__ void k_c14_finish(const int64_t* __restrict__ fx, double* __restrict__ f, size_t n,
+                             const double* __restrict__ part, unsigned nblk,
+                             double* __restrict__ w_out) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    f[k] += (double)fx[k] * DYN_FX_INV_SCALE;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double w = 0;
+    for (unsigned b = 0; b < nblk; ++b) w += part[b];
+    *w_out = w;
+  }
 }
 
 // One thread per pair, grid-stride (src/cuda/cuda.cu:232-239).  Forces go
@@ -310,6 +352,34 @@ k_nbl_rows(NblRowArgs A, uint32_t n_outer, uint32_t* __restrict__ row_count,
 
 }  // namespace
 
+// The allocation-free 1-4 correction's device memory (kC14ScratchBytes per
+// device, held until the process ends).  In host code this name is only a
+// stand-in for the symbol: kernels get the address c14_scratch_ptr returns.
+static __device__ __align__(256) unsigned char scratch[kC14ScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (that device must be current):
+// looked up once per device, at context creation or at the first
+// allocation-free call there.
+unsigned char* c14_scratch_ptr(int device) {
+  DYN_REQUIRE(device >= 0 && device < kMaxDevices, "device %d beyond the scratch table", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw Error(DYN_ERR_HIP, fmt("hipGetSymbolAddress(scratch) on device %d: %s", device,
+                                   hipGetErrorString(e)));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
@@ -405,6 +475,7 @@ extern "C" int dyn_nb_create(dyn_nb_ctx** out, int device, uint32_t n_dyn, uint3
     DYN_HIP(hipMemsetAsync(c->pos.p, 0, c->pos.bytes(), c->stream));
     DYN_HIP(hipMemsetAsync(c->fx.p, 0, c->fx.bytes(), c->stream));
     DYN_HIP(hipStreamSynchronize(c->stream));
+    (void)c14_scratch_ptr(device);  // this device's scratch address, once
     *out = c.release();
     return DYN_OK;
   } catch (const Error& e) {
@@ -628,6 +699,65 @@ extern "C" int dyn_nb_correct_14(dyn_nb_ctx* ctx, uint32_t n14, const uint32_t* 
     for (size_t k = 0; k < hf.size(); ++k) f_dyn[k] += (double)hf[k] * DYN_FX_INV_SCALE;
     double w = 0;
     for (unsigned b = 0; b < grid; ++b) w += hp[b];
+    if (virial) *virial += w;
+  });
+}
+
+// dyn_nb_correct_14 without any allocation: the same kernel and the same
+// finishing arithmetic, with inputs, sums and partials staged in this
+// device's `scratch` (through its looked-up address).  Calls on one device
+// share that memory; overlapping calls there race.
+extern "C" int dyn_nb_correct_14_noalloc(dyn_nb_ctx* ctx, uint32_t n14, const uint32_t* i14,
+                                         const uint32_t* j14, const float* pos_dyn,
+                                         const float* q_dyn, const float cell_extent[3],
+                                         double* f_dyn, double* virial) {
+  if (!ctx) {
+    set_global_error("null context");
+    return DYN_ERR_ARG;
+  }
+  DYN_API_BODY(ctx, {
+    DeviceGuard g(ctx->device);
+    const uint32_t nd = ctx->n_dyn;
+    if (n14 == 0) return DYN_OK;
+    DYN_REQUIRE(i14 && j14 && pos_dyn && q_dyn && cell_extent && f_dyn, "null argument");
+    if (nd > kC14MaxAtoms || n14 > kC14MaxPairs)
+      throw Error(DYN_ERR_LIMIT,
+                  fmt("the allocation-free 1-4 correction takes at most %u solute atoms and %u "
+                      "pairs (got %u atoms, %u pairs); use dyn_nb_correct_14",
+                      kC14MaxAtoms, kC14MaxPairs, nd, n14));
+    for (uint32_t k = 0; k < n14; ++k)
+      DYN_REQUIRE(i14[k] < nd && j14[k] < nd, "1-4 pair %u index out of range", k);
+    for (int k = 0; k < 3; ++k) DYN_REQUIRE(cell_extent[k] > 0.f, "cell extent must be > 0");
+    unsigned char* base = c14_scratch_ptr(ctx->device);
+    int64_t* fx = reinterpret_cast<int64_t*>(base + kC14OffFx);
+    double* f = reinterpret_cast<double*>(base + kC14OffF);
+    double* part = reinterpret_cast<double*>(base + kC14OffPart);
+    float* pos = reinterpret_cast<float*>(base + kC14OffPos);
+    float* q = reinterpret_cast<float*>(base + kC14OffQ);
+    uint32_t* ii = reinterpret_cast<uint32_t*>(base + kC14OffI);
+    uint32_t* jj = reinterpret_cast<uint32_t*>(base + kC14OffJ);
+    hipStream_t s = ctx->stream;
+    DYN_HIP(hipMemcpyAsync(ii, i14, 4 * (size_t)n14, hipMemcpyHostToDevice, s));
+    DYN_HIP(hipMemcpyAsync(jj, j14, 4 * (size_t)n14, hipMemcpyHostToDevice, s));
+    DYN_HIP(hipMemcpyAsync(pos, pos_dyn, 12 * (size_t)nd, hipMemcpyHostToDevice, s));
+    DYN_HIP(hipMemcpyAsync(q, q_dyn, 4 * (size_t)nd, hipMemcpyHostToDevice, s));
+    DYN_HIP(hipMemcpyAsync(f, f_dyn, 24 * (size_t)nd, hipMemcpyHostToDevice, s));
+    const size_t n3 = 3 * (size_t)nd;
+    const unsigned gz = grid_for(n3, 256, 1024);
+    hipLaunchKernelGGL(k_fill_zero_i64, dim3(gz), dim3(256), 0, s, fx, n3);
+    DYN_CHECK_LAUNCH("k_fill_zero_i64");
+    const DynRefParams P = make_ref_params(ctx, cell_extent, false, false, 0.0);
+    const unsigned grid = grid_for(n14, kRefBlock, kC14MaxBlocks);
+    hipLaunchKernelGGL(k_corr14_vacuum, dim3(grid), dim3(kRefBlock), 0, s, ii, jj, n14, pos, q, P,
+                       fx, part);
+    DYN_CHECK_LAUNCH("k_corr14_vacuum");
+    hipLaunchKernelGGL(k_c14_finish, dim3(gz), dim3(256), 0, s, fx, f, n3, part, grid,
+                       part + kC14MaxBlocks);
+    DYN_CHECK_LAUNCH("k_c14_finish");
+    double w = 0.0;
+    DYN_HIP(hipMemcpyAsync(f_dyn, f, 24 * (size_t)nd, hipMemcpyDeviceToHost, s));
+    DYN_HIP(hipMemcpyAsync(&w, part + kC14MaxBlocks, 8, hipMemcpyDeviceToHost, s));
+    DYN_HIP(hipStreamSynchronize(s));
     if (virial) *virial += w;
   });
 }

@@ -113,6 +113,19 @@ int dyn_nb_correct_14(dyn_nb_ctx* ctx, uint32_t n14, const uint32_t* i14, const 
                       const float* pos_dyn, const float* q_dyn, const float cell_extent[3],
                       double* f_dyn, double* virial);
 
+/* No counterpart in the reference.  dyn_nb_correct_14 for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same arguments, kernel and results
+ * (bit for bit), but no hipMalloc, hipFree or other allocating call — inputs,
+ * fixed-point sums and partials are staged in a fixed per-device scratch area
+ * of 6.3 MB held until the process ends.  At most 65,536 solute atoms (the
+ * context's n_dyn) and 262,144 pairs per call; larger calls return
+ * DYN_ERR_LIMIT.  Calls on one device share the scratch area: calls that
+ * overlap there (several threads or contexts at once) race. */
+int dyn_nb_correct_14_noalloc(dyn_nb_ctx* ctx, uint32_t n14, const uint32_t* i14,
+                              const uint32_t* j14, const float* pos_dyn, const float* q_dyn,
+                              const float cell_extent[3], double* f_dyn, double* virial);
+
 /* build_neighbors (src/neighbors.rs:180-281): for each outer point, the inner
  * indices with |min_image(p_outer − p_inner)|² < skin_sq_w_cutoff, min image
  * and |·|² evaluated exactly as the reference's f32 CPU code does, so the
