@@ -66,7 +66,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="about the wall time of the CPU-baseline sample (0 skips it)")
     ap.add_argument("--mode", choices=["md", "eval", "copy", "batch-lambda", "batch-uneven",
-                                       "precision"],
+                                       "precision", "tune"],
                     default="md",
                     help="md: device-resident MD steps; eval: nonbonded evaluations only; "
                          "copy: a plain 256 MiB device copy, three times (the known-size "
@@ -74,7 +74,9 @@ def main() -> int:
                          "alchemical λ-windows of the solvated chain in one batch; "
                          "batch-uneven: 128 small solvated boxes of uneven sizes in one batch; "
                          "precision: the 128 A water box with the pair kernel and SPME each in "
-                         "FP32 and FP64, errors against the engine's FP64 evaluation")
+                         "FP32 and FP64, errors against the engine's FP64 evaluation; "
+                         "tune: the autotuner's table for the MD workload (SPME order and "
+                         "grid, list buffer, work-item length)")
     ap.add_argument("--systems", type=int, default=None,
                     help="batch modes: number of systems (16 windows / 128 boxes)")
     ap.add_argument("--relax-steps", type=int, default=400,
@@ -111,6 +113,8 @@ def main() -> int:
         return run_batch_uneven(args)
     if args.mode == "precision":
         return run_precision(args)
+    if args.mode == "tune":
+        return run_tune(args)
 
     import numpy as np
     from dynamics_amd import api, systems
@@ -584,6 +588,35 @@ def run_precision(args) -> int:
                                "with a 1 A list buffer"},
         "precision_sweep": rows,
     }
+    print(json.dumps(out))
+    return 0
+
+
+def run_tune(args) -> int:
+    """The autotuner (dynamics_amd.tune) on the MD workload from the same
+    thermal start as --mode md: SPME (order, grid) candidates against a
+    converged FP64 evaluation and their SPME time, then MD ms/step over list
+    buffers × work-item lengths.  Prints the choice and the table."""
+    from dynamics_amd import api, systems, tune
+
+    s = systems.config1()
+    K = (64, 64, 64)
+    eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
+                     mode=api.MODE_REFERENCE, list_buffer=MD_BUFFER)
+    ff = systems.chain_force_field(s)
+    pd0, pw0 = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=args.relax_steps)
+    s = systems.with_positions(s, pd0, pw0)
+    st = equilibrate(eng, ff, s, pd0, pw0, args.equil_steps)
+    eng.close()
+    print("bench: relaxed and equilibrated", file=sys.stderr, flush=True)
+    sys_end = systems.with_positions(s, st[0], st[2])
+    res = tune.tune_md(sys_end, ff, st, grid=K, order=4, steps=max(args.steps // 5, 20))
+    out = {"metric": "autotuner choice for the MD workload, 1 GPU",
+           "value": res["choice"]["ms_per_step"], "unit": "ms/step",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": "chain2600_opc8600_box64 MD (Langevin-middle, dt 2 fs), "
+                                  "thermalised start"},
+           "choice": res["choice"], "table": res["table"]}
     print(json.dumps(out))
     return 0
 

@@ -84,7 +84,7 @@ class EngineParams(C.Structure):
     _fields_ = [("cutoff", C.c_float), ("lj_cutoff", C.c_float), ("beta", C.c_float),
                 ("grid", C.c_uint32 * 3), ("order", C.c_uint32), ("mode", C.c_uint32),
                 ("list_buffer", C.c_float), ("precision", C.c_uint32),
-                ("mesh_spacing", C.c_float)]
+                ("mesh_spacing", C.c_float), ("item_groups", C.c_uint32)]
 
 
 class Energies(C.Structure):
@@ -378,13 +378,14 @@ class Engine:
 
     def __init__(self, system, cutoff=10.0, lj_cutoff=None, beta=0.3123, grid=(0, 0, 0),
                  order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0, precision=0,
-                 mesh_spacing=0.0):
+                 mesh_spacing=0.0, item_groups=0):
         """precision: PREC_PAIR_F64 | PREC_PME_F64 bits (0: FP32).  mesh_spacing
-        > 0 (with grid 0): the SPME grid follows the box (regen_pme)."""
+        > 0 (with grid 0): the SPME grid follows the box (regen_pme).
+        item_groups: j groups per work item of the pair kernel (0: 16)."""
         s = system
         prm = EngineParams(cutoff, cutoff if lj_cutoff is None else lj_cutoff, beta,
                            (C.c_uint32 * 3)(*[int(g) for g in grid]), order, mode, list_buffer,
-                           precision, mesh_spacing)
+                           precision, mesh_spacing, item_groups)
         self.h = C.c_void_p()
         q = _c(s.q_dyn, np.float32)
         t = _c(s.type_dyn, np.int32)

@@ -168,6 +168,14 @@ struct Profiler {
     Profiler* p;
     Rec r;
     std::chrono::steady_clock::time_point t0;
+    // inside a capture the record becomes an event node of the graph (an
+    // external record: each launch of the graph records the event)
+    void record(hipEvent_t e) {
+      if (g_launch_mode == kLaunchCapture)
+        DYN_HIP(hipEventRecordWithFlags(e, p->stream, hipEventRecordExternal));
+      else
+        DYN_HIP(hipEventRecord(e, p->stream));
+    }
     Scope(Profiler& prof, const char* name)
         : p(prof.enabled && g_launch_mode != kLaunchDry &&
                     (!prof.only || std::strcmp(prof.only, name) == 0)
@@ -177,14 +185,14 @@ struct Profiler {
         r.name = name;
         r.a = p->ev();
         r.b = p->ev();
-        DYN_HIP(hipEventRecord(r.a, p->stream));
+        record(r.a);
         t0 = std::chrono::steady_clock::now();
       }
     }
     ~Scope() noexcept(false) {
       if (p) {
         r.host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        DYN_HIP(hipEventRecord(r.b, p->stream));
+        record(r.b);
         if (g_launch_mode == kLaunchCapture && p->capture_sink) {
           r.keep = true;  // a graph holds the events: they stay until the profiler goes
           p->graph_events.push_back(r.a);
@@ -202,7 +210,13 @@ struct Profiler {
     DYN_HIP(hipStreamSynchronize(stream));
     for (auto& r : pending) {
       float ms = 0.f;
-      DYN_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      const hipError_t te = hipEventElapsedTime(&ms, r.a, r.b);
+      if (te != hipSuccess) {
+        // events of a graph that the runtime cannot time: left out
+        (void)hipGetLastError();
+        if (r.keep) continue;
+        DYN_HIP(te);
+      }
       auto it = totals.find(r.name);
       if (it == totals.end()) {
         totals[r.name] = {0.0, 0};

@@ -13,7 +13,7 @@
 #define DYN_CLS_WATER_Q 2
 
 #define DYN_CL_SIZE 8  // sites per cluster; a wave holds 8 i slots × 8 j slots
-#define DYN_CHUNK 16   // j groups (of 8 j slots) per work item of the pair kernel
+#define DYN_CHUNK 16   // default j groups (of 8 j slots) per work item of the pair kernel
 
 // A row of the pair list (one i cluster) holds j slots: every slot of a
 // nearby cluster that lies within the list radius of at least one i slot it

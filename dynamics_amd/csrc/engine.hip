@@ -7,7 +7,7 @@
 //              8, write cluster slots and bounding boxes             (rebuild)
 //   list       full cluster-pair list: every cluster lists every cluster
 //              whose box is within the cutoff (+ buffer), both ways; rows
-//              are cut into work items of ≤ DYN_CHUNK entries; solute–solute
+//              are cut into work items of ≤ item_groups groups; solute–solute
 //              entries get exclusion masks                            (rebuild)
 //   slots      gather current positions into cluster slots       (reuse path)
 //   pairs      8×8 wave tiles over the work items: LJ + erfc Coulomb; each
@@ -344,6 +344,7 @@ struct ListArgs {
   uint32_t n_dyn;
   uint64_t cap_entries, cap_chunks;  // groups, work items
   uint32_t row_cap;                  // entries per row slice of ent_tmp (multiple of 8)
+  uint32_t chunk;                    // j groups per work item
   uint32_t null_slot;                // padding j slot (exclusion group: padding)
   uint32_t* longest;                 // longest row that overflowed its slice
   unsigned int* status;
@@ -502,7 +503,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
       }
       const uint32_t m = min(n_out, A.row_cap) / DYN_CL_SIZE;
       row_cnt[I] = m;
-      row_chunks[I] = (m + DYN_CHUNK - 1) / DYN_CHUNK;
+      row_chunks[I] = (m + A.chunk - 1) / A.chunk;
     }
   }
 }
@@ -527,10 +528,10 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
     } else if (lane == 0) {
       atomicOr(A.status, kStEntries);
     }
-    const uint32_t c0 = chunk_ptr[I], nch = (m + DYN_CHUNK - 1) / DYN_CHUNK;
+    const uint32_t c0 = chunk_ptr[I], nch = (m + A.chunk - 1) / A.chunk;
     if ((uint64_t)c0 + nch <= A.cap_chunks) {
       for (uint32_t k = lane; k < nch; k += kWave)
-        chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * DYN_CHUNK);
+        chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * A.chunk);
     } else if (lane == 0) {
       atomicOr(A.status, kStChunks);
     }
@@ -650,6 +651,7 @@ struct PairArgs {
   int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
+  uint32_t chunk;            // j groups per work item
   const unsigned int* status;  // a list-build overflow leaves the list unusable
 };
 
@@ -748,7 +750,7 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
   else T.n_c += h;
 }
 
-// One wave per work item (≤ DYN_CHUNK groups of one row); lane ia + 8·jb
+// One wave per work item (≤ A.chunk groups of one row); lane ia + 8·jb
 // holds the pair (i slot 8I+ia, j entry jb of the group).  The next group's
 // j data is loaded before the current group is evaluated.  A work item
 // stores its 8 i-slot forces in its own row of fitem (no atomics) and its
@@ -769,7 +771,7 @@ k_eng_pairs(PairArgs A) {
     const uint2 ch = A.chunks[item];
     const uint32_t I = __builtin_amdgcn_readfirstlane(ch.x);
     const uint32_t g0 = __builtin_amdgcn_readfirstlane(ch.y);
-    const uint32_t g1 = min(g0 + (uint32_t)DYN_CHUNK, A.row_ptr[I + 1]);
+    const uint32_t g1 = min(g0 + A.chunk, A.row_ptr[I + 1]);
     const float4 piv = A.xyzq[I * DYN_CL_SIZE + ia];
     const float pi[4] = {piv.x, piv.y, piv.z, piv.w};
     const int ti = A.slot_type[I * DYN_CL_SIZE + ia];
@@ -1464,7 +1466,7 @@ void dyn_engine::set_geometry() {
   reserve_rows((uint32_t)std::min<double>(row_est, 1 << 20));
   const double avg_groups = dens * 4.19 * std::pow(rl + 2.0, 3) / DYN_CL_SIZE + 3;
   const uint64_t groups = (uint64_t)(cap_cl * avg_groups);
-  reserve_list(groups, groups / DYN_CHUNK + cap_cl);
+  reserve_list(groups, groups / prm.item_groups + cap_cl);
   DYN_HIP(hipMemcpyAsync(col_class.p, h_col_class.data(), sizeof(int) * n_cols,
                          hipMemcpyHostToDevice, stream));
   // the binning kernel counts into zeroed column counts; the column sort
@@ -1595,6 +1597,7 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     A.cap_entries = e.cap_entries;
     A.cap_chunks = e.cap_chunks;
     A.row_cap = e.row_cap;
+    A.chunk = e.prm.item_groups;
     A.null_slot = e.cap_slots;
     A.longest = e.counters.p + 4;
     A.status = e.status.p;
@@ -1814,6 +1817,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.fitem = e.fitem.p;
       A.partials = e.pair_part.p;
       A.cap_chunks = e.cap_chunks;
+      A.chunk = e.prm.item_groups;
       A.status = e.status.p;
     }
     // a wave per work item (the count is known on the device only)
@@ -1985,6 +1989,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_REQUIRE(p.mesh_spacing >= 0.f && std::isfinite(p.mesh_spacing), "bad mesh_spacing");
     DYN_REQUIRE(p.mesh_spacing == 0.f || (p.grid[0] == 0 && p.grid[1] == 0 && p.grid[2] == 0),
                 "give either a PME grid or a mesh spacing");
+    DYN_REQUIRE(p.item_groups <= 64, "item_groups must be in 1..64 (0: 16)");
     DYN_REQUIRE(n_dyn + 4ull * n_water > 0, "empty system");
     DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 30), "system too large");
     DYN_REQUIRE(n_dyn == 0 || (q_dyn && type_dyn), "null solute arrays");
@@ -2003,6 +2008,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_HIP(hipEventCreateWithFlags(&E->ev_join, hipEventDisableTiming));
     E->prof.stream = E->stream;
     E->prm = p;
+    if (!E->prm.item_groups) E->prm.item_groups = DYN_CHUNK;
     E->n_dyn = n_dyn;
     E->n_water = n_water;
     const uint32_t ns = n_dyn + 4 * n_water;

@@ -194,6 +194,9 @@ typedef struct dyn_engine_params {
                         * dyn_pme_grid_n at every set_box (the reference's
                         * regen_pme, src/non_bonded.rs:850-875); grid must
                         * then be 0.  0: the fixed grid above              */
+  uint32_t item_groups; /* j groups (of 8 j slots) per work item of the pair
+                         * kernel, 1..64; 0: 16.  A tuning knob: results are
+                         * the same for any value to f32 summation order   */
 } dyn_engine_params;
 
 /* Precision options (the precision sweep; the default FP32 path is the one
@@ -273,14 +276,13 @@ int dyn_engine_get_pme_grid(dyn_engine* eng, uint32_t out[3]);
 int dyn_engine_set_alchemical(dyn_engine* eng, uint32_t first, uint32_t count, double lambda);
 
 /* One nonbonded evaluation.  Rebuilds the cluster-pair list when asked, on
- * the first call, after set_box, or when a site moved more than
- * list_buffer/2 since the last build.  With DYN_COMPUTE_ASYNC the decision
- * comes from the displacement readings of earlier evaluations (at most two
- * evaluations old; the call waits for older ones) and the list is rebuilt
- * at 0.25 × list_buffer; should a site have reached list_buffer/2 before,
- * the evaluation is flagged on the device and its forces are not valid:
- * dyn_engine_sync then returns DYN_ERR_CAPACITY, the list is rebuilt at the
- * next compute, and the caller evaluates again. */
+ * the first call, after set_box, without a list buffer, or — decided on the
+ * device by the evaluation itself — when a site has moved half the list
+ * buffer since the last build; the list in use is therefore always valid,
+ * also with DYN_COMPUTE_ASYNC (no host reading is involved).  A list or
+ * SPME buffer that overflows is reported by the next wait as
+ * DYN_ERR_CAPACITY after the buffer has been enlarged (a synchronous
+ * compute evaluates once more by itself). */
 int dyn_engine_compute(dyn_engine* eng, uint32_t flags);
 int dyn_engine_sync(dyn_engine* eng);
 

@@ -191,7 +191,10 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
   static float fi[64][3];
   for (uint64_t it = 0; it < n_items; ++it) {
     const uint32_t I = items[2 * it], g0 = items[2 * it + 1];
-    const uint32_t g1 = std::min<uint32_t>(g0 + DYN_CHUNK, row_ptr[I + 1]);
+    // a work item ends where the next one of its row begins (the items of a
+    // row are consecutive), else at the row's end: any work-item length
+    const bool next_same = it + 1 < n_items && items[2 * (it + 1)] == I;
+    const uint32_t g1 = next_same ? items[2 * (it + 1) + 1] : row_ptr[I + 1];
     memset(fi, 0, sizeof fi);
     const int ci = cl_class[I];
     double ie[6] = {0, 0, 0, 0, 0, 0};
