@@ -84,6 +84,8 @@ def main() -> int:
     ap.add_argument("--equil-steps", type=int, default=2000,
                     help="untimed Langevin steps (γ = 10/ps, 300 K) that thermalise the "
                          "relaxed structure before the timed MD")
+    ap.add_argument("--direct-launch", action="store_true",
+                    help="MD modes: launch every kernel directly (no step graphs)")
     ap.add_argument("--list-buffer", type=float, default=None,
                     help="list buffer in Å (MD modes default %.1f; eval default 0: rebuild "
                          "every step)" % MD_BUFFER)
@@ -134,7 +136,8 @@ def main() -> int:
     if args.mode == "md":
         st = equilibrate(eng, ff, s, pd0, pw0, args.equil_steps)
         print(f"bench: equilibrated ({args.equil_steps} steps)", file=sys.stderr, flush=True)
-        md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=1)
+        md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=1,
+                    direct_launch=args.direct_launch)
         md.set_state(*st)
 
         def run(n):
@@ -251,6 +254,7 @@ def main() -> int:
                      "moved half the buffer"),
             "list_builds_in_timed_steps": builds,
             "steps_in_graph_launches": graph_steps,
+            "launch": "direct" if args.direct_launch else "graphs of 16 steady steps",
             "start": (f"relaxed ({args.relax_steps} steps), then {args.equil_steps} untimed "
                       "Langevin steps at 300 K, γ 10/ps" if md is not None else
                       f"relaxed ({args.relax_steps} steps)"),

@@ -66,7 +66,7 @@ class MdParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("temperature", C.c_float), ("gamma", C.c_float),
                 ("shake_tol", C.c_float), ("seed", C.c_uint64), ("spme_ratio", C.c_uint32),
                 ("nstpcouple", C.c_uint32), ("pressure_target", C.c_float), ("tau_p", C.c_float),
-                ("compressibility", C.c_double)]
+                ("compressibility", C.c_double), ("direct_launch", C.c_uint32)]
 
 
 class MdEnergies(C.Structure):
@@ -530,13 +530,14 @@ class Md:
 
     def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0,
                  spme_ratio=1, nstpcouple=0, pressure_target=1.0, tau_p=5.0,
-                 compressibility=4.5e-5):
+                 compressibility=4.5e-5, direct_launch=False):
         """nstpcouple > 0 couples the pressure (C-rescale) every nstpcouple
-        steps to pressure_target bar with time constant tau_p ps."""
+        steps to pressure_target bar with time constant tau_p ps.
+        direct_launch: launch every kernel directly (no step graphs)."""
         self.engine = engine
         self.h = C.c_void_p()
         p = MdParams(dt, temperature, gamma, shake_tol, seed, spme_ratio, nstpcouple,
-                     pressure_target, tau_p, compressibility)
+                     pressure_target, tau_p, compressibility, 1 if direct_launch else 0)
         arrs = [_c(ff.mass, np.float32),
                 _c(ff.bonds, np.uint32), _c(ff.bond_k_r0, np.float32),
                 _c(ff.angles, np.uint32), _c(ff.angle_k_t0, np.float32),

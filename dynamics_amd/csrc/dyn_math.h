@@ -36,6 +36,9 @@
 struct float2 {
   float x, y;
 };
+struct float4 {
+  float x, y, z, w;
+};
 struct double2 {
   double x, y;
 };
@@ -86,6 +89,14 @@ DYN_HD double dyn_fma(double a, double b, double c) {
   return __builtin_fma(a, b, c);
 #else
   return fma(a, b, c);
+#endif
+}
+
+DYN_HD float dyn_floorf(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_floorf(x);
+#else
+  return floorf(x);
 #endif
 }
 
@@ -278,6 +289,63 @@ DYN_HD void dyn_coul_erfc(float r2, float rinv, float qq, float beta,
   const float gr = g * rinv;
   *e = qe * gr;
   *s = qe * (rinv * rinv) * (gr + two_b_sqrtpi);
+}
+
+// Tabulated erfc Coulomb of the pair kernel.  With x = βr the force scalar
+// is s = qq/r³ · φ(x), φ(x) = erfc(x) + (2/√π) x e^{−x²}, and the energy
+// E = qq/r · erfc(x).  Both functions of x are tabulated on [0, 4] as
+// cubic Hermite pieces (values and derivatives from the closed forms, in
+// double; φ'(x) = −(4/√π) x² e^{−x²}, erfc'(x) = −(2/√π) e^{−x²}) over
+// DYN_COUL_TAB_N intervals: entry 2i holds φ's cubic on interval i, entry
+// 2i+1 erfc's, each as (c0, c1, c2, c3) in the local coordinate t ∈ [0, 1).
+// Interpolation error ≤ h⁴/384·max|f⁗| ≈ 1e-10 (h = 1/128), below f32
+// rounding, so the table reproduces the closed forms to f32 accuracy with
+// one load and three fmas in place of exp(−x²)·erfcx(x).  Evaluation uses
+// only IEEE-exact operations (mul, floor, conversion, fma), so the host
+// replay, given the same table, reproduces it bit for bit.
+#define DYN_COUL_TAB_N 512
+#define DYN_COUL_TAB_XMAX 4.0f
+#define DYN_COUL_TAB_SCALE 128.0f  // intervals per unit of x
+
+DYN_HD float dyn_tab_cubic(const float4* tab, int entry_off, float x) {
+  const float u = x * DYN_COUL_TAB_SCALE;
+  const float fl = dyn_floorf(u);
+  int i = (int)fl;
+  i = i < 0 ? 0 : (i > DYN_COUL_TAB_N - 1 ? DYN_COUL_TAB_N - 1 : i);
+  const float t = u - fl;
+  const float4 c = tab[2 * i + entry_off];
+  return dyn_fmaf(dyn_fmaf(dyn_fmaf(c.w, t, c.z), t, c.y), t, c.x);
+}
+
+DYN_HD void dyn_coul_erfc_tab(float r2, float rinv, float qq, float beta, const float4* tab,
+                              float* s, float* e) {
+  const float x = (r2 * rinv) * beta;
+  const float phi = dyn_tab_cubic(tab, 0, x);
+  *s = qq * ((rinv * rinv) * rinv) * phi;
+  *e = qq * rinv * dyn_tab_cubic(tab, 1, x);
+}
+
+// The table (2·DYN_COUL_TAB_N float4 as 8·DYN_COUL_TAB_N floats), built on
+// the host in double from the closed forms (a host function).
+static inline void dyn_coul_tab_build(float* out) {
+  const double h = 1.0 / (double)DYN_COUL_TAB_SCALE;
+  const double k2 = 1.1283791670955126;  // 2/√π
+  auto phi = [&](double x) { return erfc(x) + k2 * x * exp(-x * x); };
+  auto dphi = [&](double x) { return -2.0 * k2 * x * x * exp(-x * x); };
+  auto ef = [&](double x) { return erfc(x); };
+  auto def = [&](double x) { return -k2 * exp(-x * x); };
+  for (int i = 0; i < DYN_COUL_TAB_N; ++i) {
+    const double x0 = i * h, x1 = (i + 1) * h;
+    for (int which = 0; which < 2; ++which) {
+      const double f0 = which ? ef(x0) : phi(x0), f1 = which ? ef(x1) : phi(x1);
+      const double d0 = h * (which ? def(x0) : dphi(x0)), d1 = h * (which ? def(x1) : dphi(x1));
+      float* c = out + 4 * (2 * i + which);
+      c[0] = (float)f0;
+      c[1] = (float)d0;
+      c[2] = (float)(3.0 * (f1 - f0) - 2.0 * d0 - d1);
+      c[3] = (float)(2.0 * (f0 - f1) + d0 + d1);
+    }
+  }
 }
 
 // erf-screened Coulomb qq·erf(βr)/r, i.e. the part of a pair's Coulomb

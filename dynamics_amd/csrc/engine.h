@@ -67,6 +67,7 @@ struct DynPairParams {
   int n_types;      // LJ type tables are (n_types × n_types) float2
   int alch_on;      // a decoupled molecule exists
   DynSchedule sch;  // staged decoupling schedule at the current λ
+  const float4* coul_tab;  // the erfc Coulomb table (dyn_coul_tab_build)
 };
 
 // One lane of an 8×8 tile: the pair (i, j) of the given kind.  Returns the
@@ -106,7 +107,7 @@ DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, 
   }
   if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SQ || KIND == DYN_KIND_QQ) {
     float sc, ee;
-    dyn_coul_erfc(r2s, rinv, pi[3] * pj[3], P.beta, P.beta2, P.two_b_sqrtpi, &sc, &ee);
+    dyn_coul_erfc_tab(r2s, rinv, pi[3] * pj[3], P.beta, P.coul_tab, &sc, &ee);
     s = s + (in ? sc : 0.f);
     ec = in ? ee : 0.f;
   }

@@ -236,21 +236,18 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
   return m >= 0 ? m : -2 - s;
 }
 
-// One workgroup per column: sort the column's sites by (z, site) and cut
+// One column, by one workgroup: sort the column's sites by (z, site) and cut
 // them into clusters of 8 (the last padded), writing the slot arrays and each
 // cluster's bounding box.  The column's count is cleared for the next
 // binning pass (the binning kernel counts with atomics into zeroed counts).
 constexpr int kSortBlock = 256;
+constexpr unsigned kBuildGrid = 1024;  // workgroups of the grid-stride build kernels
 __device__ __forceinline__ void
-k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
-              const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
-              const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
-              float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
-              unsigned int* __restrict__ status, uint32_t n_cols) {
-  __shared__ unsigned long long key[kColCap];
-  if (!build_on(G)) return;
-  const int c = blockIdx.x;
-  if ((uint32_t)c >= n_cols) return;
+colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
+            const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
+            const int* __restrict__ col_class, const SlotArgs& S, uint8_t* __restrict__ cl_class,
+            float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
+            unsigned int* __restrict__ status, unsigned long long* key) {
   const uint32_t n = col_count[c];
   __syncthreads();
   if (threadIdx.x == 0 && n) col_count[c] = 0;
@@ -326,6 +323,23 @@ k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __r
   }
 }
 
+__device__ __forceinline__ void
+k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
+              const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
+              const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
+              float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
+              unsigned int* __restrict__ status, uint32_t n_cols) {
+  __shared__ unsigned long long key[kColCap];
+  if (!build_on(G)) return;
+  // a capped grid walks the columns (a closed gate then costs few workgroups)
+  for (uint32_t c = blockIdx.x; c < n_cols; c += gridDim.x) {
+    colsort_one(c, col_count, col_start, items, col_cl_start, col_class, S, cl_class, bb_c, bb_h,
+                n_sites, status, key);
+    __syncthreads();  // key[] is reused by the next column
+  }
+}
+
+
 struct ListArgs {
   BuildGate gate;
   EngGeom g;
@@ -355,38 +369,57 @@ struct ListArgs {
 // that passed the box test.
 struct WalkLds {
   uint32_t js[kWave], beg[kWave], end[kWave];
+  float sx[kWave], sy[kWave];  // the column's periodic image offset seen from I
   uint32_t okJ[kWave];
+  float2 okS[kWave];
 };
 
 // Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
 // order (class, column x, column y, cluster), keep those whose bounding box
 // is within rlist of I's box (periodic), test their 8 slots against I's 8
-// slots (the same f32 min image and r² as the pair kernel; eight candidates
-// per round, a lane per j slot), and append every j slot within rlist of at
-// least one i slot it can interact with.  Each
+// slots (eight candidates per round, a lane per j slot), and append every
+// j slot within rlist of at least one non-padding i slot.  A candidate
+// column is seen as the periodic image the column walk reached it in, so x
+// and y differences need no minimum image (z does; so do x and y when the
+// column range spans the whole box).  Slots near only their own water
+// molecule's sites may be listed: the pair kernel masks such pairs.  Each
 // class segment is padded to a multiple of 8 with the null slot.  The row
 // goes into its fixed-capacity slice of ent_tmp; counts are in groups of 8.
+constexpr int kListBlock = 256;  // four waves, each on its own rows
+constexpr int kListWaves = kListBlock / kWave;
+
+// The LDS accesses of one wave stay in program order around this point (a
+// wave's lanes run in lockstep; the rows of one workgroup's waves are
+// independent, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void
 k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
                 uint32_t* __restrict__ row_chunks) {
-  __shared__ WalkLds L;
+  __shared__ WalkLds Ls[kListWaves];
   if (!build_on(A.gate)) return;
+  WalkLds& L = Ls[threadIdx.x / kWave];
   const uint32_t n_cl = *A.n_cl;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const unsigned long long lt = (1ull << lane) - 1ull;
   // a column is skipped when its x–y rectangle is beyond rlist of I's box;
   // the margin keeps the test conservative against rounding of the binning
   const float col_r2 = A.rlist2 * 1.0001f + 1e-3f;
-  for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
+  for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
+       I += gridDim.x * kListWaves) {
     uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
     const int ci = A.cl_class[I];
     const float4 c = A.bb_c[I], h = A.bb_h[I];
     float4 pI[DYN_CL_SIZE];
-    int gI[DYN_CL_SIZE];
+    bool iv[DYN_CL_SIZE];  // i slot holds a site
 #pragma unroll
     for (int u = 0; u < DYN_CL_SIZE; ++u) {
       pI[u] = A.xyzq[I * DYN_CL_SIZE + u];
-      gI[u] = A.slot_gid[I * DYN_CL_SIZE + u];
+      iv[u] = A.slot_gid[I * DYN_CL_SIZE + u] != DYN_GID_PAD;
     }
     uint32_t n_out = 0;
     for (int cls = 0; cls < 3; ++cls) {
@@ -411,6 +444,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
       for (int cb = 0; cb < n_colc; cb += kWave) {
         const int k = cb + lane;
         uint32_t js = 0, cnt = 0;
+        float sx = 0.f, sy = 0.f;
         if (k < n_colc) {
           const int xx = x0 + k / ny, yy = y0 + k % ny;
           float dx = 0.f, dy = 0.f;
@@ -422,6 +456,8 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
             const int col = cg.col_off + cx * cg.ncy + cy;
             js = A.col_cl_start[col];
             cnt = A.col_cl_start[col + 1] - js;
+            sx = (float)((xx - cx) / cg.ncx) * A.g.L[0];
+            sy = (float)((yy - cy) / cg.ncy) * A.g.L[1];
           }
         }
         uint32_t incl = cnt;
@@ -433,11 +469,14 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
         L.js[lane] = js;
         L.beg[lane] = incl - cnt;
         L.end[lane] = incl;
-        __syncthreads();
+        L.sx[lane] = sx;
+        L.sy[lane] = sy;
+        wave_sync();
         for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
           const uint32_t t = t0 + lane;
           bool ok = t < tot;
           uint32_t J = 0;
+          float2 sh = make_float2(0.f, 0.f);
           if (ok) {
             int lo = 0, hi = kWave - 1;  // first column whose range ends after t
             while (lo < hi) {
@@ -446,6 +485,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
               else lo = mid + 1;
             }
             J = L.js[lo] + (t - L.beg[lo]);
+            sh = make_float2(L.sx[lo], L.sy[lo]);
             const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
             float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
             dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
@@ -458,8 +498,11 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
           }
           const unsigned long long bal = __ballot(ok);
           const uint32_t n_ok = (uint32_t)__popcll(bal);
-          if (ok) L.okJ[__popcll(bal & lt)] = J;
-          __syncthreads();
+          if (ok) {
+            L.okJ[__popcll(bal & lt)] = J;
+            L.okS[__popcll(bal & lt)] = sh;
+          }
+          wave_sync();
           // eight candidate clusters per round: lane jb + 8·c tests j slot
           // jb of candidate c against all eight i slots (held in registers),
           // so the kept slots come out candidate by candidate, each in slot
@@ -471,16 +514,30 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
             uint32_t J = 0;
             if (kk < n_ok) {
               J = L.okJ[kk];
+              const float2 sh = L.okS[kk];
               const float4 pj = A.xyzq[J * DYN_CL_SIZE + jj];
-              const int gj = A.slot_gid[J * DYN_CL_SIZE + jj];
+              const bool jv = A.slot_gid[J * DYN_CL_SIZE + jj] != DYN_GID_PAD;
+              if (!full_x && !full_y) {
+                const float px = pj.x + sh.x, py = pj.y + sh.y;
 #pragma unroll
-              for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                const float d0 = dyn_min_image_rint(pI[u].x - pj.x, A.g.L[0], A.g.invL[0]);
-                const float d1 = dyn_min_image_rint(pI[u].y - pj.y, A.g.L[1], A.g.invL[1]);
-                const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
-                const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                keep = keep || (r2 < A.rlist2 && dyn_slot_pair_valid(gI[u], gj));
+                for (int u = 0; u < DYN_CL_SIZE; ++u) {
+                  const float d0 = pI[u].x - px;
+                  const float d1 = pI[u].y - py;
+                  const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
+                  const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+                  keep = keep || (r2 < A.rlist2 && iv[u]);
+                }
+              } else {
+#pragma unroll
+                for (int u = 0; u < DYN_CL_SIZE; ++u) {
+                  const float d0 = dyn_min_image_rint(pI[u].x - pj.x, A.g.L[0], A.g.invL[0]);
+                  const float d1 = dyn_min_image_rint(pI[u].y - pj.y, A.g.L[1], A.g.invL[1]);
+                  const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
+                  const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
+                  keep = keep || (r2 < A.rlist2 && iv[u]);
+                }
               }
+              keep = keep && jv;
             }
             const unsigned long long kb = __ballot(keep);
             if (keep) {
@@ -489,7 +546,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
             }
             n_out += (uint32_t)__popcll(kb);
           }
-          __syncthreads();
+          wave_sync();
         }
       }
       const uint32_t pad = (DYN_CL_SIZE - n_out % DYN_CL_SIZE) % DYN_CL_SIZE;
@@ -517,8 +574,9 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 uint2* __restrict__ chunks) {
   if (!build_on(A.gate)) return;
   const uint32_t n_cl = *A.n_cl;
-  const int lane = threadIdx.x;
-  for (uint32_t I = blockIdx.x; I < n_cl; I += gridDim.x) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
+       I += gridDim.x * kListWaves) {
     const uint32_t m = row_cnt[I];
     const uint64_t base = row_ptr[I];
     if (base + m <= A.cap_entries) {
@@ -728,9 +786,15 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
   } else {
     dyn_tile_pair<KIND>(pi, ti, pj, tj, active, A.P, A.lj_table, d, &s, &elj, &ec, &r2, &hit);
   }
-  T.fi[0] += s * d[0];
-  T.fi[1] += s * d[1];
-  T.fi[2] += s * d[2];
+  if constexpr (F64) {
+    T.fi[0] += s * d[0];
+    T.fi[1] += s * d[1];
+    T.fi[2] += s * d[2];
+  } else {
+    T.fi[0] = dyn_fmaf(s, d[0], T.fi[0]);
+    T.fi[1] = dyn_fmaf(s, d[1], T.fi[1]);
+    T.fi[2] = dyn_fmaf(s, d[2], T.fi[2]);
+  }
   if (!ENERGY) return;
   const bool solute = KIND == DYN_KIND_SS || KIND == DYN_KIND_SO || KIND == DYN_KIND_SQ;
   if (KIND != DYN_KIND_SQ && KIND != DYN_KIND_QQ) {
@@ -1284,6 +1348,7 @@ struct dyn_engine {
   DevBuf<float2> lj_table;   // (c6, c12) per type pair
   DevBuf<float2> lj_se;      // (σ, ε) per type pair (soft-core)
   DevBuf<double2> lj_table64;  // (c6, c12) in double (FP64 pair option)
+  DevBuf<float4> coul_tab;     // erfc Coulomb table of the pair kernel (dyn_coul_tab_build)
   std::vector<int32_t> h_type;  // per-site type words (host copy)
   // decoupled molecule: solute sites [alch0, alch1), λ
   uint32_t alch0 = 0, alch1 = 0;
@@ -1558,7 +1623,7 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
   }
   {
     Profiler::Scope ps(prof, "nbl_colsort");
-    const unsigned gc = max_grid(S, [](dyn_engine& e) { return e.n_cols; });
+    const unsigned gc = max_grid(S, [](dyn_engine& e) { return std::min(e.n_cols, kBuildGrid); });
     launch<k_eng_colsort, kSortBlock>(R, "k_eng_colsort", 0, gc, kSortBlock, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
       SlotArgs A;
@@ -1602,10 +1667,13 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     A.longest = e.counters.p + 4;
     A.status = e.status.p;
   }
-  const unsigned list_grid = max_grid(S, [](dyn_engine& e) { return std::min<uint32_t>(e.cap_cl, 16384); });
+  const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
+    return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kBuildGrid);
+  });
   {
     Profiler::Scope ps(prof, "nbl_list_walk");
-    launch<k_eng_list_walk, kWave>(R, "k_eng_list_walk", 0, list_grid, kWave, 0, [&](uint32_t i) {
+    launch<k_eng_list_walk, kListBlock>(R, "k_eng_list_walk", 0, list_grid, kListBlock, 0,
+                                        [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p);
     });
@@ -1618,7 +1686,8 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
   }
   {
     Profiler::Scope ps(prof, "nbl_list_fill");
-    launch<k_eng_list_fill, kWave>(R, "k_eng_list_fill", 0, list_grid, kWave, 0, [&](uint32_t i) {
+    launch<k_eng_list_fill, kListBlock>(R, "k_eng_list_fill", 0, list_grid, kListBlock, 0,
+                                        [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_list_fill>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_ptr.p, e.chunk_ptr.p,
                                    e.cj.p, e.chunks.p);
@@ -1629,7 +1698,7 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
       const unsigned gm = max_grid(S, [](dyn_engine& e) {
         const uint32_t sol_rows =
             e.n_dyn / DYN_CL_SIZE + (uint32_t)(e.geom.cg[0].ncx * e.geom.cg[0].ncy) + 1;
-        return std::min<uint32_t>(sol_rows, 4096);
+        return std::min<uint32_t>(sol_rows, kBuildGrid);
       });
       launch<k_eng_list_mask, kWave>(R, "k_eng_list_mask", 0, gm, kWave, 0, [&](uint32_t i) {
         dyn_engine& e = *E[i];
@@ -1686,6 +1755,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
     pl.alch_on = e.alch1 > e.alch0;
     P.alch_on = pl.alch_on ? 1 : 0;
     P.sch = dyn_schedule((float)e.lambda);
+    P.coul_tab = e.coul_tab.p;
     // the reference's SPME cadence: keep the last SPME forces, energy and
     // virial (reference mode, no decoupled molecule, same box)
     pl.reuse = e.pme_on && (flags & DYN_COMPUTE_RECIP_REUSE) && !(flags & DYN_COMPUTE_RECIP) &&
@@ -1743,8 +1813,8 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
     });
   }
   // the second stream: SPME (spread → FFT → convolution → inverse FFT →
-  // gather) and the caller's side work, overlapping the list build, the
-  // pair kernel and the corrections on the main stream
+  // gather), the exclusion and 1-4 corrections and the caller's side work,
+  // overlapping the list build and the pair kernel on the main stream
   stream_order(R.stream, RS.stream, S.ev_fork);
   if (any_recip) {
     std::vector<PmeRun> ra(ns), rb(ns);
@@ -1777,6 +1847,35 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
     }
     pme_run(RS, prof, ra.data(), 0);
     if (any_b) pme_run(RS, prof, rb.data(), 1);
+  }
+  // the exclusion and 1-4 corrections (site accumulators; finalize reads them)
+  if (any_corr) {
+    Profiler::Scope ps(prof, "corr");
+    const unsigned gc = max_grid(S, [](dyn_engine& e) { return e.corr_blocks; });
+    launch<k_eng_corr, kCorrBlock>(RS, "k_eng_corr", 0, gc, kCorrBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      CorrArgs C;
+      std::memset(&C, 0, sizeof C);
+      C.P = PL[i].P;
+      C.wpos = e.wpos.p;
+      C.ltype = e.ltype.p;
+      C.lj_table = e.lj_table.p;
+      C.lj_se = e.lj_se.p;
+      C.ci = e.corr_i.p;
+      C.cj = e.corr_j.p;
+      C.kind = e.corr_kind.p;
+      C.n = PL[i].corr_on ? e.n_corr : 0u;
+      C.mode = (int)e.prm.mode;
+      C.recip_on = PL[i].recip_on ? 1 : 0;
+      C.nblk = e.corr_blocks;
+      C.fx = e.fx_site.p;
+      C.n_sites = e.n_sites;
+      C.partials = energy_on ? e.corr_part.p : nullptr;
+      C.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
+      C.mol_of = e.mol_of.p;
+      C.n_mol = e.n_mol;
+      return args<k_eng_corr>(C);
+    });
   }
   if (side_work) side_work(RS);
   set_build_lists(S, gate);
@@ -1839,34 +1938,6 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
     else
       launch<k_eng_pairs<false, true>, kPairBlock>(R, "k_eng_pairs<false, f64>", 0, gp, kPairBlock, 0,
           [&](uint32_t i) { return args<k_eng_pairs<false, true>>(pa(i)); });
-  }
-  if (any_corr) {
-    Profiler::Scope ps(prof, "corr");
-    const unsigned gc = max_grid(S, [](dyn_engine& e) { return e.corr_blocks; });
-    launch<k_eng_corr, kCorrBlock>(R, "k_eng_corr", 0, gc, kCorrBlock, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      CorrArgs C;
-      std::memset(&C, 0, sizeof C);
-      C.P = PL[i].P;
-      C.wpos = e.wpos.p;
-      C.ltype = e.ltype.p;
-      C.lj_table = e.lj_table.p;
-      C.lj_se = e.lj_se.p;
-      C.ci = e.corr_i.p;
-      C.cj = e.corr_j.p;
-      C.kind = e.corr_kind.p;
-      C.n = PL[i].corr_on ? e.n_corr : 0u;
-      C.mode = (int)e.prm.mode;
-      C.recip_on = PL[i].recip_on ? 1 : 0;
-      C.nblk = e.corr_blocks;
-      C.fx = e.fx_site.p;
-      C.n_sites = e.n_sites;
-      C.partials = energy_on ? e.corr_part.p : nullptr;
-      C.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
-      C.mol_of = e.mol_of.p;
-      C.n_mol = e.n_mol;
-      return args<k_eng_corr>(C);
-    });
   }
   stream_order(RS.stream, R.stream, S.ev_join);
   {
@@ -2155,6 +2226,14 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->counters.reserve(8);
     E->status.reserve(1);
     E->dflags.reserve(4);
+    {
+      std::vector<float> tab(8 * DYN_COUL_TAB_N);
+      dyn_coul_tab_build(tab.data());
+      E->coul_tab.reserve(2 * DYN_COUL_TAB_N);
+      DYN_HIP(hipMemcpyAsync(E->coul_tab.p, tab.data(), sizeof(float) * tab.size(),
+                             hipMemcpyHostToDevice, E->stream));
+      DYN_HIP(hipStreamSynchronize(E->stream));
+    }
     // checkpoint copies of an MD call (allocated here: a step call allocates
     // nothing, so captured graphs of its steps stay current)
     E->fx_out_ck.reserve(3 * (size_t)ns);

@@ -1501,7 +1501,7 @@ bool graphs_usable(const MdSet& S) {
   // pair kernel) records its events into the graphs
   if (!S.graphs || S.graphs->disabled || (S.prof->enabled && !S.prof->only)) return false;
   for (dyn_md* m : S.m)
-    if (!engine_graph_safe(m->eng) || !m->forces_valid) return false;
+    if (!engine_graph_safe(m->eng) || !m->forces_valid || m->prm.direct_launch) return false;
   return true;
 }
 

@@ -421,6 +421,10 @@ typedef struct dyn_md_params {
   float pressure_target;  /* bar */
   float tau_p;            /* ps */
   double compressibility; /* 1/bar */
+  /* How the steps of a call are enqueued (a tuning knob; results are the same
+   * either way): 0 = runs of 16 steady steps are recorded into HIP graphs and
+   * replayed; 1 = every kernel is launched directly. */
+  uint32_t direct_launch;
 } dyn_md_params;
 
 /* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],
@@ -501,9 +505,10 @@ int dyn_md_get_energies(dyn_md* md, dyn_md_energies* out);
  * one MdState at a time, src/integrate.rs:100-317).  The handles must share
  * a device and the integrator kind (all Langevin or all plain velocity
  * Verlet); while in the batch, dyn_md_step refuses them (get/set_state,
- * get_energies and the engine queries work as before).  A list rebuild
- * that one system needs rebuilds all.  Destroy the batch before its
- * members. */
+ * get_energies and the engine queries work as before).  Each system's list
+ * is rebuilt when its own sites have moved far enough (the device decides
+ * per system; the build kernels of the others return at once).  Destroy the
+ * batch before its members. */
 typedef struct dyn_batch dyn_batch;
 int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds);
 void dyn_batch_destroy(dyn_batch* b);

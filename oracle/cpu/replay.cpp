@@ -30,6 +30,24 @@ void orc_coul_erfc(float r2, float qq, float beta, float* s, float* e) {
   const float rinv = dyn_rsqrtf(r2);
   dyn_coul_erfc(r2, rinv, qq, beta, beta * beta, 2.0f * beta * DYN_INV_SQRT_PI, s, e);
 }
+// The pair tiles' tabulated erfc Coulomb (dyn_coul_erfc_tab) and its table.
+static const float4* coul_table() {
+  static std::vector<float> tab;
+  if (tab.empty()) {
+    tab.resize(8 * DYN_COUL_TAB_N);
+    dyn_coul_tab_build(tab.data());
+  }
+  return reinterpret_cast<const float4*>(tab.data());
+}
+void orc_coul_erfc_tab(float r2, float qq, float beta, float* s, float* e) {
+  const float rinv = dyn_rsqrtf(r2);
+  dyn_coul_erfc_tab(r2, rinv, qq, beta, coul_table(), s, e);
+}
+// The two tabulated functions at x: φ(x) and erfc(x).
+void orc_coul_tab_eval(float x, float* phi, float* erfc_x) {
+  *phi = dyn_tab_cubic(coul_table(), 0, x);
+  *erfc_x = dyn_tab_cubic(coul_table(), 1, x);
+}
 void orc_coul_erf(float r2, float qq, float beta, float* s, float* e) {
   const float rinv = dyn_rsqrtf(r2);
   dyn_coul_erf(r2, rinv, qq, beta, beta * beta, 2.0f * beta * DYN_INV_SQRT_PI, s, e);
@@ -142,9 +160,9 @@ static void tile(const float* xyzq, const int32_t* slot_type, const int32_t* slo
       dyn_tile_pair_alch<KIND>(pi, ti, pj, tj, active, P, lj, lj_se, d, &s, &elj, &ec, &r2, &hit, &dh);
     else
       dyn_tile_pair<KIND>(pi, ti, pj, tj, active, P, lj, d, &s, &elj, &ec, &r2, &hit);
-    fi[l][0] += s * d[0];
-    fi[l][1] += s * d[1];
-    fi[l][2] += s * d[2];
+    fi[l][0] = dyn_fmaf(s, d[0], fi[l][0]);
+    fi[l][1] = dyn_fmaf(s, d[1], fi[l][1]);
+    fi[l][2] = dyn_fmaf(s, d[2], fi[l][2]);
     if (e) {
       // e: e_lj water, e_lj solute, e_coul water, e_coul solute, virial, dH/dλ
       // (every pair twice: once from each of its two rows)
@@ -183,6 +201,12 @@ void orc_engine_tiles(const float* xyzq, const int32_t* slot_type, const int32_t
   P.n_types = n_types;
   P.alch_on = alch_on;
   P.sch = dyn_schedule(lambda);
+  static std::vector<float> coul_tab;
+  if (coul_tab.empty()) {
+    coul_tab.resize(8 * DYN_COUL_TAB_N);
+    dyn_coul_tab_build(coul_tab.data());
+  }
+  P.coul_tab = reinterpret_cast<const float4*>(coul_tab.data());
   const float2* lj = reinterpret_cast<const float2*>(lj_table);
   const float2* lse = reinterpret_cast<const float2*>(lj_se_table);
   memset(fx, 0, sizeof(int64_t) * 3 * n_slots);
@@ -469,6 +493,9 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
           const int cx = ((xx % g.ncx[k]) + g.ncx[k]) % g.ncx[k];
           const int cy = ((yy % g.ncy[k]) + g.ncy[k]) % g.ncy[k];
           const int cc = g.col_off[k] + cx * g.ncy[k] + cy;
+          // the column's periodic image as the walk reached it
+          const float sx = (float)((xx - cx) / g.ncx[k]) * L[0];
+          const float sy = (float)((yy - cy) / g.ncy[k]) * L[1];
           for (uint32_t J = col_cl[cc]; J < col_cl[cc + 1]; ++J) {
             float d[3];
             for (int a = 0; a < 3; ++a) {
@@ -481,15 +508,25 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
               const size_t js = (size_t)J * DYN_CL_SIZE + jb;
               float pj[3];
               slot_pos(js, pj);
+              // within rlist of a non-padding i slot (x and y through the
+              // column's image unless the column range spans the box)
               bool keep = false;
+              const float px = pj[0] + sx, py = pj[1] + sy;
               for (int ia = 0; ia < 8 && !keep; ++ia) {
-                const float d0 = dyn_min_image_rint(pi[ia][0] - pj[0], L[0], invL[0]);
-                const float d1 = dyn_min_image_rint(pi[ia][1] - pj[1], L[1], invL[1]);
+                if (gid[I * DYN_CL_SIZE + ia] == DYN_GID_PAD) continue;
+                float d0, d1;
+                if (!full_x && !full_y) {
+                  d0 = pi[ia][0] - px;
+                  d1 = pi[ia][1] - py;
+                } else {
+                  d0 = dyn_min_image_rint(pi[ia][0] - pj[0], L[0], invL[0]);
+                  d1 = dyn_min_image_rint(pi[ia][1] - pj[1], L[1], invL[1]);
+                }
                 const float d2 = dyn_min_image_rint(pi[ia][2] - pj[2], L[2], invL[2]);
                 const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                keep = r2 < rlist2 && dyn_slot_pair_valid(gid[I * DYN_CL_SIZE + ia], gid[js]);
+                keep = r2 < rlist2;
               }
-              if (keep) row.push_back((uint32_t)js | tag);
+              if (keep && gid[js] != DYN_GID_PAD) row.push_back((uint32_t)js | tag);
             }
           }
         }

@@ -54,7 +54,7 @@ def oracle_lib():
             getattr(L, name).argtypes = [f]
         L.orc_fx.restype = C.c_int64
         L.orc_fx.argtypes = [f]
-        for name in ["orc_coul_erfc", "orc_coul_erf"]:
+        for name in ["orc_coul_erfc", "orc_coul_erfc_tab", "orc_coul_erf"]:
             getattr(L, name).argtypes = [f, f, f, C.POINTER(f), C.POINTER(f)]
         L.orc_lj.argtypes = [f, f, f, C.POINTER(f), C.POINTER(f)]
         L.orc_soft_core.argtypes = [f, f, f, f, C.POINTER(f), C.POINTER(f), C.POINTER(f)]
