@@ -86,6 +86,11 @@ def main() -> int:
                          "relaxed structure before the timed MD")
     ap.add_argument("--direct-launch", action="store_true",
                     help="MD modes: launch every kernel directly (no step graphs)")
+    ap.add_argument("--staged-builds", action="store_true",
+                    help="MD modes: list builds beside the pair kernel, serving from the next step")
+    ap.add_argument("--single-stream", action="store_true",
+                    help="every kernel on the engine's stream (no second stream for SPME, "
+                         "corrections and bonded forces)")
     ap.add_argument("--list-buffer", type=float, default=None,
                     help="list buffer in Å (MD modes default %.1f; eval default 0: rebuild "
                          "every step)" % MD_BUFFER)
@@ -124,7 +129,8 @@ def main() -> int:
     s = systems.config1()
     K = (64, 64, 64)
     eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
-                     mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
+                     mode=api.MODE_REFERENCE, list_buffer=args.list_buffer,
+                     single_stream=args.single_stream)
     # the built chain has close contacts: quench them first (untimed), so that
     # both modes run on a physical configuration
     ff = systems.chain_force_field(s)
@@ -137,7 +143,7 @@ def main() -> int:
         st = equilibrate(eng, ff, s, pd0, pw0, args.equil_steps)
         print(f"bench: equilibrated ({args.equil_steps} steps)", file=sys.stderr, flush=True)
         md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=1,
-                    direct_launch=args.direct_launch)
+                    direct_launch=args.direct_launch, staged_builds=args.staged_builds)
         md.set_state(*st)
 
         def run(n):
@@ -254,7 +260,9 @@ def main() -> int:
                      "moved half the buffer"),
             "list_builds_in_timed_steps": builds,
             "steps_in_graph_launches": graph_steps,
-            "launch": "direct" if args.direct_launch else "graphs of 16 steady steps",
+            "launch": ("direct" if args.direct_launch else "graphs of 16 steady steps") +
+                      (", one stream" if args.single_stream else ", two streams") +
+                      (", staged list builds" if args.staged_builds else ""),
             "start": (f"relaxed ({args.relax_steps} steps), then {args.equil_steps} untimed "
                       "Langevin steps at 300 K, γ 10/ps" if md is not None else
                       f"relaxed ({args.relax_steps} steps)"),

@@ -66,7 +66,8 @@ class MdParams(C.Structure):
     _fields_ = [("dt", C.c_float), ("temperature", C.c_float), ("gamma", C.c_float),
                 ("shake_tol", C.c_float), ("seed", C.c_uint64), ("spme_ratio", C.c_uint32),
                 ("nstpcouple", C.c_uint32), ("pressure_target", C.c_float), ("tau_p", C.c_float),
-                ("compressibility", C.c_double), ("direct_launch", C.c_uint32)]
+                ("compressibility", C.c_double), ("direct_launch", C.c_uint32),
+                ("staged_builds", C.c_uint32)]
 
 
 class MdEnergies(C.Structure):
@@ -84,7 +85,8 @@ class EngineParams(C.Structure):
     _fields_ = [("cutoff", C.c_float), ("lj_cutoff", C.c_float), ("beta", C.c_float),
                 ("grid", C.c_uint32 * 3), ("order", C.c_uint32), ("mode", C.c_uint32),
                 ("list_buffer", C.c_float), ("precision", C.c_uint32),
-                ("mesh_spacing", C.c_float), ("item_groups", C.c_uint32)]
+                ("mesh_spacing", C.c_float), ("item_groups", C.c_uint32),
+                ("single_stream", C.c_uint32)]
 
 
 class Energies(C.Structure):
@@ -162,6 +164,8 @@ _sig("dyn_engine_forces_device", C.c_void_p, [_p])
 _sig("dyn_engine_get_energies", C.c_int, [_p, C.POINTER(Energies)])
 _sig("dyn_engine_debug_item_partials", C.c_int, [_p, C.c_void_p, C.c_uint64])
 _sig("dyn_debug_arg_uploads", C.c_uint64, [])
+_sig("dyn_debug_layout_generation", C.c_uint64, [])
+_sig("dyn_debug_capture_refusal", C.c_char_p, [])
 _sig("dyn_engine_debug_inject_status", C.c_int, [_p, C.c_uint32, C.c_uint32, C.c_uint32])
 _sig("dyn_engine_debug_export", C.c_int, [_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                           C.POINTER(C.c_uint64), _p, _p, _p, _p, _p, _p, _p, _p,
@@ -195,7 +199,8 @@ EXPORTED = [
     "dyn_md_get_state", "dyn_md_step", "dyn_md_get_energies",
     "dyn_engine_sync", "dyn_engine_get_forces", "dyn_engine_get_fixed_forces",
     "dyn_engine_forces_device", "dyn_engine_get_energies", "dyn_engine_debug_export",
-    "dyn_engine_debug_item_partials", "dyn_debug_arg_uploads", "dyn_engine_debug_inject_status",
+    "dyn_engine_debug_item_partials", "dyn_debug_arg_uploads", "dyn_debug_layout_generation", "dyn_debug_capture_refusal",
+    "dyn_engine_debug_inject_status",
     "dyn_engine_profile",
     "dyn_engine_profile_read",
     "dyn_batch_create", "dyn_batch_destroy", "dyn_batch_last_error", "dyn_batch_step",
@@ -214,6 +219,17 @@ def _c(a, dt):
 def arg_uploads() -> int:
     """Kernel argument-pack uploads so far in this process."""
     return int(lib.dyn_debug_arg_uploads())
+
+
+def layout_generation() -> int:
+    """The library's device-layout generation (step graphs are recorded and
+    replayed only while it holds still)."""
+    return int(lib.dyn_debug_layout_generation())
+
+
+def capture_refusal() -> str:
+    """Why the last step-graph capture was refused ("" if none was)."""
+    return lib.dyn_debug_capture_refusal().decode(errors="replace")
 
 
 def device_count() -> int:
@@ -378,14 +394,15 @@ class Engine:
 
     def __init__(self, system, cutoff=10.0, lj_cutoff=None, beta=0.3123, grid=(0, 0, 0),
                  order=4, mode=MODE_REFERENCE, list_buffer=0.0, device=0, precision=0,
-                 mesh_spacing=0.0, item_groups=0):
+                 mesh_spacing=0.0, item_groups=0, single_stream=False):
         """precision: PREC_PAIR_F64 | PREC_PME_F64 bits (0: FP32).  mesh_spacing
         > 0 (with grid 0): the SPME grid follows the box (regen_pme).
-        item_groups: j groups per work item of the pair kernel (0: 16)."""
+        item_groups: j groups per work item of the pair kernel (0: 16).
+        single_stream: every kernel on the engine's stream (no second stream)."""
         s = system
         prm = EngineParams(cutoff, cutoff if lj_cutoff is None else lj_cutoff, beta,
                            (C.c_uint32 * 3)(*[int(g) for g in grid]), order, mode, list_buffer,
-                           precision, mesh_spacing, item_groups)
+                           precision, mesh_spacing, item_groups, int(bool(single_stream)))
         self.h = C.c_void_p()
         q = _c(s.q_dyn, np.float32)
         t = _c(s.type_dyn, np.int32)
@@ -530,14 +547,17 @@ class Md:
 
     def __init__(self, engine, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=0,
                  spme_ratio=1, nstpcouple=0, pressure_target=1.0, tau_p=5.0,
-                 compressibility=4.5e-5, direct_launch=False):
+                 compressibility=4.5e-5, direct_launch=False, staged_builds=False):
         """nstpcouple > 0 couples the pressure (C-rescale) every nstpcouple
         steps to pressure_target bar with time constant tau_p ps.
-        direct_launch: launch every kernel directly (no step graphs)."""
+        direct_launch: launch every kernel directly (no step graphs).
+        staged_builds: list builds run beside the pair kernel and serve from
+        the next step on (dyn_md_params.staged_builds)."""
         self.engine = engine
         self.h = C.c_void_p()
         p = MdParams(dt, temperature, gamma, shake_tol, seed, spme_ratio, nstpcouple,
-                     pressure_target, tau_p, compressibility, 1 if direct_launch else 0)
+                     pressure_target, tau_p, compressibility, 1 if direct_launch else 0,
+                     1 if staged_builds else 0)
         arrs = [_c(ff.mass, np.float32),
                 _c(ff.bonds, np.uint32), _c(ff.bond_k_r0, np.float32),
                 _c(ff.angles, np.uint32), _c(ff.angle_k_t0, np.float32),

@@ -174,7 +174,9 @@ class ArgCache {
   }
 
  private:
-  static constexpr size_t kVersions = 4;
+  // a step call uses up to ~6 packs per site (first, steady and last
+  // evaluations, staged or not, times the SPME cadence)
+  static constexpr size_t kVersions = 8;
   struct Copy {
     DevBuf<char> dev;
     std::vector<char> last;

@@ -64,6 +64,8 @@ inline std::string fmt(const char* f, ...) {
 // are still those of live buffers.  A new buffer leaves it: no graph can
 // hold an address that did not exist.
 inline std::atomic<uint64_t> g_layout_gen{1};
+// Why the last step-graph capture was refused (diagnostics).
+inline std::string g_capture_refusal;
 
 // Device buffer owned by a handle.  Size in elements; zero-size buffers hold
 // no allocation (the pointer is null and no kernel may touch it).

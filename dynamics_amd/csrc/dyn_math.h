@@ -296,8 +296,9 @@ DYN_HD void dyn_coul_erfc(float r2, float rinv, float qq, float beta,
 // E = qq/r · erfc(x).  Both functions of x are tabulated on [0, 4] as
 // cubic Hermite pieces (values and derivatives from the closed forms, in
 // double; φ'(x) = −(4/√π) x² e^{−x²}, erfc'(x) = −(2/√π) e^{−x²}) over
-// DYN_COUL_TAB_N intervals: entry 2i holds φ's cubic on interval i, entry
-// 2i+1 erfc's, each as (c0, c1, c2, c3) in the local coordinate t ∈ [0, 1).
+// DYN_COUL_TAB_N intervals: entry i holds φ's cubic on interval i, entry
+// DYN_COUL_TAB_N + i erfc's (so a forces-only evaluation needs the first
+// half only), each as (c0, c1, c2, c3) in the local coordinate t ∈ [0, 1).
 // Interpolation error ≤ h⁴/384·max|f⁗| ≈ 1e-10 (h = 1/128), below f32
 // rounding, so the table reproduces the closed forms to f32 accuracy with
 // one load and three fmas in place of exp(−x²)·erfcx(x).  Evaluation uses
@@ -313,7 +314,7 @@ DYN_HD float dyn_tab_cubic(const float4* tab, int entry_off, float x) {
   int i = (int)fl;
   i = i < 0 ? 0 : (i > DYN_COUL_TAB_N - 1 ? DYN_COUL_TAB_N - 1 : i);
   const float t = u - fl;
-  const float4 c = tab[2 * i + entry_off];
+  const float4 c = tab[entry_off * DYN_COUL_TAB_N + i];
   return dyn_fmaf(dyn_fmaf(dyn_fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
 
@@ -339,7 +340,7 @@ static inline void dyn_coul_tab_build(float* out) {
     for (int which = 0; which < 2; ++which) {
       const double f0 = which ? ef(x0) : phi(x0), f1 = which ? ef(x1) : phi(x1);
       const double d0 = h * (which ? def(x0) : dphi(x0)), d1 = h * (which ? def(x1) : dphi(x1));
-      float* c = out + 4 * (2 * i + which);
+      float* c = out + 4 * (which * DYN_COUL_TAB_N + i);
       c[0] = (float)f0;
       c[1] = (float)d0;
       c[2] = (float)(3.0 * (f1 - f0) - 2.0 * d0 - d1);

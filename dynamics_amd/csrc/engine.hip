@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <type_traits>
@@ -49,13 +50,28 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kPairBlock = 256;       // 4 waves, one work item each at a time
+constexpr uint64_t kPairGridCap = 6144;  // ~4 × the pair kernel's resident workgroups
 constexpr int kColCap = 4096;         // sites per column (LDS sort)
 constexpr int kMaxColsPerClass = 4096;
 constexpr int kNumPart = 9;           // f64 partials per work item of the pair kernel
 
 // status bits
 constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u, kStMask = 16u,
-                   kStPme = 32u;   // an SPME brick list overflowed
+                   kStPme = 32u,   // an SPME brick list overflowed
+                   kStDisp = 64u;  // a site moved half the list buffer since the list's build
+
+// Words of an engine's device flags (dflags).
+constexpr int kFlDisp = 0;       // largest squared displacement this evaluation (float bits)
+constexpr int kFlBuilds = 1;     // list builds so far
+constexpr int kFlBinDone = 2;    // finished workgroups of the binning launch
+constexpr int kFlWalkDone = 3;   // finished workgroups of the walk launch
+constexpr int kFlReady = 4;      // 1: the staging list holds a finished build
+constexpr int kFlSel = 5;        // which copy of cj, cmask and chunks the last evaluation used
+constexpr int kFlCommit = 6;     // 1: this evaluation's preparation kernel took the staged list
+constexpr int kFlFillDone = 8;   // finished workgroups of the fill launch
+constexpr int kFlPrevDisp = 9;   // the previous evaluation's largest displacement (float bits, Å)
+constexpr int kFlGrowth = 10;    // decaying maximum of its growth per evaluation (float bits, Å)
+constexpr int kFlWords = 16;
 
 struct ClassGrid {
   int ncx, ncy;
@@ -72,20 +88,35 @@ struct EngGeom {
   uint32_t n_dyn, n_water;
 };
 
-// Whether this evaluation rebuilds the pair list.  The host asks for a build
+// Whether this evaluation runs a list build.  The host asks for a build
 // (force: the first evaluation, a new box, positions set from the host, no
-// list buffer), or the preparation kernel of this evaluation has found a
-// site that moved half the list buffer since the last build (the largest
-// squared displacement, as float bits, in *disp_bits).  Every build kernel
-// reads the same word the preparation kernel completed, so they all agree;
-// the finalize kernel clears it for the next evaluation.
+// list buffer), or the device decides from words of the engine's flags that
+// hold still during the evaluation — the largest squared displacement since
+// the current list's build (complete once the preparation kernel is done),
+// the recent growth of that displacement and whether the preparation kernel
+// took a staged list:
+// - a build before the pair kernel (staged = 0) when a site has moved half
+//   the list buffer (limit: that distance squared, as float bits);
+// - a staged build (staged = 1) when the largest displacement is within
+//   max(margin_min, 2 × growth) of half the buffer and no staged list was
+//   taken in this evaluation.
+// Every kernel of a build evaluates the same words, so they all agree.
 struct BuildGate {
-  const unsigned int* disp_bits;
-  unsigned int limit;  // bits of (buffer/2)² less a rounding margin
+  const unsigned int* fl;
+  unsigned int limit;
+  float half, margin_min;
+  int staged;
   int force;
+  int unused;  // no implicit padding: argument packs are compared byte for byte
 };
+static_assert(sizeof(BuildGate) == 32, "BuildGate has no padding");
 __device__ __forceinline__ bool build_on(const BuildGate& G) {
-  return G.force != 0 || *G.disp_bits >= G.limit;
+  if (G.force) return true;
+  const unsigned int d2 = G.fl[kFlDisp];
+  if (!G.staged) return d2 >= G.limit;
+  if (G.fl[kFlCommit]) return false;
+  const float growth = __uint_as_float(G.fl[kFlGrowth]);
+  return sqrtf(__uint_as_float(d2)) >= G.half - fmaxf(G.margin_min, 2.0f * growth);
 }
 
 __device__ __forceinline__ float wrap1(float x, float L, float invL) {
@@ -111,61 +142,38 @@ __device__ __forceinline__ uint32_t site_column(const ClassGrid& cg, float x, fl
   return (uint32_t)(cx * cg.ncy + cy);
 }
 
-// Find every site's x–y column (from the positions the preparation kernel
-// wrapped) and take a rank in it; the positions become the reference of
-// the displacement readings.  The rank order depends on atomic timing; the
-// column sort that follows orders each column by (z, site), so the
-// clusters do not.
-__device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __restrict__ wpos,
-                                                EngGeom g, float4* __restrict__ wpos_ref,
-                                                uint32_t* __restrict__ col_count,
-                                                uint2* __restrict__ site_col,
-                                                unsigned int* __restrict__ n_builds) {
-  if (!build_on(G)) return;
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) ++*n_builds;
-  const bool valid = s < g.n_dyn + 4u * g.n_water;
-  uint32_t col = 0;
-  if (valid) {
-    const float4 w = wpos[s];
-    wpos_ref[s] = w;
-    const bool solute = s < g.n_dyn;
-    const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
-    const ClassGrid cg = solute ? g.cg[0] : (oxy ? g.cg[1] : g.cg[2]);
-    col = (uint32_t)cg.col_off + site_column(cg, w.x, w.y);
-  }
-  // one atomic per distinct column of the wave
-  const uint32_t rank = wave_rank(col, valid, col_count);
-  if (valid) site_col[s] = make_uint2(col, rank);
+// The last workgroup of a launch to finish: every workgroup calls this once,
+// after its last write the next step reads; the one that returns true sees
+// every other workgroup's writes (release fence before the count, acquire
+// after) and resets the counter for the next launch.
+__device__ __forceinline__ bool last_workgroup(unsigned int* __restrict__ done) {
+  __shared__ unsigned int s_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) s_last = atomicAdd(done, 1u) == gridDim.x - 1u ? 1u : 0u;
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  if (threadIdx.x == 0) *done = 0u;
+  return true;
 }
 
-__device__ __forceinline__ void k_eng_bin_scatter(BuildGate G, const uint2* __restrict__ site_col,
-                                  const uint32_t* __restrict__ col_start, uint32_t n,
-                                  uint32_t* __restrict__ items) {
-  if (!build_on(G)) return;
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint2 cr = site_col[s];
-  const uint32_t k = col_start[cr.x] + cr.y;
-  if (k < n) items[k] = s;
-}
-
-// Single-workgroup exclusive scans of two uint32 sequences of length n (n
-// from *n_dev when non-null).  MODE 0 (columns): a = site count, b =
-// ceil(a/8) clusters.  MODE 1 (rows): a = entries, b = work items.  Writes
-// out_x[0..n] with out_x[n] = total, the totals to tot_x and max(a) to *max_a.
-constexpr int kScanBlock = 1024;
-template <int MODE>
-__device__ __forceinline__ void
-k_eng_scan2(BuildGate G, const uint32_t* __restrict__ in_a, const uint32_t* __restrict__ in_b,
-            uint32_t n_host, const uint32_t* __restrict__ n_dev, uint32_t* __restrict__ out_a,
-            uint32_t* __restrict__ out_b, uint32_t* __restrict__ tot_a,
-            uint32_t* __restrict__ tot_b, uint32_t* __restrict__ max_a) {
-  __shared__ uint32_t wa[kScanBlock / kWave], wb[kScanBlock / kWave], wm[kScanBlock / kWave];
-  if (!build_on(G)) return;
-  const uint32_t n = n_dev ? *n_dev : n_host;
+// Exclusive scans of two uint32 sequences of length n by one workgroup of NT
+// threads (after last_workgroup's acquire fence, plain loads see the other
+// workgroups' writes).  MODE 0 (columns): a = site count, b = ceil(a/8) clusters.  MODE
+// 1 (rows): a = entries, b = work items.  Writes out_x[0..n] with out_x[n] =
+// total, the totals to tot_x and max(a) to *max_a (each when non-null).
+template <int MODE, int NT>
+__device__ __forceinline__ void scan2_block(const uint32_t* __restrict__ in_a,
+                                            const uint32_t* __restrict__ in_b, uint32_t n,
+                                            uint32_t* __restrict__ out_a,
+                                            uint32_t* __restrict__ out_b,
+                                            uint32_t* __restrict__ tot_a,
+                                            uint32_t* __restrict__ tot_b,
+                                            uint32_t* __restrict__ max_a) {
+  __shared__ uint32_t wa[NT / kWave], wb[NT / kWave], wm[NT / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  const uint32_t per = (n + kScanBlock - 1) / kScanBlock;
+  const uint32_t per = (n + NT - 1) / NT;
   const uint32_t b0 = t * per, e0 = min(n, b0 + per);
   auto val_b = [&](uint32_t i, uint32_t a) {
     return MODE == 0 ? (a + DYN_CL_SIZE - 1) / DYN_CL_SIZE : in_b[i];
@@ -193,7 +201,7 @@ k_eng_scan2(BuildGate G, const uint32_t* __restrict__ in_a, const uint32_t* __re
   if (lane == 0) wm[w] = im;
   __syncthreads();
   uint32_t ba = 0, bb = 0, ta = 0, tb = 0, tm = 0;
-  for (int k = 0; k < kScanBlock / kWave; ++k) {
+  for (int k = 0; k < NT / kWave; ++k) {
     if (k < w) {
       ba += wa[k];
       bb += wb[k];
@@ -205,13 +213,13 @@ k_eng_scan2(BuildGate G, const uint32_t* __restrict__ in_a, const uint32_t* __re
   uint32_t ra = ba + ia - sa, rb = bb + ib - sb;
   for (uint32_t i = b0; i < e0; ++i) {
     const uint32_t a = in_a[i];
-    out_a[i] = ra;
+    if (out_a) out_a[i] = ra;
     out_b[i] = rb;
     ra += a;
     rb += val_b(i, a);
   }
   if (t == 0) {
-    out_a[n] = ta;
+    if (out_a) out_a[n] = ta;
     out_b[n] = tb;
     if (tot_a) *tot_a = ta;
     if (tot_b) *tot_b = tb;
@@ -219,17 +227,69 @@ k_eng_scan2(BuildGate G, const uint32_t* __restrict__ in_a, const uint32_t* __re
   }
 }
 
+// Find every site's x–y column (from the positions the preparation kernel
+// wrapped), take a rank in it and write the site into the column's slice of
+// items (col_slice entries per column; the column sort flags a column that
+// would hold more, and the call is repeated with longer slices); the
+// positions become the reference of the
+// displacement readings.  The rank order depends on atomic timing; the
+// column sort that follows orders each column by (z, site), so the
+// clusters do not.  The last workgroup numbers the clusters: col_cl_start
+// (first cluster of each column) and their total *n_cl.
+constexpr int kBinBlock = 256;
+__device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __restrict__ wpos,
+                                                EngGeom g, float4* __restrict__ wpos_ref,
+                                                uint32_t* __restrict__ col_count,
+                                                uint32_t* __restrict__ items,
+                                                unsigned int* __restrict__ n_builds,
+                                                unsigned int* __restrict__ done, uint32_t n_cols,
+                                                uint32_t col_slice,
+                                                uint32_t* __restrict__ col_cl_start,
+                                                uint32_t* __restrict__ n_cl) {
+  if (!build_on(G)) return;
+  const uint32_t n = g.n_dyn + 4u * g.n_water;
+  for (uint32_t s0 = blockIdx.x * kBinBlock; s0 < n; s0 += gridDim.x * kBinBlock) {
+    const uint32_t s = s0 + threadIdx.x;
+    if (s == 0) ++*n_builds;
+    const bool valid = s < n;
+    uint32_t col = 0;
+    if (valid) {
+      const float4 w = wpos[s];
+      wpos_ref[s] = w;
+      const bool solute = s < g.n_dyn;
+      const bool oxy = !solute && ((s - g.n_dyn) & 3u) == 0u;
+      // the class grid's fields by selects (a runtime-indexed struct copy
+      // would go through scratch memory)
+      ClassGrid cg;
+      cg.wx = solute ? g.cg[0].wx : (oxy ? g.cg[1].wx : g.cg[2].wx);
+      cg.wy = solute ? g.cg[0].wy : (oxy ? g.cg[1].wy : g.cg[2].wy);
+      cg.ncx = solute ? g.cg[0].ncx : (oxy ? g.cg[1].ncx : g.cg[2].ncx);
+      cg.ncy = solute ? g.cg[0].ncy : (oxy ? g.cg[1].ncy : g.cg[2].ncy);
+      const int col_off = solute ? g.cg[0].col_off : (oxy ? g.cg[1].col_off : g.cg[2].col_off);
+      col = (uint32_t)col_off + site_column(cg, w.x, w.y);
+    }
+    // one atomic per distinct column of the wave
+    const uint32_t rank = wave_rank(col, valid, col_count);
+    if (valid && rank < col_slice) items[(size_t)col * col_slice + rank] = s;
+  }
+  if (!last_workgroup(done)) return;
+  scan2_block<0, kBinBlock>(col_count, nullptr, n_cols, nullptr, col_cl_start, nullptr, n_cl,
+                            nullptr);
+}
+
 struct SlotArgs {
   const float4* wpos;
   const int32_t* ltype;   // per site
   const int32_t* wmol;    // per site: water molecule or −1
-  int none_type;
   float4* xyzq;
   int32_t* slot_type;
   int32_t* slot_gid;
   int32_t* slot_site;
   uint32_t* slot_of;
+  int none_type;
+  int unused;  // no implicit padding: argument packs are compared byte for byte
 };
+static_assert(sizeof(SlotArgs) == 72, "SlotArgs has no padding");
 
 __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
   const int32_t m = wmol[s];
@@ -243,18 +303,18 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
 constexpr int kSortBlock = 256;
 constexpr unsigned kBuildGrid = 1024;  // workgroups of the grid-stride build kernels
 __device__ __forceinline__ void
-colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
-            const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
+colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ items,
+            uint32_t col_slice, const uint32_t* __restrict__ col_cl_start,
             const int* __restrict__ col_class, const SlotArgs& S, uint8_t* __restrict__ cl_class,
             float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
             unsigned int* __restrict__ status, unsigned long long* key) {
   const uint32_t n = col_count[c];
   __syncthreads();
   if (threadIdx.x == 0 && n) col_count[c] = 0;
-  const uint32_t b = col_start[c];
+  const size_t b = (size_t)c * col_slice;
   const uint32_t cl0 = col_cl_start[c], ncl = col_cl_start[c + 1] - cl0;
   if (ncl == 0) return;
-  if (n > (uint32_t)kColCap || b + n > n_sites) {
+  if (n > col_slice) {
     if (threadIdx.x == 0) atomicOr(status, kStColumn);
     return;
   }
@@ -324,8 +384,8 @@ colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __rest
 }
 
 __device__ __forceinline__ void
-k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ col_start,
-              const uint32_t* __restrict__ items, const uint32_t* __restrict__ col_cl_start,
+k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ items,
+              uint32_t col_slice, const uint32_t* __restrict__ col_cl_start,
               const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
               float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
               unsigned int* __restrict__ status, uint32_t n_cols) {
@@ -333,7 +393,7 @@ k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __r
   if (!build_on(G)) return;
   // a capped grid walks the columns (a closed gate then costs few workgroups)
   for (uint32_t c = blockIdx.x; c < n_cols; c += gridDim.x) {
-    colsort_one(c, col_count, col_start, items, col_cl_start, col_class, S, cl_class, bb_c, bb_h,
+    colsort_one(c, col_count, items, col_slice, col_cl_start, col_class, S, cl_class, bb_c, bb_h,
                 n_sites, status, key);
     __syncthreads();  // key[] is reused by the next column
   }
@@ -362,17 +422,41 @@ struct ListArgs {
   uint32_t null_slot;                // padding j slot (exclusion group: padding)
   uint32_t* longest;                 // longest row that overflowed its slice
   unsigned int* status;
+  uint8_t* xb_tmp;                   // per entry of ent_tmp: its exclusion byte
+  unsigned int* done;                // workgroups of the walk that have finished
+  uint32_t* tot;                     // groups, work items, longest row (from the scan)
 };
 
 // LDS of the row walk: a batch of up to 64 columns (first cluster, range in
 // the flattened candidate sequence), then the batch's candidate clusters
 // that passed the box test.
+// For a solute row, also the exclusion partners of I's sites as a hash
+// table (linear probing, at most half full): key slot << 8 | the bits of
+// the i slots the partner is excluded from.
+constexpr int kExHash = 1024;
+constexpr uint32_t kExEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kExPartners = kExHash / 2;  // partners of one cluster's sites
 struct WalkLds {
   uint32_t js[kWave], beg[kWave], end[kWave];
   float sx[kWave], sy[kWave];  // the column's periodic image offset seen from I
   uint32_t okJ[kWave];
   float2 okS[kWave];
+  uint32_t ex[kExHash];
 };
+__device__ __forceinline__ uint32_t ex_hash(uint32_t slot) {
+  return (slot * 2654435761u) >> (32 - 10);  // 10 bits: kExHash = 1024
+}
+static_assert(kExHash == 1024, "ex_hash takes 10 bits");
+
+// Bits (i slot ia → bit ia) of the i slots whose site is a 1-2, 1-3 or 1-4
+// partner of j slot `slot`'s site.
+__device__ __forceinline__ uint32_t ex_bits(const uint32_t* ex, uint32_t slot) {
+  for (uint32_t h = ex_hash(slot);; h = (h + 1) & (kExHash - 1)) {
+    const uint32_t v = ex[h];
+    if (v == kExEmpty) return 0u;
+    if ((v >> 8) == slot) return v & 0xFFu;
+  }
+}
 
 // Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
 // order (class, column x, column y, cluster), keep those whose bounding box
@@ -384,7 +468,11 @@ struct WalkLds {
 // column range spans the whole box).  Slots near only their own water
 // molecule's sites may be listed: the pair kernel masks such pairs.  Each
 // class segment is padded to a multiple of 8 with the null slot.  The row
-// goes into its fixed-capacity slice of ent_tmp; counts are in groups of 8.
+// goes into its fixed-capacity slice of ent_tmp, each entry's exclusion
+// byte (solute–solute entries: bit ia set when the j site is a 1-2, 1-3 or
+// 1-4 partner of i slot ia's site) into the same place of xb_tmp; counts
+// are in groups of 8.  The last workgroup to finish scans the row counts:
+// row_ptr and chunk_ptr, and the totals.
 constexpr int kListBlock = 256;  // four waves, each on its own rows
 constexpr int kListWaves = kListBlock / kWave;
 
@@ -399,7 +487,8 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ void
 k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
-                uint32_t* __restrict__ row_chunks) {
+                uint32_t* __restrict__ row_chunks, uint32_t* __restrict__ row_ptr,
+                uint32_t* __restrict__ chunk_ptr) {
   __shared__ WalkLds Ls[kListWaves];
   if (!build_on(A.gate)) return;
   WalkLds& L = Ls[threadIdx.x / kWave];
@@ -412,14 +501,68 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
   for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
        I += gridDim.x * kListWaves) {
     uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
+    uint8_t* xb = A.xb_tmp + (size_t)I * A.row_cap;
     const int ci = A.cl_class[I];
     const float4 c = A.bb_c[I], h = A.bb_h[I];
-    float4 pI[DYN_CL_SIZE];
-    bool iv[DYN_CL_SIZE];  // i slot holds a site
+    float pIx[DYN_CL_SIZE], pIy[DYN_CL_SIZE], pIz[DYN_CL_SIZE];
+    uint32_t iv = 0;  // bit u: i slot u holds a site
 #pragma unroll
     for (int u = 0; u < DYN_CL_SIZE; ++u) {
-      pI[u] = A.xyzq[I * DYN_CL_SIZE + u];
-      iv[u] = A.slot_gid[I * DYN_CL_SIZE + u] != DYN_GID_PAD;
+      const float4 p = A.xyzq[I * DYN_CL_SIZE + u];
+      pIx[u] = p.x;
+      pIy[u] = p.y;
+      pIz[u] = p.z;
+      if (A.slot_gid[I * DYN_CL_SIZE + u] != DYN_GID_PAD) iv |= 1u << u;
+    }
+    // solute row: the exclusion partners of its sites, by slot
+    uint32_t n_ex = 0;
+    if (ci == DYN_CLS_SOLUTE) {
+      uint32_t len = 0, ex0 = 0;
+      if (lane < DYN_CL_SIZE) {
+        const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
+        if (sa >= 0) {
+          ex0 = A.ex_ptr[sa];
+          len = A.ex_ptr[sa + 1] - ex0;
+        }
+      }
+      uint32_t incl = len;
+      for (int off = 1; off < DYN_CL_SIZE; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+      }
+      n_ex = (uint32_t)__shfl(incl, DYN_CL_SIZE - 1, kWave);
+      if (n_ex > kExPartners) {
+        if (lane == 0) atomicOr(A.status, kStMask);
+        n_ex = 0;
+      }
+      for (uint32_t k = lane; k < (uint32_t)kExHash; k += kWave) L.ex[k] = kExEmpty;
+      wave_sync();
+      // lane t < n_ex inserts partner t (of i slot ia: beg[ia] ≤ t < beg[ia+1])
+      const uint32_t beg_l = incl - len;
+      for (uint32_t t0 = 0; t0 < n_ex; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        int ia = 0;
+        for (int u = 1; u < DYN_CL_SIZE; ++u)
+          if ((uint32_t)__shfl(beg_l, u, kWave) <= t) ia = u;
+        const uint32_t e0 = (uint32_t)__shfl(ex0, ia, kWave);
+        const uint32_t b0 = (uint32_t)__shfl(beg_l, ia, kWave);
+        if (t < n_ex) {
+          const uint32_t slot = A.slot_of[A.ex_idx[e0 + (t - b0)]];
+          const uint32_t bit = 1u << ia;
+          for (uint32_t hh = ex_hash(slot);; hh = (hh + 1) & (kExHash - 1)) {
+            uint32_t cur = L.ex[hh];
+            if (cur == kExEmpty) {
+              cur = atomicCAS(&L.ex[hh], kExEmpty, (slot << 8) | bit);
+              if (cur == kExEmpty) break;
+            }
+            if ((cur >> 8) == slot) {
+              atomicOr(&L.ex[hh], bit);
+              break;
+            }
+          }
+        }
+      }
+      wave_sync();
     }
     uint32_t n_out = 0;
     for (int cls = 0; cls < 3; ++cls) {
@@ -472,21 +615,33 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
         L.sx[lane] = sx;
         L.sy[lane] = sy;
         wave_sync();
+        // candidate t of the batch (t < tot): its cluster, the image shift of
+        // its column and its bounding box; the next 64 candidates' are
+        // loaded while the current ones are tested
+        auto cand = [&](uint32_t t, uint32_t& J, float2& sh, float4& cJ, float4& hJ) {
+          int lo = 0, hi = kWave - 1;  // first column whose range ends after t
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L.end[mid] > t) hi = mid;
+            else lo = mid + 1;
+          }
+          J = L.js[lo] + (t - L.beg[lo]);
+          sh = make_float2(L.sx[lo], L.sy[lo]);
+          cJ = A.bb_c[J];
+          hJ = A.bb_h[J];
+        };
+        uint32_t Jn = 0;
+        float2 shn = make_float2(0.f, 0.f);
+        float4 cJn = make_float4(0.f, 0.f, 0.f, 0.f), hJn = cJn;
+        if ((uint32_t)lane < tot) cand(lane, Jn, shn, cJn, hJn);
         for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
           const uint32_t t = t0 + lane;
           bool ok = t < tot;
-          uint32_t J = 0;
-          float2 sh = make_float2(0.f, 0.f);
+          const uint32_t J = Jn;
+          const float2 sh = shn;
+          const float4 cJ = cJn, hJ = hJn;
+          if (t + kWave < tot) cand(t + kWave, Jn, shn, cJn, hJn);
           if (ok) {
-            int lo = 0, hi = kWave - 1;  // first column whose range ends after t
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (L.end[mid] > t) hi = mid;
-              else lo = mid + 1;
-            }
-            J = L.js[lo] + (t - L.beg[lo]);
-            sh = make_float2(L.sx[lo], L.sy[lo]);
-            const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
             float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
             dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
             dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
@@ -506,43 +661,62 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
           // eight candidate clusters per round: lane jb + 8·c tests j slot
           // jb of candidate c against all eight i slots (held in registers),
           // so the kept slots come out candidate by candidate, each in slot
-          // order, as a one-candidate-at-a-time walk would write them
-          for (uint32_t k0 = 0; k0 < n_ok; k0 += kWave / DYN_CL_SIZE) {
+          // order, as a one-candidate-at-a-time walk would write them; the
+          // next round's slots are loaded while this round's are tested
+          const int jj = lane & (DYN_CL_SIZE - 1);
+          auto slots = [&](uint32_t k0, uint32_t& Jr, float2& shr, float4& pj, bool& jv) {
             const uint32_t kk = k0 + (uint32_t)(lane / DYN_CL_SIZE);
-            const int jj = lane & (DYN_CL_SIZE - 1);
-            bool keep = false;
-            uint32_t J = 0;
+            jv = false;
             if (kk < n_ok) {
-              J = L.okJ[kk];
-              const float2 sh = L.okS[kk];
-              const float4 pj = A.xyzq[J * DYN_CL_SIZE + jj];
-              const bool jv = A.slot_gid[J * DYN_CL_SIZE + jj] != DYN_GID_PAD;
+              Jr = L.okJ[kk];
+              shr = L.okS[kk];
+              pj = A.xyzq[Jr * DYN_CL_SIZE + jj];
+              jv = A.slot_gid[Jr * DYN_CL_SIZE + jj] != DYN_GID_PAD;
+            }
+          };
+          uint32_t Jr_n = 0;
+          float2 shr_n = make_float2(0.f, 0.f);
+          float4 pj_n = make_float4(0.f, 0.f, 0.f, 0.f);
+          bool jv_n = false;
+          slots(0, Jr_n, shr_n, pj_n, jv_n);
+          for (uint32_t k0 = 0; k0 < n_ok; k0 += kWave / DYN_CL_SIZE) {
+            const uint32_t Jr = Jr_n;
+            const float2 shr = shr_n;
+            const float4 pj = pj_n;
+            const bool jv = jv_n;
+            if (k0 + kWave / DYN_CL_SIZE < n_ok) slots(k0 + kWave / DYN_CL_SIZE, Jr_n, shr_n, pj_n, jv_n);
+            bool keep = false;
+            if (jv) {
               if (!full_x && !full_y) {
-                const float px = pj.x + sh.x, py = pj.y + sh.y;
+                const float px = pj.x + shr.x, py = pj.y + shr.y;
 #pragma unroll
                 for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                  const float d0 = pI[u].x - px;
-                  const float d1 = pI[u].y - py;
-                  const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
+                  const float d0 = pIx[u] - px;
+                  const float d1 = pIy[u] - py;
+                  const float d2 = dyn_min_image_rint(pIz[u] - pj.z, A.g.L[2], A.g.invL[2]);
                   const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                  keep = keep || (r2 < A.rlist2 && iv[u]);
+                  keep = keep || (r2 < A.rlist2 && ((iv >> u) & 1u));
                 }
               } else {
 #pragma unroll
                 for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                  const float d0 = dyn_min_image_rint(pI[u].x - pj.x, A.g.L[0], A.g.invL[0]);
-                  const float d1 = dyn_min_image_rint(pI[u].y - pj.y, A.g.L[1], A.g.invL[1]);
-                  const float d2 = dyn_min_image_rint(pI[u].z - pj.z, A.g.L[2], A.g.invL[2]);
+                  const float d0 = dyn_min_image_rint(pIx[u] - pj.x, A.g.L[0], A.g.invL[0]);
+                  const float d1 = dyn_min_image_rint(pIy[u] - pj.y, A.g.L[1], A.g.invL[1]);
+                  const float d2 = dyn_min_image_rint(pIz[u] - pj.z, A.g.L[2], A.g.invL[2]);
                   const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                  keep = keep || (r2 < A.rlist2 && iv[u]);
+                  keep = keep || (r2 < A.rlist2 && ((iv >> u) & 1u));
                 }
               }
-              keep = keep && jv;
             }
             const unsigned long long kb = __ballot(keep);
             if (keep) {
               const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
-              if (pos < A.row_cap) ent[pos] = (J * DYN_CL_SIZE + jj) | tag;
+              const uint32_t slot = Jr * DYN_CL_SIZE + jj;
+              const uint32_t xbits = (n_ex && cls == DYN_CLS_SOLUTE) ? ex_bits(L.ex, slot) : 0u;
+              if (pos < A.row_cap) {
+                ent[pos] = slot | tag;
+                xb[pos] = (uint8_t)xbits;
+              }
             }
             n_out += (uint32_t)__popcll(kb);
           }
@@ -550,7 +724,10 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
         }
       }
       const uint32_t pad = (DYN_CL_SIZE - n_out % DYN_CL_SIZE) % DYN_CL_SIZE;
-      if ((uint32_t)lane < pad && n_out + lane < A.row_cap) ent[n_out + lane] = A.null_slot | tag;
+      if ((uint32_t)lane < pad && n_out + lane < A.row_cap) {
+        ent[n_out + lane] = A.null_slot | tag;
+        xb[n_out + lane] = 0;
+      }
       n_out += pad;
     }
     if (lane == 0) {
@@ -563,16 +740,41 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
       row_chunks[I] = (m + A.chunk - 1) / A.chunk;
     }
   }
+  if (!last_workgroup(A.done)) return;
+  scan2_block<1, kListBlock>(row_cnt, row_chunks, n_cl, row_ptr, chunk_ptr, A.tot + 0, A.tot + 1,
+                             A.tot + 2);
 }
 
-// Pass 1: compact the rows into cj (groups of 8 entries) and write the work
-// items (I, first group).
+// The two copies of the per-group arrays (the pair kernel reads the one
+// flags[kFlSel] names while a staged build fills the other).
+struct ListCopies {
+  uint32_t *cj0, *cj1;
+  unsigned long long *cmask0, *cmask1;
+  uint2 *chunks0, *chunks1;
+  unsigned int* fl;
+  int staged;  // this build fills the staging list
+  int unused;  // no implicit padding: argument packs are compared byte for byte
+};
+static_assert(sizeof(ListCopies) == 64, "ListCopies has no padding");
+
+// Pass 1: compact the rows into cj (groups of 8 entries), write the work
+// items (I, first group) and, for the solute–solute groups (solute rows:
+// solute clusters are numbered first; the solute segment comes first in
+// every row), the 64-bit masks: bit ia + 8·jb of a group is cleared when the
+// sites of i slot ia and of the group's entry jb are a 1-2, 1-3 or 1-4 pair
+// (entry jb's exclusion byte, shifted into place).
 __device__ __forceinline__ void
 k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
-                const uint32_t* __restrict__ chunk_ptr, uint32_t* __restrict__ cj,
-                uint2* __restrict__ chunks) {
+                const uint32_t* __restrict__ chunk_ptr, ListCopies C) {
   if (!build_on(A.gate)) return;
+  // the copy this evaluation's pair kernel reads (a build before it) or the
+  // other one (a staged build, current from the next evaluation on; none
+  // runs in an evaluation that took a staged list)
+  const bool second = ((C.fl[kFlSel] ^ C.fl[kFlCommit]) != 0u) != (C.staged != 0);
+  uint32_t* __restrict__ cj = second ? C.cj1 : C.cj0;
+  unsigned long long* __restrict__ cmask = second ? C.cmask1 : C.cmask0;
+  uint2* __restrict__ chunks = second ? C.chunks1 : C.chunks0;
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x & (kWave - 1);
   for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
@@ -583,6 +785,13 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
       const uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
       uint32_t* out = cj + base * DYN_CL_SIZE;
       for (uint32_t k = lane; k < m * DYN_CL_SIZE; k += kWave) out[k] = ent[k];
+      if (A.cl_class[I] == DYN_CLS_SOLUTE) {
+        // row_cap is a multiple of 64, so each group's 8 bytes are aligned
+        const unsigned long long* xb =
+            reinterpret_cast<const unsigned long long*>(A.xb_tmp + (size_t)I * A.row_cap);
+        for (uint32_t g = lane; g < m; g += kWave)
+          if ((ent[g * DYN_CL_SIZE] >> DYN_JCLS_SHIFT) == DYN_CLS_SOLUTE) cmask[base + g] = ~xb[g];
+      }
     } else if (lane == 0) {
       atomicOr(A.status, kStEntries);
     }
@@ -594,99 +803,8 @@ k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
       atomicOr(A.status, kStChunks);
     }
   }
-}
-
-// Pass 2 (solute rows only; solute clusters are numbered first, and the
-// solute segment comes first in every row): the 64-bit masks of the
-// solute–solute groups.  Bit ia + 8·jb of a group is cleared when the sites
-// in i slot ia and the group's j entry jb are a 1-2, 1-3 or 1-4 pair; the
-// lanes walk the i sites' exclusion lists.
-constexpr int kMaskCap = 2048;  // solute entries of one solute row
-constexpr int kMaskHash = 2 * kMaskCap;
-constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
-__device__ __forceinline__ uint32_t mask_hash(uint32_t slot) {
-  return (slot * 2654435761u) >> (32 - 12);  // 12 bits: kMaskHash = 4096
-}
-static_assert(kMaskHash == 4096, "mask_hash takes 12 bits");
-__device__ __forceinline__ void
-k_eng_list_mask(ListArgs A, const uint32_t* __restrict__ ent_tmp,
-                const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
-                const uint32_t* __restrict__ col_cl_start, unsigned long long* __restrict__ cmask) {
-  // the row's solute entries as an LDS hash table (slot → entry index,
-  // linear probing, at most half full), so each exclusion finds its entry
-  // in a few probes
-  __shared__ uint32_t h_key[kMaskHash];
-  __shared__ uint16_t h_val[kMaskHash];
-  __shared__ unsigned long long s_mask[kMaskCap / DYN_CL_SIZE];
-  __shared__ uint32_t s_beg[DYN_CL_SIZE + 1], s_ex0[DYN_CL_SIZE];
-  if (!build_on(A.gate)) return;
-  const uint32_t n_sol = col_cl_start[A.g.cg[1].col_off];
-  const int lane = threadIdx.x;
-  for (uint32_t I = blockIdx.x; I < n_sol; I += gridDim.x) {
-    const uint32_t m = row_cnt[I] * DYN_CL_SIZE;
-    const uint64_t base = row_ptr[I];
-    if (base + row_cnt[I] > A.cap_entries) continue;  // the fill pass has flagged it
-    const uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
-    uint32_t nss = 0;
-    for (uint32_t k0 = 0; k0 < m; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      const unsigned long long bal = __ballot(k < m && (ent[k] >> DYN_JCLS_SHIFT) == DYN_CLS_SOLUTE);
-      nss += (uint32_t)__popcll(bal);
-      if (bal != ~0ull) break;
-    }
-    if (nss > (uint32_t)kMaskCap) {
-      if (lane == 0) atomicOr(A.status, kStMask);
-      continue;
-    }
-    for (uint32_t h = lane; h < (uint32_t)kMaskHash; h += kWave) h_key[h] = kHashEmpty;
-    for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) s_mask[g] = ~0ull;
-    __syncthreads();
-    for (uint32_t k = lane; k < nss; k += kWave) {
-      const uint32_t slot = ent[k] & DYN_JSLOT_MASK;
-      if (slot == A.null_slot) continue;  // padding: no exclusion partner
-      uint32_t h = mask_hash(slot);
-      while (atomicCAS(&h_key[h], kHashEmpty, slot) != kHashEmpty) h = (h + 1) & (kMaskHash - 1);
-      h_val[h] = (uint16_t)k;
-    }
-    uint32_t len = 0, ex0 = 0;
-    if (lane < DYN_CL_SIZE) {
-      const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
-      if (sa >= 0) {
-        ex0 = A.ex_ptr[sa];
-        len = A.ex_ptr[sa + 1] - ex0;
-      }
-    }
-    uint32_t incl = len;
-    for (int off = 1; off < DYN_CL_SIZE; off <<= 1) {
-      const uint32_t v = __shfl_up(incl, off, kWave);
-      if (lane >= off) incl += v;
-    }
-    if (lane < DYN_CL_SIZE) {
-      s_beg[lane] = incl - len;
-      s_ex0[lane] = ex0;
-    }
-    if (lane == DYN_CL_SIZE - 1) s_beg[DYN_CL_SIZE] = incl;
-    __syncthreads();
-    const uint32_t tot = s_beg[DYN_CL_SIZE];
-    for (uint32_t t = lane; t < tot; t += kWave) {
-      int ia = 0;
-      while (ia + 1 < DYN_CL_SIZE && s_beg[ia + 1] <= t) ++ia;
-      const uint32_t sb = A.ex_idx[s_ex0[ia] + (t - s_beg[ia])];
-      const uint32_t slot = A.slot_of[sb];
-      for (uint32_t h = mask_hash(slot);; h = (h + 1) & (kMaskHash - 1)) {
-        const uint32_t key = h_key[h];
-        if (key == slot) {
-          const uint32_t k = h_val[h];
-          atomicAnd(&s_mask[k / DYN_CL_SIZE], ~(1ull << (ia + DYN_CL_SIZE * (k % DYN_CL_SIZE))));
-          break;
-        }
-        if (key == kHashEmpty) break;  // the partner is not in this row
-      }
-    }
-    __syncthreads();
-    for (uint32_t g = lane; g < nss / DYN_CL_SIZE; g += kWave) cmask[base + g] = s_mask[g];
-    __syncthreads();
-  }
+  // a staged build is complete when the launch's last workgroup is done
+  if (C.staged && last_workgroup(C.fl + kFlFillDone) && threadIdx.x == 0) C.fl[kFlReady] = 1u;
 }
 
 struct PairArgs {
@@ -696,9 +814,15 @@ struct PairArgs {
   const int32_t* slot_gid;
   const uint8_t* cl_class;
   const uint32_t* row_ptr;   // groups
-  const uint32_t* cj;        // 8 entries per group
+  const uint32_t* cj;        // 8 entries per group (copy 0; cj2: copy 1)
   const unsigned long long* cmask;
   const uint2* chunks;
+  const uint32_t* cj2;
+  const unsigned long long* cmask2;
+  const uint2* chunks2;
+  const unsigned int* fl;    // the engine's flags: the copy in use, the largest displacement
+  int staged;                // staged list builds: a stale current list voids the evaluation
+  unsigned int hard_limit;   // half the list buffer, squared (float bits)
   const uint32_t* n_chunks;
   const float2* lj_table;    // (c6, c12) per type pair
   const float2* lj_se;       // (σ, ε) per type pair, for soft-core
@@ -710,7 +834,7 @@ struct PairArgs {
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
   uint32_t chunk;            // j groups per work item
-  const unsigned int* status;  // a list-build overflow leaves the list unusable
+  unsigned int* status;      // a list-build overflow leaves the list unusable
 };
 
 __device__ __forceinline__ float xor_sum(float v, int m) { return v + __shfl_xor(v, m, kWave); }
@@ -814,13 +938,25 @@ __device__ __forceinline__ void tile_entry(const PairArgs& A, const float pi[4],
   else T.n_c += h;
 }
 
-// One wave per work item (≤ A.chunk groups of one row); lane ia + 8·jb
-// holds the pair (i slot 8I+ia, j entry jb of the group).  The next group's
-// j data is loaded before the current group is evaluated.  A work item
-// stores its 8 i-slot forces in its own row of fitem (no atomics) and its
-// energies, virial and pair counts as one partial per value; the grid has a
-// wave per work item, so the hardware's wave dispatch balances the load and
-// every sum keeps a fixed order.
+// LDS of the FP32 pair kernel: the erfc Coulomb table, then the LJ
+// (c6, c12) table of n_types² entries.  Per-lane table reads from LDS are
+// counted apart from the global loads, so the j data prefetched for the next
+// group stays in flight while a group reads its table values (a global table
+// read would make the wave wait for every load issued before it).
+constexpr size_t kPairTabEntries = 2 * DYN_COUL_TAB_N;  // float4
+__host__ __device__ constexpr size_t pair_lds_bytes(int n_types, bool energy) {
+  return (energy ? kPairTabEntries : DYN_COUL_TAB_N) * 16 + (size_t)n_types * (size_t)n_types * 8;
+}
+
+// One wave per work item at a time (≤ A.chunk groups of one row); lane
+// ia + 8·jb holds the pair (i slot 8I+ia, j entry jb of the group).  The next
+// group's j data is loaded before the current group is evaluated.  A work
+// item stores its 8 i-slot forces in its own row of fitem (no atomics) and
+// its energies, virial and pair counts as one partial per value, so every
+// sum keeps a fixed order whichever wave runs the item; the grid holds a few
+// times as many workgroups as fit on the device at once, each copying the
+// tables to LDS once and taking items in a grid stride, so the hardware's
+// workgroup dispatch still balances the load.
 template <bool ENERGY, bool F64>
 __device__ __forceinline__ void
 k_eng_pairs(PairArgs A) {
@@ -828,6 +964,31 @@ k_eng_pairs(PairArgs A) {
   const int lane = threadIdx.x & (kWave - 1);
   const int ia = lane & 7, jb = lane >> 3;
   if (*A.status) return;
+  // in staged mode a site may have moved half the list buffer since the
+  // current list's build: the list may then miss pairs, and the evaluation
+  // is void (every workgroup reads the same words)
+  if (A.staged && A.fl[kFlDisp] >= A.hard_limit) {
+    if (threadIdx.x == 0) atomicOr(A.status, kStDisp);
+    return;
+  }
+  if ((A.fl[kFlSel] ^ A.fl[kFlCommit]) != 0u) {
+    A.cj = A.cj2;
+    A.cmask = A.cmask2;
+    A.chunks = A.chunks2;
+  }
+  if constexpr (!F64) {
+    extern __shared__ float4 pair_lds[];
+    float4* tab = pair_lds;
+    float2* ljt = reinterpret_cast<float2*>(pair_lds + (ENERGY ? kPairTabEntries : DYN_COUL_TAB_N));
+    const int nt2 = A.P.n_types * A.P.n_types;
+    // forces only: the φ half of the Coulomb table (the erfc half serves energies)
+    const int ntab = ENERGY ? (int)kPairTabEntries : DYN_COUL_TAB_N;
+    for (int k = threadIdx.x; k < ntab; k += kPairBlock) tab[k] = A.P.coul_tab[k];
+    for (int k = threadIdx.x; k < nt2; k += kPairBlock) ljt[k] = A.lj_table[k];
+    __syncthreads();
+    A.P.coul_tab = tab;
+    A.lj_table = ljt;
+  }
   const uint32_t n_items = (uint32_t)min((uint64_t)*A.n_chunks, A.cap_chunks);
   const uint32_t wave0 = blockIdx.x * (kPairBlock / kWave) + (threadIdx.x / kWave);
   const uint32_t nwaves = gridDim.x * (kPairBlock / kWave);
@@ -1100,16 +1261,30 @@ __device__ __forceinline__ float4 wrapped(const float4* __restrict__ pos,
                      wrap1(p.z, g.L[2], g.invL[2]), q[s]);
 }
 
+// The list arrays a build writes that the evaluations after it read (the
+// current list's, or the staging copies an asynchronous build fills).
+struct ListSlots {
+  float4* wpos_ref;   // positions of the build (the displacement reference)
+  int32_t* slot_site;
+  int32_t* slot_type;
+  int32_t* slot_gid;
+  uint8_t* cl_class;
+  uint32_t* slot_of;
+  float4* xyzq;
+  uint32_t* row_ptr;
+  uint32_t* chunk_ptr;
+  uint32_t* counters;  // [0] clusters, [1] groups, [2] work items, [3] longest row, [4] overflow
+};
+
 struct PrepArgs {
   const float4* pos;
   const float* q;
   EngGeom g;
   float4* wpos;
-  const float4* wpos_ref;
-  unsigned int* disp_bits;
-  const int32_t* slot_site;
-  const uint32_t* n_cl;
-  float4* xyzq;
+  ListSlots cur;      // the current list (slot positions refreshed here)
+  ListSlots stg;      // the staging list, copied into cur when it holds a finished build
+  unsigned int* fl;   // the engine's device flags (kFl*)
+  int force;          // the host asks for a build before the pair kernel: a staged one is dropped
   const uint32_t* qsite;
   uint32_t nq;
   float4* pme_xyzq;   // null: no SPME this evaluation
@@ -1123,15 +1298,40 @@ struct PrepArgs {
   int bin_on;
 };
 
+// Copies n values of a staging array into the current list's (grid stride).
+template <class T>
+__device__ __forceinline__ void copy_grid(T* __restrict__ dst, const T* __restrict__ src, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+// The first launch of every evaluation.  A finished staged build (from the
+// previous evaluation's second stream) becomes the current list here: its
+// slot and row arrays are copied in and the other copy of cj, cmask and
+// chunks becomes current.  Then: wrapping, the displacement of every site
+// from the current list's build positions (block max, one atomic per
+// workgroup; squared distances are non-negative floats, ordered like their
+// bit patterns), the slot positions of the current list, the SPME charges
+// and their binning, and clearing of the accumulators.  Slots and charges
+// take their wrapped positions from pos directly (the same wrap1 of the same
+// input as wpos).  The build kernels and the pair kernel decide from the
+// largest displacement (BuildGate, k_eng_pairs); the finalize kernel makes
+// the taken staged list the current one for good.
 __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
   __shared__ unsigned int red[256 / kWave];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = A.g.n_dyn + 4u * A.g.n_water;
+  // every workgroup reads the same words and writes the same decision; the
+  // finalize kernel clears them
+  const bool commit = !A.force && A.fl[kFlReady] != 0u;
+  if (t == 0) A.fl[kFlCommit] = commit ? 1u : 0u;
+  const ListSlots& L = commit ? A.stg : A.cur;
   unsigned int v = 0u;
   if (t < n) {
     const float4 w = wrapped(A.pos, A.q, A.g, t);
     A.wpos[t] = w;
-    const float4 b = A.wpos_ref[t];
+    const float4 b = L.wpos_ref[t];
     float dx = w.x - b.x, dy = w.y - b.y, dz = w.z - b.z;
     dx -= A.g.L[0] * rintf(dx * A.g.invL[0]);
     dy -= A.g.L[1] * rintf(dy * A.g.invL[1]);
@@ -1161,16 +1361,29 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
     // every lane of the wave takes part in the binning
     if (A.bin_on) pme_bin_charge(A.bin, p, qv, t);
   }
-  if (t < *A.n_cl * DYN_CL_SIZE) {
-    const int32_t s = A.slot_site[t];
+  const uint32_t n_cl = L.counters[0];
+  for (uint32_t k = t; k < n_cl * DYN_CL_SIZE; k += gridDim.x * blockDim.x) {
+    const int32_t s = L.slot_site[k];
     if (s >= 0) {
-      A.xyzq[t] = wrapped(A.pos, A.q, A.g, (uint32_t)s);
+      A.cur.xyzq[k] = wrapped(A.pos, A.q, A.g, (uint32_t)s);
     } else {
       float4 p = wrapped(A.pos, A.q, A.g,
-                         (uint32_t)A.slot_site[t & ~(uint32_t)(DYN_CL_SIZE - 1)]);
+                         (uint32_t)L.slot_site[k & ~(uint32_t)(DYN_CL_SIZE - 1)]);
       p.w = 0.0f;
-      A.xyzq[t] = p;
+      A.cur.xyzq[k] = p;
     }
+  }
+  if (commit) {
+    const size_t ns = (size_t)n_cl * DYN_CL_SIZE;
+    copy_grid(A.cur.slot_site, A.stg.slot_site, ns);
+    copy_grid(A.cur.slot_type, A.stg.slot_type, ns);
+    copy_grid(A.cur.slot_gid, A.stg.slot_gid, ns);
+    copy_grid(A.cur.cl_class, A.stg.cl_class, n_cl);
+    copy_grid(A.cur.slot_of, A.stg.slot_of, n);
+    copy_grid(A.cur.wpos_ref, A.stg.wpos_ref, n);
+    copy_grid(A.cur.row_ptr, A.stg.row_ptr, (size_t)n_cl + 1);
+    copy_grid(A.cur.chunk_ptr, A.stg.chunk_ptr, (size_t)n_cl + 1);
+    copy_grid(A.cur.counters, A.stg.counters, 4);
   }
   for (int m = kWave / 2; m >= 1; m >>= 1) v = max(v, (unsigned int)__shfl_xor((int)v, m, kWave));
   if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
@@ -1178,7 +1391,7 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
   if (threadIdx.x == 0) {
     unsigned int mx = 0u;
     for (int w = 0; w < 256 / kWave; ++w) mx = max(mx, red[w]);
-    if (mx) atomicMax(A.disp_bits, mx);
+    if (mx) atomicMax(A.fl + kFlDisp, mx);
   }
 }
 
@@ -1225,9 +1438,27 @@ __device__ __forceinline__ void k_eng_finalize(const int64_t* __restrict__ fx_si
                                const uint32_t* __restrict__ chunk_ptr, uint64_t cap_chunks,
                                const uint32_t* __restrict__ slot_of, uint32_t n_sites,
                                uint32_t n_slots, int64_t* __restrict__ out,
-                               unsigned int* __restrict__ disp_bits) {
+                               unsigned int* __restrict__ fl, int forced) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) *disp_bits = 0u;  // for the next evaluation's displacement reading
+  if (s == 0) {
+    // the end of the evaluation's list bookkeeping: a staged list taken by
+    // the preparation kernel is the current one from now on; the growth of
+    // the largest displacement (a decaying maximum of its increase per
+    // evaluation; a new reference makes the increase negative, and a forced
+    // build's reading, against the reference it replaces, counts as none);
+    // the displacement reading starts again at zero
+    if (fl[kFlCommit]) {
+      fl[kFlSel] ^= 1u;
+      fl[kFlReady] = 0u;
+      fl[kFlCommit] = 0u;
+    }
+    const float d = sqrtf(__uint_as_float(fl[kFlDisp]));
+    float growth = __uint_as_float(fl[kFlGrowth]);
+    if (!forced) growth = fmaxf(0.97f * growth, d - __uint_as_float(fl[kFlPrevDisp]));
+    fl[kFlGrowth] = __float_as_uint(growth);
+    fl[kFlPrevDisp] = __float_as_uint(forced ? 0.0f : d);
+    fl[kFlDisp] = 0u;
+  }
   if (s >= n_sites) return;
   int64_t f[3] = {fx_site[s], fx_site[(size_t)n_sites + s], fx_site[2 * (size_t)n_sites + s]};
   if (fx_recip) {
@@ -1376,14 +1607,15 @@ struct dyn_engine {
   // clustering (capacities fixed by the box geometry)
   EngGeom geom{};
   DevBuf<int> col_class;
-  DevBuf<uint32_t> col_count, col_start, col_cl_start, items;
-  DevBuf<uint2> site_col;
+  DevBuf<uint32_t> col_count, col_cl_start, items;
+  uint32_t col_slice = 0;  // entries of items per column (a column's sites, unsorted)
   uint32_t n_cols = 0, cap_cl = 0, cap_slots = 0;
   DevBuf<int32_t> slot_site, slot_type, slot_gid;
   DevBuf<uint8_t> cl_class;
   DevBuf<uint32_t> slot_of;
   DevBuf<float4> xyzq, bb_c, bb_h;
   DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj, ent_tmp;
+  DevBuf<uint8_t> xb_tmp;  // exclusion byte of each ent_tmp entry
   DevBuf<unsigned long long> cmask;
   DevBuf<uint2> chunks;
   uint64_t cap_entries = 0, cap_chunks = 0;  // groups of 8 j entries, work items
@@ -1391,10 +1623,31 @@ struct dyn_engine {
   // device counters: [0] n_cl, [1] n_groups, [2] n_items, [3] max row (groups),
   // [4] longest row that overflowed its slice (entries)
   DevBuf<uint32_t> counters;
+  // the staging list of an asynchronous build (slot and row arrays, copied
+  // into the ones above by the next evaluation's preparation kernel) and the
+  // second copy of the per-group arrays (dflags[kFlSel] names the current)
+  DevBuf<float4> stg_wpos_ref, stg_xyzq;
+  DevBuf<int32_t> stg_slot_site, stg_slot_type, stg_slot_gid;
+  DevBuf<uint8_t> stg_cl_class;
+  DevBuf<uint32_t> stg_slot_of, stg_row_ptr, stg_chunk_ptr, stg_counters;
+  DevBuf<uint32_t> cj2;
+  DevBuf<unsigned long long> cmask2;
+  DevBuf<uint2> chunks2;
+  ListSlots slots_cur() {
+    return ListSlots{wpos_ref.p, slot_site.p, slot_type.p, slot_gid.p, cl_class.p, slot_of.p,
+                     xyzq.p, row_ptr.p, chunk_ptr.p, counters.p};
+  }
+  ListSlots slots_stg() {
+    return ListSlots{stg_wpos_ref.p, stg_slot_site.p, stg_slot_type.p, stg_slot_gid.p,
+                     stg_cl_class.p, stg_slot_of.p, stg_xyzq.p, stg_row_ptr.p, stg_chunk_ptr.p,
+                     stg_counters.p};
+  }
+  ListCopies copies(bool staged) {
+    return ListCopies{cj.p, cj2.p, cmask.p, cmask2.p, chunks.p, chunks2.p, dflags.p,
+                      staged ? 1 : 0, 0};
+  }
   DevBuf<unsigned int> status;
-  // [0] largest squared displacement since the last build (float bits; the
-  // preparation kernel maximises it, the finalize kernel clears it), [1] list
-  // builds so far
+  // device flags (kFlWords words, kFl* indices)
   DevBuf<unsigned int> dflags;
   // accumulators
   DevBuf<int64_t> fitem, fx_site, fx_out, fx_slot;  // fx_slot: inspection only
@@ -1411,7 +1664,8 @@ struct dyn_engine {
   // host copies
   PinnedBuf<int64_t> h_fx;
   PinnedBuf<double> h_eng;
-  PinnedBuf<uint32_t> h_misc;  // [0] status, [1..5] counters, [6] displacement bits
+  // [0] status, [1..5] counters, [6] builds, [7] current copy, [8..12] staging counters
+  PinnedBuf<uint32_t> h_misc;
   bool fx_fresh = false;
 
   hipStream_t own_stream = nullptr;  // the engine's stream (a batch may lend it another)
@@ -1448,7 +1702,7 @@ struct dyn_engine {
     S.R.cache = &own_cache;
     S.R.n = 1;
     // fully profiled evaluations stay on one stream (their per-kernel times)
-    if ((!prof.enabled || prof.only) && stream == own_stream) {
+    if ((!prof.enabled || prof.only) && stream == own_stream && !prm.single_stream) {
       S.side.stream = side_stream;
       S.side.cache = &side_cache;
       S.side.n = 1;
@@ -1495,11 +1749,23 @@ void dyn_engine::set_geometry() {
   // clusters: Σ ceil(count/8) ≤ n_sites/8 + n_cols
   cap_cl = n_sites / DYN_CL_SIZE + n_cols + 1;
   cap_slots = cap_cl * DYN_CL_SIZE;
+  // the walk's exclusion tables key a slot in 24 bits
+  DYN_REQUIRE(cap_slots + DYN_CL_SIZE < (1u << 24), "too many sites (%u cluster slots; at most "
+              "16,777,208)", cap_slots);
   col_class.reserve(std::max<uint32_t>(n_cols, 1));
   col_count.reserve(n_cols + 1);
-  col_start.reserve(n_cols + 1);
   col_cl_start.reserve(n_cols + 1);
-  items.reserve(n_sites);
+  // a column holds 8·Lz/s sites of its class on average (s: the edge of a
+  // cube of 8 of them); its slice takes three times that, at least 64
+  {
+    double most = 0.0;
+    for (int c = 0; c < 3; ++c)
+      if (geom.cg[c].n) most = std::max(most, (double)geom.cg[c].n / (geom.cg[c].ncx * geom.cg[c].ncy));
+    const uint32_t want = (uint32_t)std::min<double>(kColCap, 3.0 * most + 64.0);
+    col_slice = std::max(col_slice, (want + 63u) & ~63u);
+    col_slice = std::min<uint32_t>(col_slice, kColCap);
+  }
+  items.reserve((size_t)n_cols * col_slice);
   // slot arrays hold one more cluster: the null slot (index cap_slots) pads
   // the j groups of the pair list
   slot_site.reserve(cap_slots + DYN_CL_SIZE);
@@ -1513,7 +1779,13 @@ void dyn_engine::set_geometry() {
   row_chunks.reserve(cap_cl + 1);
   row_ptr.reserve(cap_cl + 1);
   chunk_ptr.reserve(cap_cl + 1);
-  site_col.reserve(n_sites);
+  stg_slot_site.reserve(cap_slots + DYN_CL_SIZE);
+  stg_slot_type.reserve(cap_slots + DYN_CL_SIZE);
+  stg_slot_gid.reserve(cap_slots + DYN_CL_SIZE);
+  stg_cl_class.reserve(cap_cl);
+  stg_xyzq.reserve(cap_slots + DYN_CL_SIZE);
+  stg_row_ptr.reserve(cap_cl + 1);
+  stg_chunk_ptr.reserve(cap_cl + 1);
   {
     const int32_t none = type_none, pad = DYN_GID_PAD, nosite = -1;
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1549,6 +1821,7 @@ void dyn_engine::set_geometry() {
 void dyn_engine::reserve_rows(uint32_t entries) {
   row_cap = std::max(row_cap, (entries + 63u) & ~63u);
   ent_tmp.reserve((size_t)cap_cl * row_cap);  // grows with either factor
+  xb_tmp.reserve((size_t)cap_cl * row_cap);
 }
 
 // List capacity in groups of 8 j entries, and work items.
@@ -1556,10 +1829,13 @@ void dyn_engine::reserve_list(uint64_t groups, uint64_t n_items) {
   if (groups > cap_entries) {
     cj.reserve(groups * DYN_CL_SIZE);
     cmask.reserve(groups);
+    cj2.reserve(groups * DYN_CL_SIZE);
+    cmask2.reserve(groups);
     cap_entries = groups;
   }
   if (n_items > cap_chunks) {
     chunks.reserve(n_items);
+    chunks2.reserve(n_items);
     fitem.reserve((size_t)3 * DYN_CL_SIZE * n_items);
     pair_part.reserve((size_t)kNumPart * n_items);
     cap_chunks = n_items;
@@ -1587,38 +1863,47 @@ static unsigned int build_limit(const dyn_engine& e) {
   return b;
 }
 
+// A staged build starts when the largest displacement is at least this far
+// (or twice its recent growth per evaluation, if more) below half the list
+// buffer: the staged list becomes current one evaluation later.
+constexpr float kStageMargin = 0.25f;
+
 // The gate of engine e's build kernels in this evaluation.
 static BuildGate build_gate(const dyn_engine& e, bool force) {
   BuildGate G;
-  G.disp_bits = e.dflags.p + 0;
+  std::memset(&G, 0, sizeof G);
+  G.fl = e.dflags.p;
   G.limit = build_limit(e);
+  G.half = 0.5f * e.prm.list_buffer;
+  G.margin_min = std::min(kStageMargin, 0.2f * G.half);
+  G.staged = 0;
   G.force = (force || e.prm.list_buffer <= 0.f) ? 1 : 0;
   return G;
 }
 
 // Cluster-pair list build of every engine of the set; the kernels of an
 // engine whose gate is closed return at once.
-static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
+// The list build of every engine of the set, on the run R: into the current
+// list (before the pair kernel) or, staged, into the staging list and the
+// other copy of the per-group arrays (beside the pair kernel, current from
+// the next evaluation on).  The kernels of an engine whose gate is closed
+// return at once.
+static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<BuildGate>& gate,
+                            bool staged) {
   auto& E = S.e;
-  const BatchRun& R = S.R;
   Profiler& prof = *S.prof;
+  std::vector<ListSlots> T(E.size());
+  for (size_t i = 0; i < E.size(); ++i) T[i] = staged ? E[i]->slots_stg() : E[i]->slots_cur();
   {
     Profiler::Scope ps(prof, "nbl_bin");
-    const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_bin_count>(R, "k_eng_bin_count", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_bin_count>(gate[i], e.wpos.p, e.geom, e.wpos_ref.p, e.col_count.p,
-                                   e.site_col.p, e.dflags.p + 1);
+    const unsigned gs = max_grid(S, [](dyn_engine& e) {
+      return std::min<uint32_t>(grid_for(e.n_sites, kBinBlock), kBuildGrid);
     });
-    launch<k_eng_scan2<0>, kScanBlock>(R, "k_eng_scan2<0>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
+    launch<k_eng_bin_count, kBinBlock>(R, "k_eng_bin_count", 0, gs, kBinBlock, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_scan2<0>>(gate[i], e.col_count.p, (const uint32_t*)nullptr, e.n_cols,
-                                  (const uint32_t*)nullptr, e.col_start.p, e.col_cl_start.p,
-                                  (uint32_t*)nullptr, e.counters.p + 0, (uint32_t*)nullptr);
-    });
-    launch<k_eng_bin_scatter>(R, "k_eng_bin_scatter", 0, gs, 256, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_bin_scatter>(gate[i], e.site_col.p, e.col_start.p, e.n_sites, e.items.p);
+      return args<k_eng_bin_count>(gate[i], e.wpos.p, e.geom, T[i].wpos_ref, e.col_count.p,
+                                   e.items.p, e.dflags.p + kFlBuilds, e.dflags.p + kFlBinDone,
+                                   e.n_cols, e.col_slice, e.col_cl_start.p, T[i].counters + 0);
     });
   }
   {
@@ -1628,10 +1913,10 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
       dyn_engine& e = *E[i];
       SlotArgs A;
       std::memset(&A, 0, sizeof A);
-      A = SlotArgs{e.wpos.p, e.ltype.p, e.wmol.p, e.type_none, e.xyzq.p, e.slot_type.p,
-                   e.slot_gid.p, e.slot_site.p, e.slot_of.p};
-      return args<k_eng_colsort>(gate[i], e.col_count.p, e.col_start.p, e.items.p,
-                                 e.col_cl_start.p, e.col_class.p, A, e.cl_class.p, e.bb_c.p,
+      A = SlotArgs{e.wpos.p, e.ltype.p, e.wmol.p, T[i].xyzq, T[i].slot_type,
+                   T[i].slot_gid, T[i].slot_site, T[i].slot_of, e.type_none, 0};
+      return args<k_eng_colsort>(gate[i], e.col_count.p, e.items.p, e.col_slice,
+                                 e.col_cl_start.p, e.col_class.p, A, T[i].cl_class, e.bb_c.p,
                                  e.bb_h.p, e.n_sites, e.status.p, e.n_cols);
     });
   }
@@ -1647,15 +1932,15 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     A.rlist = e.prm.cutoff + std::max(0.f, e.prm.list_buffer) +
               ((e.prm.precision & DYN_PREC_PAIR_F64) ? 0.01f : 0.f);
     A.rlist2 = A.rlist * A.rlist;
-    A.n_cl = e.counters.p + 0;
+    A.n_cl = T[i].counters + 0;
     A.bb_c = e.bb_c.p;
     A.bb_h = e.bb_h.p;
     A.col_cl_start = e.col_cl_start.p;
-    A.cl_class = e.cl_class.p;
-    A.slot_site = e.slot_site.p;
-    A.xyzq = e.xyzq.p;
-    A.slot_gid = e.slot_gid.p;
-    A.slot_of = e.slot_of.p;
+    A.cl_class = T[i].cl_class;
+    A.slot_site = T[i].slot_site;
+    A.xyzq = T[i].xyzq;
+    A.slot_gid = T[i].slot_gid;
+    A.slot_of = T[i].slot_of;
     A.ex_ptr = e.ex_ptr.p;
     A.ex_idx = e.ex_idx.p;
     A.n_dyn = e.n_dyn;
@@ -1664,8 +1949,11 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     A.row_cap = e.row_cap;
     A.chunk = e.prm.item_groups;
     A.null_slot = e.cap_slots;
-    A.longest = e.counters.p + 4;
+    A.longest = T[i].counters + 4;
     A.status = e.status.p;
+    A.xb_tmp = e.xb_tmp.p;
+    A.done = e.dflags.p + kFlWalkDone;
+    A.tot = T[i].counters + 1;
   }
   const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
     return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kBuildGrid);
@@ -1675,13 +1963,8 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     launch<k_eng_list_walk, kListBlock>(R, "k_eng_list_walk", 0, list_grid, kListBlock, 0,
                                         [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p);
-    });
-    launch<k_eng_scan2<1>, kScanBlock>(R, "k_eng_scan2<1>", 0, 1, kScanBlock, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_scan2<1>>(gate[i], e.row_cnt.p, e.row_chunks.p, 0u, e.counters.p + 0,
-                                  e.row_ptr.p, e.chunk_ptr.p, e.counters.p + 1, e.counters.p + 2,
-                                  e.counters.p + 3);
+      return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p,
+                                   T[i].row_ptr, T[i].chunk_ptr);
     });
   }
   {
@@ -1689,23 +1972,9 @@ static void set_build_lists(EngineSet& S, const std::vector<BuildGate>& gate) {
     launch<k_eng_list_fill, kListBlock>(R, "k_eng_list_fill", 0, list_grid, kListBlock, 0,
                                         [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_list_fill>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_ptr.p, e.chunk_ptr.p,
-                                   e.cj.p, e.chunks.p);
+      return args<k_eng_list_fill>(LA[i], e.ent_tmp.p, e.row_cnt.p, T[i].row_ptr,
+                                   T[i].chunk_ptr, e.copies(staged));
     });
-    bool any_solute = false;
-    for (dyn_engine* e : E) any_solute = any_solute || e->n_dyn > 0;
-    if (any_solute) {
-      const unsigned gm = max_grid(S, [](dyn_engine& e) {
-        const uint32_t sol_rows =
-            e.n_dyn / DYN_CL_SIZE + (uint32_t)(e.geom.cg[0].ncx * e.geom.cg[0].ncy) + 1;
-        return std::min<uint32_t>(sol_rows, kBuildGrid);
-      });
-      launch<k_eng_list_mask, kWave>(R, "k_eng_list_mask", 0, gm, kWave, 0, [&](uint32_t i) {
-        dyn_engine& e = *E[i];
-        return args<k_eng_list_mask>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_ptr.p,
-                                     e.col_cl_start.p, e.cmask.p);
-      });
-    }
   }
 }
 
@@ -1718,13 +1987,31 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
   Profiler& prof = *S.prof;
   // host-forced builds: asked for, or no list (never built, a new box,
   // positions set from the host, a failed evaluation); the device decides
-  // for the others from their displacement readings
-  std::vector<BuildGate> gate(ns);
+  // for the others from their displacement readings.  Staged mode: the
+  // others build a staging list beside the pair kernel when their sites
+  // have moved (buffer/2 − kStageMargin) since their list's build, and their
+  // evaluation is void if a site has moved buffer/2; otherwise the build
+  // runs before the pair kernel when a site has moved buffer/2.
+  const bool staged_mode = (flags & kComputeStagedBuild) != 0;
+  std::vector<BuildGate> gate(ns), sgate(ns);
+  std::vector<char> forced(ns);
+  bool any_inline = false, any_staged = false;
   for (uint32_t i = 0; i < ns; ++i) {
     dyn_engine* e = E[i];
     DYN_REQUIRE(e->box_set, "box not set");
     if (!e->pos_set) throw Error(DYN_ERR_STATE, "positions not set");
-    gate[i] = build_gate(*e, (flags & DYN_COMPUTE_REBUILD) || !e->list_valid);
+    forced[i] = (flags & DYN_COMPUTE_REBUILD) || !e->list_valid || e->prm.list_buffer <= 0.f;
+    gate[i] = build_gate(*e, forced[i]);
+    sgate[i] = gate[i];
+    sgate[i].staged = 1;
+    sgate[i].force = 0;
+    if (staged_mode && !forced[i]) {
+      gate[i].limit = ~0u;  // no build before the pair kernel: squared distances stay below
+      any_staged = true;
+    } else {
+      sgate[i].half = std::numeric_limits<float>::infinity();  // closed: distances stay below
+    }
+    any_inline = any_inline || !staged_mode || forced[i];
   }
   const bool short_on = (flags & DYN_COMPUTE_SHORT) != 0;
   const bool energy_on = (flags & DYN_COMPUTE_NO_ENERGY) == 0;
@@ -1789,11 +2076,10 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.q = e.q.p;
       A.g = e.geom;
       A.wpos = e.wpos.p;
-      A.wpos_ref = e.wpos_ref.p;
-      A.disp_bits = e.dflags.p + 0;
-      A.slot_site = e.slot_site.p;
-      A.n_cl = e.counters.p + 0;
-      A.xyzq = e.xyzq.p;
+      A.cur = e.slots_cur();
+      A.stg = e.slots_stg();
+      A.fl = e.dflags.p;
+      A.force = forced[i] ? 1 : 0;
       A.qsite = e.qsite.p;
       A.nq = e.n_q;
       A.pme_xyzq = pl.recip_on ? e.pme_xyzq.p : nullptr;
@@ -1814,8 +2100,15 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
   }
   // the second stream: SPME (spread → FFT → convolution → inverse FFT →
   // gather), the exclusion and 1-4 corrections and the caller's side work,
-  // overlapping the list build and the pair kernel on the main stream
+  // overlapping the list build and the pair kernel on the main stream (a
+  // fork later in the step delays the side work more than it gains: an
+  // event wait between streams takes 10–40 µs to release)
   stream_order(R.stream, RS.stream, S.ev_fork);
+  // SPME: beside the list build and the pair kernel on the second stream,
+  // or — with staged builds, where the pair kernel follows the preparation
+  // kernel at once and would take the compute units the FFT planes need —
+  // before the pair kernel on the main stream
+  const BatchRun& RP = staged_mode ? R : RS;
   if (any_recip) {
     std::vector<PmeRun> ra(ns), rb(ns);
     for (uint32_t i = 0; i < ns; ++i) {
@@ -1845,8 +2138,8 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
         rb[i] = r;
       }
     }
-    pme_run(RS, prof, ra.data(), 0);
-    if (any_b) pme_run(RS, prof, rb.data(), 1);
+    pme_run(RP, prof, ra.data(), 0);
+    if (any_b) pme_run(RP, prof, rb.data(), 1);
   }
   // the exclusion and 1-4 corrections (site accumulators; finalize reads them)
   if (any_corr) {
@@ -1878,7 +2171,11 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
     });
   }
   if (side_work) side_work(RS);
-  set_build_lists(S, gate);
+  // a staged build on the second stream, beside the pair kernel (after it
+  // when there is one stream)
+  const bool one_stream = RS.stream == R.stream;
+  if (any_staged && !one_stream) set_build_lists(S, RS, sgate, true);
+  if (any_inline) set_build_lists(S, R, gate, false);
   for (uint32_t i = 0; i < ns; ++i) {
     dyn_engine* e = E[i];
     if (!e->dbg_count) continue;
@@ -1906,6 +2203,12 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.cj = e.cj.p;
       A.cmask = e.cmask.p;
       A.chunks = e.chunks.p;
+      A.cj2 = e.cj2.p;
+      A.cmask2 = e.cmask2.p;
+      A.chunks2 = e.chunks2.p;
+      A.fl = e.dflags.p;
+      A.staged = staged_mode && !forced[i] ? 1 : 0;
+      A.hard_limit = build_limit(e);
       A.n_chunks = e.counters.p + 2;
       A.lj_table = e.lj_table.p;
       A.lj_se = e.lj_se.p;
@@ -1919,18 +2222,22 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.chunk = e.prm.item_groups;
       A.status = e.status.p;
     }
-    // a wave per work item (the count is known on the device only)
+    // a wave per work item, up to kPairGridCap workgroups (the item count is
+    // known on the device only; the waves take items in a grid stride)
     const unsigned gp = max_grid(S, [](dyn_engine& e) {
-      return (unsigned)std::max<uint64_t>(1, (e.cap_chunks + 3) / 4);
+      return (unsigned)std::min<uint64_t>(kPairGridCap, std::max<uint64_t>(1, (e.cap_chunks + 3) / 4));
     });
     // FP64 pair arithmetic when the first system asks for it (a set shares it)
     const bool f64 = (E[0]->prm.precision & DYN_PREC_PAIR_F64) != 0;
+    int nt_max = 0;
+    for (dyn_engine* e : E) nt_max = std::max(nt_max, e->n_types);
+    const size_t shm = pair_lds_bytes(nt_max, energy_on);
     auto pa = [&](uint32_t i) { return PA[i]; };
     if (energy_on && !f64)
-      launch<k_eng_pairs<true, false>, kPairBlock>(R, "k_eng_pairs<true>", 0, gp, kPairBlock, 0,
+      launch<k_eng_pairs<true, false>, kPairBlock>(R, "k_eng_pairs<true>", 0, gp, kPairBlock, shm,
           [&](uint32_t i) { return args<k_eng_pairs<true, false>>(pa(i)); });
     else if (!f64)
-      launch<k_eng_pairs<false, false>, kPairBlock>(R, "k_eng_pairs<false>", 0, gp, kPairBlock, 0,
+      launch<k_eng_pairs<false, false>, kPairBlock>(R, "k_eng_pairs<false>", 0, gp, kPairBlock, shm,
           [&](uint32_t i) { return args<k_eng_pairs<false, false>>(pa(i)); });
     else if (energy_on)
       launch<k_eng_pairs<true, true>, kPairBlock>(R, "k_eng_pairs<true, f64>", 0, gp, kPairBlock, 0,
@@ -1939,6 +2246,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       launch<k_eng_pairs<false, true>, kPairBlock>(R, "k_eng_pairs<false, f64>", 0, gp, kPairBlock, 0,
           [&](uint32_t i) { return args<k_eng_pairs<false, true>>(pa(i)); });
   }
+  if (any_staged && one_stream) set_build_lists(S, R, sgate, true);
   stream_order(RS.stream, R.stream, S.ev_join);
   {
     Profiler::Scope ps(prof, "finalize");
@@ -1950,7 +2258,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
                                   (pl.recip_on || pl.reuse) ? e.fx_recip.p : (const int64_t*)nullptr,
                                   short_on ? e.fitem.p : (const int64_t*)nullptr, e.chunk_ptr.p,
                                   e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p,
-                                  e.dflags.p + 0);
+                                  e.dflags.p, forced[i] ? 1 : 0);
     });
     if (short_on && energy_on)
       launch<k_eng_energy_reduce1, kRed1Block>(R, "k_eng_energy_reduce1", 0, kRed1Grid, kRed1Block,
@@ -2003,13 +2311,20 @@ void engine_set_check(EngineSet& S) {
 void dyn_engine::check_status() {
   DYN_HIP(hipMemcpyAsync(h_misc.p, status.p, 4, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipMemcpyAsync(h_misc.p + 1, counters.p, 20, hipMemcpyDeviceToHost, stream));
-  DYN_HIP(hipMemcpyAsync(h_misc.p + 6, dflags.p + 1, 4, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 6, dflags.p + kFlBuilds, 4, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 7, dflags.p + kFlSel, 4, hipMemcpyDeviceToHost, stream));
+  DYN_HIP(hipMemcpyAsync(h_misc.p + 8, stg_counters.p, 20, hipMemcpyDeviceToHost, stream));
   DYN_HIP(hipStreamSynchronize(stream));
   const uint32_t st = h_misc.p[0];
   if (!st) return;
   DYN_HIP(hipMemsetAsync(status.p, 0, 4, stream));
   DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
   list_valid = false;
+  if (st & kStDisp) {
+    if ((st & ~(kStDisp | kStPme)) == 0)
+      throw Error(DYN_ERR_CAPACITY, "a site moved half the list buffer since its list was built "
+                                    "(the list was staged too late for the step); call again");
+  }
   if (st & kStPme) {
     pme.grow();
     if (st == kStPme)
@@ -2017,17 +2332,23 @@ void dyn_engine::check_status() {
                                         "charges per brick", pme.cap_brick));
   }
   // fixed limits of the design: calling again does not help
-  if (st & kStColumn)
+  if ((st & kStColumn) && col_slice >= (uint32_t)kColCap)
     throw Error(DYN_ERR_LIMIT, fmt("a cluster column holds more than %d sites", kColCap));
   if (st & kStMask)
-    throw Error(DYN_ERR_LIMIT, fmt("a solute cluster has more than %d solute neighbours "
-                                   "(reduce list_buffer)", kMaskCap));
+    throw Error(DYN_ERR_LIMIT, fmt("the sites of a solute cluster have more than %u exclusion "
+                                   "partners (1-2, 1-3, 1-4) together", kExPartners));
+  if (st & kStColumn) {
+    col_slice = std::min<uint32_t>(2 * col_slice, kColCap);
+    items.reserve((size_t)n_cols * col_slice);
+  }
   if (st & kStRow) {
-    const uint32_t longest = h_misc.p[5];
+    // the longest row of either list's build (counters [4])
+    const uint32_t longest = std::max(h_misc.p[5], h_misc.p[12]);
     DYN_HIP(hipMemsetAsync(counters.p + 4, 0, 4, stream));
+    DYN_HIP(hipMemsetAsync(stg_counters.p + 4, 0, 4, stream));
     reserve_rows(std::max<uint32_t>(longest + longest / 4, row_cap + row_cap / 2));
   }
-  const uint64_t ne = h_misc.p[2], nch = h_misc.p[3];
+  const uint64_t ne = std::max(h_misc.p[2], h_misc.p[9]), nch = std::max(h_misc.p[3], h_misc.p[10]);
   reserve_list(ne + ne / 4 + 1024, nch + nch / 4 + 256);
   throw Error(DYN_ERR_CAPACITY, fmt("pair list outgrew its buffers (%llu groups, %llu work "
                                     "items, row slices of %u entries); they have been enlarged, "
@@ -2061,6 +2382,7 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_REQUIRE(p.mesh_spacing == 0.f || (p.grid[0] == 0 && p.grid[1] == 0 && p.grid[2] == 0),
                 "give either a PME grid or a mesh spacing");
     DYN_REQUIRE(p.item_groups <= 64, "item_groups must be in 1..64 (0: 16)");
+    DYN_REQUIRE(p.single_stream <= 1, "single_stream must be 0 or 1");
     DYN_REQUIRE(n_dyn + 4ull * n_water > 0, "empty system");
     DYN_REQUIRE(n_dyn + 4ull * n_water < (1ull << 30), "system too large");
     DYN_REQUIRE(n_dyn == 0 || (q_dyn && type_dyn), "null solute arrays");
@@ -2215,6 +2537,8 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->wpos.reserve(ns);
     E->wpos_ref.reserve(ns);
     E->slot_of.reserve(ns);
+    E->stg_wpos_ref.reserve(ns);
+    E->stg_slot_of.reserve(ns);
     E->fx_site.reserve(3 * (size_t)ns);
     E->fx_out.reserve(3 * (size_t)ns);
     E->fx_recip.reserve(3 * (size_t)ns);
@@ -2222,10 +2546,11 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     E->eng_out.reserve(kEngOut);
     E->red1.reserve((size_t)kRed1Grid * kNumPart);
     E->h_eng.reserve(kEngOut);
-    E->h_misc.reserve(8);
+    E->h_misc.reserve(16);
     E->counters.reserve(8);
+    E->stg_counters.reserve(8);
     E->status.reserve(1);
-    E->dflags.reserve(4);
+    E->dflags.reserve(kFlWords);
     {
       std::vector<float> tab(8 * DYN_COUL_TAB_N);
       dyn_coul_tab_build(tab.data());
@@ -2245,7 +2570,12 @@ extern "C" int dyn_engine_create(dyn_engine** out, int device, const dyn_engine_
     DYN_HIP(hipMemsetAsync(E->counters.p, 0, 32, E->stream));
     DYN_HIP(hipMemsetAsync(E->slot_of.p, 0, 4 * (size_t)ns, E->stream));
     DYN_HIP(hipMemsetAsync(E->status.p, 0, 4, E->stream));
-    DYN_HIP(hipMemsetAsync(E->dflags.p, 0, 16, E->stream));
+    DYN_HIP(hipMemsetAsync(E->dflags.p, 0, 4 * kFlWords, E->stream));
+    DYN_HIP(hipMemsetAsync(E->stg_counters.p, 0, 4 * 8, E->stream));
+    // reference positions are read before the first build writes them (the
+    // first evaluation's reading is discarded): keep them defined
+    DYN_HIP(hipMemsetAsync(E->wpos_ref.p, 0, sizeof(float4) * ns, E->stream));
+    DYN_HIP(hipMemsetAsync(E->stg_wpos_ref.p, 0, sizeof(float4) * ns, E->stream));
     DYN_HIP(hipStreamSynchronize(E->stream));
     *out = E.release();
     return DYN_OK;
@@ -2596,9 +2926,13 @@ extern "C" int dyn_engine_debug_export(dyn_engine* eng, uint32_t* n_slots, uint3
     if (slot_gid) DYN_HIP(hipMemcpyAsync(slot_gid, eng->slot_gid.p, 4 * ns, hipMemcpyDeviceToHost, s));
     if (cl_class) DYN_HIP(hipMemcpyAsync(cl_class, eng->cl_class.p, nc, hipMemcpyDeviceToHost, s));
     if (row_ptr) DYN_HIP(hipMemcpyAsync(row_ptr, eng->row_ptr.p, 4 * (nc + 1), hipMemcpyDeviceToHost, s));
-    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, eng->cj.p, 4 * DYN_CL_SIZE * ne, hipMemcpyDeviceToHost, s));
-    if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, eng->cmask.p, 8 * ne, hipMemcpyDeviceToHost, s));
-    if (items && nit) DYN_HIP(hipMemcpyAsync(items, eng->chunks.p, 8 * nit, hipMemcpyDeviceToHost, s));
+    const bool second = eng->h_misc.p[7] != 0;  // the current copy of the per-group arrays
+    const uint32_t* cj_d = second ? eng->cj2.p : eng->cj.p;
+    const unsigned long long* cm_d = second ? eng->cmask2.p : eng->cmask.p;
+    const uint2* ch_d = second ? eng->chunks2.p : eng->chunks.p;
+    if (cj && ne) DYN_HIP(hipMemcpyAsync(cj, cj_d, 4 * DYN_CL_SIZE * ne, hipMemcpyDeviceToHost, s));
+    if (cmask && ne) DYN_HIP(hipMemcpyAsync(cmask, cm_d, 8 * ne, hipMemcpyDeviceToHost, s));
+    if (items && nit) DYN_HIP(hipMemcpyAsync(items, ch_d, 8 * nit, hipMemcpyDeviceToHost, s));
     if (fx_slot) {
       eng->fx_slot.reserve(3 * (size_t)eng->cap_slots);
       EngineSet S = eng->self_set();
@@ -2635,6 +2969,8 @@ extern "C" int dyn_engine_debug_item_partials(dyn_engine* eng, double* out, uint
 }
 
 extern "C" uint64_t dyn_debug_arg_uploads(void) { return g_arg_uploads.load(); }
+extern "C" uint64_t dyn_debug_layout_generation(void) { return g_layout_gen.load(); }
+extern "C" const char* dyn_debug_capture_refusal(void) { return g_capture_refusal.c_str(); }
 
 extern "C" int dyn_engine_profile(dyn_engine* eng, int enable) {
   ENG_CHECK(eng);
@@ -2679,6 +3015,7 @@ const unsigned int* engine_status_device(const dyn_engine* eng) { return eng->st
 bool engine_graph_safe(const dyn_engine* eng) {
   return !eng->dbg_count && (!eng->prof.enabled || eng->prof.only);
 }
+bool engine_single_stream(const dyn_engine* eng) { return eng->prm.single_stream != 0; }
 void engine_set_stream(dyn_engine* eng, hipStream_t s) {
   g_layout_gen.fetch_add(1);
   eng->stream = s ? s : eng->own_stream;

@@ -34,6 +34,11 @@ struct EngineSet {
 // its pair list (DYN_COMPUTE_REBUILD, or a list never built, forces it).
 // side_work, when given, enqueues further work on the side stream (or the
 // main one) after the SPME chain; it must not read the nonbonded forces.
+// Internal compute flag (beside the public DYN_COMPUTE_* bits): list builds
+// the device decides on go to the staging list on the second stream, beside
+// the pair kernel, and become current at the next evaluation; an evaluation
+// whose current list has gone stale is void (status) and its call repeats.
+constexpr uint32_t kComputeStagedBuild = 1u << 30;
 void engine_set_enqueue(EngineSet& S, uint32_t flags,
                         const std::function<void(const BatchRun&)>& side_work = nullptr);
 // Wait for the set's work and check each engine's device status (throws the
@@ -49,6 +54,8 @@ const unsigned int* engine_status_device(const dyn_engine* eng);
 // Whether the engine's evaluations may be recorded into a graph (no test
 // hook pending, no profiling).
 bool engine_graph_safe(const dyn_engine* eng);
+// Whether the engine asks for every kernel on one stream (no second stream).
+bool engine_single_stream(const dyn_engine* eng);
 void engine_set_stream(dyn_engine* eng, hipStream_t s);  // null: its own stream again
 int engine_device(const dyn_engine* eng);
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water);

@@ -1122,6 +1122,9 @@ struct MdSet {
   BatchRun side;  // MD work on the engines' second stream (bonded forces)
   Profiler* prof = nullptr;
   StepGraphs* graphs = nullptr;
+  // steady evaluations stage their list builds beside the pair kernel (off
+  // for the repeat of a call that found a stale list)
+  bool staged_builds = true;
 };
 
 }  // namespace dyn
@@ -1206,7 +1209,8 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
     engine_set_compute_sync(S.es, flags);
     bonded(S.R);
   } else {
-    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC, bonded);
+    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC | (S.staged_builds ? kComputeStagedBuild : 0u),
+                       bonded);
   }
   for (dyn_md* m : S.m) m->forces_valid = true;
 }
@@ -1499,7 +1503,11 @@ constexpr uint32_t kGraphSteps = 16;  // steps per graph (a multiple of every SP
 bool graphs_usable(const MdSet& S) {
   // a profile of every kernel runs them one by one; timing one kernel (the
   // pair kernel) records its events into the graphs
-  if (!S.graphs || S.graphs->disabled || (S.prof->enabled && !S.prof->only)) return false;
+  // a repeat (it builds before the pair kernel) runs without the graphs of
+  // staged steps
+  if (!S.graphs || S.graphs->disabled || (S.prof->enabled && !S.prof->only) ||
+      S.staged_builds != (S.m[0]->prm.staged_builds != 0))
+    return false;
   for (dyn_md* m : S.m)
     if (!engine_graph_safe(m->eng) || !m->forces_valid || m->prm.direct_launch) return false;
   return true;
@@ -1548,13 +1556,20 @@ void md_graph_block(MdSet& S) {
   hipGraph_t graph = nullptr;
   try {
     steps();
-  } catch (...) {
+  } catch (const std::exception& ex) {
+    // the refusal may come between a fork and its join: join the second
+    // stream into the capture so that it ends cleanly, then drop the graph
+    // (nothing of the recorded steps has run)
+    if (S.es.side.stream && S.es.side.stream != S.R.stream)
+      stream_order(S.es.side.stream, S.R.stream, S.es.ev_join);
     S.prof->capture_sink = nullptr;
     g_launch_mode = kLaunchDirect;
     (void)hipStreamEndCapture(S.R.stream, &graph);
     if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
     C.disabled = true;
-    throw CaptureRefused("the steps could not be recorded into a graph");
+    g_capture_refusal = ex.what();
+    throw CaptureRefused(std::string("the steps could not be recorded into a graph: ") + ex.what());
   }
   S.prof->capture_sink = nullptr;
   g_launch_mode = kLaunchDirect;
@@ -1581,6 +1596,9 @@ void md_run(MdSet& S, uint32_t n) {
   std::vector<MdCk> ck;
   md_checkpoint(S, ck);
   for (int attempt = 0;; ++attempt) {
+    // a repeat builds its lists before the pair kernel (it may follow a
+    // stale staged list)
+    S.staged_builds = attempt == 0 && S.m[0]->prm.staged_builds != 0;
     try {
       const auto t0 = std::chrono::steady_clock::now();
       // a barostat step waits mid-call, and may find an overflow there
@@ -2056,8 +2074,9 @@ struct dyn_batch {
     S.es.R = BatchRun{stream, &eng_cache, n};
     S.es.prof = &prof;
     S.R = BatchRun{stream, &md_cache, n};
-    // fully profiled steps stay on one stream (their per-kernel times)
-    if (!prof.enabled || prof.only) {
+    // fully profiled steps stay on one stream (their per-kernel times), and
+    // so do the steps of members that ask for one stream
+    if ((!prof.enabled || prof.only) && !engine_single_stream(m[0]->eng)) {
       S.es.side = BatchRun{side_stream, &eng_side_cache, n};
       S.es.ev_fork = ev_fork;
       S.es.ev_join = ev_join;
@@ -2102,6 +2121,8 @@ extern "C" int dyn_batch_create(dyn_batch** out, uint32_t n, dyn_md* const* mds)
                   "MD handle %u: Langevin and plain velocity Verlet cannot share a batch", i);
       DYN_REQUIRE(x->prm.nstpcouple == mds[0]->prm.nstpcouple,
                   "MD handle %u: members of a batch share nstpcouple", i);
+      DYN_REQUIRE((x->prm.staged_builds != 0) == (mds[0]->prm.staged_builds != 0),
+                  "MD handle %u: members of a batch share staged_builds", i);
       DYN_REQUIRE(!x->prm.nstpcouple || x->step == mds[0]->step, "MD handle %u: members of a "
                   "batch with pressure coupling share the step count (%llu vs %llu)", i,
                   (unsigned long long)x->step, (unsigned long long)mds[0]->step);

@@ -44,15 +44,17 @@ __device__ __forceinline__ uint32_t brick_of(const int k0[3], const uint32_t nbk
 // per axis, bricks per axis, the per-brick counts and fixed-capacity lists,
 // and where an overflow is flagged.
 struct BinArgs {
+  uint32_t* count;
+  uint32_t* items;
+  unsigned int* status;
   int K[3];
   float sc[3];
   uint32_t nb[3];
-  uint32_t* count;
-  uint32_t* items;
   uint32_t cap;
-  unsigned int* status;
   unsigned int status_bit;
+  uint32_t unused;  // no implicit padding: argument packs are compared byte for byte
 };
+static_assert(sizeof(BinArgs) == 72, "BinArgs has no padding");
 
 // Lists charge i (position p) under its home brick.  Every lane of the wave
 // must call it (valid marks the lanes with a charge): one atomic per

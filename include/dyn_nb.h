@@ -197,6 +197,10 @@ typedef struct dyn_engine_params {
   uint32_t item_groups; /* j groups (of 8 j slots) per work item of the pair
                          * kernel, 1..64; 0: 16.  A tuning knob: results are
                          * the same for any value to f32 summation order   */
+  uint32_t single_stream; /* 1: every kernel of an evaluation on the engine's
+                         * stream; 0: SPME, corrections and bonded forces on a
+                         * second stream beside the list build and the pair
+                         * kernel.  A tuning knob: results are bit-identical */
 } dyn_engine_params;
 
 /* Precision options (the precision sweep; the default FP32 path is the one
@@ -365,6 +369,14 @@ int dyn_engine_debug_inject_status(dyn_engine* eng, uint32_t bits, uint32_t skip
  * packs so far in this process (packs are uploaded only when they change,
  * so steady-state steps add none). */
 uint64_t dyn_debug_arg_uploads(void);
+/* Diagnostics: the device-layout generation (advanced whenever a captured
+ * step graph may hold stale addresses or arguments: a buffer freed, a
+ * setter called, an argument copy reused; graphs of MD steps are recorded
+ * only while it holds still). */
+uint64_t dyn_debug_layout_generation(void);
+/* Diagnostics: why the last step-graph capture was refused ("" if none was);
+ * the steps then run with direct launches. */
+const char* dyn_debug_capture_refusal(void);
 
 /* Per-stage device time from HIP events recorded on the engine's stream
  * around every stage while profiling is on (enable 1), or around the pair
@@ -425,6 +437,15 @@ typedef struct dyn_md_params {
    * either way): 0 = runs of 16 steady steps are recorded into HIP graphs and
    * replayed; 1 = every kernel is launched directly. */
   uint32_t direct_launch;
+  /* How list builds the device decides on are placed (a tuning knob; the
+   * forces are exact either way): 0 = before the pair kernel of the
+   * evaluation that needs one (when a site has moved half the list buffer);
+   * 1 = staged: a build starts beside the pair kernel when the largest
+   * displacement comes within max(0.25 Å, twice its recent growth per step)
+   * of half the buffer, and becomes the current list at the next evaluation;
+   * should a site reach half the buffer first, the call repeats from its
+   * start with builds before the pair kernel. */
+  uint32_t staged_builds;
 } dyn_md_params;
 
 /* The engine's box must be set.  mass_dyn[n_dyn] (amu).  Bonds: ij[2n],

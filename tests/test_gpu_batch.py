@@ -28,29 +28,37 @@ def _members():
     ]
 
 
-def _make(s, alch, lam, buffer, seed):
+def _make(s, alch, lam, buffer, seed, staged=False):
     ff = systems.chain_force_field(s) if s.n_dyn else systems.water_force_field(s)
     K = api.pme_grid_n(s.box, 1.0)
     eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=BETA, grid=K, order=4,
                      mode=api.MODE_REFERENCE, list_buffer=buffer)
+    # quench the built structures' close contacts first, as a real run does
+    pd, pw = api.relax(eng, ff, s.pos_dyn, systems.water_positions(s), steps=200)
     if alch:
         eng.set_alchemical(alch[0], alch[1], lam)
-    m = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=seed)
+    m = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=seed,
+               staged_builds=staged)
     v, vw = systems.initial_velocities(s, ff)
-    m.set_state(s.pos_dyn, v, systems.water_positions(s), vw)
+    m.set_state(pd, v, pw, vw)
     return eng, m
 
 
-def _run(buffer, steps, batched, profile=False):
+def _run(buffer, steps, batched, profile=False, staged=False):
     """steps: an int, or a sequence of step-call lengths."""
     calls = [steps] if isinstance(steps, int) else list(steps)
-    made = [_make(s, a, lam, buffer, seed=11 + k) for k, (s, a, lam) in enumerate(_members())]
+    made = [_make(s, a, lam, buffer, seed=11 + k, staged=staged)
+            for k, (s, a, lam) in enumerate(_members())]
     if batched:
         b = api.Batch([m for _, m in made])
         if profile:
             b.profile(True)
         for n in calls:
+            g0, u0 = api.layout_generation(), api.arg_uploads()
             b.step(n)
+            print(f"batch call of {n} steps: layout generation +{api.layout_generation() - g0}, "
+                  f"argument uploads +{api.arg_uploads() - u0}, capture refusal "
+                  f"{api.capture_refusal()!r}", flush=True)
         b.close()
     else:
         for e, m in made:
@@ -85,20 +93,25 @@ def _same_states(a, b):
         assert ea["total"] == eb["total"], k
 
 
-def test_batch_with_list_buffer(gpu):
+@pytest.mark.parametrize("staged", [False, True])
+def test_batch_with_list_buffer(gpu, staged):
     """With a 1 Å list buffer and three calls of 20 steps (the second records
     a graph of 16 steady steps, the third replays it), the batch and each
     member alone take the same list builds and reach the same state bit for
     bit; so does a profiled run (direct launches on one stream)."""
-    print("batch of 5 with a 1 A list buffer, 3 x 20 steps", flush=True)
-    a = _run(1.0, (20, 20, 20), batched=False)
-    b = _run(1.0, (20, 20, 20), batched=True)
+    print(f"batch of 5 with a 1 A list buffer, 3 x 20 steps, staged builds {staged}", flush=True)
+    a = _run(1.0, (20, 20, 20), batched=False, staged=staged)
+    b = _run(1.0, (20, 20, 20), batched=True, staged=staged)
+    for name, r in (("alone", a), ("batch", b)):
+        print(name, "repeats", [m["repeats"] for _, m, _ in r], "graph steps",
+              [m["graph_steps"] for _, m, _ in r], "builds", [e["list_builds"] for _, _, e in r],
+              flush=True)
     _same_states(a, b)
     assert [m["graph_steps"] for _, m, _ in a] == [32] * len(a)
     assert [m["graph_steps"] for _, m, _ in b] == [32] * len(b)
     assert all(e["list_builds"] > 1 for _, _, e in a)
     print("the same members stepped with direct launches on one stream", flush=True)
-    c = _run(1.0, (20, 20, 20), batched=True, profile=True)
+    c = _run(1.0, (20, 20, 20), batched=True, profile=True, staged=staged)
     _same_states(a, c)
     assert all(m["graph_steps"] == 0 for _, m, _ in c)
 

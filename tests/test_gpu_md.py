@@ -17,10 +17,11 @@ F64 = np.float64
 
 
 def _setup(s, ff, mode=api.MODE_REFERENCE, K=(27, 27, 27), rc=10.0, gamma=1.0, dt=0.002,
-           buffer=1.0, seed=5):
+           buffer=1.0, seed=5, staged=False):
     eng = api.Engine(s, cutoff=rc, lj_cutoff=rc, beta=BETA, grid=K, order=4, mode=mode,
                      list_buffer=buffer)
-    m = api.Md(eng, ff, dt=dt, temperature=300.0, gamma=gamma, shake_tol=1e-5, seed=seed)
+    m = api.Md(eng, ff, dt=dt, temperature=300.0, gamma=gamma, shake_tol=1e-5, seed=seed,
+               staged_builds=staged)
     v, vw = systems.initial_velocities(s, ff)
     m.set_state(s.pos_dyn, v, systems.water_positions(s), vw)
     return eng, m, v, vw
@@ -30,10 +31,11 @@ def _min_image(d, box):
     return d - box * np.round(d / box)
 
 
-def test_steps_vs_fp64(gpu):
+@pytest.mark.parametrize("staged", [False, True])
+def test_steps_vs_fp64(gpu, staged):
     s = systems.small_solvated()
     ff = systems.chain_force_field(s)
-    eng, m, v, vw = _setup(s, ff)
+    eng, m, v, vw = _setup(s, ff, staged=staged)
     model = md_fp64.Model(s, ff, 10.0, BETA, (27, 27, 27), dt=0.002, gamma=1.0, seed=5)
     st = md_fp64.MdState(s.pos_dyn.astype(F64), v.astype(F64),
                          systems.water_positions(s).astype(F64), vw.astype(F64))
@@ -51,13 +53,20 @@ def test_steps_vs_fp64(gpu):
         assert dx_dyn < 1e-4 and dx_w < 1e-4
         assert dv < 2e-3
     # the last step of each call evaluates nonbonded energies: a fresh
-    # evaluation at the same positions agrees bit for bit
+    # evaluation at the same positions agrees — it may use the list staged
+    # during that step (the same pairs, summed in another order), so counts
+    # agree exactly and energies to f32 summation order
     e_step = eng.energies()
     eng.compute(api.COMPUTE_ALL)
     e_again = eng.energies()
-    for k in ("list_builds",):
+    for k in ("list_builds", "list_entries"):
         e_step.pop(k), e_again.pop(k)
-    assert e_again == e_step
+    for k, v in e_step.items():
+        if isinstance(v, int):
+            assert e_again[k] == v, k
+        else:
+            a, b = np.asarray(e_again[k], np.float64), np.asarray(v, np.float64)
+            assert np.all(np.abs(a - b) <= 1e-7 * (np.abs(b) + 1.0)), (k, v, e_again[k])
     m.close()
     eng.close()
 
@@ -211,3 +220,55 @@ def test_joined_steps_bit_exact(gpu, gamma, nst):
             assert np.array_equal(a, b)
     if nst:
         assert ref[2] == 2
+
+
+def _unlisted_pairs(s, dbg, rc=10.0):
+    """Interacting site pairs within rc, at the slot positions the exported
+    list was last used with, that the list lacks in either direction."""
+    from conftest import NULL_ENTRY
+    from test_oracle_list import INTERACT, _classes_gids
+    slot_site = dbg["slot_site"].astype(np.int64)
+    ns = s.n_dyn + 4 * s.n_water
+    cls, gid = _classes_gids(s)
+    ok = slot_site >= 0
+    slot_of = np.empty(ns, np.int64)
+    slot_of[slot_site[ok]] = np.nonzero(ok)[0]
+    p = np.empty((ns, 3))
+    p[slot_site[ok]] = dbg["xyzq"][ok, :3]
+    row_ptr = dbg["row_ptr"].astype(np.int64)
+    cj = dbg["cj"][:8 * row_ptr[-1]]
+    jslot = (cj & NULL_ENTRY).astype(np.int64)
+    ent_row = np.repeat(np.repeat(np.arange(len(row_ptr) - 1), np.diff(row_ptr)), 8)
+    listed = set(zip(ent_row.tolist(), jslot.tolist()))
+    box = np.asarray(s.box, np.float64)
+    missing = []
+    for a in range(ns):
+        d = p[a + 1:] - p[a]
+        d -= box * np.round(d / box)
+        r2 = (d * d).sum(1)
+        b = np.nonzero(r2 < rc * rc)[0] + a + 1
+        b = b[INTERACT[cls[a], cls[b]] & (gid[b] != gid[a])]
+        for bb in b:
+            if (slot_of[a] // 8, slot_of[bb]) not in listed or (slot_of[bb] // 8, slot_of[a]) not in listed:
+                missing.append((a, int(bb)))
+    return missing
+
+
+def test_staged_lists_cover_cutoff(gpu):
+    """Staged list builds (beside the pair kernel, current from the next
+    evaluation on): after each call the list the last evaluation used holds
+    every interacting pair within the cutoff at the positions it was used
+    with, and its energies equal a fresh evaluation's at those positions."""
+    s = systems.small_solvated()
+    ff = systems.chain_force_field(s)
+    eng, m, _, _ = _setup(s, ff, buffer=1.0, staged=True)
+    for call, n in enumerate((1, 2, 3, 5)):
+        m.step(n)
+        dbg = eng.debug_export()
+        miss = _unlisted_pairs(s, dbg)
+        e = eng.energies()
+        print(f"call {call}: {n} steps, builds {e['list_builds']}, pairs {e['pair_count']}, "
+              f"unlisted {len(miss)} {miss[:4]}", flush=True)
+        assert not miss
+    m.close()
+    eng.close()
