@@ -9,11 +9,11 @@ change its speed but not the Hamiltonian beyond a stated tolerance:
   the requested reference discretisation's, the one whose SPME evaluation
   takes the least time;
 * the pair-list buffer (the rebuild interval against the pair kernel's lane
-  efficiency), the work-item length of the pair kernel (j groups per wave)
-  and how steps are enqueued (graphs or direct launches, one stream or
-  two): the combination
-  whose MD steps take the least time, each timed over whole step calls
-  after the call that records the step graphs.
+  efficiency), the work-item length of the pair kernel (j groups per wave),
+  how steps are enqueued (graphs or direct launches, one stream or two) and
+  where list builds run (before the pair kernel, or staged beside it): the
+  combination whose MD steps take the least time, each timed over whole
+  step calls after the call that records the step graphs.
 
 The reference fixes all of these (SPME order 4 in the ewald crate, a 4 Å
 neighbor skin, src/config.rs:127, :205-210); the tuner is this engine's own.
@@ -78,7 +78,7 @@ def tune_spme(system, cutoff, beta, grid, order=4, spacings=(1.25, 1.1, 1.0, 0.9
 
 
 def time_md(system, ff, state, cutoff, beta, grid, order, list_buffer, item_groups, steps=40,
-            seed=1, direct_launch=False, single_stream=False):
+            seed=1, direct_launch=False, single_stream=False, staged_builds=False):
     """Milliseconds per MD step (Langevin-middle, dt 2 fs, 300 K) from a
     thermal state, timed over one call of `steps` after two untimed calls
     (the second records the step graphs)."""
@@ -86,7 +86,7 @@ def time_md(system, ff, state, cutoff, beta, grid, order, list_buffer, item_grou
                      mode=api.MODE_REFERENCE, list_buffer=list_buffer, item_groups=item_groups,
                      single_stream=single_stream)
     md = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=1.0, shake_tol=1e-5, seed=seed,
-                direct_launch=direct_launch)
+                direct_launch=direct_launch, staged_builds=staged_builds)
     md.set_state(*state)
     md.step(steps)
     md.step(steps)
@@ -103,8 +103,9 @@ def time_md(system, ff, state, cutoff, beta, grid, order, list_buffer, item_grou
 
 
 def tune_md(system, ff, state, cutoff=10.0, beta=0.3123, grid=(64, 64, 64), order=4,
-            buffers=(0.8, 1.0, 1.25, 1.5, 2.0, 2.5), item_groups=(8, 16, 32), steps=40,
-            spme=True, launch_modes=(False, True), stream_modes=(False, True)):
+            buffers=(1.0, 1.5, 2.0, 2.5, 3.0), item_groups=(8, 16, 32), steps=40,
+            spme=True, launch_modes=(False, True), stream_modes=(False, True),
+            staged_modes=(False, True)):
     """The tuned settings for MD of `system` from `state` (pos_dyn, vel_dyn,
     pos_water, vel_water): a dict with the choice and the measured table."""
     table = {}
@@ -114,21 +115,23 @@ def tune_md(system, ff, state, cutoff=10.0, beta=0.3123, grid=(64, 64, 64), orde
         choice = {"order": best["order"], "grid": best["grid"]}
         table["spme"] = rows
     md_rows = []
-    for one in stream_modes:
-        for direct in launch_modes:
-            for b in buffers:
-                if cutoff + b >= 0.5 * float(min(system.box)):
-                    continue
-                for g in item_groups:
-                    ms, builds = time_md(system, ff, state, cutoff, beta, tuple(choice["grid"]),
-                                         choice["order"], b, g, steps, direct_launch=direct,
-                                         single_stream=one)
-                    md_rows.append({"list_buffer": b, "item_groups": g, "direct_launch": direct,
-                                    "single_stream": one, "ms_per_step": ms,
-                                    "list_builds_per_step": builds / steps})
+    for staged in staged_modes:
+        for one in stream_modes:
+            for direct in launch_modes:
+                for b in buffers:
+                    if cutoff + b >= 0.5 * float(min(system.box)):
+                        continue
+                    for g in item_groups:
+                        ms, builds = time_md(system, ff, state, cutoff, beta,
+                                             tuple(choice["grid"]), choice["order"], b, g, steps,
+                                             direct_launch=direct, single_stream=one,
+                                             staged_builds=staged)
+                        md_rows.append({"list_buffer": b, "item_groups": g,
+                                        "direct_launch": direct, "single_stream": one,
+                                        "staged_builds": staged, "ms_per_step": ms,
+                                        "list_builds_per_step": builds / steps})
     best = min(md_rows, key=lambda r: r["ms_per_step"])
-    choice.update({"list_buffer": best["list_buffer"], "item_groups": best["item_groups"],
-                   "direct_launch": best["direct_launch"], "single_stream": best["single_stream"],
-                   "ms_per_step": best["ms_per_step"]})
+    choice.update({k: best[k] for k in ("list_buffer", "item_groups", "direct_launch",
+                                        "single_stream", "staged_builds", "ms_per_step")})
     table["md"] = md_rows
     return {"choice": choice, "table": table}
