@@ -89,15 +89,25 @@ __device__ __forceinline__ void call_body(const P& p, std::index_sequence<I...>)
   F(pack_get<I>(p)...);
 }
 
+// Waves per SIMD that body F asks the register allocator to leave room for
+// (1: no request).  A body specialises this when latency, not issue, bounds
+// it and more resident waves hide that latency.
+template <auto F>
+struct MinWavesPerSimd {
+  static constexpr int value = 1;
+};
+
 template <auto F, int LB>
-__global__ void __launch_bounds__(LB) k_batched(const typename BodySig<F>::P* __restrict__ a) {
+__global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(MinWavesPerSimd<F>::value)))
+k_batched(const typename BodySig<F>::P* __restrict__ a) {
   call_body<F>(a[blockIdx.y], std::make_index_sequence<BodySig<F>::N>{});
 }
 
 // A single system: the pack travels as the kernel argument itself (kernarg
 // segment, no dependent load at kernel start).
 template <auto F, int LB>
-__global__ void __launch_bounds__(LB) k_single(const typename BodySig<F>::P a) {
+__global__ void __launch_bounds__(LB) __attribute__((amdgpu_waves_per_eu(MinWavesPerSimd<F>::value)))
+k_single(const typename BodySig<F>::P a) {
   call_body<F>(a, std::make_index_sequence<BodySig<F>::N>{});
 }
 

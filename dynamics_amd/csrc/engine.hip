@@ -88,6 +88,20 @@ struct EngGeom {
   uint32_t n_dyn, n_water;
 };
 
+// Class cls's grid by selects (an indexed copy out of an argument pack held
+// in memory would go through scratch memory).
+__device__ __forceinline__ ClassGrid class_grid(const EngGeom& g, int cls) {
+  ClassGrid r;
+  r.ncx = cls == 0 ? g.cg[0].ncx : (cls == 1 ? g.cg[1].ncx : g.cg[2].ncx);
+  r.ncy = cls == 0 ? g.cg[0].ncy : (cls == 1 ? g.cg[1].ncy : g.cg[2].ncy);
+  r.wx = cls == 0 ? g.cg[0].wx : (cls == 1 ? g.cg[1].wx : g.cg[2].wx);
+  r.wy = cls == 0 ? g.cg[0].wy : (cls == 1 ? g.cg[1].wy : g.cg[2].wy);
+  r.col_off = cls == 0 ? g.cg[0].col_off : (cls == 1 ? g.cg[1].col_off : g.cg[2].col_off);
+  r.n = cls == 0 ? g.cg[0].n : (cls == 1 ? g.cg[1].n : g.cg[2].n);
+  r.site_base = cls == 0 ? g.cg[0].site_base : (cls == 1 ? g.cg[1].site_base : g.cg[2].site_base);
+  return r;
+}
+
 // Whether this evaluation runs a list build.  The host asks for a build
 // (force: the first evaluation, a new box, positions set from the host, no
 // list buffer), or the device decides from words of the engine's flags that
@@ -379,7 +393,9 @@ colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __rest
       hi[2] = fmaxf(hi[2], p.z);
     }
     bb_c[cl] = make_float4(0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2]), 0.f);
-    bb_h[cl] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]), 0.f);
+    // w: the cluster's sites (its first slots; the rest are padding)
+    bb_h[cl] = make_float4(0.5f * (hi[0] - lo[0]), 0.5f * (hi[1] - lo[1]), 0.5f * (hi[2] - lo[2]),
+                           __uint_as_float(min(n - k * DYN_CL_SIZE, (uint32_t)DYN_CL_SIZE)));
   }
 }
 
@@ -427,22 +443,38 @@ struct ListArgs {
   uint32_t* tot;                     // groups, work items, longest row (from the scan)
 };
 
-// LDS of the row walk: a batch of up to 64 columns (first cluster, range in
-// the flattened candidate sequence), then the batch's candidate clusters
-// that passed the box test.
-// For a solute row, also the exclusion partners of I's sites as a hash
-// table (linear probing, at most half full): key slot << 8 | the bits of
-// the i slots the partner is excluded from.
+// LDS of the row walk, per wave: I's slot positions, a batch of up to 64
+// columns (first cluster, range in the flattened candidate sequence, image
+// seen from I), the codes of a sweep's candidates that passed the box test,
+// and for a solute row the exclusion partners of I's sites as a hash table
+// (linear probing, at most half full): key slot << 8 | the bits of the i
+// slots the partner is excluded from.
 constexpr int kExHash = 1024;
 constexpr uint32_t kExEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kExPartners = kExHash / 2;  // partners of one cluster's sites
+constexpr int kListBlock = 256;                // four waves, each on its own rows
+constexpr int kListWaves = kListBlock / kWave;
+constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk (rows beyond: grid stride)
 struct WalkLds {
+  float4 pI[DYN_CL_SIZE];  // I's slot positions (padding slots: NaN)
   uint32_t js[kWave], beg[kWave], end[kWave];
-  float sx[kWave], sy[kWave];  // the column's periodic image offset seen from I
-  uint32_t okJ[kWave];
-  float2 okS[kWave];
+  uint32_t img[kWave];     // the column's periodic image seen from I (cand_code bits 21-24)
+  uint32_t ok[kWave];      // cand_code of the sweep's passing candidates, in walk order
   uint32_t ex[kExHash];
 };
+
+// A passing candidate in one word: its cluster (bits 0-20; the slot count
+// is below 2^24), its column's image in x and y (bits 21-22, 23-24: k + 1,
+// the shift being k·L), in z (bits 25-26: k + 1 for the image of its
+// bounding box nearest I's, used for every slot when no slot pair can be
+// within rlist in another image; 3: each slot pair's own minimum image) and
+// its number of sites less one (bits 27-29).
+constexpr uint32_t kCandJ = (1u << 21) - 1u;
+constexpr uint32_t kZSlot = 3u;
+__device__ __forceinline__ float cand_shift(uint32_t code, int sh, float ext) {
+  return (float)((int)((code >> sh) & 3u) - 1) * ext;
+}
+
 __device__ __forceinline__ uint32_t ex_hash(uint32_t slot) {
   return (slot * 2654435761u) >> (32 - 10);  // 10 bits: kExHash = 1024
 }
@@ -458,24 +490,6 @@ __device__ __forceinline__ uint32_t ex_bits(const uint32_t* ex, uint32_t slot) {
   }
 }
 
-// Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
-// order (class, column x, column y, cluster), keep those whose bounding box
-// is within rlist of I's box (periodic), test their 8 slots against I's 8
-// slots (eight candidates per round, a lane per j slot), and append every
-// j slot within rlist of at least one non-padding i slot.  A candidate
-// column is seen as the periodic image the column walk reached it in, so x
-// and y differences need no minimum image (z does; so do x and y when the
-// column range spans the whole box).  Slots near only their own water
-// molecule's sites may be listed: the pair kernel masks such pairs.  Each
-// class segment is padded to a multiple of 8 with the null slot.  The row
-// goes into its fixed-capacity slice of ent_tmp, each entry's exclusion
-// byte (solute–solute entries: bit ia set when the j site is a 1-2, 1-3 or
-// 1-4 partner of i slot ia's site) into the same place of xb_tmp; counts
-// are in groups of 8.  The last workgroup to finish scans the row counts:
-// row_ptr and chunk_ptr, and the totals.
-constexpr int kListBlock = 256;  // four waves, each on its own rows
-constexpr int kListWaves = kListBlock / kWave;
-
 // The LDS accesses of one wave stay in program order around this point (a
 // wave's lanes run in lockstep; the rows of one workgroup's waves are
 // independent, so no workgroup barrier is needed).
@@ -485,8 +499,37 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive sum over the wave's lanes (DPP: shifts within rows of 16 lanes,
+// then the row totals broadcast to the rows above).
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
+// order (class, column x, column y, cluster), keep those whose bounding box
+// is within rlist of I's box (periodic), test their 8 slots against I's 8
+// slots (eight candidates per round, a lane per j slot), and append every
+// j slot within rlist of at least one non-padding i slot.  A candidate
+// column is seen as the periodic image the column walk reached it in, so x
+// and y differences need no minimum image (z neither when rlist and both
+// boxes' half-heights are short of half the box: then only the image of
+// the candidate's box nearest I's can hold a slot pair within rlist; x and y
+// do when the column range spans the whole box).  Slots near only their own
+// water molecule's sites may be listed: the pair kernel masks such pairs.
+// Each class segment is padded to a multiple of 8 with the null slot.  The
+// row goes into its fixed-capacity slice of ent_tmp, each entry's exclusion
+// byte (solute–solute entries: bit ia set when the j site is a 1-2, 1-3 or
+// 1-4 partner of i slot ia's site) into the same place of xb_tmp; counts
+// are in groups of 8.  The last workgroup to finish scans the row counts:
+// row_ptr and chunk_ptr, and the totals.
 __device__ __forceinline__ void
-k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
+k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
                 uint32_t* __restrict__ row_chunks, uint32_t* __restrict__ row_ptr,
                 uint32_t* __restrict__ chunk_ptr) {
   __shared__ WalkLds Ls[kListWaves];
@@ -495,6 +538,8 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned long long lt = (1ull << lane) - 1ull;
+  const int jj = lane & (DYN_CL_SIZE - 1);             // j slot of the lane in a round
+  const uint32_t jc = (uint32_t)(lane / DYN_CL_SIZE);  // candidate of the lane in a round
   // a column is skipped when its x–y rectangle is beyond rlist of I's box;
   // the margin keeps the test conservative against rounding of the binning
   const float col_r2 = A.rlist2 * 1.0001f + 1e-3f;
@@ -504,15 +549,12 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
     uint8_t* xb = A.xb_tmp + (size_t)I * A.row_cap;
     const int ci = A.cl_class[I];
     const float4 c = A.bb_c[I], h = A.bb_h[I];
-    float pIx[DYN_CL_SIZE], pIy[DYN_CL_SIZE], pIz[DYN_CL_SIZE];
-    uint32_t iv = 0;  // bit u: i slot u holds a site
-#pragma unroll
-    for (int u = 0; u < DYN_CL_SIZE; ++u) {
-      const float4 p = A.xyzq[I * DYN_CL_SIZE + u];
-      pIx[u] = p.x;
-      pIy[u] = p.y;
-      pIz[u] = p.z;
-      if (A.slot_gid[I * DYN_CL_SIZE + u] != DYN_GID_PAD) iv |= 1u << u;
+    wave_sync();  // the previous row's reads of L are done
+    // a padding i slot's position is NaN, so no distance to it is below rlist
+    if (lane < DYN_CL_SIZE) {
+      float4 p = A.xyzq[I * DYN_CL_SIZE + lane];
+      if ((uint32_t)lane >= __float_as_uint(h.w)) p.x = p.y = p.z = __int_as_float(0x7fc00000);
+      L.pI[lane] = p;
     }
     // solute row: the exclusion partners of its sites, by slot
     uint32_t n_ex = 0;
@@ -525,12 +567,8 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
           len = A.ex_ptr[sa + 1] - ex0;
         }
       }
-      uint32_t incl = len;
-      for (int off = 1; off < DYN_CL_SIZE; off <<= 1) {
-        const uint32_t v = __shfl_up(incl, off, kWave);
-        if (lane >= off) incl += v;
-      }
-      n_ex = (uint32_t)__shfl(incl, DYN_CL_SIZE - 1, kWave);
+      const uint32_t incl = wave_incl_sum(len);
+      n_ex = (uint32_t)__builtin_amdgcn_readlane((int)incl, DYN_CL_SIZE - 1);
       if (n_ex > kExPartners) {
         if (lane == 0) atomicOr(A.status, kStMask);
         n_ex = 0;
@@ -542,8 +580,9 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
       for (uint32_t t0 = 0; t0 < n_ex; t0 += kWave) {
         const uint32_t t = t0 + lane;
         int ia = 0;
+#pragma unroll
         for (int u = 1; u < DYN_CL_SIZE; ++u)
-          if ((uint32_t)__shfl(beg_l, u, kWave) <= t) ia = u;
+          if ((uint32_t)__builtin_amdgcn_readlane((int)beg_l, u) <= t) ia = u;
         const uint32_t e0 = (uint32_t)__shfl(ex0, ia, kWave);
         const uint32_t b0 = (uint32_t)__shfl(beg_l, ia, kWave);
         if (t < n_ex) {
@@ -562,14 +601,15 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
           }
         }
       }
-      wave_sync();
     }
+    wave_sync();
     uint32_t n_out = 0;
     for (int cls = 0; cls < 3; ++cls) {
       if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
-      const ClassGrid cg = A.g.cg[cls];
+      const ClassGrid cg = class_grid(A.g, cls);
       if (cg.n == 0) continue;
       const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
+      const bool exl = n_ex != 0 && cls == DYN_CLS_SOLUTE;
       int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
       int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
       bool full_x = false, full_y = false;
@@ -583,11 +623,11 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
         y1 = cg.ncy - 1;
         full_y = true;
       }
+      const bool full_xy = full_x || full_y;
       const int ny = y1 - y0 + 1, n_colc = (x1 - x0 + 1) * ny;
       for (int cb = 0; cb < n_colc; cb += kWave) {
         const int k = cb + lane;
-        uint32_t js = 0, cnt = 0;
-        float sx = 0.f, sy = 0.f;
+        uint32_t js = 0, cnt = 0, img = 0;
         if (k < n_colc) {
           const int xx = x0 + k / ny, yy = y0 + k % ny;
           float dx = 0.f, dy = 0.f;
@@ -599,120 +639,95 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
             const int col = cg.col_off + cx * cg.ncy + cy;
             js = A.col_cl_start[col];
             cnt = A.col_cl_start[col + 1] - js;
-            sx = (float)((xx - cx) / cg.ncx) * A.g.L[0];
-            sy = (float)((yy - cy) / cg.ncy) * A.g.L[1];
+            img = (uint32_t)((xx - cx) / cg.ncx + 1) << 21 | (uint32_t)((yy - cy) / cg.ncy + 1) << 23;
           }
         }
-        uint32_t incl = cnt;
-        for (int off = 1; off < kWave; off <<= 1) {
-          const uint32_t v = __shfl_up(incl, off, kWave);
-          if (lane >= off) incl += v;
-        }
-        const uint32_t tot = (uint32_t)__shfl(incl, kWave - 1, kWave);
+        const uint32_t incl = wave_incl_sum(cnt);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
         L.js[lane] = js;
         L.beg[lane] = incl - cnt;
         L.end[lane] = incl;
-        L.sx[lane] = sx;
-        L.sy[lane] = sy;
+        L.img[lane] = img;
         wave_sync();
-        // candidate t of the batch (t < tot): its cluster, the image shift of
-        // its column and its bounding box; the next 64 candidates' are
-        // loaded while the current ones are tested
-        auto cand = [&](uint32_t t, uint32_t& J, float2& sh, float4& cJ, float4& hJ) {
-          int lo = 0, hi = kWave - 1;  // first column whose range ends after t
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (L.end[mid] > t) hi = mid;
-            else lo = mid + 1;
-          }
-          J = L.js[lo] + (t - L.beg[lo]);
-          sh = make_float2(L.sx[lo], L.sy[lo]);
-          cJ = A.bb_c[J];
-          hJ = A.bb_h[J];
-        };
-        uint32_t Jn = 0;
-        float2 shn = make_float2(0.f, 0.f);
-        float4 cJn = make_float4(0.f, 0.f, 0.f, 0.f), hJn = cJn;
-        if ((uint32_t)lane < tot) cand(lane, Jn, shn, cJn, hJn);
         for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
+          // candidate t: its cluster (first column whose range ends after
+          // t), the image of its column and its bounding box
           const uint32_t t = t0 + lane;
           bool ok = t < tot;
-          const uint32_t J = Jn;
-          const float2 sh = shn;
-          const float4 cJ = cJn, hJ = hJn;
-          if (t + kWave < tot) cand(t + kWave, Jn, shn, cJn, hJn);
+          uint32_t code = 0;
           if (ok) {
+            int lo = 0, hi = kWave - 1;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (L.end[mid] > t) hi = mid;
+              else lo = mid + 1;
+            }
+            const uint32_t J = L.js[lo] + (t - L.beg[lo]);
+            const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
             float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
+            const float kz = rintf(dz * A.g.invL[2]);
             dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
             dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
-            dz = fabsf(dz - A.g.L[2] * rintf(dz * A.g.invL[2]));
+            dz = fabsf(dz - A.g.L[2] * kz);
+            const uint32_t zc =
+                A.rlist + h.z + hJ.z < 0.45f * A.g.L[2] ? (uint32_t)((int)kz + 1) : kZSlot;
             dx = fmaxf(0.f, dx - h.x - hJ.x);
             dy = fmaxf(0.f, dy - h.y - hJ.y);
             dz = fmaxf(0.f, dz - h.z - hJ.z);
             ok = dx * dx + dy * dy + dz * dz < A.rlist2;
+            code = J | L.img[lo] | zc << 25 | (__float_as_uint(hJ.w) - 1u) << 27;
           }
           const unsigned long long bal = __ballot(ok);
           const uint32_t n_ok = (uint32_t)__popcll(bal);
-          if (ok) {
-            L.okJ[__popcll(bal & lt)] = J;
-            L.okS[__popcll(bal & lt)] = sh;
-          }
+          if (ok) L.ok[__popcll(bal & lt)] = code;
           wave_sync();
-          // eight candidate clusters per round: lane jb + 8·c tests j slot
-          // jb of candidate c against all eight i slots (held in registers),
-          // so the kept slots come out candidate by candidate, each in slot
-          // order, as a one-candidate-at-a-time walk would write them; the
-          // next round's slots are loaded while this round's are tested
-          const int jj = lane & (DYN_CL_SIZE - 1);
-          auto slots = [&](uint32_t k0, uint32_t& Jr, float2& shr, float4& pj, bool& jv) {
-            const uint32_t kk = k0 + (uint32_t)(lane / DYN_CL_SIZE);
-            jv = false;
-            if (kk < n_ok) {
-              Jr = L.okJ[kk];
-              shr = L.okS[kk];
-              pj = A.xyzq[Jr * DYN_CL_SIZE + jj];
-              jv = A.slot_gid[Jr * DYN_CL_SIZE + jj] != DYN_GID_PAD;
-            }
-          };
-          uint32_t Jr_n = 0;
-          float2 shr_n = make_float2(0.f, 0.f);
-          float4 pj_n = make_float4(0.f, 0.f, 0.f, 0.f);
-          bool jv_n = false;
-          slots(0, Jr_n, shr_n, pj_n, jv_n);
+          // eight candidate clusters per round: lane jj + 8·jc tests j slot
+          // jj of candidate jc against all eight i slots, so the kept slots
+          // come out candidate by candidate, each in slot order, as a
+          // one-candidate-at-a-time walk would write them
           for (uint32_t k0 = 0; k0 < n_ok; k0 += kWave / DYN_CL_SIZE) {
-            const uint32_t Jr = Jr_n;
-            const float2 shr = shr_n;
-            const float4 pj = pj_n;
-            const bool jv = jv_n;
-            if (k0 + kWave / DYN_CL_SIZE < n_ok) slots(k0 + kWave / DYN_CL_SIZE, Jr_n, shr_n, pj_n, jv_n);
+            const uint32_t kk = k0 + jc;
+            uint32_t cd = 0;
+            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (kk < n_ok) {
+              cd = L.ok[kk];
+              pj = A.xyzq[(cd & kCandJ) * DYN_CL_SIZE + jj];
+            }
+            const bool jv = kk < n_ok && (uint32_t)jj <= ((cd >> 27) & 7u);
+            const bool zslot = jv && ((cd >> 25) & 3u) == kZSlot;
             bool keep = false;
-            if (jv) {
-              if (!full_x && !full_y) {
-                const float px = pj.x + shr.x, py = pj.y + shr.y;
+            if (!full_xy && __ballot(zslot) == 0ull) {
+              const float px = pj.x + cand_shift(cd, 21, A.g.L[0]);
+              const float py = pj.y + cand_shift(cd, 23, A.g.L[1]);
+              const float pz = pj.z + cand_shift(cd, 25, A.g.L[2]);
 #pragma unroll
-                for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                  const float d0 = pIx[u] - px;
-                  const float d1 = pIy[u] - py;
-                  const float d2 = dyn_min_image_rint(pIz[u] - pj.z, A.g.L[2], A.g.invL[2]);
-                  const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                  keep = keep || (r2 < A.rlist2 && ((iv >> u) & 1u));
-                }
-              } else {
+              for (int u = 0; u < DYN_CL_SIZE; ++u) {
+                const float4 q = L.pI[u];
+                const float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
+                keep |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
+              }
+            } else {
+              const float px = full_xy ? pj.x : pj.x + cand_shift(cd, 21, A.g.L[0]);
+              const float py = full_xy ? pj.y : pj.y + cand_shift(cd, 23, A.g.L[1]);
+              const float pz = zslot ? pj.z : pj.z + cand_shift(cd, 25, A.g.L[2]);
 #pragma unroll
-                for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                  const float d0 = dyn_min_image_rint(pIx[u] - pj.x, A.g.L[0], A.g.invL[0]);
-                  const float d1 = dyn_min_image_rint(pIy[u] - pj.y, A.g.L[1], A.g.invL[1]);
-                  const float d2 = dyn_min_image_rint(pIz[u] - pj.z, A.g.L[2], A.g.invL[2]);
-                  const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
-                  keep = keep || (r2 < A.rlist2 && ((iv >> u) & 1u));
+              for (int u = 0; u < DYN_CL_SIZE; ++u) {
+                const float4 q = L.pI[u];
+                float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
+                if (full_xy) {
+                  d0 = dyn_min_image_rint(d0, A.g.L[0], A.g.invL[0]);
+                  d1 = dyn_min_image_rint(d1, A.g.L[1], A.g.invL[1]);
                 }
+                if (zslot) d2 = dyn_min_image_rint(d2, A.g.L[2], A.g.invL[2]);
+                keep |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
               }
             }
+            keep = keep && jv;
             const unsigned long long kb = __ballot(keep);
             if (keep) {
               const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
-              const uint32_t slot = Jr * DYN_CL_SIZE + jj;
-              const uint32_t xbits = (n_ex && cls == DYN_CLS_SOLUTE) ? ex_bits(L.ex, slot) : 0u;
+              const uint32_t slot = (cd & kCandJ) * DYN_CL_SIZE + jj;
+              const uint32_t xbits = exl ? ex_bits(L.ex, slot) : 0u;
               if (pos < A.row_cap) {
                 ent[pos] = slot | tag;
                 xb[pos] = (uint8_t)xbits;
@@ -720,7 +735,7 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
             }
             n_out += (uint32_t)__popcll(kb);
           }
-          wave_sync();
+          wave_sync();  // L.ok is rewritten by the next sweep
         }
       }
       const uint32_t pad = (DYN_CL_SIZE - n_out % DYN_CL_SIZE) % DYN_CL_SIZE;
@@ -745,6 +760,18 @@ k_eng_list_walk(ListArgs A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict
                              A.tot + 2);
 }
 
+}  // namespace
+
+// The walk is bound by the latency of its dependent loads: room for six
+// waves per SIMD (six workgroups per compute unit, within its LDS), so that
+// one launch holds every row of a 37,000-site system at once.
+template <>
+struct dyn::MinWavesPerSimd<k_eng_list_walk> {
+  static constexpr int value = 6;
+};
+
+namespace {
+
 // The two copies of the per-group arrays (the pair kernel reads the one
 // flags[kFlSel] names while a staged build fills the other).
 struct ListCopies {
@@ -764,7 +791,7 @@ static_assert(sizeof(ListCopies) == 64, "ListCopies has no padding");
 // sites of i slot ia and of the group's entry jb are a 1-2, 1-3 or 1-4 pair
 // (entry jb's exclusion byte, shifted into place).
 __device__ __forceinline__ void
-k_eng_list_fill(ListArgs A, const uint32_t* __restrict__ ent_tmp,
+k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
                 const uint32_t* __restrict__ chunk_ptr, ListCopies C) {
   if (!build_on(A.gate)) return;
@@ -1955,12 +1982,16 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     A.done = e.dflags.p + kFlWalkDone;
     A.tot = T[i].counters + 1;
   }
+  // one row per wave; the walk's grid holds every row at once
+  const unsigned walk_grid = max_grid(S, [](dyn_engine& e) {
+    return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kWalkGrid);
+  });
   const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
     return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kBuildGrid);
   });
   {
     Profiler::Scope ps(prof, "nbl_list_walk");
-    launch<k_eng_list_walk, kListBlock>(R, "k_eng_list_walk", 0, list_grid, kListBlock, 0,
+    launch<k_eng_list_walk, kListBlock>(R, "k_eng_list_walk", 0, walk_grid, kListBlock, 0,
                                         [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p,

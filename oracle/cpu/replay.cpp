@@ -497,13 +497,20 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
           const float sx = (float)((xx - cx) / g.ncx[k]) * L[0];
           const float sy = (float)((yy - cy) / g.ncy[k]) * L[1];
           for (uint32_t J = col_cl[cc]; J < col_cl[cc + 1]; ++J) {
-            float d[3];
+            float d[3], kz = 0.f;
             for (int a = 0; a < 3; ++a) {
               float v = c[a] - bbc[3 * J + a];
-              v = std::fabs(v - L[a] * std::rint(v * invL[a]));
+              const float k = std::rint(v * invL[a]);
+              if (a == 2) kz = k;
+              v = std::fabs(v - L[a] * k);
               d[a] = std::fmax(0.f, v - h[a] - bbh[3 * J + a]);
             }
             if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rlist2)) continue;
+            // z: through the image of J's box nearest I's when no slot pair
+            // can be within rlist in another image (rlist and both
+            // half-heights short of half the box), else per slot pair
+            const bool zc = rlist + h[2] + bbh[3 * J + 2] < 0.45f * L[2];
+            const float sz = kz * L[2];
             for (int jb = 0; jb < 8; ++jb) {
               const size_t js = (size_t)J * DYN_CL_SIZE + jb;
               float pj[3];
@@ -522,7 +529,8 @@ int orc_engine_list(uint32_t n_dyn, uint32_t n_water, const float* pos4, const f
                   d0 = dyn_min_image_rint(pi[ia][0] - pj[0], L[0], invL[0]);
                   d1 = dyn_min_image_rint(pi[ia][1] - pj[1], L[1], invL[1]);
                 }
-                const float d2 = dyn_min_image_rint(pi[ia][2] - pj[2], L[2], invL[2]);
+                const float d2 = zc ? pi[ia][2] - (pj[2] + sz)
+                                    : dyn_min_image_rint(pi[ia][2] - pj[2], L[2], invL[2]);
                 const float r2 = dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0));
                 keep = r2 < rlist2;
               }
