@@ -63,8 +63,7 @@ constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u,
 // Words of an engine's device flags (dflags).
 constexpr int kFlDisp = 0;       // largest squared displacement this evaluation (float bits)
 constexpr int kFlBuilds = 1;     // list builds so far
-constexpr int kFlBinDone = 2;    // finished workgroups of the binning launch
-constexpr int kFlWalkDone = 3;   // finished workgroups of the walk launch
+// words 2 and 3 are unused
 constexpr int kFlReady = 4;      // 1: the staging list holds a finished build
 constexpr int kFlSel = 5;        // which copy of cj, cmask and chunks the last evaluation used
 constexpr int kFlCommit = 6;     // 1: this evaluation's preparation kernel took the staged list
@@ -172,42 +171,44 @@ __device__ __forceinline__ bool last_workgroup(unsigned int* __restrict__ done) 
   return true;
 }
 
-// Exclusive scans of two uint32 sequences of length n by one workgroup of NT
-// threads (after last_workgroup's acquire fence, plain loads see the other
-// workgroups' writes).  MODE 0 (columns): a = site count, b = ceil(a/8) clusters.  MODE
-// 1 (rows): a = entries, b = work items.  Writes out_x[0..n] with out_x[n] =
-// total, the totals to tot_x and max(a) to *max_a (each when non-null).
-template <int MODE, int NT>
-__device__ __forceinline__ void scan2_block(const uint32_t* __restrict__ in_a,
-                                            const uint32_t* __restrict__ in_b, uint32_t n,
-                                            uint32_t* __restrict__ out_a,
-                                            uint32_t* __restrict__ out_b,
-                                            uint32_t* __restrict__ tot_a,
-                                            uint32_t* __restrict__ tot_b,
-                                            uint32_t* __restrict__ max_a) {
+// Inclusive sum over the wave's lanes (DPP: shifts within rows of 16 lanes,
+// then the row totals broadcast to the rows above).
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
+// Exclusive scans of two uint32 sequences (a, b) = f(i), i < n, by one
+// workgroup of NT threads (each thread takes a contiguous range, twice: sums,
+// then outputs).  Writes out_a[0..n] and out_b[0..n] (out_x[n] = total;
+// out_a may be null), val_a[i] = a (when non-null), the totals to tot_x and
+// max(a) to *max_a (each when non-null).
+template <int NT, class F>
+__device__ __forceinline__ void scan_pairs(uint32_t n, F f, uint32_t* __restrict__ out_a,
+                                           uint32_t* __restrict__ out_b,
+                                           uint32_t* __restrict__ val_a,
+                                           uint32_t* __restrict__ tot_a,
+                                           uint32_t* __restrict__ tot_b,
+                                           uint32_t* __restrict__ max_a) {
   __shared__ uint32_t wa[NT / kWave], wb[NT / kWave], wm[NT / kWave];
   const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
   const uint32_t per = (n + NT - 1) / NT;
   const uint32_t b0 = t * per, e0 = min(n, b0 + per);
-  auto val_b = [&](uint32_t i, uint32_t a) {
-    return MODE == 0 ? (a + DYN_CL_SIZE - 1) / DYN_CL_SIZE : in_b[i];
-  };
   uint32_t sa = 0, sb = 0, m = 0;
   for (uint32_t i = b0; i < e0; ++i) {
-    const uint32_t a = in_a[i];
-    sa += a;
-    sb += val_b(i, a);
-    m = max(m, a);
+    const uint2 v = f(i);
+    sa += v.x;
+    sb += v.y;
+    m = max(m, v.x);
   }
-  uint32_t ia = sa, ib = sb, im = m;
-  for (int off = 1; off < kWave; off <<= 1) {
-    const uint32_t va = __shfl_up(ia, off, kWave), vb = __shfl_up(ib, off, kWave);
-    if (lane >= off) {
-      ia += va;
-      ib += vb;
-    }
-    im = max(im, __shfl_xor(im, off, kWave));
-  }
+  const uint32_t ia = wave_incl_sum(sa), ib = wave_incl_sum(sb);
+  uint32_t im = m;
+  for (int off = 1; off < kWave; off <<= 1) im = max(im, __shfl_xor(im, off, kWave));
   if (lane == kWave - 1) {
     wa[w] = ia;
     wb[w] = ib;
@@ -226,11 +227,12 @@ __device__ __forceinline__ void scan2_block(const uint32_t* __restrict__ in_a,
   }
   uint32_t ra = ba + ia - sa, rb = bb + ib - sb;
   for (uint32_t i = b0; i < e0; ++i) {
-    const uint32_t a = in_a[i];
+    const uint2 v = f(i);
     if (out_a) out_a[i] = ra;
     out_b[i] = rb;
-    ra += a;
-    rb += val_b(i, a);
+    if (val_a) val_a[i] = v.x;
+    ra += v.x;
+    rb += v.y;
   }
   if (t == 0) {
     if (out_a) out_a[n] = ta;
@@ -245,21 +247,16 @@ __device__ __forceinline__ void scan2_block(const uint32_t* __restrict__ in_a,
 // wrapped), take a rank in it and write the site into the column's slice of
 // items (col_slice entries per column; the column sort flags a column that
 // would hold more, and the call is repeated with longer slices); the
-// positions become the reference of the
-// displacement readings.  The rank order depends on atomic timing; the
-// column sort that follows orders each column by (z, site), so the
-// clusters do not.  The last workgroup numbers the clusters: col_cl_start
-// (first cluster of each column) and their total *n_cl.
+// positions become the reference of the displacement readings.  The rank
+// order depends on atomic timing; the column sort that follows orders each
+// column by (z, site), so the clusters do not.
 constexpr int kBinBlock = 256;
 __device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __restrict__ wpos,
                                                 EngGeom g, float4* __restrict__ wpos_ref,
                                                 uint32_t* __restrict__ col_count,
                                                 uint32_t* __restrict__ items,
                                                 unsigned int* __restrict__ n_builds,
-                                                unsigned int* __restrict__ done, uint32_t n_cols,
-                                                uint32_t col_slice,
-                                                uint32_t* __restrict__ col_cl_start,
-                                                uint32_t* __restrict__ n_cl) {
+                                                uint32_t col_slice) {
   if (!build_on(G)) return;
   const uint32_t n = g.n_dyn + 4u * g.n_water;
   for (uint32_t s0 = blockIdx.x * kBinBlock; s0 < n; s0 += gridDim.x * kBinBlock) {
@@ -286,9 +283,23 @@ __device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __res
     const uint32_t rank = wave_rank(col, valid, col_count);
     if (valid && rank < col_slice) items[(size_t)col * col_slice + rank] = s;
   }
-  if (!last_workgroup(done)) return;
-  scan2_block<0, kBinBlock>(col_count, nullptr, n_cols, nullptr, col_cl_start, nullptr, n_cl,
-                            nullptr);
+}
+
+// The clusters of the columns, numbered in column order: col_cl_start
+// (first cluster of each column) and their total *n_cl.  One workgroup; a
+// launch of its own (a last-workgroup scan inside the binning launch would
+// need a device-scope fence per workgroup, which writes back the L2).
+__device__ __forceinline__ void k_eng_col_scan(BuildGate G, const uint32_t* __restrict__ col_count,
+                                               uint32_t n_cols, uint32_t* __restrict__ col_cl_start,
+                                               uint32_t* __restrict__ n_cl) {
+  if (!build_on(G)) return;
+  scan_pairs<kBinBlock>(
+      n_cols,
+      [&](uint32_t i) {
+        const uint32_t a = col_count[i];
+        return make_uint2(a, (a + DYN_CL_SIZE - 1) / DYN_CL_SIZE);
+      },
+      nullptr, col_cl_start, nullptr, nullptr, n_cl, nullptr);
 }
 
 struct SlotArgs {
@@ -433,13 +444,13 @@ struct ListArgs {
   const uint32_t* ex_idx;
   uint32_t n_dyn;
   uint64_t cap_entries, cap_chunks;  // groups, work items
-  uint32_t row_cap;                  // entries per row slice of ent_tmp (multiple of 8)
+  uint32_t row_cap;                  // entries per unit slice of ent_tmp (multiple of 64)
   uint32_t chunk;                    // j groups per work item
   uint32_t null_slot;                // padding j slot (exclusion group: padding)
   uint32_t* longest;                 // longest row that overflowed its slice
   unsigned int* status;
   uint8_t* xb_tmp;                   // per entry of ent_tmp: its exclusion byte
-  unsigned int* done;                // workgroups of the walk that have finished
+  uint32_t* unit_cnt;                // per walk unit: entries written
   uint32_t* tot;                     // groups, work items, longest row (from the scan)
 };
 
@@ -454,7 +465,7 @@ constexpr uint32_t kExEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kExPartners = kExHash / 2;  // partners of one cluster's sites
 constexpr int kListBlock = 256;                // four waves, each on its own rows
 constexpr int kListWaves = kListBlock / kWave;
-constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk (rows beyond: grid stride)
+constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk (units beyond: grid stride)
 struct WalkLds {
   float4 pI[DYN_CL_SIZE];  // I's slot positions (padding slots: NaN)
   uint32_t js[kWave], beg[kWave], end[kWave];
@@ -499,43 +510,34 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inclusive sum over the wave's lanes (DPP: shifts within rows of 16 lanes,
-// then the row totals broadcast to the rows above).
-__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return v;
-}
 
-// Pass 0, one wave per i cluster I: walk the candidate clusters in a fixed
-// order (class, column x, column y, cluster), keep those whose bounding box
-// is within rlist of I's box (periodic), test their 8 slots against I's 8
-// slots (eight candidates per round, a lane per j slot), and append every
-// j slot within rlist of at least one non-padding i slot.  A candidate
-// column is seen as the periodic image the column walk reached it in, so x
-// and y differences need no minimum image (z neither when rlist and both
-// boxes' half-heights are short of half the box: then only the image of
-// the candidate's box nearest I's can hold a slot pair within rlist; x and y
-// do when the column range spans the whole box).  Slots near only their own
-// water molecule's sites may be listed: the pair kernel masks such pairs.
-// Each class segment is padded to a multiple of 8 with the null slot.  The
-// row goes into its fixed-capacity slice of ent_tmp, each entry's exclusion
-// byte (solute–solute entries: bit ia set when the j site is a 1-2, 1-3 or
-// 1-4 partner of i slot ia's site) into the same place of xb_tmp; counts
-// are in groups of 8.  The last workgroup to finish scans the row counts:
-// row_ptr and chunk_ptr, and the totals.
+// Pass 0, one wave per unit (i cluster I, class cls, half of the class's
+// column window; unit u = kUnits·I + 2·cls + half): walk the unit's
+// candidate clusters in a fixed order (column x, column y, cluster), keep
+// those whose bounding box is within rlist of I's box (periodic), test
+// their 8 slots against I's 8 slots (eight candidates per round, a lane per
+// j slot), and append every j slot within rlist of at least one non-padding
+// i slot.  Splitting rows into units keeps every wave's chain of dependent
+// loads short: a row's time would otherwise be the sum of its sweeps and
+// rounds.  A candidate column is seen as the periodic image the column walk
+// reached it in, so x and y differences need no minimum image (z neither
+// when rlist and both boxes' half-heights are short of half the box: then
+// only the image of the candidate's box nearest I's can hold a slot pair
+// within rlist; x and y do when the column range spans the whole box).
+// Slots near only their own water molecule's sites may be listed: the pair
+// kernel masks such pairs.  The unit's entries go into its fixed-capacity
+// slice of ent_tmp and, for solute–solute units, each entry's exclusion byte
+// (bit ia set when the j site is a 1-2, 1-3 or 1-4 partner of i slot ia's
+// site) into the same place of xb_tmp; the list fill concatenates the two
+// halves of each class and pads the class segment to a multiple of 8.
+constexpr uint32_t kUnits = 6;  // walk units per row: 3 classes × 2 halves of the column window
 __device__ __forceinline__ void
-k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __restrict__ row_cnt,
-                uint32_t* __restrict__ row_chunks, uint32_t* __restrict__ row_ptr,
-                uint32_t* __restrict__ chunk_ptr) {
+k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp) {
   __shared__ WalkLds Ls[kListWaves];
   if (!build_on(A.gate)) return;
   WalkLds& L = Ls[threadIdx.x / kWave];
   const uint32_t n_cl = *A.n_cl;
+  const uint32_t n_units = n_cl * kUnits;
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned long long lt = (1ull << lane) - 1ull;
   const int jj = lane & (DYN_CL_SIZE - 1);             // j slot of the lane in a round
@@ -543,73 +545,73 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
   // a column is skipped when its x–y rectangle is beyond rlist of I's box;
   // the margin keeps the test conservative against rounding of the binning
   const float col_r2 = A.rlist2 * 1.0001f + 1e-3f;
-  for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
-       I += gridDim.x * kListWaves) {
-    uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
-    uint8_t* xb = A.xb_tmp + (size_t)I * A.row_cap;
+  for (uint32_t u = blockIdx.x * kListWaves + threadIdx.x / kWave; u < n_units;
+       u += gridDim.x * kListWaves) {
+    const uint32_t I = u / kUnits;
+    const int cls = (int)(u % kUnits) >> 1, half = (int)(u % kUnits) & 1;
     const int ci = A.cl_class[I];
-    const float4 c = A.bb_c[I], h = A.bb_h[I];
-    wave_sync();  // the previous row's reads of L are done
-    // a padding i slot's position is NaN, so no distance to it is below rlist
-    if (lane < DYN_CL_SIZE) {
-      float4 p = A.xyzq[I * DYN_CL_SIZE + lane];
-      if ((uint32_t)lane >= __float_as_uint(h.w)) p.x = p.y = p.z = __int_as_float(0x7fc00000);
-      L.pI[lane] = p;
-    }
-    // solute row: the exclusion partners of its sites, by slot
-    uint32_t n_ex = 0;
-    if (ci == DYN_CLS_SOLUTE) {
-      uint32_t len = 0, ex0 = 0;
+    const ClassGrid cg = class_grid(A.g, cls);
+    uint32_t n_out = 0;
+    if (dyn_kind(ci, cls) != DYN_KIND_NONE && cg.n != 0) {
+      uint32_t* ent = ent_tmp + (size_t)u * A.row_cap;
+      uint8_t* xb = A.xb_tmp + (size_t)u * A.row_cap;
+      const float4 c = A.bb_c[I], h = A.bb_h[I];
+      wave_sync();  // the previous unit's reads of L are done
+      // a padding i slot's position is NaN, so no distance to it is below rlist
       if (lane < DYN_CL_SIZE) {
-        const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
-        if (sa >= 0) {
-          ex0 = A.ex_ptr[sa];
-          len = A.ex_ptr[sa + 1] - ex0;
+        float4 p = A.xyzq[I * DYN_CL_SIZE + lane];
+        if ((uint32_t)lane >= __float_as_uint(h.w)) p.x = p.y = p.z = __int_as_float(0x7fc00000);
+        L.pI[lane] = p;
+      }
+      // solute–solute unit: the exclusion partners of I's sites, by slot
+      bool exl = false;
+      if (ci == DYN_CLS_SOLUTE && cls == DYN_CLS_SOLUTE) {
+        uint32_t len = 0, ex0 = 0;
+        if (lane < DYN_CL_SIZE) {
+          const int32_t sa = A.slot_site[I * DYN_CL_SIZE + lane];
+          if (sa >= 0) {
+            ex0 = A.ex_ptr[sa];
+            len = A.ex_ptr[sa + 1] - ex0;
+          }
         }
-      }
-      const uint32_t incl = wave_incl_sum(len);
-      n_ex = (uint32_t)__builtin_amdgcn_readlane((int)incl, DYN_CL_SIZE - 1);
-      if (n_ex > kExPartners) {
-        if (lane == 0) atomicOr(A.status, kStMask);
-        n_ex = 0;
-      }
-      for (uint32_t k = lane; k < (uint32_t)kExHash; k += kWave) L.ex[k] = kExEmpty;
-      wave_sync();
-      // lane t < n_ex inserts partner t (of i slot ia: beg[ia] ≤ t < beg[ia+1])
-      const uint32_t beg_l = incl - len;
-      for (uint32_t t0 = 0; t0 < n_ex; t0 += kWave) {
-        const uint32_t t = t0 + lane;
-        int ia = 0;
+        const uint32_t incl = wave_incl_sum(len);
+        uint32_t n_ex = (uint32_t)__builtin_amdgcn_readlane((int)incl, DYN_CL_SIZE - 1);
+        if (n_ex > kExPartners) {
+          if (lane == 0) atomicOr(A.status, kStMask);
+          n_ex = 0;
+        }
+        exl = n_ex != 0;
+        for (uint32_t k = lane; k < (uint32_t)kExHash; k += kWave) L.ex[k] = kExEmpty;
+        wave_sync();
+        // lane t < n_ex inserts partner t (of i slot ia: beg[ia] ≤ t < beg[ia+1])
+        const uint32_t beg_l = incl - len;
+        for (uint32_t t0 = 0; t0 < n_ex; t0 += kWave) {
+          const uint32_t t = t0 + lane;
+          int ia = 0;
 #pragma unroll
-        for (int u = 1; u < DYN_CL_SIZE; ++u)
-          if ((uint32_t)__builtin_amdgcn_readlane((int)beg_l, u) <= t) ia = u;
-        const uint32_t e0 = (uint32_t)__shfl(ex0, ia, kWave);
-        const uint32_t b0 = (uint32_t)__shfl(beg_l, ia, kWave);
-        if (t < n_ex) {
-          const uint32_t slot = A.slot_of[A.ex_idx[e0 + (t - b0)]];
-          const uint32_t bit = 1u << ia;
-          for (uint32_t hh = ex_hash(slot);; hh = (hh + 1) & (kExHash - 1)) {
-            uint32_t cur = L.ex[hh];
-            if (cur == kExEmpty) {
-              cur = atomicCAS(&L.ex[hh], kExEmpty, (slot << 8) | bit);
-              if (cur == kExEmpty) break;
-            }
-            if ((cur >> 8) == slot) {
-              atomicOr(&L.ex[hh], bit);
-              break;
+          for (int v = 1; v < DYN_CL_SIZE; ++v)
+            if ((uint32_t)__builtin_amdgcn_readlane((int)beg_l, v) <= t) ia = v;
+          const uint32_t e0 = (uint32_t)__shfl(ex0, ia, kWave);
+          const uint32_t b0 = (uint32_t)__shfl(beg_l, ia, kWave);
+          if (t < n_ex) {
+            const uint32_t slot = A.slot_of[A.ex_idx[e0 + (t - b0)]];
+            const uint32_t bit = 1u << ia;
+            for (uint32_t hh = ex_hash(slot);; hh = (hh + 1) & (kExHash - 1)) {
+              uint32_t cur = L.ex[hh];
+              if (cur == kExEmpty) {
+                cur = atomicCAS(&L.ex[hh], kExEmpty, (slot << 8) | bit);
+                if (cur == kExEmpty) break;
+              }
+              if ((cur >> 8) == slot) {
+                atomicOr(&L.ex[hh], bit);
+                break;
+              }
             }
           }
         }
       }
-    }
-    wave_sync();
-    uint32_t n_out = 0;
-    for (int cls = 0; cls < 3; ++cls) {
-      if (dyn_kind(ci, cls) == DYN_KIND_NONE) continue;
-      const ClassGrid cg = class_grid(A.g, cls);
-      if (cg.n == 0) continue;
+      wave_sync();
       const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
-      const bool exl = n_ex != 0 && cls == DYN_CLS_SOLUTE;
       int x0 = (int)floorf((c.x - h.x - A.rlist) / cg.wx), x1 = (int)floorf((c.x + h.x + A.rlist) / cg.wx);
       int y0 = (int)floorf((c.y - h.y - A.rlist) / cg.wy), y1 = (int)floorf((c.y + h.y + A.rlist) / cg.wy);
       bool full_x = false, full_y = false;
@@ -625,10 +627,13 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
       }
       const bool full_xy = full_x || full_y;
       const int ny = y1 - y0 + 1, n_colc = (x1 - x0 + 1) * ny;
-      for (int cb = 0; cb < n_colc; cb += kWave) {
+      // this unit's half of the window's columns (in walk order)
+      const int kmid = (n_colc + 1) / 2;
+      const int kbeg = half ? kmid : 0, kend = half ? n_colc : kmid;
+      for (int cb = kbeg; cb < kend; cb += kWave) {
         const int k = cb + lane;
         uint32_t js = 0, cnt = 0, img = 0;
-        if (k < n_colc) {
+        if (k < kend) {
           const int xx = x0 + k / ny, yy = y0 + k % ny;
           float dx = 0.f, dy = 0.f;
           if (!full_x) dx = fmaxf(0.f, fmaxf(xx * cg.wx - (c.x + h.x), (c.x - h.x) - (xx + 1) * cg.wx));
@@ -701,8 +706,8 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
               const float py = pj.y + cand_shift(cd, 23, A.g.L[1]);
               const float pz = pj.z + cand_shift(cd, 25, A.g.L[2]);
 #pragma unroll
-              for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                const float4 q = L.pI[u];
+              for (int v = 0; v < DYN_CL_SIZE; ++v) {
+                const float4 q = L.pI[v];
                 const float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
                 keep |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
               }
@@ -711,8 +716,8 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
               const float py = full_xy ? pj.y : pj.y + cand_shift(cd, 23, A.g.L[1]);
               const float pz = zslot ? pj.z : pj.z + cand_shift(cd, 25, A.g.L[2]);
 #pragma unroll
-              for (int u = 0; u < DYN_CL_SIZE; ++u) {
-                const float4 q = L.pI[u];
+              for (int v = 0; v < DYN_CL_SIZE; ++v) {
+                const float4 q = L.pI[v];
                 float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
                 if (full_xy) {
                   d0 = dyn_min_image_rint(d0, A.g.L[0], A.g.invL[0]);
@@ -727,10 +732,9 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
             if (keep) {
               const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
               const uint32_t slot = (cd & kCandJ) * DYN_CL_SIZE + jj;
-              const uint32_t xbits = exl ? ex_bits(L.ex, slot) : 0u;
               if (pos < A.row_cap) {
                 ent[pos] = slot | tag;
-                xb[pos] = (uint8_t)xbits;
+                if (exl) xb[pos] = (uint8_t)ex_bits(L.ex, slot);
               }
             }
             n_out += (uint32_t)__popcll(kb);
@@ -738,26 +742,36 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp, uint32_t* __r
           wave_sync();  // L.ok is rewritten by the next sweep
         }
       }
-      const uint32_t pad = (DYN_CL_SIZE - n_out % DYN_CL_SIZE) % DYN_CL_SIZE;
-      if ((uint32_t)lane < pad && n_out + lane < A.row_cap) {
-        ent[n_out + lane] = A.null_slot | tag;
-        xb[n_out + lane] = 0;
-      }
-      n_out += pad;
     }
     if (lane == 0) {
       if (n_out > A.row_cap) {
         atomicOr(A.status, kStRow);
         atomicMax(A.longest, n_out);
       }
-      const uint32_t m = min(n_out, A.row_cap) / DYN_CL_SIZE;
-      row_cnt[I] = m;
-      row_chunks[I] = (m + A.chunk - 1) / A.chunk;
+      A.unit_cnt[u] = min(n_out, A.row_cap);
     }
   }
-  if (!last_workgroup(A.done)) return;
-  scan2_block<1, kListBlock>(row_cnt, row_chunks, n_cl, row_ptr, chunk_ptr, A.tot + 0, A.tot + 1,
-                             A.tot + 2);
+}
+
+// The rows in groups of 8 (each class segment, the two halves' entries,
+// padded to 8) and their scans: row_cnt, row_ptr and chunk_ptr, the totals
+// and the longest row.  One workgroup, a launch of its own.
+__device__ __forceinline__ void k_eng_row_scan(const ListArgs& A, uint32_t* __restrict__ row_cnt,
+                                               uint32_t* __restrict__ row_ptr,
+                                               uint32_t* __restrict__ chunk_ptr) {
+  if (!build_on(A.gate)) return;
+  const uint32_t n_cl = *A.n_cl;
+  scan_pairs<kListBlock>(
+      n_cl,
+      [&](uint32_t I) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          m += (A.unit_cnt[I * kUnits + 2 * k] + A.unit_cnt[I * kUnits + 2 * k + 1] +
+                DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+        return make_uint2(m, (m + A.chunk - 1) / A.chunk);
+      },
+      row_ptr, chunk_ptr, row_cnt, A.tot + 0, A.tot + 1, A.tot + 2);
 }
 
 }  // namespace
@@ -784,7 +798,8 @@ struct ListCopies {
 };
 static_assert(sizeof(ListCopies) == 64, "ListCopies has no padding");
 
-// Pass 1: compact the rows into cj (groups of 8 entries), write the work
+// Pass 1: compact the rows into cj (groups of 8 entries; each class segment
+// the two halves of its window, padded with the null slot), write the work
 // items (I, first group) and, for the solute–solute groups (solute rows:
 // solute clusters are numbered first; the solute segment comes first in
 // every row), the 64-bit masks: bit ia + 8·jb of a group is cleared when the
@@ -809,15 +824,39 @@ k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
     const uint32_t m = row_cnt[I];
     const uint64_t base = row_ptr[I];
     if (base + m <= A.cap_entries) {
-      const uint32_t* ent = ent_tmp + (size_t)I * A.row_cap;
       uint32_t* out = cj + base * DYN_CL_SIZE;
-      for (uint32_t k = lane; k < m * DYN_CL_SIZE; k += kWave) out[k] = ent[k];
-      if (A.cl_class[I] == DYN_CLS_SOLUTE) {
-        // row_cap is a multiple of 64, so each group's 8 bytes are aligned
-        const unsigned long long* xb =
-            reinterpret_cast<const unsigned long long*>(A.xb_tmp + (size_t)I * A.row_cap);
-        for (uint32_t g = lane; g < m; g += kWave)
-          if ((ent[g * DYN_CL_SIZE] >> DYN_JCLS_SHIFT) == DYN_CLS_SOLUTE) cmask[base + g] = ~xb[g];
+      const bool srow = A.cl_class[I] == DYN_CLS_SOLUTE;
+      uint32_t off = 0;  // entries of the row written
+      for (int cls = 0; cls < 3; ++cls) {
+        const uint32_t u0 = I * kUnits + 2 * cls;
+        const uint32_t n0 = A.unit_cnt[u0], n1 = A.unit_cnt[u0 + 1], n = n0 + n1;
+        if (n == 0) continue;
+        const uint32_t np = (n + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
+        const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
+        const bool ss = srow && cls == DYN_CLS_SOLUTE;
+        for (uint32_t p0 = 0; p0 < np; p0 += kWave) {
+          const uint32_t p = p0 + lane;
+          // entry p of the segment: from the first half, the second, or padding
+          const size_t src = p < n0 ? (size_t)u0 * A.row_cap + p
+                                    : (size_t)(u0 + 1) * A.row_cap + (p - n0);
+          uint32_t e = A.null_slot | tag;
+          unsigned long long xbits = 0ull;
+          if (p < n) {
+            e = ent_tmp[src];
+            if (ss) xbits = (unsigned long long)A.xb_tmp[src] << (DYN_CL_SIZE * (p & 7u));
+          }
+          if (p < np) out[off + p] = e;
+          if (ss) {
+            // the group's 8 bytes: an OR over its 8 lanes (groups start at
+            // multiples of 8 within the segment, and the segment at a
+            // multiple of 8 within the row)
+            xbits |= __shfl_xor(xbits, 1, kWave);
+            xbits |= __shfl_xor(xbits, 2, kWave);
+            xbits |= __shfl_xor(xbits, 4, kWave);
+            if ((lane & 7) == 0 && p < np) cmask[base + (off + p) / DYN_CL_SIZE] = ~xbits;
+          }
+        }
+        off += np;
       }
     } else if (lane == 0) {
       atomicOr(A.status, kStEntries);
@@ -1641,12 +1680,12 @@ struct dyn_engine {
   DevBuf<uint8_t> cl_class;
   DevBuf<uint32_t> slot_of;
   DevBuf<float4> xyzq, bb_c, bb_h;
-  DevBuf<uint32_t> row_cnt, row_chunks, row_ptr, chunk_ptr, cj, ent_tmp;
+  DevBuf<uint32_t> row_cnt, row_ptr, chunk_ptr, cj, ent_tmp, unit_cnt;
   DevBuf<uint8_t> xb_tmp;  // exclusion byte of each ent_tmp entry
   DevBuf<unsigned long long> cmask;
   DevBuf<uint2> chunks;
   uint64_t cap_entries = 0, cap_chunks = 0;  // groups of 8 j entries, work items
-  uint32_t row_cap = 0;                      // j entries per row slice of ent_tmp
+  uint32_t row_cap = 0;                      // j entries per walk-unit slice of ent_tmp
   // device counters: [0] n_cl, [1] n_groups, [2] n_items, [3] max row (groups),
   // [4] longest row that overflowed its slice (entries)
   DevBuf<uint32_t> counters;
@@ -1803,7 +1842,7 @@ void dyn_engine::set_geometry() {
   bb_c.reserve(cap_cl);
   bb_h.reserve(cap_cl);
   row_cnt.reserve(cap_cl + 1);
-  row_chunks.reserve(cap_cl + 1);
+  unit_cnt.reserve((size_t)cap_cl * kUnits);
   row_ptr.reserve(cap_cl + 1);
   chunk_ptr.reserve(cap_cl + 1);
   stg_slot_site.reserve(cap_slots + DYN_CL_SIZE);
@@ -1823,11 +1862,12 @@ void dyn_engine::set_geometry() {
     DYN_HIP(hipStreamSynchronize(stream));
   }
   // first guesses from the site density, grown on overflow: a row holds the
-  // sites within about rlist + 3 Å of its cluster (all classes, padding)
+  // sites within about rlist + 3 Å of its cluster (all classes, padding); a
+  // walk unit (half of one class's window) well under 0.6 of that
   const double dens = n_sites / V;
   const double rl = prm.cutoff + prm.list_buffer;
   const double row_est = dens * 4.19 * std::pow(rl + 3.0, 3) * 1.2 + 3 * DYN_CL_SIZE;
-  reserve_rows((uint32_t)std::min<double>(row_est, 1 << 20));
+  reserve_rows((uint32_t)std::min<double>(0.6 * row_est, 1 << 20));
   const double avg_groups = dens * 4.19 * std::pow(rl + 2.0, 3) / DYN_CL_SIZE + 3;
   const uint64_t groups = (uint64_t)(cap_cl * avg_groups);
   reserve_list(groups, groups / prm.item_groups + cap_cl);
@@ -1843,12 +1883,12 @@ void dyn_engine::set_geometry() {
   list_valid = false;
 }
 
-// Row slices of the list walk: `entries` j entries per cluster (a multiple
-// of 64).
+// Unit slices of the list walk: `entries` j entries per unit (a multiple
+// of 64), kUnits units per cluster.
 void dyn_engine::reserve_rows(uint32_t entries) {
   row_cap = std::max(row_cap, (entries + 63u) & ~63u);
-  ent_tmp.reserve((size_t)cap_cl * row_cap);  // grows with either factor
-  xb_tmp.reserve((size_t)cap_cl * row_cap);
+  ent_tmp.reserve((size_t)cap_cl * kUnits * row_cap);  // grows with either factor
+  xb_tmp.reserve((size_t)cap_cl * kUnits * row_cap);
 }
 
 // List capacity in groups of 8 j entries, and work items.
@@ -1929,8 +1969,12 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     launch<k_eng_bin_count, kBinBlock>(R, "k_eng_bin_count", 0, gs, kBinBlock, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_bin_count>(gate[i], e.wpos.p, e.geom, T[i].wpos_ref, e.col_count.p,
-                                   e.items.p, e.dflags.p + kFlBuilds, e.dflags.p + kFlBinDone,
-                                   e.n_cols, e.col_slice, e.col_cl_start.p, T[i].counters + 0);
+                                   e.items.p, e.dflags.p + kFlBuilds, e.col_slice);
+    });
+    launch<k_eng_col_scan, kBinBlock>(R, "k_eng_col_scan", 0, 1, kBinBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_col_scan>(gate[i], e.col_count.p, e.n_cols, e.col_cl_start.p,
+                                  T[i].counters + 0);
     });
   }
   {
@@ -1979,12 +2023,12 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     A.longest = T[i].counters + 4;
     A.status = e.status.p;
     A.xb_tmp = e.xb_tmp.p;
-    A.done = e.dflags.p + kFlWalkDone;
+    A.unit_cnt = e.unit_cnt.p;
     A.tot = T[i].counters + 1;
   }
-  // one row per wave; the walk's grid holds every row at once
+  // the walk: one unit per wave; the fill: one row per wave
   const unsigned walk_grid = max_grid(S, [](dyn_engine& e) {
-    return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kWalkGrid);
+    return std::min<uint32_t>((e.cap_cl * kUnits + kListWaves - 1) / kListWaves, kWalkGrid);
   });
   const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
     return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kBuildGrid);
@@ -1994,8 +2038,11 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     launch<k_eng_list_walk, kListBlock>(R, "k_eng_list_walk", 0, walk_grid, kListBlock, 0,
                                         [&](uint32_t i) {
       dyn_engine& e = *E[i];
-      return args<k_eng_list_walk>(LA[i], e.ent_tmp.p, e.row_cnt.p, e.row_chunks.p,
-                                   T[i].row_ptr, T[i].chunk_ptr);
+      return args<k_eng_list_walk>(LA[i], e.ent_tmp.p);
+    });
+    launch<k_eng_row_scan, kListBlock>(R, "k_eng_row_scan", 0, 1, kListBlock, 0, [&](uint32_t i) {
+      dyn_engine& e = *E[i];
+      return args<k_eng_row_scan>(LA[i], e.row_cnt.p, T[i].row_ptr, T[i].chunk_ptr);
     });
   }
   {

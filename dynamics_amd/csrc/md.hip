@@ -967,7 +967,11 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
 // index of the step that ends here; the launch advances it once every
 // workgroup has used it: each workgroup counts itself in step_dev[1] when
 // its threads are done, and the last one stores the new index and clears
-// the count.
+// the count.  No fence precedes the count: a workgroup's reads of the
+// index have completed (their values are used before the barrier), and the
+// positions and velocities it wrote are read by later launches only, which
+// the end of this launch makes visible (a device-scope fence per workgroup
+// would write back the L2 each time).
 __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
                                              float4* __restrict__ v,
                                              uint64_t* __restrict__ step_dev) {
@@ -976,7 +980,6 @@ __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
   if (g < A.water0 + A.G.n_water) step_group<true>(A, x, v, g, step_dev);
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
     unsigned* done = reinterpret_cast<unsigned*>(step_dev + 1);
     if (atomicAdd(done, 1u) == gridDim.x - 1) {
       step_dev[0] = step_dev[0] + 1;
