@@ -285,23 +285,6 @@ __device__ __forceinline__ void k_eng_bin_count(BuildGate G, const float4* __res
   }
 }
 
-// The clusters of the columns, numbered in column order: col_cl_start
-// (first cluster of each column) and their total *n_cl.  One workgroup; a
-// launch of its own (a last-workgroup scan inside the binning launch would
-// need a device-scope fence per workgroup, which writes back the L2).
-__device__ __forceinline__ void k_eng_col_scan(BuildGate G, const uint32_t* __restrict__ col_count,
-                                               uint32_t n_cols, uint32_t* __restrict__ col_cl_start,
-                                               uint32_t* __restrict__ n_cl) {
-  if (!build_on(G)) return;
-  scan_pairs<kBinBlock>(
-      n_cols,
-      [&](uint32_t i) {
-        const uint32_t a = col_count[i];
-        return make_uint2(a, (a + DYN_CL_SIZE - 1) / DYN_CL_SIZE);
-      },
-      nullptr, col_cl_start, nullptr, nullptr, n_cl, nullptr);
-}
-
 struct SlotArgs {
   const float4* wpos;
   const int32_t* ltype;   // per site
@@ -328,16 +311,12 @@ __device__ __forceinline__ int32_t site_gid(const int32_t* wmol, int32_t s) {
 constexpr int kSortBlock = 256;
 constexpr unsigned kBuildGrid = 1024;  // workgroups of the grid-stride build kernels
 __device__ __forceinline__ void
-colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ items,
-            uint32_t col_slice, const uint32_t* __restrict__ col_cl_start,
-            const int* __restrict__ col_class, const SlotArgs& S, uint8_t* __restrict__ cl_class,
-            float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
+colsort_one(uint32_t c, uint32_t n, uint32_t cl0, const uint32_t* __restrict__ items,
+            uint32_t col_slice, const int* __restrict__ col_class, const SlotArgs& S,
+            uint8_t* __restrict__ cl_class, float4* __restrict__ bb_c, float4* __restrict__ bb_h,
             unsigned int* __restrict__ status, unsigned long long* key) {
-  const uint32_t n = col_count[c];
-  __syncthreads();
-  if (threadIdx.x == 0 && n) col_count[c] = 0;
   const size_t b = (size_t)c * col_slice;
-  const uint32_t cl0 = col_cl_start[c], ncl = col_cl_start[c + 1] - cl0;
+  const uint32_t ncl = (n + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
   if (ncl == 0) return;
   if (n > col_slice) {
     if (threadIdx.x == 0) atomicOr(status, kStColumn);
@@ -411,18 +390,56 @@ colsort_one(uint32_t c, uint32_t* __restrict__ col_count, const uint32_t* __rest
 }
 
 __device__ __forceinline__ void
-k_eng_colsort(BuildGate G, uint32_t* __restrict__ col_count, const uint32_t* __restrict__ items,
-              uint32_t col_slice, const uint32_t* __restrict__ col_cl_start,
+k_eng_colsort(BuildGate G, const uint32_t* __restrict__ col_count, const uint32_t* __restrict__ items,
+              uint32_t col_slice, uint32_t* __restrict__ col_cl_start,
               const int* __restrict__ col_class, SlotArgs S, uint8_t* __restrict__ cl_class,
-              float4* __restrict__ bb_c, float4* __restrict__ bb_h, uint32_t n_sites,
-              unsigned int* __restrict__ status, uint32_t n_cols) {
+              float4* __restrict__ bb_c, float4* __restrict__ bb_h,
+              unsigned int* __restrict__ status, uint32_t n_cols, uint32_t* __restrict__ n_cl) {
   __shared__ unsigned long long key[kColCap];
+  __shared__ uint32_t wsum[kSortBlock / kWave];
+  __shared__ uint32_t s_start;
   if (!build_on(G)) return;
+  // every workgroup scans the columns' cluster counts itself (a scan launch
+  // of its own would cost a dependent launch; one inside the binning launch
+  // a device-scope fence per workgroup); workgroup 0 writes col_cl_start
+  // and the total *n_cl
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const uint32_t per = (n_cols + kSortBlock - 1) / kSortBlock;
+  const uint32_t b0 = min(n_cols, t * per), e0 = min(n_cols, b0 + per);
+  uint32_t sum = 0;
+  for (uint32_t i = b0; i < e0; ++i) sum += (col_count[i] + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+  const uint32_t incl = wave_incl_sum(sum);
+  if (lane == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  uint32_t base = 0, total = 0;
+  for (int k = 0; k < kSortBlock / kWave; ++k) {
+    if (k < w) base += wsum[k];
+    total += wsum[k];
+  }
+  const uint32_t excl = base + incl - sum;  // clusters of the columns before b0
+  if (blockIdx.x == 0) {
+    uint32_t r = excl;
+    for (uint32_t i = b0; i < e0; ++i) {
+      col_cl_start[i] = r;
+      r += (col_count[i] + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+    }
+    if (t == 0) {
+      col_cl_start[n_cols] = total;
+      *n_cl = total;
+    }
+  }
   // a capped grid walks the columns (a closed gate then costs few workgroups)
   for (uint32_t c = blockIdx.x; c < n_cols; c += gridDim.x) {
-    colsort_one(c, col_count, items, col_slice, col_cl_start, col_class, S, cl_class, bb_c, bb_h,
-                n_sites, status, key);
-    __syncthreads();  // key[] is reused by the next column
+    if ((uint32_t)t == c / per) {
+      uint32_t r = excl;
+      for (uint32_t i = b0; i < c; ++i) r += (col_count[i] + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+      s_start = r;
+    }
+    __syncthreads();
+    const uint32_t cl0 = s_start;
+    colsort_one(c, col_count[c], cl0, items, col_slice, col_class, S, cl_class, bb_c, bb_h, status,
+                key);
+    __syncthreads();  // key[] and s_start are reused by the next column
   }
 }
 
@@ -451,6 +468,9 @@ struct ListArgs {
   unsigned int* status;
   uint8_t* xb_tmp;                   // per entry of ent_tmp: its exclusion byte
   uint32_t* unit_cnt;                // per walk unit: entries written
+  uint32_t* col_count;               // cleared by the walk for the next binning pass
+  uint32_t n_cols;
+  uint32_t unused;                   // no implicit padding: argument packs are compared byte for byte
   uint32_t* tot;                     // groups, work items, longest row (from the scan)
 };
 
@@ -535,6 +555,10 @@ __device__ __forceinline__ void
 k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp) {
   __shared__ WalkLds Ls[kListWaves];
   if (!build_on(A.gate)) return;
+  // the column sort has read the column counts: clear them for the next
+  // binning pass (it counts with atomics into zeroed counts)
+  for (uint32_t k = blockIdx.x * kListBlock + threadIdx.x; k < A.n_cols; k += gridDim.x * kListBlock)
+    A.col_count[k] = 0;
   WalkLds& L = Ls[threadIdx.x / kWave];
   const uint32_t n_cl = *A.n_cl;
   const uint32_t n_units = n_cl * kUnits;
@@ -804,7 +828,9 @@ static_assert(sizeof(ListCopies) == 64, "ListCopies has no padding");
 // solute clusters are numbered first; the solute segment comes first in
 // every row), the 64-bit masks: bit ia + 8·jb of a group is cleared when the
 // sites of i slot ia and of the group's entry jb are a 1-2, 1-3 or 1-4 pair
-// (entry jb's exclusion byte, shifted into place).
+// (entry jb's exclusion byte, shifted into place).  One wave per class
+// segment of a row; the segment's entries move four per lane at a time.
+constexpr int kFillUnroll = 4;
 __device__ __forceinline__ void
 k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
                 const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
@@ -819,54 +845,63 @@ k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
   uint2* __restrict__ chunks = second ? C.chunks1 : C.chunks0;
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x & (kWave - 1);
-  for (uint32_t I = blockIdx.x * kListWaves + threadIdx.x / kWave; I < n_cl;
-       I += gridDim.x * kListWaves) {
+  for (uint32_t sg = blockIdx.x * kListWaves + threadIdx.x / kWave; sg < n_cl * 3u;
+       sg += gridDim.x * kListWaves) {
+    const uint32_t I = sg / 3u;
+    const int cls = (int)(sg % 3u);
     const uint32_t m = row_cnt[I];
     const uint64_t base = row_ptr[I];
-    if (base + m <= A.cap_entries) {
-      uint32_t* out = cj + base * DYN_CL_SIZE;
-      const bool srow = A.cl_class[I] == DYN_CLS_SOLUTE;
-      uint32_t off = 0;  // entries of the row written
-      for (int cls = 0; cls < 3; ++cls) {
-        const uint32_t u0 = I * kUnits + 2 * cls;
-        const uint32_t n0 = A.unit_cnt[u0], n1 = A.unit_cnt[u0 + 1], n = n0 + n1;
-        if (n == 0) continue;
-        const uint32_t np = (n + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
-        const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
-        const bool ss = srow && cls == DYN_CLS_SOLUTE;
-        for (uint32_t p0 = 0; p0 < np; p0 += kWave) {
-          const uint32_t p = p0 + lane;
-          // entry p of the segment: from the first half, the second, or padding
-          const size_t src = p < n0 ? (size_t)u0 * A.row_cap + p
-                                    : (size_t)(u0 + 1) * A.row_cap + (p - n0);
-          uint32_t e = A.null_slot | tag;
-          unsigned long long xbits = 0ull;
-          if (p < n) {
-            e = ent_tmp[src];
-            if (ss) xbits = (unsigned long long)A.xb_tmp[src] << (DYN_CL_SIZE * (p & 7u));
-          }
-          if (p < np) out[off + p] = e;
-          if (ss) {
-            // the group's 8 bytes: an OR over its 8 lanes (groups start at
-            // multiples of 8 within the segment, and the segment at a
-            // multiple of 8 within the row)
-            xbits |= __shfl_xor(xbits, 1, kWave);
-            xbits |= __shfl_xor(xbits, 2, kWave);
-            xbits |= __shfl_xor(xbits, 4, kWave);
-            if ((lane & 7) == 0 && p < np) cmask[base + (off + p) / DYN_CL_SIZE] = ~xbits;
-          }
-        }
-        off += np;
+    if (cls == 0) {
+      const uint32_t c0 = chunk_ptr[I], nch = (m + A.chunk - 1) / A.chunk;
+      if ((uint64_t)c0 + nch <= A.cap_chunks) {
+        for (uint32_t k = lane; k < nch; k += kWave)
+          chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * A.chunk);
+      } else if (lane == 0) {
+        atomicOr(A.status, kStChunks);
       }
-    } else if (lane == 0) {
-      atomicOr(A.status, kStEntries);
+      if (base + m > A.cap_entries && lane == 0) atomicOr(A.status, kStEntries);
     }
-    const uint32_t c0 = chunk_ptr[I], nch = (m + A.chunk - 1) / A.chunk;
-    if ((uint64_t)c0 + nch <= A.cap_chunks) {
-      for (uint32_t k = lane; k < nch; k += kWave)
-        chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * A.chunk);
-    } else if (lane == 0) {
-      atomicOr(A.status, kStChunks);
+    if (base + m > A.cap_entries) continue;
+    const uint32_t* uc = A.unit_cnt + (size_t)I * kUnits;
+    uint32_t off = 0;  // the segment's first entry in the row
+    for (int k = 0; k < cls; ++k) off += (uc[2 * k] + uc[2 * k + 1] + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
+    const uint32_t n0 = uc[2 * cls], n = n0 + uc[2 * cls + 1];
+    if (n == 0) continue;
+    const uint32_t np = (n + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
+    const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
+    const bool ss = A.cl_class[I] == DYN_CLS_SOLUTE && cls == DYN_CLS_SOLUTE;
+    const size_t u0 = (size_t)I * kUnits + 2 * cls;
+    uint32_t* out = cj + (base * DYN_CL_SIZE + off);
+    for (uint32_t p0 = 0; p0 < np; p0 += kFillUnroll * kWave) {
+      uint32_t e[kFillUnroll];
+      unsigned long long xbits[kFillUnroll];
+#pragma unroll
+      for (int r = 0; r < kFillUnroll; ++r) {
+        // entry p of the segment: from the first half, the second, or padding
+        const uint32_t p = p0 + r * kWave + lane;
+        const size_t src = p < n0 ? u0 * A.row_cap + p : (u0 + 1) * A.row_cap + (p - n0);
+        e[r] = A.null_slot | tag;
+        xbits[r] = 0ull;
+        if (p < n) {
+          e[r] = ent_tmp[src];
+          if (ss) xbits[r] = (unsigned long long)A.xb_tmp[src] << (DYN_CL_SIZE * (p & 7u));
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kFillUnroll; ++r) {
+        const uint32_t p = p0 + r * kWave + lane;
+        if (p < np) out[p] = e[r];
+        if (ss) {
+          // the group's 8 bytes: an OR over its 8 lanes (groups start at
+          // multiples of 8 within the segment, and the segment at a
+          // multiple of 8 within the row)
+          unsigned long long x = xbits[r];
+          x |= __shfl_xor(x, 1, kWave);
+          x |= __shfl_xor(x, 2, kWave);
+          x |= __shfl_xor(x, 4, kWave);
+          if ((lane & 7) == 0 && p < np) cmask[base + (off + p) / DYN_CL_SIZE] = ~x;
+        }
+      }
     }
   }
   // a staged build is complete when the launch's last workgroup is done
@@ -1873,7 +1908,7 @@ void dyn_engine::set_geometry() {
   reserve_list(groups, groups / prm.item_groups + cap_cl);
   DYN_HIP(hipMemcpyAsync(col_class.p, h_col_class.data(), sizeof(int) * n_cols,
                          hipMemcpyHostToDevice, stream));
-  // the binning kernel counts into zeroed column counts; the column sort
+  // the binning kernel counts into zeroed column counts; the list walk
   // clears them again after use
   DYN_HIP(hipMemsetAsync(col_count.p, 0, sizeof(uint32_t) * (n_cols + 1), stream));
   // no clusters until the next build: the preparation kernel gathers slot
@@ -1971,11 +2006,6 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
       return args<k_eng_bin_count>(gate[i], e.wpos.p, e.geom, T[i].wpos_ref, e.col_count.p,
                                    e.items.p, e.dflags.p + kFlBuilds, e.col_slice);
     });
-    launch<k_eng_col_scan, kBinBlock>(R, "k_eng_col_scan", 0, 1, kBinBlock, 0, [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_col_scan>(gate[i], e.col_count.p, e.n_cols, e.col_cl_start.p,
-                                  T[i].counters + 0);
-    });
   }
   {
     Profiler::Scope ps(prof, "nbl_colsort");
@@ -1988,7 +2018,7 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
                    T[i].slot_gid, T[i].slot_site, T[i].slot_of, e.type_none, 0};
       return args<k_eng_colsort>(gate[i], e.col_count.p, e.items.p, e.col_slice,
                                  e.col_cl_start.p, e.col_class.p, A, T[i].cl_class, e.bb_c.p,
-                                 e.bb_h.p, e.n_sites, e.status.p, e.n_cols);
+                                 e.bb_h.p, e.status.p, e.n_cols, T[i].counters + 0);
     });
   }
   std::vector<ListArgs> LA(E.size());
@@ -2024,14 +2054,16 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     A.status = e.status.p;
     A.xb_tmp = e.xb_tmp.p;
     A.unit_cnt = e.unit_cnt.p;
+    A.col_count = e.col_count.p;
+    A.n_cols = e.n_cols;
     A.tot = T[i].counters + 1;
   }
-  // the walk: one unit per wave; the fill: one row per wave
+  // the walk: one unit per wave; the fill: one class segment per wave
   const unsigned walk_grid = max_grid(S, [](dyn_engine& e) {
     return std::min<uint32_t>((e.cap_cl * kUnits + kListWaves - 1) / kListWaves, kWalkGrid);
   });
   const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
-    return std::min<uint32_t>((e.cap_cl + kListWaves - 1) / kListWaves, kBuildGrid);
+    return std::min<uint32_t>((e.cap_cl * 3 + kListWaves - 1) / kListWaves, kWalkGrid);
   });
   {
     Profiler::Scope ps(prof, "nbl_list_walk");
