@@ -308,8 +308,8 @@ DYN_HD void dyn_coul_erfc(float r2, float rinv, float qq, float beta,
 #define DYN_COUL_TAB_XMAX 4.0f
 #define DYN_COUL_TAB_SCALE 128.0f  // intervals per unit of x
 
-DYN_HD float dyn_tab_cubic(const float4* tab, int entry_off, float x) {
-  const float u = x * DYN_COUL_TAB_SCALE;
+// The table's cubic at u = x · DYN_COUL_TAB_SCALE (u given).
+DYN_HD float dyn_tab_cubic_u(const float4* tab, int entry_off, float u) {
   const float fl = dyn_floorf(u);
   int i = (int)fl;
   i = i < 0 ? 0 : (i > DYN_COUL_TAB_N - 1 ? DYN_COUL_TAB_N - 1 : i);
@@ -318,12 +318,18 @@ DYN_HD float dyn_tab_cubic(const float4* tab, int entry_off, float x) {
   return dyn_fmaf(dyn_fmaf(dyn_fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
 
+DYN_HD float dyn_tab_cubic(const float4* tab, int entry_off, float x) {
+  return dyn_tab_cubic_u(tab, entry_off, x * DYN_COUL_TAB_SCALE);
+}
+
+// u = βr·DYN_COUL_TAB_SCALE in one product: the scale is a power of two, so
+// r·(β·scale) equals (r·β)·scale bit for bit.
 DYN_HD void dyn_coul_erfc_tab(float r2, float rinv, float qq, float beta, const float4* tab,
                               float* s, float* e) {
-  const float x = (r2 * rinv) * beta;
-  const float phi = dyn_tab_cubic(tab, 0, x);
+  const float u = (r2 * rinv) * (beta * DYN_COUL_TAB_SCALE);
+  const float phi = dyn_tab_cubic_u(tab, 0, u);
   *s = qq * ((rinv * rinv) * rinv) * phi;
-  *e = qq * rinv * dyn_tab_cubic(tab, 1, x);
+  *e = qq * rinv * dyn_tab_cubic_u(tab, 1, u);
 }
 
 // The table (2·DYN_COUL_TAB_N float4 as 8·DYN_COUL_TAB_N floats), built on

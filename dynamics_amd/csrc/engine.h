@@ -108,7 +108,7 @@ DYN_HD void dyn_tile_pair(const float pi[4], int ti, const float pj[4], int tj, 
   if (KIND == DYN_KIND_SS || KIND == DYN_KIND_SQ || KIND == DYN_KIND_QQ) {
     float sc, ee;
     dyn_coul_erfc_tab(r2s, rinv, pi[3] * pj[3], P.beta, P.coul_tab, &sc, &ee);
-    s = s + (in ? sc : 0.f);
+    s = KIND == DYN_KIND_SS ? s + (in ? sc : 0.f) : (in ? sc : 0.f);
     ec = in ? ee : 0.f;
   }
   *s_out = s;

@@ -486,11 +486,12 @@ constexpr uint32_t kExPartners = kExHash / 2;  // partners of one cluster's site
 constexpr int kListBlock = 256;                // four waves, each on its own rows
 constexpr int kListWaves = kListBlock / kWave;
 constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk (units beyond: grid stride)
+constexpr int kWalkPer = 1;                    // candidates per lane in a sweep and in a round
 struct WalkLds {
   float4 pI[DYN_CL_SIZE];  // I's slot positions (padding slots: NaN)
   uint32_t js[kWave], beg[kWave], end[kWave];
   uint32_t img[kWave];     // the column's periodic image seen from I (cand_code bits 21-24)
-  uint32_t ok[kWave];      // cand_code of the sweep's passing candidates, in walk order
+  uint32_t ok[kWalkPer * kWave];  // cand_code of the sweep's passing candidates, in walk order
   uint32_t ex[kExHash];
 };
 
@@ -678,90 +679,126 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp) {
         L.end[lane] = incl;
         L.img[lane] = img;
         wave_sync();
-        for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
-          // candidate t: its cluster (first column whose range ends after
-          // t), the image of its column and its bounding box
-          const uint32_t t = t0 + lane;
-          bool ok = t < tot;
-          uint32_t code = 0;
-          if (ok) {
-            int lo = 0, hi = kWave - 1;
-            while (lo < hi) {
-              const int mid = (lo + hi) >> 1;
-              if (L.end[mid] > t) hi = mid;
-              else lo = mid + 1;
+        // sweeps of kWalkPer·64 candidates, kWalkPer per lane (their loads
+        // in flight together): candidate t is lane t mod 64 of part t / 64
+        for (uint32_t t0 = 0; t0 < tot; t0 += kWalkPer * kWave) {
+          bool ok[kWalkPer];
+          uint32_t code[kWalkPer];
+#pragma unroll
+          for (int r = 0; r < kWalkPer; ++r) {
+            // candidate t: its cluster (first column whose range ends after
+            // t), the image of its column and its bounding box
+            const uint32_t t = t0 + r * kWave + lane;
+            ok[r] = t < tot;
+            code[r] = 0;
+            if (ok[r]) {
+              int lo = 0, hi = kWave - 1;
+              while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (L.end[mid] > t) hi = mid;
+                else lo = mid + 1;
+              }
+              const uint32_t J = L.js[lo] + (t - L.beg[lo]);
+              const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
+              float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
+              const float kz = rintf(dz * A.g.invL[2]);
+              dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
+              dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
+              dz = fabsf(dz - A.g.L[2] * kz);
+              const uint32_t zc =
+                  A.rlist + h.z + hJ.z < 0.45f * A.g.L[2] ? (uint32_t)((int)kz + 1) : kZSlot;
+              dx = fmaxf(0.f, dx - h.x - hJ.x);
+              dy = fmaxf(0.f, dy - h.y - hJ.y);
+              dz = fmaxf(0.f, dz - h.z - hJ.z);
+              ok[r] = dx * dx + dy * dy + dz * dz < A.rlist2;
+              code[r] = J | L.img[lo] | zc << 25 | (__float_as_uint(hJ.w) - 1u) << 27;
             }
-            const uint32_t J = L.js[lo] + (t - L.beg[lo]);
-            const float4 cJ = A.bb_c[J], hJ = A.bb_h[J];
-            float dx = c.x - cJ.x, dy = c.y - cJ.y, dz = c.z - cJ.z;
-            const float kz = rintf(dz * A.g.invL[2]);
-            dx = fabsf(dx - A.g.L[0] * rintf(dx * A.g.invL[0]));
-            dy = fabsf(dy - A.g.L[1] * rintf(dy * A.g.invL[1]));
-            dz = fabsf(dz - A.g.L[2] * kz);
-            const uint32_t zc =
-                A.rlist + h.z + hJ.z < 0.45f * A.g.L[2] ? (uint32_t)((int)kz + 1) : kZSlot;
-            dx = fmaxf(0.f, dx - h.x - hJ.x);
-            dy = fmaxf(0.f, dy - h.y - hJ.y);
-            dz = fmaxf(0.f, dz - h.z - hJ.z);
-            ok = dx * dx + dy * dy + dz * dz < A.rlist2;
-            code = J | L.img[lo] | zc << 25 | (__float_as_uint(hJ.w) - 1u) << 27;
           }
-          const unsigned long long bal = __ballot(ok);
-          const uint32_t n_ok = (uint32_t)__popcll(bal);
-          if (ok) L.ok[__popcll(bal & lt)] = code;
+          uint32_t n_ok = 0;
+#pragma unroll
+          for (int r = 0; r < kWalkPer; ++r) {
+            const unsigned long long bal = __ballot(ok[r]);
+            if (ok[r]) L.ok[n_ok + __popcll(bal & lt)] = code[r];
+            n_ok += (uint32_t)__popcll(bal);
+          }
           wave_sync();
-          // eight candidate clusters per round: lane jj + 8·jc tests j slot
-          // jj of candidate jc against all eight i slots, so the kept slots
-          // come out candidate by candidate, each in slot order, as a
+          // rounds of kWalkPer·8 candidates: lane jj + 8·jc tests j slot jj of
+          // candidates k0 + 8·r + jc against all eight i slots, so the kept
+          // slots come out candidate by candidate, each in slot order, as a
           // one-candidate-at-a-time walk would write them
-          for (uint32_t k0 = 0; k0 < n_ok; k0 += kWave / DYN_CL_SIZE) {
-            const uint32_t kk = k0 + jc;
-            uint32_t cd = 0;
-            float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (kk < n_ok) {
-              cd = L.ok[kk];
-              pj = A.xyzq[(cd & kCandJ) * DYN_CL_SIZE + jj];
-            }
-            const bool jv = kk < n_ok && (uint32_t)jj <= ((cd >> 27) & 7u);
-            const bool zslot = jv && ((cd >> 25) & 3u) == kZSlot;
-            bool keep = false;
-            if (!full_xy && __ballot(zslot) == 0ull) {
-              const float px = pj.x + cand_shift(cd, 21, A.g.L[0]);
-              const float py = pj.y + cand_shift(cd, 23, A.g.L[1]);
-              const float pz = pj.z + cand_shift(cd, 25, A.g.L[2]);
+          for (uint32_t k0 = 0; k0 < n_ok; k0 += kWalkPer * (kWave / DYN_CL_SIZE)) {
+            uint32_t cd[kWalkPer];
+            float4 pj[kWalkPer];
+            bool jv[kWalkPer], zslot[kWalkPer], keep[kWalkPer];
+            bool any_zslot = false;
 #pragma unroll
-              for (int v = 0; v < DYN_CL_SIZE; ++v) {
-                const float4 q = L.pI[v];
-                const float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
-                keep |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
+            for (int r = 0; r < kWalkPer; ++r) {
+              const uint32_t kk = k0 + r * (kWave / DYN_CL_SIZE) + jc;
+              cd[r] = 0;
+              pj[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+              if (kk < n_ok) {
+                cd[r] = L.ok[kk];
+                pj[r] = A.xyzq[(cd[r] & kCandJ) * DYN_CL_SIZE + jj];
               }
-            } else {
-              const float px = full_xy ? pj.x : pj.x + cand_shift(cd, 21, A.g.L[0]);
-              const float py = full_xy ? pj.y : pj.y + cand_shift(cd, 23, A.g.L[1]);
-              const float pz = zslot ? pj.z : pj.z + cand_shift(cd, 25, A.g.L[2]);
+              jv[r] = kk < n_ok && (uint32_t)jj <= ((cd[r] >> 27) & 7u);
+              zslot[r] = jv[r] && ((cd[r] >> 25) & 3u) == kZSlot;
+              any_zslot = any_zslot || zslot[r];
+            }
+            if (!full_xy && __ballot(any_zslot) == 0ull) {
+              float px[kWalkPer], py[kWalkPer], pz[kWalkPer];
+              bool kp[kWalkPer];
+#pragma unroll
+              for (int r = 0; r < kWalkPer; ++r) {
+                px[r] = pj[r].x + cand_shift(cd[r], 21, A.g.L[0]);
+                py[r] = pj[r].y + cand_shift(cd[r], 23, A.g.L[1]);
+                pz[r] = pj[r].z + cand_shift(cd[r], 25, A.g.L[2]);
+                kp[r] = false;
+              }
 #pragma unroll
               for (int v = 0; v < DYN_CL_SIZE; ++v) {
                 const float4 q = L.pI[v];
-                float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
-                if (full_xy) {
-                  d0 = dyn_min_image_rint(d0, A.g.L[0], A.g.invL[0]);
-                  d1 = dyn_min_image_rint(d1, A.g.L[1], A.g.invL[1]);
+#pragma unroll
+                for (int r = 0; r < kWalkPer; ++r) {
+                  const float d0 = q.x - px[r], d1 = q.y - py[r], d2 = q.z - pz[r];
+                  kp[r] |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
                 }
-                if (zslot) d2 = dyn_min_image_rint(d2, A.g.L[2], A.g.invL[2]);
-                keep |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
+              }
+#pragma unroll
+              for (int r = 0; r < kWalkPer; ++r) keep[r] = kp[r] && jv[r];
+            } else {
+#pragma unroll
+              for (int r = 0; r < kWalkPer; ++r) {
+                const float px = full_xy ? pj[r].x : pj[r].x + cand_shift(cd[r], 21, A.g.L[0]);
+                const float py = full_xy ? pj[r].y : pj[r].y + cand_shift(cd[r], 23, A.g.L[1]);
+                const float pz = zslot[r] ? pj[r].z : pj[r].z + cand_shift(cd[r], 25, A.g.L[2]);
+                bool kp = false;
+#pragma unroll
+                for (int v = 0; v < DYN_CL_SIZE; ++v) {
+                  const float4 q = L.pI[v];
+                  float d0 = q.x - px, d1 = q.y - py, d2 = q.z - pz;
+                  if (full_xy) {
+                    d0 = dyn_min_image_rint(d0, A.g.L[0], A.g.invL[0]);
+                    d1 = dyn_min_image_rint(d1, A.g.L[1], A.g.invL[1]);
+                  }
+                  if (zslot[r]) d2 = dyn_min_image_rint(d2, A.g.L[2], A.g.invL[2]);
+                  kp |= dyn_fmaf(d2, d2, dyn_fmaf(d1, d1, d0 * d0)) < A.rlist2;
+                }
+                keep[r] = kp && jv[r];
               }
             }
-            keep = keep && jv;
-            const unsigned long long kb = __ballot(keep);
-            if (keep) {
-              const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
-              const uint32_t slot = (cd & kCandJ) * DYN_CL_SIZE + jj;
-              if (pos < A.row_cap) {
-                ent[pos] = slot | tag;
-                if (exl) xb[pos] = (uint8_t)ex_bits(L.ex, slot);
+#pragma unroll
+            for (int r = 0; r < kWalkPer; ++r) {
+              const unsigned long long kb = __ballot(keep[r]);
+              if (keep[r]) {
+                const uint32_t pos = n_out + (uint32_t)__popcll(kb & lt);
+                const uint32_t slot = (cd[r] & kCandJ) * DYN_CL_SIZE + jj;
+                if (pos < A.row_cap) {
+                  ent[pos] = slot | tag;
+                  if (exl) xb[pos] = (uint8_t)ex_bits(L.ex, slot);
+                }
               }
+              n_out += (uint32_t)__popcll(kb);
             }
-            n_out += (uint32_t)__popcll(kb);
           }
           wave_sync();  // L.ok is rewritten by the next sweep
         }
@@ -779,20 +816,24 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp) {
 
 // The rows in groups of 8 (each class segment, the two halves' entries,
 // padded to 8) and their scans: row_cnt, row_ptr and chunk_ptr, the totals
-// and the longest row.  One workgroup, a launch of its own.
+// and the longest row.  One workgroup of 1,024 threads, a launch of its own.
+constexpr int kRowScanBlock = 1024;
 __device__ __forceinline__ void k_eng_row_scan(const ListArgs& A, uint32_t* __restrict__ row_cnt,
                                                uint32_t* __restrict__ row_ptr,
                                                uint32_t* __restrict__ chunk_ptr) {
   if (!build_on(A.gate)) return;
   const uint32_t n_cl = *A.n_cl;
-  scan_pairs<kListBlock>(
+  static_assert(kUnits == 6, "a row's unit counts are three 8-byte words");
+  scan_pairs<kRowScanBlock>(
       n_cl,
       [&](uint32_t I) {
+        const uint2* uc = reinterpret_cast<const uint2*>(A.unit_cnt + (size_t)I * kUnits);
         uint32_t m = 0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          m += (A.unit_cnt[I * kUnits + 2 * k] + A.unit_cnt[I * kUnits + 2 * k + 1] +
-                DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+        for (int k = 0; k < 3; ++k) {
+          const uint2 h = uc[k];
+          m += (h.x + h.y + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+        }
         return make_uint2(m, (m + A.chunk - 1) / A.chunk);
       },
       row_ptr, chunk_ptr, row_cnt, A.tot + 0, A.tot + 1, A.tot + 2);
@@ -2072,7 +2113,8 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
       dyn_engine& e = *E[i];
       return args<k_eng_list_walk>(LA[i], e.ent_tmp.p);
     });
-    launch<k_eng_row_scan, kListBlock>(R, "k_eng_row_scan", 0, 1, kListBlock, 0, [&](uint32_t i) {
+    launch<k_eng_row_scan, kRowScanBlock>(R, "k_eng_row_scan", 0, 1, kRowScanBlock, 0,
+                                          [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_row_scan>(LA[i], e.row_cnt.p, T[i].row_ptr, T[i].chunk_ptr);
     });
