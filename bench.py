@@ -10,7 +10,8 @@ resident in HBM.
 (Langevin-middle, γ = 1/ps, 300 K; rigid water; SHAKE/RATTLE on bonds to
 hydrogen; harmonic bonds and angles and threefold torsions on the chain):
 nonbonded forces with the pair list reused inside a list buffer (rebuilt
-when a site has moved 0.8 × buffer/2), bonded forces, integration.
+on the device when a site has moved half the buffer), bonded forces,
+integration.
 --mode eval: one step is one nonbonded evaluation with the list rebuilt every
 time (no integration).
 ns/day = steps/s × dt (1 step/s = 0.1728 ns/day at 2 fs).
@@ -37,8 +38,8 @@ FP32_PEAK_TFLOPS = 157.3
 FLOPS_LJ_COUL, FLOPS_LJ, FLOPS_COUL = 63, 50, 51
 
 
-# list buffer of the MD modes (Å): the list is rebuilt when a site has moved
-# 0.25 × buffer since the last build (readings at most two steps old)
+# list buffer of the MD modes (Å): the device rebuilds the list when a site
+# has moved half the buffer since the last build
 MD_BUFFER = 2.5
 
 
@@ -51,8 +52,12 @@ def equilibrate(eng, ff, s, pd, pw, steps, seed=7):
     v, vw = systems.initial_velocities(s, ff, 300.0)
     m = api.Md(eng, ff, dt=0.002, temperature=300.0, gamma=10.0, shake_tol=1e-5, seed=seed)
     m.set_state(pd, v, pw, vw)
-    if steps:
-        m.step(steps)
+    done = 0
+    while done < steps:  # in calls of 500 steps, each reported on stderr
+        k = min(500, steps - done)
+        m.step(k)
+        done += k
+        print(f"bench: equilibration {done}/{steps} steps", file=sys.stderr, flush=True)
     st = m.state()
     m.close()
     return st

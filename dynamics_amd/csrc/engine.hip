@@ -60,17 +60,7 @@ constexpr unsigned kStColumn = 1u, kStRow = 2u, kStEntries = 4u, kStChunks = 8u,
                    kStPme = 32u,   // an SPME brick list overflowed
                    kStDisp = 64u;  // a site moved half the list buffer since the list's build
 
-// Words of an engine's device flags (dflags).
-constexpr int kFlDisp = 0;       // largest squared displacement this evaluation (float bits)
-constexpr int kFlBuilds = 1;     // list builds so far
-// words 2 and 3 are unused
-constexpr int kFlReady = 4;      // 1: the staging list holds a finished build
-constexpr int kFlSel = 5;        // which copy of cj, cmask and chunks the last evaluation used
-constexpr int kFlCommit = 6;     // 1: this evaluation's preparation kernel took the staged list
-constexpr int kFlFillDone = 8;   // finished workgroups of the fill launch
-constexpr int kFlPrevDisp = 9;   // the previous evaluation's largest displacement (float bits, Å)
-constexpr int kFlGrowth = 10;    // decaying maximum of its growth per evaluation (float bits, Å)
-constexpr int kFlWords = 16;
+// The words of the engine's device flags are defined in forces.h.
 
 struct ClassGrid {
   int ncx, ncy;
@@ -973,6 +963,10 @@ struct PairArgs {
   const int32_t* mol_of;     // per solute atom: its molecule
   uint32_t n_mol;
   int64_t* fitem;            // per work item: 3 components × 8 i slots, fixed point
+  const int32_t* slot_site;  // per slot: its site (−1: padding)
+  int64_t* fx_site;          // site forces (fixed point, component-major): the i forces are added
+  uint32_t n_sites;
+  uint32_t unused;           // no implicit padding: argument packs are compared byte for byte
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
   uint32_t chunk;            // j groups per work item
@@ -1210,16 +1204,21 @@ k_eng_pairs(PairArgs A) {
       T.fi[c] = f;
     }
     if (jb == 0) {
+      // the item's row of i forces (kept for inspection), and the same
+      // fixed-point values added to the sites' sums (integer sums: the order
+      // of the items does not change them)
       int64_t* o = A.fitem + (size_t)item * (3 * DYN_CL_SIZE);
-      if constexpr (F64) {
-        o[ia] = dyn_fx64(T.fi[0]);
-        o[DYN_CL_SIZE + ia] = dyn_fx64(T.fi[1]);
-        o[2 * DYN_CL_SIZE + ia] = dyn_fx64(T.fi[2]);
-      } else {
-        o[ia] = dyn_fx(T.fi[0]);
-        o[DYN_CL_SIZE + ia] = dyn_fx(T.fi[1]);
-        o[2 * DYN_CL_SIZE + ia] = dyn_fx(T.fi[2]);
+      int64_t q[3];
+      for (int c = 0; c < 3; ++c) {
+        if constexpr (F64) q[c] = dyn_fx64(T.fi[c]);
+        else q[c] = dyn_fx(T.fi[c]);
+        o[c * DYN_CL_SIZE + ia] = q[c];
       }
+      const int32_t site = A.slot_site[I * DYN_CL_SIZE + ia];
+      if (site >= 0)
+        for (int c = 0; c < 3; ++c)
+          atomicAdd(reinterpret_cast<unsigned long long*>(A.fx_site + (size_t)c * A.n_sites + site),
+                    (unsigned long long)q[c]);
     }
     if (!ENERGY) continue;
     double acc[6] = {T.e_lj_w, T.e_lj_s, T.e_c_w, T.e_c_s, T.vir, T.dh};
@@ -1537,39 +1536,6 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
   }
 }
 
-// Sum of the work-item force rows of slot k's cluster row (fixed point).
-__device__ __forceinline__ void slot_item_forces(const int64_t* __restrict__ fitem,
-                                                 const uint32_t* __restrict__ chunk_ptr,
-                                                 uint64_t cap_chunks, uint32_t k, int64_t f[3]) {
-  const uint32_t I = k / DYN_CL_SIZE, ia = k % DYN_CL_SIZE;
-  const uint64_t c0 = min((uint64_t)chunk_ptr[I], cap_chunks);
-  const uint64_t c1 = min((uint64_t)chunk_ptr[I + 1], cap_chunks);
-  uint64_t c = c0;
-  // four rows per round: their twelve loads are independent and issued
-  // together (integer sums: the order does not change the result)
-  for (; c + 4 <= c1; c += 4) {
-    int64_t v[4][3];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t* o = fitem + (c + u) * (3 * DYN_CL_SIZE);
-      v[u][0] = o[ia];
-      v[u][1] = o[DYN_CL_SIZE + ia];
-      v[u][2] = o[2 * DYN_CL_SIZE + ia];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      f[0] += v[u][0];
-      f[1] += v[u][1];
-      f[2] += v[u][2];
-    }
-  }
-  for (; c < c1; ++c) {
-    const int64_t* o = fitem + c * (3 * DYN_CL_SIZE);
-    f[0] += o[ia];
-    f[1] += o[DYN_CL_SIZE + ia];
-    f[2] += o[2 * DYN_CL_SIZE + ia];
-  }
-}
 
 // Site-order forces: the site accumulators (corrections), the SPME forces
 // (this evaluation's or the kept ones of an earlier one) and the pair
@@ -1582,33 +1548,19 @@ __device__ __forceinline__ void k_eng_finalize(const int64_t* __restrict__ fx_si
                                uint32_t n_slots, int64_t* __restrict__ out,
                                unsigned int* __restrict__ fl, int forced) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) {
-    // the end of the evaluation's list bookkeeping: a staged list taken by
-    // the preparation kernel is the current one from now on; the growth of
-    // the largest displacement (a decaying maximum of its increase per
-    // evaluation; a new reference makes the increase negative, and a forced
-    // build's reading, against the reference it replaces, counts as none);
-    // the displacement reading starts again at zero
-    if (fl[kFlCommit]) {
-      fl[kFlSel] ^= 1u;
-      fl[kFlReady] = 0u;
-      fl[kFlCommit] = 0u;
-    }
-    const float d = sqrtf(__uint_as_float(fl[kFlDisp]));
-    float growth = __uint_as_float(fl[kFlGrowth]);
-    if (!forced) growth = fmaxf(0.97f * growth, d - __uint_as_float(fl[kFlPrevDisp]));
-    fl[kFlGrowth] = __float_as_uint(growth);
-    fl[kFlPrevDisp] = __float_as_uint(forced ? 0.0f : d);
-    fl[kFlDisp] = 0u;
-  }
+  if (s == 0) evaluation_flags_end(fl, forced);
   if (s >= n_sites) return;
-  int64_t f[3] = {fx_site[s], fx_site[(size_t)n_sites + s], fx_site[2 * (size_t)n_sites + s]};
-  if (fx_recip) {
-    f[0] += fx_recip[s];
-    f[1] += fx_recip[(size_t)n_sites + s];
-    f[2] += fx_recip[2 * (size_t)n_sites + s];
-  }
-  if (fitem) slot_item_forces(fitem, chunk_ptr, cap_chunks, min(slot_of[s], n_slots - 1), f);
+  ForceParts P;
+  P.fx_site = fx_site;
+  P.fx_recip = fx_recip;
+  P.fitem = fitem;
+  P.chunk_ptr = chunk_ptr;
+  P.slot_of = slot_of;
+  P.cap_chunks = cap_chunks;
+  P.n_sites = n_sites;
+  P.n_slots = n_slots;
+  int64_t f[3];
+  site_force_sum(P, s, f);
   for (int c = 0; c < 3; ++c) out[(size_t)c * n_sites + s] = f[c];
 }
 
@@ -1809,6 +1761,10 @@ struct dyn_engine {
   // [0] status, [1..5] counters, [6] builds, [7] current copy, [8..12] staging counters
   PinnedBuf<uint32_t> h_misc;
   bool fx_fresh = false;
+  // the last evaluation's force parts; fx_stale: enqueued with
+  // kComputeFusedFinalize, so fx_out does not hold its forces
+  ForceParts parts{};
+  bool fx_stale = false;
 
   hipStream_t own_stream = nullptr;  // the engine's stream (a batch may lend it another)
   hipStream_t side_stream = nullptr;  // its second stream (SPME), same priority
@@ -2369,6 +2325,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.mol_of = e.mol_of.p;
       A.n_mol = e.n_mol;
       A.fitem = e.fitem.p;
+      A.slot_site = e.slot_site.p;
+      A.fx_site = e.fx_site.p;
+      A.n_sites = e.n_sites;
       A.partials = e.pair_part.p;
       A.cap_chunks = e.cap_chunks;
       A.chunk = e.prm.item_groups;
@@ -2400,15 +2359,33 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
   }
   if (any_staged && one_stream) set_build_lists(S, R, sgate, true);
   stream_order(RS.stream, R.stream, S.ev_join);
+  const bool fused = (flags & kComputeFusedFinalize) != 0;
+  for (uint32_t i = 0; i < ns; ++i) {
+    dyn_engine& e = *E[i];
+    const Plan& pl = PL[i];
+    ForceParts& P = e.parts;
+    std::memset(&P, 0, sizeof P);
+    P.fx_site = e.fx_site.p;
+    P.fx_recip = (pl.recip_on || pl.reuse) ? e.fx_recip.p : nullptr;
+    P.fitem = nullptr;  // the pair kernel adds its i forces to fx_site
+    P.chunk_ptr = e.chunk_ptr.p;
+    P.slot_of = e.slot_of.p;
+    P.cap_chunks = e.cap_chunks;
+    P.n_sites = e.n_sites;
+    P.n_slots = e.cap_slots;
+    P.fl = e.dflags.p;
+    P.forced = forced[i] ? 1 : 0;
+    e.fx_stale = fused;
+  }
   {
     Profiler::Scope ps(prof, "finalize");
     const unsigned gs = max_grid(S, [](dyn_engine& e) { return grid_for(e.n_sites, 256); });
-    launch<k_eng_finalize>(R, "k_eng_finalize", 0, gs, 256, 0, [&](uint32_t i) {
+    if (!fused) launch<k_eng_finalize>(R, "k_eng_finalize", 0, gs, 256, 0, [&](uint32_t i) {
       dyn_engine& e = *E[i];
       const Plan& pl = PL[i];
       return args<k_eng_finalize>(e.fx_site.p,
                                   (pl.recip_on || pl.reuse) ? e.fx_recip.p : (const int64_t*)nullptr,
-                                  short_on ? e.fitem.p : (const int64_t*)nullptr, e.chunk_ptr.p,
+                                  (const int64_t*)nullptr, e.chunk_ptr.p,
                                   e.cap_chunks, e.slot_of.p, e.n_sites, e.cap_slots, e.fx_out.p,
                                   e.dflags.p, forced[i] ? 1 : 0);
     });
@@ -2956,6 +2933,7 @@ extern "C" int dyn_engine_sync(dyn_engine* eng) {
 
 static void fetch_fx(dyn_engine* eng) {
   if (!eng->computed) throw Error(DYN_ERR_STATE, "no compute yet");
+  if (eng->fx_stale) throw Error(DYN_ERR_STATE, "the last evaluation's forces were summed by its caller");
   if (eng->fx_fresh) return;
   eng->check_status();
   DYN_HIP(hipMemcpyAsync(eng->h_fx.p, eng->fx_out.p, sizeof(int64_t) * 3 * eng->n_sites,
@@ -3177,6 +3155,8 @@ void engine_set_stream(dyn_engine* eng, hipStream_t s) {
   eng->pme.cache.invalidate();
 }
 const int64_t* engine_forces_device(const dyn_engine* eng) { return eng->fx_out.p; }
+ForceParts engine_force_parts(const dyn_engine* eng) { return eng->parts; }
+bool engine_forces_stale(const dyn_engine* eng) { return eng->fx_stale; }
 EngineSet engine_self_set(dyn_engine* eng) { return eng->self_set(); }
 int engine_device(const dyn_engine* eng) { return eng->device; }
 void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water) {
@@ -3192,6 +3172,7 @@ int engine_mode(const dyn_engine* eng) { return (int)eng->prm.mode; }
 // on the engine's stream; engine_restore puts them back and marks the list
 // for a rebuild (it was built for positions the caller is discarding).
 void engine_checkpoint(dyn_engine* e) {
+  if (e->fx_stale) throw Error(DYN_ERR_STATE, "the last evaluation's forces were summed by its caller");
   const size_t nf = 3 * (size_t)e->n_sites;
   e->fx_out_ck.reserve(nf);
   e->eng_ck.reserve(kEngOut);
@@ -3223,6 +3204,7 @@ void engine_restore(dyn_engine* e) {
   e->energies_valid = e->ck.energies_valid;
   e->mol_valid = false;
   e->fx_fresh = false;
+  e->fx_stale = false;
   e->list_valid = false;
 }
 }  // namespace dyn

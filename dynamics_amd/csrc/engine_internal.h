@@ -10,6 +10,7 @@
 
 #include "batch.h"
 #include "common.h"
+#include "forces.h"
 
 struct dyn_engine;
 
@@ -39,6 +40,11 @@ struct EngineSet {
 // the pair kernel, and become current at the next evaluation; an evaluation
 // whose current list has gone stale is void (status) and its call repeats.
 constexpr uint32_t kComputeStagedBuild = 1u << 30;
+// The caller's next launch sums the site forces from their parts and ends
+// the evaluation's list bookkeeping (engine_force_parts); no finalize launch,
+// and the engine's force array stays stale until a later evaluation
+// finalizes.
+constexpr uint32_t kComputeFusedFinalize = 1u << 29;
 void engine_set_enqueue(EngineSet& S, uint32_t flags,
                         const std::function<void(const BatchRun&)>& side_work = nullptr);
 // Wait for the set's work and check each engine's device status (throws the
@@ -62,6 +68,10 @@ void engine_sizes(const dyn_engine* eng, uint32_t* n_dyn, uint32_t* n_water);
 void engine_box(const dyn_engine* eng, float box[3]);
 int engine_mode(const dyn_engine* eng);  // DYN_MODE_*
 const int64_t* engine_forces_device(const dyn_engine* eng);
+// The parts of the last evaluation's forces, and whether the engine's force
+// array is stale (the evaluation was enqueued with kComputeFusedFinalize).
+ForceParts engine_force_parts(const dyn_engine* eng);
+bool engine_forces_stale(const dyn_engine* eng);
 // Keep / put back the last evaluation's forces, SPME forces and energies
 // (enqueued on the engine's stream); a restore also invalidates the list.
 void engine_checkpoint(dyn_engine* eng);

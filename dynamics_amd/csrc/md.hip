@@ -123,13 +123,24 @@ struct Fx {
   const int64_t* nb;  // engine forces, site order, component-major
   const int64_t* bd;  // bonded forces over solute atoms, component-major
   uint32_t n_sites, n_dyn;
+  // parts != 0: the engine left its forces as parts (no finalize launch);
+  // they are summed here, as the finalize kernel would
+  ForceParts P;
+  int parts;
+  int unused;  // no implicit padding: argument packs are compared byte for byte
 };
+__device__ __forceinline__ void engine_force(const Fx& F, uint32_t s, int64_t f[3]) {
+  if (F.parts) {
+    site_force_sum(F.P, s, f);
+  } else {
+    for (int c = 0; c < 3; ++c) f[c] = F.nb[(size_t)c * F.n_sites + s];
+  }
+}
 __device__ __forceinline__ V3 force_of(const Fx& F, uint32_t s) {
   int64_t f[3];
-  for (int c = 0; c < 3; ++c) {
-    f[c] = F.nb[(size_t)c * F.n_sites + s];
+  engine_force(F, s, f);
+  for (int c = 0; c < 3; ++c)
     if (s < F.n_dyn) f[c] += F.bd[(size_t)c * F.n_dyn + s];
-  }
   return {(float)((double)f[0] * DYN_FX_INV_SCALE), (float)((double)f[1] * DYN_FX_INV_SCALE),
           (float)((double)f[2] * DYN_FX_INV_SCALE)};
 }
@@ -657,10 +668,11 @@ struct GroupArgs {
 };
 
 __device__ __forceinline__ void force_of_d(const Fx& F, uint32_t s, double f[3]) {
+  int64_t v[3];
+  engine_force(F, s, v);
   for (int c = 0; c < 3; ++c) {
-    int64_t v = F.nb[(size_t)c * F.n_sites + s];
-    if (s < F.n_dyn) v += F.bd[(size_t)c * F.n_dyn + s];
-    f[c] = (double)v * DYN_FX_INV_SCALE;
+    if (s < F.n_dyn) v[c] += F.bd[(size_t)c * F.n_dyn + s];
+    f[c] = (double)v[c] * DYN_FX_INV_SCALE;
   }
 }
 
@@ -975,6 +987,8 @@ __device__ __forceinline__ void k_md_step_a(StepArgs A, float4* __restrict__ x,
 __device__ __forceinline__ void k_md_step_ba(StepArgs A, float4* __restrict__ x,
                                              float4* __restrict__ v,
                                              uint64_t* __restrict__ step_dev) {
+  // in place of the engine's finalize launch: the evaluation's bookkeeping
+  if (A.F.parts && blockIdx.x == 0 && threadIdx.x == 0) evaluation_flags_end(A.F.P.fl, A.F.P.forced);
   if (*A.status) return;  // every workgroup reads the same word: none counts itself
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < A.water0 + A.G.n_water) step_group<true>(A, x, v, g, step_dev);
@@ -1153,7 +1167,7 @@ void engine_set_compute_sync(EngineSet& S, uint32_t flags) {
 // is found at the wait that ends the step call).  Nonbonded energies are
 // evaluated only when asked (the last step of a step call), the forces
 // alone otherwise.
-void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
+void md_forces(MdSet& S, bool energies, bool fb_clear = true, bool fuse = false) {
   bool first = false, cadence_skip = true;
   for (dyn_md* m : S.m) {
     first = first || !m->forces_valid;
@@ -1212,10 +1226,28 @@ void md_forces(MdSet& S, bool energies, bool fb_clear = true) {
     engine_set_compute_sync(S.es, flags);
     bonded(S.R);
   } else {
-    engine_set_enqueue(S.es, flags | DYN_COMPUTE_ASYNC | (S.staged_builds ? kComputeStagedBuild : 0u),
+    engine_set_enqueue(S.es,
+                       flags | DYN_COMPUTE_ASYNC | (S.staged_builds ? kComputeStagedBuild : 0u) |
+                           (fuse ? kComputeFusedFinalize : 0u),
                        bonded);
   }
   for (dyn_md* m : S.m) m->forces_valid = true;
+}
+
+// The forces of the engine's last evaluation: its force array, or its parts
+// when the evaluation left the sum to the step kernel.
+Fx force_view(const dyn_md& m) {
+  Fx F;
+  std::memset(&F, 0, sizeof F);
+  F.nb = engine_forces_device(m.eng);
+  F.bd = m.fb.p;
+  F.n_sites = m.n_sites;
+  F.n_dyn = m.n_dyn;
+  if (engine_forces_stale(m.eng)) {
+    F.P = engine_force_parts(m.eng);
+    F.parts = 1;
+  }
+  return F;
 }
 
 ConsArgs cons_args(const dyn_md& M) {
@@ -1252,11 +1284,7 @@ void md_pressure_enqueue(MdSet& S) {
   for (dyn_md* m : S.m) g = std::max(g, m->pblocks);
   launch<k_md_press, kBlock>(S.R, "k_md_press", 0, g, kBlock, 0, [&](uint32_t i) {
     dyn_md& m = *S.m[i];
-    Fx F;
-    F.nb = engine_forces_device(m.eng);
-    F.bd = m.fb.p;
-    F.n_sites = m.n_sites;
-    F.n_dyn = m.n_dyn;
+    const Fx F = force_view(m);
     return args<k_md_press>(group_args(m), (const float4*)m.x.p, (const float4*)m.v.p, F,
                             m.ppart.p, m.pblocks);
   });
@@ -1319,10 +1347,7 @@ unsigned step_grid(const MdSet& S) {
 StepArgs step_args(const dyn_md& m) {
   StepArgs A;
   std::memset(&A, 0, sizeof A);
-  A.F.nb = engine_forces_device(m.eng);
-  A.F.bd = m.fb.p;
-  A.F.n_sites = m.n_sites;
-  A.F.n_dyn = m.n_dyn;
+  A.F = force_view(m);
   A.acc_conv = m.acc_conv.p;
   A.G = group_args(m);
   A.C = cons_args(m);
@@ -1404,7 +1429,9 @@ bool md_one_step(MdSet& S, bool energies, bool begun, bool may_join) {
       return args<k_md_step_a>(step_args(m), m.x.p, m.v.p, (const uint64_t*)m.step_dev.p);
     });
   }
-  md_forces(S, energies, /*fb_clear=*/false);
+  // a joined step without pressure coupling reads the forces in k_md_step_ba
+  // only: that launch sums them from their parts (no finalize launch)
+  md_forces(S, energies, /*fb_clear=*/false, /*fuse=*/may_join && !couple);
   if (couple && langevin) md_barostat(S);  // before the last half-kick (src/integrate.rs:172-192)
   const bool join = may_join && !(couple && !langevin);
   if (join) {
