@@ -39,8 +39,11 @@ FLOPS_LJ_COUL, FLOPS_LJ, FLOPS_COUL = 63, 50, 51
 
 
 # list buffer of the MD modes (Å): the device rebuilds the list when a site
-# has moved half the buffer since the last build
-MD_BUFFER = 2.5
+# has moved half the buffer since the last build; with 32 j groups per work
+# item of the pair kernel, the autotuner's choice for the MD workload
+# (`--mode tune`)
+MD_BUFFER = 2.0
+MD_ITEM_GROUPS = 32
 
 
 def equilibrate(eng, ff, s, pd, pw, steps, seed=7):
@@ -96,6 +99,8 @@ def main() -> int:
     ap.add_argument("--single-stream", action="store_true",
                     help="every kernel on the engine's stream (no second stream for SPME, "
                          "corrections and bonded forces)")
+    ap.add_argument("--item-groups", type=int, default=MD_ITEM_GROUPS,
+                    help="j groups (of 8 j slots) per work item of the pair kernel (MD modes)")
     ap.add_argument("--list-buffer", type=float, default=None,
                     help="list buffer in Å (MD modes default %.1f; eval default 0: rebuild "
                          "every step)" % MD_BUFFER)
@@ -135,7 +140,7 @@ def main() -> int:
     K = (64, 64, 64)
     eng = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
                      mode=api.MODE_REFERENCE, list_buffer=args.list_buffer,
-                     single_stream=args.single_stream)
+                     item_groups=args.item_groups, single_stream=args.single_stream)
     # the built chain has close contacts: quench them first (untimed), so that
     # both modes run on a physical configuration
     ff = systems.chain_force_field(s)
@@ -263,6 +268,7 @@ def main() -> int:
             "list": ("rebuilt every step" if args.list_buffer <= 0 else
                      f"buffer {args.list_buffer} A, rebuilt on the device when a site has "
                      "moved half the buffer"),
+            "item_groups": args.item_groups,
             "list_builds_in_timed_steps": builds,
             "steps_in_graph_launches": graph_steps,
             "launch": ("direct" if args.direct_launch else "graphs of 16 steady steps") +
@@ -373,7 +379,8 @@ def run_batch_lambda(args) -> int:
 
     def engine():
         return api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=K, order=4,
-                          mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
+                          mode=api.MODE_REFERENCE, list_buffer=args.list_buffer,
+                          item_groups=args.item_groups)
 
     e0 = engine()
     pd0, pw0 = api.relax(e0, ff, s.pos_dyn, systems.water_positions(s), steps=args.relax_steps)

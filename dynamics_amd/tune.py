@@ -23,6 +23,7 @@ same procedure.
 
 from __future__ import annotations
 
+import sys
 import time
 
 import numpy as np
@@ -130,6 +131,8 @@ def tune_md(system, ff, state, cutoff=10.0, beta=0.3123, grid=(64, 64, 64), orde
                                         "direct_launch": direct, "single_stream": one,
                                         "staged_builds": staged, "ms_per_step": ms,
                                         "list_builds_per_step": builds / steps})
+                        print(f"tune: buffer {b} groups {g} direct {direct} one stream {one} "
+                              f"staged {staged}: {ms:.4f} ms/step", file=sys.stderr, flush=True)
     best = min(md_rows, key=lambda r: r["ms_per_step"])
     choice.update({k: best[k] for k in ("list_buffer", "item_groups", "direct_launch",
                                         "single_stream", "staged_builds", "ms_per_step")})
