@@ -966,7 +966,7 @@ struct PairArgs {
   const int32_t* slot_site;  // per slot: its site (−1: padding)
   int64_t* fx_site;          // site forces (fixed point, component-major): the i forces are added
   uint32_t n_sites;
-  uint32_t unused;           // no implicit padding: argument packs are compared byte for byte
+  uint32_t keep_items;       // write each item's row of i forces to fitem (for inspection)
   double* partials;          // kNumPart values × cap_chunks (value-major), per work item
   uint64_t cap_chunks;
   uint32_t chunk;            // j groups per work item
@@ -1204,15 +1204,15 @@ k_eng_pairs(PairArgs A) {
       T.fi[c] = f;
     }
     if (jb == 0) {
-      // the item's row of i forces (kept for inspection), and the same
-      // fixed-point values added to the sites' sums (integer sums: the order
-      // of the items does not change them)
+      // the i forces in fixed point, added to the sites' sums (integer sums:
+      // the order of the items does not change them) and, for inspection,
+      // kept as the item's row
       int64_t* o = A.fitem + (size_t)item * (3 * DYN_CL_SIZE);
       int64_t q[3];
       for (int c = 0; c < 3; ++c) {
         if constexpr (F64) q[c] = dyn_fx64(T.fi[c]);
         else q[c] = dyn_fx(T.fi[c]);
-        o[c * DYN_CL_SIZE + ia] = q[c];
+        if (A.keep_items) o[c * DYN_CL_SIZE + ia] = q[c];
       }
       const int32_t site = A.slot_site[I * DYN_CL_SIZE + ia];
       if (site >= 0)
@@ -1426,18 +1426,51 @@ struct PrepArgs {
   ListSlots stg;      // the staging list, copied into cur when it holds a finished build
   unsigned int* fl;   // the engine's device flags (kFl*)
   int force;          // the host asks for a build before the pair kernel: a staged one is dropped
-  const uint32_t* qsite;
-  uint32_t nq;
-  float4* pme_xyzq;   // null: no SPME this evaluation
-  float4* pme_b;      // null: one SPME evaluation
-  uint32_t alch0, alch1;
   int64_t* fx_site;
-  int64_t* fx_recip;  // null: SPME forces kept
   int64_t* mol_e;
   uint32_t n_mol2;
-  BinArgs bin;        // SPME binning of the charges written here
-  int bin_on;
+  uint32_t unused;    // no implicit padding: argument packs are compared byte for byte
 };
+
+// The SPME charges of an evaluation that runs SPME, by the first launch of
+// the second stream (off the main stream's path): their wrapped positions
+// (and the second evaluation's charges), their listing under the grid
+// bricks, and the cleared SPME force sums.
+struct PmePrepArgs {
+  const float4* pos;
+  const float* q;
+  EngGeom g;
+  const uint32_t* qsite;
+  uint32_t nq;
+  uint32_t alch0, alch1;
+  uint32_t unused;
+  float4* pme_xyzq;
+  float4* pme_b;      // null: one SPME evaluation
+  int64_t* fx_recip;
+  BinArgs bin;
+};
+__device__ __forceinline__ void k_eng_pme_prep(PmePrepArgs A) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = A.g.n_dyn + 4u * A.g.n_water;
+  if (t < n) {
+    A.fx_recip[t] = 0;
+    A.fx_recip[(size_t)n + t] = 0;
+    A.fx_recip[2 * (size_t)n + t] = 0;
+  }
+  // whole waves past the charges return; every lane of a wave with charges
+  // takes part in the binning
+  if ((t & ~(uint32_t)(kWave - 1)) >= A.nq) return;
+  const bool qv = t < A.nq;
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (qv) {
+    const uint32_t site = A.qsite[t];
+    p = wrapped(A.pos, A.q, A.g, site);
+    A.pme_xyzq[t] = p;
+    if (A.pme_b)
+      A.pme_b[t] = make_float4(p.x, p.y, p.z, site >= A.alch0 && site < A.alch1 ? -p.w : p.w);
+  }
+  pme_bin_charge(A.bin, p, qv, t);
+}
 
 // Copies n values of a staging array into the current list's (grid stride).
 template <class T>
@@ -1481,27 +1514,9 @@ __device__ __forceinline__ void k_eng_prep(PrepArgs A) {
     A.fx_site[t] = 0;
     A.fx_site[(size_t)n + t] = 0;
     A.fx_site[2 * (size_t)n + t] = 0;
-    if (A.fx_recip) {
-      A.fx_recip[t] = 0;
-      A.fx_recip[(size_t)n + t] = 0;
-      A.fx_recip[2 * (size_t)n + t] = 0;
-    }
   }
   if (A.mol_e)
     for (uint32_t k = t; k < A.n_mol2; k += gridDim.x * blockDim.x) A.mol_e[k] = 0;
-  if (A.pme_xyzq) {
-    const bool qv = t < A.nq;
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (qv) {
-      const uint32_t site = A.qsite[t];
-      p = wrapped(A.pos, A.q, A.g, site);
-      A.pme_xyzq[t] = p;
-      if (A.pme_b)
-        A.pme_b[t] = make_float4(p.x, p.y, p.z, site >= A.alch0 && site < A.alch1 ? -p.w : p.w);
-    }
-    // every lane of the wave takes part in the binning
-    if (A.bin_on) pme_bin_charge(A.bin, p, qv, t);
-  }
   const uint32_t n_cl = L.counters[0];
   for (uint32_t k = t; k < n_cl * DYN_CL_SIZE; k += gridDim.x * blockDim.x) {
     const int32_t s = L.slot_site[k];
@@ -2188,21 +2203,9 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.stg = e.slots_stg();
       A.fl = e.dflags.p;
       A.force = forced[i] ? 1 : 0;
-      A.qsite = e.qsite.p;
-      A.nq = e.n_q;
-      A.pme_xyzq = pl.recip_on ? e.pme_xyzq.p : nullptr;
-      A.pme_b = pl.recip_b ? e.pme_b.p : nullptr;
-      A.alch0 = e.alch0;
-      A.alch1 = e.alch1;
       A.fx_site = e.fx_site.p;
-      A.fx_recip = pl.recip_on ? e.fx_recip.p : nullptr;
       A.mol_e = energy_on && e.n_mol ? e.mol_e.p : nullptr;
       A.n_mol2 = e.n_mol * e.n_mol;
-      if (pl.recip_on) {  // the SPME charges are listed under their bricks here
-        e.pme.ensure_cap(e.n_q);
-        A.bin = e.pme.bin_args(e.status.p, kStPme);
-        A.bin_on = 1;
-      }
       return args<k_eng_prep>(A);
     });
   }
@@ -2218,6 +2221,37 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
   // before the pair kernel on the main stream
   const BatchRun& RP = staged_mode ? R : RS;
   if (any_recip) {
+    {
+      Profiler::Scope ps(prof, "pme_prep");
+      const unsigned gq = max_grid(S, [](dyn_engine& e) {
+        return std::max(grid_for(e.n_sites, 256), grid_for(e.n_q, 256));
+      });
+      launch<k_eng_pme_prep>(RP, "k_eng_pme_prep", 0, gq, 256, 0, [&](uint32_t i) {
+        dyn_engine& e = *E[i];
+        const Plan& pl = PL[i];
+        PmePrepArgs A;
+        std::memset(&A, 0, sizeof A);
+        A.pos = e.pos_dev;
+        A.q = e.q.p;
+        A.g = e.geom;
+        A.qsite = e.qsite.p;
+        A.alch0 = e.alch0;
+        A.alch1 = e.alch1;
+        A.pme_xyzq = e.pme_xyzq.p;
+        A.pme_b = pl.recip_b ? e.pme_b.p : nullptr;
+        A.fx_recip = e.fx_recip.p;
+        if (pl.recip_on) {  // the SPME charges are listed under their bricks here
+          e.pme.ensure_cap(e.n_q);
+          A.nq = e.n_q;
+          A.bin = e.pme.bin_args(e.status.p, kStPme);
+        } else {
+          // a system of the batch without SPME this evaluation: nothing to do
+          A.g.n_dyn = 0;
+          A.g.n_water = 0;
+        }
+        return args<k_eng_pme_prep>(A);
+      });
+    }
     std::vector<PmeRun> ra(ns), rb(ns);
     for (uint32_t i = 0; i < ns; ++i) {
       dyn_engine& e = *E[i];
@@ -2236,7 +2270,7 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       r.res = e.eng_out.p + 16;
       r.status = e.status.p;
       r.status_bit = kStPme;
-      r.binned = true;  // the preparation launch has listed the charges
+      r.binned = true;  // k_eng_pme_prep has listed the charges
       ra[i] = r;
       r.binned = false;
       if (pl.recip_b) {
@@ -2328,6 +2362,8 @@ void engine_set_enqueue(EngineSet& S, uint32_t flags,
       A.slot_site = e.slot_site.p;
       A.fx_site = e.fx_site.p;
       A.n_sites = e.n_sites;
+      // a fused evaluation's caller reads site sums only
+      A.keep_items = (flags & kComputeFusedFinalize) ? 0u : 1u;
       A.partials = e.pair_part.p;
       A.cap_chunks = e.cap_chunks;
       A.chunk = e.prm.item_groups;
