@@ -475,7 +475,7 @@ constexpr uint32_t kExEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kExPartners = kExHash / 2;  // partners of one cluster's sites
 constexpr int kListBlock = 256;                // four waves, each on its own rows
 constexpr int kListWaves = kListBlock / kWave;
-constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk (units beyond: grid stride)
+constexpr unsigned kWalkGrid = 16384;          // workgroups of the walk and fill (units beyond: grid stride; a resident-sized grid with a static stride balanced the walk worse)
 constexpr int kWalkPer = 1;                    // candidates per lane in a sweep and in a round
 struct WalkLds {
   float4 pI[DYN_CL_SIZE];  // I's slot positions (padding slots: NaN)
