@@ -173,65 +173,6 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
   return v;
 }
 
-// Exclusive scans of two uint32 sequences (a, b) = f(i), i < n, by one
-// workgroup of NT threads (each thread takes a contiguous range, twice: sums,
-// then outputs).  Writes out_a[0..n] and out_b[0..n] (out_x[n] = total;
-// out_a may be null), val_a[i] = a (when non-null), the totals to tot_x and
-// max(a) to *max_a (each when non-null).
-template <int NT, class F>
-__device__ __forceinline__ void scan_pairs(uint32_t n, F f, uint32_t* __restrict__ out_a,
-                                           uint32_t* __restrict__ out_b,
-                                           uint32_t* __restrict__ val_a,
-                                           uint32_t* __restrict__ tot_a,
-                                           uint32_t* __restrict__ tot_b,
-                                           uint32_t* __restrict__ max_a) {
-  __shared__ uint32_t wa[NT / kWave], wb[NT / kWave], wm[NT / kWave];
-  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-  const uint32_t per = (n + NT - 1) / NT;
-  const uint32_t b0 = t * per, e0 = min(n, b0 + per);
-  uint32_t sa = 0, sb = 0, m = 0;
-  for (uint32_t i = b0; i < e0; ++i) {
-    const uint2 v = f(i);
-    sa += v.x;
-    sb += v.y;
-    m = max(m, v.x);
-  }
-  const uint32_t ia = wave_incl_sum(sa), ib = wave_incl_sum(sb);
-  uint32_t im = m;
-  for (int off = 1; off < kWave; off <<= 1) im = max(im, __shfl_xor(im, off, kWave));
-  if (lane == kWave - 1) {
-    wa[w] = ia;
-    wb[w] = ib;
-  }
-  if (lane == 0) wm[w] = im;
-  __syncthreads();
-  uint32_t ba = 0, bb = 0, ta = 0, tb = 0, tm = 0;
-  for (int k = 0; k < NT / kWave; ++k) {
-    if (k < w) {
-      ba += wa[k];
-      bb += wb[k];
-    }
-    ta += wa[k];
-    tb += wb[k];
-    tm = max(tm, wm[k]);
-  }
-  uint32_t ra = ba + ia - sa, rb = bb + ib - sb;
-  for (uint32_t i = b0; i < e0; ++i) {
-    const uint2 v = f(i);
-    if (out_a) out_a[i] = ra;
-    out_b[i] = rb;
-    if (val_a) val_a[i] = v.x;
-    ra += v.x;
-    rb += v.y;
-  }
-  if (t == 0) {
-    if (out_a) out_a[n] = ta;
-    out_b[n] = tb;
-    if (tot_a) *tot_a = ta;
-    if (tot_b) *tot_b = tb;
-    if (max_a) *max_a = tm;
-  }
-}
 
 // Find every site's x–y column (from the positions the preparation kernel
 // wrapped), take a rank in it and write the site into the column's slice of
@@ -804,30 +745,6 @@ k_eng_list_walk(const ListArgs& A, uint32_t* __restrict__ ent_tmp) {
   }
 }
 
-// The rows in groups of 8 (each class segment, the two halves' entries,
-// padded to 8) and their scans: row_cnt, row_ptr and chunk_ptr, the totals
-// and the longest row.  One workgroup of 1,024 threads, a launch of its own.
-constexpr int kRowScanBlock = 1024;
-__device__ __forceinline__ void k_eng_row_scan(const ListArgs& A, uint32_t* __restrict__ row_cnt,
-                                               uint32_t* __restrict__ row_ptr,
-                                               uint32_t* __restrict__ chunk_ptr) {
-  if (!build_on(A.gate)) return;
-  const uint32_t n_cl = *A.n_cl;
-  static_assert(kUnits == 6, "a row's unit counts are three 8-byte words");
-  scan_pairs<kRowScanBlock>(
-      n_cl,
-      [&](uint32_t I) {
-        const uint2* uc = reinterpret_cast<const uint2*>(A.unit_cnt + (size_t)I * kUnits);
-        uint32_t m = 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const uint2 h = uc[k];
-          m += (h.x + h.y + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
-        }
-        return make_uint2(m, (m + A.chunk - 1) / A.chunk);
-      },
-      row_ptr, chunk_ptr, row_cnt, A.tot + 0, A.tot + 1, A.tot + 2);
-}
 
 }  // namespace
 
@@ -862,10 +779,28 @@ static_assert(sizeof(ListCopies) == 64, "ListCopies has no padding");
 // (entry jb's exclusion byte, shifted into place).  One wave per class
 // segment of a row; the segment's entries move four per lane at a time.
 constexpr int kFillUnroll = 4;
+constexpr uint32_t kFillRows = 4;  // rows per workgroup of the fill (3 class segments each)
+
+// A row's groups of 8 entries: each class segment (the two halves' entries)
+// padded to 8.
+__device__ __forceinline__ uint32_t row_groups(const uint32_t* __restrict__ unit_cnt, uint32_t I) {
+  static_assert(kUnits == 6, "a row's unit counts are three 8-byte words");
+  const uint2* uc = reinterpret_cast<const uint2*>(unit_cnt + (size_t)I * kUnits);
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint2 h = uc[k];
+    m += (h.x + h.y + DYN_CL_SIZE - 1) / DYN_CL_SIZE;
+  }
+  return m;
+}
+
 __device__ __forceinline__ void
 k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
-                const uint32_t* __restrict__ row_cnt, const uint32_t* __restrict__ row_ptr,
-                const uint32_t* __restrict__ chunk_ptr, ListCopies C) {
+                uint32_t* __restrict__ row_cnt, uint32_t* __restrict__ row_ptr,
+                uint32_t* __restrict__ chunk_ptr, ListCopies C) {
+  __shared__ uint32_t s_m[kFillRows], s_g[kFillRows], s_i[kFillRows];
+  __shared__ uint32_t s_red[2][kListWaves];
   if (!build_on(A.gate)) return;
   // the copy this evaluation's pair kernel reads (a build before it) or the
   // other one (a staged build, current from the next evaluation on; none
@@ -876,64 +811,117 @@ k_eng_list_fill(const ListArgs& A, const uint32_t* __restrict__ ent_tmp,
   uint2* __restrict__ chunks = second ? C.chunks1 : C.chunks0;
   const uint32_t n_cl = *A.n_cl;
   const int lane = threadIdx.x & (kWave - 1);
-  for (uint32_t sg = blockIdx.x * kListWaves + threadIdx.x / kWave; sg < n_cl * 3u;
-       sg += gridDim.x * kListWaves) {
-    const uint32_t I = sg / 3u;
-    const int cls = (int)(sg % 3u);
-    const uint32_t m = row_cnt[I];
-    const uint64_t base = row_ptr[I];
-    if (cls == 0) {
-      const uint32_t c0 = chunk_ptr[I], nch = (m + A.chunk - 1) / A.chunk;
-      if ((uint64_t)c0 + nch <= A.cap_chunks) {
-        for (uint32_t k = lane; k < nch; k += kWave)
-          chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * A.chunk);
-      } else if (lane == 0) {
-        atomicOr(A.status, kStChunks);
-      }
-      if (base + m > A.cap_entries && lane == 0) atomicOr(A.status, kStEntries);
+  const int w = threadIdx.x / kWave;
+  if (n_cl == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    row_ptr[0] = 0;
+    chunk_ptr[0] = 0;
+    A.tot[0] = 0;
+    A.tot[1] = 0;
+  }
+  for (uint32_t blk = blockIdx.x; blk * kFillRows < n_cl; blk += gridDim.x) {
+    const uint32_t r0 = blk * kFillRows, nr = min(kFillRows, n_cl - r0);
+    // the groups and work items of the rows before this block (every
+    // workgroup sums them itself: a scan launch of its own would cost a
+    // dependent launch on every step, and a scan inside the walk a
+    // device-scope fence per workgroup)
+    uint32_t sg = 0, si = 0;
+    for (uint32_t I = threadIdx.x; I < r0; I += kListBlock) {
+      const uint32_t m = row_groups(A.unit_cnt, I);
+      sg += m;
+      si += (m + A.chunk - 1) / A.chunk;
     }
-    if (base + m > A.cap_entries) continue;
-    const uint32_t* uc = A.unit_cnt + (size_t)I * kUnits;
-    uint32_t off = 0;  // the segment's first entry in the row
-    for (int k = 0; k < cls; ++k) off += (uc[2 * k] + uc[2 * k + 1] + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
-    const uint32_t n0 = uc[2 * cls], n = n0 + uc[2 * cls + 1];
-    if (n == 0) continue;
-    const uint32_t np = (n + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
-    const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
-    const bool ss = A.cl_class[I] == DYN_CLS_SOLUTE && cls == DYN_CLS_SOLUTE;
-    const size_t u0 = (size_t)I * kUnits + 2 * cls;
-    uint32_t* out = cj + (base * DYN_CL_SIZE + off);
-    for (uint32_t p0 = 0; p0 < np; p0 += kFillUnroll * kWave) {
-      uint32_t e[kFillUnroll];
-      unsigned long long xbits[kFillUnroll];
-#pragma unroll
-      for (int r = 0; r < kFillUnroll; ++r) {
-        // entry p of the segment: from the first half, the second, or padding
-        const uint32_t p = p0 + r * kWave + lane;
-        const size_t src = p < n0 ? u0 * A.row_cap + p : (u0 + 1) * A.row_cap + (p - n0);
-        e[r] = A.null_slot | tag;
-        xbits[r] = 0ull;
-        if (p < n) {
-          e[r] = ent_tmp[src];
-          if (ss) xbits[r] = (unsigned long long)A.xb_tmp[src] << (DYN_CL_SIZE * (p & 7u));
-        }
+    sg = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_sum(sg), kWave - 1);
+    si = (uint32_t)__builtin_amdgcn_readlane((int)wave_incl_sum(si), kWave - 1);
+    if (lane == 0) {
+      s_red[0][w] = sg;
+      s_red[1][w] = si;
+    }
+    if (threadIdx.x < kFillRows) s_m[threadIdx.x] = threadIdx.x < nr ? row_groups(A.unit_cnt, r0 + threadIdx.x) : 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t bg = 0, bi = 0;
+      for (int k = 0; k < kListWaves; ++k) {
+        bg += s_red[0][k];
+        bi += s_red[1][k];
       }
-#pragma unroll
-      for (int r = 0; r < kFillUnroll; ++r) {
-        const uint32_t p = p0 + r * kWave + lane;
-        if (p < np) out[p] = e[r];
-        if (ss) {
-          // the group's 8 bytes: an OR over its 8 lanes (groups start at
-          // multiples of 8 within the segment, and the segment at a
-          // multiple of 8 within the row)
-          unsigned long long x = xbits[r];
-          x |= __shfl_xor(x, 1, kWave);
-          x |= __shfl_xor(x, 2, kWave);
-          x |= __shfl_xor(x, 4, kWave);
-          if ((lane & 7) == 0 && p < np) cmask[base + (off + p) / DYN_CL_SIZE] = ~x;
-        }
+      for (uint32_t k = 0; k < kFillRows; ++k) {
+        s_g[k] = bg;
+        s_i[k] = bi;
+        bg += s_m[k];
+        bi += (s_m[k] + A.chunk - 1) / A.chunk;
+      }
+      if (r0 + nr == n_cl) {  // the block with the last row: the totals
+        row_ptr[n_cl] = bg;
+        chunk_ptr[n_cl] = bi;
+        A.tot[0] = bg;
+        A.tot[1] = bi;
       }
     }
+    __syncthreads();
+    if (threadIdx.x < nr) {
+      row_cnt[r0 + threadIdx.x] = s_m[threadIdx.x];
+      row_ptr[r0 + threadIdx.x] = s_g[threadIdx.x];
+      chunk_ptr[r0 + threadIdx.x] = s_i[threadIdx.x];
+    }
+    for (uint32_t sgi = (uint32_t)w; sgi < nr * 3u; sgi += kListWaves) {
+      const uint32_t I = r0 + sgi / 3u;
+      const int cls = (int)(sgi % 3u);
+      const uint32_t m = s_m[sgi / 3u];
+      const uint64_t base = s_g[sgi / 3u];
+      if (cls == 0) {
+        const uint32_t c0 = s_i[sgi / 3u], nch = (m + A.chunk - 1) / A.chunk;
+        if ((uint64_t)c0 + nch <= A.cap_chunks) {
+          for (uint32_t k = lane; k < nch; k += kWave)
+            chunks[c0 + k] = make_uint2(I, (uint32_t)base + k * A.chunk);
+        } else if (lane == 0) {
+          atomicOr(A.status, kStChunks);
+        }
+        if (base + m > A.cap_entries && lane == 0) atomicOr(A.status, kStEntries);
+      }
+      if (base + m > A.cap_entries) continue;
+      const uint32_t* uc = A.unit_cnt + (size_t)I * kUnits;
+      uint32_t off = 0;  // the segment's first entry in the row
+      for (int k = 0; k < cls; ++k) off += (uc[2 * k] + uc[2 * k + 1] + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
+      const uint32_t n0 = uc[2 * cls], n = n0 + uc[2 * cls + 1];
+      if (n == 0) continue;
+      const uint32_t np = (n + DYN_CL_SIZE - 1) & ~(uint32_t)(DYN_CL_SIZE - 1);
+      const uint32_t tag = (uint32_t)cls << DYN_JCLS_SHIFT;
+      const bool ss = A.cl_class[I] == DYN_CLS_SOLUTE && cls == DYN_CLS_SOLUTE;
+      const size_t u0 = (size_t)I * kUnits + 2 * cls;
+      uint32_t* out = cj + (base * DYN_CL_SIZE + off);
+      for (uint32_t p0 = 0; p0 < np; p0 += kFillUnroll * kWave) {
+        uint32_t e[kFillUnroll];
+        unsigned long long xbits[kFillUnroll];
+#pragma unroll
+        for (int r = 0; r < kFillUnroll; ++r) {
+          // entry p of the segment: from the first half, the second, or padding
+          const uint32_t p = p0 + r * kWave + lane;
+          const size_t src = p < n0 ? u0 * A.row_cap + p : (u0 + 1) * A.row_cap + (p - n0);
+          e[r] = A.null_slot | tag;
+          xbits[r] = 0ull;
+          if (p < n) {
+            e[r] = ent_tmp[src];
+            if (ss) xbits[r] = (unsigned long long)A.xb_tmp[src] << (DYN_CL_SIZE * (p & 7u));
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < kFillUnroll; ++r) {
+          const uint32_t p = p0 + r * kWave + lane;
+          if (p < np) out[p] = e[r];
+          if (ss) {
+            // the group's 8 bytes: an OR over its 8 lanes (groups start at
+            // multiples of 8 within the segment, and the segment at a
+            // multiple of 8 within the row)
+            unsigned long long x = xbits[r];
+            x |= __shfl_xor(x, 1, kWave);
+            x |= __shfl_xor(x, 2, kWave);
+            x |= __shfl_xor(x, 4, kWave);
+            if ((lane & 7) == 0 && p < np) cmask[base + (off + p) / DYN_CL_SIZE] = ~x;
+          }
+        }
+      }
+    }
+    __syncthreads();  // s_m, s_g, s_i and s_red are reused by the next block
   }
   // a staged build is complete when the launch's last workgroup is done
   if (C.staged && last_workgroup(C.fl + kFlFillDone) && threadIdx.x == 0) C.fl[kFlReady] = 1u;
@@ -2070,12 +2058,12 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
     A.n_cols = e.n_cols;
     A.tot = T[i].counters + 1;
   }
-  // the walk: one unit per wave; the fill: one class segment per wave
+  // the walk: one unit per wave; the fill: kFillRows rows per workgroup
   const unsigned walk_grid = max_grid(S, [](dyn_engine& e) {
     return std::min<uint32_t>((e.cap_cl * kUnits + kListWaves - 1) / kListWaves, kWalkGrid);
   });
   const unsigned list_grid = max_grid(S, [](dyn_engine& e) {
-    return std::min<uint32_t>((e.cap_cl * 3 + kListWaves - 1) / kListWaves, kWalkGrid);
+    return std::min<uint32_t>((e.cap_cl + kFillRows - 1) / kFillRows, kWalkGrid);
   });
   {
     Profiler::Scope ps(prof, "nbl_list_walk");
@@ -2083,11 +2071,6 @@ static void set_build_lists(EngineSet& S, const BatchRun& R, const std::vector<B
                                         [&](uint32_t i) {
       dyn_engine& e = *E[i];
       return args<k_eng_list_walk>(LA[i], e.ent_tmp.p);
-    });
-    launch<k_eng_row_scan, kRowScanBlock>(R, "k_eng_row_scan", 0, 1, kRowScanBlock, 0,
-                                          [&](uint32_t i) {
-      dyn_engine& e = *E[i];
-      return args<k_eng_row_scan>(LA[i], e.row_cnt.p, T[i].row_ptr, T[i].chunk_ptr);
     });
   }
   {
