@@ -464,7 +464,8 @@ def run_batch_uneven(args) -> int:
     def make(s, gamma, temp, seed):
         ff = systems.chain_force_field(s)
         e = api.Engine(s, cutoff=10.0, lj_cutoff=10.0, beta=0.3123, grid=api.pme_grid_n(s.box, 1.0),
-                       order=4, mode=api.MODE_REFERENCE, list_buffer=args.list_buffer)
+                       order=4, mode=api.MODE_REFERENCE, list_buffer=args.list_buffer,
+                       item_groups=args.item_groups)
         m = api.Md(e, ff, dt=0.002, temperature=temp, gamma=gamma, shake_tol=1e-5, seed=seed)
         return e, m, ff
 
